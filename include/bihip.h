@@ -1,0 +1,250 @@
+/*
+ * bihip.h -- C ABI of the MI355X (gfx950) particle-filter library.
+ *
+ * LibBi has no FFI: its operator API is a set of C++ template concepts
+ * resolved at compile time by Location (ON_HOST / ON_DEVICE).  This header
+ * is the boundary that the ON_DEVICE side of those concepts calls; the C++
+ * shims in include/bi/amd/ (Filter, BootstrapPF, Resampler, integrators)
+ * keep the reference's call sites.  Every entry point below names the
+ * reference function it replaces (paths relative to the reference root).
+ *
+ * Conventions (as the reference):
+ *  - state X is column-major, P rows (particles) x d columns, leading
+ *    dimension ld >= P, so each variable is contiguous across particles
+ *    (share/src/bi/state/State.hpp:469-479);
+ *  - log-weights are double, ancestors are 0-based int;
+ *  - lws carries the running log-likelihood: after a resample
+ *    lws[:] = logLikelihood (share/src/bi/resampler/Resampler.hpp:177).
+ *
+ * Pointers named *_dev are device pointers; *_host are host pointers.
+ * All work is ordered on the context's stream.  Calls that return host
+ * scalars synchronise that stream.  Every function returns BIHIP_OK or an
+ * error code; bihip_last_error() gives the calling thread's message.  No
+ * C++ exception crosses this boundary: the C++ shims turn
+ * BIHIP_DEGENERATE into bi::ParticleFilterDegeneratedException
+ * (share/src/bi/misc/exception.hpp:47-49).
+ */
+#ifndef BIHIP_H
+#define BIHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define BIHIP_OK 0
+#define BIHIP_DEGENERATE 1 /* sum of weights <= 0 (ParticleFilterDegeneratedException) */
+#define BIHIP_EINVAL 2     /* invalid argument */
+#define BIHIP_EHIP 3       /* HIP runtime error */
+#define BIHIP_ESTATE 4     /* object used out of order */
+
+/* ---- model, integrator, resampler and precision identifiers ---- */
+#define BIHIP_MODEL_PZ 0  /* predator-prey, docs/src/users.tex:36-85 */
+#define BIHIP_MODEL_L96 1 /* Lorenz-96 with additive noise */
+#define BIHIP_MODEL_LG 2  /* scalar linear-Gaussian (Kalman-checkable) */
+
+#define BIHIP_RK4 0    /* share/src/bi/ode/RK4Integrator.hpp */
+#define BIHIP_DOPRI5 1 /* share/src/bi/ode/DOPRI5Integrator.hpp */
+
+#define BIHIP_SYSTEMATIC 0  /* share/src/bi/resampler/SystematicResampler.hpp */
+#define BIHIP_STRATIFIED 1  /* share/src/bi/resampler/StratifiedResampler.hpp */
+#define BIHIP_MULTINOMIAL 2 /* share/src/bi/resampler/MultinomialResampler.hpp */
+
+#define BIHIP_F32 0 /* state and transition arithmetic in float */
+#define BIHIP_F64 1 /* state and transition arithmetic in double */
+
+/* random-stream tags of the Philox counter {index, step, tag|(batch<<8), block} */
+#define BIHIP_TAG_INIT 1
+#define BIHIP_TAG_NOISE 2
+#define BIHIP_TAG_SYSTEMATIC 3
+#define BIHIP_TAG_STRATIFIED 4
+#define BIHIP_TAG_MULTINOMIAL 5
+#define BIHIP_TAG_OBS 6
+
+/*
+ * Model description: the runtime counterpart of a generated Model<name>
+ * (share/tt/cpp/model.hpp.tt) plus the ode block's integrator settings
+ * (share/tt/cpp/block/ode.hpp.tt:39-54).
+ * par[] meaning per model:
+ *  PZ : mu, sigma, c, e, m_l, m_q, obs_sd, P0_logmean, P0_logsd, Z0_logmean, Z0_logsd
+ *  L96: F, noise_sd, obs_sd, init_sd      (d in {8, 40})
+ *  LG : lambda, noise_sd, obs_sd, m0, s0  (d = 1)
+ */
+typedef struct {
+  int model;      /* BIHIP_MODEL_* */
+  int d;          /* state dimension */
+  int integrator; /* BIHIP_RK4 / BIHIP_DOPRI5 */
+  int nsteps;     /* DOPRI5 step limit per interval (IntegratorConstants.cpp:86) */
+  int host_quirk; /* DOPRI5: 1 = k7 evaluated at x5 as the reference host does */
+  int precision;  /* BIHIP_F32 / BIHIP_F64 */
+  double h0, atol, rtol;
+  double par[16];
+} bihip_model;
+
+typedef struct bihip_ctx bihip_ctx;
+typedef struct bihip_pf bihip_pf;
+
+/* ================================================================ context */
+/* bi_init (share/src/bi/init.hpp:33-49): choose the device, create the context. */
+int bihip_ctx_create(int device, bihip_ctx** ctx);
+int bihip_ctx_destroy(bihip_ctx* ctx);
+/* Order all work on this hipStream_t (NULL = the null stream). */
+int bihip_ctx_set_stream(bihip_ctx* ctx, void* stream);
+int bihip_ctx_synchronize(bihip_ctx* ctx);
+const char* bihip_last_error(void);
+/* Number of observed variables of a model (Mask size per observation). */
+int bihip_model_nobs(const bihip_model* m);
+/* Bytes per state element of a model (4 or 8). */
+int bihip_model_elem_size(const bihip_model* m);
+
+/* ================================================================ random numbers */
+/* Philox4x32-10 blocks of n counters (ctr_dev: n x 4 words) under one key;
+ * replaces the cuRAND XORWOW state of share/src/bi/cuda/random/RngGPU.cuh:19-206. */
+int bihip_philox(bihip_ctx* ctx, const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, int n,
+                 uint32_t* out_dev);
+/* u_j in [0,1), j < n, from counter {j, step, tag|(batch<<8), 0}
+ * (Random::uniforms, share/src/bi/random/Random.hpp:44-380). */
+int bihip_uniforms(bihip_ctx* ctx, uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch,
+                   int n, double* u_dev);
+/* n standard normals for one index (normal j from block j/4), in the given
+ * precision (float normals use 24-bit uniforms). */
+int bihip_normals(bihip_ctx* ctx, uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                  uint32_t batch, int n, int precision, void* z_dev);
+
+/* ================================================================ model */
+/* Initial-state sampling (Model::initialSamples, share/tt/cpp/model.hpp.tt).
+ * mean0_host: d initial means (L96) or NULL. */
+int bihip_init_samples(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                       int P, void* X_dev, int ld, const double* mean0_host);
+/*
+ * Model::transitionSamples(rng, t1, t2, onDelta, s) (share/tt/cpp/model.hpp.tt:566-573):
+ * noise pdf_ block then the ode block, one interval [t1,t2], for P particles.
+ * Replaces RK4IntegratorHost/DOPRI5IntegratorHost::update
+ * (share/src/bi/host/ode/RK4IntegratorHost.hpp:42-100,
+ *  share/src/bi/host/ode/DOPRI5IntegratorHost.hpp:42-142) and the GPU kernels
+ * share/src/bi/cuda/ode/{RK4,DOPRI5}KernelGPU.cuh.
+ * Reads row as_dev[p] of Xin (as_dev NULL = row p), writes row p of Xout;
+ * Xin == Xout is allowed when as_dev is NULL.  nsteps_dev (nullable) gets the
+ * integrator step count per particle.
+ */
+int bihip_transition(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                     uint32_t step, double t1, double t2, int P, const void* Xin_dev,
+                     void* Xout_dev, int ld, const int* as_dev, int* nsteps_dev);
+/* Model::observationLogDensities(s, mask, lp) (share/tt/cpp/model.hpp.tt:757-765):
+ * lws[p] += log p(y | x_p); y_dev holds nobs values. */
+int bihip_obs_logdensity(bihip_ctx* ctx, const bihip_model* m, const double* y_dev, int P,
+                         const void* X_dev, int ld, double* lws_dev);
+/* Model::observationSamples for P particles: y_dev is P x nobs (row-major). */
+int bihip_obs_sample(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                     uint32_t step, int P, const void* X_dev, int ld, double* y_dev);
+
+/* ================================================================ reductions */
+/* ess_reduce (share/src/bi/primitive/vector_primitive.hpp:959-974):
+ * returns ESS = (sum w)^2 / sum w^2 and lW = max + log sum w - log P. */
+int bihip_ess_reduce(bihip_ctx* ctx, const double* lws_dev, int P, double* ess_host,
+                     double* lW_host);
+/* logsumexp_reduce (vector_primitive.hpp:932-943). */
+int bihip_logsumexp_reduce(bihip_ctx* ctx, const double* lws_dev, int P, double* out_host);
+/* max_reduce with NaN below every number (vector_primitive.hpp:860-872). */
+int bihip_max_reduce(bihip_ctx* ctx, const double* x_dev, int n, double* out_host);
+/* sumexpu_inclusive_scan (vector_primitive.hpp:1075-1084): Ws = inclusive
+ * scan of nanexp(lws - max); single-pass decoupled look-back. mx_host nullable. */
+int bihip_sumexpu_inclusive_scan(bihip_ctx* ctx, const double* lws_dev, int P, double* Ws_dev,
+                                 double* mx_host);
+
+/* ================================================================ resamplers */
+/* SystematicResampler::cumulativeOffspring (SystematicResampler.hpp:110-130):
+ * O_i = min(n, (int)(Ws_i / W * n + a)), W = Ws[P-1]. BIHIP_DEGENERATE if W <= 0. */
+int bihip_systematic_cumulative_offspring(bihip_ctx* ctx, const double* Ws_dev, int P, int n,
+                                          double a, int* Os_dev);
+/* StratifiedResampler::cumulativeOffspring / StratifiedResamplerHost::op
+ * (share/src/bi/host/resampler/StratifiedResamplerHost.hpp:25-74); alphas_dev has n values. */
+int bihip_stratified_cumulative_offspring(bihip_ctx* ctx, const double* Ws_dev, int P, int n,
+                                          const double* alphas_dev, int* Os_dev);
+/* MultinomialResampler::ancestors (MultinomialResampler.hpp:82-92):
+ * as_j = #{i : Ws_i <= W U_j} for n uniforms U_dev in [0,1). */
+int bihip_multinomial_ancestors(bihip_ctx* ctx, const double* Ws_dev, int P, int n,
+                                const double* U_dev, int* as_dev);
+/* ancestorsToOffspring (share/src/bi/resampler/misc.hpp:100-108). */
+int bihip_ancestors_to_offspring(bihip_ctx* ctx, const int* as_dev, int n, int P, int* os_dev);
+/* inclusive scan of offspring counts into cumulative offspring. */
+int bihip_offspring_to_cumulative(bihip_ctx* ctx, const int* os_dev, int P, int* Os_dev);
+/* cumulativeOffspringToAncestors (misc.hpp:130-138; ResamplerHost.hpp:109-131). */
+int bihip_cumulative_offspring_to_ancestors(bihip_ctx* ctx, const int* Os_dev, int P,
+                                            int* as_dev);
+/* cumulativeOffspringToAncestorsPermute (misc.hpp:140-149; host layout of
+ * ResamplerHost.hpp:133-168): as[i]=i for particles with offspring, extra
+ * copies fill the other slots in ascending order. Bit-exact with the host. */
+int bihip_cumulative_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* Os_dev, int P,
+                                                    int* as_dev);
+/* offspringToAncestorsPermute (misc.hpp:120-128; ResamplerHost.hpp:86-107). */
+int bihip_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* os_dev, int P,
+                                         int* as_dev);
+/* gather_rows in place, X[i,:] = X[as[i],:] (State::gather, State.hpp:905-909;
+ * share/src/bi/cuda/primitive/matrix_primitive.cuh:67-78). Requires the
+ * permuted layout (as[as[i]] == as[i]). elem_size is 4 or 8. */
+int bihip_gather_rows(bihip_ctx* ctx, const int* as_dev, int P, int ncols, void* X_dev, int ld,
+                      int elem_size);
+
+/* ================================================================ filter */
+/*
+ * Device-resident bootstrap particle filter: Filter<BootstrapPF>::filter
+ * (share/src/bi/filter/Filter.hpp:56-71, share/src/bi/filter/BootstrapPF.hpp:156-196)
+ * with the ESS test, logLikelihood and logIncrements kept on the device, so a
+ * step never waits for the host.  Resampling is triggered when the
+ * observation at the current time has been assimilated and
+ * ess < essRel * P (Resampler.hpp:168).  Schedule: t_k = t0 + k dt,
+ * observed for k = 1..T.
+ */
+typedef struct {
+  int P;             /* particles */
+  int T;             /* observations */
+  int resampler;     /* BIHIP_SYSTEMATIC / BIHIP_STRATIFIED / BIHIP_MULTINOMIAL */
+  double essRel;     /* relative ESS threshold */
+  double t0, dt;     /* schedule */
+  uint64_t seed;     /* Philox key */
+  uint32_t batch;    /* Philox batch id (theta index under SMC^2) */
+  int record;        /* 1 = keep ancestors of every step on the device */
+  int use_graph;     /* 1 = replay each step as a captured hipGraph */
+} bihip_pf_config;
+
+int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config* cfg,
+                    bihip_pf** pf);
+int bihip_pf_destroy(bihip_pf* pf);
+/* Observations, T x nobs row-major host array (Observer::update, Observer.hpp:115-126). */
+int bihip_pf_set_observations(bihip_pf* pf, const double* y_host);
+/* Filter::init + output0: sample initial particles, lws = 0, logLikelihood = 0. */
+int bihip_pf_init(bihip_pf* pf, const double* mean0_host);
+/* BootstrapPF::step: resample (device decision) + predict + correct for the
+ * next n observation intervals. Asynchronous. */
+int bihip_pf_step(bihip_pf* pf, int n);
+/* BootstrapPF::term: logZ = logsumexp(lws) - log P; synchronises. */
+int bihip_pf_term(bihip_pf* pf, double* logZ_host);
+/* Status after the steps so far: BIHIP_OK or BIHIP_DEGENERATE; synchronises. */
+int bihip_pf_status(bihip_pf* pf);
+/* Copies of the filter's outputs (host pointers; any may be NULL):
+ * logIncrements[T], ess[T], resampled[T], logLikelihood (running). */
+int bihip_pf_outputs(bihip_pf* pf, double* logIncrements_host, double* ess_host,
+                     int* resampled_host, double* logLikelihood_host);
+/* Ancestors of step k (record=1), or of the last resample when k < 0. */
+int bihip_pf_ancestors(bihip_pf* pf, int k, int* as_host);
+/* Current state buffer (device pointer, leading dimension) and log-weights. */
+int bihip_pf_state(bihip_pf* pf, void** X_dev, int* ld, double** lws_dev);
+/* Steps completed so far. */
+int bihip_pf_position(bihip_pf* pf);
+/*
+ * Kernel timing with HIP events on the filter's stream: enable, then query
+ * the summed milliseconds and launch count of one kernel class
+ * (0 = transition, 1 = resample chain, 2 = weight reduction).
+ */
+int bihip_pf_set_timing(bihip_pf* pf, int enable);
+int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIHIP_H */

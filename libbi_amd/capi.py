@@ -1,0 +1,304 @@
+"""ctypes binding of the C ABI in include/bihip.h (libbi_amd/lib/libbihip.so).
+
+Device buffers are torch tensors on a ROCm device (torch is used for device
+memory and streams only); every call passes plain pointers.  The library is
+loaded from the package tree; a missing library raises immediately -- there is
+no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libbihip.so")
+
+OK, DEGENERATE, EINVAL, EHIP, ESTATE = 0, 1, 2, 3, 4
+MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
+RK4, DOPRI5 = 0, 1
+SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
+F32, F64 = 0, 1
+TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
+
+# exported entry points (must match include/bihip.h)
+EXPORTS = [
+    "bihip_ctx_create", "bihip_ctx_destroy", "bihip_ctx_set_stream", "bihip_ctx_synchronize",
+    "bihip_last_error", "bihip_model_nobs", "bihip_model_elem_size",
+    "bihip_philox", "bihip_uniforms", "bihip_normals",
+    "bihip_init_samples", "bihip_transition", "bihip_obs_logdensity", "bihip_obs_sample",
+    "bihip_ess_reduce", "bihip_logsumexp_reduce", "bihip_max_reduce",
+    "bihip_sumexpu_inclusive_scan",
+    "bihip_systematic_cumulative_offspring", "bihip_stratified_cumulative_offspring",
+    "bihip_multinomial_ancestors", "bihip_ancestors_to_offspring",
+    "bihip_offspring_to_cumulative", "bihip_cumulative_offspring_to_ancestors",
+    "bihip_cumulative_offspring_to_ancestors_permute", "bihip_offspring_to_ancestors_permute",
+    "bihip_gather_rows",
+    "bihip_pf_create", "bihip_pf_destroy", "bihip_pf_set_observations", "bihip_pf_init",
+    "bihip_pf_step", "bihip_pf_term", "bihip_pf_status", "bihip_pf_outputs",
+    "bihip_pf_ancestors", "bihip_pf_state", "bihip_pf_position", "bihip_pf_set_timing",
+    "bihip_pf_kernel_time",
+]
+
+
+class BihipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bihip error {code}: {msg}")
+        self.code = code
+
+
+class ParticleFilterDegeneratedException(BihipError):
+    """Raised for BIHIP_DEGENERATE, as bi::ParticleFilterDegeneratedException
+    (share/src/bi/misc/exception.hpp:47-49)."""
+
+
+class Model(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int), ("d", ctypes.c_int), ("integrator", ctypes.c_int),
+        ("nsteps", ctypes.c_int), ("host_quirk", ctypes.c_int), ("precision", ctypes.c_int),
+        ("h0", ctypes.c_double), ("atol", ctypes.c_double), ("rtol", ctypes.c_double),
+        ("par", ctypes.c_double * 16),
+    ]
+
+
+class PFConfig(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int), ("T", ctypes.c_int), ("resampler", ctypes.c_int),
+        ("essRel", ctypes.c_double), ("t0", ctypes.c_double), ("dt", ctypes.c_double),
+        ("seed", ctypes.c_uint64), ("batch", ctypes.c_uint32), ("record", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with libbi_amd.build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    c = ctypes
+    P = c.c_void_p
+    i, u32, u64, d = c.c_int, c.c_uint32, c.c_uint64, c.c_double
+    pd, pi, pl = c.POINTER(c.c_double), c.POINTER(c.c_int), c.POINTER(c.c_long)
+    pm, pc = c.POINTER(Model), c.POINTER(PFConfig)
+    sig = {
+        "bihip_ctx_create": [i, c.POINTER(P)],
+        "bihip_ctx_destroy": [P],
+        "bihip_ctx_set_stream": [P, P],
+        "bihip_ctx_synchronize": [P],
+        "bihip_model_nobs": [pm],
+        "bihip_model_elem_size": [pm],
+        "bihip_philox": [P, P, u32, u32, i, P],
+        "bihip_uniforms": [P, u64, u32, u32, u32, i, P],
+        "bihip_normals": [P, u64, u32, u32, u32, u32, i, i, P],
+        "bihip_init_samples": [P, pm, u64, u32, i, P, i, pd],
+        "bihip_transition": [P, pm, u64, u32, u32, d, d, i, P, P, i, P, P],
+        "bihip_obs_logdensity": [P, pm, P, i, P, i, P],
+        "bihip_obs_sample": [P, pm, u64, u32, u32, i, P, i, P],
+        "bihip_ess_reduce": [P, P, i, pd, pd],
+        "bihip_logsumexp_reduce": [P, P, i, pd],
+        "bihip_max_reduce": [P, P, i, pd],
+        "bihip_sumexpu_inclusive_scan": [P, P, i, P, pd],
+        "bihip_systematic_cumulative_offspring": [P, P, i, i, d, P],
+        "bihip_stratified_cumulative_offspring": [P, P, i, i, P, P],
+        "bihip_multinomial_ancestors": [P, P, i, i, P, P],
+        "bihip_ancestors_to_offspring": [P, P, i, i, P],
+        "bihip_offspring_to_cumulative": [P, P, i, P],
+        "bihip_cumulative_offspring_to_ancestors": [P, P, i, P],
+        "bihip_cumulative_offspring_to_ancestors_permute": [P, P, i, P],
+        "bihip_offspring_to_ancestors_permute": [P, P, i, P],
+        "bihip_gather_rows": [P, P, i, i, P, i, i],
+        "bihip_pf_create": [P, pm, pc, c.POINTER(P)],
+        "bihip_pf_destroy": [P],
+        "bihip_pf_set_observations": [P, pd],
+        "bihip_pf_init": [P, pd],
+        "bihip_pf_step": [P, i],
+        "bihip_pf_term": [P, pd],
+        "bihip_pf_status": [P],
+        "bihip_pf_outputs": [P, pd, pd, pi, pd],
+        "bihip_pf_ancestors": [P, i, pi],
+        "bihip_pf_state": [P, c.POINTER(P), pi, c.POINTER(P)],
+        "bihip_pf_position": [P],
+        "bihip_pf_set_timing": [P, i],
+        "bihip_pf_kernel_time": [P, i, pd, pl],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = c.c_int
+    L.bihip_last_error.argtypes = []
+    L.bihip_last_error.restype = c.c_char_p
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().bihip_last_error().decode(errors="replace")
+        if rc == DEGENERATE:
+            raise ParticleFilterDegeneratedException(rc, msg)
+        raise BihipError(rc, msg)
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+# ---------------------------------------------------------------- models
+def make_model(kind, d=None, integrator=None, precision=F64, h0=None, atol=1e-5, rtol=1e-5,
+               nsteps=1000, host_quirk=0, **par):
+    """Model description with the same parameter names as the oracle's make_model."""
+    m = Model()
+    if kind == "pz":
+        m.model, m.d = MODEL_PZ, 2
+        m.integrator = RK4 if integrator is None else integrator
+        m.h0 = 0.1 if h0 is None else h0
+        vals = [par.get("mu", 0.5), par.get("sigma", 0.2), par.get("c", 0.25), par.get("e", 0.3),
+                par.get("m_l", 0.1), par.get("m_q", 0.1), par.get("obs_sd", 0.2),
+                par.get("P0_logmean", np.log(2.0)), par.get("P0_logsd", 0.2),
+                par.get("Z0_logmean", np.log(2.0)), par.get("Z0_logsd", 0.1)]
+    elif kind == "l96":
+        m.model, m.d = MODEL_L96, 40 if d is None else d
+        m.integrator = DOPRI5 if integrator is None else integrator
+        m.h0 = 0.05 if h0 is None else h0
+        vals = [par.get("F", 8.0), par.get("noise_sd", 0.1), par.get("obs_sd", 1.0),
+                par.get("init_sd", 1.0)]
+    elif kind == "lg":
+        m.model, m.d = MODEL_LG, 1 if d is None else d
+        m.integrator = DOPRI5 if integrator is None else integrator
+        m.h0 = 0.1 if h0 is None else h0
+        vals = [par.get("lam", 0.5), par.get("noise_sd", 0.5), par.get("obs_sd", 0.5),
+                par.get("m0", 0.0), par.get("s0", 1.0)]
+    else:
+        raise ValueError(kind)
+    m.precision, m.nsteps, m.host_quirk = precision, nsteps, host_quirk
+    m.atol, m.rtol = atol, rtol
+    for k, v in enumerate(vals):
+        m.par[k] = v
+    return m
+
+
+def nobs(m):
+    return lib().bihip_model_nobs(ctypes.byref(m))
+
+
+class Context:
+    """bihip_ctx bound to one device and to torch's current stream."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        self.device = device
+        h = ctypes.c_void_p()
+        check(lib().bihip_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        s = stream if stream is not None else torch.cuda.current_stream(device)
+        self.stream = s
+        check(lib().bihip_ctx_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if self.h:
+            lib().bihip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().bihip_ctx_synchronize(self.h))
+
+
+class ParticleFilter:
+    """bihip_pf: device-resident bootstrap particle filter."""
+
+    def __init__(self, ctx, model, P, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0,
+                 batch=0, record=False, use_graph=False):
+        self.ctx, self.model = ctx, model
+        self.cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
+                            seed=seed, batch=batch, record=1 if record else 0,
+                            use_graph=1 if use_graph else 0)
+        h = ctypes.c_void_p()
+        check(lib().bihip_pf_create(ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
+                                    ctypes.byref(h)))
+        self.h = h
+        self.P, self.T, self.nobs = P, T, nobs(model)
+
+    def close(self):
+        if self.h:
+            lib().bihip_pf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.T, self.nobs)
+        check(lib().bihip_pf_set_observations(self.h, y.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def init(self, mean0=None):
+        m = None
+        if mean0 is not None:
+            self._mean0 = np.ascontiguousarray(mean0, dtype=np.float64)
+            m = self._mean0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(lib().bihip_pf_init(self.h, m))
+
+    def step(self, n=1):
+        check(lib().bihip_pf_step(self.h, n))
+
+    def term(self):
+        z = ctypes.c_double()
+        check(lib().bihip_pf_term(self.h, ctypes.byref(z)))
+        return z.value
+
+    def status(self):
+        return check(lib().bihip_pf_status(self.h))
+
+    def outputs(self):
+        li = np.zeros(self.T)
+        ess = np.zeros(self.T)
+        res = np.zeros(self.T, np.int32)
+        ll = ctypes.c_double()
+        D = ctypes.POINTER(ctypes.c_double)
+        check(lib().bihip_pf_outputs(self.h, li.ctypes.data_as(D), ess.ctypes.data_as(D),
+                                     res.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     ctypes.byref(ll)))
+        return dict(logIncrements=li, ess=ess, resampled=res, logLikelihood=ll.value)
+
+    def ancestors(self, k=-1):
+        a = np.zeros(self.P, np.int32)
+        check(lib().bihip_pf_ancestors(self.h, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return a
+
+    def state_ptr(self):
+        X = ctypes.c_void_p()
+        ld = ctypes.c_int()
+        lws = ctypes.c_void_p()
+        check(lib().bihip_pf_state(self.h, ctypes.byref(X), ctypes.byref(ld), ctypes.byref(lws)))
+        return X.value, ld.value, lws.value
+
+    def position(self):
+        return lib().bihip_pf_position(self.h)
+
+    def set_timing(self, on):
+        check(lib().bihip_pf_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, kind):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        check(lib().bihip_pf_kernel_time(self.h, kind, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,864 @@
+// C ABI of the particle-filter library (declared in include/bihip.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace bihip {
+
+static thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+template <class T>
+static int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  BIHIP_HIP(hipMalloc((void**)p, n * sizeof(T)));
+  return BIHIP_OK;
+}
+
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static int ws_alloc(ScanWorkspace* ws, int tiles) {
+  ws->max_tiles = tiles;
+  int rc;
+  if ((rc = dalloc(&ws->state, 1))) return rc;
+  if ((rc = dalloc(&ws->flags, tiles))) return rc;
+  if ((rc = dalloc(&ws->aggregates, tiles))) return rc;
+  BIHIP_HIP(hipMemset(ws->state, 0, sizeof(ScanState)));
+  BIHIP_HIP(hipMemset(ws->flags, 0, sizeof(uint32_t) * tiles));
+  return BIHIP_OK;
+}
+
+static void ws_free(ScanWorkspace* ws) {
+  dfree(ws->state);
+  dfree(ws->flags);
+  dfree(ws->aggregates);
+  *ws = ScanWorkspace{};
+}
+
+static int check_model(const bihip_model* m) {
+  if (m == nullptr) return set_error(BIHIP_EINVAL, "model is NULL");
+  if (!model_supported(m))
+    return set_error(BIHIP_EINVAL,
+                     "unsupported model (model %d, d %d, integrator %d, precision %d)",
+                     m->model, m->d, m->integrator, m->precision);
+  if (m->integrator == BIHIP_DOPRI5 && m->nsteps <= 0)
+    return set_error(BIHIP_EINVAL, "nsteps must be positive");
+  if (!(m->h0 > 0.0)) return set_error(BIHIP_EINVAL, "h0 must be positive");
+  if (m->integrator == BIHIP_DOPRI5 && m->nsteps > RK4_MAX_STEPS)
+    return set_error(BIHIP_EINVAL, "nsteps above %d", RK4_MAX_STEPS);
+  return BIHIP_OK;
+}
+
+// An interval must take a bounded number of fixed RK4 steps.
+static int check_interval(const bihip_model* m, double t1, double t2) {
+  if (!(t1 <= t2)) return set_error(BIHIP_EINVAL, "need t1 <= t2");
+  if (m->integrator == BIHIP_RK4 && (t2 - t1) / m->h0 > 0.5 * RK4_MAX_STEPS)
+    return set_error(BIHIP_EINVAL, "interval needs more than %d RK4 steps", RK4_MAX_STEPS / 2);
+  return BIHIP_OK;
+}
+
+static int nobs_of(const bihip_model* m) {
+  switch (m->model) {
+    case BIHIP_MODEL_L96: return m->d / 2;
+    default: return 1;
+  }
+}
+
+}  // namespace bihip
+
+using namespace bihip;
+
+// ================================================================ context
+struct bihip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // scratch for the standalone primitives (grown on demand)
+  double* partials = nullptr;
+  int partials_cap = 0;
+  double* out = nullptr;  // 5 doubles
+  ScanWorkspace ws{};
+  uint64_t* EF = nullptr;
+  int EF_cap = 0;
+};
+
+static int ctx_enter(bihip_ctx* ctx) {
+  if (ctx == nullptr) return set_error(BIHIP_EINVAL, "context is NULL");
+  BIHIP_HIP(hipSetDevice(ctx->device));
+  return BIHIP_OK;
+}
+
+static int ctx_partials(bihip_ctx* ctx, int nb) {
+  if (nb > ctx->partials_cap) {
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+    dfree(ctx->partials);
+    ctx->partials_cap = 0;
+    int rc = dalloc(&ctx->partials, (size_t)3 * nb);
+    if (rc) return rc;
+    ctx->partials_cap = nb;
+  }
+  return BIHIP_OK;
+}
+
+static int ctx_scan(bihip_ctx* ctx, int P) {
+  const int nt = scan_tiles(P);
+  if (nt > ctx->ws.max_tiles) {
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+    ws_free(&ctx->ws);
+    int rc = ws_alloc(&ctx->ws, nt);
+    if (rc) return rc;
+  }
+  return BIHIP_OK;
+}
+
+static int ctx_EF(bihip_ctx* ctx, int P) {
+  if (P > ctx->EF_cap) {
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+    dfree(ctx->EF);
+    ctx->EF_cap = 0;
+    int rc = dalloc(&ctx->EF, P);
+    if (rc) return rc;
+    ctx->EF_cap = P;
+  }
+  return BIHIP_OK;
+}
+
+extern "C" {
+
+const char* bihip_last_error(void) { return g_error.c_str(); }
+
+int bihip_ctx_create(int device, bihip_ctx** out) {
+  if (out == nullptr) return set_error(BIHIP_EINVAL, "out is NULL");
+  int n = 0;
+  BIHIP_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_error(BIHIP_EINVAL, "no device %d", device);
+  BIHIP_HIP(hipSetDevice(device));
+  bihip_ctx* c = new bihip_ctx();
+  c->device = device;
+  int rc = dalloc(&c->out, 5);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return BIHIP_OK;
+}
+
+int bihip_ctx_destroy(bihip_ctx* ctx) {
+  if (ctx == nullptr) return BIHIP_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  dfree(ctx->partials);
+  dfree(ctx->out);
+  dfree(ctx->EF);
+  ws_free(&ctx->ws);
+  delete ctx;
+  return BIHIP_OK;
+}
+
+int bihip_ctx_set_stream(bihip_ctx* ctx, void* stream) {
+  if (ctx == nullptr) return set_error(BIHIP_EINVAL, "context is NULL");
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return BIHIP_OK;
+}
+
+int bihip_ctx_synchronize(bihip_ctx* ctx) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  return BIHIP_OK;
+}
+
+int bihip_model_nobs(const bihip_model* m) { return m ? nobs_of(m) : 0; }
+
+int bihip_model_elem_size(const bihip_model* m) {
+  return m && m->precision == BIHIP_F64 ? 8 : 4;
+}
+
+// ================================================================ random numbers
+int bihip_philox(bihip_ctx* ctx, const uint32_t* ctr, uint32_t k0, uint32_t k1, int n,
+                 uint32_t* out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  return launch_philox(ctr, k0, k1, n, out, ctx->stream);
+}
+
+int bihip_uniforms(bihip_ctx* ctx, uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch,
+                   int n, double* u) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  return launch_uniforms(seed, step, tag, batch, n, u, ctx->stream);
+}
+
+int bihip_normals(bihip_ctx* ctx, uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                  uint32_t batch, int n, int precision, void* z) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  return launch_normals(seed, index, step, tag, batch, n, precision, z, ctx->stream);
+}
+
+// ================================================================ model
+int bihip_init_samples(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                       int P, void* X, int ld, const double* mean0_host) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  double* mean0 = nullptr;
+  if (m->model == BIHIP_MODEL_L96) {
+    if (mean0_host == nullptr) return set_error(BIHIP_EINVAL, "L96 needs mean0");
+    if ((rc = dalloc(&mean0, m->d))) return rc;
+    BIHIP_HIP(hipMemcpyAsync(mean0, mean0_host, sizeof(double) * m->d, hipMemcpyHostToDevice,
+                             ctx->stream));
+  }
+  rc = launch_init_samples(m, seed, batch, P, X, ld, mean0, ctx->stream);
+  if (mean0) {
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+    dfree(mean0);
+  }
+  return rc;
+}
+
+int bihip_transition(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                     uint32_t step, double t1, double t2, int P, const void* Xin, void* Xout,
+                     int ld, const int* as, int* nsteps) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  if (as != nullptr && Xin == Xout)
+    return set_error(BIHIP_EINVAL, "a gathering transition cannot run in place");
+  if ((rc = check_interval(m, t1, t2))) return rc;
+  TransLaunch L{};
+  L.model = m;
+  L.Xin = Xin;
+  L.Xout = Xout;
+  L.ld = ld;
+  L.P = P;
+  L.as = as;
+  L.t1 = t1;
+  L.t2 = t2;
+  L.seed = seed;
+  L.step = step;
+  L.batch = batch;
+  L.nsteps_out = nsteps;
+  return launch_transition(L, ctx->stream);
+}
+
+int bihip_obs_logdensity(bihip_ctx* ctx, const bihip_model* m, const double* y, int P,
+                         const void* X, int ld, double* lws) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  return launch_obs_logdensity(m, y, P, X, ld, lws, ctx->stream);
+}
+
+int bihip_obs_sample(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
+                     uint32_t step, int P, const void* X, int ld, double* y) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  return launch_obs_sample(m, seed, batch, step, P, X, ld, y, ctx->stream);
+}
+
+// ================================================================ reductions
+static int reduce_to_out(bihip_ctx* ctx, const double* lws, int P, double host[5]) {
+  if (P <= 0) return set_error(BIHIP_EINVAL, "reduction of an empty vector");
+  const int nb = ceil_div(P, RED_TILE);
+  int rc = ctx_partials(ctx, nb);
+  if (rc) return rc;
+  if ((rc = launch_lws_partials(lws, P, ctx->partials, ctx->stream))) return rc;
+  if ((rc = launch_finalize(ctx->partials, nb, P, ctx->out, ctx->stream))) return rc;
+  if (host) {
+    BIHIP_HIP(hipMemcpyAsync(host, ctx->out, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return BIHIP_OK;
+}
+
+int bihip_ess_reduce(bihip_ctx* ctx, const double* lws, int P, double* ess, double* lW) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  double h[5];
+  if ((rc = reduce_to_out(ctx, lws, P, h))) return rc;
+  if (ess) *ess = h[3];
+  if (lW) *lW = h[4];
+  return BIHIP_OK;
+}
+
+int bihip_logsumexp_reduce(bihip_ctx* ctx, const double* lws, int P, double* out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  double h[5];
+  if ((rc = reduce_to_out(ctx, lws, P, h))) return rc;
+  if (out) *out = h[0] + log(h[1]);
+  return BIHIP_OK;
+}
+
+int bihip_max_reduce(bihip_ctx* ctx, const double* x, int n, double* out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  double h[5];
+  if ((rc = reduce_to_out(ctx, x, n, h))) return rc;
+  if (out) *out = h[0];
+  return BIHIP_OK;
+}
+
+int bihip_sumexpu_inclusive_scan(bihip_ctx* ctx, const double* lws, int P, double* Ws,
+                                 double* mx_host) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P <= 0) return set_error(BIHIP_EINVAL, "scan of an empty vector");
+  if ((rc = reduce_to_out(ctx, lws, P, nullptr))) return rc;
+  if ((rc = ctx_scan(ctx, P))) return rc;
+  if ((rc = launch_sumexp_scan(lws, P, ctx->out, nullptr, Ws, ctx->ws, ctx->stream))) return rc;
+  if (mx_host) {
+    BIHIP_HIP(hipMemcpyAsync(mx_host, ctx->out, sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return BIHIP_OK;
+}
+
+// ================================================================ resamplers
+// The host branch `if (pre.W > 0)` of the reference (SystematicResampler.hpp:118).
+static int check_total(bihip_ctx* ctx, const double* Ws, int P) {
+  if (P <= 0) return set_error(BIHIP_EINVAL, "no particles");
+  double W = 0.0;
+  BIHIP_HIP(hipMemcpyAsync(&W, Ws + (P - 1), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  if (!(W > 0.0)) return set_error(BIHIP_DEGENERATE, "particle filter degenerated (W <= 0)");
+  return BIHIP_OK;
+}
+
+int bihip_systematic_cumulative_offspring(bihip_ctx* ctx, const double* Ws, int P, int n,
+                                          double a, int* Os) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n <= 0) return set_error(BIHIP_EINVAL, "n must be positive");
+  if (!(a >= 0.0 && a <= 1.0)) return set_error(BIHIP_EINVAL, "a must lie in [0,1]");
+  if ((rc = check_total(ctx, Ws, P))) return rc;
+  OffspringLaunch L{};
+  L.mode = BIHIP_SYSTEMATIC;
+  L.Ws = Ws;
+  L.P = P;
+  L.n = n;
+  L.a = a;
+  L.Os = Os;
+  return launch_cumulative_offspring(L, ctx->stream);
+}
+
+int bihip_stratified_cumulative_offspring(bihip_ctx* ctx, const double* Ws, int P, int n,
+                                          const double* alphas, int* Os) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n <= 0) return set_error(BIHIP_EINVAL, "n must be positive");
+  if (alphas == nullptr) return set_error(BIHIP_EINVAL, "alphas is NULL");
+  if ((rc = check_total(ctx, Ws, P))) return rc;
+  OffspringLaunch L{};
+  L.mode = BIHIP_STRATIFIED;
+  L.Ws = Ws;
+  L.P = P;
+  L.n = n;
+  L.alphas = alphas;
+  L.Os = Os;
+  return launch_cumulative_offspring(L, ctx->stream);
+}
+
+int bihip_multinomial_ancestors(bihip_ctx* ctx, const double* Ws, int P, int n, const double* U,
+                                int* as) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  if (U == nullptr) return set_error(BIHIP_EINVAL, "U is NULL");
+  if ((rc = check_total(ctx, Ws, P))) return rc;
+  MultinomialLaunch L{};
+  L.Ws = Ws;
+  L.P = P;
+  L.n = n;
+  L.U = U;
+  L.as = as;
+  return launch_multinomial(L, ctx->stream);
+}
+
+int bihip_ancestors_to_offspring(bihip_ctx* ctx, const int* as, int n, int P, int* os) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0 || P <= 0) return set_error(BIHIP_EINVAL, "bad sizes");
+  if ((rc = launch_zero_ints(os, P, nullptr, ctx->stream))) return rc;
+  return launch_ancestors_to_offspring(as, n, P, os, ctx->stream);
+}
+
+int bihip_offspring_to_cumulative(bihip_ctx* ctx, const int* os, int P, int* Os) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P <= 0) return set_error(BIHIP_EINVAL, "P must be positive");
+  if ((rc = ctx_scan(ctx, P))) return rc;
+  return launch_count_scan(os, P, Os, ctx->ws, ctx->stream);
+}
+
+int bihip_cumulative_offspring_to_ancestors(bihip_ctx* ctx, const int* Os, int P, int* as) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P <= 0) return set_error(BIHIP_EINVAL, "P must be positive");
+  return launch_cumulative_to_ancestors(Os, P, as, ctx->stream);
+}
+
+int bihip_cumulative_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* Os, int P,
+                                                    int* as) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P <= 0) return set_error(BIHIP_EINVAL, "P must be positive");
+  if ((rc = ctx_scan(ctx, P))) return rc;
+  if ((rc = ctx_EF(ctx, P))) return rc;
+  return launch_ancestors_permute(Os, true, P, as, ctx->EF, ctx->ws, nullptr, ctx->stream);
+}
+
+int bihip_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* os, int P, int* as) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P <= 0) return set_error(BIHIP_EINVAL, "P must be positive");
+  if ((rc = ctx_scan(ctx, P))) return rc;
+  if ((rc = ctx_EF(ctx, P))) return rc;
+  return launch_ancestors_permute(os, false, P, as, ctx->EF, ctx->ws, nullptr, ctx->stream);
+}
+
+int bihip_gather_rows(bihip_ctx* ctx, const int* as, int P, int ncols, void* X, int ld,
+                      int elem) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (P < 0 || ncols < 0 || ld < P) return set_error(BIHIP_EINVAL, "bad sizes");
+  return launch_gather_rows(as, P, ncols, X, ld, elem, ctx->stream);
+}
+
+}  // extern "C"
+
+// ================================================================ filter
+struct bihip_pf {
+  bihip_ctx* ctx = nullptr;
+  bihip_model model{};
+  bihip_pf_config cfg{};
+  int ld = 0, elem = 4, nobs = 1, nparts = 0;
+  void* X[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* lws = nullptr;
+  double* Ws = nullptr;
+  int* Os = nullptr;
+  int* os = nullptr;
+  int* as = nullptr;
+  uint64_t* EF = nullptr;
+  double* partials = nullptr;
+  double* out = nullptr;
+  PFDevice* dev = nullptr;
+  double* y = nullptr;
+  double* hist_li = nullptr;
+  double* hist_ess = nullptr;
+  int* hist_res = nullptr;
+  int* hist_anc = nullptr;
+  ScanWorkspace ws{};
+  int k = 0;
+  bool ready = false, have_obs = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev[3];
+  size_t ev_used[3] = {0, 0, 0};
+  // graphs (one per buffer parity)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
+  auto& v = pf->ev[kind];
+  size_t& u = pf->ev_used[kind];
+  while (v.size() < u + 2) {
+    hipEvent_t e;
+    BIHIP_HIP(hipEventCreate(&e));
+    v.push_back(e);
+  }
+  *a = v[u];
+  *b = v[u + 1];
+  u += 2;
+  return BIHIP_OK;
+}
+
+// One observation interval: resample (device-gated) -> record -> predict+correct -> reduce.
+static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
+  const int P = pf->cfg.P;
+  int rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // ---- resample (Resampler::resample, gated by dev->resample)
+  if (timed) {
+    if ((rc = pf_event_pair(pf, 1, &e0, &e1))) return rc;
+    BIHIP_HIP(hipEventRecord(e0, s));
+  }
+  if ((rc = launch_sumexp_scan(pf->lws, P, &pf->dev->mx, pf->dev, pf->Ws, pf->ws, s))) return rc;
+  if (pf->cfg.resampler == BIHIP_MULTINOMIAL) {
+    if ((rc = launch_zero_ints(pf->os, P, pf->dev, s))) return rc;
+    MultinomialLaunch L{};
+    L.Ws = pf->Ws;
+    L.P = P;
+    L.n = P;
+    L.seed = pf->cfg.seed;
+    L.batch = pf->cfg.batch;
+    L.gate = pf->dev;
+    L.status = pf->dev;
+    L.os = pf->os;
+    if ((rc = launch_multinomial(L, s))) return rc;
+    if ((rc = launch_ancestors_permute(pf->os, false, P, pf->as, pf->EF, pf->ws, pf->dev, s)))
+      return rc;
+  } else {
+    OffspringLaunch L{};
+    L.mode = pf->cfg.resampler;
+    L.Ws = pf->Ws;
+    L.P = P;
+    L.n = P;
+    L.philox = true;
+    L.seed = pf->cfg.seed;
+    L.batch = pf->cfg.batch;
+    L.gate = pf->dev;
+    L.status = pf->dev;
+    L.Os = pf->Os;
+    if ((rc = launch_cumulative_offspring(L, s))) return rc;
+    if ((rc = launch_ancestors_permute(pf->Os, true, P, pf->as, pf->EF, pf->ws, pf->dev, s)))
+      return rc;
+  }
+  if (pf->hist_anc) {
+    if ((rc = launch_record_ancestors(pf->as, P, pf->dev, pf->hist_anc, s))) return rc;
+  }
+  if (timed) BIHIP_HIP(hipEventRecord(e1, s));
+
+  // ---- predict + correct (fused, ancestor gather on read)
+  if (timed) {
+    if ((rc = pf_event_pair(pf, 0, &e0, &e1))) return rc;
+    BIHIP_HIP(hipEventRecord(e0, s));
+  }
+  TransLaunch T{};
+  T.model = &pf->model;
+  T.Xin = pf->X[pf->cur];
+  T.Xout = pf->X[1 - pf->cur];
+  T.ld = pf->ld;
+  T.P = P;
+  T.as = pf->as;
+  T.lws = pf->lws;
+  T.seed = pf->cfg.seed;
+  T.batch = pf->cfg.batch;
+  T.partials = pf->partials;
+  T.dev = pf->dev;
+  T.t0 = pf->cfg.t0;
+  T.dt = pf->cfg.dt;
+  T.y_base = pf->y;
+  T.nobs = pf->nobs;
+  if ((rc = launch_transition(T, s))) return rc;
+  if (timed) BIHIP_HIP(hipEventRecord(e1, s));
+
+  // ---- BootstrapPF::correct bookkeeping
+  if (timed) {
+    if ((rc = pf_event_pair(pf, 2, &e0, &e1))) return rc;
+    BIHIP_HIP(hipEventRecord(e0, s));
+  }
+  PFHistory H{pf->hist_li, pf->hist_ess, pf->hist_res};
+  if ((rc = launch_pf_correct_finalize(pf->partials, pf->nparts, P, pf->cfg.essRel, pf->dev, H,
+                                       s)))
+    return rc;
+  if (timed) BIHIP_HIP(hipEventRecord(e1, s));
+  return BIHIP_OK;
+}
+
+static int pf_capture(bihip_pf* pf, int parity) {
+  hipStream_t cs;
+  BIHIP_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  const int saved = pf->cur;
+  pf->cur = parity;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = BIHIP_OK;
+  if (e == hipSuccess) {
+    rc = pf_enqueue_step(pf, cs, false);
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (rc == BIHIP_OK && e2 != hipSuccess)
+      rc = set_error(BIHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+  } else {
+    rc = set_error(BIHIP_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+  }
+  pf->cur = saved;
+  if (rc == BIHIP_OK) {
+    hipError_t e3 = hipGraphInstantiate(&pf->graph[parity], g, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) rc = set_error(BIHIP_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  return rc;
+}
+
+extern "C" {
+
+int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config* cfg,
+                    bihip_pf** out) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (cfg == nullptr || out == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (cfg->P <= 0 || cfg->T <= 0) return set_error(BIHIP_EINVAL, "need P > 0 and T > 0");
+  if (cfg->resampler < BIHIP_SYSTEMATIC || cfg->resampler > BIHIP_MULTINOMIAL)
+    return set_error(BIHIP_EINVAL, "unknown resampler %d", cfg->resampler);
+  if (!(cfg->essRel >= 0.0 && cfg->essRel <= 1.0))
+    return set_error(BIHIP_EINVAL, "essRel must lie in [0,1]");
+  if (!(cfg->dt > 0.0)) return set_error(BIHIP_EINVAL, "dt must be positive");
+  if ((rc = check_interval(m, cfg->t0, cfg->t0 + cfg->dt))) return rc;
+  bihip_pf* pf = new bihip_pf();
+  pf->ctx = ctx;
+  pf->model = *m;
+  pf->cfg = *cfg;
+  const int P = cfg->P;
+  pf->ld = (P + 63) / 64 * 64;
+  pf->elem = bihip_model_elem_size(m);
+  pf->nobs = nobs_of(m);
+  pf->nparts = transition_blocks(m, P);
+  const size_t xbytes = (size_t)pf->ld * m->d * pf->elem;
+#define PF_ALLOC(ptr, n)                       \
+  if ((rc = dalloc(&(ptr), (n))) != BIHIP_OK) { \
+    bihip_pf_destroy(pf);                      \
+    return rc;                                 \
+  }
+  uint8_t* x0 = nullptr;
+  uint8_t* x1 = nullptr;
+  PF_ALLOC(x0, xbytes);
+  PF_ALLOC(x1, xbytes);
+  pf->X[0] = x0;
+  pf->X[1] = x1;
+  PF_ALLOC(pf->lws, P);
+  PF_ALLOC(pf->Ws, P);
+  PF_ALLOC(pf->Os, P);
+  PF_ALLOC(pf->os, P);
+  PF_ALLOC(pf->as, P);
+  PF_ALLOC(pf->EF, P);
+  PF_ALLOC(pf->partials, (size_t)3 * pf->nparts);
+  PF_ALLOC(pf->out, 5);
+  PF_ALLOC(pf->dev, 1);
+  PF_ALLOC(pf->y, (size_t)cfg->T * pf->nobs);
+  PF_ALLOC(pf->hist_li, cfg->T);
+  PF_ALLOC(pf->hist_ess, cfg->T);
+  PF_ALLOC(pf->hist_res, cfg->T);
+  if (cfg->record) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
+#undef PF_ALLOC
+  if ((rc = ws_alloc(&pf->ws, scan_tiles(P)))) {
+    bihip_pf_destroy(pf);
+    return rc;
+  }
+  *out = pf;
+  return BIHIP_OK;
+}
+
+int bihip_pf_destroy(bihip_pf* pf) {
+  if (pf == nullptr) return BIHIP_OK;
+  (void)hipSetDevice(pf->ctx->device);
+  (void)hipStreamSynchronize(pf->ctx->stream);
+  for (int i = 0; i < 2; ++i) {
+    if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
+    dfree(pf->X[i]);
+  }
+  for (auto& v : pf->ev)
+    for (auto e : v) (void)hipEventDestroy(e);
+  dfree(pf->lws);
+  dfree(pf->Ws);
+  dfree(pf->Os);
+  dfree(pf->os);
+  dfree(pf->as);
+  dfree(pf->EF);
+  dfree(pf->partials);
+  dfree(pf->out);
+  dfree(pf->dev);
+  dfree(pf->y);
+  dfree(pf->hist_li);
+  dfree(pf->hist_ess);
+  dfree(pf->hist_res);
+  dfree(pf->hist_anc);
+  ws_free(&pf->ws);
+  delete pf;
+  return BIHIP_OK;
+}
+
+int bihip_pf_set_observations(bihip_pf* pf, const double* y_host) {
+  if (pf == nullptr || y_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipMemcpyAsync(pf->y, y_host, sizeof(double) * pf->cfg.T * pf->nobs,
+                           hipMemcpyHostToDevice, pf->ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  pf->have_obs = true;
+  return BIHIP_OK;
+}
+
+int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  pf->cur = 0;
+  pf->k = 0;
+  if ((rc = bihip_init_samples(pf->ctx, &pf->model, pf->cfg.seed, pf->cfg.batch, pf->cfg.P,
+                               pf->X[0], pf->ld, mean0_host)))
+    return rc;
+  BIHIP_HIP(hipMemsetAsync(pf->lws, 0, sizeof(double) * pf->cfg.P, s));
+  PFDevice d{};
+  d.status = BIHIP_OK;
+  BIHIP_HIP(hipMemcpyAsync(pf->dev, &d, sizeof(d), hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipMemsetAsync(pf->hist_li, 0, sizeof(double) * pf->cfg.T, s));
+  BIHIP_HIP(hipMemsetAsync(pf->hist_ess, 0, sizeof(double) * pf->cfg.T, s));
+  BIHIP_HIP(hipMemsetAsync(pf->hist_res, 0, sizeof(int) * pf->cfg.T, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  for (auto& u : pf->ev_used) u = 0;
+  pf->ready = true;
+  return BIHIP_OK;
+}
+
+int bihip_pf_step(bihip_pf* pf, int n) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  if (!pf->ready) return set_error(BIHIP_ESTATE, "bihip_pf_init has not been called");
+  if (!pf->have_obs) return set_error(BIHIP_ESTATE, "no observations set");
+  if (n < 0 || pf->k + n > pf->cfg.T)
+    return set_error(BIHIP_EINVAL, "steps beyond the schedule (k=%d, n=%d, T=%d)", pf->k, n,
+                     pf->cfg.T);
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  for (int i = 0; i < n; ++i) {
+    if (pf->cfg.use_graph && !pf->timing) {
+      if (pf->graph[pf->cur] == nullptr) {
+        if ((rc = pf_capture(pf, pf->cur))) return rc;
+      }
+      BIHIP_HIP(hipGraphLaunch(pf->graph[pf->cur], s));
+    } else {
+      if ((rc = pf_enqueue_step(pf, s, pf->timing))) return rc;
+    }
+    pf->cur = 1 - pf->cur;
+    ++pf->k;
+  }
+  return BIHIP_OK;
+}
+
+int bihip_pf_term(bihip_pf* pf, double* logZ) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  double h[5];
+  const int P = pf->cfg.P;
+  const int nb = ceil_div(P, RED_TILE);
+  int rc2 = ctx_partials(pf->ctx, nb);
+  if (rc2) return rc2;
+  hipStream_t s = pf->ctx->stream;
+  if ((rc = launch_lws_partials(pf->lws, P, pf->ctx->partials, s))) return rc;
+  if ((rc = launch_finalize(pf->ctx->partials, nb, P, pf->out, s))) return rc;
+  BIHIP_HIP(hipMemcpyAsync(h, pf->out, sizeof(h), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  if (logZ) *logZ = h[0] + log(h[1]) - log((double)P);
+  return bihip_pf_status(pf);
+}
+
+int bihip_pf_status(bihip_pf* pf) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  PFDevice d;
+  ScanState st;
+  hipStream_t s = pf->ctx->stream;
+  BIHIP_HIP(hipMemcpyAsync(&d, pf->dev, sizeof(d), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipMemcpyAsync(&st, pf->ws.state, sizeof(st), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  if (st.error) return set_error(BIHIP_EHIP, "scan look-back exceeded its spin limit");
+  if (d.status == BIHIP_DEGENERATE)
+    return set_error(BIHIP_DEGENERATE, "particle filter degenerated (W <= 0)");
+  return BIHIP_OK;
+}
+
+int bihip_pf_outputs(bihip_pf* pf, double* li, double* ess, int* res, double* ll) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  const int T = pf->cfg.T;
+  if (li) BIHIP_HIP(hipMemcpyAsync(li, pf->hist_li, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+  if (ess)
+    BIHIP_HIP(hipMemcpyAsync(ess, pf->hist_ess, sizeof(double) * T, hipMemcpyDeviceToHost, s));
+  if (res) BIHIP_HIP(hipMemcpyAsync(res, pf->hist_res, sizeof(int) * T, hipMemcpyDeviceToHost, s));
+  PFDevice d;
+  BIHIP_HIP(hipMemcpyAsync(&d, pf->dev, sizeof(d), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  if (ll) *ll = d.logLikelihood;
+  return BIHIP_OK;
+}
+
+int bihip_pf_ancestors(bihip_pf* pf, int k, int* as_host) {
+  if (pf == nullptr || as_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  const int P = pf->cfg.P;
+  if (k >= 0) {
+    if (pf->hist_anc == nullptr) return set_error(BIHIP_ESTATE, "ancestors were not recorded");
+    if (k >= pf->k) return set_error(BIHIP_EINVAL, "step %d not run yet", k);
+    BIHIP_HIP(hipMemcpyAsync(as_host, pf->hist_anc + (size_t)k * P, sizeof(int) * P,
+                             hipMemcpyDeviceToHost, s));
+  } else {
+    BIHIP_HIP(hipMemcpyAsync(as_host, pf->as, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  }
+  BIHIP_HIP(hipStreamSynchronize(s));
+  return BIHIP_OK;
+}
+
+int bihip_pf_state(bihip_pf* pf, void** X, int* ld, double** lws) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  if (X) *X = pf->X[pf->cur];
+  if (ld) *ld = pf->ld;
+  if (lws) *lws = pf->lws;
+  return BIHIP_OK;
+}
+
+int bihip_pf_position(bihip_pf* pf) { return pf ? pf->k : -1; }
+
+int bihip_pf_set_timing(bihip_pf* pf, int enable) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  pf->timing = enable != 0;
+  for (auto& u : pf->ev_used) u = 0;
+  return BIHIP_OK;
+}
+
+int bihip_pf_kernel_time(bihip_pf* pf, int kind, double* ms, long* count) {
+  if (pf == nullptr || kind < 0 || kind > 2) return set_error(BIHIP_EINVAL, "bad argument");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  double total = 0.0;
+  const size_t used = pf->ev_used[kind];
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    float t = 0.f;
+    BIHIP_HIP(hipEventElapsedTime(&t, pf->ev[kind][i], pf->ev[kind][i + 1]));
+    total += t;
+  }
+  if (ms) *ms = total;
+  if (count) *count = (long)(used / 2);
+  return BIHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Internal (host-side) interfaces between the kernel files and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bihip.h"
+
+namespace bihip {
+
+// ---------------------------------------------------------------- errors
+// Sets the calling thread's last-error string; returns the status code.
+int set_error(int code, const char* fmt, ...);
+
+#define BIHIP_HIP(call)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return ::bihip::set_error(BIHIP_EHIP, "%s failed: %s (%s:%d)", #call,      \
+                                hipGetErrorString(e_), __FILE__, __LINE__);      \
+  } while (0)
+
+#define BIHIP_LAUNCH_CHECK()                                                     \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess)                                                        \
+      return ::bihip::set_error(BIHIP_EHIP, "kernel launch failed: %s (%s:%d)",  \
+                                hipGetErrorString(e_), __FILE__, __LINE__);      \
+  } while (0)
+
+struct ModelArgs;
+
+// Filter scalars kept on the device so that a step never waits for the host.
+struct PFDevice {
+  double logLikelihood;  // running log-likelihood (FilterState::logLikelihood)
+  double ess;            // ESS after the last correct
+  double mx;             // max log-weight after the last correct
+  double lW;             // last incremental estimate
+  double W;              // sum of weights found by the last resampling scan
+  int resample;          // 1: resample at the current time (ess < essRel * P)
+  int status;            // BIHIP_OK or BIHIP_DEGENERATE
+  int k;                 // steps completed
+  int pad;
+};
+
+// Per-step histories on the device (T entries each; ancestors T x P when recorded).
+struct PFHistory {
+  double* logIncrements;
+  double* ess;
+  int* resampled;
+};
+
+// Single-pass scan bookkeeping (ticket and epoch, then per-tile aggregates).
+struct ScanState {
+  uint32_t ticket;
+  uint32_t epoch;
+  uint32_t error;
+  uint32_t pad;
+};
+
+struct ScanWorkspace {
+  ScanState* state;     // device
+  uint32_t* flags;      // device, one per tile
+  uint64_t* aggregates; // device, one per tile (bit pattern of the value)
+  int max_tiles;
+};
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int RED_BLOCK = 256;
+constexpr int RED_ITEMS = 8;
+constexpr int RED_TILE = RED_BLOCK * RED_ITEMS;
+constexpr int TRANS_BLOCK = 256;
+constexpr int FINAL_BLOCK = 1024;
+// RK4 steps per interval are bounded (t + h must advance t).
+constexpr int RK4_MAX_STEPS = 1 << 20;
+
+__host__ __device__ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---- transition.hip
+struct TransLaunch {
+  const bihip_model* model;
+  const void* Xin;
+  void* Xout;
+  int ld, P;
+  const int* as;            // gather rows (nullable)
+  double* lws;              // nullable
+  const double* y;          // nullable: no correction
+  double t1, t2;
+  uint64_t seed;
+  uint32_t step, batch;
+  int* nsteps_out;          // nullable
+  double* partials;         // nullable: 3 doubles per block
+  const PFDevice* dev;      // nullable: device-side resample decision
+  // with dev: the step k is read from dev->k; t1 = t0 + k dt, t2 = t0 + (k+1) dt,
+  // Philox step k+1 and observations y_base + k * nobs
+  double t0, dt;
+  const double* y_base;
+  int nobs;
+};
+int transition_blocks(const bihip_model* m, int P);
+int launch_transition(const TransLaunch& L, hipStream_t s);
+int launch_init_samples(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X,
+                        int ld, const double* mean0_dev, hipStream_t s);
+int launch_obs_logdensity(const bihip_model* m, const double* y, int P, const void* X, int ld,
+                          double* lws, hipStream_t s);
+int launch_obs_sample(const bihip_model* m, uint64_t seed, uint32_t batch, uint32_t step, int P,
+                      const void* X, int ld, double* y, hipStream_t s);
+int launch_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, int n, uint32_t* out,
+                  hipStream_t s);
+int launch_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, int n, double* u,
+                    hipStream_t s);
+int launch_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+                   int n, int precision, void* z, hipStream_t s);
+bool model_supported(const bihip_model* m);
+
+// ---- reduce.hip
+// Per-tile (max, sum w, sum w^2) partials of log-weights.
+int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t s);
+// Combine nb partials: out = {mx, S1, S2, ess, lW} (5 doubles).
+int launch_finalize(const double* partials, int nb, int P, double* out, hipStream_t s);
+// Combine partials into the filter scalars and histories.
+int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
+                               PFDevice* dev, PFHistory hist, hipStream_t s);
+
+// ---- scan.hip
+int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,
+                       double* Ws, ScanWorkspace ws, hipStream_t s);
+int launch_count_scan(const int* os, int P, int* Os, ScanWorkspace ws, hipStream_t s);
+int scan_tiles(int P);
+
+// ---- resample.hip
+// mode: 0 systematic (a from uniform a_host or Philox when gate != nullptr),
+//       1 stratified (alphas array or Philox), with Ws/W on device.
+struct OffspringLaunch {
+  int mode;                 // BIHIP_SYSTEMATIC / BIHIP_STRATIFIED
+  const double* Ws;
+  int P, n;
+  double a;                 // systematic offset (when alphas == nullptr and !philox)
+  const double* alphas;     // stratified alphas (nullable)
+  bool philox;              // draw a / alphas from Philox(seed, step, tag, batch)
+  uint64_t seed;
+  uint32_t step, batch;
+  const PFDevice* gate;     // nullable: skip unless gate->resample; step = gate->k
+  PFDevice* status;         // nullable: where to flag W <= 0
+  int* Os;
+};
+int launch_cumulative_offspring(const OffspringLaunch& L, hipStream_t s);
+// Permutation: counts-or-cumulative input -> ancestors in the host layout.
+int launch_ancestors_permute(const int* in, bool cumulative, int P, int* as, uint64_t* EF,
+                             ScanWorkspace ws, const PFDevice* gate, hipStream_t s);
+int launch_cumulative_to_ancestors(const int* Os, int P, int* as, hipStream_t s);
+struct MultinomialLaunch {
+  const double* Ws;
+  int P, n;
+  const double* U;          // nullable: draw from Philox
+  uint64_t seed;
+  uint32_t step, batch;
+  const PFDevice* gate;     // nullable: skip unless gate->resample; step = gate->k
+  PFDevice* status;
+  int* as;                  // nullable: ancestors out
+  int* os;                  // nullable: offspring counts (atomic histogram; must be zeroed)
+};
+int launch_multinomial(const MultinomialLaunch& L, hipStream_t s);
+int launch_zero_ints(int* p, int n, const PFDevice* gate, hipStream_t s);
+int launch_ancestors_to_offspring(const int* as, int n, int P, int* os, hipStream_t s);
+int launch_gather_rows(const int* as, int P, int ncols, void* X, int ld, int elem,
+                       hipStream_t s);
+int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
+                            hipStream_t s);
+
+}  // namespace bihip

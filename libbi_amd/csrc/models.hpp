@@ -1,0 +1,353 @@
+// Device-side models and ODE integrators.
+//
+// One particle is owned by a group of LPP consecutive lanes of a wavefront;
+// lane g of the group holds state dimensions [g*DL, (g+1)*DL) in registers
+// (DL = D / LPP).  The Lorenz-96 stencil fetches its three halo values from
+// the neighbouring lanes of the group; error norms are group butterfly sums,
+// so every lane of a group computes bit-identical step-control decisions and
+// the group never diverges internally.
+//
+// Semantics follow the reference host integrators:
+//   RK4    share/src/bi/host/ode/RK4IntegratorHost.hpp:63-97,  share/src/bi/ode/RK4Stage.hpp:26-76
+//   DOPRI5 share/src/bi/host/ode/DOPRI5IntegratorHost.hpp:65-139, share/src/bi/ode/DOPRI5Stage.hpp:26-141,
+//          controller constants share/src/bi/host/ode/IntegratorConstants.cpp:77-87.
+// Step control (t, h, the controller) runs in double; stage arithmetic runs
+// in the state type T.
+#pragma once
+
+#include "common.hpp"
+
+namespace bihip {
+
+constexpr double HALF_LOG_TWO_PI = 0.91893853320467274178;
+
+// Arguments shared by the model kernels (uniform across the launch).
+struct ModelArgs {
+  int model, d, integrator, nsteps, host_quirk, precision;
+  double h0, atol, rtol;
+  double par[16];
+};
+
+// ---------------------------------------------------------------- lane groups
+template <int LPP>
+struct Group {
+  int lane;  // lane in the wavefront
+  int g;     // lane in the group
+  __device__ __forceinline__ Group() {
+    lane = (int)(threadIdx.x & (WAVE - 1));
+    g = lane & (LPP - 1);
+  }
+  template <class T>
+  __device__ __forceinline__ T prev(T v) const {
+    if constexpr (LPP == 1) {
+      return v;
+    } else {
+      const int src = (lane & ~(LPP - 1)) | ((g + LPP - 1) & (LPP - 1));
+      return __shfl(v, src, WAVE);
+    }
+  }
+  template <class T>
+  __device__ __forceinline__ T next(T v) const {
+    if constexpr (LPP == 1) {
+      return v;
+    } else {
+      const int src = (lane & ~(LPP - 1)) | ((g + 1) & (LPP - 1));
+      return __shfl(v, src, WAVE);
+    }
+  }
+  template <class T>
+  __device__ __forceinline__ T sum(T v) const {
+#pragma unroll
+    for (int o = 1; o < LPP; o <<= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+  }
+};
+
+// ---------------------------------------------------------------- models
+// PZ: docs/src/users.tex:36-85.  State (P, Z); alpha ~ N(mu, sigma) per interval.
+template <class T>
+struct ModelPZ {
+  static constexpr int D = 2, LPP = 1, DL = 2, NOBS = 1;
+  static constexpr bool NOISE_FIRST = true;
+  T c, e, ml, mq, alpha;
+  double mu, sigma, obs_sd;
+  __device__ __forceinline__ explicit ModelPZ(const ModelArgs& a) {
+    mu = a.par[0];
+    sigma = a.par[1];
+    c = (T)a.par[2];
+    e = (T)a.par[3];
+    ml = (T)a.par[4];
+    mq = (T)a.par[5];
+    obs_sd = a.par[6];
+    alpha = T(0);
+  }
+  template <class G>
+  __device__ __forceinline__ void rhs(const G&, const T (&x)[DL], T (&k)[DL]) const {
+    const T P = x[0], Z = x[1];
+    k[0] = alpha * P - c * P * Z;
+    k[1] = e * c * P * Z - ml * Z - mq * Z * Z;
+  }
+};
+
+// Lorenz-96: dx_i/dt = (x_{i+1} - x_{i-2}) x_{i-1} - x_i + F, cyclic.
+template <class T, int D_, int LPP_>
+struct ModelL96 {
+  static constexpr int D = D_, LPP = LPP_, DL = D_ / LPP_, NOBS = D_ / 2;
+  static constexpr bool NOISE_FIRST = false;
+  static_assert(D_ % LPP_ == 0 && DL >= 2 && DL % 2 == 0, "lane slice must be even, >= 2");
+  T F;
+  double noise_sd, obs_sd;
+  __device__ __forceinline__ explicit ModelL96(const ModelArgs& a) {
+    F = (T)a.par[0];
+    noise_sd = a.par[1];
+    obs_sd = a.par[2];
+  }
+  template <class G>
+  __device__ __forceinline__ void rhs(const G& grp, const T (&x)[DL], T (&k)[DL]) const {
+    T xm2, xm1, xp1;  // x[lo-2], x[lo-1], x[hi]
+    if constexpr (LPP == 1) {
+      xm2 = x[DL - 2];
+      xm1 = x[DL - 1];
+      xp1 = x[0];
+    } else {
+      xm2 = grp.prev(x[DL - 2]);
+      xm1 = grp.prev(x[DL - 1]);
+      xp1 = grp.next(x[0]);
+    }
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      const T a = (i + 1 < DL) ? x[i + 1 < DL ? i + 1 : 0] : xp1;
+      const T b = (i >= 2) ? x[i >= 2 ? i - 2 : 0] : (i == 1 ? xm1 : xm2);
+      const T c = (i >= 1) ? x[i >= 1 ? i - 1 : 0] : xm1;
+      k[i] = (a - b) * c - x[i] + F;
+    }
+  }
+};
+
+// Scalar linear-Gaussian: dx/dt = -lambda x.
+template <class T>
+struct ModelLG {
+  static constexpr int D = 1, LPP = 1, DL = 1, NOBS = 1;
+  static constexpr bool NOISE_FIRST = false;
+  T lambda;
+  double noise_sd, obs_sd;
+  __device__ __forceinline__ explicit ModelLG(const ModelArgs& a) {
+    lambda = (T)a.par[0];
+    noise_sd = a.par[1];
+    obs_sd = a.par[2];
+  }
+  template <class G>
+  __device__ __forceinline__ void rhs(const G&, const T (&x)[DL], T (&k)[DL]) const {
+    k[0] = -lambda * x[0];
+  }
+};
+
+// ---------------------------------------------------------------- integrators
+// RK4IntegratorHost::update for one particle; returns the number of steps.
+template <class M, class T, class G>
+__device__ __forceinline__ int integrate_rk4(const M& m, const G& grp, T (&x0)[M::DL],
+                                             double t1, double t2, double h0) {
+  constexpr int DL = M::DL;
+  double t = t1, h = h0;
+  int n = 0;
+  while (t < t2 && n < RK4_MAX_STEPS) {
+    if (t + 1.01 * h - t2 > 0.0) {
+      h = t2 - t;
+      if (h <= 0.0) break;
+    }
+    const T hT = (T)h;
+    T k[DL], xs[DL], x4[DL];
+    m.rhs(grp, x0, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      xs[i] = hT * (T(0.5) * k[i]) + x0[i];
+      x4[i] = T(1.0 / 6.0) * k[i];
+    }
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      x4[i] += T(1.0 / 3.0) * k[i];
+      xs[i] = hT * (T(0.5) * k[i]) + x0[i];
+    }
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      x4[i] += T(1.0 / 3.0) * k[i];
+      xs[i] = hT * k[i] + x0[i];
+    }
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      x4[i] += T(1.0 / 6.0) * k[i];
+      x0[i] = hT * x4[i] + x0[i];
+    }
+    t += h;
+    ++n;
+  }
+  return n;
+}
+
+template <class T>
+__device__ __forceinline__ T err_scale(T err, T h, T atol, T rtol, T a, T b) {
+  const T den = atol + rtol * fmax(fabs(a), fabs(b));
+  if constexpr (sizeof(T) == 4) {
+    return __fdividef(err * h, den);
+  } else {
+    return err * h / den;
+  }
+}
+
+// DOPRI5IntegratorHost::update for one particle; returns the number of steps.
+template <class M, class T, class G>
+__device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0)[M::DL],
+                                                double t1, double t2, const ModelArgs& a) {
+  constexpr int DL = M::DL;
+  const double safe = 0.9, facl = 0.2, facr = 10.0, beta = 0.04;
+  const double logsafe = -0.10536051565782630123;  // log(0.9)
+  const double expo = 0.5 * (0.2 - beta * 0.75);
+  const T atol = (T)a.atol, rtol = (T)a.rtol;
+  const double invD = 1.0 / (double)M::D;
+  const bool quirk = a.host_quirk != 0;
+  (void)safe;
+
+  T k1[DL];
+  double t = t1, h = a.h0, logfacold = -9.2103403719761836;  // log(1e-4)
+  bool k1in = false;
+  int n = 0;
+  while (t < t2 && n < a.nsteps) {
+    if (t + 1.01 * h - t2 > 0.0) {
+      h = t2 - t;
+      if (h <= 0.0) break;
+    }
+    const T hT = (T)h;
+    if (!k1in) m.rhs(grp, x0, k1);
+    k1in = true;
+    T xs[DL], a3[DL], a4[DL], a5[DL], a6[DL], y6[DL], er[DL], k[DL];
+    // stage 1
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      xs[i] = hT * (T(0.2) * k1[i]) + x0[i];
+      a3[i] = T(3.0 / 40.0) * k1[i];
+      a4[i] = T(44.0 / 45.0) * k1[i];
+      a5[i] = T(19372.0 / 6561.0) * k1[i];
+      a6[i] = T(9017.0 / 3168.0) * k1[i];
+      y6[i] = T(35.0 / 384.0) * k1[i];
+      er[i] = T(71.0 / 57600.0) * k1[i];
+    }
+    // stage 2
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      a3[i] += T(9.0 / 40.0) * k[i];
+      a4[i] += T(-56.0 / 15.0) * k[i];
+      a5[i] += T(-25360.0 / 2187.0) * k[i];
+      a6[i] += T(-355.0 / 33.0) * k[i];
+      xs[i] = hT * a3[i] + x0[i];
+    }
+    // stage 3
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      a4[i] += T(32.0 / 9.0) * k[i];
+      a5[i] += T(64448.0 / 6561.0) * k[i];
+      a6[i] += T(46732.0 / 5247.0) * k[i];
+      y6[i] += T(500.0 / 1113.0) * k[i];
+      er[i] += T(-71.0 / 16695.0) * k[i];
+      xs[i] = hT * a4[i] + x0[i];
+    }
+    // stage 4
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      a5[i] += T(-212.0 / 729.0) * k[i];
+      a6[i] += T(49.0 / 176.0) * k[i];
+      y6[i] += T(125.0 / 192.0) * k[i];
+      er[i] += T(71.0 / 1920.0) * k[i];
+      xs[i] = hT * a5[i] + x0[i];
+    }
+    // stage 5
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      a6[i] += T(-5103.0 / 18656.0) * k[i];
+      y6[i] += T(-2187.0 / 6784.0) * k[i];
+      er[i] += T(-17253.0 / 339200.0) * k[i];
+      xs[i] = hT * a6[i] + x0[i];
+    }
+    // stage 6: xs now holds x5
+    m.rhs(grp, xs, k);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      y6[i] += T(11.0 / 84.0) * k[i];
+      er[i] += T(22.0 / 525.0) * k[i];
+      y6[i] = hT * y6[i] + x0[i];
+    }
+    // stage 7 (FSAL): k7 at x6, or at x5 as the reference host does
+    T k7[DL];
+    if (quirk) {
+#pragma unroll
+      for (int i = 0; i < DL; ++i) k7[i] = k[i];
+    } else {
+      m.rhs(grp, y6, k7);
+    }
+    T e2l = T(0);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) {
+      er[i] += T(-1.0 / 40.0) * k7[i];
+      const T e = err_scale(er[i], hT, atol, rtol, x0[i], y6[i]);
+      e2l += e * e;
+    }
+    const double e2 = (double)grp.sum(e2l) * invD;
+    if (e2 <= 1.0) {
+      t += h;
+#pragma unroll
+      for (int i = 0; i < DL; ++i) {
+        x0[i] = y6[i];
+        k1[i] = k7[i];
+      }
+    }
+    if (t < t2) {
+      const double logfac11 = expo * log(e2);
+      if (e2 > 1.0) {
+        h *= fmax(facl, exp(logsafe - logfac11));
+      } else {
+        double fac = exp(beta * logfacold + logsafe - logfac11);
+        fac = fmin(facr, fmax(facl, fac));
+        h *= fac;
+        logfacold = 0.5 * log(fmax(e2, 1.0e-8));
+      }
+    }
+    ++n;
+  }
+  return n;
+}
+
+// Normals for the lane's slice [lo, lo+DL) of a D-vector drawn with the
+// layout "normal j = word pair (j%4)/2 of Philox block j/4".
+template <class T, int DL>
+__device__ __forceinline__ void slice_normals(uint64_t seed, uint32_t index, uint32_t step,
+                                              uint32_t tag, uint32_t batch, int lo,
+                                              T (&z)[DL]) {
+  constexpr int NB = (DL + 3) / 4 + 1;
+  T Z[4 * NB];
+  const int b0 = lo >> 2, s = lo & 3;
+  const bool need_last = (s + DL) > 4 * (NB - 1);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    T q[4];
+    if (b < NB - 1 || need_last) {
+      normals4<T>(seed, index, step, tag, batch, (uint32_t)(b0 + b), q);
+    } else {
+      q[0] = q[1] = q[2] = q[3] = T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Z[4 * b + i] = q[i];
+  }
+#pragma unroll
+  for (int i = 0; i < DL; ++i) {
+    const T v0 = Z[i], v1 = Z[i + 1], v2 = Z[i + 2], v3 = Z[i + 3];
+    z[i] = s == 0 ? v0 : (s == 1 ? v1 : (s == 2 ? v2 : v3));
+  }
+}
+
+}  // namespace bihip

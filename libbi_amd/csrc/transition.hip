@@ -1,0 +1,495 @@
+// Fused predict + correct kernel: (ancestor gather) -> noise -> ODE flow ->
+// observation log-density -> per-block weight partials.  One particle per
+// lane group; state in registers; each state column is read and written
+// once, coalesced across particles (structure-of-arrays).
+//
+// Replaces, for the ON_DEVICE location:
+//   Model::transitionSamples       share/tt/cpp/model.hpp.tt:566-573
+//   StaticSamplerHost::samples     share/src/bi/host/updater/StaticSamplerHost.hpp:43-65
+//   RK4/DOPRI5IntegratorHost       share/src/bi/host/ode/{RK4,DOPRI5}IntegratorHost.hpp
+//   observationLogDensities        share/src/bi/host/updater/SparseStaticLogDensityHost.hpp:46-68
+//   gather_rows (fused read)       share/src/bi/host/primitive/matrix_primitive.hpp:65-75
+#include "models.hpp"
+#include "blockreduce.hpp"
+#include "internal.hpp"
+
+namespace bihip {
+
+struct TransArgs {
+  ModelArgs m;
+  const void* Xin;
+  void* Xout;
+  int ld, P;
+  const int* as;
+  double* lws;
+  const double* y;
+  double t1, t2;
+  uint64_t seed;
+  uint32_t step, batch;
+  int* nsteps_out;
+  double* partials;
+  const PFDevice* dev;
+  double t0, dt;
+  const double* y_base;
+  int nobs;
+};
+
+static ModelArgs to_args(const bihip_model* m) {
+  ModelArgs a;
+  a.model = m->model;
+  a.d = m->d;
+  a.integrator = m->integrator;
+  a.nsteps = m->nsteps;
+  a.host_quirk = m->host_quirk;
+  a.precision = m->precision;
+  a.h0 = m->h0;
+  a.atol = m->atol;
+  a.rtol = m->rtol;
+  for (int i = 0; i < 16; ++i) a.par[i] = m->par[i];
+  return a;
+}
+
+// ---------------------------------------------------------------- observation terms
+// Gaussian / log-normal log-density terms of the lane's slice
+// (share/tt/cpp/action/gaussian.hpp.tt:48-70), in double.
+template <class T>
+__device__ __forceinline__ double obs_terms(const ModelPZ<T>& m, const T (&x)[2],
+                                            const double* y, int) {
+  const double yy = y[0];
+  const double mu = log((double)x[0]);
+  const double z = (log(yy) - mu) / m.obs_sd;
+  return -0.5 * (z * z) - HALF_LOG_TWO_PI - log(m.obs_sd * yy);
+}
+
+template <class T, int D, int LPP>
+__device__ __forceinline__ double obs_terms(const ModelL96<T, D, LPP>& m,
+                                            const T (&x)[ModelL96<T, D, LPP>::DL],
+                                            const double* y, int lo) {
+  constexpr int DL = ModelL96<T, D, LPP>::DL;
+  const double inv = 1.0 / m.obs_sd;
+  const double c = -HALF_LOG_TWO_PI - log(m.obs_sd);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < DL; i += 2) {
+    const double z = (y[(lo + i) >> 1] - (double)x[i]) * inv;
+    s += -0.5 * (z * z) + c;
+  }
+  return s;
+}
+
+template <class T>
+__device__ __forceinline__ double obs_terms(const ModelLG<T>& m, const T (&x)[1],
+                                            const double* y, int) {
+  const double z = (y[0] - (double)x[0]) / m.obs_sd;
+  return -0.5 * (z * z) - HALF_LOG_TWO_PI - log(m.obs_sd);
+}
+
+// ---------------------------------------------------------------- noise
+template <class T>
+__device__ __forceinline__ void noise_first(ModelPZ<T>& m, const TransArgs& a, int p,
+                                            uint32_t step) {
+  T z[4];
+  normals4<T>(a.seed, (uint32_t)p, step, TAG_NOISE, a.batch, 0, z);
+  m.alpha = (T)m.mu + (T)m.sigma * z[0];
+}
+
+template <class M>
+__device__ __forceinline__ void noise_after(const M& m, const TransArgs& a, int p, int lo,
+                                            uint32_t step, typename M::real_t (&x)[M::DL]) {
+  using T = typename M::real_t;
+  T z[M::DL];
+  slice_normals<T, M::DL>(a.seed, (uint32_t)p, step, TAG_NOISE, a.batch, lo, z);
+  const T sd = (T)m.noise_sd;
+#pragma unroll
+  for (int i = 0; i < M::DL; ++i) x[i] = x[i] + sd * z[i];
+}
+
+template <class T>
+struct PZ : ModelPZ<T> {
+  using real_t = T;
+  using ModelPZ<T>::ModelPZ;
+};
+template <class T, int D, int LPP>
+struct L96 : ModelL96<T, D, LPP> {
+  using real_t = T;
+  using ModelL96<T, D, LPP>::ModelL96;
+};
+template <class T>
+struct LG : ModelLG<T> {
+  using real_t = T;
+  using ModelLG<T>::ModelLG;
+};
+
+template <class T>
+__device__ __forceinline__ double obs_of(const PZ<T>& m, const T (&x)[2], const double* y,
+                                         int lo) {
+  return obs_terms(static_cast<const ModelPZ<T>&>(m), x, y, lo);
+}
+template <class T, int D, int LPP>
+__device__ __forceinline__ double obs_of(const L96<T, D, LPP>& m,
+                                         const T (&x)[ModelL96<T, D, LPP>::DL],
+                                         const double* y, int lo) {
+  return obs_terms(static_cast<const ModelL96<T, D, LPP>&>(m), x, y, lo);
+}
+template <class T>
+__device__ __forceinline__ double obs_of(const LG<T>& m, const T (&x)[1], const double* y,
+                                         int lo) {
+  return obs_terms(static_cast<const ModelLG<T>&>(m), x, y, lo);
+}
+
+// ---------------------------------------------------------------- the kernel
+template <class M, int INTEG>
+__global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
+  using T = typename M::real_t;
+  constexpr int LPP = M::LPP, DL = M::DL;
+  const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
+  const int p = tid / LPP;
+  const Group<LPP> grp;
+  const int lo = grp.g * DL;
+  const bool active = p < a.P;
+
+  bool gather = a.as != nullptr;
+  bool lws_const = false;
+  double lws_base = 0.0;
+  double t1 = a.t1, t2 = a.t2;
+  uint32_t step = a.step;
+  const double* y = a.y;
+  if (a.dev != nullptr) {
+    gather = a.dev->resample != 0 && a.dev->status == BIHIP_OK;
+    lws_const = gather;
+    lws_base = a.dev->logLikelihood;
+    const int k = a.dev->k;
+    t1 = a.t0 + k * a.dt;
+    t2 = a.t0 + (k + 1) * a.dt;
+    step = (uint32_t)(k + 1);
+    y = a.y_base + (size_t)k * a.nobs;
+  }
+
+  double lw = __builtin_nan("");
+  if (active) {
+    const int src = gather ? a.as[p] : p;
+    const T* Xin = static_cast<const T*>(a.Xin);
+    T x[DL];
+#pragma unroll
+    for (int i = 0; i < DL; ++i) x[i] = Xin[(size_t)(lo + i) * a.ld + src];
+
+    M m(a.m);
+    if constexpr (M::NOISE_FIRST) noise_first(m, a, p, step);
+    int n;
+    if constexpr (INTEG == BIHIP_RK4) {
+      n = integrate_rk4(m, grp, x, t1, t2, a.m.h0);
+    } else {
+      n = integrate_dopri5(m, grp, x, t1, t2, a.m);
+    }
+    if constexpr (!M::NOISE_FIRST) noise_after(m, a, p, lo, step, x);
+
+    T* Xout = static_cast<T*>(a.Xout);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) Xout[(size_t)(lo + i) * a.ld + p] = x[i];
+    if (a.nsteps_out != nullptr && grp.g == 0) a.nsteps_out[p] = n;
+
+    if (y != nullptr) {
+      const double part = grp.sum(obs_of(m, x, y, lo));
+      if (grp.g == 0) {
+        const double base = lws_const ? lws_base : a.lws[p];
+        lw = base + part;
+        a.lws[p] = lw;
+      }
+    }
+  }
+  if (a.partials != nullptr) block_weight_partial<TRANS_BLOCK>(lw, a.partials + 3 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------- initial states
+template <class M>
+__global__ __launch_bounds__(TRANS_BLOCK) void k_init(ModelArgs ma, uint64_t seed,
+                                                      uint32_t batch, int P, void* Xv, int ld,
+                                                      const double* mean0) {
+  using T = typename M::real_t;
+  constexpr int LPP = M::LPP, DL = M::DL;
+  const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
+  const int p = tid / LPP;
+  const Group<LPP> grp;
+  const int lo = grp.g * DL;
+  if (p >= P) return;
+  T* X = static_cast<T*>(Xv);
+  T z[DL];
+  slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, batch, lo, z);
+#pragma unroll
+  for (int i = 0; i < DL; ++i) {
+    double v;
+    if (ma.model == BIHIP_MODEL_PZ) {
+      v = i == 0 ? exp(ma.par[7] + ma.par[8] * (double)z[0])
+                 : exp(ma.par[9] + ma.par[10] * (double)z[1]);
+    } else if (ma.model == BIHIP_MODEL_L96) {
+      v = mean0[lo + i] + ma.par[3] * (double)z[i];
+    } else {
+      v = ma.par[3] + ma.par[4] * (double)z[i];
+    }
+    X[(size_t)(lo + i) * ld + p] = (T)v;
+  }
+}
+
+// ---------------------------------------------------------------- standalone correction
+template <class M>
+__global__ __launch_bounds__(TRANS_BLOCK) void k_obs_logdensity(ModelArgs ma, const double* y,
+                                                                int P, const void* Xv, int ld,
+                                                                double* lws) {
+  using T = typename M::real_t;
+  constexpr int LPP = M::LPP, DL = M::DL;
+  const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
+  const int p = tid / LPP;
+  const Group<LPP> grp;
+  const int lo = grp.g * DL;
+  if (p >= P) return;
+  const T* X = static_cast<const T*>(Xv);
+  T x[DL];
+#pragma unroll
+  for (int i = 0; i < DL; ++i) x[i] = X[(size_t)(lo + i) * ld + p];
+  M m(ma);
+  const double part = grp.sum(obs_of(m, x, y, lo));
+  if (grp.g == 0) lws[p] = lws[p] + part;
+}
+
+// Observation samples (Model::observationSamples), y is P x nobs.
+template <class M>
+__global__ __launch_bounds__(TRANS_BLOCK) void k_obs_sample(ModelArgs ma, uint64_t seed,
+                                                            uint32_t batch, uint32_t step, int P,
+                                                            const void* Xv, int ld, double* y) {
+  using T = typename M::real_t;
+  const int p = blockIdx.x * TRANS_BLOCK + threadIdx.x;
+  if (p >= P) return;
+  const T* X = static_cast<const T*>(Xv);
+  constexpr int NOBS = M::NOBS;
+  for (int b = 0; 4 * b < NOBS; ++b) {
+    T z[4];
+    normals4<T>(seed, (uint32_t)p, step, TAG_OBS, batch, (uint32_t)b, z);
+    for (int i = 0; i < 4 && 4 * b + i < NOBS; ++i) {
+      const int o = 4 * b + i;
+      double v;
+      if (ma.model == BIHIP_MODEL_PZ) {
+        v = exp(log((double)X[p]) + ma.par[6] * (double)z[i]);
+      } else if (ma.model == BIHIP_MODEL_L96) {
+        v = (double)X[(size_t)(2 * o) * ld + p] + ma.par[2] * (double)z[i];
+      } else {
+        v = (double)X[p] + ma.par[2] * (double)z[i];
+      }
+      y[(size_t)p * NOBS + o] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- random-number utilities
+__global__ void k_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, int n, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32x4 r = philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], k0, k1);
+  out[4 * i] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+__global__ void k_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, int n,
+                           double* u) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const u32x4 r = philox_block(seed, (uint32_t)j, step, tag, batch, 0);
+  u[j] = u53(r.x, r.y);
+}
+
+template <class T>
+__global__ void k_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                          uint32_t batch, int n, T* z) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (4 * b >= n) return;
+  T q[4];
+  normals4<T>(seed, index, step, tag, batch, (uint32_t)b, q);
+  for (int i = 0; i < 4 && 4 * b + i < n; ++i) z[4 * b + i] = q[i];
+}
+
+// ---------------------------------------------------------------- dispatch
+bool model_supported(const bihip_model* m) {
+  if (m->integrator != BIHIP_RK4 && m->integrator != BIHIP_DOPRI5) return false;
+  if (m->precision != BIHIP_F32 && m->precision != BIHIP_F64) return false;
+  switch (m->model) {
+    case BIHIP_MODEL_PZ: return m->d == 2;
+    case BIHIP_MODEL_L96: return m->d == 8 || m->d == 40;
+    case BIHIP_MODEL_LG: return m->d == 1;
+  }
+  return false;
+}
+
+// Calls F::template run<M>() for the model type selected by m.
+template <class F>
+static int with_model(const bihip_model* m, F& f) {
+  const bool f64 = m->precision == BIHIP_F64;
+  switch (m->model) {
+    case BIHIP_MODEL_PZ:
+      return f64 ? f.template run<PZ<double>>() : f.template run<PZ<float>>();
+    case BIHIP_MODEL_L96:
+      if (m->d == 40)
+        return f64 ? f.template run<L96<double, 40, 4>>() : f.template run<L96<float, 40, 4>>();
+      return f64 ? f.template run<L96<double, 8, 2>>() : f.template run<L96<float, 8, 2>>();
+    default:
+      return f64 ? f.template run<LG<double>>() : f.template run<LG<float>>();
+  }
+}
+
+int transition_blocks(const bihip_model* m, int P) {
+  int lpp = 1;
+  if (m->model == BIHIP_MODEL_L96) lpp = m->d == 40 ? 4 : 2;
+  return ceil_div((long)P * lpp, TRANS_BLOCK);
+}
+
+struct TransitionRunner {
+  TransArgs a;
+  int blocks;
+  hipStream_t s;
+  int integ;
+  template <class M>
+  int run() {
+    if (integ == BIHIP_RK4) {
+      hipLaunchKernelGGL((k_transition<M, BIHIP_RK4>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5>), dim3(blocks), dim3(TRANS_BLOCK), 0, s,
+                         a);
+    }
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+};
+
+int launch_transition(const TransLaunch& L, hipStream_t s) {
+  if (L.P <= 0) return BIHIP_OK;
+  TransitionRunner r;
+  r.a.m = to_args(L.model);
+  r.a.Xin = L.Xin;
+  r.a.Xout = L.Xout;
+  r.a.ld = L.ld;
+  r.a.P = L.P;
+  r.a.as = L.as;
+  r.a.lws = L.lws;
+  r.a.y = L.y;
+  r.a.t1 = L.t1;
+  r.a.t2 = L.t2;
+  r.a.seed = L.seed;
+  r.a.step = L.step;
+  r.a.batch = L.batch;
+  r.a.nsteps_out = L.nsteps_out;
+  r.a.partials = L.partials;
+  r.a.dev = L.dev;
+  r.a.t0 = L.t0;
+  r.a.dt = L.dt;
+  r.a.y_base = L.y_base;
+  r.a.nobs = L.nobs;
+  r.blocks = transition_blocks(L.model, L.P);
+  r.s = s;
+  r.integ = L.model->integrator;
+  return with_model(L.model, r);
+}
+
+struct InitRunner {
+  ModelArgs ma;
+  uint64_t seed;
+  uint32_t batch;
+  int P, ld, blocks;
+  void* X;
+  const double* mean0;
+  hipStream_t s;
+  template <class M>
+  int run() {
+    hipLaunchKernelGGL((k_init<M>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, ma, seed, batch, P, X,
+                       ld, mean0);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+};
+
+int launch_init_samples(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X,
+                        int ld, const double* mean0_dev, hipStream_t s) {
+  if (P <= 0) return BIHIP_OK;
+  InitRunner r{to_args(m), seed, batch, P, ld, transition_blocks(m, P), X, mean0_dev, s};
+  return with_model(m, r);
+}
+
+struct ObsRunner {
+  ModelArgs ma;
+  const double* y;
+  int P, ld, blocks;
+  const void* X;
+  double* lws;
+  hipStream_t s;
+  template <class M>
+  int run() {
+    hipLaunchKernelGGL((k_obs_logdensity<M>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, ma, y, P, X,
+                       ld, lws);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+};
+
+int launch_obs_logdensity(const bihip_model* m, const double* y, int P, const void* X, int ld,
+                          double* lws, hipStream_t s) {
+  if (P <= 0) return BIHIP_OK;
+  ObsRunner r{to_args(m), y, P, ld, transition_blocks(m, P), X, lws, s};
+  return with_model(m, r);
+}
+
+struct ObsSampleRunner {
+  ModelArgs ma;
+  uint64_t seed;
+  uint32_t batch, step;
+  int P, ld;
+  const void* X;
+  double* y;
+  hipStream_t s;
+  template <class M>
+  int run() {
+    hipLaunchKernelGGL((k_obs_sample<M>), dim3(ceil_div(P, TRANS_BLOCK)), dim3(TRANS_BLOCK), 0, s,
+                       ma, seed, batch, step, P, X, ld, y);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+};
+
+int launch_obs_sample(const bihip_model* m, uint64_t seed, uint32_t batch, uint32_t step, int P,
+                      const void* X, int ld, double* y, hipStream_t s) {
+  if (P <= 0) return BIHIP_OK;
+  ObsSampleRunner r{to_args(m), seed, batch, step, P, ld, X, y, s};
+  return with_model(m, r);
+}
+
+int launch_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, int n, uint32_t* out,
+                  hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  hipLaunchKernelGGL(k_philox, dim3(ceil_div(n, 256)), dim3(256), 0, s, ctr, k0, k1, n, out);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+int launch_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, int n, double* u,
+                    hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  hipLaunchKernelGGL(k_uniforms, dim3(ceil_div(n, 256)), dim3(256), 0, s, seed, step, tag, batch,
+                     n, u);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+int launch_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+                   int n, int precision, void* z, hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  const int nb = ceil_div(n, 4);
+  if (precision == BIHIP_F64) {
+    hipLaunchKernelGGL(k_normals<double>, dim3(ceil_div(nb, 256)), dim3(256), 0, s, seed, index,
+                       step, tag, batch, n, static_cast<double*>(z));
+  } else {
+    hipLaunchKernelGGL(k_normals<float>, dim3(ceil_div(nb, 256)), dim3(256), 0, s, seed, index,
+                       step, tag, batch, n, static_cast<float*>(z));
+  }
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+}  // namespace bihip

@@ -1,0 +1,397 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker that the HIP path is compared against.
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module.  The product (libbi_amd/) never does.
+
+Parity status of each oracle function (how it is pinned) is listed in the
+module-level ``PINNED`` table; the tests under tests/ implement each check.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
+RK4, DOPRI5 = 0, 1
+SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
+TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
+OK, DEGENERATE = 0, 1
+
+# function -> how its parity is pinned (each entry is exercised in tests/)
+PINNED = {
+    "orc_philox4x32_10": "Random123 known-answer vectors + rocRAND philox4x32_10 fixture",
+    "orc_normals": "Box-Muller closed form on fixed words; moments/KS on 1e6 draws",
+    "orc_rhs": "closed forms (L96 fixed point x=F, PZ hand values, LG linear)",
+    "orc_flow(RK4)": "x'=-x closed form; 4th-order convergence slope",
+    "orc_flow(DOPRI5)": "x'=-x closed form within tolerance; scipy RK45 on L96; "
+                        "5th-order slope; host-quirk order drop",
+    "orc_obs_logdensity": "scipy.stats norm/lognorm logpdf",
+    "orc_ess_reduce": "closed forms (equal weights, one-hot), long-double brute force, "
+                      "scipy.special.logsumexp, NaN/-inf rules",
+    "orc_sumexpu_inclusive_scan": "exact integer weights; numpy cumsum",
+    "orc_systematic_cumulative_offspring": "O_{P-1}=P, floor/ceil bounds, exhaustive offset grid",
+    "orc_stratified_cumulative_offspring": "O_{P-1}=P, per-stratum count bounds",
+    "orc_multinomial_ancestors": "searchsorted(side=right) definition; chi-square frequency test",
+    "orc_cumulative_offspring_to_ancestors_permute": "brute force over all offspring vectors P<=6 "
+                                                     "against a literal restatement of the "
+                                                     "reference loop",
+    "orc_permute": "in-place property and multiset preservation, brute force P<=6",
+    "orc_gather_rows": "in-place == out-of-place gather",
+    "orc_pf_run": "Kalman filter exact logZ on the LG model (essRel 0, 0.5, 1)",
+}
+
+
+class Model(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int),
+        ("d", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("nsteps", ctypes.c_int),
+        ("host_quirk", ctypes.c_int),
+        ("noise_bits", ctypes.c_int),
+        ("h0", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("rtol", ctypes.c_double),
+        ("par", ctypes.c_double * 16),
+    ]
+
+
+class PFConfig(ctypes.Structure):
+    _fields_ = [
+        ("P", ctypes.c_int),
+        ("T", ctypes.c_int),
+        ("resampler", ctypes.c_int),
+        ("essRel", ctypes.c_double),
+        ("t0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("batch", ctypes.c_uint32),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _declare(L):
+    c = ctypes
+    L.orc_philox4x32_10.argtypes = [_U32, _U32, _U32]
+    L.orc_philox_block.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
+                                   c.c_uint32, _U32]
+    L.orc_u53.argtypes = [c.c_uint32, c.c_uint32]
+    L.orc_u53.restype = c.c_double
+    L.orc_uniforms.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_int, _D]
+    L.orc_normals.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
+                              c.c_int, c.c_int, _D]
+    L.orc_nobs.argtypes = [c.POINTER(Model)]
+    L.orc_nobs.restype = c.c_int
+    L.orc_init_samples.argtypes = [c.POINTER(Model), c.c_uint64, c.c_uint32, c.c_int, _D, c.c_int,
+                                   _D]
+    L.orc_transition.argtypes = [c.POINTER(Model), c.c_uint64, c.c_uint32, c.c_uint32, c.c_double,
+                                 c.c_double, c.c_int, _D, c.c_int, _I]
+    L.orc_obs_logdensity.argtypes = [c.POINTER(Model), _D, c.c_int, _D, c.c_int, _D]
+    L.orc_obs_sample.argtypes = [c.POINTER(Model), c.c_uint64, c.c_uint32, c.c_uint32, _D, _D]
+    L.orc_flow.argtypes = [c.POINTER(Model), c.c_double, c.c_double, c.c_double, _D]
+    L.orc_flow.restype = c.c_int
+    L.orc_rhs.argtypes = [c.POINTER(Model), c.c_double, c.c_double, _D, _D]
+    for name in ("orc_max_reduce", "orc_logsumexp_reduce"):
+        getattr(L, name).argtypes = [_D, c.c_int]
+        getattr(L, name).restype = c.c_double
+    L.orc_ess_reduce.argtypes = [_D, c.c_int, _D]
+    L.orc_ess_reduce.restype = c.c_double
+    L.orc_sumexpu_inclusive_scan.argtypes = [_D, c.c_int, _D]
+    L.orc_sumexpu_inclusive_scan.restype = c.c_double
+    L.orc_systematic_cumulative_offspring.argtypes = [_D, c.c_int, c.c_int, c.c_double, _I]
+    L.orc_stratified_cumulative_offspring.argtypes = [_D, c.c_int, c.c_int, _D, _I]
+    L.orc_multinomial_ancestors.argtypes = [_D, c.c_int, c.c_int, _D, _I]
+    L.orc_ancestors_to_offspring.argtypes = [_I, c.c_int, c.c_int, _I]
+    L.orc_offspring_to_cumulative.argtypes = [_I, c.c_int, _I]
+    L.orc_cumulative_offspring_to_ancestors.argtypes = [_I, c.c_int, _I]
+    L.orc_cumulative_offspring_to_ancestors_permute.argtypes = [_I, c.c_int, _I]
+    L.orc_offspring_to_ancestors_permute.argtypes = [_I, c.c_int, _I]
+    L.orc_permute.argtypes = [_I, c.c_int]
+    L.orc_gather_rows.argtypes = [_I, c.c_int, c.c_int, _D, c.c_int]
+    L.orc_pf_run.argtypes = [c.POINTER(Model), c.POINTER(PFConfig), _D, _D, _D, _D, _D, _I, _I, _D]
+    L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
+    L.orc_kalman_loglik_lg.restype = c.c_double
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- models
+def make_model(kind, d=None, integrator=None, h0=None, atol=1e-5, rtol=1e-5, nsteps=1000,
+               host_quirk=0, noise_bits=32, **par):
+    """Model configuration; par names follow the model (see oracle.h)."""
+    m = Model()
+    if kind == "pz":
+        m.model, m.d = MODEL_PZ, 2
+        m.integrator = RK4 if integrator is None else integrator
+        m.h0 = 0.1 if h0 is None else h0
+        vals = [par.get("mu", 0.5), par.get("sigma", 0.2), par.get("c", 0.25), par.get("e", 0.3),
+                par.get("m_l", 0.1), par.get("m_q", 0.1), par.get("obs_sd", 0.2),
+                par.get("P0_logmean", np.log(2.0)), par.get("P0_logsd", 0.2),
+                par.get("Z0_logmean", np.log(2.0)), par.get("Z0_logsd", 0.1)]
+    elif kind == "l96":
+        m.model, m.d = MODEL_L96, 40 if d is None else d
+        m.integrator = DOPRI5 if integrator is None else integrator
+        m.h0 = 0.05 if h0 is None else h0
+        vals = [par.get("F", 8.0), par.get("noise_sd", 0.1), par.get("obs_sd", 1.0),
+                par.get("init_sd", 1.0)]
+    elif kind == "lg":
+        m.model, m.d = MODEL_LG, 1 if d is None else d
+        m.integrator = DOPRI5 if integrator is None else integrator
+        m.h0 = 0.1 if h0 is None else h0
+        vals = [par.get("lam", 0.5), par.get("noise_sd", 0.5), par.get("obs_sd", 0.5),
+                par.get("m0", 0.0), par.get("s0", 1.0)]
+    else:
+        raise ValueError(kind)
+    m.nsteps, m.host_quirk, m.noise_bits = nsteps, host_quirk, noise_bits
+    m.atol, m.rtol = atol, rtol
+    for i, v in enumerate(vals):
+        m.par[i] = v
+    return m
+
+
+def nobs(m):
+    return lib().orc_nobs(ctypes.byref(m))
+
+
+# ---------------------------------------------------------------- random
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def philox_block(seed, index, step, tag, batch, block):
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox_block(seed, index, step, tag, batch, block, o)
+    return list(o)
+
+
+def uniforms(seed, step, tag, n, batch=0):
+    u = np.empty(n, np.float64)
+    lib().orc_uniforms(seed, step, tag, batch, n, _dp(u))
+    return u
+
+
+def normals(seed, index, step, tag, n, bits=32, batch=0):
+    z = np.empty(n, np.float64)
+    lib().orc_normals(seed, index, step, tag, batch, n, bits, _dp(z))
+    return z
+
+
+# ---------------------------------------------------------------- model functions
+def init_samples(m, seed, P, mean0=None, batch=0):
+    X = np.zeros((m.d, P), np.float64)  # column-major P x d == row-major d x P
+    mean = _f64(mean0) if mean0 is not None else np.zeros(m.d)
+    lib().orc_init_samples(ctypes.byref(m), seed, batch, P, _dp(X), P, _dp(mean))
+    return X
+
+
+def transition(m, seed, step, t1, t2, X, batch=0, return_steps=False):
+    X = _f64(X).copy()
+    P = X.shape[1]
+    ns = np.zeros(P, np.int32)
+    lib().orc_transition(ctypes.byref(m), seed, batch, step, t1, t2, P, _dp(X), P, _ip(ns))
+    return (X, ns) if return_steps else X
+
+
+def obs_logdensity(m, y, X, lws):
+    X = _f64(X)
+    lws = _f64(lws).copy()
+    lib().orc_obs_logdensity(ctypes.byref(m), _dp(_f64(y)), X.shape[1], _dp(X), X.shape[1],
+                             _dp(lws))
+    return lws
+
+
+def obs_sample(m, seed, step, x, batch=0):
+    y = np.empty(nobs(m), np.float64)
+    lib().orc_obs_sample(ctypes.byref(m), seed, batch, step, _dp(_f64(x)), _dp(y))
+    return y
+
+
+def flow(m, x, t1, t2, alpha=0.0):
+    x = _f64(x).copy()
+    n = lib().orc_flow(ctypes.byref(m), alpha, t1, t2, _dp(x))
+    return x, n
+
+
+def rhs(m, x, alpha=0.0, t=0.0):
+    x = _f64(x)
+    dx = np.empty_like(x)
+    lib().orc_rhs(ctypes.byref(m), alpha, t, _dp(x), _dp(dx))
+    return dx
+
+
+def simulate(m, seed, T, dt, x0, t0=0.0):
+    """Truth trajectory (T+1 x d) and observations (T x nobs) of one particle."""
+    xs = [np.array(x0, np.float64)]
+    ys = []
+    x = np.array(x0, np.float64).reshape(m.d, 1)
+    for k in range(T):
+        x = transition(m, seed, k + 1, t0 + k * dt, t0 + (k + 1) * dt, x)
+        xs.append(x[:, 0].copy())
+        ys.append(obs_sample(m, seed, k + 1, x[:, 0]))
+    return np.array(xs), np.array(ys)
+
+
+# ---------------------------------------------------------------- primitives
+def max_reduce(x):
+    x = _f64(x)
+    return lib().orc_max_reduce(_dp(x), len(x))
+
+
+def logsumexp_reduce(x):
+    x = _f64(x)
+    return lib().orc_logsumexp_reduce(_dp(x), len(x))
+
+
+def ess_reduce(lws):
+    lws = _f64(lws)
+    lW = ctypes.c_double()
+    ess = lib().orc_ess_reduce(_dp(lws), len(lws), ctypes.byref(lW))
+    return ess, lW.value
+
+
+def sumexpu_inclusive_scan(lws):
+    lws = _f64(lws)
+    Ws = np.empty_like(lws)
+    mx = lib().orc_sumexpu_inclusive_scan(_dp(lws), len(lws), _dp(Ws))
+    return Ws, mx
+
+
+def systematic_cumulative_offspring(Ws, a, n=None):
+    Ws = _f64(Ws)
+    n = len(Ws) if n is None else n
+    Os = np.empty(len(Ws), np.int32)
+    rc = lib().orc_systematic_cumulative_offspring(_dp(Ws), len(Ws), n, a, _ip(Os))
+    return Os, rc
+
+
+def stratified_cumulative_offspring(Ws, alphas, n=None):
+    Ws = _f64(Ws)
+    n = len(Ws) if n is None else n
+    Os = np.empty(len(Ws), np.int32)
+    rc = lib().orc_stratified_cumulative_offspring(_dp(Ws), len(Ws), n, _dp(_f64(alphas)),
+                                                   _ip(Os))
+    return Os, rc
+
+
+def multinomial_ancestors(Ws, U):
+    Ws, U = _f64(Ws), _f64(U)
+    a = np.empty(len(U), np.int32)
+    rc = lib().orc_multinomial_ancestors(_dp(Ws), len(Ws), len(U), _dp(U), _ip(a))
+    return a, rc
+
+
+def ancestors_to_offspring(a, P):
+    a = _i32(a)
+    os_ = np.empty(P, np.int32)
+    lib().orc_ancestors_to_offspring(_ip(a), len(a), P, _ip(os_))
+    return os_
+
+
+def offspring_to_cumulative(os_):
+    os_ = _i32(os_)
+    O = np.empty_like(os_)
+    lib().orc_offspring_to_cumulative(_ip(os_), len(os_), _ip(O))
+    return O
+
+
+def cumulative_offspring_to_ancestors(O):
+    O = _i32(O)
+    a = np.empty(int(O[-1]), np.int32)
+    lib().orc_cumulative_offspring_to_ancestors(_ip(O), len(O), _ip(a))
+    return a
+
+
+def cumulative_offspring_to_ancestors_permute(O):
+    O = _i32(O)
+    a = np.empty(len(O), np.int32)
+    lib().orc_cumulative_offspring_to_ancestors_permute(_ip(O), len(O), _ip(a))
+    return a
+
+
+def offspring_to_ancestors_permute(os_):
+    os_ = _i32(os_)
+    a = np.empty(len(os_), np.int32)
+    lib().orc_offspring_to_ancestors_permute(_ip(os_), len(os_), _ip(a))
+    return a
+
+
+def permute(a):
+    a = _i32(a).copy()
+    lib().orc_permute(_ip(a), len(a))
+    return a
+
+
+def gather_rows(a, X):
+    X = _f64(X).copy()
+    lib().orc_gather_rows(_ip(_i32(a)), X.shape[1], X.shape[0], _dp(X), X.shape[1])
+    return X
+
+
+# ---------------------------------------------------------------- filter
+def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=None,
+           batch=0, want_ancestors=False, want_state=False):
+    y = _f64(y).reshape(len(y), -1)
+    T = y.shape[0]
+    cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt, seed=seed,
+                   batch=batch)
+    logZ = ctypes.c_double()
+    li = np.zeros(T)
+    ess = np.zeros(T)
+    res = np.zeros(T, np.int32)
+    anc = np.zeros((T, P), np.int32) if want_ancestors else None
+    Xo = np.zeros((m.d, P)) if want_state else None
+    mean = _f64(mean0) if mean0 is not None else np.zeros(m.d)
+    rc = lib().orc_pf_run(ctypes.byref(m), ctypes.byref(cfg), _dp(y), _dp(mean),
+                          ctypes.byref(logZ), _dp(li), _dp(ess), _ip(res), _ip(anc), _dp(Xo))
+    return dict(rc=rc, logZ=logZ.value, logIncrements=li, ess=ess, resampled=res,
+                ancestors=anc, X=Xo)
+
+
+def kalman_loglik_lg(m, dt, y):
+    y = _f64(y).ravel()
+    return lib().orc_kalman_loglik_lg(ctypes.byref(m), dt, len(y), _dp(y))

@@ -1,0 +1,138 @@
+/*
+ * FP64 CPU oracle for LibBi's bootstrap particle-filter step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation in libbi_amd/ is compared against.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load it;
+ * the product path never calls, links or executes anything in oracle/.
+ *
+ * It restates, in plain C99 + OpenMP, the host (OpenMP) semantics of the
+ * reference (lawmurray/LibBi v1.4.5).  Every function cites the reference
+ * file:line it follows.  Arithmetic is double precision throughout and is
+ * compiled with -ffp-contract=off so that floating-point expressions are
+ * evaluated exactly in the order written.
+ *
+ * Counter-based random numbers (Philox4x32-10, Salmon et al. 2011) replace
+ * the reference's per-OpenMP-thread boost::mt19937 streams
+ * (share/src/bi/host/random/RandomHost.cpp:12-27), so the oracle and the GPU
+ * consume identical noise and uniforms.  The counter layout is:
+ *   counter = { index, step, tag | (batch << 8), block },
+ *   key     = { seed & 0xffffffff, seed >> 32 }.
+ */
+#ifndef LIBBI_AMD_ORACLE_H
+#define LIBBI_AMD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random-stream tags (shared with the HIP library) ---- */
+enum {
+  ORC_TAG_INIT = 1,        /* initial-state draws */
+  ORC_TAG_NOISE = 2,       /* transition noise */
+  ORC_TAG_SYSTEMATIC = 3,  /* systematic resampler offset */
+  ORC_TAG_STRATIFIED = 4,  /* stratified resampler alphas */
+  ORC_TAG_MULTINOMIAL = 5, /* multinomial resampler uniforms */
+  ORC_TAG_OBS = 6          /* observation simulation */
+};
+
+/* ---- models ---- */
+enum { ORC_MODEL_PZ = 0, ORC_MODEL_L96 = 1, ORC_MODEL_LG = 2 };
+enum { ORC_RK4 = 0, ORC_DOPRI5 = 1 };
+enum { ORC_SYSTEMATIC = 0, ORC_STRATIFIED = 1, ORC_MULTINOMIAL = 2 };
+enum { ORC_OK = 0, ORC_DEGENERATE = 1, ORC_EINVAL = 2 };
+
+/*
+ * Model and integrator configuration.
+ *
+ * par[] meaning per model:
+ *  PZ : mu, sigma, c, e, m_l, m_q, obs_sd, P0_logmean, P0_logsd, Z0_logmean, Z0_logsd
+ *  L96: F, noise_sd, obs_sd, init_sd            (initial mean passed separately)
+ *  LG : lambda, noise_sd, obs_sd, m0, s0
+ */
+typedef struct {
+  int model;        /* ORC_MODEL_* */
+  int d;            /* state dimension */
+  int integrator;   /* ORC_RK4 or ORC_DOPRI5 */
+  int nsteps;       /* DOPRI5 maximum number of steps per interval */
+  int host_quirk;   /* DOPRI5: 1 = evaluate k7 at x5 as the reference host does */
+  int noise_bits;   /* 32: normals from 32-bit uniforms; 24: from 24-bit (FP32 kernels) */
+  double h0, atol, rtol;
+  double par[16];
+} orc_model;
+
+/* ---- Philox and variates ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_philox_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+    uint32_t batch, uint32_t block, uint32_t out[4]);
+double orc_u53(uint32_t r0, uint32_t r1);
+/* n uniforms in [0,1): u_j from counter {j, step, tag|(batch<<8), 0}, words 0,1 */
+void orc_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, int n, double* u);
+/* n standard normals for one index: normal j from block j/4, word pair (j%4)/2 */
+void orc_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+    int n, int bits, double* z);
+
+/* ---- model functions (X is column-major, leading dimension ld >= P) ---- */
+int orc_nobs(const orc_model* m);
+void orc_init_samples(const orc_model* m, uint64_t seed, uint32_t batch, int P, double* X,
+    int ld, const double* mean0);
+void orc_transition(const orc_model* m, uint64_t seed, uint32_t batch, uint32_t step,
+    double t1, double t2, int P, double* X, int ld, int* nsteps_out);
+void orc_obs_logdensity(const orc_model* m, const double* y, int P, const double* X, int ld,
+    double* lws);
+void orc_obs_sample(const orc_model* m, uint64_t seed, uint32_t batch, uint32_t step,
+    const double* x, double* y);
+/* deterministic flow of one particle over [t1,t2] (no noise); returns steps taken */
+int orc_flow(const orc_model* m, double alpha, double t1, double t2, double* x);
+void orc_rhs(const orc_model* m, double alpha, double t, const double* x, double* dx);
+
+/* ---- primitives (share/src/bi/primitive/vector_primitive.hpp) ---- */
+double orc_max_reduce(const double* x, int n);
+double orc_logsumexp_reduce(const double* x, int n);
+double orc_ess_reduce(const double* lws, int n, double* lW);
+double orc_sumexpu_inclusive_scan(const double* lws, int n, double* Ws);
+
+/* ---- resamplers ---- */
+int orc_systematic_cumulative_offspring(const double* Ws, int P, int n, double a, int* Os);
+int orc_stratified_cumulative_offspring(const double* Ws, int P, int n, const double* alphas,
+    int* Os);
+int orc_multinomial_ancestors(const double* Ws, int P, int n, const double* U, int* as);
+void orc_ancestors_to_offspring(const int* as, int n, int P, int* os);
+void orc_offspring_to_cumulative(const int* os, int P, int* Os);
+void orc_cumulative_offspring_to_ancestors(const int* Os, int P, int* as);
+void orc_cumulative_offspring_to_ancestors_permute(const int* Os, int P, int* as);
+void orc_offspring_to_ancestors_permute(const int* os, int P, int* as);
+void orc_permute(int* as, int P);
+void orc_gather_rows(const int* as, int P, int ncols, double* X, int ld);
+
+/* ---- bootstrap particle filter ---- */
+typedef struct {
+  int P;            /* number of particles */
+  int T;            /* number of observations, at t0 + k*dt, k = 1..T */
+  int resampler;    /* ORC_SYSTEMATIC / ORC_STRATIFIED / ORC_MULTINOMIAL */
+  double essRel;    /* resample when ess < essRel * P */
+  double t0, dt;
+  uint64_t seed;
+  uint32_t batch;
+} orc_pf_cfg;
+
+/*
+ * Run the filter.  y is T x nobs (row k = observation at t0+(k+1)dt).
+ * Outputs (any may be NULL): logIncrements[T], ess[T], resampled[T],
+ * ancestors[T*P] (ancestry stored at each step, identity if not resampled),
+ * Xout (final state, column-major ld=P).  Returns ORC_OK or ORC_DEGENERATE.
+ */
+int orc_pf_run(const orc_model* m, const orc_pf_cfg* cfg, const double* y,
+    const double* mean0, double* logZ, double* logIncrements, double* ess,
+    int* resampled, int* ancestors, double* Xout);
+
+/* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
+double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

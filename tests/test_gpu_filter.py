@@ -1,0 +1,125 @@
+"""Whole-filter parity of the device-resident bootstrap particle filter
+(bihip_pf_*) with the oracle's filter on identical Philox inputs."""
+import numpy as np
+import pytest
+
+from libbi_amd import capi
+from gpu_helpers import l96_spun_up, oracle_model
+
+pytestmark = pytest.mark.gpu
+
+
+def _pz_data(orc, T, seed=100):
+    m = orc.make_model("pz")
+    x0 = orc.init_samples(m, seed, 1)[:, 0]
+    _, y = orc.simulate(m, seed, T, 1.0, x0)
+    return y
+
+
+def _run_gpu(ctx, gm, y, P, dt, seed, resampler, ess_rel, mean0=None, record=True,
+             use_graph=False):
+    T = len(y)
+    pf = capi.ParticleFilter(ctx, gm, P, T, dt, seed, resampler=resampler, ess_rel=ess_rel,
+                             record=record, use_graph=use_graph)
+    pf.set_observations(y)
+    pf.init(mean0)
+    pf.step(T)
+    logZ = pf.term()
+    out = pf.outputs()
+    anc = np.array([pf.ancestors(k) for k in range(T)]) if record else None
+    pf.close()
+    return logZ, out, anc
+
+
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.STRATIFIED, capi.MULTINOMIAL])
+def test_pz_filter_fp64_parity(ctx, orc, resampler):
+    """Predator-prey bootstrap PF, N=4096, FP64, RK4 h=0.1, resample every step, T=100."""
+    y = _pz_data(orc, 100)
+    gm = capi.make_model("pz", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    P = 4096
+    logZ, out, anc = _run_gpu(ctx, gm, y, P, 1.0, 42, resampler, 1.0)
+    ref = orc.pf_run(om, y, P, 1.0, 42, resampler=resampler, ess_rel=1.0, want_ancestors=True)
+    # FP64 everywhere: the estimate agrees to rounding (relative 1e-10)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    # ancestors are bit-exact at every step
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+def test_l96_filter_fp64_parity(ctx, orc):
+    """Lorenz-96 d=40, DOPRI5 rtol=atol=1e-5, FP64 state, N=4096, T=20, ESS<0.5N."""
+    om0 = orc.make_model("l96", d=40)
+    mean0 = l96_spun_up(orc)
+    _, y = orc.simulate(om0, 7, 20, 0.05, mean0)
+    gm = capi.make_model("l96", d=40, precision=capi.F64)
+    om = oracle_model(orc, gm)
+    P = 4096
+    logZ, out, anc = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.5, mean0=mean0)
+    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0, want_ancestors=True)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-9)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+def test_l96_filter_fp32_mixed_precision(ctx, orc):
+    """FP32 transitions with FP64 log-weights/cumsum/logZ against the FP64 oracle."""
+    om0 = orc.make_model("l96", d=40)
+    mean0 = l96_spun_up(orc)
+    _, y = orc.simulate(om0, 7, 20, 0.05, mean0)
+    gm = capi.make_model("l96", d=40, precision=capi.F32)
+    om = oracle_model(orc, gm)   # same 24-bit noise, FP64 arithmetic
+    P = 4096
+    logZ, out, _ = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.5, mean0=mean0,
+                            record=False)
+    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0)
+    # FP32 state: the log-likelihood agrees to 1e-5 relative over 20 steps
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-5)
+
+
+def test_lg_filter_matches_kalman(ctx, orc):
+    gm = capi.make_model("lg", precision=capi.F64, lam=0.5, noise_sd=0.5, obs_sd=0.5,
+                         atol=1e-10, rtol=1e-10, h0=0.25)
+    om = oracle_model(orc, gm)
+    x0 = orc.init_samples(om, 11, 1)[:, 0]
+    _, y = orc.simulate(om, 11, 40, 1.0, x0)
+    exact = orc.kalman_loglik_lg(om, 1.0, y)
+    zs = [_run_gpu(ctx, gm, y, 1 << 16, 1.0, s, capi.SYSTEMATIC, 0.5, record=False)[0]
+          for s in (1, 2)]
+    assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
+
+
+def test_graph_replay_equals_eager(ctx, orc):
+    y = _pz_data(orc, 30)
+    gm = capi.make_model("pz", precision=capi.F64)
+    a = _run_gpu(ctx, gm, y, 2048, 1.0, 5, capi.SYSTEMATIC, 0.7, use_graph=False)
+    b = _run_gpu(ctx, gm, y, 2048, 1.0, 5, capi.SYSTEMATIC, 0.7, use_graph=True)
+    assert a[0] == b[0]
+    np.testing.assert_array_equal(a[1]["logIncrements"], b[1]["logIncrements"])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_degenerate_weights_do_not_fail(ctx, orc):
+    gm = capi.make_model("lg", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    x0 = orc.init_samples(om, 3, 1)[:, 0]
+    _, y = orc.simulate(om, 3, 5, 1.0, x0)
+    y = y.copy()
+    y[1] = np.inf
+    logZ, out, _ = _run_gpu(ctx, gm, y, 256, 1.0, 3, capi.SYSTEMATIC, 1.0, record=False)
+    assert logZ == -np.inf
+    assert np.isnan(out["ess"][1]) and not out["resampled"][2:].any()
+
+
+def test_step_beyond_schedule_is_rejected(ctx, orc):
+    gm = capi.make_model("pz", precision=capi.F64)
+    pf = capi.ParticleFilter(ctx, gm, 64, 3, 1.0, 1)
+    pf.set_observations(np.ones(3))
+    with pytest.raises(capi.BihipError):
+        pf.step(1)  # not initialised
+    pf.init()
+    pf.step(3)
+    with pytest.raises(capi.BihipError):
+        pf.step(1)
+    pf.close()
