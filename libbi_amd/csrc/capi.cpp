@@ -39,7 +39,7 @@ static int ws_alloc(ScanWorkspace* ws, int tiles) {
   int rc;
   if ((rc = dalloc(&ws->state, 1))) return rc;
   if ((rc = dalloc(&ws->flags, tiles))) return rc;
-  if ((rc = dalloc(&ws->aggregates, tiles))) return rc;
+  if ((rc = dalloc(&ws->aggregates, (size_t)2 * tiles))) return rc;
   BIHIP_HIP(hipMemset(ws->state, 0, sizeof(ScanState)));
   BIHIP_HIP(hipMemset(ws->flags, 0, sizeof(uint32_t) * tiles));
   return BIHIP_OK;

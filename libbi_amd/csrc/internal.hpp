@@ -61,7 +61,7 @@ struct ScanState {
 struct ScanWorkspace {
   ScanState* state;     // device
   uint32_t* flags;      // device, one per tile
-  uint64_t* aggregates; // device, one per tile (bit pattern of the value)
+  uint64_t* aggregates; // device, two words per tile
   int max_tiles;
 };
 

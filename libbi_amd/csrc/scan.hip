@@ -7,22 +7,38 @@ namespace bihip {
 
 int scan_tiles(int P) { return ceil_div(P, SCAN_TILE); }
 
+// Weights w = nanexp(lw - max) lie in [0, 1]; they are summed exactly as
+// 128-bit integers with 62 fractional bits (w * 2^62 is exact in double and
+// truncates to an integer below 2^62) and each prefix is rounded to double
+// once.  Exact integer sums make the scan deterministic and monotone: a
+// floating-point parallel scan rounds differently across tile boundaries and
+// can step backwards, which would give a particle negative offspring.
+constexpr double FIX_SCALE = 0x1.0p62;
+constexpr double FIX_UNIT = 0x1.0p-62;
+
 struct SumExpLoader {
   const double* lws;
   const double* mx;
-  __device__ double operator()(long i) const { return nanexp(lws[i] - *mx); }
+  __device__ u128 operator()(long i) const {
+    const double w = nanexp(lws[i] - *mx);
+    return u128((uint64_t)(w * FIX_SCALE));
+  }
 };
 
 struct StoreInclusive {
   double* out;
-  __device__ void operator()(long i, double incl, double) const { out[i] = incl; }
+  __device__ void operator()(long i, u128 incl, u128) const {
+    // hi * 2^64 * 2^-62 is exact; lo is rounded once; the sum rounds once
+    // (monotone in (hi, lo), including when lo wraps into hi).
+    out[i] = (double)incl.hi * 4.0 + (double)incl.lo * FIX_UNIT;
+  }
 };
 
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,
                        double* Ws, ScanWorkspace ws, hipStream_t s) {
   const int nt = scan_tiles(P);
   if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
-  hipLaunchKernelGGL((k_tile_scan<double, SumExpLoader, StoreInclusive, PFGate>), dim3(nt),
+  hipLaunchKernelGGL((k_tile_scan<u128, SumExpLoader, StoreInclusive, PFGate>), dim3(nt),
                      dim3(SCAN_BLOCK), 0, s, SumExpLoader{lws, mx_dev}, StoreInclusive{Ws},
                      PFGate{gate}, P, ws);
   BIHIP_LAUNCH_CHECK();

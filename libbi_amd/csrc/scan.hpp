@@ -2,8 +2,9 @@
 // (after Merrill & Garland, "Single-pass parallel prefix scan with decoupled
 // look-back", 2016).  Each tile publishes only its aggregate; a tile's
 // prefix is the sum of all earlier aggregates, reduced by the whole block in
-// a fixed order.  Tiles never wait on each other's prefixes, and the result
-// is bit-identical from run to run for floating-point values.
+// a fixed order.  Tiles never wait on each other's prefixes.  Values are
+// integers (counts, or weights in 128-bit fixed point), so every association
+// order gives the same exact, monotone result.
 //
 // Tiles are taken from an atomic ticket, so every tile waited on has already
 // started; a spin limit turns a lost tile into an error flag instead of a
@@ -18,22 +19,67 @@ namespace bihip {
 
 constexpr uint32_t SCAN_SPIN_LIMIT = 1u << 22;
 
-template <class V>
-__device__ __forceinline__ uint64_t to_bits(V v) {
-  if constexpr (sizeof(V) == 8 && !__is_integral(V)) {
-    return (uint64_t)__double_as_longlong(v);
-  } else {
-    return (uint64_t)v;
+// Unsigned 128-bit integer for exact fixed-point prefix sums.
+struct u128 {
+  uint64_t lo, hi;
+  u128() = default;
+  __device__ __forceinline__ explicit u128(uint64_t v) : lo(v), hi(0) {}
+  __device__ __forceinline__ u128(uint64_t l, uint64_t h) : lo(l), hi(h) {}
+  __device__ __forceinline__ u128 operator+(const u128& o) const {
+    const uint64_t l = lo + o.lo;
+    return u128(l, hi + o.hi + (l < lo ? 1u : 0u));
   }
-}
+};
+
+// Values travel between tiles as two 64-bit words.
+template <class V>
+struct Words {
+  __device__ static void pack(const V& v, uint64_t& a, uint64_t& b) {
+    a = (uint64_t)v;
+    b = 0;
+  }
+  __device__ static V unpack(uint64_t a, uint64_t) { return (V)a; }
+};
+template <>
+struct Words<u128> {
+  __device__ static void pack(const u128& v, uint64_t& a, uint64_t& b) {
+    a = v.lo;
+    b = v.hi;
+  }
+  __device__ static u128 unpack(uint64_t a, uint64_t b) { return u128(a, b); }
+};
 
 template <class V>
-__device__ __forceinline__ V from_bits(uint64_t b) {
-  if constexpr (sizeof(V) == 8 && !__is_integral(V)) {
-    return __longlong_as_double((long long)b);
-  } else {
-    return (V)b;
-  }
+__device__ __forceinline__ V shfl_up_v(const V& v, int o) {
+  return __shfl_up(v, o, WAVE);
+}
+template <>
+__device__ __forceinline__ u128 shfl_up_v<u128>(const u128& v, int o) {
+  return u128(__shfl_up(v.lo, o, WAVE), __shfl_up(v.hi, o, WAVE));
+}
+template <class V>
+__device__ __forceinline__ V shfl_xor_v(const V& v, int o) {
+  return __shfl_xor(v, o, WAVE);
+}
+template <>
+__device__ __forceinline__ u128 shfl_xor_v<u128>(const u128& v, int o) {
+  return u128(__shfl_xor(v.lo, o, WAVE), __shfl_xor(v.hi, o, WAVE));
+}
+
+// Block sum in a fixed order for any V with operator+.
+template <class V>
+__device__ __forceinline__ V block_sum_v(V v, V* lds) {
+  constexpr int NW = SCAN_BLOCK / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) v = v + shfl_xor_v(v, o);
+  const int w = threadIdx.x / WAVE, l = threadIdx.x & (WAVE - 1);
+  __syncthreads();
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  V r = lds[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r = r + lds[i];
+  return r;
 }
 
 // Loader: V operator()(long i) const, valid for i < P.
@@ -78,10 +124,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
   V incl = total;
 #pragma unroll
   for (int o = 1; o < WAVE; o <<= 1) {
-    const V u = __shfl_up(incl, o, WAVE);
+    const V u = shfl_up_v(incl, o);
     if (lane >= o) incl = u + incl;
   }
-  V excl = __shfl_up(incl, 1, WAVE);
+  V excl = shfl_up_v(incl, 1);
   if (lane == 0) excl = V(0);
   if (lane == WAVE - 1) lds[w] = incl;
   __syncthreads();
@@ -95,7 +141,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
 
   // publish this tile's aggregate
   if (threadIdx.x == 0) {
-    __hip_atomic_store(&ws.aggregates[tile], to_bits(agg), __ATOMIC_RELAXED,
+    uint64_t a0, a1;
+    Words<V>::pack(agg, a0, a1);
+    __hip_atomic_store(&ws.aggregates[2 * tile], a0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ws.aggregates[2 * tile + 1], a1, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ws.flags[tile], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -111,10 +160,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    part = part + from_bits<V>(__hip_atomic_load(&ws.aggregates[j], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT));
+    const uint64_t a0 =
+        __hip_atomic_load(&ws.aggregates[2 * j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t a1 =
+        __hip_atomic_load(&ws.aggregates[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    part = part + Words<V>::unpack(a0, a1);
   }
-  const V prefix = block_sum<SCAN_BLOCK>(part, lds);
+  const V prefix = block_sum_v(part, lds);
 
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i) {

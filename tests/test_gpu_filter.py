@@ -63,19 +63,30 @@ def test_l96_filter_fp64_parity(ctx, orc):
     np.testing.assert_array_equal(anc, ref["ancestors"])
 
 
-def test_l96_filter_fp32_mixed_precision(ctx, orc):
-    """FP32 transitions with FP64 log-weights/cumsum/logZ against the FP64 oracle."""
+@pytest.mark.parametrize("T", [1, 10])
+def test_l96_filter_fp32_mixed_precision(ctx, orc, T):
+    """FP32 transitions with FP64 log-weights and logZ against the FP64 oracle on
+    identical 24-bit Philox noise.  Without resampling the two filters carry
+    the same particles, so the difference is FP32 rounding alone.  (With
+    resampling, a single ancestor decided differently by rounding sends the
+    filters down different Monte Carlo paths; at N=4096 this model's logZ
+    spreads over hundreds of nats between seeds, so that comparison is not a
+    precision test.)"""
     om0 = orc.make_model("l96", d=40)
     mean0 = l96_spun_up(orc)
-    _, y = orc.simulate(om0, 7, 20, 0.05, mean0)
+    _, y = orc.simulate(om0, 7, T, 0.05, mean0)
     gm = capi.make_model("l96", d=40, precision=capi.F32)
-    om = oracle_model(orc, gm)   # same 24-bit noise, FP64 arithmetic
+    om = oracle_model(orc, gm)
     P = 4096
-    logZ, out, _ = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.5, mean0=mean0,
+    logZ, out, _ = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.0, mean0=mean0,
                             record=False)
-    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0)
-    # FP32 state: the log-likelihood agrees to 1e-5 relative over 20 steps
-    assert logZ == pytest.approx(ref["logZ"], rel=1e-5)
+    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.0, mean0=mean0)
+    assert not out["resampled"].any()
+    # FP32 state: relative error of logZ below 1e-6 after one step, 1e-5 after ten
+    tol = 1e-6 if T == 1 else 1e-5
+    assert logZ == pytest.approx(ref["logZ"], rel=tol)
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=tol * 10,
+                               atol=tol * 100)
 
 
 def test_lg_filter_matches_kalman(ctx, orc):

@@ -13,9 +13,17 @@ from gpu_helpers import dev, empty, host, l96_spun_up, oracle_model
 
 pytestmark = pytest.mark.gpu
 L = capi.lib
+_LIVE = []  # tensors passed to the library stay alive until the test ends
+
+
+@pytest.fixture(autouse=True)
+def _keep_alive():
+    yield
+    _LIVE.clear()
 
 
 def _c(x):
+    _LIVE.append(x)
     return ctypes.c_void_p(x.data_ptr())
 
 
