@@ -42,8 +42,8 @@ def parse():
     ap.add_argument("--resampler", default="systematic",
                     choices=["systematic", "stratified", "multinomial"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=1 << 16)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-particles", type=int, default=1 << 18)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     return ap.parse_args()
 
 

@@ -100,7 +100,25 @@ int bihip_model_nobs(const bihip_model* m);
 /* Bytes per state element of a model (4 or 8). */
 int bihip_model_elem_size(const bihip_model* m);
 
+/* ================================================================ memory */
+/* Device storage for the C++ shims (containers of State<B,ON_DEVICE>). */
+int bihip_malloc(bihip_ctx* ctx, size_t bytes, void** ptr_dev);
+int bihip_free(bihip_ctx* ctx, void* ptr_dev);
+/* Synchronous copies host <-> device; asynchronous device -> device. */
+int bihip_memcpy_to_device(bihip_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int bihip_memcpy_to_host(bihip_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+int bihip_memcpy_device(bihip_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+/* set_elements(x, value) (share/src/bi/primitive/vector_primitive.hpp). */
+int bihip_fill(bihip_ctx* ctx, double* x_dev, int n, double value);
+/* seq_elements(x, 0): x[i] = i. */
+int bihip_iota(bihip_ctx* ctx, int* x_dev, int n);
+
 /* ================================================================ random numbers */
+/* One uniform in [0,1) computed on the host from Philox counter
+ * {index, step, tag|(batch<<8), 0} (the host draws of Random::uniform,
+ * share/src/bi/random/Random.hpp, e.g. the systematic offset). */
+double bihip_uniform_host(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                          uint32_t batch);
 /* Philox4x32-10 blocks of n counters (ctr_dev: n x 4 words) under one key;
  * replaces the cuRAND XORWOW state of share/src/bi/cuda/random/RngGPU.cuh:19-206. */
 int bihip_philox(bihip_ctx* ctx, const uint32_t* ctr_dev, uint32_t key0, uint32_t key1, int n,
@@ -133,6 +151,12 @@ int bihip_init_samples(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint
 int bihip_transition(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32_t batch,
                      uint32_t step, double t1, double t2, int P, const void* Xin_dev,
                      void* Xout_dev, int ld, const int* as_dev, int* nsteps_dev);
+/* The ode block alone: RK4Integrator / DOPRI5Integrator::update(t1, t2, s)
+ * (share/src/bi/ode/RK4Integrator.hpp:43-75, share/src/bi/ode/DOPRI5Integrator.hpp:44-76),
+ * in place, without noise.  aux_dev: the per-particle noise variable the
+ * right-hand side reads (PZ alpha), or NULL (PZ then uses alpha = mu). */
+int bihip_flow(bihip_ctx* ctx, const bihip_model* m, double t1, double t2, int P, void* X_dev,
+               int ld, const double* aux_dev, int* nsteps_dev);
 /* Model::observationLogDensities(s, mask, lp) (share/tt/cpp/model.hpp.tt:757-765):
  * lws[p] += log p(y | x_p); y_dev holds nobs values. */
 int bihip_obs_logdensity(bihip_ctx* ctx, const bihip_model* m, const double* y_dev, int P,

@@ -24,8 +24,11 @@ TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 
 EXPORTS = [
     "bihip_ctx_create", "bihip_ctx_destroy", "bihip_ctx_set_stream", "bihip_ctx_synchronize",
     "bihip_last_error", "bihip_model_nobs", "bihip_model_elem_size",
+    "bihip_malloc", "bihip_free", "bihip_memcpy_to_device", "bihip_memcpy_to_host",
+    "bihip_memcpy_device", "bihip_fill", "bihip_iota", "bihip_uniform_host",
     "bihip_philox", "bihip_uniforms", "bihip_normals",
-    "bihip_init_samples", "bihip_transition", "bihip_obs_logdensity", "bihip_obs_sample",
+    "bihip_init_samples", "bihip_transition", "bihip_flow", "bihip_obs_logdensity",
+    "bihip_obs_sample",
     "bihip_ess_reduce", "bihip_logsumexp_reduce", "bihip_max_reduce",
     "bihip_sumexpu_inclusive_scan",
     "bihip_systematic_cumulative_offspring", "bihip_stratified_cumulative_offspring",
@@ -97,6 +100,14 @@ def _declare(L):
         "bihip_ctx_synchronize": [P],
         "bihip_model_nobs": [pm],
         "bihip_model_elem_size": [pm],
+        "bihip_malloc": [P, c.c_size_t, c.POINTER(P)],
+        "bihip_free": [P, P],
+        "bihip_memcpy_to_device": [P, P, P, c.c_size_t],
+        "bihip_memcpy_to_host": [P, P, P, c.c_size_t],
+        "bihip_memcpy_device": [P, P, P, c.c_size_t],
+        "bihip_fill": [P, P, i, d],
+        "bihip_iota": [P, P, i],
+        "bihip_flow": [P, pm, d, d, i, P, i, P, P],
         "bihip_philox": [P, P, u32, u32, i, P],
         "bihip_uniforms": [P, u64, u32, u32, u32, i, P],
         "bihip_normals": [P, u64, u32, u32, u32, u32, i, i, P],
@@ -137,6 +148,8 @@ def _declare(L):
         f.restype = c.c_int
     L.bihip_last_error.argtypes = []
     L.bihip_last_error.restype = c.c_char_p
+    L.bihip_uniform_host.argtypes = [u64, u32, u32, u32, u32]
+    L.bihip_uniform_host.restype = c.c_double
 
 
 def check(rc):

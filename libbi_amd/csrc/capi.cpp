@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,21 @@ static int ws_alloc(ScanWorkspace* ws, int tiles) {
   BIHIP_HIP(hipMemset(ws->state, 0, sizeof(ScanState)));
   BIHIP_HIP(hipMemset(ws->flags, 0, sizeof(uint32_t) * tiles));
   return BIHIP_OK;
+}
+
+static int rw_alloc(ReduceWork* w, int nb) {
+  w->max_blocks = reduce_blocks(nb);
+  int rc;
+  if ((rc = dalloc(&w->l2, (size_t)3 * w->max_blocks))) return rc;
+  if ((rc = dalloc(&w->counter, 1))) return rc;
+  BIHIP_HIP(hipMemset(w->counter, 0, sizeof(uint32_t)));
+  return BIHIP_OK;
+}
+
+static void rw_free(ReduceWork* w) {
+  dfree(w->l2);
+  dfree(w->counter);
+  *w = ReduceWork{};
 }
 
 static void ws_free(ScanWorkspace* ws) {
@@ -93,6 +109,7 @@ struct bihip_ctx {
   double* partials = nullptr;
   int partials_cap = 0;
   double* out = nullptr;  // 5 doubles
+  ReduceWork rw{};
   ScanWorkspace ws{};
   uint64_t* EF = nullptr;
   int EF_cap = 0;
@@ -108,9 +125,11 @@ static int ctx_partials(bihip_ctx* ctx, int nb) {
   if (nb > ctx->partials_cap) {
     BIHIP_HIP(hipStreamSynchronize(ctx->stream));
     dfree(ctx->partials);
+    rw_free(&ctx->rw);
     ctx->partials_cap = 0;
     int rc = dalloc(&ctx->partials, (size_t)3 * nb);
     if (rc) return rc;
+    if ((rc = rw_alloc(&ctx->rw, nb))) return rc;
     ctx->partials_cap = nb;
   }
   return BIHIP_OK;
@@ -167,6 +186,7 @@ int bihip_ctx_destroy(bihip_ctx* ctx) {
   dfree(ctx->partials);
   dfree(ctx->out);
   dfree(ctx->EF);
+  rw_free(&ctx->rw);
   ws_free(&ctx->ws);
   delete ctx;
   return BIHIP_OK;
@@ -191,7 +211,85 @@ int bihip_model_elem_size(const bihip_model* m) {
   return m && m->precision == BIHIP_F64 ? 8 : 4;
 }
 
+// ================================================================ memory
+int bihip_malloc(bihip_ctx* ctx, size_t bytes, void** ptr) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (ptr == nullptr) return set_error(BIHIP_EINVAL, "ptr is NULL");
+  BIHIP_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return BIHIP_OK;
+}
+
+int bihip_free(bihip_ctx* ctx, void* ptr) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  if (ptr) BIHIP_HIP(hipFree(ptr));
+  return BIHIP_OK;
+}
+
+int bihip_memcpy_to_device(bihip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  return BIHIP_OK;
+}
+
+int bihip_memcpy_to_host(bihip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+  return BIHIP_OK;
+}
+
+int bihip_memcpy_device(bihip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return BIHIP_OK;
+}
+
+int bihip_fill(bihip_ctx* ctx, double* x, int n, double v) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  return launch_fill_f64(x, n, v, ctx->stream);
+}
+
+int bihip_iota(bihip_ctx* ctx, int* x, int n) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  return launch_iota(x, n, ctx->stream);
+}
+
 // ================================================================ random numbers
+// Philox4x32-10 on the host (Salmon et al., SC'11), as the device's.
+static void host_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint32_t)p1;
+    c[2] = n2;
+    c[3] = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+double bihip_uniform_host(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                          uint32_t batch) {
+  uint32_t c[4] = {index, step, tag | (batch << 8), 0u};
+  host_philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t m = ((((uint64_t)c[0]) << 32) | c[1]) >> 11;
+  return (double)m * 0x1.0p-53;
+}
+
 int bihip_philox(bihip_ctx* ctx, const uint32_t* ctr, uint32_t k0, uint32_t k1, int n,
                  uint32_t* out) {
   int rc = ctx_enter(ctx);
@@ -264,6 +362,28 @@ int bihip_transition(bihip_ctx* ctx, const bihip_model* m, uint64_t seed, uint32
   return launch_transition(L, ctx->stream);
 }
 
+int bihip_flow(bihip_ctx* ctx, const bihip_model* m, double t1, double t2, int P, void* X,
+               int ld, const double* aux, int* nsteps) {
+  int rc = ctx_enter(ctx);
+  if (rc) return rc;
+  if ((rc = check_model(m))) return rc;
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  if ((rc = check_interval(m, t1, t2))) return rc;
+  if (!(t2 - t1 > 0.0)) return BIHIP_OK;  // DOPRI5Integrator.hpp:51: nothing to do
+  TransLaunch L{};
+  L.model = m;
+  L.Xin = X;
+  L.Xout = X;
+  L.ld = ld;
+  L.P = P;
+  L.t1 = t1;
+  L.t2 = t2;
+  L.nsteps_out = nsteps;
+  L.flow_only = 1;
+  L.aux = aux;
+  return launch_transition(L, ctx->stream);
+}
+
 int bihip_obs_logdensity(bihip_ctx* ctx, const bihip_model* m, const double* y, int P,
                          const void* X, int ld, double* lws) {
   int rc = ctx_enter(ctx);
@@ -289,7 +409,7 @@ static int reduce_to_out(bihip_ctx* ctx, const double* lws, int P, double host[5
   int rc = ctx_partials(ctx, nb);
   if (rc) return rc;
   if ((rc = launch_lws_partials(lws, P, ctx->partials, ctx->stream))) return rc;
-  if ((rc = launch_finalize(ctx->partials, nb, P, ctx->out, ctx->stream))) return rc;
+  if ((rc = launch_finalize(ctx->partials, nb, P, ctx->out, ctx->rw, ctx->stream))) return rc;
   if (host) {
     BIHIP_HIP(hipMemcpyAsync(host, ctx->out, 5 * sizeof(double), hipMemcpyDeviceToHost,
                              ctx->stream));
@@ -471,6 +591,7 @@ struct bihip_pf {
   uint64_t* EF = nullptr;
   double* partials = nullptr;
   double* out = nullptr;
+  ReduceWork rw{};
   PFDevice* dev = nullptr;
   double* y = nullptr;
   double* hist_li = nullptr;
@@ -579,7 +700,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
   }
   PFHistory H{pf->hist_li, pf->hist_ess, pf->hist_res};
   if ((rc = launch_pf_correct_finalize(pf->partials, pf->nparts, P, pf->cfg.essRel, pf->dev, H,
-                                       s)))
+                                       pf->rw, s)))
     return rc;
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
   return BIHIP_OK;
@@ -662,7 +783,8 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   PF_ALLOC(pf->hist_res, cfg->T);
   if (cfg->record) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
 #undef PF_ALLOC
-  if ((rc = ws_alloc(&pf->ws, scan_tiles(P)))) {
+  if ((rc = ws_alloc(&pf->ws, scan_tiles(P))) ||
+      (rc = rw_alloc(&pf->rw, std::max(pf->nparts, ceil_div(P, RED_TILE))))) {
     bihip_pf_destroy(pf);
     return rc;
   }
@@ -694,6 +816,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->hist_ess);
   dfree(pf->hist_res);
   dfree(pf->hist_anc);
+  rw_free(&pf->rw);
   ws_free(&pf->ws);
   delete pf;
   return BIHIP_OK;
@@ -769,7 +892,7 @@ int bihip_pf_term(bihip_pf* pf, double* logZ) {
   if (rc2) return rc2;
   hipStream_t s = pf->ctx->stream;
   if ((rc = launch_lws_partials(pf->lws, P, pf->ctx->partials, s))) return rc;
-  if ((rc = launch_finalize(pf->ctx->partials, nb, P, pf->out, s))) return rc;
+  if ((rc = launch_finalize(pf->ctx->partials, nb, P, pf->out, pf->ctx->rw, s))) return rc;
   BIHIP_HIP(hipMemcpyAsync(h, pf->out, sizeof(h), hipMemcpyDeviceToHost, s));
   BIHIP_HIP(hipStreamSynchronize(s));
   if (logZ) *logZ = h[0] + log(h[1]) - log((double)P);

@@ -84,11 +84,11 @@ struct Normal<float> {
   __device__ __forceinline__ static void pair(uint32_t r0, uint32_t r1, float& z0, float& z1) {
     const float u1 = 1.0f - (float)(r0 >> 8) * 0x1.0p-24f;
     const float u2 = (float)(r1 >> 8) * 0x1.0p-24f;
-    const float rad = sqrtf(-2.0f * __logf(u1));
-    float s, c;
-    sincospif(2.0f * u2, &s, &c);
-    z0 = rad * c;
-    z1 = rad * s;
+    // native v_log_f32 (log2), v_sqrt_f32, and v_sin/v_cos_f32, which take
+    // their argument in revolutions: sin(2 pi u2) is v_sin_f32(u2)
+    const float rad = __builtin_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(u1));
+    z0 = rad * __builtin_amdgcn_cosf(u2);
+    z1 = rad * __builtin_amdgcn_sinf(u2);
   }
 };
 

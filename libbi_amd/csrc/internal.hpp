@@ -72,7 +72,7 @@ constexpr int RED_BLOCK = 256;
 constexpr int RED_ITEMS = 8;
 constexpr int RED_TILE = RED_BLOCK * RED_ITEMS;
 constexpr int TRANS_BLOCK = 256;
-constexpr int FINAL_BLOCK = 1024;
+constexpr int COMB_BLOCK = 256;
 // RK4 steps per interval are bounded (t + h must advance t).
 constexpr int RK4_MAX_STEPS = 1 << 20;
 
@@ -98,6 +98,8 @@ struct TransLaunch {
   double t0, dt;
   const double* y_base;
   int nobs;
+  int flow_only;            // 1: no noise (RK4Integrator / DOPRI5Integrator::update)
+  const double* aux;        // flow_only: per-particle noise variable, e.g. PZ alpha
 };
 int transition_blocks(const bihip_model* m, int P);
 int launch_transition(const TransLaunch& L, hipStream_t s);
@@ -114,15 +116,25 @@ int launch_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, 
 int launch_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
                    int n, int precision, void* z, hipStream_t s);
 bool model_supported(const bihip_model* m);
+int launch_fill_f64(double* x, int n, double v, hipStream_t s);
+int launch_iota(int* x, int n, hipStream_t s);
 
 // ---- reduce.hip
+// Level-2 partials and the last-block ticket of the two-level reduction.
+struct ReduceWork {
+  double* l2;         // 3 doubles per combining block
+  uint32_t* counter;  // zero between launches
+  int max_blocks;
+};
+int reduce_blocks(int nb);
 // Per-tile (max, sum w, sum w^2) partials of log-weights.
 int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t s);
 // Combine nb partials: out = {mx, S1, S2, ess, lW} (5 doubles).
-int launch_finalize(const double* partials, int nb, int P, double* out, hipStream_t s);
+int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWork w,
+                    hipStream_t s);
 // Combine partials into the filter scalars and histories.
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
-                               PFDevice* dev, PFHistory hist, hipStream_t s);
+                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s);
 
 // ---- scan.hip
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,

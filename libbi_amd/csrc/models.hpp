@@ -29,6 +29,25 @@ struct ModelArgs {
 };
 
 // ---------------------------------------------------------------- lane groups
+// Cross-lane moves inside a group.  Groups of four use DPP quad permutations
+// (a VALU operand modifier, no LDS traffic); other sizes use ds_bpermute.
+constexpr int QP_PREV = 3 | (0 << 2) | (1 << 4) | (2 << 6);  // lane g reads g-1 (mod 4)
+constexpr int QP_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane g reads g+1 (mod 4)
+constexpr int QP_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+constexpr int QP_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int LPP>
 struct Group {
   int lane;  // lane in the wavefront
@@ -41,6 +60,8 @@ struct Group {
   __device__ __forceinline__ T prev(T v) const {
     if constexpr (LPP == 1) {
       return v;
+    } else if constexpr (LPP == 4) {
+      return dpp<QP_PREV>(v);
     } else {
       const int src = (lane & ~(LPP - 1)) | ((g + LPP - 1) & (LPP - 1));
       return __shfl(v, src, WAVE);
@@ -50,16 +71,25 @@ struct Group {
   __device__ __forceinline__ T next(T v) const {
     if constexpr (LPP == 1) {
       return v;
+    } else if constexpr (LPP == 4) {
+      return dpp<QP_NEXT>(v);
     } else {
       const int src = (lane & ~(LPP - 1)) | ((g + 1) & (LPP - 1));
       return __shfl(v, src, WAVE);
     }
   }
+  // butterfly sum: every lane of the group gets the same bits
   template <class T>
   __device__ __forceinline__ T sum(T v) const {
+    if constexpr (LPP == 4) {
+      v += dpp<QP_XOR1>(v);
+      v += dpp<QP_XOR2>(v);
+      return v;
+    } else {
 #pragma unroll
-    for (int o = 1; o < LPP; o <<= 1) v += __shfl_xor(v, o, WAVE);
-    return v;
+      for (int o = 1; o < LPP; o <<= 1) v += __shfl_xor(v, o, WAVE);
+      return v;
+    }
   }
 };
 
@@ -191,27 +221,37 @@ template <class T>
 __device__ __forceinline__ T err_scale(T err, T h, T atol, T rtol, T a, T b) {
   const T den = atol + rtol * fmax(fabs(a), fabs(b));
   if constexpr (sizeof(T) == 4) {
-    return __fdividef(err * h, den);
+    return err * h * __builtin_amdgcn_rcpf(den);  // v_rcp_f32 (1 ulp): the norm only steers h
   } else {
     return err * h / den;
   }
 }
 
 // DOPRI5IntegratorHost::update for one particle; returns the number of steps.
+// The step-size controller (IntegratorConstants.cpp:45-61, 77-87):
+// accept iff e2 <= 1; rejected: h *= max(facl, safe e2^-expo); accepted:
+// h *= clamp(safe e2^-expo errold^beta, facl, facr), errold = sqrt(max(e2, 1e-8)).
+// Its logarithms run in double for double states and in float (native log2 /
+// exp2) for float states.
+__device__ __forceinline__ double ctl_log(double x) { return log(x); }
+__device__ __forceinline__ double ctl_exp(double x) { return exp(x); }
+__device__ __forceinline__ float ctl_log(float x) { return __logf(x); }
+__device__ __forceinline__ float ctl_exp(float x) { return __expf(x); }
+
 template <class M, class T, class G>
 __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0)[M::DL],
                                                 double t1, double t2, const ModelArgs& a) {
   constexpr int DL = M::DL;
-  const double safe = 0.9, facl = 0.2, facr = 10.0, beta = 0.04;
-  const double logsafe = -0.10536051565782630123;  // log(0.9)
-  const double expo = 0.5 * (0.2 - beta * 0.75);
+  const T facl = T(0.2), facr = T(10.0), beta = T(0.04);
+  const T logsafe = T(-0.10536051565782630123);  // log(0.9)
+  const T expo = T(0.5 * (0.2 - 0.04 * 0.75));
   const T atol = (T)a.atol, rtol = (T)a.rtol;
-  const double invD = 1.0 / (double)M::D;
+  const T invD = T(1.0 / (double)M::D);
   const bool quirk = a.host_quirk != 0;
-  (void)safe;
 
   T k1[DL];
-  double t = t1, h = a.h0, logfacold = -9.2103403719761836;  // log(1e-4)
+  double t = t1, h = a.h0;
+  T logfacold = T(-9.2103403719761836);  // log(1e-4)
   bool k1in = false;
   int n = 0;
   while (t < t2 && n < a.nsteps) {
@@ -297,8 +337,8 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       const T e = err_scale(er[i], hT, atol, rtol, x0[i], y6[i]);
       e2l += e * e;
     }
-    const double e2 = (double)grp.sum(e2l) * invD;
-    if (e2 <= 1.0) {
+    const T e2 = grp.sum(e2l) * invD;
+    if (e2 <= T(1)) {
       t += h;
 #pragma unroll
       for (int i = 0; i < DL; ++i) {
@@ -307,14 +347,14 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       }
     }
     if (t < t2) {
-      const double logfac11 = expo * log(e2);
-      if (e2 > 1.0) {
-        h *= fmax(facl, exp(logsafe - logfac11));
+      const T logfac11 = expo * ctl_log(e2);
+      if (e2 > T(1)) {
+        h *= (double)fmax(facl, ctl_exp(logsafe - logfac11));
       } else {
-        double fac = exp(beta * logfacold + logsafe - logfac11);
+        T fac = ctl_exp(beta * logfacold + logsafe - logfac11);
         fac = fmin(facr, fmax(facl, fac));
-        h *= fac;
-        logfacold = 0.5 * log(fmax(e2, 1.0e-8));
+        h *= (double)fac;
+        logfacold = T(0.5) * ctl_log(fmax(e2, T(1.0e-8)));
       }
     }
     ++n;

@@ -40,53 +40,86 @@ __global__ __launch_bounds__(RED_BLOCK) void k_lws_partials(const double* lws, i
   }
 }
 
-// Combine nb partials (one block): M = max m_b, S1 = sum s1_b e^{m_b - M},
-// S2 = sum s2_b e^{2(m_b - M)}.  Every thread returns the totals.
-__device__ void combine_partials(const double* partials, int nb, double& M, double& S1,
-                                 double& S2) {
-  __shared__ double lds[FINAL_BLOCK / WAVE];
+// (m, s1, s2) triples -> (M, S1, S2) over [lo, hi) in a fixed order; every
+// thread of the block gets the totals.  `load(i, m, s1, s2)` fetches triple i.
+template <class Load>
+__device__ void combine(int lo, int hi, Load load, double* lds, double& M, double& S1,
+                        double& S2) {
   double m = __builtin_nan("");
-  for (int b = threadIdx.x; b < nb; b += FINAL_BLOCK) m = nan_max(m, partials[3 * b]);
-  M = block_max<FINAL_BLOCK>(m, lds);
+  for (int i = lo + threadIdx.x; i < hi; i += COMB_BLOCK) {
+    double a, b, c;
+    load(i, a, b, c);
+    m = nan_max(m, a);
+  }
+  M = block_max<COMB_BLOCK>(m, lds);
   double s1 = 0.0, s2 = 0.0;
-  for (int b = threadIdx.x; b < nb; b += FINAL_BLOCK) {
-    const double a1 = partials[3 * b + 1], a2 = partials[3 * b + 2];
-    if (a1 != 0.0) {
-      const double f = exp(partials[3 * b] - M);
-      s1 += a1 * f;
-      s2 += a2 * (f * f);
+  for (int i = lo + threadIdx.x; i < hi; i += COMB_BLOCK) {
+    double a, b, c;
+    load(i, a, b, c);
+    if (b != 0.0) {
+      const double f = exp(a - M);
+      s1 += b * f;
+      s2 += c * (f * f);
     }
   }
-  S1 = block_sum<FINAL_BLOCK>(s1, lds);
-  S2 = block_sum<FINAL_BLOCK>(s2, lds);
+  S1 = block_sum<COMB_BLOCK>(s1, lds);
+  S2 = block_sum<COMB_BLOCK>(s2, lds);
 }
 
-__global__ __launch_bounds__(FINAL_BLOCK) void k_finalize(const double* partials, int nb, int P,
-                                                          double* out) {
+// Two-level combination of nb partials: block b combines partials
+// [b*COMB_BLOCK, (b+1)*COMB_BLOCK) into a level-2 triple; the last block to
+// finish combines the level-2 triples (fixed order, so the result does not
+// depend on which block is last) and finishes:
+//   - with dev: BootstrapPF::correct's bookkeeping -- ess, lW = ess_reduce(lws),
+//     logIncrements[k] = lW - logLikelihood, logLikelihood = lW, and the
+//     resampling decision ess < essRel * P for the next time;
+//   - with out: {max, sum w, sum w^2, ess, lW}.
+__global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, int nb, int P,
+                                                       double essRel, PFDevice* dev,
+                                                       PFHistory hist, double* out,
+                                                       ReduceWork w) {
+  __shared__ double lds[COMB_BLOCK / WAVE];
+  __shared__ int last;
+  const int lo = blockIdx.x * COMB_BLOCK, hi = min(nb, lo + COMB_BLOCK);
   double M, S1, S2;
-  combine_partials(partials, nb, M, S1, S2);
+  combine(lo, hi,
+          [&](int i, double& a, double& b, double& c) {
+            a = partials[3 * i];
+            b = partials[3 * i + 1];
+            c = partials[3 * i + 2];
+          },
+          lds, M, S1, S2);
   if (threadIdx.x == 0) {
+    __hip_atomic_store(&w.l2[3 * blockIdx.x], M, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&w.l2[3 * blockIdx.x + 1], S1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&w.l2[3 * blockIdx.x + 2], S2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t =
+        __hip_atomic_fetch_add(w.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+
+  combine(0, (int)gridDim.x,
+          [&](int i, double& a, double& b, double& c) {
+            a = __hip_atomic_load(&w.l2[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            b = __hip_atomic_load(&w.l2[3 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            c = __hip_atomic_load(&w.l2[3 * i + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          },
+          lds, M, S1, S2);
+  if (threadIdx.x != 0) return;
+  __hip_atomic_store(w.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double ess = S1 * S1 / S2;
+  const double lW = M + log(S1) - log((double)P);
+  if (out != nullptr) {
     out[0] = M;
     out[1] = S1;
     out[2] = S2;
-    out[3] = S1 * S1 / S2;                    // ESS
-    out[4] = M + log(S1) - log((double)P);    // lW
+    out[3] = ess;
+    out[4] = lW;
   }
-}
-
-// BootstrapPF::correct bookkeeping on the device:
-//   ess, lW = ess_reduce(lws); logIncrements[k] = lW - logLikelihood;
-//   logLikelihood = lW; resample at the next time iff ess < essRel * P.
-__global__ __launch_bounds__(FINAL_BLOCK) void k_pf_correct_finalize(const double* partials,
-                                                                     int nb, int P,
-                                                                     double essRel, PFDevice* dev,
-                                                                     PFHistory hist) {
-  double M, S1, S2;
-  combine_partials(partials, nb, M, S1, S2);
-  if (threadIdx.x == 0) {
+  if (dev != nullptr) {
     const int k = dev->k;
-    const double ess = S1 * S1 / S2;
-    const double lW = M + log(S1) - log((double)P);
     if (hist.resampled) hist.resampled[k] = dev->resample != 0 && dev->status == BIHIP_OK;
     if (hist.logIncrements) hist.logIncrements[k] = lW - dev->logLikelihood;
     if (hist.ess) hist.ess[k] = ess;
@@ -99,6 +132,8 @@ __global__ __launch_bounds__(FINAL_BLOCK) void k_pf_correct_finalize(const doubl
   }
 }
 
+int reduce_blocks(int nb) { return ceil_div(nb, COMB_BLOCK); }
+
 int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t s) {
   hipLaunchKernelGGL(k_lws_partials, dim3(ceil_div(P, RED_TILE)), dim3(RED_BLOCK), 0, s, lws, P,
                      partials);
@@ -106,16 +141,18 @@ int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t 
   return BIHIP_OK;
 }
 
-int launch_finalize(const double* partials, int nb, int P, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FINAL_BLOCK), 0, s, partials, nb, P, out);
+int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWork w,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
+                     0.0, (PFDevice*)nullptr, PFHistory{}, out, w);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }
 
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
-                               PFDevice* dev, PFHistory hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_pf_correct_finalize, dim3(1), dim3(FINAL_BLOCK), 0, s, partials, nb, P,
-                     essRel, dev, hist);
+                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
+                     essRel, dev, hist, (double*)nullptr, w);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

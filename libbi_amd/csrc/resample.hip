@@ -311,4 +311,28 @@ int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out
   return BIHIP_OK;
 }
 
+__global__ void k_fill_f64(double* x, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+int launch_fill_f64(double* x, int n, double v, hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div(n, 256)), dim3(256), 0, s, x, n, v);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+__global__ void k_iota(int* x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = i;
+}
+
+int launch_iota(int* x, int n, hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  hipLaunchKernelGGL(k_iota, dim3(ceil_div(n, 256)), dim3(256), 0, s, x, n);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
 }  // namespace bihip

@@ -149,17 +149,22 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
     __hip_atomic_store(&ws.flags[tile], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 
-  // prefix = sum of the aggregates of tiles [0, tile), fixed order
-  V part = V(0);
+  // prefix = sum of the aggregates of tiles [0, tile), fixed order.  Flags
+  // are polled with relaxed loads (an acquire per poll would invalidate the
+  // caches on every iteration); one acquire fence orders the aggregate reads.
   for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(&ws.flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    while (__hip_atomic_load(&ws.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
       if (++spins > SCAN_SPIN_LIMIT) {
         __hip_atomic_store(&ws.state->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  V part = V(0);
+  for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK) {
     const uint64_t a0 =
         __hip_atomic_load(&ws.aggregates[2 * j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t a1 =

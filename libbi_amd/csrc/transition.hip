@@ -32,6 +32,8 @@ struct TransArgs {
   double t0, dt;
   const double* y_base;
   int nobs;
+  int flow_only;        // 1: deterministic flow, no noise (integrator update)
+  const double* aux;    // flow_only: per-particle noise variable (PZ alpha) or null
 };
 
 static ModelArgs to_args(const bihip_model* m) {
@@ -174,14 +176,22 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
     for (int i = 0; i < DL; ++i) x[i] = Xin[(size_t)(lo + i) * a.ld + src];
 
     M m(a.m);
-    if constexpr (M::NOISE_FIRST) noise_first(m, a, p, step);
+    if constexpr (M::NOISE_FIRST) {
+      if (!a.flow_only) {
+        noise_first(m, a, p, step);
+      } else {
+        m.alpha = a.aux != nullptr ? (T)a.aux[p] : (T)m.mu;
+      }
+    }
     int n;
     if constexpr (INTEG == BIHIP_RK4) {
       n = integrate_rk4(m, grp, x, t1, t2, a.m.h0);
     } else {
       n = integrate_dopri5(m, grp, x, t1, t2, a.m);
     }
-    if constexpr (!M::NOISE_FIRST) noise_after(m, a, p, lo, step, x);
+    if constexpr (!M::NOISE_FIRST) {
+      if (!a.flow_only) noise_after(m, a, p, lo, step, x);
+    }
 
     T* Xout = static_cast<T*>(a.Xout);
 #pragma unroll
@@ -383,6 +393,8 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.dt = L.dt;
   r.a.y_base = L.y_base;
   r.a.nobs = L.nobs;
+  r.a.flow_only = L.flow_only;
+  r.a.aux = L.aux;
   r.blocks = transition_blocks(L.model, L.P);
   r.s = s;
   r.integ = L.model->integrator;

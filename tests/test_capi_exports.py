@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "bihip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(bihip_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|double|const char\*)\s+(bihip_\w+)\s*\(", text,
+                                 re.M)))
 
 
 def test_header_declares_expected_set():
@@ -34,6 +35,15 @@ def test_binding_loads():
     assert capi.nobs(m) == 20
     assert L.bihip_model_elem_size(ctypes.byref(capi.make_model("pz", precision=capi.F64))) == 8
     assert L.bihip_model_elem_size(ctypes.byref(capi.make_model("pz", precision=capi.F32))) == 4
+
+
+def test_host_uniform_matches_oracle(orc):
+    L = capi.lib()
+    for (seed, idx, step, tag, batch) in [(0, 0, 0, 3, 0), (2 ** 64 - 1, 7, 99, 4, 5),
+                                          (123456789, 0, 17, 3, 1)]:
+        u = L.bihip_uniform_host(seed, idx, step, tag, batch)
+        r = orc.philox_block(seed, idx, step, tag, batch, 0)
+        assert u == orc.lib().orc_u53(r[0], r[1])
 
 
 def test_struct_layouts_match_header():

@@ -323,3 +323,29 @@ def test_gather_rows_in_place(ctx, orc, torch_gpu, elem):
     Xd = dev(X)
     capi.check(L().bihip_gather_rows(ctx.h, _c(dev(a)), P, d, _c(Xd), P, elem))
     np.testing.assert_array_equal(host(Xd), X[:, a])
+
+
+@pytest.mark.parametrize("kind,integ,prec", [("l96", capi.DOPRI5, capi.F64),
+                                             ("l96", capi.DOPRI5, capi.F32),
+                                             ("pz", capi.RK4, capi.F64),
+                                             ("lg", capi.DOPRI5, capi.F64)])
+def test_flow_is_noise_free_transition(ctx, orc, torch_gpu, kind, integ, prec):
+    """bihip_flow (RK4Integrator / DOPRI5Integrator::update) equals the oracle's
+    transition with the noise switched off (noise_sd = 0; PZ: sigma = 0)."""
+    P = 1500
+    gm, om, X0, _ = _setup(orc, kind, integ, prec, 0, P)
+    if kind == "pz":
+        om.par[1] = 0.0
+    else:
+        om.par[1] = 0.0
+    npdt = np.float64 if prec == capi.F64 else np.float32
+    X = dev(X0.astype(npdt))
+    ns = empty(P, torch_gpu.int32)
+    t1, t2 = (0.0, 1.0) if kind != "l96" else (0.0, 0.05)
+    capi.check(L().bihip_flow(ctx.h, ctypes.byref(gm), t1, t2, P, _c(X), P, None, _c(ns)))
+    ref, rns = orc.transition(om, 1, 1, t1, t2, X0.astype(npdt).astype(np.float64),
+                              return_steps=True)
+    tol = _tol(prec, kind)
+    np.testing.assert_allclose(host(X).astype(np.float64), ref, rtol=tol, atol=tol)
+    if prec == capi.F64:
+        np.testing.assert_array_equal(host(ns), rns)
