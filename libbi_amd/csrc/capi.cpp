@@ -39,32 +39,31 @@ static int ws_alloc(ScanWorkspace* ws, int tiles) {
   ws->max_tiles = tiles;
   int rc;
   if ((rc = dalloc(&ws->state, 1))) return rc;
-  if ((rc = dalloc(&ws->flags, tiles))) return rc;
-  if ((rc = dalloc(&ws->aggregates, (size_t)2 * tiles))) return rc;
+  if ((rc = dalloc(&ws->words, (size_t)SCAN_WORDS * tiles))) return rc;
   BIHIP_HIP(hipMemset(ws->state, 0, sizeof(ScanState)));
-  BIHIP_HIP(hipMemset(ws->flags, 0, sizeof(uint32_t) * tiles));
+  BIHIP_HIP(hipMemset(ws->words, 0, sizeof(uint64_t) * SCAN_WORDS * tiles));
   return BIHIP_OK;
 }
 
 static int rw_alloc(ReduceWork* w, int nb) {
   w->max_blocks = reduce_blocks(nb);
   int rc;
-  if ((rc = dalloc(&w->l2, (size_t)3 * w->max_blocks))) return rc;
-  if ((rc = dalloc(&w->counter, 1))) return rc;
-  BIHIP_HIP(hipMemset(w->counter, 0, sizeof(uint32_t)));
+  if ((rc = dalloc(&w->words, (size_t)6 * w->max_blocks))) return rc;
+  if ((rc = dalloc(&w->state, 1))) return rc;
+  BIHIP_HIP(hipMemset(w->words, 0, sizeof(uint64_t) * 6 * w->max_blocks));
+  BIHIP_HIP(hipMemset(w->state, 0, sizeof(ScanState)));
   return BIHIP_OK;
 }
 
 static void rw_free(ReduceWork* w) {
-  dfree(w->l2);
-  dfree(w->counter);
+  dfree(w->words);
+  dfree(w->state);
   *w = ReduceWork{};
 }
 
 static void ws_free(ScanWorkspace* ws) {
   dfree(ws->state);
-  dfree(ws->flags);
-  dfree(ws->aggregates);
+  dfree(ws->words);
   *ws = ScanWorkspace{};
 }
 
@@ -904,12 +903,15 @@ int bihip_pf_status(bihip_pf* pf) {
   int rc = ctx_enter(pf->ctx);
   if (rc) return rc;
   PFDevice d;
-  ScanState st;
+  ScanState st, rt;
   hipStream_t s = pf->ctx->stream;
   BIHIP_HIP(hipMemcpyAsync(&d, pf->dev, sizeof(d), hipMemcpyDeviceToHost, s));
   BIHIP_HIP(hipMemcpyAsync(&st, pf->ws.state, sizeof(st), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipMemcpyAsync(&rt, pf->rw.state, sizeof(rt), hipMemcpyDeviceToHost, s));
   BIHIP_HIP(hipStreamSynchronize(s));
-  if (st.error) return set_error(BIHIP_EHIP, "scan look-back exceeded its spin limit");
+  if (st.error || rt.error)
+    return set_error(BIHIP_EHIP, "cross-block wait exceeded its spin limit (scan %u, reduce %u)",
+                     st.error, rt.error);
   if (d.status == BIHIP_DEGENERATE)
     return set_error(BIHIP_DEGENERATE, "particle filter degenerated (W <= 0)");
   return BIHIP_OK;

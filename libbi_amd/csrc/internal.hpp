@@ -59,11 +59,11 @@ struct ScanState {
 };
 
 struct ScanWorkspace {
-  ScanState* state;     // device
-  uint32_t* flags;      // device, one per tile
-  uint64_t* aggregates; // device, two words per tile
+  ScanState* state;  // device: ticket, epoch, error
+  uint64_t* words;   // device: SCAN_WORDS epoch-tagged words per tile
   int max_tiles;
 };
+constexpr int SCAN_WORDS = 4;
 
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -122,8 +122,8 @@ int launch_iota(int* x, int n, hipStream_t s);
 // ---- reduce.hip
 // Level-2 partials and the last-block ticket of the two-level reduction.
 struct ReduceWork {
-  double* l2;         // 3 doubles per combining block
-  uint32_t* counter;  // zero between launches
+  uint64_t* words;    // 6 epoch-tagged words (3 doubles) per combining block
+  ScanState* state;   // ticket (zero between launches), epoch, error
   int max_blocks;
 };
 int reduce_blocks(int nb);

@@ -5,8 +5,7 @@
 // (share/src/bi/primitive/vector_primitive.hpp:860-974) and the
 // bookkeeping of BootstrapPF::correct (share/src/bi/filter/BootstrapPF.hpp:171-181)
 // with Resampler::resample's trigger (share/src/bi/resampler/Resampler.hpp:168).
-#include "blockreduce.hpp"
-#include "internal.hpp"
+#include "scan.hpp"
 
 namespace bihip {
 
@@ -43,7 +42,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_lws_partials(const double* lws, i
 // (m, s1, s2) triples -> (M, S1, S2) over [lo, hi) in a fixed order; every
 // thread of the block gets the totals.  `load(i, m, s1, s2)` fetches triple i.
 template <class Load>
-__device__ void combine(int lo, int hi, Load load, double* lds, double& M, double& S1,
+__device__ __forceinline__ void combine(int lo, int hi, Load load, double* lds, double& M, double& S1,
                         double& S2) {
   double m = __builtin_nan("");
   for (int i = lo + threadIdx.x; i < hi; i += COMB_BLOCK) {
@@ -67,9 +66,10 @@ __device__ void combine(int lo, int hi, Load load, double* lds, double& M, doubl
 }
 
 // Two-level combination of nb partials: block b combines partials
-// [b*COMB_BLOCK, (b+1)*COMB_BLOCK) into a level-2 triple; the last block to
-// finish combines the level-2 triples (fixed order, so the result does not
-// depend on which block is last) and finishes:
+// [b*COMB_BLOCK, (b+1)*COMB_BLOCK) into a level-2 triple, published as
+// epoch-tagged words; the block that takes the last ticket combines the
+// level-2 triples (fixed order, so the result does not depend on which block
+// is last) and finishes:
 //   - with dev: BootstrapPF::correct's bookkeeping -- ess, lW = ess_reduce(lws),
 //     logIncrements[k] = lW - logLikelihood, logLikelihood = lW, and the
 //     resampling decision ess < essRel * P for the next time;
@@ -80,6 +80,11 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, i
                                                        ReduceWork w) {
   __shared__ double lds[COMB_BLOCK / WAVE];
   __shared__ int last;
+  __shared__ uint32_t s_epoch;
+  if (threadIdx.x == 0)
+    s_epoch = __hip_atomic_load(&w.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+  __syncthreads();
+  const uint32_t epoch = s_epoch;
   const int lo = blockIdx.x * COMB_BLOCK, hi = min(nb, lo + COMB_BLOCK);
   double M, S1, S2;
   combine(lo, hi,
@@ -90,11 +95,12 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, i
           },
           lds, M, S1, S2);
   if (threadIdx.x == 0) {
-    __hip_atomic_store(&w.l2[3 * blockIdx.x], M, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&w.l2[3 * blockIdx.x + 1], S1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&w.l2[3 * blockIdx.x + 2], S2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* my = w.words + 6 * (size_t)blockIdx.x;
+    publish(my, epoch, M);
+    publish(my + 2, epoch, S1);
+    publish(my + 4, epoch, S2);
     const uint32_t t =
-        __hip_atomic_fetch_add(w.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&w.state->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = t == gridDim.x - 1;
   }
   __syncthreads();
@@ -102,13 +108,15 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, i
 
   combine(0, (int)gridDim.x,
           [&](int i, double& a, double& b, double& c) {
-            a = __hip_atomic_load(&w.l2[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            b = __hip_atomic_load(&w.l2[3 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            c = __hip_atomic_load(&w.l2[3 * i + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t* q = w.words + 6 * (size_t)i;
+            a = await_value<double>(q, epoch, &w.state->error);
+            b = await_value<double>(q + 2, epoch, &w.state->error);
+            c = await_value<double>(q + 4, epoch, &w.state->error);
           },
           lds, M, S1, S2);
   if (threadIdx.x != 0) return;
-  __hip_atomic_store(w.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&w.state->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&w.state->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const double ess = S1 * S1 / S2;
   const double lW = M + log(S1) - log((double)P);
   if (out != nullptr) {

@@ -6,10 +6,14 @@
 // integers (counts, or weights in 128-bit fixed point), so every association
 // order gives the same exact, monotone result.
 //
+// Publication needs no memory fence: an aggregate is split into 32-bit
+// chunks and each chunk is stored as one 64-bit word (epoch << 32 | chunk)
+// with a relaxed device-scope atomic, so every word validates itself.  (On
+// gfx950 a release at device scope writes back the whole L2, once per tile.)
+// The epoch lives on the device and advances once per launch, so no memset
+// is needed between launches and a captured graph can replay the scan.
 // Tiles are taken from an atomic ticket, so every tile waited on has already
-// started; a spin limit turns a lost tile into an error flag instead of a
-// hang.  Flags carry a per-launch epoch kept on the device, so no memset is
-// needed between launches (and a captured graph can replay the scan).
+// started; a spin limit turns a lost tile into an error flag, never a hang.
 #pragma once
 
 #include "blockreduce.hpp"
@@ -31,23 +35,77 @@ struct u128 {
   }
 };
 
-// Values travel between tiles as two 64-bit words.
+// A value as 32-bit chunks.
 template <class V>
-struct Words {
-  __device__ static void pack(const V& v, uint64_t& a, uint64_t& b) {
-    a = (uint64_t)v;
-    b = 0;
+struct Chunks;
+template <>
+struct Chunks<uint64_t> {
+  static constexpr int N = 2;
+  __device__ static void split(uint64_t v, uint32_t (&c)[N]) {
+    c[0] = (uint32_t)v;
+    c[1] = (uint32_t)(v >> 32);
   }
-  __device__ static V unpack(uint64_t a, uint64_t) { return (V)a; }
+  __device__ static uint64_t join(const uint32_t (&c)[N]) {
+    return (uint64_t)c[0] | ((uint64_t)c[1] << 32);
+  }
 };
 template <>
-struct Words<u128> {
-  __device__ static void pack(const u128& v, uint64_t& a, uint64_t& b) {
-    a = v.lo;
-    b = v.hi;
+struct Chunks<u128> {
+  static constexpr int N = 4;
+  __device__ static void split(const u128& v, uint32_t (&c)[N]) {
+    c[0] = (uint32_t)v.lo;
+    c[1] = (uint32_t)(v.lo >> 32);
+    c[2] = (uint32_t)v.hi;
+    c[3] = (uint32_t)(v.hi >> 32);
   }
-  __device__ static u128 unpack(uint64_t a, uint64_t b) { return u128(a, b); }
+  __device__ static u128 join(const uint32_t (&c)[N]) {
+    return u128((uint64_t)c[0] | ((uint64_t)c[1] << 32), (uint64_t)c[2] | ((uint64_t)c[3] << 32));
+  }
 };
+
+template <>
+struct Chunks<double> {
+  static constexpr int N = 2;
+  __device__ static void split(double v, uint32_t (&c)[N]) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    c[0] = (uint32_t)b;
+    c[1] = (uint32_t)(b >> 32);
+  }
+  __device__ static double join(const uint32_t (&c)[N]) {
+    return __longlong_as_double((long long)((uint64_t)c[0] | ((uint64_t)c[1] << 32)));
+  }
+};
+
+// Publish / await a value in epoch-tagged words (see the file comment).
+template <class V>
+__device__ __forceinline__ void publish(uint64_t* words, uint32_t epoch, const V& v) {
+  uint32_t c[Chunks<V>::N];
+  Chunks<V>::split(v, c);
+#pragma unroll
+  for (int i = 0; i < Chunks<V>::N; ++i)
+    __hip_atomic_store(&words[i], ((uint64_t)epoch << 32) | c[i], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class V>
+__device__ __forceinline__ V await_value(uint64_t* words, uint32_t epoch, uint32_t* error) {
+  uint32_t c[Chunks<V>::N];
+#pragma unroll
+  for (int i = 0; i < Chunks<V>::N; ++i) {
+    uint32_t spins = 0;
+    uint64_t w;
+    while (((w = __hip_atomic_load(&words[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >>
+            32) != epoch) {
+      if (++spins > SCAN_SPIN_LIMIT) {
+        __hip_atomic_store(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    c[i] = (uint32_t)w;
+  }
+  return Chunks<V>::join(c);
+}
 
 template <class V>
 __device__ __forceinline__ V shfl_up_v(const V& v, int o) {
@@ -89,15 +147,18 @@ template <class V, class Loader, class Writer, class Gate>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer write, Gate gate,
                                                            int P, ScanWorkspace ws) {
   constexpr int NW = SCAN_BLOCK / WAVE;
+  constexpr int NC = Chunks<V>::N;
   __shared__ uint32_t s_tile, s_epoch;
   __shared__ V lds[NW];
   if (!gate()) return;
 
   if (threadIdx.x == 0) {
+    // the epoch is read before the ticket; the last tile advances it only
+    // after every other tile has published (and so has read it)
     const uint32_t e =
-        __hip_atomic_load(&ws.state->epoch, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        __hip_atomic_load(&ws.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
     const uint32_t t =
-        __hip_atomic_fetch_add(&ws.state->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ws.state->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_tile = t;
     s_epoch = e;
   }
@@ -139,38 +200,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
   }
   const V texcl = wpre + excl;
 
-  // publish this tile's aggregate
-  if (threadIdx.x == 0) {
-    uint64_t a0, a1;
-    Words<V>::pack(agg, a0, a1);
-    __hip_atomic_store(&ws.aggregates[2 * tile], a0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&ws.aggregates[2 * tile + 1], a1, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&ws.flags[tile], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) publish(ws.words + (size_t)NC * tile, epoch, agg);
 
-  // prefix = sum of the aggregates of tiles [0, tile), fixed order.  Flags
-  // are polled with relaxed loads (an acquire per poll would invalidate the
-  // caches on every iteration); one acquire fence orders the aggregate reads.
-  for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(&ws.flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-      if (++spins > SCAN_SPIN_LIMIT) {
-        __hip_atomic_store(&ws.state->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // prefix = sum of the aggregates of tiles [0, tile), fixed order
   V part = V(0);
-  for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK) {
-    const uint64_t a0 =
-        __hip_atomic_load(&ws.aggregates[2 * j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t a1 =
-        __hip_atomic_load(&ws.aggregates[2 * j + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    part = part + Words<V>::unpack(a0, a1);
-  }
+  for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK)
+    part = part + await_value<V>(ws.words + (size_t)NC * j, epoch, &ws.state->error);
   const V prefix = block_sum_v(part, lds);
 
 #pragma unroll
@@ -184,7 +219,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
   // the last ticket rearms the workspace for the next launch
   if (threadIdx.x == 0 && tile == (uint32_t)(ntiles - 1)) {
     __hip_atomic_store(&ws.state->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(&ws.state->epoch, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ws.state->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -22,7 +22,8 @@ def test_client_built_and_prints_usage():
 def _run(tmp_path, model, res, P, dt, seed, ess_rel, y, mean0=None, fused=False):
     obs = tmp_path / "obs.bin"
     out = tmp_path / "out.bin"
-    np.asarray(y, np.float64).tofile(obs)
+    y = np.asarray(y, np.float64)
+    y.tofile(obs)
     args = [CLIENT, model, str(res), str(P), repr(dt), str(seed), repr(ess_rel), str(obs), str(out)]
     if mean0 is not None:
         m0 = tmp_path / "mean0.bin"
@@ -32,7 +33,7 @@ def _run(tmp_path, model, res, P, dt, seed, ess_rel, y, mean0=None, fused=False)
         args.append("fused")
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    T = len(y)
+    T = y.shape[0]
     raw = out.read_bytes()
     head = np.frombuffer(raw[: 8 * (T + 1)], np.float64)
     anc = np.frombuffer(raw[8 * (T + 1):], np.int32).reshape(T + 1, P)
@@ -47,7 +48,7 @@ def test_shim_pz_filter_matches_oracle(tmp_path, orc, torch_gpu, res, fused):
     x0 = orc.init_samples(m, 100, 1)[:, 0]
     _, y = orc.simulate(m, 100, 40, 1.0, x0)
     P = 2048
-    logZ, li, anc = _run(tmp_path, "pz", res, P, 1.0, 42, 0.8, y.ravel(), fused=fused)
+    logZ, li, anc = _run(tmp_path, "pz", res, P, 1.0, 42, 0.8, y, fused=fused)
     ref = orc.pf_run(m, y, P, 1.0, 42, resampler=res, ess_rel=0.8, want_ancestors=True)
     assert logZ == pytest.approx(ref["logZ"], rel=1e-10)
     np.testing.assert_allclose(li, ref["logIncrements"], rtol=1e-9, atol=1e-9)
@@ -61,7 +62,7 @@ def test_shim_l96_filter_matches_oracle(tmp_path, orc, torch_gpu):
     mean0 = l96_spun_up(orc)
     _, y = orc.simulate(om, 7, 10, 0.05, mean0)
     P = 1024
-    logZ, li, anc = _run(tmp_path, "l96", 0, P, 0.05, 9, 0.5, y.ravel(), mean0=mean0)
+    logZ, li, anc = _run(tmp_path, "l96", 0, P, 0.05, 9, 0.5, y, mean0=mean0)
     ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0, want_ancestors=True)
     assert logZ == pytest.approx(ref["logZ"], rel=1e-9)
     np.testing.assert_array_equal(anc[1:], ref["ancestors"])
