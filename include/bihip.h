@@ -267,6 +267,56 @@ int bihip_pf_position(bihip_pf* pf);
 int bihip_pf_set_timing(bihip_pf* pf, int enable);
 int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_host);
 
+/* ================================================================ batched inner filters */
+/*
+ * B bootstrap particle filters with their own particle counts N_j
+ * (1 <= N_j <= 16384) and parameters theta_j, advanced together: the inner
+ * loop of MarginalSIR::step (share/src/bi/sampler/MarginalSIR.hpp:346-368),
+ * which the reference runs serially over theta with equal N
+ * (share/src/bi/state/MarginalSIRState.hpp:198-207).  One persistent fused
+ * launch per observation interval: workgroups take whole filters from a work
+ * queue (largest first) and run BootstrapPF::step for each.  Filter j draws
+ * its Philox streams with batch = j, so it equals bihip_pf (and the
+ * reference filter semantics) run with that batch id.  Models: PZ and LG.
+ * theta_host is B x ntheta; column i replaces bihip_model::par[i].
+ */
+typedef struct bihip_batch bihip_batch;
+typedef struct {
+  int B;          /* number of inner filters */
+  int T;          /* observations */
+  int resampler;  /* BIHIP_SYSTEMATIC or BIHIP_STRATIFIED */
+  double essRel;
+  double t0, dt;
+  uint64_t seed;
+  int record;     /* 1 = keep every step's ancestors (T x total ints) */
+  int use_graph;  /* 1 = replay each step as a captured hipGraph */
+  int block;      /* workgroup size 256/512/1024, or 0 = autotune at the first step */
+} bihip_batch_config;
+
+int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_config* cfg,
+                       const int* N_host, const double* theta_host, int ntheta,
+                       bihip_batch** b);
+int bihip_batch_destroy(bihip_batch* b);
+int bihip_batch_set_observations(bihip_batch* b, const double* y_host);
+/* Initial particles of every filter; lws = 0; logLikelihood = 0. */
+int bihip_batch_init(bihip_batch* b);
+/* n observation intervals of every filter (asynchronous). */
+int bihip_batch_step(bihip_batch* b, int n);
+/* Host copies (any may be NULL): logLikelihood[B], logIncrements[B x T],
+ * ess[B x T], resampled[B x T], status[B]. */
+int bihip_batch_outputs(bihip_batch* b, double* logLikelihood_host, double* logIncrements_host,
+                        double* ess_host, int* resampled_host, int* status_host);
+/* Ancestors of filter j at step k (record = 1): N_j ints. */
+int bihip_batch_ancestors(bihip_batch* b, int j, int k, int* as_host);
+/* The autotuner's choice (workgroup size, persistent grid) and its timings
+ * of one step at 256 / 512 / 1024 threads (ms). */
+int bihip_batch_tuning(bihip_batch* b, int* block, int* grid, double* ms3_host);
+/* HIP-event timing of each fused step launch. */
+int bihip_batch_set_timing(bihip_batch* b, int enable);
+int bihip_batch_kernel_time(bihip_batch* b, double* ms_host, long* count_host);
+/* Total particles over all inner filters. */
+long bihip_batch_particles(bihip_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,10 @@ EXPORTS = [
     "bihip_pf_step", "bihip_pf_term", "bihip_pf_status", "bihip_pf_outputs",
     "bihip_pf_ancestors", "bihip_pf_state", "bihip_pf_position", "bihip_pf_set_timing",
     "bihip_pf_kernel_time",
+    "bihip_batch_create", "bihip_batch_destroy", "bihip_batch_set_observations",
+    "bihip_batch_init", "bihip_batch_step", "bihip_batch_outputs", "bihip_batch_ancestors",
+    "bihip_batch_tuning", "bihip_batch_set_timing", "bihip_batch_kernel_time",
+    "bihip_batch_particles",
 ]
 
 
@@ -60,6 +64,15 @@ class Model(ctypes.Structure):
         ("nsteps", ctypes.c_int), ("host_quirk", ctypes.c_int), ("precision", ctypes.c_int),
         ("h0", ctypes.c_double), ("atol", ctypes.c_double), ("rtol", ctypes.c_double),
         ("par", ctypes.c_double * 16),
+    ]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int), ("T", ctypes.c_int), ("resampler", ctypes.c_int),
+        ("essRel", ctypes.c_double), ("t0", ctypes.c_double), ("dt", ctypes.c_double),
+        ("seed", ctypes.c_uint64), ("record", ctypes.c_int), ("use_graph", ctypes.c_int),
+        ("block", ctypes.c_int),
     ]
 
 
@@ -141,6 +154,16 @@ def _declare(L):
         "bihip_pf_position": [P],
         "bihip_pf_set_timing": [P, i],
         "bihip_pf_kernel_time": [P, i, pd, pl],
+        "bihip_batch_create": [P, pm, c.POINTER(BatchConfig), pi, pd, i, c.POINTER(P)],
+        "bihip_batch_destroy": [P],
+        "bihip_batch_set_observations": [P, pd],
+        "bihip_batch_init": [P],
+        "bihip_batch_step": [P, i],
+        "bihip_batch_outputs": [P, pd, pd, pd, pi, pi],
+        "bihip_batch_ancestors": [P, i, i, pi],
+        "bihip_batch_tuning": [P, pi, pi, pd],
+        "bihip_batch_set_timing": [P, i],
+        "bihip_batch_kernel_time": [P, pd, pl],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -150,6 +173,8 @@ def _declare(L):
     L.bihip_last_error.restype = c.c_char_p
     L.bihip_uniform_host.argtypes = [u64, u32, u32, u32, u32]
     L.bihip_uniform_host.restype = c.c_double
+    L.bihip_batch_particles.argtypes = [P]
+    L.bihip_batch_particles.restype = c.c_long
 
 
 def check(rc):
@@ -315,3 +340,84 @@ class ParticleFilter:
         n = ctypes.c_long()
         check(lib().bihip_pf_kernel_time(self.h, kind, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class BatchFilter:
+    """bihip_batch: B inner bootstrap filters with their own N_j and theta_j."""
+
+    def __init__(self, ctx, model, N, theta, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5,
+                 t0=0.0, record=False, use_graph=False, block=0):
+        self.ctx, self.model = ctx, model
+        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        self.theta = np.ascontiguousarray(theta, dtype=np.float64)
+        B = len(self.N)
+        ntheta = self.theta.shape[1] if self.theta.ndim == 2 else 0
+        self.cfg = BatchConfig(B=B, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
+                               seed=seed, record=1 if record else 0,
+                               use_graph=1 if use_graph else 0, block=block)
+        h = ctypes.c_void_p()
+        check(lib().bihip_batch_create(
+            ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
+            self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            self.theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ntheta, ctypes.byref(h)))
+        self.h = h
+        self.B, self.T, self.nobs = B, T, nobs(model)
+
+    def close(self):
+        if self.h:
+            lib().bihip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.T, self.nobs)
+        check(lib().bihip_batch_set_observations(self.h, y.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def init(self):
+        check(lib().bihip_batch_init(self.h))
+
+    def step(self, n=1):
+        check(lib().bihip_batch_step(self.h, n))
+
+    def outputs(self):
+        D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        ll = np.zeros(self.B)
+        li = np.zeros((self.B, self.T))
+        ess = np.zeros((self.B, self.T))
+        res = np.zeros((self.B, self.T), np.int32)
+        st = np.zeros(self.B, np.int32)
+        check(lib().bihip_batch_outputs(self.h, ll.ctypes.data_as(D), li.ctypes.data_as(D),
+                                        ess.ctypes.data_as(D), res.ctypes.data_as(I),
+                                        st.ctypes.data_as(I)))
+        return dict(logLikelihood=ll, logIncrements=li, ess=ess, resampled=res, status=st)
+
+    def ancestors(self, j, k):
+        a = np.zeros(int(self.N[j]), np.int32)
+        check(lib().bihip_batch_ancestors(self.h, j, k, a.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+        return a
+
+    def tuning(self):
+        blk, grid = ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(3)
+        check(lib().bihip_batch_tuning(self.h, ctypes.byref(blk), ctypes.byref(grid),
+                                       ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return blk.value, grid.value, ms
+
+    def set_timing(self, on):
+        check(lib().bihip_batch_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        check(lib().bihip_batch_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def particles(self):
+        return lib().bihip_batch_particles(self.h)

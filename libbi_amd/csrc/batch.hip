@@ -1,0 +1,448 @@
+// Batched inner filters: B bootstrap particle filters with uneven particle
+// counts N_j and their own parameters theta_j, advanced one observation
+// interval per persistent fused launch.
+//
+// This is the inner loop of MarginalSIR::step (share/src/bi/sampler/MarginalSIR.hpp:346-368),
+// which the reference runs serially over theta-particles with equal N
+// (share/src/bi/state/MarginalSIRState.hpp:198-207).  Here every workgroup
+// takes whole filters from a work queue (largest first) and runs the full
+// BootstrapPF step for each: the resampling decision, the exact fixed-point
+// weight scan, systematic/stratified offspring, the permuted ancestors (a
+// search table in LDS), the gathered transition, the observation density and
+// the ESS / log-likelihood reduction.  Filter j draws its Philox streams
+// with batch = j, so each inner filter equals the single-filter path (and the
+// oracle's filter) run with that batch id.
+#include "model_ops.hpp"
+#include "scan.hpp"
+
+namespace bihip {
+
+// Largest inner filter: the permutation's search table E (values below N)
+// is kept in LDS as 16-bit integers.
+constexpr int BATCH_NMAX = 16384;
+
+struct BatchArgs {
+  ModelArgs m;
+  const double* theta;  // B x ntheta: par[0..ntheta) of filter j
+  int ntheta;
+  const int* order;     // processing order (largest filter first)
+  int B;
+  BatchFilter* f;
+  const void* Xin;
+  void* Xout;
+  long ld;              // column stride of the flattened state
+  double* lws;
+  double* Ws;           // scratch: cumulative weights
+  int* as;              // scratch: ancestors
+  int* anc_rec;         // nullable: ancestors of step k at anc_rec[k * ld + off + i]
+  double* hist_li;      // B x T
+  double* hist_ess;     // B x T
+  int* hist_res;        // B x T
+  const double* y;      // T x nobs
+  int nobs, T;
+  double t0, dt;
+  uint64_t seed;
+  double essRel;
+  int resampler;
+  uint32_t* counter;    // work queue
+};
+
+template <class M>
+__device__ __forceinline__ M make_model(const BatchArgs& a, int j) {
+  ModelArgs ma = a.m;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)  // constant indices keep par[] in registers
+    if (i < a.ntheta) ma.par[i] = a.theta[(size_t)j * a.ntheta + i];
+  return M(ma);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ u128 block_incl_scan_u128(u128 v, u128* lds, u128& total) {
+  constexpr int NW = BLOCK / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const u128 u = shfl_up_v(v, o);
+    if (lane >= o) v = u + v;
+  }
+  if (lane == WAVE - 1) lds[w] = v;
+  __syncthreads();
+  u128 pre(0), tot(0);
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    if (i < w) pre = pre + lds[i];
+    tot = tot + lds[i];
+  }
+  __syncthreads();
+  total = tot;
+  return pre + v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* lds,
+                                                        uint64_t& total) {
+  constexpr int NW = BLOCK / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  uint64_t incl = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint64_t u = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += u;
+  }
+  if (lane == WAVE - 1) lds[w] = incl;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    if (i < w) pre += lds[i];
+    tot += lds[i];
+  }
+  __syncthreads();
+  total = tot;
+  return pre + incl - v;
+}
+
+__device__ __forceinline__ double fix_to_double(const u128& s) {
+  return (double)s.hi * 4.0 + (double)s.lo * 0x1.0p-62;
+}
+
+__device__ __forceinline__ int sys_offspring(double Ws, double W, int n, double a) {
+#pragma clang fp contract(off)
+  const double v = Ws / W * (double)n + a;
+  const int o = (int)v;
+  return o < n ? o : n;
+}
+
+__device__ __forceinline__ int strat_offspring(double Ws, double W, int n, uint64_t seed,
+                                               uint32_t step, uint32_t batch) {
+#pragma clang fp contract(off)
+  const double reach = Ws / W * (double)n;
+  int k = (int)reach;
+  if (k > n - 1) k = n - 1;
+  const u32x4 r = philox_block(seed, (uint32_t)k, step, TAG_STRATIFIED, batch, 0);
+  const double alpha = u53(r.x, r.y);
+  const int o = (int)(reach + alpha);
+  return o < n ? o : n;
+}
+
+template <class M, int INTEG, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
+  using T = typename M::real_t;
+  constexpr int DL = M::DL;
+  static_assert(M::LPP == 1, "batched filters use one lane per particle");
+  __shared__ uint16_t E[BATCH_NMAX];
+  __shared__ u128 l128[BLOCK / WAVE];
+  __shared__ uint64_t l64[BLOCK / WAVE];
+  __shared__ double ld[BLOCK / WAVE];
+  __shared__ int s_job;
+  __shared__ int s_res;
+  const int tid = threadIdx.x;
+  const Group<1> grp;
+
+  for (;;) {
+    if (tid == 0)
+      s_job = (int)__hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int job = s_job;
+    if (job >= a.B) break;
+    const int j = a.order[job];
+    const BatchFilter F = a.f[j];
+    const int N = F.N, k = F.k;
+    const long off = F.off;
+    const uint32_t batch = (uint32_t)j;
+    if (tid == 0) s_res = F.resample != 0 && F.status == BIHIP_OK;
+    __syncthreads();
+    bool res = s_res != 0;
+
+    // ---- Resampler::resample: exact weight scan, offspring, permuted ancestors
+    if (res) {
+      u128 carry(0);
+      for (int q = 0; q < N; q += BLOCK) {
+        const int i = q + tid;
+        u128 v(0);
+        if (i < N) {
+          const double w = nanexp(a.lws[off + i] - F.mx);
+          v = u128((uint64_t)(w * 0x1.0p62));
+        }
+        u128 rowtot;
+        const u128 incl = block_incl_scan_u128<BLOCK>(v, l128, rowtot);
+        if (i < N) a.Ws[off + i] = fix_to_double(carry + incl);
+        carry = carry + rowtot;
+      }
+      const double W = fix_to_double(carry);
+      if (!(W > 0.0)) {
+        if (tid == 0) a.f[j].status = BIHIP_DEGENERATE;
+        res = false;
+      } else {
+        __syncthreads();  // Ws of the whole filter written
+        double av = 0.0;
+        if (a.resampler == BIHIP_SYSTEMATIC) {
+          const u32x4 r = philox_block(a.seed, 0, (uint32_t)k, TAG_SYSTEMATIC, batch, 0);
+          av = u53(r.x, r.y);
+        }
+        uint64_t efcarry = 0;
+        for (int q = 0; q < N; q += BLOCK) {
+          const int i = q + tid;
+          int o = 0;
+          if (i < N) {
+            int O1, O0 = 0;
+            if (a.resampler == BIHIP_SYSTEMATIC) {
+              O1 = sys_offspring(a.Ws[off + i], W, N, av);
+              if (i > 0) O0 = sys_offspring(a.Ws[off + i - 1], W, N, av);
+            } else {
+              O1 = strat_offspring(a.Ws[off + i], W, N, a.seed, (uint32_t)k, batch);
+              if (i > 0) O0 = strat_offspring(a.Ws[off + i - 1], W, N, a.seed, (uint32_t)k, batch);
+            }
+            o = O1 - O0;
+          }
+          const uint64_t e = o > 1 ? (uint64_t)(o - 1) : 0u;
+          const uint64_t fz = (i < N && o == 0) ? 1u : 0u;
+          uint64_t rowtot;
+          const uint64_t ex = efcarry + block_excl_scan_u64<BLOCK>(e | (fz << 32), l64, rowtot);
+          if (i < N) {
+            E[i] = (uint16_t)(uint32_t)ex;
+            a.as[off + i] = o > 0 ? i : -1 - (int)(uint32_t)(ex >> 32);
+          }
+          efcarry += rowtot;
+        }
+        __syncthreads();  // E complete
+        for (int q = 0; q < N; q += BLOCK) {
+          const int i = q + tid;
+          if (i >= N) continue;
+          const int v = a.as[off + i];
+          if (v >= 0) continue;
+          const uint32_t r = (uint32_t)(-1 - v);
+          int lo = 0, hi = N;  // first i' with E[i'] > r
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)E[mid] <= r) lo = mid + 1;
+            else hi = mid;
+          }
+          a.as[off + i] = lo > 0 ? lo - 1 : 0;
+        }
+      }
+    }
+
+    // ---- predict + correct (ancestor gather on read), weight partials
+    M m = make_model<M>(a, j);
+    const double t1 = a.t0 + k * a.dt, t2 = a.t0 + (k + 1) * a.dt;
+    const uint32_t step = (uint32_t)(k + 1);
+    const double* y = a.y + (size_t)k * a.nobs;
+    const T* Xin = static_cast<const T*>(a.Xin);
+    T* Xout = static_cast<T*>(a.Xout);
+    double tm = -__builtin_inf(), ts1 = 0.0, ts2 = 0.0;  // running (max, sum w, sum w^2)
+    for (int q = 0; q < N; q += BLOCK) {
+      const int i = q + tid;
+      if (i >= N) continue;
+      const int src = res ? a.as[off + i] : i;
+      if (a.anc_rec != nullptr) a.anc_rec[(size_t)k * a.ld + off + i] = src;
+      T x[DL];
+#pragma unroll
+      for (int c = 0; c < DL; ++c) x[c] = Xin[(size_t)c * a.ld + off + src];
+      M mm = m;
+      if constexpr (M::NOISE_FIRST) noise_first(mm, a.seed, batch, i, step);
+      if constexpr (INTEG == BIHIP_RK4) {
+        integrate_rk4(mm, grp, x, t1, t2, a.m.h0);
+      } else {
+        integrate_dopri5(mm, grp, x, t1, t2, a.m);
+      }
+      if constexpr (!M::NOISE_FIRST) noise_after(mm, a.seed, batch, i, 0, step, x);
+#pragma unroll
+      for (int c = 0; c < DL; ++c) Xout[(size_t)c * a.ld + off + i] = x[c];
+      const double lw = (res ? F.logLikelihood : a.lws[off + i]) + obs_of(mm, x, y, 0);
+      a.lws[off + i] = lw;
+      if (!isnan(lw)) {
+        if (lw > tm) {
+          const double f = nanexp(tm - lw);
+          ts1 = ts1 * f + 1.0;
+          ts2 = ts2 * (f * f) + 1.0;
+          tm = lw;
+        } else {
+          const double z = nanexp(lw - tm);
+          ts1 += z;
+          ts2 += z * z;
+        }
+      }
+    }
+    // block combination in a fixed order
+    const double Mx = block_max<BLOCK>(ts1 > 0.0 ? tm : __builtin_nan(""), ld);
+    double c1 = 0.0, c2 = 0.0;
+    if (ts1 > 0.0) {
+      const double f = exp(tm - Mx);
+      c1 = ts1 * f;
+      c2 = ts2 * (f * f);
+    }
+    const double S1 = block_sum<BLOCK>(c1, ld);
+    const double S2 = block_sum<BLOCK>(c2, ld);
+    if (tid == 0) {
+      const double ess = S1 * S1 / S2;
+      const double lW = Mx + log(S1) - log((double)N);
+      const size_t h = (size_t)j * a.T + k;
+      a.hist_li[h] = lW - F.logLikelihood;
+      a.hist_ess[h] = ess;
+      a.hist_res[h] = res ? 1 : 0;
+      BatchFilter G = F;
+      G.status = a.f[j].status;
+      G.logLikelihood = lW;
+      G.ess = ess;
+      G.mx = Mx;
+      G.lW = lW;
+      G.resample = ess < a.essRel * (double)N ? 1 : 0;
+      G.k = k + 1;
+      a.f[j] = G;
+    }
+    __syncthreads();  // LDS and s_job reused by the next filter
+  }
+}
+
+// Initial particles of every filter (filter of a particle by a search over offsets).
+template <class M>
+__global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, const long* off, int B,
+                             long total, uint64_t seed, void* Xv, long ld) {
+  using T = typename M::real_t;
+  constexpr int DL = M::DL;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = B;  // last filter with off <= g
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const int j = lo;
+  const int p = (int)(g - off[j]);
+  T z[DL];
+  slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, (uint32_t)j, 0, z);
+  T* X = static_cast<T*>(Xv);
+  for (int c = 0; c < DL; ++c) {
+    double par[16];
+    for (int i = 0; i < 16; ++i) par[i] = i < ntheta ? theta[(size_t)j * ntheta + i] : ma.par[i];
+    double v;
+    if (ma.model == BIHIP_MODEL_PZ) {
+      v = c == 0 ? exp(par[7] + par[8] * (double)z[0]) : exp(par[9] + par[10] * (double)z[1]);
+    } else {
+      v = par[3] + par[4] * (double)z[c];
+    }
+    X[(size_t)c * ld + g] = (T)v;
+  }
+}
+
+__global__ void k_batch_reset(uint32_t* counter) { *counter = 0u; }
+
+// ---------------------------------------------------------------- dispatch
+template <int BLOCK>
+struct BatchRunner {
+  BatchArgs a;
+  int grid, integ;
+  hipStream_t s;
+  template <class M>
+  int run() {
+    if (integ == BIHIP_RK4) {
+      hipLaunchKernelGGL((k_batch_step<M, BIHIP_RK4, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_batch_step<M, BIHIP_DOPRI5, BLOCK>), dim3(grid), dim3(BLOCK), 0, s,
+                         a);
+    }
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+  template <class M>
+  int occupancy() {
+    int n = 0;
+    if (integ == BIHIP_RK4) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &n, reinterpret_cast<const void*>(&k_batch_step<M, BIHIP_RK4, BLOCK>), BLOCK, 0) !=
+          hipSuccess)
+        n = 1;
+    } else {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &n, reinterpret_cast<const void*>(&k_batch_step<M, BIHIP_DOPRI5, BLOCK>), BLOCK,
+              0) != hipSuccess)
+        n = 1;
+    }
+    return n;
+  }
+};
+
+template <class F>
+static int with_small_model(const bihip_model* m, F& f) {
+  const bool f64 = m->precision == BIHIP_F64;
+  if (m->model == BIHIP_MODEL_PZ)
+    return f64 ? f.template run<PZ<double>>() : f.template run<PZ<float>>();
+  return f64 ? f.template run<LG<double>>() : f.template run<LG<float>>();
+}
+
+template <class F>
+static int occupancy_of(const bihip_model* m, F& f) {
+  const bool f64 = m->precision == BIHIP_F64;
+  if (m->model == BIHIP_MODEL_PZ)
+    return f64 ? f.template occupancy<PZ<double>>() : f.template occupancy<PZ<float>>();
+  return f64 ? f.template occupancy<LG<double>>() : f.template occupancy<LG<float>>();
+}
+
+bool batch_model_supported(const bihip_model* m) {
+  return model_supported(m) && (m->model == BIHIP_MODEL_PZ || m->model == BIHIP_MODEL_LG);
+}
+
+int batch_nmax() { return BATCH_NMAX; }
+
+template <int BLOCK>
+static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
+  BatchRunner<BLOCK> r;
+  r.a = BatchArgs{to_args(L.model), L.theta, L.ntheta, L.order, L.B, L.f, L.Xin, L.Xout, L.ld,
+                  L.lws, L.Ws, L.as, L.anc_rec, L.hist_li, L.hist_ess, L.hist_res, L.y, L.nobs,
+                  L.T, L.t0, L.dt, L.seed, L.essRel, L.resampler, L.counter};
+  r.integ = L.model->integrator;
+  r.s = s;
+  int cus = 0, dev = 0;
+  BIHIP_HIP(hipGetDevice(&dev));
+  BIHIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int per_cu = std::max(1, occupancy_of(L.model, r));
+  r.grid = std::max(1, std::min(L.B, cus * per_cu));
+  if (grid_out) *grid_out = r.grid;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, L.counter);
+  BIHIP_LAUNCH_CHECK();
+  return with_small_model(L.model, r);
+}
+
+int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out) {
+  switch (block) {
+    case 256: return launch_block<256>(L, s, grid_out);
+    case 512: return launch_block<512>(L, s, grid_out);
+    case 1024: return launch_block<1024>(L, s, grid_out);
+  }
+  return set_error(BIHIP_EINVAL, "batch block size must be 256, 512 or 1024");
+}
+
+struct BatchInitRunner {
+  ModelArgs ma;
+  const double* theta;
+  int ntheta;
+  const long* off;
+  int B;
+  long total;
+  uint64_t seed;
+  void* X;
+  long ld;
+  hipStream_t s;
+  template <class M>
+  int run() {
+    hipLaunchKernelGGL((k_batch_init<M>), dim3(ceil_div(total, 256)), dim3(256), 0, s, ma, theta,
+                       ntheta, off, B, total, seed, X, ld);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+  template <class M>
+  int occupancy() {
+    return 1;
+  }
+};
+
+int launch_batch_init(const bihip_model* m, const double* theta, int ntheta, const long* off,
+                      int B, long total, uint64_t seed, void* X, long ld, hipStream_t s) {
+  if (total <= 0) return BIHIP_OK;
+  BatchInitRunner r{to_args(m), theta, ntheta, off, B, total, seed, X, ld, s};
+  return with_small_model(m, r);
+}
+
+}  // namespace bihip

@@ -23,17 +23,6 @@ int set_error(int code, const char* fmt, ...) {
   return code;
 }
 
-template <class T>
-static int dalloc(T** p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  BIHIP_HIP(hipMalloc((void**)p, n * sizeof(T)));
-  return BIHIP_OK;
-}
-
-static void dfree(void* p) {
-  if (p) (void)hipFree(p);
-}
 
 static int ws_alloc(ScanWorkspace* ws, int tiles) {
   ws->max_tiles = tiles;
@@ -119,6 +108,11 @@ static int ctx_enter(bihip_ctx* ctx) {
   BIHIP_HIP(hipSetDevice(ctx->device));
   return BIHIP_OK;
 }
+
+namespace bihip {
+int ctx_enter_device(bihip_ctx* ctx) { return ctx_enter(ctx); }
+hipStream_t ctx_stream_of(bihip_ctx* ctx) { return ctx->stream; }
+}  // namespace bihip
 
 static int ctx_partials(bihip_ctx* ctx, int nb) {
   if (nb > ctx->partials_cap) {

@@ -43,6 +43,13 @@ struct PFDevice {
   int pad;
 };
 
+// Scalars of one inner filter of a batch.
+struct BatchFilter {
+  double logLikelihood, ess, mx, lW;
+  int resample, status, k, N;
+  long off;  // first particle in the flattened state
+};
+
 // Per-step histories on the device (T entries each; ancestors T x P when recorded).
 struct PFHistory {
   double* logIncrements;
@@ -181,5 +188,54 @@ int launch_gather_rows(const int* as, int P, int ncols, void* X, int ld, int ele
                        hipStream_t s);
 int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
                             hipStream_t s);
+
+// ---- batch.hip
+struct BatchLaunch {
+  const bihip_model* model;
+  const double* theta;
+  int ntheta;
+  const int* order;
+  int B;
+  BatchFilter* f;
+  const void* Xin;
+  void* Xout;
+  long ld;
+  double* lws;
+  double* Ws;
+  int* as;
+  int* anc_rec;
+  double* hist_li;
+  double* hist_ess;
+  int* hist_res;
+  const double* y;
+  int nobs, T;
+  double t0, dt;
+  uint64_t seed;
+  double essRel;
+  int resampler;
+  uint32_t* counter;
+};
+bool batch_model_supported(const bihip_model* m);
+int batch_nmax();
+int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out);
+int launch_batch_init(const bihip_model* m, const double* theta, int ntheta, const long* off,
+                      int B, long total, uint64_t seed, void* X, long ld, hipStream_t s);
+
+// ---- context access for the other host files (capi.cpp)
+int ctx_enter_device(bihip_ctx* ctx);     // makes the context's device current
+hipStream_t ctx_stream_of(bihip_ctx* ctx);
+
+// ---- device memory helpers
+template <class T>
+inline int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  BIHIP_HIP(hipMalloc((void**)p, n * sizeof(T)));
+  return BIHIP_OK;
+}
+
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
 
 }  // namespace bihip

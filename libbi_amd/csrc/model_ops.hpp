@@ -1,0 +1,114 @@
+// Model wrappers shared by the single-filter and the batched kernels:
+// observation log-density terms, transition noise, and the model types the
+// kernels are instantiated on.
+#pragma once
+
+#include "models.hpp"
+
+namespace bihip {
+
+inline ModelArgs to_args(const bihip_model* m) {
+  ModelArgs a;
+  a.model = m->model;
+  a.d = m->d;
+  a.integrator = m->integrator;
+  a.nsteps = m->nsteps;
+  a.host_quirk = m->host_quirk;
+  a.precision = m->precision;
+  a.h0 = m->h0;
+  a.atol = m->atol;
+  a.rtol = m->rtol;
+  for (int i = 0; i < 16; ++i) a.par[i] = m->par[i];
+  return a;
+}
+
+// ---------------------------------------------------------------- observation terms
+// Gaussian / log-normal log-density terms of the lane's slice
+// (share/tt/cpp/action/gaussian.hpp.tt:48-70), in double.
+template <class T>
+__device__ __forceinline__ double obs_terms(const ModelPZ<T>& m, const T (&x)[2],
+                                            const double* y, int) {
+  const double yy = y[0];
+  const double mu = log((double)x[0]);
+  const double z = (log(yy) - mu) / m.obs_sd;
+  return -0.5 * (z * z) - HALF_LOG_TWO_PI - log(m.obs_sd * yy);
+}
+
+template <class T, int D, int LPP>
+__device__ __forceinline__ double obs_terms(const ModelL96<T, D, LPP>& m,
+                                            const T (&x)[ModelL96<T, D, LPP>::DL],
+                                            const double* y, int lo) {
+  constexpr int DL = ModelL96<T, D, LPP>::DL;
+  const double inv = 1.0 / m.obs_sd;
+  const double c = -HALF_LOG_TWO_PI - log(m.obs_sd);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < DL; i += 2) {
+    const double z = (y[(lo + i) >> 1] - (double)x[i]) * inv;
+    s += -0.5 * (z * z) + c;
+  }
+  return s;
+}
+
+template <class T>
+__device__ __forceinline__ double obs_terms(const ModelLG<T>& m, const T (&x)[1],
+                                            const double* y, int) {
+  const double z = (y[0] - (double)x[0]) / m.obs_sd;
+  return -0.5 * (z * z) - HALF_LOG_TWO_PI - log(m.obs_sd);
+}
+
+// ---------------------------------------------------------------- noise
+template <class T>
+__device__ __forceinline__ void noise_first(ModelPZ<T>& m, uint64_t seed, uint32_t batch, int p,
+                                            uint32_t step) {
+  T z[4];
+  normals4<T>(seed, (uint32_t)p, step, TAG_NOISE, batch, 0, z);
+  m.alpha = (T)m.mu + (T)m.sigma * z[0];
+}
+
+template <class M>
+__device__ __forceinline__ void noise_after(const M& m, uint64_t seed, uint32_t batch, int p,
+                                            int lo, uint32_t step,
+                                            typename M::real_t (&x)[M::DL]) {
+  using T = typename M::real_t;
+  T z[M::DL];
+  slice_normals<T, M::DL>(seed, (uint32_t)p, step, TAG_NOISE, batch, lo, z);
+  const T sd = (T)m.noise_sd;
+#pragma unroll
+  for (int i = 0; i < M::DL; ++i) x[i] = x[i] + sd * z[i];
+}
+
+template <class T>
+struct PZ : ModelPZ<T> {
+  using real_t = T;
+  using ModelPZ<T>::ModelPZ;
+};
+template <class T, int D, int LPP>
+struct L96 : ModelL96<T, D, LPP> {
+  using real_t = T;
+  using ModelL96<T, D, LPP>::ModelL96;
+};
+template <class T>
+struct LG : ModelLG<T> {
+  using real_t = T;
+  using ModelLG<T>::ModelLG;
+};
+
+template <class T>
+__device__ __forceinline__ double obs_of(const PZ<T>& m, const T (&x)[2], const double* y,
+                                         int lo) {
+  return obs_terms(static_cast<const ModelPZ<T>&>(m), x, y, lo);
+}
+template <class T, int D, int LPP>
+__device__ __forceinline__ double obs_of(const L96<T, D, LPP>& m,
+                                         const T (&x)[ModelL96<T, D, LPP>::DL],
+                                         const double* y, int lo) {
+  return obs_terms(static_cast<const ModelL96<T, D, LPP>&>(m), x, y, lo);
+}
+template <class T>
+__device__ __forceinline__ double obs_of(const LG<T>& m, const T (&x)[1], const double* y,
+                                         int lo) {
+  return obs_terms(static_cast<const ModelLG<T>&>(m), x, y, lo);
+}
+
+}  // namespace bihip

@@ -153,12 +153,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_place(const int* in, int cumulativ
 int launch_ancestors_permute(const int* in, bool cumulative, int P, int* as, uint64_t* EF,
                              ScanWorkspace ws, const PFDevice* gate, hipStream_t s) {
   if (P <= 0) return BIHIP_OK;
-  const int nt = scan_tiles(P);
-  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
-  hipLaunchKernelGGL((k_tile_scan<uint64_t, PermLoader, StoreExclusive64, PFGate>), dim3(nt),
-                     dim3(SCAN_BLOCK), 0, s, PermLoader{in, cumulative ? 1 : 0},
-                     StoreExclusive64{EF}, PFGate{gate}, P, ws);
-  BIHIP_LAUNCH_CHECK();
+  int rc = launch_tile_scan<uint64_t>(PermLoader{in, cumulative ? 1 : 0}, StoreExclusive64{EF},
+                                     PFGate{gate}, P, ws, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_place, dim3(ceil_div(P, RS_BLOCK)), dim3(RS_BLOCK), 0, s, in,
                      cumulative ? 1 : 0, P, EF, as, gate);
   BIHIP_LAUNCH_CHECK();

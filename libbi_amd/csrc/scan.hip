@@ -7,6 +7,18 @@ namespace bihip {
 
 int scan_tiles(int P) { return ceil_div(P, SCAN_TILE); }
 
+int scan_resident_tiles() {
+  static int limit = -1;
+  if (limit < 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 0;
+    limit = 2 * cus;
+  }
+  return limit;
+}
+
 // Weights w = nanexp(lw - max) lie in [0, 1]; they are summed exactly as
 // 128-bit integers with 62 fractional bits (w * 2^62 is exact in double and
 // truncates to an integer below 2^62) and each prefix is rounded to double
@@ -36,13 +48,8 @@ struct StoreInclusive {
 
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,
                        double* Ws, ScanWorkspace ws, hipStream_t s) {
-  const int nt = scan_tiles(P);
-  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
-  hipLaunchKernelGGL((k_tile_scan<u128, SumExpLoader, StoreInclusive, PFGate>), dim3(nt),
-                     dim3(SCAN_BLOCK), 0, s, SumExpLoader{lws, mx_dev}, StoreInclusive{Ws},
-                     PFGate{gate}, P, ws);
-  BIHIP_LAUNCH_CHECK();
-  return BIHIP_OK;
+  return launch_tile_scan<u128>(SumExpLoader{lws, mx_dev}, StoreInclusive{Ws}, PFGate{gate}, P,
+                                ws, s);
 }
 
 struct CountLoader {
@@ -56,13 +63,8 @@ struct StoreCountInclusive {
 };
 
 int launch_count_scan(const int* os, int P, int* Os, ScanWorkspace ws, hipStream_t s) {
-  const int nt = scan_tiles(P);
-  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
-  hipLaunchKernelGGL((k_tile_scan<uint64_t, CountLoader, StoreCountInclusive, AlwaysGate>),
-                     dim3(nt), dim3(SCAN_BLOCK), 0, s, CountLoader{os}, StoreCountInclusive{Os},
-                     AlwaysGate{}, P, ws);
-  BIHIP_LAUNCH_CHECK();
-  return BIHIP_OK;
+  return launch_tile_scan<uint64_t>(CountLoader{os}, StoreCountInclusive{Os}, AlwaysGate{}, P, ws,
+                                    s);
 }
 
 }  // namespace bihip

@@ -143,42 +143,45 @@ __device__ __forceinline__ V block_sum_v(V v, V* lds) {
 // Loader: V operator()(long i) const, valid for i < P.
 // Writer: void operator()(long i, V inclusive, V exclusive) const.
 // Gate:   bool operator()() const (false: the launch does nothing).
-template <class V, class Loader, class Writer, class Gate>
+// ITEMS consecutive elements per thread; TICKET: tiles are numbered by an
+// atomic ticket (any grid size) or by blockIdx.x (grids that fit on the
+// device at once, so every awaited tile is running).
+template <class V, int ITEMS, bool TICKET, class Loader, class Writer, class Gate>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer write, Gate gate,
                                                            int P, ScanWorkspace ws) {
   constexpr int NW = SCAN_BLOCK / WAVE;
   constexpr int NC = Chunks<V>::N;
+  constexpr int TILE = SCAN_BLOCK * ITEMS;
   __shared__ uint32_t s_tile, s_epoch;
   __shared__ V lds[NW];
   if (!gate()) return;
 
   if (threadIdx.x == 0) {
-    // the epoch is read before the ticket; the last tile advances it only
-    // after every other tile has published (and so has read it)
-    const uint32_t e =
+    // the epoch is read before the tile number; the last tile advances it
+    // only after every other tile has published (and so has read it)
+    s_epoch =
         __hip_atomic_load(&ws.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-    const uint32_t t =
-        __hip_atomic_fetch_add(&ws.state->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_tile = t;
-    s_epoch = e;
+    s_tile = TICKET ? __hip_atomic_fetch_add(&ws.state->ticket, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                    : blockIdx.x;
   }
   __syncthreads();
   const uint32_t tile = s_tile, epoch = s_epoch;
-  const int ntiles = ceil_div(P, SCAN_TILE);
+  const int ntiles = ceil_div(P, TILE);
   if (tile >= (uint32_t)ntiles) {  // a stale ticket: report, touch nothing
     if (threadIdx.x == 0)
       __hip_atomic_store(&ws.state->error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
 
-  // thread-local inclusive scan of SCAN_ITEMS consecutive elements
-  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
-  V v[SCAN_ITEMS];
+  // thread-local inclusive scan of ITEMS consecutive elements
+  const long base = (long)tile * TILE + (long)threadIdx.x * ITEMS;
+  V v[ITEMS];
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) v[i] = (base + i < P) ? load(base + i) : V(0);
+  for (int i = 0; i < ITEMS; ++i) v[i] = (base + i < P) ? load(base + i) : V(0);
 #pragma unroll
-  for (int i = 1; i < SCAN_ITEMS; ++i) v[i] = v[i - 1] + v[i];
-  const V total = v[SCAN_ITEMS - 1];
+  for (int i = 1; i < ITEMS; ++i) v[i] = v[i - 1] + v[i];
+  const V total = v[ITEMS - 1];
 
   // block exclusive scan of thread totals: wave scan, then wave prefixes
   const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
@@ -202,25 +205,79 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
 
   if (threadIdx.x == 0) publish(ws.words + (size_t)NC * tile, epoch, agg);
 
-  // prefix = sum of the aggregates of tiles [0, tile), fixed order
+  // prefix = sum of the aggregates of tiles [0, tile) in a fixed order:
+  // thread t adds tiles t, t+256, t+512, ...; the words of up to four tiles
+  // are loaded at once, and only words not yet published are polled again
   V part = V(0);
-  for (uint32_t j = threadIdx.x; j < tile; j += SCAN_BLOCK)
-    part = part + await_value<V>(ws.words + (size_t)NC * j, epoch, &ws.state->error);
+  for (uint32_t j0 = threadIdx.x; j0 < tile; j0 += 4 * SCAN_BLOCK) {
+    uint64_t wd[4][NC];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t j = j0 + q * SCAN_BLOCK;
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        wd[q][c] = j < tile ? __hip_atomic_load(ws.words + (size_t)NC * j + c, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : ((uint64_t)epoch << 32);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t j = j0 + q * SCAN_BLOCK;
+      if (j >= tile) break;
+      uint32_t c32[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        uint64_t x = wd[q][c];
+        uint32_t spins = 0;
+        while ((x >> 32) != epoch) {
+          if (++spins > SCAN_SPIN_LIMIT) {
+            __hip_atomic_store(&ws.state->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          x = __hip_atomic_load(ws.words + (size_t)NC * j + c, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+        }
+        c32[c] = (uint32_t)x;
+      }
+      part = part + Chunks<V>::join(c32);
+    }
+  }
   const V prefix = block_sum_v(part, lds);
 
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     if (base + i < P) {
       const V before = (i == 0) ? V(0) : v[i - 1];
       write(base + i, prefix + (texcl + v[i]), prefix + (texcl + before));
     }
   }
 
-  // the last ticket rearms the workspace for the next launch
+  // the last tile rearms the workspace for the next launch
   if (threadIdx.x == 0 && tile == (uint32_t)(ntiles - 1)) {
-    __hip_atomic_store(&ws.state->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (TICKET) __hip_atomic_store(&ws.state->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ws.state->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// Tiles that are certainly resident together (two per CU); grids up to this
+// size number their tiles by blockIdx.x, larger ones take tickets.
+int scan_resident_tiles();
+
+template <class V, class Loader, class Writer, class Gate>
+int launch_tile_scan(Loader load, Writer write, Gate gate, int P, ScanWorkspace ws,
+                     hipStream_t s) {
+  const int nt = ceil_div(P, SCAN_TILE);
+  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
+  if (nt <= scan_resident_tiles()) {
+    hipLaunchKernelGGL((k_tile_scan<V, SCAN_ITEMS, false, Loader, Writer, Gate>), dim3(nt),
+                       dim3(SCAN_BLOCK), 0, s, load, write, gate, P, ws);
+  } else {
+    hipLaunchKernelGGL((k_tile_scan<V, SCAN_ITEMS, true, Loader, Writer, Gate>), dim3(nt),
+                       dim3(SCAN_BLOCK), 0, s, load, write, gate, P, ws);
+  }
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
 }
 
 struct AlwaysGate {

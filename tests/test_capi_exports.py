@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "bihip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|double|const char\*)\s+(bihip_\w+)\s*\(", text,
+    return sorted(set(re.findall(r"^\s*(?:int|long|double|const char\*)\s+(bihip_\w+)\s*\(", text,
                                  re.M)))
 
 
