@@ -20,6 +20,7 @@ MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
 RK4, DOPRI5 = 0, 1
 SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
+TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER = 7, 8, 9, 10
 OK, DEGENERATE = 0, 1
 
 # function -> how its parity is pinned (each entry is exercised in tests/)
@@ -43,6 +44,11 @@ PINNED = {
     "orc_permute": "in-place property and multiset preservation, brute force P<=6",
     "orc_gather_rows": "in-place == out-of-place gather",
     "orc_pf_run": "Kalman filter exact logZ on the LG model (essRel 0, 0.5, 1)",
+    "orc_pf_step": "equals orc_pf_run step by step (orc_pf_run is built on it)",
+    "orc_smc2_run": "no moves and no outer resampling: lw_j equals orc_pf_run with theta_j and "
+                    "batch j; logZ = logsumexp(lw) - log B; LG model: evidence and posterior "
+                    "mean of lambda against Gauss-Hermite quadrature of the Kalman likelihood; "
+                    "acceptance decisions against proposal filters run through orc_pf_run",
 }
 
 
@@ -71,6 +77,24 @@ class PFConfig(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("seed", ctypes.c_uint64),
         ("batch", ctypes.c_uint32),
+    ]
+
+
+class SMC2Config(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int),
+        ("T", ctypes.c_int),
+        ("ntheta", ctypes.c_int),
+        ("nmoves", ctypes.c_int),
+        ("resampler", ctypes.c_int),
+        ("essRel_inner", ctypes.c_double),
+        ("essRel_outer", ctypes.c_double),
+        ("t0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("prior_mean", ctypes.c_double * 8),
+        ("prior_sd", ctypes.c_double * 8),
+        ("prop_sd", ctypes.c_double * 8),
     ]
 
 
@@ -136,6 +160,9 @@ def _declare(L):
     L.orc_permute.argtypes = [_I, c.c_int]
     L.orc_gather_rows.argtypes = [_I, c.c_int, c.c_int, _D, c.c_int]
     L.orc_pf_run.argtypes = [c.POINTER(Model), c.POINTER(PFConfig), _D, _D, _D, _D, _D, _I, _I, _D]
+    L.orc_uniform_at.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, _D]
+    L.orc_smc2_run.argtypes = [c.POINTER(Model), c.POINTER(SMC2Config), _I, _D, _D, _D, _D, _I,
+                               _D, _I]
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
 
@@ -390,6 +417,43 @@ def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=N
                           ctypes.byref(logZ), _dp(li), _dp(ess), _ip(res), _ip(anc), _dp(Xo))
     return dict(rc=rc, logZ=logZ.value, logIncrements=li, ess=ess, resampled=res,
                 ancestors=anc, X=Xo)
+
+
+def uniform_at(seed, index, step, tag, batch=0):
+    u = ctypes.c_double()
+    lib().orc_uniform_at(seed, index, step, tag, batch, ctypes.byref(u))
+    return u.value
+
+
+def smc2_config(B, T, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1, nmoves=1,
+                resampler=SYSTEMATIC, ess_rel_inner=0.5, ess_rel_outer=0.5, t0=0.0):
+    nt = len(prior_mean)
+    assert nt <= 8 and len(prior_sd) == nt and len(prop_sd) == nt
+    c = SMC2Config(B=B, T=T, ntheta=nt, nmoves=nmoves, resampler=resampler,
+                   essRel_inner=ess_rel_inner, essRel_outer=ess_rel_outer, t0=t0, dt=dt,
+                   seed=seed)
+    for i in range(nt):
+        c.prior_mean[i] = prior_mean[i]
+        c.prior_sd[i] = prior_sd[i]
+        c.prop_sd[i] = prop_sd[i]
+    return c
+
+
+def smc2_run(m, cfg, N, y):
+    """SMC^2 (MarginalSIR) over log(theta) with inner filters of N[j] particles."""
+    y = _f64(y).reshape(len(y), -1)
+    assert y.shape[0] == cfg.T and len(N) == cfg.B
+    N = _i32(N)
+    logZ = ctypes.c_double()
+    theta = np.zeros((cfg.B, cfg.ntheta))
+    lw = np.zeros(cfg.B)
+    acc = np.zeros(cfg.T + 1, np.int32)
+    ess = np.zeros(cfg.T)
+    res = np.zeros(cfg.T, np.int32)
+    rc = lib().orc_smc2_run(ctypes.byref(m), ctypes.byref(cfg), _ip(N), _dp(y),
+                            ctypes.byref(logZ), _dp(theta), _dp(lw), _ip(acc), _dp(ess),
+                            _ip(res))
+    return dict(rc=rc, logZ=logZ.value, theta=theta, lw=lw, accepts=acc, ess=ess, resampled=res)
 
 
 def kalman_loglik_lg(m, dt, y):

@@ -35,7 +35,11 @@ enum {
   ORC_TAG_SYSTEMATIC = 3,  /* systematic resampler offset */
   ORC_TAG_STRATIFIED = 4,  /* stratified resampler alphas */
   ORC_TAG_MULTINOMIAL = 5, /* multinomial resampler uniforms */
-  ORC_TAG_OBS = 6          /* observation simulation */
+  ORC_TAG_OBS = 6,         /* observation simulation */
+  ORC_TAG_THETA = 7,       /* SMC^2 prior draws of theta */
+  ORC_TAG_PROPOSAL = 8,    /* SMC^2 random-walk proposals */
+  ORC_TAG_ACCEPT = 9,      /* SMC^2 acceptance uniforms */
+  ORC_TAG_OUTER = 10       /* SMC^2 outer systematic offset */
 };
 
 /* ---- models ---- */
@@ -70,6 +74,9 @@ void orc_philox_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag
 double orc_u53(uint32_t r0, uint32_t r1);
 /* n uniforms in [0,1): u_j from counter {j, step, tag|(batch<<8), 0}, words 0,1 */
 void orc_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, int n, double* u);
+/* one uniform in [0,1) from block {index, step, tag|(batch<<8), 0} */
+void orc_uniform_at(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+    double* u);
 /* n standard normals for one index: normal j from block j/4, word pair (j%4)/2 */
 void orc_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
     int n, int bits, double* z);
@@ -127,6 +134,33 @@ typedef struct {
 int orc_pf_run(const orc_model* m, const orc_pf_cfg* cfg, const double* y,
     const double* mean0, double* logZ, double* logIncrements, double* ess,
     int* resampled, int* ancestors, double* Xout);
+
+/* step-wise filter (the state of one BootstrapPF) */
+typedef struct orc_pf orc_pf;
+orc_pf* orc_pf_new(const orc_model* m, int P, int resampler, double essRel, double t0,
+    double dt, uint64_t seed, uint32_t batch);
+void orc_pf_free(orc_pf* f);
+void orc_pf_init(orc_pf* f, const double* mean0);
+int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled, int* ancestors);
+double orc_pf_loglikelihood(const orc_pf* f);
+double orc_pf_ess(const orc_pf* f);
+int orc_pf_size(const orc_pf* f);
+double orc_pf_term(const orc_pf* f);
+void orc_pf_copy(orc_pf* dst, const orc_pf* src);
+
+/* SMC^2 over theta in log space (MarginalSIR), inner bootstrap filters with
+ * particle counts N[j]; see filter.c for the keyed draws. */
+typedef struct {
+  int B, T, ntheta, nmoves, resampler;
+  double essRel_inner, essRel_outer, t0, dt;
+  uint64_t seed;
+  double prior_mean[8], prior_sd[8], prop_sd[8];
+} orc_smc2_cfg;
+/* outputs (nullable): logZ, theta_out[B x ntheta], lw_out[B], accepts_out[T+1],
+ * ess_out[T], resampled_out[T] */
+int orc_smc2_run(const orc_model* base, const orc_smc2_cfg* c, const int* N, const double* y,
+    double* logZ, double* theta_out, double* lw_out, int* accepts_out, double* ess_out,
+    int* resampled_out);
 
 /* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
 double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);

@@ -57,6 +57,13 @@ void orc_uniforms(uint64_t seed, uint32_t step, uint32_t tag, uint32_t batch, in
   }
 }
 
+void orc_uniform_at(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+    double* u) {
+  uint32_t r[4];
+  orc_philox_block(seed, index, step, tag, batch, 0, r);
+  *u = orc_u53(r[0], r[1]);
+}
+
 /*
  * Box-Muller pair from two 32-bit words.  bits=32: u1 = 1 - r0*2^-32 in
  * (0,1], u2 = r1*2^-32 in [0,1).  bits=24: the top 24 bits of each word, as
