@@ -63,6 +63,10 @@ extern "C" {
 #define BIHIP_TAG_STRATIFIED 4
 #define BIHIP_TAG_MULTINOMIAL 5
 #define BIHIP_TAG_OBS 6
+#define BIHIP_TAG_THETA 7     /* SMC^2 prior draws of theta */
+#define BIHIP_TAG_PROPOSAL 8  /* SMC^2 random-walk proposals */
+#define BIHIP_TAG_ACCEPT 9    /* SMC^2 acceptance uniforms */
+#define BIHIP_TAG_OUTER 10    /* SMC^2 outer systematic offset */
 
 /*
  * Model description: the runtime counterpart of a generated Model<name>
@@ -276,9 +280,11 @@ int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_
  * (share/src/bi/state/MarginalSIRState.hpp:198-207).  One persistent fused
  * launch per observation interval: workgroups take whole filters from a work
  * queue (largest first) and run BootstrapPF::step for each.  Filter j draws
- * its Philox streams with batch = j, so it equals bihip_pf (and the
- * reference filter semantics) run with that batch id.  Models: PZ and LG.
- * theta_host is B x ntheta; column i replaces bihip_model::par[i].
+ * its Philox streams with batch id stream_base + j (fixed at
+ * bihip_batch_init and kept when the filter's content is replaced), so it
+ * equals bihip_pf (and the reference filter semantics) run with that batch
+ * id.  Models: PZ and LG.  theta_host is B x ntheta; column i replaces
+ * bihip_model::par[i].
  */
 typedef struct bihip_batch bihip_batch;
 typedef struct {
@@ -291,6 +297,8 @@ typedef struct {
   int record;     /* 1 = keep every step's ancestors (T x total ints) */
   int use_graph;  /* 1 = replay each step as a captured hipGraph */
   int block;      /* workgroup size 256/512/1024, or 0 = autotune at the first step */
+  long capacity;  /* particles the layout may grow to (0 = the initial total) */
+  uint32_t stream_base; /* Philox batch id of filter 0; ids stay below 2^24 */
 } bihip_batch_config;
 
 int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_config* cfg,
@@ -316,6 +324,63 @@ int bihip_batch_set_timing(bihip_batch* b, int enable);
 int bihip_batch_kernel_time(bihip_batch* b, double* ms_host, long* count_host);
 /* Total particles over all inner filters. */
 long bihip_batch_particles(bihip_batch* b);
+/* Replace theta (B x ntheta) for the following steps or initialisation. */
+int bihip_batch_set_theta(bihip_batch* b, const double* theta_host);
+/* New particle counts (total within the capacity); bihip_batch_init must follow. */
+int bihip_batch_set_sizes(bihip_batch* b, const int* N_host);
+/* Philox batch id of filter 0 from the next bihip_batch_init on. */
+int bihip_batch_set_stream_base(bihip_batch* b, uint32_t base);
+/* logIncrements of step k of every filter (B doubles). */
+int bihip_batch_increments(bihip_batch* b, int k, double* li_host);
+/*
+ * Resampling of whole filters, MarginalSIR's interact with the state gather
+ * of MarginalSIRState (share/src/bi/sampler/MarginalSIR.hpp:370-393,
+ * share/src/bi/state/MarginalSIRState.hpp): filter j takes the particles,
+ * log-weights, log-likelihood, size and theta of filter anc_host[j] and keeps
+ * its own random streams.  Recorded ancestors of earlier steps are dropped.
+ */
+int bihip_batch_resample_filters(bihip_batch* b, const int* anc_host);
+/*
+ * Accepted Metropolis-Hastings proposals (MarginalSIR.hpp:446-466, s1.swap(s2)):
+ * filter j of dst takes the whole state of filter j of src where
+ * accept_host[j] != 0.  Both batches must have equal sizes, position, model
+ * and ntheta; dst's filters keep their own random streams.
+ */
+int bihip_batch_accept(bihip_batch* dst, bihip_batch* src, const int* accept_host);
+
+/* ================================================================ SMC^2 */
+/*
+ * SMC^2 over theta (MarginalSIR::sample, share/src/bi/sampler/MarginalSIR.hpp:281-320)
+ * with B theta-particles whose inner bootstrap filters (N_host[j] particles)
+ * run as one bihip_batch.  theta_i = exp(lth_i) replaces bihip_model::par[i]
+ * (i < ntheta); lth_i has the prior N(prior_mean[i], prior_sd[i]^2) and the
+ * random-walk proposal lth_i + prop_sd[i] z.  A move (nmoves proposals per
+ * slot, each a fresh filter over the observations so far) follows every
+ * outer resampling, also after the last observation.  The outer resampler is
+ * systematic with threshold essRel_outer * B; the inner filters resample
+ * with `resampler` at essRel_inner * N_j.
+ */
+typedef struct bihip_smc2 bihip_smc2;
+typedef struct {
+  int B, T, ntheta, nmoves, resampler;
+  double essRel_inner, essRel_outer, t0, dt;
+  uint64_t seed;
+  double prior_mean[8], prior_sd[8], prop_sd[8];
+  int use_graph;  /* inner steps replayed as hipGraphs */
+  int block;      /* inner workgroup size, 0 = autotune */
+} bihip_smc2_config;
+
+int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                      const int* N_host, bihip_smc2** s);
+int bihip_smc2_destroy(bihip_smc2* s);
+/* y_host: T x nobs observations at t0 + k dt, k = 1..T. */
+int bihip_smc2_set_observations(bihip_smc2* s, const double* y_host);
+/* The whole sampler: init, T x (move, step, interact), final move. */
+int bihip_smc2_run(bihip_smc2* s);
+/* Host copies (any may be NULL): log evidence, theta [B x ntheta], outer
+ * log-weights [B], accepted moves [T + 1], outer ESS [T], resampled [T]. */
+int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* lw_host,
+                       int* accepts_host, double* ess_host, int* resampled_host);
 
 #ifdef __cplusplus
 }

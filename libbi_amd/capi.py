@@ -19,6 +19,7 @@ RK4, DOPRI5 = 0, 1
 SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
 F32, F64 = 0, 1
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
+TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER = 7, 8, 9, 10
 
 # exported entry points (must match include/bihip.h)
 EXPORTS = [
@@ -43,7 +44,11 @@ EXPORTS = [
     "bihip_batch_create", "bihip_batch_destroy", "bihip_batch_set_observations",
     "bihip_batch_init", "bihip_batch_step", "bihip_batch_outputs", "bihip_batch_ancestors",
     "bihip_batch_tuning", "bihip_batch_set_timing", "bihip_batch_kernel_time",
-    "bihip_batch_particles",
+    "bihip_batch_particles", "bihip_batch_set_theta", "bihip_batch_set_sizes",
+    "bihip_batch_set_stream_base", "bihip_batch_increments", "bihip_batch_resample_filters",
+    "bihip_batch_accept",
+    "bihip_smc2_create", "bihip_smc2_destroy", "bihip_smc2_set_observations", "bihip_smc2_run",
+    "bihip_smc2_outputs",
 ]
 
 
@@ -72,7 +77,18 @@ class BatchConfig(ctypes.Structure):
         ("B", ctypes.c_int), ("T", ctypes.c_int), ("resampler", ctypes.c_int),
         ("essRel", ctypes.c_double), ("t0", ctypes.c_double), ("dt", ctypes.c_double),
         ("seed", ctypes.c_uint64), ("record", ctypes.c_int), ("use_graph", ctypes.c_int),
-        ("block", ctypes.c_int),
+        ("block", ctypes.c_int), ("capacity", ctypes.c_long), ("stream_base", ctypes.c_uint32),
+    ]
+
+
+class SMC2Config(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int), ("T", ctypes.c_int), ("ntheta", ctypes.c_int),
+        ("nmoves", ctypes.c_int), ("resampler", ctypes.c_int),
+        ("essRel_inner", ctypes.c_double), ("essRel_outer", ctypes.c_double),
+        ("t0", ctypes.c_double), ("dt", ctypes.c_double), ("seed", ctypes.c_uint64),
+        ("prior_mean", ctypes.c_double * 8), ("prior_sd", ctypes.c_double * 8),
+        ("prop_sd", ctypes.c_double * 8), ("use_graph", ctypes.c_int), ("block", ctypes.c_int),
     ]
 
 
@@ -164,6 +180,17 @@ def _declare(L):
         "bihip_batch_tuning": [P, pi, pi, pd],
         "bihip_batch_set_timing": [P, i],
         "bihip_batch_kernel_time": [P, pd, pl],
+        "bihip_batch_set_theta": [P, pd],
+        "bihip_batch_set_sizes": [P, pi],
+        "bihip_batch_set_stream_base": [P, u32],
+        "bihip_batch_increments": [P, i, pd],
+        "bihip_batch_resample_filters": [P, pi],
+        "bihip_batch_accept": [P, P, pi],
+        "bihip_smc2_create": [P, pm, c.POINTER(SMC2Config), pi, c.POINTER(P)],
+        "bihip_smc2_destroy": [P],
+        "bihip_smc2_set_observations": [P, pd],
+        "bihip_smc2_run": [P],
+        "bihip_smc2_outputs": [P, pd, pd, pd, pi, pd, pi],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -346,7 +373,7 @@ class BatchFilter:
     """bihip_batch: B inner bootstrap filters with their own N_j and theta_j."""
 
     def __init__(self, ctx, model, N, theta, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5,
-                 t0=0.0, record=False, use_graph=False, block=0):
+                 t0=0.0, record=False, use_graph=False, block=0, capacity=0, stream_base=0):
         self.ctx, self.model = ctx, model
         self.N = np.ascontiguousarray(N, dtype=np.int32)
         self.theta = np.ascontiguousarray(theta, dtype=np.float64)
@@ -354,7 +381,8 @@ class BatchFilter:
         ntheta = self.theta.shape[1] if self.theta.ndim == 2 else 0
         self.cfg = BatchConfig(B=B, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
                                seed=seed, record=1 if record else 0,
-                               use_graph=1 if use_graph else 0, block=block)
+                               use_graph=1 if use_graph else 0, block=block, capacity=capacity,
+                               stream_base=stream_base)
         h = ctypes.c_void_p()
         check(lib().bihip_batch_create(
             ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
@@ -421,3 +449,94 @@ class BatchFilter:
 
     def particles(self):
         return lib().bihip_batch_particles(self.h)
+
+    def set_theta(self, theta):
+        self.theta = np.ascontiguousarray(theta, dtype=np.float64)
+        check(lib().bihip_batch_set_theta(self.h, self.theta.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def set_sizes(self, N):
+        N = np.ascontiguousarray(N, dtype=np.int32)
+        check(lib().bihip_batch_set_sizes(self.h, N.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        self.N = N
+
+    def set_stream_base(self, base):
+        check(lib().bihip_batch_set_stream_base(self.h, base))
+
+    def increments(self, k):
+        li = np.zeros(self.B)
+        check(lib().bihip_batch_increments(self.h, k, li.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+        return li
+
+    def resample_filters(self, anc):
+        anc = np.ascontiguousarray(anc, dtype=np.int32)
+        check(lib().bihip_batch_resample_filters(self.h, anc.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+        self.N = self.N[anc]
+        if self.theta.ndim == 2:
+            self.theta = self.theta[anc]
+
+    def accept(self, src, mask):
+        mask = np.ascontiguousarray(mask, dtype=np.int32)
+        check(lib().bihip_batch_accept(self.h, src.h, mask.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+        if self.theta.ndim == 2:
+            self.theta = np.where(mask[:, None] != 0, src.theta, self.theta)
+
+
+class SMC2:
+    """bihip_smc2: SMC^2 (MarginalSIR) over log theta with batched inner filters."""
+
+    def __init__(self, ctx, model, N, T, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
+                 nmoves=1, resampler=SYSTEMATIC, ess_rel_inner=0.5, ess_rel_outer=0.5, t0=0.0,
+                 use_graph=False, block=0):
+        nt = len(prior_mean)
+        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        B = len(self.N)
+        cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=nmoves, resampler=resampler,
+                         essRel_inner=ess_rel_inner, essRel_outer=ess_rel_outer, t0=t0, dt=dt,
+                         seed=seed, use_graph=1 if use_graph else 0, block=block)
+        for i in range(nt):
+            cfg.prior_mean[i] = prior_mean[i]
+            cfg.prior_sd[i] = prior_sd[i]
+            cfg.prop_sd[i] = prop_sd[i]
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(lib().bihip_smc2_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg),
+                                      self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      ctypes.byref(h)))
+        self.h = h
+        self.B, self.T, self.nt, self.nobs = B, T, nt, nobs(model)
+
+    def close(self):
+        if self.h:
+            lib().bihip_smc2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.T, self.nobs)
+        check(lib().bihip_smc2_set_observations(self.h, y.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def run(self):
+        check(lib().bihip_smc2_run(self.h))
+
+    def outputs(self):
+        D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        logZ = ctypes.c_double()
+        theta = np.zeros((self.B, self.nt))
+        lw = np.zeros(self.B)
+        acc = np.zeros(self.T + 1, np.int32)
+        ess = np.zeros(self.T)
+        res = np.zeros(self.T, np.int32)
+        check(lib().bihip_smc2_outputs(self.h, ctypes.byref(logZ), theta.ctypes.data_as(D),
+                                       lw.ctypes.data_as(D), acc.ctypes.data_as(I),
+                                       ess.ctypes.data_as(D), res.ctypes.data_as(I)))
+        return dict(logZ=logZ.value, theta=theta, lw=lw, accepts=acc, ess=ess, resampled=res)

@@ -10,8 +10,9 @@
 // weight scan, systematic/stratified offspring, the permuted ancestors (a
 // search table in LDS), the gathered transition, the observation density and
 // the ESS / log-likelihood reduction.  Filter j draws its Philox streams
-// with batch = j, so each inner filter equals the single-filter path (and the
-// oracle's filter) run with that batch id.
+// with its own batch id (stream base + j at initialisation, kept when the
+// filter's content is replaced by a copy), so each inner filter equals the
+// single-filter path run with that batch id.
 #include "model_ops.hpp"
 #include "scan.hpp"
 
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
     const BatchFilter F = a.f[j];
     const int N = F.N, k = F.k;
     const long off = F.off;
-    const uint32_t batch = (uint32_t)j;
+    const uint32_t batch = F.stream;
     if (tid == 0) s_res = F.resample != 0 && F.status == BIHIP_OK;
     __syncthreads();
     bool res = s_res != 0;
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
 // Initial particles of every filter (filter of a particle by a search over offsets).
 template <class M>
 __global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, const long* off, int B,
-                             long total, uint64_t seed, void* Xv, long ld) {
+                             long total, uint64_t seed, uint32_t stream_base, void* Xv, long ld) {
   using T = typename M::real_t;
   constexpr int DL = M::DL;
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,7 +313,7 @@ __global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, cons
   const int j = lo;
   const int p = (int)(g - off[j]);
   T z[DL];
-  slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, (uint32_t)j, 0, z);
+  slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, stream_base + (uint32_t)j, 0, z);
   T* X = static_cast<T*>(Xv);
   for (int c = 0; c < DL; ++c) {
     double par[16];
@@ -328,6 +329,46 @@ __global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, cons
 }
 
 __global__ void k_batch_reset(uint32_t* counter) { *counter = 0u; }
+
+// Whole-filter copies: one lane per destination particle, the filter found by
+// a search over the destination offsets (strictly increasing, N_j >= 1).
+template <class T>
+__global__ void k_batch_copy(const long* dst_off, const long* src_off, const int* len,
+                             const int* mask, int B, long total, int d, const T* Xs, long lds,
+                             const double* lws_s, T* Xd, long ldd, double* lws_d) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = B;  // last filter with dst_off <= g
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (dst_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const int j = lo;
+  const long p = g - dst_off[j];
+  if (p >= len[j] || (mask != nullptr && mask[j] == 0)) return;
+  const long src = src_off[j] + p;
+  for (int c = 0; c < d; ++c) Xd[(size_t)c * ldd + g] = Xs[(size_t)c * lds + src];
+  lws_d[g] = lws_s[src];
+}
+
+int launch_batch_copy(const long* dst_off, const long* src_off, const int* len, const int* mask,
+                      int B, long total, int d, int elem, const void* Xs, long lds,
+                      const double* lws_s, void* Xd, long ldd, double* lws_d, hipStream_t s) {
+  if (total <= 0) return BIHIP_OK;
+  const dim3 grid(ceil_div(total, 256)), block(256);
+  if (elem == 8) {
+    hipLaunchKernelGGL(k_batch_copy<double>, grid, block, 0, s, dst_off, src_off, len, mask, B,
+                       total, d, static_cast<const double*>(Xs), lds, lws_s,
+                       static_cast<double*>(Xd), ldd, lws_d);
+  } else {
+    hipLaunchKernelGGL(k_batch_copy<float>, grid, block, 0, s, dst_off, src_off, len, mask, B,
+                       total, d, static_cast<const float*>(Xs), lds, lws_s,
+                       static_cast<float*>(Xd), ldd, lws_d);
+  }
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
 
 // ---------------------------------------------------------------- dispatch
 template <int BLOCK>
@@ -422,13 +463,14 @@ struct BatchInitRunner {
   int B;
   long total;
   uint64_t seed;
+  uint32_t stream_base;
   void* X;
   long ld;
   hipStream_t s;
   template <class M>
   int run() {
     hipLaunchKernelGGL((k_batch_init<M>), dim3(ceil_div(total, 256)), dim3(256), 0, s, ma, theta,
-                       ntheta, off, B, total, seed, X, ld);
+                       ntheta, off, B, total, seed, stream_base, X, ld);
     BIHIP_LAUNCH_CHECK();
     return BIHIP_OK;
   }
@@ -439,9 +481,10 @@ struct BatchInitRunner {
 };
 
 int launch_batch_init(const bihip_model* m, const double* theta, int ntheta, const long* off,
-                      int B, long total, uint64_t seed, void* X, long ld, hipStream_t s) {
+                      int B, long total, uint64_t seed, uint32_t stream_base, void* X, long ld,
+                      hipStream_t s) {
   if (total <= 0) return BIHIP_OK;
-  BatchInitRunner r{to_args(m), theta, ntheta, off, B, total, seed, X, ld, s};
+  BatchInitRunner r{to_args(m), theta, ntheta, off, B, total, seed, stream_base, X, ld, s};
   return with_small_model(m, r);
 }
 

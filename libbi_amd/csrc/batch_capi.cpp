@@ -12,10 +12,16 @@ struct bihip_batch {
   bihip_model model{};
   bihip_batch_config cfg{};
   int ntheta = 0, nobs = 1, d = 1, elem = 8;
-  long ld = 0, total = 0;
+  long ld = 0, total = 0, cap = 0;  // ld: column stride, >= cap >= total
   std::vector<int> N;
   std::vector<long> off;
+  std::vector<double> theta_h;      // host copy of theta (B x ntheta)
+  uint32_t stream_base = 0;
+  int anc_from = 0;                 // first step whose recorded ancestors hold
   double* theta = nullptr;
+  long* aux_src = nullptr;          // whole-filter copies: source offsets,
+  int* aux_len = nullptr;           // lengths
+  int* aux_mask = nullptr;          // and selection
   int* order = nullptr;
   long* offd = nullptr;
   BatchFilter* f = nullptr;
@@ -39,6 +45,42 @@ struct bihip_batch {
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
 };
+
+// Offsets (exclusive scan of N) and the largest-first processing order of a
+// layout, uploaded to the device; returns the total or -1 beyond capacity.
+static long batch_layout(bihip_batch* b, const int* N_host) {
+  const int B = b->cfg.B;
+  std::vector<long> off(B);
+  long total = 0;
+  for (int j = 0; j < B; ++j) {
+    off[j] = total;
+    total += N_host[j];
+  }
+  if (b->cap > 0 && total > b->cap) return -1;
+  b->N.assign(N_host, N_host + B);
+  b->off = off;
+  b->total = total;
+  return total;
+}
+
+static int batch_upload_layout(bihip_batch* b, hipStream_t s) {
+  const int B = b->cfg.B;
+  std::vector<int> ord(B);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->N[x] > b->N[y]; });
+  BIHIP_HIP(hipMemcpyAsync(b->order, ord.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipMemcpyAsync(b->offd, b->off.data(), sizeof(long) * B, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  return BIHIP_OK;
+}
+
+static int check_sizes(const int* N_host, int B) {
+  for (int j = 0; j < B; ++j)
+    if (N_host[j] <= 0 || N_host[j] > batch_nmax())
+      return set_error(BIHIP_EINVAL, "inner filter %d has N = %d outside [1, %d]", j, N_host[j],
+                       batch_nmax());
+  return BIHIP_OK;
+}
 
 static BatchLaunch batch_launch(bihip_batch* b) {
   BatchLaunch L{};
@@ -204,23 +246,15 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   b->d = m->d;
   b->elem = m->precision == BIHIP_F64 ? 8 : 4;
   b->block = cfg->block;
-  b->N.assign(N_host, N_host + B);
-  b->off.resize(B);
-  long total = 0;
-  for (int j = 0; j < B; ++j) {
-    if (b->N[j] <= 0 || b->N[j] > batch_nmax()) {
-      delete b;
-      return set_error(BIHIP_EINVAL, "inner filter %d has N = %d outside [1, %d]", j, N_host[j],
-                       batch_nmax());
-    }
-    b->off[j] = total;
-    total += b->N[j];
+  b->stream_base = cfg->stream_base;
+  if ((rc = check_sizes(N_host, B)) != BIHIP_OK) {
+    delete b;
+    return rc;
   }
-  b->total = total;
-  b->ld = (total + 63) / 64 * 64;
-  std::vector<int> ord(B);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return b->N[x] > b->N[y]; });
+  batch_layout(b, N_host);
+  b->cap = std::max(cfg->capacity, b->total);
+  b->ld = (b->cap + 63) / 64 * 64;
+  if (ntheta > 0) b->theta_h.assign(theta_host, theta_host + (size_t)B * ntheta);
 #define B_ALLOC(ptr, n)                         \
   if ((rc = dalloc(&(ptr), (n))) != BIHIP_OK) { \
     bihip_batch_destroy(b);                     \
@@ -235,9 +269,12 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   B_ALLOC(b->order, B);
   B_ALLOC(b->offd, B);
   B_ALLOC(b->f, B);
-  B_ALLOC(b->lws, total);
-  B_ALLOC(b->Ws, total);
-  B_ALLOC(b->as, total);
+  B_ALLOC(b->lws, b->cap);
+  B_ALLOC(b->Ws, b->cap);
+  B_ALLOC(b->as, b->cap);
+  B_ALLOC(b->aux_src, B);
+  B_ALLOC(b->aux_len, B);
+  B_ALLOC(b->aux_mask, B);
   B_ALLOC(b->y, (size_t)cfg->T * b->nobs);
   B_ALLOC(b->hist_li, (size_t)B * cfg->T);
   B_ALLOC(b->hist_ess, (size_t)B * cfg->T);
@@ -249,9 +286,10 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   if (ntheta > 0)
     BIHIP_HIP(hipMemcpyAsync(b->theta, theta_host, sizeof(double) * B * ntheta,
                              hipMemcpyHostToDevice, s));
-  BIHIP_HIP(hipMemcpyAsync(b->order, ord.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
-  BIHIP_HIP(hipMemcpyAsync(b->offd, b->off.data(), sizeof(long) * B, hipMemcpyHostToDevice, s));
-  BIHIP_HIP(hipStreamSynchronize(s));
+  if ((rc = batch_upload_layout(b, s)) != BIHIP_OK) {
+    bihip_batch_destroy(b);
+    return rc;
+  }
   *out = b;
   return BIHIP_OK;
 }
@@ -278,6 +316,9 @@ int bihip_batch_destroy(bihip_batch* b) {
   dfree(b->hist_ess);
   dfree(b->hist_res);
   dfree(b->counter);
+  dfree(b->aux_src);
+  dfree(b->aux_len);
+  dfree(b->aux_mask);
   delete b;
   return BIHIP_OK;
 }
@@ -302,7 +343,7 @@ int bihip_batch_init(bihip_batch* b) {
   b->cur = 0;
   b->k = 0;
   if ((rc = launch_batch_init(&b->model, b->theta, b->ntheta, b->offd, b->cfg.B, b->total,
-                              b->cfg.seed, b->X[0], b->ld, s)))
+                              b->cfg.seed, b->stream_base, b->X[0], b->ld, s)))
     return rc;
   std::vector<BatchFilter> F(b->cfg.B);
   for (int j = 0; j < b->cfg.B; ++j) {
@@ -310,6 +351,7 @@ int bihip_batch_init(bihip_batch* b) {
     F[j].status = BIHIP_OK;
     F[j].N = b->N[j];
     F[j].off = b->off[j];
+    F[j].stream = b->stream_base + (uint32_t)j;
   }
   BIHIP_HIP(hipMemcpyAsync(b->f, F.data(), sizeof(BatchFilter) * F.size(), hipMemcpyHostToDevice,
                            s));
@@ -319,6 +361,7 @@ int bihip_batch_init(bihip_batch* b) {
   BIHIP_HIP(hipMemsetAsync(b->hist_res, 0, sizeof(int) * b->cfg.B * b->cfg.T, s));
   BIHIP_HIP(hipStreamSynchronize(s));
   b->ev_used = 0;
+  b->anc_from = 0;
   b->ready = true;
   return BIHIP_OK;
 }
@@ -387,7 +430,9 @@ int bihip_batch_outputs(bihip_batch* b, double* ll, double* li, double* ess, int
 int bihip_batch_ancestors(bihip_batch* b, int j, int k, int* as_host) {
   if (b == nullptr || as_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
   if (b->anc == nullptr) return set_error(BIHIP_ESTATE, "ancestors were not recorded");
-  if (j < 0 || j >= b->cfg.B || k < 0 || k >= b->k) return set_error(BIHIP_EINVAL, "bad j or k");
+  if (j < 0 || j >= b->cfg.B || k < b->anc_from || k >= b->k)
+    return set_error(BIHIP_EINVAL, "bad j or k (recorded steps %d..%d of the current layout)",
+                     b->anc_from, b->k - 1);
   int rc = ctx_enter_device(b->ctx);
   if (rc) return rc;
   hipStream_t s = ctx_stream_of(b->ctx);
@@ -430,5 +475,159 @@ int bihip_batch_kernel_time(bihip_batch* b, double* ms, long* count) {
 }
 
 long bihip_batch_particles(bihip_batch* b) { return b ? b->total : -1; }
+
+int bihip_batch_set_theta(bihip_batch* b, const double* theta_host) {
+  if (b == nullptr || theta_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (b->ntheta == 0) return set_error(BIHIP_ESTATE, "the batch was created with ntheta = 0");
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(b->ctx);
+  const size_t n = (size_t)b->cfg.B * b->ntheta;
+  b->theta_h.assign(theta_host, theta_host + n);
+  BIHIP_HIP(hipMemcpyAsync(b->theta, b->theta_h.data(), sizeof(double) * n,
+                           hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  return BIHIP_OK;
+}
+
+int bihip_batch_set_sizes(bihip_batch* b, const int* N_host) {
+  if (b == nullptr || N_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = check_sizes(N_host, b->cfg.B);
+  if (rc) return rc;
+  if ((rc = ctx_enter_device(b->ctx))) return rc;
+  hipStream_t s = ctx_stream_of(b->ctx);
+  BIHIP_HIP(hipStreamSynchronize(s));
+  if (batch_layout(b, N_host) < 0)
+    return set_error(BIHIP_EINVAL, "the sizes exceed the batch capacity of %ld particles", b->cap);
+  b->ready = false;
+  return batch_upload_layout(b, s);
+}
+
+int bihip_batch_set_stream_base(bihip_batch* b, uint32_t base) {
+  if (b == nullptr) return set_error(BIHIP_EINVAL, "NULL batch");
+  if ((uint64_t)base + (uint64_t)b->cfg.B > (1u << 24))
+    return set_error(BIHIP_EINVAL, "stream ids must stay below 2^24");
+  b->stream_base = base;
+  return BIHIP_OK;
+}
+
+int bihip_batch_increments(bihip_batch* b, int k, double* li_host) {
+  if (b == nullptr || li_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (k < 0 || k >= b->k) return set_error(BIHIP_EINVAL, "step %d not taken yet", k);
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(b->ctx);
+  BIHIP_HIP(hipMemcpy2DAsync(li_host, sizeof(double), b->hist_li + k,
+                             sizeof(double) * b->cfg.T, sizeof(double), b->cfg.B,
+                             hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  return BIHIP_OK;
+}
+
+int bihip_batch_resample_filters(bihip_batch* b, const int* anc_host) {
+  if (b == nullptr || anc_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (!b->ready) return set_error(BIHIP_ESTATE, "bihip_batch_init has not been called");
+  const int B = b->cfg.B;
+  for (int j = 0; j < B; ++j)
+    if (anc_host[j] < 0 || anc_host[j] >= B)
+      return set_error(BIHIP_EINVAL, "ancestor %d of filter %d outside [0, %d)", anc_host[j], j,
+                       B);
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(b->ctx);
+  std::vector<BatchFilter> F(B), G(B);
+  BIHIP_HIP(hipMemcpyAsync(F.data(), b->f, sizeof(BatchFilter) * B, hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  const std::vector<long> off_old = b->off;
+  std::vector<int> Nn(B), len(B);
+  std::vector<long> src(B);
+  for (int j = 0; j < B; ++j) {
+    Nn[j] = b->N[anc_host[j]];
+    src[j] = off_old[anc_host[j]];
+  }
+  const std::vector<int> N_old = b->N;
+  if (batch_layout(b, Nn.data()) < 0) {
+    batch_layout(b, N_old.data());
+    return set_error(BIHIP_EINVAL, "the resampled sizes exceed the batch capacity of %ld "
+                     "particles", b->cap);
+  }
+  for (int j = 0; j < B; ++j) {
+    G[j] = F[anc_host[j]];
+    G[j].N = b->N[j];
+    G[j].off = b->off[j];
+    G[j].stream = F[j].stream;  // each filter keeps its own random streams
+  }
+  if ((rc = batch_upload_layout(b, s))) return rc;
+  BIHIP_HIP(hipMemcpyAsync(b->aux_src, src.data(), sizeof(long) * B, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipMemcpyAsync(b->aux_len, b->N.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+  // state into the other buffer, weights through the scratch Ws
+  if ((rc = launch_batch_copy(b->offd, b->aux_src, b->aux_len, nullptr, B, b->total, b->d,
+                              b->elem, b->X[b->cur], b->ld, b->lws, b->X[1 - b->cur], b->ld,
+                              b->Ws, s)))
+    return rc;
+  BIHIP_HIP(hipMemcpyAsync(b->lws, b->Ws, sizeof(double) * b->total, hipMemcpyDeviceToDevice, s));
+  BIHIP_HIP(hipMemcpyAsync(b->f, G.data(), sizeof(BatchFilter) * B, hipMemcpyHostToDevice, s));
+  if (b->ntheta > 0) {
+    std::vector<double> th((size_t)B * b->ntheta);
+    for (int j = 0; j < B; ++j)
+      std::copy(b->theta_h.begin() + (size_t)anc_host[j] * b->ntheta,
+                b->theta_h.begin() + (size_t)(anc_host[j] + 1) * b->ntheta,
+                th.begin() + (size_t)j * b->ntheta);
+    b->theta_h = th;
+    BIHIP_HIP(hipMemcpyAsync(b->theta, b->theta_h.data(), sizeof(double) * th.size(),
+                             hipMemcpyHostToDevice, s));
+  }
+  BIHIP_HIP(hipStreamSynchronize(s));
+  b->cur = 1 - b->cur;
+  b->anc_from = b->k;
+  return BIHIP_OK;
+}
+
+int bihip_batch_accept(bihip_batch* dst, bihip_batch* src, const int* accept_host) {
+  if (dst == nullptr || src == nullptr || accept_host == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  if (dst == src) return set_error(BIHIP_EINVAL, "source and destination are the same batch");
+  if (dst->ctx != src->ctx) return set_error(BIHIP_EINVAL, "batches of different contexts");
+  if (!dst->ready || !src->ready) return set_error(BIHIP_ESTATE, "batch not initialised");
+  const int B = dst->cfg.B;
+  if (src->cfg.B != B || src->N != dst->N || src->k != dst->k || src->d != dst->d ||
+      src->elem != dst->elem || src->ntheta != dst->ntheta)
+    return set_error(BIHIP_EINVAL, "the batches differ in layout, position, model or theta");
+  int rc = ctx_enter_device(dst->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(dst->ctx);
+  std::vector<BatchFilter> Fd(B), Fs(B);
+  BIHIP_HIP(hipMemcpyAsync(Fd.data(), dst->f, sizeof(BatchFilter) * B, hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipMemcpyAsync(Fs.data(), src->f, sizeof(BatchFilter) * B, hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  std::vector<int> mask(B);
+  for (int j = 0; j < B; ++j) {
+    mask[j] = accept_host[j] != 0;
+    if (!mask[j]) continue;
+    const uint32_t stream = Fd[j].stream;
+    Fd[j] = Fs[j];
+    Fd[j].off = dst->off[j];
+    Fd[j].stream = stream;  // the filter keeps drawing from its own streams
+    if (dst->ntheta > 0)
+      std::copy(src->theta_h.begin() + (size_t)j * dst->ntheta,
+                src->theta_h.begin() + (size_t)(j + 1) * dst->ntheta,
+                dst->theta_h.begin() + (size_t)j * dst->ntheta);
+  }
+  BIHIP_HIP(hipMemcpyAsync(dst->aux_mask, mask.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipMemcpyAsync(dst->aux_src, src->off.data(), sizeof(long) * B, hipMemcpyHostToDevice,
+                           s));
+  BIHIP_HIP(hipMemcpyAsync(dst->aux_len, dst->N.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+  if ((rc = launch_batch_copy(dst->offd, dst->aux_src, dst->aux_len, dst->aux_mask, B, dst->total,
+                              dst->d, dst->elem, src->X[src->cur], src->ld, src->lws,
+                              dst->X[dst->cur], dst->ld, dst->lws, s)))
+    return rc;
+  BIHIP_HIP(hipMemcpyAsync(dst->f, Fd.data(), sizeof(BatchFilter) * B, hipMemcpyHostToDevice, s));
+  if (dst->ntheta > 0)
+    BIHIP_HIP(hipMemcpyAsync(dst->theta, dst->theta_h.data(),
+                             sizeof(double) * dst->theta_h.size(), hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  dst->anc_from = dst->k;
+  return BIHIP_OK;
+}
 
 }  // extern "C"

@@ -260,6 +260,7 @@ int bihip_iota(bihip_ctx* ctx, int* x, int n) {
 
 // ================================================================ random numbers
 // Philox4x32-10 on the host (Salmon et al., SC'11), as the device's.
+
 static void host_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
@@ -277,8 +278,8 @@ static void host_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
 
 double bihip_uniform_host(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
                           uint32_t batch) {
-  uint32_t c[4] = {index, step, tag | (batch << 8), 0u};
-  host_philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint32_t c[4];
+  bihip::philox_host_block(seed, index, step, tag, batch, 0u, c);
   const uint64_t m = ((((uint64_t)c[0]) << 32) | c[1]) >> 11;
   return (double)m * 0x1.0p-53;
 }
@@ -981,3 +982,14 @@ int bihip_pf_kernel_time(bihip_pf* pf, int kind, double* ms, long* count) {
 }
 
 }  // extern "C"
+
+namespace bihip {
+void philox_host_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                       uint32_t batch, uint32_t block, uint32_t out[4]) {
+  out[0] = index;
+  out[1] = step;
+  out[2] = tag | (batch << 8);
+  out[3] = block;
+  host_philox(out, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+}  // namespace bihip

@@ -47,7 +47,9 @@ struct PFDevice {
 struct BatchFilter {
   double logLikelihood, ess, mx, lW;
   int resample, status, k, N;
-  long off;  // first particle in the flattened state
+  long off;         // first particle in the flattened state
+  uint32_t stream;  // Philox batch id of the filter's draws
+  int pad;
 };
 
 // Per-step histories on the device (T entries each; ancestors T x P when recorded).
@@ -219,7 +221,17 @@ bool batch_model_supported(const bihip_model* m);
 int batch_nmax();
 int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out);
 int launch_batch_init(const bihip_model* m, const double* theta, int ntheta, const long* off,
-                      int B, long total, uint64_t seed, void* X, long ld, hipStream_t s);
+                      int B, long total, uint64_t seed, uint32_t stream_base, void* X, long ld,
+                      hipStream_t s);
+// Whole-filter copies between layouts (destination particle g of filter j
+// takes source particle src_off[j] + g - dst_off[j]; mask[j] == 0 skips j).
+int launch_batch_copy(const long* dst_off, const long* src_off, const int* len, const int* mask,
+                      int B, long total, int d, int elem, const void* Xs, long lds,
+                      const double* lws_s, void* Xd, long ldd, double* lws_d, hipStream_t s);
+
+// ---- host Philox4x32-10 block {index, step, tag | batch << 8, block}
+void philox_host_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
+                       uint32_t batch, uint32_t block, uint32_t out[4]);
 
 // ---- context access for the other host files (capi.cpp)
 int ctx_enter_device(bihip_ctx* ctx);     // makes the context's device current

@@ -1,0 +1,364 @@
+// SMC^2 over the batched inner filters (bihip_smc2_*, declared in include/bihip.h).
+//
+// MarginalSIR::sample (share/src/bi/sampler/MarginalSIR.hpp:281-320): init
+// (:322-344), then for every observation a move (:395-488), a step of every
+// inner filter (:346-368) and the outer interaction (:370-393), and a final
+// move after the last observation.  The inner filters are one bihip_batch
+// (one persistent launch per step for all theta-particles); the proposal
+// filters of a move are a second batch run from the start of the schedule.
+// The outer level (B log-weights, systematic resampling of whole filters,
+// the Metropolis-Hastings decisions) is host arithmetic in FP64, as the
+// reference's host vectors of theta-particles are.
+//
+// theta lives in log space: parameter i of slot j is exp(lth[j][i]), with a
+// Gaussian prior N(prior_mean[i], prior_sd[i]^2) on lth[j][i] and a Gaussian
+// random-walk proposal of scale prop_sd[i], so prior and proposal Jacobians
+// cancel in the acceptance ratio.  Keyed draws: prior normals
+// {j, 0, THETA, 0}; proposal normals {j, k, PROPOSAL, m}; acceptance uniform
+// {j, k, ACCEPT, m}; outer systematic offset {0, k, OUTER, 0}.  Slot j's
+// filter draws with batch id j; the proposal filters of move m at time k with
+// batch ids B (1 + m + nmoves k) + j.
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace bihip;
+
+struct bihip_smc2 {
+  bihip_ctx* ctx = nullptr;
+  bihip_model model{};
+  bihip_smc2_config cfg{};
+  bihip_batch* slots = nullptr;
+  bihip_batch* prop = nullptr;
+  std::vector<int> N0, N;  // sizes at creation, current sizes per slot
+  std::vector<double> lth, lth2, lw, theta;
+  std::vector<int> accepts, resampled;
+  std::vector<double> ess;
+  double logZ = 0.0;
+  bool have_obs = false, done = false;
+};
+
+namespace {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+
+double host_u53(uint32_t r0, uint32_t r1) {
+  const uint64_t m = ((((uint64_t)r0) << 32) | r1) >> 11;
+  return (double)m * 0x1.0p-53;
+}
+
+// n standard normals at {index, step, tag|batch<<8, block}: Box-Muller in
+// FP64 on the 32-bit word pairs (0,1) and (2,3) of each Philox block, the
+// first uniform 1 - r0 2^-32 in (0,1], the second r1 2^-32.
+void host_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch,
+                  int n, double* z) {
+  for (int b = 0; 4 * b < n; ++b) {
+    uint32_t r[4];
+    philox_host_block(seed, index, step, tag, batch, (uint32_t)b, r);
+    for (int h = 0; h < 2; ++h) {
+      const double u1 = 1.0 - (double)r[2 * h] * 0x1.0p-32;
+      const double u2 = (double)r[2 * h + 1] * 0x1.0p-32;
+      const double rad = std::sqrt(-2.0 * std::log(u1));
+      const double ang = TWO_PI * u2;
+      if (4 * b + 2 * h < n) z[4 * b + 2 * h] = rad * std::cos(ang);
+      if (4 * b + 2 * h + 1 < n) z[4 * b + 2 * h + 1] = rad * std::sin(ang);
+    }
+  }
+}
+
+double host_uniform(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, uint32_t batch) {
+  uint32_t r[4];
+  philox_host_block(seed, index, step, tag, batch, 0u, r);
+  return host_u53(r[0], r[1]);
+}
+
+double nanexp_h(double x) { return std::isnan(x) ? 0.0 : std::exp(x); }
+
+// max with NaN below every number (vector_primitive.hpp:860-872)
+double nan_max_h(const std::vector<double>& x) {
+  double mx = x[0];
+  for (size_t i = 1; i < x.size(); ++i) {
+    const bool less = std::isnan(mx) ? !std::isnan(x[i]) : (!std::isnan(x[i]) && mx < x[i]);
+    if (less) mx = x[i];
+  }
+  return mx;
+}
+
+// ess_reduce over the outer log-weights, left-to-right sums
+// (vector_primitive.hpp:959-974); *lW = log mean weight.
+double outer_ess(const std::vector<double>& lw, double* lW) {
+  const double mx = nan_max_h(lw);
+  double s1 = 0.0, s2 = 0.0;
+  for (double v : lw) {
+    const double w = nanexp_h(v - mx);
+    s1 = s1 + w;
+    s2 = s2 + w * w;
+  }
+  *lW = mx + std::log(s1) - std::log((double)lw.size());
+  return s1 * s1 / s2;
+}
+
+// Systematic resampling of the B slots (SystematicResampler.hpp:110-130)
+// into ancestors in the permuted layout: a slot with offspring keeps itself,
+// the extra copies fill the slots without offspring in ascending order.
+bool outer_ancestors(const std::vector<double>& lw, double a, std::vector<int>& as) {
+  const int B = (int)lw.size();
+  const double mx = nan_max_h(lw);
+  std::vector<double> Ws(B);
+  double s = 0.0;
+  for (int i = 0; i < B; ++i) {
+    s = s + nanexp_h(lw[i] - mx);
+    Ws[i] = s;
+  }
+  const double W = Ws[B - 1];
+  if (!(W > 0.0)) return false;
+  std::vector<int> o(B);
+  int prev = 0;
+  for (int i = 0; i < B; ++i) {
+    const double v = Ws[i] / W * (double)B + a;
+    int O = (int)v;
+    if (O > B) O = B;
+    o[i] = O - prev;
+    prev = O;
+  }
+  as.assign(B, -1);
+  std::vector<int> free_slots;
+  for (int i = 0; i < B; ++i) {
+    if (o[i] > 0) as[i] = i;
+    else free_slots.push_back(i);
+  }
+  size_t next = 0;
+  for (int i = 0; i < B; ++i)
+    for (int c = 1; c < o[i]; ++c) as[free_slots[next++]] = i;
+  return true;
+}
+
+double log_prior(const bihip_smc2_config& c, const double* lth) {
+  double lp = 0.0;
+  for (int i = 0; i < c.ntheta; ++i) {
+    const double z = (lth[i] - c.prior_mean[i]) / c.prior_sd[i];
+    lp += -0.5 * z * z;
+  }
+  return lp;
+}
+
+int filter_logliks(bihip_batch* b, int B, std::vector<double>& ll) {
+  std::vector<int> st(B);
+  ll.resize(B);
+  int rc = bihip_batch_outputs(b, ll.data(), nullptr, nullptr, nullptr, st.data());
+  if (rc) return rc;
+  for (int j = 0; j < B; ++j)
+    if (st[j] != BIHIP_OK) ll[j] = -INFINITY;
+  return BIHIP_OK;
+}
+
+// PMMH move of every slot (MarginalSIR.hpp:395-488); returns accepts in *nacc.
+int smc2_move(bihip_smc2* s, int k, int* nacc) {
+  const bihip_smc2_config& c = s->cfg;
+  const int B = c.B, nt = c.ntheta;
+  int rc;
+  *nacc = 0;
+  std::vector<double> ll1, ll2, th((size_t)B * nt);
+  std::vector<int> acc(B);
+  for (int m = 0; m < c.nmoves; ++m) {
+    for (int j = 0; j < B; ++j) {
+      double z[8];
+      host_normals(c.seed, (uint32_t)j, (uint32_t)k, BIHIP_TAG_PROPOSAL, (uint32_t)m, nt, z);
+      for (int i = 0; i < nt; ++i) {
+        const size_t q = (size_t)j * nt + i;
+        s->lth2[q] = s->lth[q] + c.prop_sd[i] * z[i];
+        th[q] = std::exp(s->lth2[q]);
+      }
+    }
+    const uint32_t base = (uint32_t)(B * (1 + m + c.nmoves * k));
+    if ((rc = bihip_batch_set_sizes(s->prop, s->N.data())) ||
+        (rc = bihip_batch_set_theta(s->prop, th.data())) ||
+        (rc = bihip_batch_set_stream_base(s->prop, base)) || (rc = bihip_batch_init(s->prop)) ||
+        (rc = bihip_batch_step(s->prop, k)) || (rc = filter_logliks(s->prop, B, ll2)) ||
+        (rc = filter_logliks(s->slots, B, ll1)))
+      return rc;
+    for (int j = 0; j < B; ++j) {
+      bool accept;
+      if (!std::isfinite(ll2[j])) {
+        accept = false;
+      } else if (!std::isfinite(ll1[j])) {
+        accept = true;
+      } else {
+        const double u =
+            host_uniform(c.seed, (uint32_t)j, (uint32_t)k, BIHIP_TAG_ACCEPT, (uint32_t)m);
+        const double logratio = ll2[j] - ll1[j] + log_prior(c, &s->lth2[(size_t)j * nt]) -
+                                log_prior(c, &s->lth[(size_t)j * nt]);
+        accept = std::log(u) < logratio;
+      }
+      acc[j] = accept ? 1 : 0;
+      if (accept) {
+        for (int i = 0; i < nt; ++i) s->lth[(size_t)j * nt + i] = s->lth2[(size_t)j * nt + i];
+        ++*nacc;
+      }
+    }
+    if ((rc = bihip_batch_accept(s->slots, s->prop, acc.data()))) return rc;
+  }
+  return BIHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                      const int* N_host, bihip_smc2** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (c.B <= 0 || c.T <= 0) return set_error(BIHIP_EINVAL, "need B > 0 and T > 0");
+  if (c.ntheta < 1 || c.ntheta > 8) return set_error(BIHIP_EINVAL, "ntheta must lie in [1,8]");
+  if (c.nmoves < 0) return set_error(BIHIP_EINVAL, "nmoves must be >= 0");
+  if (!(c.essRel_outer >= 0.0 && c.essRel_outer <= 1.0))
+    return set_error(BIHIP_EINVAL, "essRel_outer must lie in [0,1]");
+  for (int i = 0; i < c.ntheta; ++i)
+    if (!(c.prior_sd[i] > 0.0) || !(c.prop_sd[i] >= 0.0))
+      return set_error(BIHIP_EINVAL, "prior_sd must be positive and prop_sd non-negative");
+  if ((uint64_t)c.B * (1 + (uint64_t)c.nmoves * (c.T + 1)) > (1u << 24))
+    return set_error(BIHIP_EINVAL, "B (1 + nmoves (T + 1)) random-stream ids exceed 2^24");
+  int maxN = 0;
+  for (int j = 0; j < c.B; ++j) maxN = N_host[j] > maxN ? N_host[j] : maxN;
+  bihip_smc2* s = new bihip_smc2();
+  s->ctx = ctx;
+  s->model = *m;
+  s->cfg = c;
+  s->N0.assign(N_host, N_host + c.B);
+  s->N = s->N0;
+  s->lth.assign((size_t)c.B * c.ntheta, 0.0);
+  s->lth2 = s->lth;
+  s->theta = s->lth;
+  s->lw.assign(c.B, 0.0);
+  s->accepts.assign(c.T + 1, 0);
+  s->resampled.assign(c.T, 0);
+  s->ess.assign(c.T, 0.0);
+  bihip_batch_config bc{};
+  bc.B = c.B;
+  bc.T = c.T;
+  bc.resampler = c.resampler;
+  bc.essRel = c.essRel_inner;
+  bc.t0 = c.t0;
+  bc.dt = c.dt;
+  bc.seed = c.seed;
+  bc.record = 0;
+  bc.use_graph = c.use_graph;
+  bc.block = c.block;
+  bc.capacity = (long)c.B * maxN;  // outer resampling can repeat the largest filter
+  bc.stream_base = 0;
+  int rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &s->slots);
+  if (rc == BIHIP_OK)
+    rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &s->prop);
+  if (rc != BIHIP_OK) {
+    bihip_batch_destroy(s->slots);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return BIHIP_OK;
+}
+
+int bihip_smc2_destroy(bihip_smc2* s) {
+  if (s == nullptr) return BIHIP_OK;
+  bihip_batch_destroy(s->prop);
+  bihip_batch_destroy(s->slots);
+  delete s;
+  return BIHIP_OK;
+}
+
+int bihip_smc2_set_observations(bihip_smc2* s, const double* y_host) {
+  if (s == nullptr || y_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = bihip_batch_set_observations(s->slots, y_host);
+  if (rc == BIHIP_OK) rc = bihip_batch_set_observations(s->prop, y_host);
+  s->have_obs = rc == BIHIP_OK;
+  return rc;
+}
+
+int bihip_smc2_run(bihip_smc2* s) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->have_obs) return set_error(BIHIP_ESTATE, "no observations set");
+  const bihip_smc2_config& c = s->cfg;
+  const int B = c.B, T = c.T, nt = c.ntheta;
+  int rc;
+  // init (:322-344): theta from the prior, every inner filter from its initial law
+  for (int j = 0; j < B; ++j) {
+    double z[8];
+    host_normals(c.seed, (uint32_t)j, 0u, BIHIP_TAG_THETA, 0u, nt, z);
+    for (int i = 0; i < nt; ++i) {
+      const size_t q = (size_t)j * nt + i;
+      s->lth[q] = c.prior_mean[i] + c.prior_sd[i] * z[i];
+      s->theta[q] = std::exp(s->lth[q]);
+    }
+  }
+  s->N = s->N0;  // a second run starts from the sizes given at creation
+  if ((rc = bihip_batch_set_sizes(s->slots, s->N.data())) ||
+      (rc = bihip_batch_set_theta(s->slots, s->theta.data())) ||
+      (rc = bihip_batch_set_stream_base(s->slots, 0u)) || (rc = bihip_batch_init(s->slots)))
+    return rc;
+  std::fill(s->lw.begin(), s->lw.end(), 0.0);
+  std::fill(s->accepts.begin(), s->accepts.end(), 0);
+  std::fill(s->resampled.begin(), s->resampled.end(), 0);
+  std::fill(s->ess.begin(), s->ess.end(), 0.0);
+  s->logZ = 0.0;
+  bool lastResample = false;
+  std::vector<double> li(B);
+  std::vector<int> st(B), as;
+  for (int k = 0; k <= T; ++k) {
+    int nacc = 0;
+    if (lastResample && (rc = smc2_move(s, k, &nacc))) return rc;
+    s->accepts[k] = nacc;
+    if (k == T) break;  // the final move (:301-303) follows the last observation
+    // step (:346-368): every inner filter, lw += logIncrement
+    if ((rc = bihip_batch_step(s->slots, 1)) ||
+        (rc = bihip_batch_increments(s->slots, k, li.data())) ||
+        (rc = bihip_batch_outputs(s->slots, nullptr, nullptr, nullptr, nullptr, st.data())))
+      return rc;
+    for (int j = 0; j < B; ++j) s->lw[j] += st[j] == BIHIP_OK ? li[j] : -INFINITY;
+    // interact (:370-393): outer ESS and marginal likelihood, then resampling
+    double lW;
+    const double e = outer_ess(s->lw, &lW);
+    s->ess[k] = e;
+    s->logZ = lW;
+    const bool r = e < c.essRel_outer * B;
+    s->resampled[k] = r ? 1 : 0;
+    if (r) {
+      const double a = host_uniform(c.seed, 0u, (uint32_t)k, BIHIP_TAG_OUTER, 0u);
+      if (!outer_ancestors(s->lw, a, as))
+        return set_error(BIHIP_DEGENERATE, "outer weights are all zero at step %d", k);
+      if ((rc = bihip_batch_resample_filters(s->slots, as.data()))) return rc;
+      std::vector<double> l2(s->lth.size());
+      std::vector<int> N2(B);
+      for (int j = 0; j < B; ++j) {
+        for (int i = 0; i < nt; ++i) l2[(size_t)j * nt + i] = s->lth[(size_t)as[j] * nt + i];
+        N2[j] = s->N[as[j]];
+      }
+      s->lth = l2;
+      s->N = N2;
+      std::fill(s->lw.begin(), s->lw.end(), lW);
+    }
+    lastResample = r;
+  }
+  for (size_t q = 0; q < s->lth.size(); ++q) s->theta[q] = std::exp(s->lth[q]);
+  s->done = true;
+  return BIHIP_OK;
+}
+
+int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* lw_host,
+                       int* accepts_host, double* ess_host, int* resampled_host) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->done) return set_error(BIHIP_ESTATE, "bihip_smc2_run has not completed");
+  if (logZ) *logZ = s->logZ;
+  if (theta_host) std::copy(s->theta.begin(), s->theta.end(), theta_host);
+  if (lw_host) std::copy(s->lw.begin(), s->lw.end(), lw_host);
+  if (accepts_host) std::copy(s->accepts.begin(), s->accepts.end(), accepts_host);
+  if (ess_host) std::copy(s->ess.begin(), s->ess.end(), ess_host);
+  if (resampled_host) std::copy(s->resampled.begin(), s->resampled.end(), resampled_host);
+  return BIHIP_OK;
+}
+
+}  // extern "C"

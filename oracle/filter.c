@@ -96,7 +96,10 @@ int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled
         orc_offspring_to_cumulative(f->os, P, f->Os);
       }
     }
-    if (rc != ORC_OK) return rc;
+    if (rc != ORC_OK) {
+      f->logLikelihood = -INFINITY;  /* a degenerate filter has no likelihood */
+      return rc;
+    }
     orc_cumulative_offspring_to_ancestors_permute(f->Os, P, f->as);
     orc_gather_rows(f->as, P, D, f->X, P);
     for (int i = 0; i < P; ++i) f->lws[i] = f->logLikelihood;

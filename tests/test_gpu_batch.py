@@ -90,6 +90,9 @@ def test_batch_equals_single_filter(ctx, orc):
 
 
 def test_batch_autotune_and_graph(ctx, orc):
+    """The autotuned size is one of the candidates; graph replay at that size
+    reproduces the direct launches bit for bit (the workgroup size sets the
+    order of the weight sums, so both runs use the same size)."""
     T = 12
     y = _pz_data(orc, T)
     gm = capi.make_model("pz", precision=capi.F64)
@@ -97,14 +100,17 @@ def test_batch_autotune_and_graph(ctx, orc):
     Ns = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), 64))).astype(int)
     theta = _thetas(64, seed=2)
     res = []
+    block = 0
     for use_graph in (False, True):
         bf = capi.BatchFilter(ctx, gm, Ns, theta, T, 1.0, seed=8, ess_rel=0.5,
-                              use_graph=use_graph, block=0)
+                              use_graph=use_graph, block=block)
         bf.set_observations(y)
         bf.init()
         bf.step(T)
         blk, grid, ms = bf.tuning()
-        assert blk in (256, 512, 1024) and grid > 0 and (ms > 0).all()
+        if block == 0:
+            assert blk in (256, 512, 1024) and grid > 0 and (ms > 0).all()
+            block = blk
         res.append(bf.outputs())
         bf.close()
     np.testing.assert_array_equal(res[0]["logIncrements"], res[1]["logIncrements"])
