@@ -13,6 +13,17 @@ ESS / log-likelihood reduction.  Observations are synthetic (a truth
 trajectory simulated with the library itself); particle-steps/s counts
 N x steps over the timed region.
 
+Other workloads (`--workload`), each printing one JSON line of the same form:
+  pz     predator-prey bootstrap PF, N=4096 FP64, RK4 h=0.1, systematic
+         resampling every step, T=100 daily observations of log P;
+  smc2   batched inner filters of SMC^2 on the predator-prey model: 1024
+         parameter particles (mu, sigma, c, e from seeded log-normal priors),
+         inner N_j = floor(exp(U(log 256, log 16384))), one persistent launch
+         per observation interval replayed as a hipGraph, autotuned workgroup;
+  copy   a plain 1 GiB device-to-device copy: the achievable HBM ceiling, and
+         the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
+         counters into bytes.
+
 Prints one JSON line.  `--gpus N` with N > 1 exits: the build runs on one GPU.
 """
 import argparse
@@ -31,11 +42,16 @@ P_DEFAULT = 1 << 20
 T_SCHEDULE = 500
 DT = 0.05
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_transition launch at N=2^20, every step resampled: rocprofv3
+# --pmc FETCH_SIZE (x2.000, the factor the same counter shows on a 1 GiB copy)
+# plus WRITE_SIZE (x1.000), averaged over 7 launches.
+L96_TRAFFIC_BYTES = 410_305_843
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="l96", choices=["l96", "pz", "smc2", "copy"])
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--particles", type=int, default=P_DEFAULT)
@@ -78,13 +94,229 @@ def cpu_baseline(args, y, mean0):
     }
 
 
+def _timed(ctx, run):
+    import torch
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _line(value, args, ms, dtype, config, roofline, cpu):
+    line = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (truth trajectory and observations simulated by the library)",
+        "config": config,
+        "roofline": roofline,
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    return line
+
+
+PZ_P, PZ_T = 4096, 100
+
+
+def bench_pz(args):
+    """Predator-prey bootstrap PF: N=4096 FP64, RK4 h=0.1, resampling every step."""
+    from libbi_amd import capi, data
+    ctx = capi.Context(0)
+    gm = capi.make_model("pz", precision=capi.F64)
+    _, y = data.simulate(ctx, gm, [2.0, 2.0], PZ_T, 1.0, seed=2024)
+    steps = min(args.steps, PZ_T - args.warmup)
+
+    def filt(use_graph):
+        pf = capi.ParticleFilter(ctx, gm, PZ_P, PZ_T, 1.0, seed=77, ess_rel=1.0,
+                                 use_graph=use_graph)
+        pf.set_observations(y)
+        pf.init()
+        pf.step(args.warmup)
+        return pf
+
+    args.steps = steps
+    pf = filt(True)
+    el = _timed(ctx, lambda: pf.step(steps))
+    out = pf.outputs()
+    pf.close()
+    pf = filt(False)  # kernel durations with events on direct launches
+    pf.set_timing(True)
+    pf.step(steps)
+    tr_ms, tr_n = pf.kernel_time(0)
+    pf.close()
+    per = 2 * 2 * 8 + 8 + 4  # state read+write, log-weight write, ancestor read (every step resamples)
+    achieved = PZ_P * per * tr_n / (tr_ms * 1e-3) / 1e9 if tr_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        m = oracle.make_model("pz")
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            oracle.pf_run(m, y, PZ_P, 1.0, 1234, ess_rel=1.0)
+            reps += 1
+        cel = time.perf_counter() - t0
+        cpu = {"value": PZ_P * PZ_T * reps / cel, "unit": "particle-steps/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
+               "kind": "port",
+               "sample": f"PZ FP64 RK4 bootstrap PF N={PZ_P}, T={PZ_T}, {reps} runs ({cel:.1f} s)"}
+    cfg = {"workload": "predator-prey bootstrap PF, N=4096 FP64, RK4 h=0.1, systematic "
+                       "resampling every step, T=100 daily observations of log P",
+           "particles": PZ_P, "resampled_steps": int(out["resampled"].sum())}
+    roof = {"bound": "hbm", "kernel": "k_transition<PZ<double>,RK4>", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "avg_launch_ms": tr_ms / max(tr_n, 1), "launches": tr_n,
+            "note": "launch-latency bound at N=4096"}
+    print(json.dumps(_line(PZ_P * steps / el, args, 1e3 * el / steps, "f64", cfg, roof, cpu)),
+          flush=True)
+    ctx.close()
+
+
+SMC2_B = 1024
+
+
+def smc2_population(B=SMC2_B, seed=2024):
+    """Inner sizes N_j = floor(exp(U(log 256, log 16384))) and theta_j =
+    (mu, sigma, c, e) from log-normal priors around the model's values."""
+    rng = np.random.default_rng(seed)
+    N = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), B))).astype(np.int32)
+    mu = np.exp(rng.normal(np.log(0.5), 0.2, B))
+    sigma = np.exp(rng.normal(np.log(0.2), 0.2, B))
+    c = np.exp(rng.normal(np.log(0.25), 0.1, B))
+    e = np.exp(rng.normal(np.log(0.3), 0.1, B))
+    return N, np.stack([mu, sigma, c, e], axis=1)
+
+
+def bench_smc2(args):
+    """One observation interval of all 1024 inner filters per step."""
+    from libbi_amd import capi, data
+    ctx = capi.Context(0)
+    gm = capi.make_model("pz", precision=capi.F64)
+    T = args.warmup + args.steps
+    _, y = data.simulate(ctx, gm, [2.0, 2.0], T, 1.0, seed=2024)
+    N, theta = smc2_population()
+    total = int(N.sum())
+    bf = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, use_graph=True,
+                          block=0)
+    bf.set_observations(y)
+    bf.init()
+    bf.step(args.warmup)
+    el = _timed(ctx, lambda: bf.step(args.steps))
+    blk, grid, tune_ms = bf.tuning()
+    out = bf.outputs()
+    bf.close()
+    # kernel duration with events (direct launches at the tuned size, same steps)
+    bt = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, block=blk)
+    bt.set_observations(y)
+    bt.init()
+    bt.step(args.warmup)
+    bt.set_timing(True)
+    bt.step(args.steps)
+    k_ms, k_n = bt.kernel_time()
+    bt.close()
+    res = out["resampled"][:, args.warmup:T]
+    per_fixed = 2 * 2 * 8 + 2 * 8   # state read + write, log-weight read + write
+    per_res = 2 * 8 + 4             # weight scan write + read, ancestor write/read
+    bytes_total = per_fixed * total * args.steps + per_res * float((res * N[:, None]).sum())
+    achieved = bytes_total / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    # the same filters one after another through the single-filter path (a sample)
+    nseq = 16
+    seq_ps, seq_el = 0, 0.0
+    for j in range(nseq):
+        m = capi.make_model("pz", precision=capi.F64, mu=theta[j, 0], sigma=theta[j, 1],
+                            c=theta[j, 2], e=theta[j, 3])
+        pf = capi.ParticleFilter(ctx, m, int(N[j]), T, 1.0, 77, ess_rel=0.5, batch=j,
+                                 use_graph=True)
+        pf.set_observations(y)
+        pf.init()
+        pf.step(args.warmup)
+        seq_el += _timed(ctx, lambda: pf.step(args.steps))
+        seq_ps += int(N[j]) * args.steps
+        pf.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        om = oracle.make_model("pz")
+        t0 = time.perf_counter()
+        cps, j = 0, 0
+        while time.perf_counter() - t0 < 15.0 and j < SMC2_B:
+            for i, v in enumerate(theta[j]):
+                om.par[i] = v
+            oracle.pf_run(om, y, int(N[j]), 1.0, 77, ess_rel=0.5, batch=j)
+            cps += int(N[j]) * T
+            j += 1
+        cel = time.perf_counter() - t0
+        cpu = {"value": cps / cel, "unit": "particle-steps/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
+               "kind": "port",
+               "sample": f"the first {j} inner filters run one after another over T={T} "
+                         f"({cel:.1f} s), OpenMP over particles"}
+    cfg = {"workload": "batched SMC^2 inner filters, predator-prey, 1024 theta-particles, "
+                       "N_j log-uniform in [256,16384], FP64, RK4 h=0.1, systematic at "
+                       "ESS<0.5N_j, one persistent launch per step (hipGraph)",
+           "particles": total, "filters": SMC2_B, "block": blk, "grid": grid,
+           "tune_ms": [float(v) for v in tune_ms],
+           "resampled_fraction": float(res.mean()),
+           "sequential_pf_particle_steps_per_s": seq_ps / seq_el,
+           "sequential_pf_sample": f"filters 0..{nseq - 1} one after another through bihip_pf "
+                                   "(hipGraph per step)"}
+    roof = {"bound": "hbm", "kernel": "k_batch_step<PZ<double>,RK4>", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n}
+    print(json.dumps(_line(total * args.steps / el, args, 1e3 * el / args.steps, "f64", cfg,
+                           roof, cpu)), flush=True)
+    ctx.close()
+
+
+def bench_copy(args):
+    """Device-to-device copy of 2^28 floats (1 GiB read + 1 GiB written)."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(max(args.warmup, 1)):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2 * 4 * n / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "copy GB/s", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "device copy of 2^28 floats (1 GiB each way)",
+                                 "bytes_read": 4 * n, "bytes_written": 4 * n},
+                      "frac_of_peak": gbs / HBM_PEAK_GBS}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding of the filter is described in DESIGN.md but not built.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the filter is not "
+              "sharded across GPUs in this build.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pz":
+        return bench_pz(args)
+    if args.workload == "smc2":
+        return bench_smc2(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     import torch
     from libbi_amd import capi, data
 
@@ -149,7 +381,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": L96_TRAFFIC_BYTES if P == P_DEFAULT else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch, scaled by "
+                              "a 1 GiB copy (factors 2.0 / 1.0)",
+            "algorithmic_bytes_per_launch": bytes_total / max(len(resampled), 1),
             "avg_launch_ms": trans_ms / max(trans_n, 1),
             "launches": trans_n,
             "resample_chain_ms_per_step": res_ms / max(args.steps, 1),

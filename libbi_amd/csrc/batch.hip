@@ -438,7 +438,10 @@ static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
   int cus = 0, dev = 0;
   BIHIP_HIP(hipGetDevice(&dev));
   BIHIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int per_cu = std::max(1, occupancy_of(L.model, r));
+  // Workgroups beyond what is resident wait in the dispatcher and then take
+  // whatever the queue still holds (or leave at once); no workgroup waits on
+  // another, so oversubscription is safe and lets the hardware fill each CU.
+  const int per_cu = std::max(8, occupancy_of(L.model, r));
   r.grid = std::max(1, std::min(L.B, cus * per_cu));
   if (grid_out) *grid_out = r.grid;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, L.counter);
