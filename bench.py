@@ -22,7 +22,12 @@ Other workloads (`--workload`), each printing one JSON line of the same form:
          per observation interval replayed as a hipGraph, autotuned workgroup;
   copy   a plain 1 GiB device-to-device copy: the achievable HBM ceiling, and
          the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
-         counters into bytes.
+         counters into bytes;
+  surrogate  the L96 filter above with the DOPRI5 flow replaced by the
+         41-256-256-256-40 BF16 MFMA network, trained on the GPU on pairs from
+         the exact integrator before the timed region; reports its rate, the
+         exact filter's rate, and the log-likelihood difference over several
+         seeds (same observations, same noise streams).
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: the build runs on one GPU.
 """
@@ -51,7 +56,11 @@ L96_TRAFFIC_BYTES = 410_305_843
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="l96", choices=["l96", "pz", "smc2", "copy"])
+    ap.add_argument("--workload", default="l96",
+                    choices=["l96", "pz", "smc2", "copy", "surrogate"])
+    ap.add_argument("--train-pairs", type=int, default=1 << 22)
+    ap.add_argument("--train-steps", type=int, default=9000)
+    ap.add_argument("--seeds", type=int, default=3)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--particles", type=int, default=P_DEFAULT)
@@ -279,6 +288,81 @@ def bench_smc2(args):
     ctx.close()
 
 
+def bench_surrogate(args):
+    """Surrogate-flow filter against the exact-integrator filter."""
+    import torch
+    from libbi_amd import capi, data, surrogate
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    X, Y = surrogate.flow_pairs(ctx, args.train_pairs + (1 << 16), seed=3)
+    n = args.train_pairs
+    third = args.train_steps // 3
+    net, norm, rep = surrogate.train(ctx, steps=(third, third, args.train_steps - 2 * third),
+                                     pairs=(X[:n], Y[:n]), seed=3)
+    rmse, inc = surrogate.flow_error(ctx, net, norm, X[n:], Y[n:])
+    del X, Y
+    train_s = time.perf_counter() - t0
+    mean0 = data.l96_spinup(ctx, d=D)
+    gm = capi.make_model("l96", d=D, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                         init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+    T = args.warmup + args.steps
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=2024)
+    P = args.particles
+
+    def run(seed, use_surrogate, timed=False):
+        pf = capi.ParticleFilter(ctx, gm, P, T, DT, seed=seed, ess_rel=0.5, use_graph=True)
+        if use_surrogate:
+            pf.set_surrogate(net, norm)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(args.warmup)
+        el = _timed(ctx, lambda: pf.step(args.steps))
+        z = pf.term()
+        pf.close()
+        return z, el
+
+    res = {"exact": [], "surrogate": []}
+    el = {"exact": 0.0, "surrogate": 0.0}
+    for seed in range(args.seeds):
+        for kind in ("exact", "surrogate"):
+            z, e = run(100 + seed, kind == "surrogate")
+            res[kind].append(z)
+            el[kind] += e
+    diff = np.array(res["surrogate"]) - np.array(res["exact"])
+    # kernel duration of the fused surrogate step (HIP events on direct launches)
+    pf = capi.ParticleFilter(ctx, gm, P, T, DT, seed=100, ess_rel=0.5)
+    pf.set_surrogate(net, norm)
+    pf.set_observations(y)
+    pf.init(mean0)
+    pf.step(args.warmup)
+    pf.set_timing(True)
+    pf.step(args.steps)
+    k_ms, k_n = pf.kernel_time(0)
+    pf.close()
+    flops = 2.0 * sum(a * b for a, b in zip(surrogate.DIMS[:-1], surrogate.DIMS[1:])) * P
+    achieved = flops * k_n / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+    v_sur = P * args.steps * args.seeds / el["surrogate"]
+    v_ex = P * args.steps * args.seeds / el["exact"]
+    cfg = {"workload": "Lorenz-96 d=40 bootstrap PF step with the DOPRI5 flow replaced by a "
+                       "41-256-256-256-40 BF16 MFMA surrogate, N=2^20 FP32 state / FP64 "
+                       "weights, systematic resampling at ESS<0.5N",
+           "particles": P, "training_pairs": rep["pairs"], "training_steps": rep["steps"],
+           "training_seconds": train_s, "flow_rmse": rmse, "flow_increment_rms": inc,
+           "exact_particle_steps_per_s": v_ex, "speedup_vs_exact": v_sur / v_ex,
+           "logZ_exact": res["exact"], "logZ_surrogate": res["surrogate"],
+           "logZ_diff_mean": float(diff.mean()),
+           "logZ_diff_sd": float(diff.std(ddof=1)) if len(diff) > 1 else None,
+           "logZ_exact_sd_over_seeds": float(np.std(res["exact"], ddof=1))
+           if len(diff) > 1 else None}
+    roof = {"bound": "mfma", "kernel": "k_surrogate<40,256,4,step>", "achieved": achieved,
+            "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
+            "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n,
+            "flop_per_launch": flops}
+    print(json.dumps(_line(v_sur, args, 1e3 * el["surrogate"] / (args.steps * args.seeds),
+                           "bf16", cfg, roof, None)), flush=True)
+    ctx.close()
+
+
 def bench_copy(args):
     """Device-to-device copy of 2^28 floats (1 GiB read + 1 GiB written)."""
     import torch
@@ -317,6 +401,8 @@ def main():
         return bench_smc2(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     import torch
     from libbi_amd import capi, data
 

@@ -382,6 +382,64 @@ int bihip_smc2_run(bihip_smc2* s);
 int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* lw_host,
                        int* accepts_host, double* ess_host, int* resampled_host);
 
+/* ================================================================ multilayer perceptron */
+/*
+ * BF16 MFMA multilayer perceptron with FP32 accumulation: layers
+ * l = 0..nlayers-1 map dims[l] -> dims[l+1] as z = W_l a + b_l, GELU in its
+ * sigmoid form z sigma(1.702 z) on hidden layers, linear output.  No counterpart in the reference (LibBi has
+ * no learned components); it is the learned surrogate flow and guided
+ * proposal trained and run beside the filter.  Parameters are FP32 (Adam's
+ * master copy) with bfloat16 copies of the weights; the flat host layout is,
+ * per layer, W_l row-major [dims[l+1]][dims[l]] then b_l.  Inputs are rounded
+ * to bfloat16, hidden activations are bfloat16, outputs FP32.
+ * Initialisation: U(-1/sqrt(dims[l]), 1/sqrt(dims[l])) from keyed Philox draws.
+ */
+typedef struct bihip_mlp bihip_mlp;
+int bihip_mlp_create(bihip_ctx* ctx, int nlayers, const int* dims, uint64_t seed,
+                     bihip_mlp** mlp);
+int bihip_mlp_destroy(bihip_mlp* mlp);
+long bihip_mlp_nparams(const bihip_mlp* mlp);
+int bihip_mlp_set_params(bihip_mlp* mlp, const float* flat_host);  /* resets Adam's moments */
+int bihip_mlp_get_params(bihip_mlp* mlp, float* flat_host);
+/* Y[n x dims[L]] (leading dimension ldy) = MLP(X[n x dims[0]], leading dimension ldx), device pointers. */
+int bihip_mlp_forward(bihip_mlp* mlp, const float* X_dev, long ldx, long n, float* Y_dev,
+                      long ldy);
+/* Mean squared error over n samples x dims[L] outputs and its gradient (flat host layout). */
+int bihip_mlp_loss_grad(bihip_mlp* mlp, const float* X_dev, long ldx, const float* T_dev,
+                        long ldt, int n, double* loss_host, float* grad_host);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8, learning rate lr) on mean-squared-error
+ * minibatches of `batch` rows (a multiple of 64), drawn without replacement from
+ * a keyed permutation of the n rows per epoch; loss_host (nullable) gets each
+ * step's minibatch loss. */
+int bihip_mlp_train(bihip_mlp* mlp, const float* X_dev, long ldx, const float* T_dev, long ldt,
+                    long n, int batch, int steps, float lr, uint64_t seed, float* loss_host);
+
+/* ================================================================ learned surrogate flow */
+/*
+ * A 41-256-256-256-40 MLP standing in for the deterministic Lorenz-96 d=40
+ * flow over one observation interval (the DOPRI5 part of
+ * Model::transitionSamples, share/tt/cpp/model.hpp.tt:566-573):
+ *   u_i = (x_i - x_shift) / x_scale (i < 40), u_40 = (par - par_shift) / par_scale,
+ *   x'_i = x_i + dx_scale * MLP(u)_i.
+ * One fused kernel per step: BF16 MFMA layers with FP32 accumulation, weights
+ * staged in LDS, activations in registers.  No counterpart in the reference.
+ */
+typedef struct {
+  float x_shift, x_scale;
+  double par_shift, par_scale;
+  float dx_scale;
+} bihip_surrogate_norm;
+/* Xout = surrogate flow of Xin (FP32, column-major, d = 40 columns of leading dimension ld). */
+int bihip_surrogate_flow(bihip_ctx* ctx, bihip_mlp* mlp, const bihip_surrogate_norm* norm,
+                         double par, const float* Xin_dev, float* Xout_dev, int ld, int P);
+/*
+ * Filter steps of an FP32 Lorenz-96 d=40 bihip_pf use the surrogate flow
+ * (par = F) in place of the integrator; noise, observation density and
+ * resampling are unchanged.  mlp = NULL restores the exact integrator.  The
+ * weights are read when attached: attach again after further training.
+ */
+int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
+
 #ifdef __cplusplus
 }
 #endif

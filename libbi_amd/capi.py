@@ -49,6 +49,9 @@ EXPORTS = [
     "bihip_batch_accept",
     "bihip_smc2_create", "bihip_smc2_destroy", "bihip_smc2_set_observations", "bihip_smc2_run",
     "bihip_smc2_outputs",
+    "bihip_mlp_create", "bihip_mlp_destroy", "bihip_mlp_nparams", "bihip_mlp_set_params",
+    "bihip_mlp_get_params", "bihip_mlp_forward", "bihip_mlp_loss_grad", "bihip_mlp_train",
+    "bihip_surrogate_flow", "bihip_pf_set_surrogate",
 ]
 
 
@@ -92,6 +95,14 @@ class SMC2Config(ctypes.Structure):
     ]
 
 
+class SurrogateNorm(ctypes.Structure):
+    _fields_ = [
+        ("x_shift", ctypes.c_float), ("x_scale", ctypes.c_float),
+        ("par_shift", ctypes.c_double), ("par_scale", ctypes.c_double),
+        ("dx_scale", ctypes.c_float),
+    ]
+
+
 class PFConfig(ctypes.Structure):
     _fields_ = [
         ("P", ctypes.c_int), ("T", ctypes.c_int), ("resampler", ctypes.c_int),
@@ -121,6 +132,7 @@ def _declare(L):
     P = c.c_void_p
     i, u32, u64, d = c.c_int, c.c_uint32, c.c_uint64, c.c_double
     pd, pi, pl = c.POINTER(c.c_double), c.POINTER(c.c_int), c.POINTER(c.c_long)
+    pf = c.POINTER(c.c_float)
     pm, pc = c.POINTER(Model), c.POINTER(PFConfig)
     sig = {
         "bihip_ctx_create": [i, c.POINTER(P)],
@@ -191,6 +203,15 @@ def _declare(L):
         "bihip_smc2_set_observations": [P, pd],
         "bihip_smc2_run": [P],
         "bihip_smc2_outputs": [P, pd, pd, pd, pi, pd, pi],
+        "bihip_mlp_create": [P, i, pi, u64, c.POINTER(P)],
+        "bihip_mlp_destroy": [P],
+        "bihip_mlp_set_params": [P, pf],
+        "bihip_mlp_get_params": [P, pf],
+        "bihip_mlp_forward": [P, P, c.c_long, c.c_long, P, c.c_long],
+        "bihip_mlp_loss_grad": [P, P, c.c_long, P, c.c_long, i, pd, pf],
+        "bihip_mlp_train": [P, P, c.c_long, P, c.c_long, c.c_long, i, i, c.c_float, u64, pf],
+        "bihip_surrogate_flow": [P, P, c.POINTER(SurrogateNorm), d, P, P, i, i],
+        "bihip_pf_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -202,6 +223,8 @@ def _declare(L):
     L.bihip_uniform_host.restype = c.c_double
     L.bihip_batch_particles.argtypes = [P]
     L.bihip_batch_particles.restype = c.c_long
+    L.bihip_mlp_nparams.argtypes = [P]
+    L.bihip_mlp_nparams.restype = c.c_long
 
 
 def check(rc):
@@ -358,6 +381,25 @@ class ParticleFilter:
 
     def position(self):
         return lib().bihip_pf_position(self.h)
+
+    def state(self):
+        """Device copies of the current state (d x ld torch tensor), ld, and the
+        log-weights (P doubles)."""
+        import torch
+        Xp, ld, lwp = self.state_ptr()
+        dt = torch.float32 if self.model.precision == F32 else torch.float64
+        X = torch.empty((self.model.d, ld), dtype=dt, device="cuda")
+        lws = torch.empty(self.P, dtype=torch.float64, device="cuda")
+        check(lib().bihip_memcpy_device(self.ctx.h, X.data_ptr(), Xp, X.numel() * X.element_size()))
+        check(lib().bihip_memcpy_device(self.ctx.h, lws.data_ptr(), lwp, 8 * self.P))
+        self.ctx.sync()
+        return X, ld, lws
+
+    def set_surrogate(self, mlp, norm):
+        """Use the learned flow (capi.MLP, SurrogateNorm) in place of the integrator;
+        mlp=None restores the integrator."""
+        check(lib().bihip_pf_set_surrogate(self.h, mlp.h if mlp is not None else None,
+                                           ctypes.byref(norm) if norm is not None else None))
 
     def set_timing(self, on):
         check(lib().bihip_pf_set_timing(self.h, 1 if on else 0))
@@ -540,3 +582,74 @@ class SMC2:
                                        lw.ctypes.data_as(D), acc.ctypes.data_as(I),
                                        ess.ctypes.data_as(D), res.ctypes.data_as(I)))
         return dict(logZ=logZ.value, theta=theta, lw=lw, accepts=acc, ess=ess, resampled=res)
+
+
+class MLP:
+    """bihip_mlp: BF16 MFMA multilayer perceptron (GELU z sigmoid(1.702 z) on
+    hidden layers, linear output).  Device arrays are torch float32 tensors."""
+
+    def __init__(self, ctx, dims, seed=1):
+        self.ctx = ctx
+        self.dims = [int(d) for d in dims]
+        d = (ctypes.c_int * len(self.dims))(*self.dims)
+        h = ctypes.c_void_p()
+        check(lib().bihip_mlp_create(ctx.h, len(self.dims) - 1, d, seed, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().bihip_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nparams(self):
+        return lib().bihip_mlp_nparams(self.h)
+
+    def set_params(self, flat):
+        f = np.ascontiguousarray(flat, dtype=np.float32)
+        assert f.size == self.nparams()
+        check(lib().bihip_mlp_set_params(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def get_params(self):
+        f = np.zeros(self.nparams(), np.float32)
+        check(lib().bihip_mlp_get_params(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return f
+
+    def forward(self, X):
+        import torch
+        assert X.dtype == torch.float32 and X.is_cuda and X.dim() == 2 and X.is_contiguous()
+        Y = torch.empty((X.shape[0], self.dims[-1]), dtype=torch.float32, device=X.device)
+        check(lib().bihip_mlp_forward(self.h, X.data_ptr(), X.shape[1], X.shape[0],
+                                      Y.data_ptr(), Y.shape[1]))
+        return Y
+
+    def loss_grad(self, X, T):
+        assert X.is_contiguous() and T.is_contiguous()
+        loss = ctypes.c_double()
+        g = np.zeros(self.nparams(), np.float32)
+        check(lib().bihip_mlp_loss_grad(self.h, X.data_ptr(), X.shape[1], T.data_ptr(),
+                                        T.shape[1], X.shape[0], ctypes.byref(loss),
+                                        g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def train(self, X, T, batch, steps, lr, seed=1):
+        assert X.is_contiguous() and T.is_contiguous()
+        losses = np.zeros(max(steps, 1), np.float32)
+        check(lib().bihip_mlp_train(self.h, X.data_ptr(), X.shape[1], T.data_ptr(), T.shape[1],
+                                    X.shape[0], batch, steps, lr, seed,
+                                    losses.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return losses[:steps]
+
+    def surrogate_flow(self, norm, par, X):
+        """Out-of-place surrogate flow of X (torch float32, d x P column-major, i.e. shape (d, ld))."""
+        import torch
+        assert X.dtype == torch.float32 and X.is_contiguous()
+        Y = torch.empty_like(X)
+        check(lib().bihip_surrogate_flow(self.ctx.h, self.h, ctypes.byref(norm), par, X.data_ptr(),
+                                         Y.data_ptr(), X.shape[1], X.shape[1]))
+        return Y

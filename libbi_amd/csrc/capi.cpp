@@ -601,6 +601,10 @@ struct bihip_pf {
   size_t ev_used[3] = {0, 0, 0};
   // graphs (one per buffer parity)
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // learned surrogate flow in place of the integrator (bihip_pf_set_surrogate)
+  bool surr = false;
+  SurrogateLaunch sl{};
+  int nparts_exact = 0;
 };
 
 static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
@@ -684,7 +688,27 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
   T.dt = pf->cfg.dt;
   T.y_base = pf->y;
   T.nobs = pf->nobs;
-  if ((rc = launch_transition(T, s))) return rc;
+  if (pf->surr) {
+    SurrogateLaunch S = pf->sl;
+    S.Xin = T.Xin;
+    S.Xout = T.Xout;
+    S.ld = pf->ld;
+    S.P = P;
+    S.step = true;
+    S.as = pf->as;
+    S.lws = pf->lws;
+    S.partials = pf->partials;
+    S.dev = pf->dev;
+    S.seed = pf->cfg.seed;
+    S.batch = pf->cfg.batch;
+    S.t0 = pf->cfg.t0;
+    S.dt = pf->cfg.dt;
+    S.y_base = pf->y;
+    S.nobs = pf->nobs;
+    if ((rc = launch_surrogate(S, s))) return rc;
+  } else if ((rc = launch_transition(T, s))) {
+    return rc;
+  }
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
 
   // ---- BootstrapPF::correct bookkeeping
@@ -956,6 +980,43 @@ int bihip_pf_state(bihip_pf* pf, void** X, int* ld, double** lws) {
 }
 
 int bihip_pf_position(bihip_pf* pf) { return pf ? pf->k : -1; }
+
+int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_norm* norm) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  if (mlp != nullptr) {
+    if (norm == nullptr) return set_error(BIHIP_EINVAL, "NULL normalisation");
+    if (pf->model.model != BIHIP_MODEL_L96 || pf->model.d != 40 ||
+        pf->model.precision != BIHIP_F32)
+      return set_error(BIHIP_EINVAL, "the surrogate flow serves FP32 Lorenz-96 d=40 filters");
+    SurrogateLaunch S{};
+    S.net = mlp_view(mlp);
+    if (S.net.L != 4 || S.net.dims[0] != 41 || S.net.dims[4] != 40)
+      return set_error(BIHIP_EINVAL, "the surrogate must be a 41-256-256-256-40 network");
+    if ((rc = mlp_inference_weights(mlp, S.Winf))) return rc;
+    S.x_shift = norm->x_shift;
+    S.x_scale = norm->x_scale;
+    S.dx_scale = norm->dx_scale;
+    S.par = pf->model.par[0];
+    S.par_shift = norm->par_shift;
+    S.par_scale = norm->par_scale;
+    S.noise_sd = (float)pf->model.par[1];
+    S.obs_sd = pf->model.par[2];
+    pf->sl = S;
+  }
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  for (int i = 0; i < 2; ++i) {  // captured steps hold the previous transition kernel
+    if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
+    pf->graph[i] = nullptr;
+  }
+  if (pf->nparts_exact == 0) pf->nparts_exact = pf->nparts;
+  pf->surr = mlp != nullptr;
+  pf->nparts = pf->surr ? surrogate_blocks(pf->cfg.P) : pf->nparts_exact;
+  if (pf->nparts > pf->nparts_exact)
+    return set_error(BIHIP_EINVAL, "partials buffer too small for the surrogate kernel");
+  return BIHIP_OK;
+}
 
 int bihip_pf_set_timing(bihip_pf* pf, int enable) {
   if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");

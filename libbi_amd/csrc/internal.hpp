@@ -229,6 +229,74 @@ int launch_batch_copy(const long* dst_off, const long* src_off, const int* len, 
                       int B, long total, int d, int elem, const void* Xs, long lds,
                       const double* lws_s, void* Xd, long ldd, double* lws_d, hipStream_t s);
 
+// ---- BF16 MFMA multilayer perceptron (mlp.hip)
+enum { EPI_STORE = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_GRAD = 3 };
+struct GemmArgs {
+  const uint16_t* A;  // A(m,k) = TA ? A[k lda + m] : A[m lda + k]  (bf16)
+  long lda;
+  const uint16_t* B;  // B(k,n) = TB ? B[n ldb + k] : B[k ldb + n]  (bf16)
+  long ldb;
+  int M, N, K;        // multiples of 64, 64, 32
+  int ksplit;         // k range per split (multiple of 32); splits write Cf + z M ldc
+  const float* bias;  // [N] (EPI_BIAS_GELU, EPI_BIAS)
+  float* Cf;          // f32 out: product, pre-activation or linear output
+  long ldc;
+  uint16_t* Cb;       // bf16 out: GELU activation or back-propagated error
+  long ldcb;
+  const float* Zprev; // EPI_GELU_GRAD: pre-activation whose GELU derivative scales the product
+  long ldz;
+};
+int launch_gemm(const GemmArgs& g, bool ta, bool tb, int epi, hipStream_t s);
+int launch_rows_to_bf16(const float* X, long ldx, const long* idx, int n, int rows, int d,
+                        uint16_t* Y, int ldy, hipStream_t s);
+int launch_mse_grad(const float* Y, int ldy, const float* T, long ldt, const long* idx, int n,
+                    int rows, int d, float scale, uint16_t* dY, float* partial, int* nblocks,
+                    hipStream_t s);
+int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* out, hipStream_t s);
+int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s);
+int launch_adam(float* w, float* m1, float* m2, const float* grad, long n, float lr, float b1,
+                float b2, float eps, float bc1, float bc2, uint16_t* wbf, hipStream_t s);
+int launch_to_bf16(const float* x, long n, uint16_t* y, hipStream_t s);
+
+// Device parameters of an MLP (bf16 weights [pad_{l+1}][pad_l], FP32 biases),
+// for the fused inference kernels.
+constexpr int MLP_MAX_LAYERS = 6;
+struct MlpView {
+  int L;
+  int dims[MLP_MAX_LAYERS + 1], pad[MLP_MAX_LAYERS + 1];
+  const uint16_t* W[MLP_MAX_LAYERS];
+  const float* b[MLP_MAX_LAYERS];
+};
+MlpView mlp_view(const bihip_mlp* m);
+// bf16 weights with the input columns of layers l >= 1 permuted for the fused
+// kernels (rebuilt from the FP32 master copy); per-layer pointers into W.
+int mlp_inference_weights(bihip_mlp* m, const uint16_t* W[MLP_MAX_LAYERS]);
+
+// ---- learned surrogate flow (surrogate.hip)
+struct SurrogateLaunch {
+  MlpView net;
+  const uint16_t* Winf[MLP_MAX_LAYERS];
+  float x_shift, x_scale, dx_scale;  // u = (x - x_shift) / x_scale; x' = x + dx_scale y
+  double par, par_shift, par_scale;  // u_d = (par - par_shift) / par_scale
+  const void* Xin;
+  void* Xout;
+  int ld, P;
+  bool step;                         // filter step: gather, noise, observation, partials
+  const int* as;
+  double* lws;
+  double* partials;
+  const PFDevice* dev;
+  uint64_t seed;
+  uint32_t batch;
+  double t0, dt;
+  const double* y_base;
+  int nobs;
+  float noise_sd;
+  double obs_sd;
+};
+int surrogate_blocks(int P);
+int launch_surrogate(const SurrogateLaunch& L, hipStream_t s);
+
 // ---- host Philox4x32-10 block {index, step, tag | batch << 8, block}
 void philox_host_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
                        uint32_t batch, uint32_t block, uint32_t out[4]);

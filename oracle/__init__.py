@@ -45,6 +45,10 @@ PINNED = {
     "orc_gather_rows": "in-place == out-of-place gather",
     "orc_pf_run": "Kalman filter exact logZ on the LG model (essRel 0, 0.5, 1)",
     "orc_pf_step": "equals orc_pf_run step by step (orc_pf_run is built on it)",
+    "orc_mlp_forward": "torch float64 forward of the same network (GELU z sigmoid(1.702 z))",
+    "orc_mlp_loss_grad": "central finite differences of the FP64 loss; bf16 mode: rounding "
+                         "points checked against numpy bf16 rounding and torch with the same "
+                         "rounded weights",
     "orc_smc2_run": "no moves and no outer resampling: lw_j equals orc_pf_run with theta_j and "
                     "batch j; logZ = logsumexp(lw) - log B; LG model: evidence and posterior "
                     "mean of lambda against Gauss-Hermite quadrature of the Kalman likelihood; "
@@ -163,6 +167,13 @@ def _declare(L):
     L.orc_uniform_at.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, _D]
     L.orc_smc2_run.argtypes = [c.POINTER(Model), c.POINTER(SMC2Config), _I, _D, _D, _D, _D, _I,
                                _D, _I]
+    L.orc_bf16.argtypes = [c.c_double]
+    L.orc_bf16.restype = c.c_double
+    L.orc_mlp_nparams.argtypes = [c.c_int, _I]
+    L.orc_mlp_nparams.restype = c.c_size_t
+    L.orc_mlp_forward.argtypes = [c.c_int, _I, _D, _D, c.c_int, c.c_int, _D]
+    L.orc_mlp_loss_grad.argtypes = [c.c_int, _I, _D, _D, _D, c.c_int, c.c_int, _D]
+    L.orc_mlp_loss_grad.restype = c.c_double
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
 
@@ -454,6 +465,37 @@ def smc2_run(m, cfg, N, y):
                             ctypes.byref(logZ), _dp(theta), _dp(lw), _ip(acc), _dp(ess),
                             _ip(res))
     return dict(rc=rc, logZ=logZ.value, theta=theta, lw=lw, accepts=acc, ess=ess, resampled=res)
+
+
+def mlp_nparams(dims):
+    d = _i32(dims)
+    return lib().orc_mlp_nparams(len(d) - 1, _ip(d))
+
+
+def mlp_forward(dims, params, X, bf16=False):
+    d = _i32(dims)
+    X = _f64(X).reshape(-1, d[0])
+    Y = np.empty((X.shape[0], d[-1]))
+    lib().orc_mlp_forward(len(d) - 1, _ip(d), _dp(_f64(params)), _dp(X), X.shape[0],
+                          1 if bf16 else 0, _dp(Y))
+    return Y
+
+
+def mlp_loss_grad(dims, params, X, T, bf16=False):
+    d = _i32(dims)
+    X = _f64(X).reshape(-1, d[0])
+    T = _f64(T).reshape(-1, d[-1])
+    g = np.empty(mlp_nparams(dims))
+    loss = lib().orc_mlp_loss_grad(len(d) - 1, _ip(d), _dp(_f64(params)), _dp(X), _dp(T),
+                                   X.shape[0], 1 if bf16 else 0, _dp(g))
+    return loss, g
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even), as a float64 array."""
+    f = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((f + 0x7FFF + ((f >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
+    return r.astype(np.float64)
 
 
 def kalman_loglik_lg(m, dt, y):

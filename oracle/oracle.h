@@ -22,6 +22,7 @@
 #ifndef LIBBI_AMD_ORACLE_H
 #define LIBBI_AMD_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -161,6 +162,16 @@ typedef struct {
 int orc_smc2_run(const orc_model* base, const orc_smc2_cfg* c, const int* N, const double* y,
     double* logZ, double* theta_out, double* lw_out, int* accepts_out, double* ess_out,
     int* resampled_out);
+
+/* ---- multilayer perceptron (mlp.c): GELU z sigma(1.702 z) hidden layers, linear output,
+ * flat parameters [W_l (d_{l+1} x d_l, row-major), b_l] for l = 0..L-1 ---- */
+double orc_bf16(double x);
+size_t orc_mlp_nparams(int L, const int* dims);
+void orc_mlp_forward(int L, const int* dims, const double* params, const double* X, int n,
+    int bf16, double* Y);
+/* mean squared error over n samples x dims[L] outputs and its gradient */
+double orc_mlp_loss_grad(int L, const int* dims, const double* params, const double* X,
+    const double* T, int n, int bf16, double* grad);
 
 /* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
 double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);
