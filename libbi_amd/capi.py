@@ -6,6 +6,7 @@ loaded from the package tree; a missing library raises immediately -- there is
 no fallback implementation.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -285,6 +286,7 @@ class Context:
     def __init__(self, device=0, stream=None):
         import torch
         self.device = device
+        self._children = weakref.WeakSet()  # objects that must be destroyed before the context
         h = ctypes.c_void_p()
         check(lib().bihip_ctx_create(device, ctypes.byref(h)))
         self.h = h
@@ -292,8 +294,14 @@ class Context:
         self.stream = s
         check(lib().bihip_ctx_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
 
+    def adopt(self, obj):
+        self._children.add(obj)
+        return obj
+
     def close(self):
         if self.h:
+            for c in list(self._children):
+                c.close()
             lib().bihip_ctx_destroy(self.h)
             self.h = None
 
@@ -320,6 +328,7 @@ class ParticleFilter:
         check(lib().bihip_pf_create(ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
                                     ctypes.byref(h)))
         self.h = h
+        ctx.adopt(self)
         self.P, self.T, self.nobs = P, T, nobs(model)
 
     def close(self):
@@ -431,6 +440,7 @@ class BatchFilter:
             self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
             self.theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ntheta, ctypes.byref(h)))
         self.h = h
+        ctx.adopt(self)
         self.B, self.T, self.nobs = B, T, nobs(model)
 
     def close(self):
@@ -549,6 +559,7 @@ class SMC2:
                                       self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                       ctypes.byref(h)))
         self.h = h
+        ctx.adopt(self)
         self.B, self.T, self.nt, self.nobs = B, T, nt, nobs(model)
 
     def close(self):
@@ -595,6 +606,7 @@ class MLP:
         h = ctypes.c_void_p()
         check(lib().bihip_mlp_create(ctx.h, len(self.dims) - 1, d, seed, ctypes.byref(h)))
         self.h = h
+        ctx.adopt(self)
 
     def close(self):
         if self.h:

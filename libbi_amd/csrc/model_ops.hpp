@@ -42,10 +42,25 @@ __device__ __forceinline__ double obs_terms(const ModelL96<T, D, LPP>& m,
   const double inv = 1.0 / m.obs_sd;
   const double c = -HALF_LOG_TWO_PI - log(m.obs_sd);
   double s = 0.0;
+  if constexpr (DL % 2 == 0) {  // even slices start on an observed dimension
 #pragma unroll
-  for (int i = 0; i < DL; i += 2) {
-    const double z = (y[(lo + i) >> 1] - (double)x[i]) * inv;
-    s += -0.5 * (z * z) + c;
+    for (int i = 0; i < DL; i += 2) {
+      const double z = (y[(lo + i) >> 1] - (double)x[i]) * inv;
+      s += -0.5 * (z * z) + c;
+    }
+  } else {  // odd slices: the observed (even) dimensions alternate with the lane
+    const int o = lo & 1;
+#pragma unroll
+    for (int i = 0; i + 1 < DL; i += 2) {
+      const int j = i + o;  // local index of an observed dimension
+      const double xv = (double)(o ? x[i + 1] : x[i]);
+      const double z = (y[(lo + j) >> 1] - xv) * inv;
+      s += -0.5 * (z * z) + c;
+    }
+    if (o == 0) {  // the last element is observed only for even starts
+      const double z = (y[(lo + DL - 1) >> 1] - (double)x[DL - 1]) * inv;
+      s += -0.5 * (z * z) + c;
+    }
   }
   return s;
 }

@@ -124,7 +124,7 @@ template <class T, int D_, int LPP_>
 struct ModelL96 {
   static constexpr int D = D_, LPP = LPP_, DL = D_ / LPP_, NOBS = D_ / 2;
   static constexpr bool NOISE_FIRST = false;
-  static_assert(D_ % LPP_ == 0 && DL >= 2 && DL % 2 == 0, "lane slice must be even, >= 2");
+  static_assert(D_ % LPP_ == 0 && DL >= 2 && D_ % 2 == 0, "even dimension, lane slice >= 2");
   T F;
   double noise_sd, obs_sd;
   __device__ __forceinline__ explicit ModelL96(const ModelArgs& a) {

@@ -221,7 +221,7 @@ bool model_supported(const bihip_model* m) {
   if (m->precision != BIHIP_F32 && m->precision != BIHIP_F64) return false;
   switch (m->model) {
     case BIHIP_MODEL_PZ: return m->d == 2;
-    case BIHIP_MODEL_L96: return m->d == 8 || m->d == 40;
+    case BIHIP_MODEL_L96: return m->d == 8 || m->d == 40 || m->d == 200;
     case BIHIP_MODEL_LG: return m->d == 1;
   }
   return false;
@@ -237,6 +237,8 @@ static int with_model(const bihip_model* m, F& f) {
     case BIHIP_MODEL_L96:
       if (m->d == 40)
         return f64 ? f.template run<L96<double, 40, 4>>() : f.template run<L96<float, 40, 4>>();
+      if (m->d == 200)
+        return f64 ? f.template run<L96<double, 200, 8>>() : f.template run<L96<float, 200, 8>>();
       return f64 ? f.template run<L96<double, 8, 2>>() : f.template run<L96<float, 8, 2>>();
     default:
       return f64 ? f.template run<LG<double>>() : f.template run<LG<float>>();
@@ -245,7 +247,7 @@ static int with_model(const bihip_model* m, F& f) {
 
 int transition_blocks(const bihip_model* m, int P) {
   int lpp = 1;
-  if (m->model == BIHIP_MODEL_L96) lpp = m->d == 40 ? 4 : 2;
+  if (m->model == BIHIP_MODEL_L96) lpp = m->d == 200 ? 8 : (m->d == 40 ? 4 : 2);
   return ceil_div((long)P * lpp, TRANS_BLOCK);
 }
 

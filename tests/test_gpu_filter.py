@@ -63,6 +63,22 @@ def test_l96_filter_fp64_parity(ctx, orc):
     np.testing.assert_array_equal(anc, ref["ancestors"])
 
 
+def test_l96_d200_filter_fp64_parity(ctx, orc):
+    """Lorenz-96 d=200 (eight lanes per particle, 25 dimensions each: the
+    observed even dimensions alternate between lanes), FP64, N=2048, T=8."""
+    om0 = orc.make_model("l96", d=200)
+    mean0 = l96_spun_up(orc, d=200)
+    _, y = orc.simulate(om0, 7, 8, 0.05, mean0)
+    gm = capi.make_model("l96", d=200, precision=capi.F64)
+    om = oracle_model(orc, gm)
+    P = 2048
+    logZ, out, anc = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.5, mean0=mean0)
+    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0, want_ancestors=True)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-9)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
 @pytest.mark.parametrize("T", [1, 10])
 def test_l96_filter_fp32_mixed_precision(ctx, orc, T):
     """FP32 transitions with FP64 log-weights and logZ against the FP64 oracle on

@@ -130,6 +130,33 @@ def test_transition_parity(ctx, orc, torch_gpu, kind, integ, prec, quirk):
 
 
 @pytest.mark.parametrize("prec", [capi.F64, capi.F32])
+def test_l96_d200_transition_and_density(ctx, orc, torch_gpu, prec):
+    """d=200: eight lanes per particle with 25-dimension slices."""
+    P = 700
+    gm = capi.make_model("l96", d=200, precision=prec)
+    om = oracle_model(orc, gm)
+    mean0 = l96_spun_up(orc, d=200)
+    npdt = np.float64 if prec == capi.F64 else np.float32
+    dt = torch_gpu.float64 if prec == capi.F64 else torch_gpu.float32
+    X0 = orc.init_samples(om, 21, P, mean0=mean0).astype(npdt)
+    Xin, Xout = dev(X0), empty((200, P), dt)
+    capi.check(L().bihip_transition(ctx.h, ctypes.byref(gm), 5, 0, 4, 0.25, 0.30, P, _c(Xin),
+                                    _c(Xout), P, None, None))
+    ref = orc.transition(om, 5, 4, 0.25, 0.30, X0.astype(np.float64))
+    tol = _tol(prec, "l96")
+    got = host(Xout).astype(np.float64)
+    np.testing.assert_allclose(got, ref, rtol=tol, atol=tol)
+    rng = np.random.default_rng(4)
+    y = rng.normal(size=100) + 2.0
+    lws0 = rng.normal(size=P)
+    lws = dev(lws0)
+    capi.check(L().bihip_obs_logdensity(ctx.h, ctypes.byref(gm), _c(dev(y)), P, _c(Xout), P,
+                                        _c(lws)))
+    ref_l = orc.obs_logdensity(om, y, got, lws0)
+    np.testing.assert_allclose(host(lws), ref_l, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("prec", [capi.F64, capi.F32])
 def test_transition_gathers_rows(ctx, orc, torch_gpu, prec):
     P = 3000
     gm, om, X0, _ = _setup(orc, "l96", capi.DOPRI5, prec, 0, P)

@@ -32,7 +32,8 @@ def test_fused_flow_matches_gemm_path_and_oracle(ctx, torch_gpu, orc, P):
     Xd = torch.from_numpy(X).cuda()
     out = net.surrogate_flow(norm, F, Xd).cpu().numpy()
     U = np.concatenate([(X.T - np.float32(2.3)) * np.float32(1 / np.float32(3.6)),
-                        np.full((P, 1), 0.5, np.float32)], axis=1).astype(np.float32)
+                        np.full((P, 1), 0.5, np.float32)], axis=1)
+    U = np.ascontiguousarray(U, dtype=np.float32)
     y_gemm = net.forward(torch.from_numpy(U).cuda()).cpu().numpy()
     y_fused = (out - X).T / np.float32(0.3)
     y_orc = orc.mlp_forward(net.dims, net.get_params().astype(np.float64), U.astype(np.float64),
