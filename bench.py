@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--workload", default="l96",
                     choices=["l96", "pz", "smc2", "copy", "surrogate"])
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
-    ap.add_argument("--train-steps", type=int, default=9000)
+    ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--seeds", type=int, default=3)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
@@ -296,9 +296,9 @@ def bench_surrogate(args):
     t0 = time.perf_counter()
     X, Y = surrogate.flow_pairs(ctx, args.train_pairs + (1 << 16), seed=3)
     n = args.train_pairs
-    third = args.train_steps // 3
-    net, norm, rep = surrogate.train(ctx, steps=(third, third, args.train_steps - 2 * third),
-                                     pairs=(X[:n], Y[:n]), seed=3)
+    q = args.train_steps // 4
+    net, norm, rep = surrogate.train(ctx, steps=(q, q, q, args.train_steps - 3 * q),
+                                     lrs=(2e-3, 1e-3, 3e-4, 1e-4), pairs=(X[:n], Y[:n]), seed=3)
     rmse, inc = surrogate.flow_error(ctx, net, norm, X[n:], Y[n:])
     del X, Y
     train_s = time.perf_counter() - t0
@@ -347,7 +347,8 @@ def bench_surrogate(args):
                        "41-256-256-256-40 BF16 MFMA surrogate, N=2^20 FP32 state / FP64 "
                        "weights, systematic resampling at ESS<0.5N",
            "particles": P, "training_pairs": rep["pairs"], "training_steps": rep["steps"],
-           "training_seconds": train_s, "flow_rmse": rmse, "flow_increment_rms": inc,
+           "training_seconds": train_s, "final_minibatch_loss": rep["final_minibatch_loss"],
+           "flow_rmse": rmse, "flow_increment_rms": inc,
            "exact_particle_steps_per_s": v_ex, "speedup_vs_exact": v_sur / v_ex,
            "logZ_exact": res["exact"], "logZ_surrogate": res["surrogate"],
            "logZ_diff_mean": float(diff.mean()),

@@ -252,7 +252,8 @@ int launch_rows_to_bf16(const float* X, long ldx, const long* idx, int n, int ro
 int launch_mse_grad(const float* Y, int ldy, const float* T, long ldt, const long* idx, int n,
                     int rows, int d, float scale, uint16_t* dY, float* partial, int* nblocks,
                     hipStream_t s);
-int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* out, hipStream_t s);
+int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* part, float* out,
+                       hipStream_t s);
 int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s);
 int launch_adam(float* w, float* m1, float* m2, const float* grad, long n, float lr, float b1,
                 float b2, float eps, float bc1, float bc2, uint16_t* wbf, hipStream_t s);

@@ -123,13 +123,26 @@ __global__ __launch_bounds__(256) void k_mse_grad(const float* Y, int ldy, const
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// out[c] = sum over rows of bf16 dZ[r][c] (fixed order)
-__global__ void k_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// part[s][c] = sum of bf16 dZ[r][c] over rows r in [256 s, 256 s + 256), in a
+// fixed order: four row phases per column, then their sum.
+__global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* dZ, int ld, int rows,
+                                                     int cols, float* part) {
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * 256;
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s += bf2f(dZ[(size_t)r * ld + c]);
-  out[c] = s;
+  if (c < cols) {
+#pragma unroll 8
+    for (int i = g; i < 256; i += 4) {
+      const int r = r0 + i;
+      if (r < rows) s += bf2f(dZ[(size_t)r * ld + c]);
+    }
+  }
+  red[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && c < cols)
+    part[(size_t)blockIdx.y * cols + c] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
 // out[i] = sum_s part[s][i] (fixed order)
@@ -216,9 +229,14 @@ int launch_mse_grad(const float* Y, int ldy, const float* T, long ldt, const lon
   return BIHIP_OK;
 }
 
-int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(ceil_div(cols, 64)), dim3(64), 0, s, dZ, ld, rows, cols,
-                     out);
+int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* part, float* out,
+                       hipStream_t s) {
+  const int S = ceil_div(rows, 256);
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(ceil_div(cols, 64), S), dim3(256), 0, s, dZ, ld, rows,
+                     cols, part);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(cols, 256)), dim3(256), 0, s, part, S,
+                     (long)cols, out);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

@@ -15,6 +15,7 @@ namespace bihip {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32v4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest even (NaN stays NaN)
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -25,6 +26,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// two floats -> packed bfloat16 pair (low half = a), round to nearest even.
+// (The integer form keeps the fused kernels' register allocation free of
+// spills, which the compiler's v_cvt_pk_bf16_f32 lowering did not.)
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
 
 // GELU in its sigmoid form z sigma(1.702 z) (Hendrycks & Gimpel 2016, sec. 2):
 // one exponential and one reciprocal per activation, which keeps the hidden
@@ -40,6 +48,10 @@ __device__ __forceinline__ float gelu_grad_f(float z) {
 }
 
 __device__ __forceinline__ f32x16 mfma32(const u16x8& a, const u16x8& b, const f32x16& c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma32(const u16x8& a, const u32v4& b, const f32x16& c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }

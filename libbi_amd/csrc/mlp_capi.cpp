@@ -127,8 +127,8 @@ int backward(bihip_mlp* m, int rows, hipStream_t s) {
     if ((rc = launch_gemm(g, true, false, EPI_STORE, s))) return rc;
     const int S = (rows + g.ksplit - 1) / g.ksplit;
     if ((rc = launch_sum_splits(m->part, S, (long)g.M * g.N, m->G + m->offW[l], s))) return rc;
-    if ((rc = launch_colsum_bf16(m->dZ[l], m->pad[l + 1], rows, m->pad[l + 1], m->G + m->offb[l],
-                                 s)))
+    if ((rc = launch_colsum_bf16(m->dZ[l], m->pad[l + 1], rows, m->pad[l + 1], m->part,
+                                 m->G + m->offb[l], s)))
       return rc;
     if (l == 0) break;
     GemmArgs h{};  // dZ_{l-1} = (dZ_l W_l) * GELU'(Z_{l-1})

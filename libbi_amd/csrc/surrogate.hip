@@ -60,11 +60,12 @@ __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ W, ui
   }
 }
 
-// one hidden layer: nxt = bf16(GELU(W cur + b)), K inputs, H outputs
+// one hidden layer: nxt = bf16(GELU(W cur + b)), K inputs, H outputs; results
+// are packed two by two into the next layer's B operand.
 template <int K, int H, int LDSK>
 __device__ __forceinline__ void hidden_layer(const uint16_t* Ws, const float* __restrict__ bias,
-                                             const u16x8 (&cur)[H / 16], u16x8 (&nxt)[H / 16],
-                                             int lane) {
+                                             const uint32_t (&cur)[H / 16][4],
+                                             uint32_t (&nxt)[H / 16][4], int lane) {
   const int h = lane >> 5, row = lane & 31;
 #pragma unroll
   for (int t = 0; t < H / 32; ++t) {
@@ -74,12 +75,14 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* Ws, const float* __
 #pragma unroll
     for (int c = 0; c < K / 16; ++c) {
       const u16x8 wa = *reinterpret_cast<const u16x8*>(&Ws[(32 * t + row) * LDSK + 16 * c + 8 * h]);
-      acc = mfma32(wa, cur[c], acc);
+      const u32v4 bv = {cur[c][0], cur[c][1], cur[c][2], cur[c][3]};
+      acc = mfma32(wa, bv, acc);
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float z = acc[r] + bias[32 * t + acc_row(r, lane)];
-      nxt[2 * t + (r >> 3)][r & 7] = f2bf(gelu_f(z));
+    for (int r = 0; r < 16; r += 2) {
+      const int n = 32 * t + acc_row(r, lane);
+      nxt[2 * t + (r >> 3)][(r & 7) >> 1] =
+          pack_bf2(gelu_f(acc[r] + bias[n]), gelu_f(acc[r + 1] + bias[n + 1]));
     }
   }
 }
@@ -111,20 +114,22 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
   const int src = active ? (gather ? a.as[p] : p) : 0;
 
   // layer-0 operand: lane holds input features 16 c + 8 h + j of its particle
-  u16x8 cur[H / 16], nxt[H / 16];
+  uint32_t cur[H / 16][4], nxt[H / 16][4];
 #pragma unroll
   for (int c = 0; c < K0 / 16; ++c) {
+    float u[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int f = 16 * c + 8 * h + j;
-      float u = 0.0f;
+      u[j] = 0.0f;
       if (f < D) {
-        if (active) u = (a.Xin[(size_t)f * a.ld + src] - a.x_shift) * a.x_inv_scale;
+        if (active) u[j] = (a.Xin[(size_t)f * a.ld + src] - a.x_shift) * a.x_inv_scale;
       } else if (f == D) {
-        u = a.par_u;
+        u[j] = a.par_u;
       }
-      cur[c][j] = f2bf(u);
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) cur[c][j] = pack_bf2(u[2 * j], u[2 * j + 1]);
   }
 
   // hidden layers
@@ -139,7 +144,7 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
     if (l & 1) hidden_layer<H, H, LDSK>(Ws, a.b[l], nxt, cur, lane);
     else hidden_layer<H, H, LDSK>(Ws, a.b[l], cur, nxt, lane);
   }
-  const u16x8(&last)[H / 16] = ((NL - 2) & 1) ? cur : nxt;
+  const uint32_t(&last)[H / 16][4] = ((NL - 2) & 1) ? cur : nxt;
 
   // output layer: y for features 32 t + acc_row(r, lane)
   __syncthreads();
@@ -155,7 +160,8 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
     for (int c = 0; c < H / 16; ++c) {
       const u16x8 wa = *reinterpret_cast<const u16x8*>(
           &Ws[(32 * t + (lane & 31)) * LDSK + 16 * c + 8 * h]);
-      acc = mfma32(wa, last[c], acc);
+      const u32v4 bv = {last[c][0], last[c][1], last[c][2], last[c][3]};
+      acc = mfma32(wa, bv, acc);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) yv[t][r] = acc[r] + a.b[NL - 1][32 * t + acc_row(r, lane)];
