@@ -395,19 +395,25 @@ int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* 
  * Initialisation: U(-1/sqrt(dims[l]), 1/sqrt(dims[l])) from keyed Philox draws.
  */
 typedef struct bihip_mlp bihip_mlp;
+#define BIHIP_LOSS_MSE 0    /* mean squared error, targets dims[L] */
+#define BIHIP_LOSS_GAUSS 1  /* Gaussian NLL: outputs (mean, log variance), targets dims[L] / 2 */
 int bihip_mlp_create(bihip_ctx* ctx, int nlayers, const int* dims, uint64_t seed,
                      bihip_mlp** mlp);
 int bihip_mlp_destroy(bihip_mlp* mlp);
 long bihip_mlp_nparams(const bihip_mlp* mlp);
 int bihip_mlp_set_params(bihip_mlp* mlp, const float* flat_host);  /* resets Adam's moments */
 int bihip_mlp_get_params(bihip_mlp* mlp, float* flat_host);
+/* Training loss of bihip_mlp_loss_grad / bihip_mlp_train (default BIHIP_LOSS_MSE).
+ * BIHIP_LOSS_GAUSS: sum over targets of [(t - mu)^2 e^{-s} + s] / 2, mean over samples
+ * and targets, with mu the first dims[L]/2 outputs and s the last. */
+int bihip_mlp_set_loss(bihip_mlp* mlp, int loss);
 /* Y[n x dims[L]] (leading dimension ldy) = MLP(X[n x dims[0]], leading dimension ldx), device pointers. */
 int bihip_mlp_forward(bihip_mlp* mlp, const float* X_dev, long ldx, long n, float* Y_dev,
                       long ldy);
-/* Mean squared error over n samples x dims[L] outputs and its gradient (flat host layout). */
+/* The loss over n samples and its gradient (flat host layout). */
 int bihip_mlp_loss_grad(bihip_mlp* mlp, const float* X_dev, long ldx, const float* T_dev,
                         long ldt, int n, double* loss_host, float* grad_host);
-/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8, learning rate lr) on mean-squared-error
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8, learning rate lr) on the loss over
  * minibatches of `batch` rows (a multiple of 64), drawn without replacement from
  * a keyed permutation of the n rows per epoch; loss_host (nullable) gets each
  * step's minibatch loss. */
@@ -439,6 +445,37 @@ int bihip_surrogate_flow(bihip_ctx* ctx, bihip_mlp* mlp, const bihip_surrogate_n
  * weights are read when attached: attach again after further training.
  */
 int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
+
+/* ================================================================ learned guided proposal */
+/*
+ * Lorenz-96 d=200 filter steps with a learned proposal and the exact
+ * importance-weight correction: after the exact flow Phi(x) (DOPRI5, gathered
+ * through the ancestors),
+ *   x' = Phi + q (m + e^{s/2} eps),  (m, s) = MLP(u) per dimension,
+ *   u = ((Phi - x_shift) / x_scale, (y - H Phi) / innov_scale, (par - par_shift) / par_scale),
+ *   log w += log p(y | x') + log N(x'; Phi, q^2) - log N(x'; Phi + q m, q^2 e^s),
+ * with eps the transition-noise stream, so m = s = 0 is the bootstrap step.
+ * The network is 301-256-256-256-400 (m for the 200 dimensions, then s),
+ * trained with BIHIP_LOSS_GAUSS on (input u, target (x' - Phi) / q).  The
+ * reference's nearest pattern is the auxiliary-weight correction of BridgePF /
+ * LookaheadPF (share/src/bi/filter/BridgePF.hpp:114-144); the learned
+ * proposal has no counterpart there.
+ */
+typedef struct {
+  float x_shift, x_scale;
+  double innov_scale, par_shift, par_scale;
+} bihip_guided_norm;
+/* One proposal for P particles in place over Phi (X_dev, FP32, d = 200 columns
+ * of leading dimension ld), observations y_dev (100 values), Philox step
+ * `step`; inc_dev[p] gets the log-weight increment. */
+int bihip_guided_propose(bihip_ctx* ctx, bihip_mlp* mlp, const bihip_guided_norm* norm, double par,
+                         float* X_dev, int ld, int P, const double* y_dev, uint64_t seed,
+                         uint32_t batch, uint32_t step, double noise_sd, double obs_sd,
+                         double* inc_dev);
+/* Filter steps of an FP32 Lorenz-96 d=200 bihip_pf use the guided proposal
+ * (par = F); mlp = NULL restores the bootstrap step.  The weights are read when
+ * attached. */
+int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* norm);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
 RK4, DOPRI5 = 0, 1
 SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
 F32, F64 = 0, 1
+LOSS_MSE, LOSS_GAUSS = 0, 1
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
 TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER = 7, 8, 9, 10
 
@@ -52,7 +53,8 @@ EXPORTS = [
     "bihip_smc2_outputs",
     "bihip_mlp_create", "bihip_mlp_destroy", "bihip_mlp_nparams", "bihip_mlp_set_params",
     "bihip_mlp_get_params", "bihip_mlp_forward", "bihip_mlp_loss_grad", "bihip_mlp_train",
-    "bihip_surrogate_flow", "bihip_pf_set_surrogate",
+    "bihip_surrogate_flow", "bihip_pf_set_surrogate", "bihip_mlp_set_loss",
+    "bihip_guided_propose", "bihip_pf_set_guided",
 ]
 
 
@@ -101,6 +103,14 @@ class SurrogateNorm(ctypes.Structure):
         ("x_shift", ctypes.c_float), ("x_scale", ctypes.c_float),
         ("par_shift", ctypes.c_double), ("par_scale", ctypes.c_double),
         ("dx_scale", ctypes.c_float),
+    ]
+
+
+class GuidedNorm(ctypes.Structure):
+    _fields_ = [
+        ("x_shift", ctypes.c_float), ("x_scale", ctypes.c_float),
+        ("innov_scale", ctypes.c_double), ("par_shift", ctypes.c_double),
+        ("par_scale", ctypes.c_double),
     ]
 
 
@@ -213,6 +223,10 @@ def _declare(L):
         "bihip_mlp_train": [P, P, c.c_long, P, c.c_long, c.c_long, i, i, c.c_float, u64, pf],
         "bihip_surrogate_flow": [P, P, c.POINTER(SurrogateNorm), d, P, P, i, i],
         "bihip_pf_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
+        "bihip_mlp_set_loss": [P, i],
+        "bihip_guided_propose": [P, P, c.POINTER(GuidedNorm), d, P, i, i, P, u64, u32, u32, d, d,
+                                 P],
+        "bihip_pf_set_guided": [P, P, c.POINTER(GuidedNorm)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -409,6 +423,12 @@ class ParticleFilter:
         mlp=None restores the integrator."""
         check(lib().bihip_pf_set_surrogate(self.h, mlp.h if mlp is not None else None,
                                            ctypes.byref(norm) if norm is not None else None))
+
+    def set_guided(self, mlp, norm):
+        """Use the learned guided proposal (capi.MLP, GuidedNorm); mlp=None restores
+        the bootstrap step."""
+        check(lib().bihip_pf_set_guided(self.h, mlp.h if mlp is not None else None,
+                                        ctypes.byref(norm) if norm is not None else None))
 
     def set_timing(self, on):
         check(lib().bihip_pf_set_timing(self.h, 1 if on else 0))
@@ -622,6 +642,9 @@ class MLP:
     def nparams(self):
         return lib().bihip_mlp_nparams(self.h)
 
+    def set_loss(self, kind):
+        check(lib().bihip_mlp_set_loss(self.h, kind))
+
     def set_params(self, flat):
         f = np.ascontiguousarray(flat, dtype=np.float32)
         assert f.size == self.nparams()
@@ -665,3 +688,17 @@ class MLP:
         check(lib().bihip_surrogate_flow(self.ctx.h, self.h, ctypes.byref(norm), par, X.data_ptr(),
                                          Y.data_ptr(), X.shape[1], X.shape[1]))
         return Y
+
+    def guided_propose(self, norm, par, Phi, y, seed, step, noise_sd=0.1, obs_sd=1.0, batch=0):
+        """In-place proposal over Phi (torch float32, shape (200, ld)); returns the
+        log-weight increments (torch float64, P)."""
+        import torch
+        assert Phi.dtype == torch.float32 and Phi.is_contiguous()
+        P = Phi.shape[1]
+        yd = torch.as_tensor(np.ascontiguousarray(y, np.float64)).cuda()
+        inc = torch.empty(P, dtype=torch.float64, device=Phi.device)
+        check(lib().bihip_guided_propose(self.ctx.h, self.h, ctypes.byref(norm), par,
+                                         Phi.data_ptr(), P, P, yd.data_ptr(), seed, batch, step,
+                                         noise_sd, obs_sd, inc.data_ptr()))
+        self.ctx.sync()
+        return inc

@@ -605,6 +605,9 @@ struct bihip_pf {
   bool surr = false;
   SurrogateLaunch sl{};
   int nparts_exact = 0;
+  // learned guided proposal (bihip_pf_set_guided)
+  bool guided = false;
+  GuidedLaunch gl{};
 };
 
 static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
@@ -688,7 +691,28 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
   T.dt = pf->cfg.dt;
   T.y_base = pf->y;
   T.nobs = pf->nobs;
-  if (pf->surr) {
+  if (pf->guided) {
+    // exact flow of the gathered particles (no noise, no correction) into Xout,
+    // then the proposal and the corrected weights in place
+    TransLaunch F = T;
+    F.flow_only = 1;
+    F.y_base = nullptr;
+    F.lws = nullptr;
+    F.partials = nullptr;
+    if ((rc = launch_transition(F, s))) return rc;
+    GuidedLaunch G = pf->gl;
+    G.X = T.Xout;
+    G.ld = pf->ld;
+    G.P = P;
+    G.seed = pf->cfg.seed;
+    G.batch = pf->cfg.batch;
+    G.dev = pf->dev;
+    G.y_base = pf->y;
+    G.nobs = pf->nobs;
+    G.lws = pf->lws;
+    G.partials = pf->partials;
+    if ((rc = launch_guided(G, s))) return rc;
+  } else if (pf->surr) {
     SurrogateLaunch S = pf->sl;
     S.Xin = T.Xin;
     S.Xout = T.Xout;
@@ -990,6 +1014,7 @@ int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_n
     if (pf->model.model != BIHIP_MODEL_L96 || pf->model.d != 40 ||
         pf->model.precision != BIHIP_F32)
       return set_error(BIHIP_EINVAL, "the surrogate flow serves FP32 Lorenz-96 d=40 filters");
+    if (pf->guided) return set_error(BIHIP_ESTATE, "detach the guided proposal first");
     SurrogateLaunch S{};
     S.net = mlp_view(mlp);
     if (S.net.L != 4 || S.net.dims[0] != 41 || S.net.dims[4] != 40)
@@ -1015,6 +1040,44 @@ int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_n
   pf->nparts = pf->surr ? surrogate_blocks(pf->cfg.P) : pf->nparts_exact;
   if (pf->nparts > pf->nparts_exact)
     return set_error(BIHIP_EINVAL, "partials buffer too small for the surrogate kernel");
+  return BIHIP_OK;
+}
+
+int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* norm) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  if (mlp != nullptr) {
+    if (norm == nullptr) return set_error(BIHIP_EINVAL, "NULL normalisation");
+    if (pf->model.model != BIHIP_MODEL_L96 || pf->model.d != 200 ||
+        pf->model.precision != BIHIP_F32)
+      return set_error(BIHIP_EINVAL, "the guided proposal serves FP32 Lorenz-96 d=200 filters");
+    if (pf->surr) return set_error(BIHIP_ESTATE, "detach the surrogate flow first");
+    GuidedLaunch G{};
+    G.net = mlp_view(mlp);
+    if (G.net.L != 4 || G.net.dims[0] != 301 || G.net.dims[4] != 400)
+      return set_error(BIHIP_EINVAL, "the proposal must be a 301-256-256-256-400 network");
+    if ((rc = mlp_inference_weights(mlp, G.Winf, G.binf, 16))) return rc;
+    G.x_shift = norm->x_shift;
+    G.x_scale = norm->x_scale;
+    G.innov_scale = norm->innov_scale;
+    G.par = pf->model.par[0];
+    G.par_shift = norm->par_shift;
+    G.par_scale = norm->par_scale;
+    G.noise_sd = (float)pf->model.par[1];
+    G.obs_sd = pf->model.par[2];
+    pf->gl = G;
+  }
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  for (int i = 0; i < 2; ++i) {  // captured steps hold the previous kernels
+    if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
+    pf->graph[i] = nullptr;
+  }
+  if (pf->nparts_exact == 0) pf->nparts_exact = pf->nparts;
+  pf->guided = mlp != nullptr;
+  pf->nparts = pf->guided ? guided_blocks(pf->cfg.P) : pf->nparts_exact;
+  if (pf->nparts > pf->nparts_exact)
+    return set_error(BIHIP_EINVAL, "partials buffer too small for the guided kernel");
   return BIHIP_OK;
 }
 

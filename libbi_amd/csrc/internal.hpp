@@ -252,6 +252,9 @@ int launch_rows_to_bf16(const float* X, long ldx, const long* idx, int n, int ro
 int launch_mse_grad(const float* Y, int ldy, const float* T, long ldt, const long* idx, int n,
                     int rows, int d, float scale, uint16_t* dY, float* partial, int* nblocks,
                     hipStream_t s);
+int launch_gauss_grad(const float* Y, int ldy, const float* T, long ldt, const long* idx, int n,
+                      int rows, int m, float scale, uint16_t* dY, float* partial, int* nblocks,
+                      hipStream_t s);
 int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* part, float* out,
                        hipStream_t s);
 int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s);
@@ -271,7 +274,12 @@ struct MlpView {
 MlpView mlp_view(const bihip_mlp* m);
 // bf16 weights with the input columns of layers l >= 1 permuted for the fused
 // kernels (rebuilt from the FP32 master copy); per-layer pointers into W.
-int mlp_inference_weights(bihip_mlp* m, const uint16_t* W[MLP_MAX_LAYERS]);
+// out_interleave > 0: the output layer's rows are reordered in blocks of
+// out_interleave outputs, [m of outputs 16 t .., then s of the same outputs],
+// for networks whose outputs are a mean (first half) and a log variance;
+// b (nullable) then gets biases in the same row order.
+int mlp_inference_weights(bihip_mlp* m, const uint16_t* W[MLP_MAX_LAYERS],
+                          const float* b[MLP_MAX_LAYERS] = nullptr, int out_interleave = 0);
 
 // ---- learned surrogate flow (surrogate.hip)
 struct SurrogateLaunch {
@@ -297,6 +305,30 @@ struct SurrogateLaunch {
 };
 int surrogate_blocks(int P);
 int launch_surrogate(const SurrogateLaunch& L, hipStream_t s);
+
+// ---- learned guided proposal (guided.hip)
+struct GuidedLaunch {
+  MlpView net;
+  const uint16_t* Winf[MLP_MAX_LAYERS];
+  const float* binf[MLP_MAX_LAYERS];
+  float x_shift, x_scale;
+  double innov_scale, par, par_shift, par_scale;
+  void* X;            // Phi in, x' out (FP32)
+  int ld, P;
+  const double* y;    // standalone
+  uint64_t seed;
+  uint32_t batch, step;
+  float noise_sd;
+  double obs_sd;
+  const PFDevice* dev;  // filter mode when non-null
+  const double* y_base;
+  int nobs;
+  double* lws;
+  double* partials;
+  double* inc;        // standalone: log-weight increments
+};
+int guided_blocks(int P);
+int launch_guided(const GuidedLaunch& L, hipStream_t s);
 
 // ---- host Philox4x32-10 block {index, step, tag | batch << 8, block}
 void philox_host_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,

@@ -123,6 +123,42 @@ __global__ __launch_bounds__(256) void k_mse_grad(const float* Y, int ldy, const
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Gaussian negative log-likelihood: Y holds the mean (columns < m) and the log
+// variance (columns m .. 2m - 1) for m targets.  dY = bf16 of the gradient of
+// sum [(t - mu)^2 e^{-s} + s] / (2 n m); partial sums of the per-target terms.
+__global__ __launch_bounds__(256) void k_gauss_grad(const float* Y, int ldy, const float* T,
+                                                    long ldt, const long* idx, int n, int rows,
+                                                    int m, float scale, uint16_t* dY,
+                                                    float* partial) {
+  __shared__ float red[256];
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float term = 0.0f;
+  if (g < (long)rows * ldy) {
+    const int i = (int)(g / ldy), j = (int)(g % ldy);
+    float gr = 0.0f;
+    if (i < n && j < 2 * m) {
+      const int o = j < m ? j : j - m;
+      const long src = idx != nullptr ? idx[i] : i;
+      const float mu = Y[(size_t)i * ldy + o], lv = Y[(size_t)i * ldy + m + o];
+      const float e = T[src * ldt + o] - mu, iv = __expf(-lv), q = e * e * iv;
+      if (j < m) {
+        gr = -scale * e * iv;
+        term = 0.5f * (q + lv);
+      } else {
+        gr = 0.5f * scale * (1.0f - q);
+      }
+    }
+    dY[g] = f2bf(gr);
+  }
+  red[threadIdx.x] = term;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // part[s][c] = sum of bf16 dZ[r][c] over rows r in [256 s, 256 s + 256), in a
 // fixed order: four row phases per column, then their sum.
 __global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* dZ, int ld, int rows,
@@ -225,6 +261,17 @@ int launch_mse_grad(const float* Y, int ldy, const float* T, long ldt, const lon
   *nblocks = ceil_div(tot, 256);
   hipLaunchKernelGGL(k_mse_grad, dim3(*nblocks), dim3(256), 0, s, Y, ldy, T, ldt, idx, n, rows, d,
                      scale, dY, partial);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+int launch_gauss_grad(const float* Y, int ldy, const float* T, long ldt, const long* idx, int n,
+                      int rows, int m, float scale, uint16_t* dY, float* partial, int* nblocks,
+                      hipStream_t s) {
+  const long tot = (long)rows * ldy;
+  *nblocks = ceil_div(tot, 256);
+  hipLaunchKernelGGL(k_gauss_grad, dim3(*nblocks), dim3(256), 0, s, Y, ldy, T, ldt, idx, n, rows,
+                     m, scale, dY, partial);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

@@ -59,4 +59,86 @@ __device__ __forceinline__ f32x16 mfma32(const u16x8& a, const u32v4& b, const f
 // row of accumulator r of lane l in a 32 x 32 tile
 __device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
+// ---------------------------------------------------------------- fused inference
+// Layers computed transposed, out^T = W in^T, for 32 particles per wave: the
+// weights are the A operand (rows staged in LDS with row stride LDSK), the
+// activations the B operand held in registers as packed bfloat16 pairs
+// (act[c][0..3] = inputs 16 c + 8 h + 0..7 of the lane's particle, h = lane / 32).
+// An output tile's accumulator lands in the lane that owns the particle, with
+// the rows of each 16-block in the order [0-3, 8-11, 4-7, 12-15] across the
+// two half-waves; layers fed by such outputs store their input columns in that
+// order (the inference weights).
+
+// rows [row0, row0 + NROWS) of a row-major [.][K] bf16 matrix into Ws
+template <int NROWS, int K, int LDSK, int THREADS>
+__device__ __forceinline__ void stage_rows(const uint16_t* __restrict__ W, int row0,
+                                           uint16_t* Ws) {
+  constexpr int PER_ROW = K / 8, V = NROWS * PER_ROW;
+  for (int v = threadIdx.x; v < V; v += THREADS) {
+    const int row = v / PER_ROW, c8 = (v - row * PER_ROW) * 8;
+    *reinterpret_cast<u16x8*>(&Ws[row * LDSK + c8]) =
+        *reinterpret_cast<const u16x8*>(&W[(size_t)(row0 + row) * K + c8]);
+  }
+}
+
+// rows [row0, row0 + nrows) (a run-time count) of a row-major [.][K] bf16 matrix
+template <int K, int LDSK, int THREADS>
+__device__ __forceinline__ void stage_rows_n(const uint16_t* __restrict__ W, int row0, int nrows,
+                                             uint16_t* Ws) {
+  constexpr int PER_ROW = K / 8;
+  const int V = nrows * PER_ROW;
+  for (int v = threadIdx.x; v < V; v += THREADS) {
+    const int row = v / PER_ROW, c8 = (v - row * PER_ROW) * 8;
+    *reinterpret_cast<u16x8*>(&Ws[row * LDSK + c8]) =
+        *reinterpret_cast<const u16x8*>(&W[(size_t)(row0 + row) * K + c8]);
+  }
+}
+
+// output tiles [T0, T1) of a GELU layer with KC input chunks: Ws holds rows
+// from 32 T0 on; out[2 t + s] receives the packed activations of tile t
+template <int KC, int NOUTC, int LDSK, int T0, int T1>
+__device__ __forceinline__ void gelu_tiles(const uint16_t* Ws, const float* __restrict__ bias,
+                                           const uint32_t (&in)[KC][4],
+                                           uint32_t (&out)[NOUTC][4], int lane) {
+  const int h = lane >> 5, row = lane & 31;
+#pragma unroll
+  for (int t = T0; t < T1; ++t) {
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      const u16x8 wa =
+          *reinterpret_cast<const u16x8*>(&Ws[(32 * (t - T0) + row) * LDSK + 16 * c + 8 * h]);
+      const u32v4 bv = {in[c][0], in[c][1], in[c][2], in[c][3]};
+      acc = mfma32(wa, bv, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+      const int n = 32 * t + acc_row(r, lane);
+      out[2 * t + (r >> 3)][(r & 7) >> 1] =
+          pack_bf2(gelu_f(acc[r] + bias[n]), gelu_f(acc[r + 1] + bias[n + 1]));
+    }
+  }
+}
+
+// one linear output tile t of a layer with KC input chunks (Ws holds rows from 32 t0)
+template <int KC, int LDSK>
+__device__ __forceinline__ f32x16 linear_tile(const uint16_t* Ws, int t, int t0,
+                                              const float* __restrict__ bias,
+                                              const uint32_t (&in)[KC][4], int lane) {
+  const int h = lane >> 5, row = lane & 31;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = bias[32 * t + acc_row(r, lane)];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) {
+    const u16x8 wa =
+        *reinterpret_cast<const u16x8*>(&Ws[(32 * (t - t0) + row) * LDSK + 16 * c + 8 * h]);
+    const u32v4 bv = {in[c][0], in[c][1], in[c][2], in[c][3]};
+    acc = mfma32(wa, bv, acc);
+  }
+  return acc;
+}
+
 }  // namespace bihip

@@ -30,6 +30,7 @@ struct bihip_mlp {
   float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
   uint16_t* Wbf = nullptr;       // bf16 weights, same offsets as the W part
   long adam_t = 0;
+  int loss = BIHIP_LOSS_MSE;
   // workspace for batches of up to cap rows
   int cap = 0;
   uint16_t* X0 = nullptr;
@@ -40,6 +41,7 @@ struct bihip_mlp {
   long part_n = 0;
   float* lossp = nullptr;
   uint16_t* Winf = nullptr;      // inference copy of the weights (permuted columns)
+  float* binf = nullptr;         // inference copy of the biases (row order of Winf)
 };
 
 namespace {
@@ -149,14 +151,28 @@ int backward(bihip_mlp* m, int rows, hipStream_t s) {
   return BIHIP_OK;
 }
 
-// loss and output error for n valid rows (rows padded); returns the summed squares
+// output error dL/dy of the configured loss for n valid rows (rows padded);
+// per-block partial sums of the loss terms into m->lossp
+int output_error(bihip_mlp* m, const float* T, long ldt, const long* idx, int n, int rows,
+                 int* nblocks, hipStream_t s) {
+  const int L = m->L, d = m->dims[L];
+  if (m->loss == BIHIP_LOSS_GAUSS)
+    return launch_gauss_grad(m->Z[L - 1], m->pad[L], T, ldt, idx, n, rows, d / 2,
+                             1.0f / ((float)n * (float)(d / 2)), m->dZ[L - 1], m->lossp, nblocks,
+                             s);
+  return launch_mse_grad(m->Z[L - 1], m->pad[L], T, ldt, idx, n, rows, d,
+                         2.0f / ((float)n * (float)d), m->dZ[L - 1], m->lossp, nblocks, s);
+}
+
+int targets_per_sample(const bihip_mlp* m) {
+  return m->loss == BIHIP_LOSS_GAUSS ? m->dims[m->L] / 2 : m->dims[m->L];
+}
+
+// loss and output error; *sumsq gets the summed loss terms
 int loss_and_error(bihip_mlp* m, const float* T, long ldt, const long* idx, int n, int rows,
                    double* sumsq, hipStream_t s) {
-  const int L = m->L, d = m->dims[L];
-  const float scale = 2.0f / ((float)n * (float)d);
   int nb = 0;
-  int rc = launch_mse_grad(m->Z[L - 1], m->pad[L], T, ldt, idx, n, rows, d, scale, m->dZ[L - 1],
-                           m->lossp, &nb, s);
+  int rc = output_error(m, T, ldt, idx, n, rows, &nb, s);
   if (rc) return rc;
   if (sumsq) {
     std::vector<float> h(nb);
@@ -208,27 +224,48 @@ uint16_t host_f2bf(float f) {  // round to nearest even
 }  // namespace
 
 namespace bihip {
-int mlp_inference_weights(bihip_mlp* m, const uint16_t* W[MLP_MAX_LAYERS]) {
+int mlp_inference_weights(bihip_mlp* m, const uint16_t* W[MLP_MAX_LAYERS],
+                          const float* b[MLP_MAX_LAYERS], int out_interleave) {
   if (m->L > MLP_MAX_LAYERS) return set_error(BIHIP_EINVAL, "too many layers");
+  const int L = m->L, dout = m->dims[L];
+  if (out_interleave > 0 && (dout % 2 || 2 * ((dout / 2 + out_interleave - 1) / out_interleave) *
+                                               out_interleave > m->pad[L]))
+    return set_error(BIHIP_EINVAL, "output interleave does not fit the padded width");
   int rc;
   if (m->Winf == nullptr && (rc = dalloc(&m->Winf, m->nW))) return rc;
+  if (m->binf == nullptr && (rc = dalloc(&m->binf, m->ntot - m->nW))) return rc;
   hipStream_t s = ctx_stream_of(m->ctx);
-  std::vector<float> P(m->nW);
-  BIHIP_HIP(hipMemcpyAsync(P.data(), m->P, sizeof(float) * m->nW, hipMemcpyDeviceToHost, s));
+  std::vector<float> P(m->ntot);
+  BIHIP_HIP(hipMemcpyAsync(P.data(), m->P, sizeof(float) * m->ntot, hipMemcpyDeviceToHost, s));
   BIHIP_HIP(hipStreamSynchronize(s));
-  std::vector<uint16_t> H(m->nW);
-  for (int l = 0; l < m->L; ++l) {
+  std::vector<uint16_t> H(m->nW, 0);
+  std::vector<float> B(m->ntot - m->nW, 0.0f);
+  for (int l = 0; l < L; ++l) {
     const int K = m->pad[l], N = m->pad[l + 1];
-    for (int n = 0; n < N; ++n)
+    for (int n = 0; n < N; ++n) {
+      // source row: natural, or for the interleaved output layer block t of
+      // 2 out_interleave rows = [m of outputs t*oi .., s of the same outputs]
+      int srow = n;
+      if (l == L - 1 && out_interleave > 0) {
+        const int oi = out_interleave, t = n / (2 * oi), r = n % (2 * oi);
+        const int dim = t * oi + (r < oi ? r : r - oi);
+        srow = dim < dout / 2 ? (r < oi ? dim : dout / 2 + dim) : -1;
+      }
+      const long boff = m->offb[l] - m->nW;
+      B[boff + n] = srow >= 0 ? P[m->offb[l] + srow] : 0.0f;
       for (int k = 0; k < K; ++k) {
         // position 16 c + p holds input 16 c + perm(p), perm = [0-3, 8-11, 4-7, 12-15]
         const int c = k >> 4, q = k & 15;
         const int src = l == 0 ? k : 16 * c + 8 * ((q & 7) >> 2) + 4 * (q >> 3) + (q & 3);
-        H[m->offW[l] + (long)n * K + k] = host_f2bf(P[m->offW[l] + (long)n * K + src]);
+        H[m->offW[l] + (long)n * K + k] =
+            srow >= 0 ? host_f2bf(P[m->offW[l] + (long)srow * K + src]) : (uint16_t)0;
       }
+    }
     W[l] = m->Winf + m->offW[l];
+    if (b != nullptr) b[l] = m->binf + (m->offb[l] - m->nW);
   }
   BIHIP_HIP(hipMemcpyAsync(m->Winf, H.data(), sizeof(uint16_t) * m->nW, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipMemcpyAsync(m->binf, B.data(), sizeof(float) * B.size(), hipMemcpyHostToDevice, s));
   BIHIP_HIP(hipStreamSynchronize(s));
   return BIHIP_OK;
 }
@@ -316,6 +353,7 @@ int bihip_mlp_destroy(bihip_mlp* m) {
   dfree(m->M2);
   dfree(m->Wbf);
   dfree(m->Winf);
+  dfree(m->binf);
   delete m;
   return BIHIP_OK;
 }
@@ -382,7 +420,7 @@ int bihip_mlp_loss_grad(bihip_mlp* m, const float* X_dev, long ldx, const float*
                         int n, double* loss_host, float* grad_host) {
   if (m == nullptr || X_dev == nullptr || T_dev == nullptr)
     return set_error(BIHIP_EINVAL, "NULL argument");
-  if (n <= 0 || ldx < m->dims[0] || ldt < m->dims[m->L])
+  if (n <= 0 || ldx < m->dims[0] || ldt < targets_per_sample(m))
     return set_error(BIHIP_EINVAL, "bad n or leading dimensions");
   int rc = ctx_enter_device(m->ctx);
   if (rc) return rc;
@@ -395,7 +433,7 @@ int bihip_mlp_loss_grad(bihip_mlp* m, const float* X_dev, long ldx, const float*
       (rc = loss_and_error(m, T_dev, ldt, nullptr, n, rows, &sq, s)) ||
       (rc = backward(m, rows, s)))
     return rc;
-  if (loss_host) *loss_host = sq / ((double)n * m->dims[m->L]);
+  if (loss_host) *loss_host = sq / ((double)n * targets_per_sample(m));
   if (grad_host) {
     std::vector<float> G(m->ntot);
     BIHIP_HIP(hipMemcpyAsync(G.data(), m->G, sizeof(float) * m->ntot, hipMemcpyDeviceToHost, s));
@@ -409,6 +447,7 @@ int bihip_mlp_train(bihip_mlp* m, const float* X_dev, long ldx, const float* T_d
                     long n, int batch, int steps, float lr, uint64_t seed, float* loss_host) {
   if (m == nullptr || X_dev == nullptr || T_dev == nullptr)
     return set_error(BIHIP_EINVAL, "NULL argument");
+  if (ldt < targets_per_sample(m)) return set_error(BIHIP_EINVAL, "ldt below the targets per sample");
   if (n <= 0 || batch <= 0 || batch % 64 || batch > n || steps < 0 || !(lr > 0.0f))
     return set_error(BIHIP_EINVAL, "need 0 < batch <= n, batch a multiple of 64, lr > 0");
   int rc = ctx_enter_device(m->ctx);
@@ -447,12 +486,10 @@ int bihip_mlp_train(bihip_mlp* m, const float* X_dev, long ldx, const float* T_d
     const long* idx = perm_d + pos;
     pos += batch;
     int nb = 0;
-    const int L = m->L, d = m->dims[L];
     if ((rc = launch_rows_to_bf16(X_dev, ldx, idx, batch, batch, m->dims[0], m->X0, m->pad[0],
                                   s)) ||
         (rc = forward(m, batch, s)) ||
-        (rc = launch_mse_grad(m->Z[L - 1], m->pad[L], T_dev, ldt, idx, batch, batch, d,
-                              2.0f / ((float)batch * (float)d), m->dZ[L - 1], m->lossp, &nb, s)) ||
+        (rc = output_error(m, T_dev, ldt, idx, batch, batch, &nb, s)) ||
         (rc = launch_sum_splits(m->lossp, nb, 1, loss_d + it, s)) || (rc = backward(m, batch, s)))
       break;
     ++m->adam_t;
@@ -469,13 +506,23 @@ int bihip_mlp_train(bihip_mlp* m, const float* X_dev, long ldx, const float* T_d
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       rc = set_error(BIHIP_EHIP, "reading the losses failed");
-    const float inv = 1.0f / ((float)batch * (float)m->dims[m->L]);
+    const float inv = 1.0f / ((float)batch * (float)targets_per_sample(m));
     for (int it = 0; it < steps && rc == BIHIP_OK; ++it) loss_host[it] = h[it] * inv;
   }
   (void)hipStreamSynchronize(s);
   dfree(perm_d);
   dfree(loss_d);
   return rc;
+}
+
+int bihip_mlp_set_loss(bihip_mlp* m, int loss) {
+  if (m == nullptr) return set_error(BIHIP_EINVAL, "NULL network");
+  if (loss != BIHIP_LOSS_MSE && loss != BIHIP_LOSS_GAUSS)
+    return set_error(BIHIP_EINVAL, "unknown loss %d", loss);
+  if (loss == BIHIP_LOSS_GAUSS && m->dims[m->L] % 2)
+    return set_error(BIHIP_EINVAL, "the Gaussian loss needs an even output width");
+  m->loss = loss;
+  return BIHIP_OK;
 }
 
 int bihip_surrogate_flow(bihip_ctx* ctx, bihip_mlp* m, const bihip_surrogate_norm* norm,
@@ -501,6 +548,37 @@ int bihip_surrogate_flow(bihip_ctx* ctx, bihip_mlp* m, const bihip_surrogate_nor
   L.P = P;
   L.step = false;
   return launch_surrogate(L, ctx_stream_of(ctx));
+}
+
+int bihip_guided_propose(bihip_ctx* ctx, bihip_mlp* m, const bihip_guided_norm* norm, double par,
+                         float* X_dev, int ld, int P, const double* y_dev, uint64_t seed,
+                         uint32_t batch, uint32_t step, double noise_sd, double obs_sd,
+                         double* inc_dev) {
+  int rc = ctx_enter_device(ctx);
+  if (rc) return rc;
+  if (m == nullptr || norm == nullptr || X_dev == nullptr || y_dev == nullptr || inc_dev == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
+  GuidedLaunch G{};
+  G.net = mlp_view(m);
+  if ((rc = mlp_inference_weights(m, G.Winf, G.binf, 16))) return rc;
+  G.x_shift = norm->x_shift;
+  G.x_scale = norm->x_scale;
+  G.innov_scale = norm->innov_scale;
+  G.par = par;
+  G.par_shift = norm->par_shift;
+  G.par_scale = norm->par_scale;
+  G.X = X_dev;
+  G.ld = ld;
+  G.P = P;
+  G.y = y_dev;
+  G.seed = seed;
+  G.batch = batch;
+  G.step = step;
+  G.noise_sd = (float)noise_sd;
+  G.obs_sd = obs_sd;
+  G.inc = inc_dev;
+  return launch_guided(G, ctx_stream_of(ctx));
 }
 
 }  // extern "C"

@@ -50,43 +50,6 @@ namespace {
 
 constexpr int SG_THREADS = 512, SG_TILE = 256;
 
-template <int N, int K, int LDSK>
-__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ W, uint16_t* Ws) {
-  constexpr int PER_ROW = K / 8, V = N * PER_ROW;
-  for (int v = threadIdx.x; v < V; v += SG_THREADS) {
-    const int row = v / PER_ROW, c8 = (v - row * PER_ROW) * 8;
-    *reinterpret_cast<u16x8*>(&Ws[row * LDSK + c8]) =
-        *reinterpret_cast<const u16x8*>(&W[(size_t)row * K + c8]);
-  }
-}
-
-// one hidden layer: nxt = bf16(GELU(W cur + b)), K inputs, H outputs; results
-// are packed two by two into the next layer's B operand.
-template <int K, int H, int LDSK>
-__device__ __forceinline__ void hidden_layer(const uint16_t* Ws, const float* __restrict__ bias,
-                                             const uint32_t (&cur)[H / 16][4],
-                                             uint32_t (&nxt)[H / 16][4], int lane) {
-  const int h = lane >> 5, row = lane & 31;
-#pragma unroll
-  for (int t = 0; t < H / 32; ++t) {
-    f32x16 acc;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < K / 16; ++c) {
-      const u16x8 wa = *reinterpret_cast<const u16x8*>(&Ws[(32 * t + row) * LDSK + 16 * c + 8 * h]);
-      const u32v4 bv = {cur[c][0], cur[c][1], cur[c][2], cur[c][3]};
-      acc = mfma32(wa, bv, acc);
-    }
-#pragma unroll
-    for (int r = 0; r < 16; r += 2) {
-      const int n = 32 * t + acc_row(r, lane);
-      nxt[2 * t + (r >> 3)][(r & 7) >> 1] =
-          pack_bf2(gelu_f(acc[r] + bias[n]), gelu_f(acc[r + 1] + bias[n + 1]));
-    }
-  }
-}
-
 template <int D, int H, int NL, bool STEP>
 __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
   constexpr int K0 = (D + 1 + 63) / 64 * 64;  // padded input width (state + parameter)
@@ -114,7 +77,7 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
   const int src = active ? (gather ? a.as[p] : p) : 0;
 
   // layer-0 operand: lane holds input features 16 c + 8 h + j of its particle
-  uint32_t cur[H / 16][4], nxt[H / 16][4];
+  uint32_t in0[K0 / 16][4], cur[H / 16][4], nxt[H / 16][4];
 #pragma unroll
   for (int c = 0; c < K0 / 16; ++c) {
     float u[8];
@@ -129,26 +92,26 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) cur[c][j] = pack_bf2(u[2 * j], u[2 * j + 1]);
+    for (int j = 0; j < 4; ++j) in0[c][j] = pack_bf2(u[2 * j], u[2 * j + 1]);
   }
 
   // hidden layers
-  stage_weights<H, K0, LDSK>(a.W[0], Ws);
+  stage_rows<H, K0, LDSK, SG_THREADS>(a.W[0], 0, Ws);
   __syncthreads();
-  hidden_layer<K0, H, LDSK>(Ws, a.b[0], cur, nxt, lane);
+  gelu_tiles<K0 / 16, H / 16, LDSK, 0, H / 32>(Ws, a.b[0], in0, nxt, lane);
 #pragma unroll
   for (int l = 1; l < NL - 1; ++l) {
     __syncthreads();  // every wave is done with the previous layer's weights
-    stage_weights<H, H, LDSK>(a.W[l], Ws);
+    stage_rows<H, H, LDSK, SG_THREADS>(a.W[l], 0, Ws);
     __syncthreads();
-    if (l & 1) hidden_layer<H, H, LDSK>(Ws, a.b[l], nxt, cur, lane);
-    else hidden_layer<H, H, LDSK>(Ws, a.b[l], cur, nxt, lane);
+    if (l & 1) gelu_tiles<H / 16, H / 16, LDSK, 0, H / 32>(Ws, a.b[l], nxt, cur, lane);
+    else gelu_tiles<H / 16, H / 16, LDSK, 0, H / 32>(Ws, a.b[l], cur, nxt, lane);
   }
   const uint32_t(&last)[H / 16][4] = ((NL - 2) & 1) ? cur : nxt;
 
   // output layer: y for features 32 t + acc_row(r, lane)
   __syncthreads();
-  stage_weights<DO, H, LDSK>(a.W[NL - 1], Ws);
+  stage_rows<DO, H, LDSK, SG_THREADS>(a.W[NL - 1], 0, Ws);
   __syncthreads();
   float yv[DO / 32][16];
 #pragma unroll

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
     t1 = a.t0 + k * a.dt;
     t2 = a.t0 + (k + 1) * a.dt;
     step = (uint32_t)(k + 1);
-    y = a.y_base + (size_t)k * a.nobs;
+    y = a.y_base != nullptr ? a.y_base + (size_t)k * a.nobs : nullptr;  // null: flow only
   }
 
   double lw = __builtin_nan("");

@@ -172,7 +172,7 @@ def _declare(L):
     L.orc_mlp_nparams.argtypes = [c.c_int, _I]
     L.orc_mlp_nparams.restype = c.c_size_t
     L.orc_mlp_forward.argtypes = [c.c_int, _I, _D, _D, c.c_int, c.c_int, _D]
-    L.orc_mlp_loss_grad.argtypes = [c.c_int, _I, _D, _D, _D, c.c_int, c.c_int, _D]
+    L.orc_mlp_loss_grad.argtypes = [c.c_int, _I, _D, _D, _D, c.c_int, c.c_int, c.c_int, _D]
     L.orc_mlp_loss_grad.restype = c.c_double
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
@@ -481,14 +481,17 @@ def mlp_forward(dims, params, X, bf16=False):
     return Y
 
 
-def mlp_loss_grad(dims, params, X, T, bf16=False):
+LOSS_MSE, LOSS_GAUSS = 0, 1
+
+
+def mlp_loss_grad(dims, params, X, T, bf16=False, loss=LOSS_MSE):
     d = _i32(dims)
     X = _f64(X).reshape(-1, d[0])
-    T = _f64(T).reshape(-1, d[-1])
+    T = _f64(T).reshape(-1, d[-1] // 2 if loss == LOSS_GAUSS else d[-1])
     g = np.empty(mlp_nparams(dims))
-    loss = lib().orc_mlp_loss_grad(len(d) - 1, _ip(d), _dp(_f64(params)), _dp(X), _dp(T),
-                                   X.shape[0], 1 if bf16 else 0, _dp(g))
-    return loss, g
+    val = lib().orc_mlp_loss_grad(len(d) - 1, _ip(d), _dp(_f64(params)), _dp(X), _dp(T),
+                                  X.shape[0], loss, 1 if bf16 else 0, _dp(g))
+    return val, g
 
 
 def bf16(x):

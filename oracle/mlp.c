@@ -10,7 +10,11 @@
  * followed by b_l in the flat parameter vector; hidden layers apply GELU in
  * its sigmoid form a = z sigma(1.702 z) (Hendrycks & Gimpel 2016, sec. 2), the
  * last layer is linear.
- * Loss = sum over samples and outputs of (y - t)^2 / (n d_L).
+ * Losses: ORC_LOSS_MSE: sum over samples and outputs of (y - t)^2 / (n d_L);
+ * ORC_LOSS_GAUSS: the outputs are a mean mu (first m = d_L / 2) and a log
+ * variance s (last m) of a diagonal Gaussian for m targets, and the loss is
+ * the negative log-likelihood without its constant,
+ * sum over samples and targets of [(t - mu)^2 e^{-s} + s] / (2 n m).
  *
  * With bf16 = 1 the oracle rounds, as the GPU kernels do, the inputs, the
  * weights, the hidden activations and the back-propagated errors dL/dy,
@@ -104,10 +108,11 @@ void orc_mlp_forward(int L, const int* dims, const double* params, const double*
 }
 
 double orc_mlp_loss_grad(int L, const int* dims, const double* params, const double* X,
-                         const double* T, int n, int bf16, double* grad) {
+                         const double* T, int n, int loss_kind, int bf16, double* grad) {
   const size_t np = layer_offset(dims, L);
   const int W = max_width(L, dims), dout = dims[L];
-  const double scale = 2.0 / ((double)n * dout);
+  const int gauss = loss_kind == ORC_LOSS_GAUSS, m = gauss ? dout / 2 : dout;
+  const double scale = gauss ? 1.0 / ((double)n * m) : 2.0 / ((double)n * dout);
   double loss = 0.0;
   memset(grad, 0, sizeof(double) * np);
   /* per-sample contributions summed in sample order for determinism */
@@ -121,11 +126,22 @@ double orc_mlp_loss_grad(int L, const int* dims, const double* params, const dou
   double* da = (double*)malloc(sizeof(double) * W);
   for (int s = 0; s < n; ++s) {
     forward_one(L, dims, params, X + (size_t)s * dims[0], bf16, a, z);
-    const double* t = T + (size_t)s * dout;
-    for (int o = 0; o < dout; ++o) {
-      const double e = z[L - 1][o] - t[o];
-      loss += e * e;
-      dz[o] = bf16 ? orc_bf16(scale * e) : scale * e;
+    const double* t = T + (size_t)s * m;
+    if (gauss) {
+      for (int o = 0; o < m; ++o) {
+        const double mu = z[L - 1][o], lv = z[L - 1][m + o];
+        const double e = t[o] - mu, q = e * e * exp(-lv);
+        loss += 0.5 * (q + lv);
+        const double gm = -scale * e * exp(-lv), gs = 0.5 * scale * (1.0 - q);
+        dz[o] = bf16 ? orc_bf16(gm) : gm;
+        dz[m + o] = bf16 ? orc_bf16(gs) : gs;
+      }
+    } else {
+      for (int o = 0; o < dout; ++o) {
+        const double e = z[L - 1][o] - t[o];
+        loss += e * e;
+        dz[o] = bf16 ? orc_bf16(scale * e) : scale * e;
+      }
     }
     for (int l = L - 1; l >= 0; --l) {
       const double* Wl = params + layer_offset(dims, l);
@@ -158,5 +174,5 @@ double orc_mlp_loss_grad(int L, const int* dims, const double* params, const dou
   free(z);
   free(dz);
   free(da);
-  return loss / ((double)n * dout);
+  return loss / ((double)n * m);
 }

@@ -169,9 +169,11 @@ double orc_bf16(double x);
 size_t orc_mlp_nparams(int L, const int* dims);
 void orc_mlp_forward(int L, const int* dims, const double* params, const double* X, int n,
     int bf16, double* Y);
-/* mean squared error over n samples x dims[L] outputs and its gradient */
+/* loss over n samples and its gradient: ORC_LOSS_MSE (targets dims[L]) or
+ * ORC_LOSS_GAUSS (outputs: mean then log variance; targets dims[L] / 2) */
+enum { ORC_LOSS_MSE = 0, ORC_LOSS_GAUSS = 1 };
 double orc_mlp_loss_grad(int L, const int* dims, const double* params, const double* X,
-    const double* T, int n, int bf16, double* grad);
+    const double* T, int n, int loss_kind, int bf16, double* grad);
 
 /* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
 double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);

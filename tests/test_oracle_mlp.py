@@ -55,6 +55,27 @@ def test_gradient_matches_finite_differences(orc):
         assert g[i] == pytest.approx((lp - lm) / (2 * h), rel=1e-6, abs=1e-9), i
 
 
+def test_gaussian_loss_gradient_matches_finite_differences(orc):
+    """Gaussian negative log-likelihood: outputs (mean, log variance)."""
+    dims = [6, 9, 8]  # 4 targets
+    p = _params(orc, dims, seed=8, scale=0.3)
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(7, dims[0]))
+    T = rng.normal(size=(7, 4))
+    val, g = orc.mlp_loss_grad(dims, p, X, T, loss=orc.LOSS_GAUSS)
+    Y = orc.mlp_forward(dims, p, X)
+    mu, lv = Y[:, :4], Y[:, 4:]
+    assert val == pytest.approx(np.mean(0.5 * ((T - mu) ** 2 * np.exp(-lv) + lv)), rel=1e-12)
+    h = 1e-6
+    for i in rng.choice(len(p), 50, replace=False):
+        q = p.copy()
+        q[i] += h
+        lp, _ = orc.mlp_loss_grad(dims, q, X, T, loss=orc.LOSS_GAUSS)
+        q[i] -= 2 * h
+        lm, _ = orc.mlp_loss_grad(dims, q, X, T, loss=orc.LOSS_GAUSS)
+        assert g[i] == pytest.approx((lp - lm) / (2 * h), rel=1e-6, abs=1e-9), i
+
+
 def test_bf16_rounding_points(orc):
     """bf16 mode: inputs, weights, hidden activations and back-propagated
     errors rounded to bfloat16, accumulations and biases in FP64."""
