@@ -23,6 +23,12 @@ Other workloads (`--workload`), each printing one JSON line of the same form:
   copy   a plain 1 GiB device-to-device copy: the achievable HBM ceiling, and
          the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
          counters into bytes;
+  guided the Lorenz-96 d=200 filter with a learned guided proposal
+         (301-256-256-256-400 BF16 MFMA network, trained on the GPU before the
+         timed region) and the exact importance-weight correction, N=2^18
+         particles for each of 64 forcing values F in [7.5, 8.5], stepped
+         together; reports its rate and, against the bootstrap filter with the
+         same N, the ESS and the log-likelihood spread over two seeds;
   surrogate  the L96 filter above with the DOPRI5 flow replaced by the
          41-256-256-256-40 BF16 MFMA network, trained on the GPU on pairs from
          the exact integrator before the timed region; reports its rate, the
@@ -57,7 +63,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="l96",
-                    choices=["l96", "pz", "smc2", "copy", "surrogate"])
+                    choices=["l96", "pz", "smc2", "copy", "surrogate", "guided"])
+    ap.add_argument("--thetas", type=int, default=64)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--seeds", type=int, default=3)
@@ -364,6 +371,104 @@ def bench_surrogate(args):
     ctx.close()
 
 
+def bench_guided(args):
+    """Guided-proposal filters for 64 forcing values against bootstrap filters."""
+    import torch
+    from libbi_amd import capi, data, guided
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+    Fs_train = [7.5, 7.8333, 8.1667, 8.5]
+    chunks = [guided.triples(ctx, (1 << 21) // len(Fs_train), F=F, seed=11 + i)
+              for i, F in enumerate(Fs_train)]
+    Phi = torch.cat([c[0] for c in chunks])
+    norm = guided.normalisation(Phi)
+    norm.par_scale = 0.5
+    Us, Ts = [], []
+    for (ph, xn, yy), F in zip(chunks, Fs_train):
+        U, T = guided.training_tensors(ph, xn, yy, norm, F=F)
+        Us.append(U)
+        Ts.append(T)
+    U, T = torch.cat(Us), torch.cat(Ts)
+    del chunks, Phi, Us, Ts
+    net = capi.MLP(ctx, guided.DIMS, seed=5)
+    net.set_loss(capi.LOSS_GAUSS)
+    losses = [net.train(U, T, batch=4096, steps=s_, lr=lr, seed=9 + k)
+              for k, (s_, lr) in enumerate(zip((3000, 2000, 1000), (1e-3, 3e-4, 1e-4)))]
+    del U, T
+    train_s = time.perf_counter() - t0
+    P = 1 << 18
+    Tn = args.warmup + args.steps
+    Fs = np.linspace(7.5, 8.5, args.thetas)
+    mean0 = data.l96_spinup(ctx, d=guided.D)
+    _, y = data.simulate(ctx, guided.model(8.0), mean0, Tn, DT, seed=2024)
+
+    def filters(use, seed):
+        out = []
+        for j, F in enumerate(Fs):
+            pf = capi.ParticleFilter(ctx, guided.model(F), P, Tn, DT, seed=seed, ess_rel=0.5,
+                                     batch=j, use_graph=True)
+            pf.set_observations(y)
+            pf.init(mean0)
+            if use:
+                pf.set_guided(net, norm)
+            out.append(pf)
+        return out
+
+    res = {}
+    for use in (False, True):
+        for seed in (1, 2):
+            fl = filters(use, seed)
+            for pf in fl:
+                pf.step(args.warmup)
+
+            def run():
+                for _ in range(args.steps):
+                    for pf in fl:
+                        pf.step(1)
+
+            el = _timed(ctx, run)
+            zs = np.array([pf.term() for pf in fl])
+            ess = np.array([pf.outputs()["ess"][args.warmup:Tn] for pf in fl])
+            for pf in fl:
+                pf.close()
+            res[(use, seed)] = (el, zs, ess)
+    # kernel duration of the proposal (events on direct launches of one filter)
+    pf = capi.ParticleFilter(ctx, guided.model(8.0), P, Tn, DT, seed=1, ess_rel=0.5)
+    pf.set_observations(y)
+    pf.init(mean0)
+    pf.set_guided(net, norm)
+    pf.set_timing(True)
+    pf.step(args.steps)
+    k_ms, k_n = pf.kernel_time(3)
+    step_ms, _ = pf.kernel_time(0)
+    pf.close()
+    flops = 2.0 * sum(a * b for a, b in zip(guided.DIMS[:-1], guided.DIMS[1:])) * P
+    ps = P * args.thetas * args.steps
+    v_g = 2 * ps / (res[(True, 1)][0] + res[(True, 2)][0])
+    v_b = 2 * ps / (res[(False, 1)][0] + res[(False, 2)][0])
+    spread = {u: float(np.std(res[(u, 1)][1] - res[(u, 2)][1]) / np.sqrt(2)) for u in (False, True)}
+    ess = {u: float(np.mean([res[(u, s_)][2].mean() for s_ in (1, 2)])) / P for u in (False, True)}
+    cfg = {"workload": "Lorenz-96 d=200 filters with a learned guided proposal (301-256-256-256-400 "
+                       "BF16 MFMA) and exact weight correction, N=2^18 FP32 state / FP64 weights "
+                       f"for each of {args.thetas} forcing values F in [7.5, 8.5], DOPRI5 "
+                       "rtol=atol=1e-5, systematic resampling at ESS<0.5N",
+           "particles_per_filter": P, "filters": args.thetas, "training_seconds": train_s,
+           "final_minibatch_loss": float(losses[-1][-100:].mean()),
+           "bootstrap_particle_steps_per_s": v_b,
+           "mean_ess_fraction_guided": ess[True], "mean_ess_fraction_bootstrap": ess[False],
+           "logZ_seed_sd_guided": spread[True], "logZ_seed_sd_bootstrap": spread[False],
+           "logZ_mean_guided_minus_bootstrap": float(np.mean(
+               (res[(True, 1)][1] + res[(True, 2)][1]) / 2 - (res[(False, 1)][1] + res[(False, 2)][1]) / 2))}
+    roof = {"bound": "mfma", "kernel": "k_guided<filter>", "achieved": flops / (k_ms / k_n * 1e-3) / 1e12
+            if k_ms > 0 else 0.0, "peak": 2500.0, "unit": "TFLOP/s", "traffic": None,
+            "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n, "flop_per_launch": flops,
+            "flow_plus_proposal_ms_per_step": step_ms / max(k_n, 1)}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    print(json.dumps(_line(v_g, args, 1e3 * (res[(True, 1)][0] + res[(True, 2)][0]) / (2 * args.steps),
+                           "f32", cfg, roof, None)), flush=True)
+    ctx.close()
+
+
 def bench_copy(args):
     """Device-to-device copy of 2^28 floats (1 GiB read + 1 GiB written)."""
     import torch
@@ -404,6 +509,8 @@ def main():
         return bench_copy(args)
     if args.workload == "surrogate":
         return bench_surrogate(args)
+    if args.workload == "guided":
+        return bench_guided(args)
     import torch
     from libbi_amd import capi, data
 

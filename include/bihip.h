@@ -266,7 +266,8 @@ int bihip_pf_position(bihip_pf* pf);
 /*
  * Kernel timing with HIP events on the filter's stream: enable, then query
  * the summed milliseconds and launch count of one kernel class
- * (0 = transition, 1 = resample chain, 2 = weight reduction).
+ * (0 = transition, 1 = resample chain, 2 = weight reduction, 3 = the guided
+ * proposal kernel alone, which class 0 also covers).
  */
 int bihip_pf_set_timing(bihip_pf* pf, int enable);
 int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_host);
@@ -381,6 +382,23 @@ int bihip_smc2_run(bihip_smc2* s);
  * log-weights [B], accepted moves [T + 1], outer ESS [T], resampled [T]. */
 int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* lw_host,
                        int* accepts_host, double* ess_host, int* resampled_host);
+
+/*
+ * PMMH (MarginalMH::sample, share/src/bi/sampler/MarginalMH.hpp:198-266) with
+ * B independent chains of C iterations (the first is the prior draw), the B
+ * proposals of an iteration filtered together in one bihip_batch.  Uses the
+ * bihip_smc2_config fields B, T, ntheta, resampler, essRel_inner, t0, dt,
+ * seed, prior_mean, prior_sd, prop_sd, use_graph, block.
+ */
+typedef struct bihip_pmmh bihip_pmmh;
+int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg, int C,
+                      const int* N_host, bihip_pmmh** s);
+int bihip_pmmh_destroy(bihip_pmmh* s);
+int bihip_pmmh_set_observations(bihip_pmmh* s, const double* y_host);
+int bihip_pmmh_run(bihip_pmmh* s);
+/* theta [C x B x ntheta], log-likelihood [C x B], accepted moves per chain [B]
+ * (the initial draw counted), any may be NULL. */
+int bihip_pmmh_outputs(bihip_pmmh* s, double* theta_host, double* ll_host, int* accepted_host);
 
 /* ================================================================ multilayer perceptron */
 /*

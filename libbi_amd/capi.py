@@ -55,6 +55,8 @@ EXPORTS = [
     "bihip_mlp_get_params", "bihip_mlp_forward", "bihip_mlp_loss_grad", "bihip_mlp_train",
     "bihip_surrogate_flow", "bihip_pf_set_surrogate", "bihip_mlp_set_loss",
     "bihip_guided_propose", "bihip_pf_set_guided",
+    "bihip_pmmh_create", "bihip_pmmh_destroy", "bihip_pmmh_set_observations", "bihip_pmmh_run",
+    "bihip_pmmh_outputs",
 ]
 
 
@@ -227,6 +229,11 @@ def _declare(L):
         "bihip_guided_propose": [P, P, c.POINTER(GuidedNorm), d, P, i, i, P, u64, u32, u32, d, d,
                                  P],
         "bihip_pf_set_guided": [P, P, c.POINTER(GuidedNorm)],
+        "bihip_pmmh_create": [P, pm, c.POINTER(SMC2Config), i, pi, c.POINTER(P)],
+        "bihip_pmmh_destroy": [P],
+        "bihip_pmmh_set_observations": [P, pd],
+        "bihip_pmmh_run": [P],
+        "bihip_pmmh_outputs": [P, pd, pd, pi],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -613,6 +620,59 @@ class SMC2:
                                        lw.ctypes.data_as(D), acc.ctypes.data_as(I),
                                        ess.ctypes.data_as(D), res.ctypes.data_as(I)))
         return dict(logZ=logZ.value, theta=theta, lw=lw, accepts=acc, ess=ess, resampled=res)
+
+
+class PMMH:
+    """bihip_pmmh: MarginalMH with B independent chains, proposals filtered as one batch."""
+
+    def __init__(self, ctx, model, N, T, C, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
+                 resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, use_graph=False, block=0):
+        nt = len(prior_mean)
+        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        B = len(self.N)
+        cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=0, resampler=resampler,
+                         essRel_inner=ess_rel, essRel_outer=0.0, t0=t0, dt=dt, seed=seed,
+                         use_graph=1 if use_graph else 0, block=block)
+        for k in range(nt):
+            cfg.prior_mean[k] = prior_mean[k]
+            cfg.prior_sd[k] = prior_sd[k]
+            cfg.prop_sd[k] = prop_sd[k]
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(lib().bihip_pmmh_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg), C,
+                                      self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      ctypes.byref(h)))
+        self.h = h
+        ctx.adopt(self)
+        self.B, self.T, self.C, self.nt, self.nobs = B, T, C, nt, nobs(model)
+
+    def close(self):
+        if self.h:
+            lib().bihip_pmmh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.T, self.nobs)
+        check(lib().bihip_pmmh_set_observations(self.h, y.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def run(self):
+        check(lib().bihip_pmmh_run(self.h))
+
+    def outputs(self):
+        D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        theta = np.zeros((self.C, self.B, self.nt))
+        ll = np.zeros((self.C, self.B))
+        acc = np.zeros(self.B, np.int32)
+        check(lib().bihip_pmmh_outputs(self.h, theta.ctypes.data_as(D), ll.ctypes.data_as(D),
+                                       acc.ctypes.data_as(I)))
+        return dict(theta=theta, ll=ll, accepted=acc)
 
 
 class MLP:

@@ -597,8 +597,8 @@ struct bihip_pf {
   bool ready = false, have_obs = false;
   // timing
   bool timing = false;
-  std::vector<hipEvent_t> ev[3];
-  size_t ev_used[3] = {0, 0, 0};
+  std::vector<hipEvent_t> ev[4];
+  size_t ev_used[4] = {0, 0, 0, 0};
   // graphs (one per buffer parity)
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   // learned surrogate flow in place of the integrator (bihip_pf_set_surrogate)
@@ -711,7 +711,13 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
     G.nobs = pf->nobs;
     G.lws = pf->lws;
     G.partials = pf->partials;
+    hipEvent_t g0 = nullptr, g1 = nullptr;
+    if (timed) {
+      if ((rc = pf_event_pair(pf, 3, &g0, &g1))) return rc;
+      BIHIP_HIP(hipEventRecord(g0, s));
+    }
     if ((rc = launch_guided(G, s))) return rc;
+    if (timed) BIHIP_HIP(hipEventRecord(g1, s));
   } else if (pf->surr) {
     SurrogateLaunch S = pf->sl;
     S.Xin = T.Xin;
@@ -1089,7 +1095,7 @@ int bihip_pf_set_timing(bihip_pf* pf, int enable) {
 }
 
 int bihip_pf_kernel_time(bihip_pf* pf, int kind, double* ms, long* count) {
-  if (pf == nullptr || kind < 0 || kind > 2) return set_error(BIHIP_EINVAL, "bad argument");
+  if (pf == nullptr || kind < 0 || kind > 3) return set_error(BIHIP_EINVAL, "bad argument");
   int rc = ctx_enter(pf->ctx);
   if (rc) return rc;
   BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));

@@ -258,6 +258,7 @@ int launch_gauss_grad(const float* Y, int ldy, const float* T, long ldt, const l
 int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* part, float* out,
                        hipStream_t s);
 int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s);
+int launch_sum_all(const float* x, int n, float* out, hipStream_t s);
 int launch_adam(float* w, float* m1, float* m2, const float* grad, long n, float lr, float b1,
                 float b2, float eps, float bc1, float bc2, uint16_t* wbf, hipStream_t s);
 int launch_to_bf16(const float* x, long n, uint16_t* y, hipStream_t s);

@@ -190,6 +190,21 @@ __global__ void k_sum_splits(const float* part, int S, long count, float* out) {
   out[i] = s;
 }
 
+// out[0] = sum of x[0 .. n) by one workgroup: strided partial sums, then a
+// fixed-order tree in LDS
+__global__ __launch_bounds__(256) void k_sum_all(const float* x, int n, float* out) {
+  __shared__ float red[256];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // Adam (Kingma & Ba 2015) with bias correction; wbf (nullable) gets the bf16 copy.
 __global__ void k_adam(float* w, float* m1, float* m2, const float* grad, long n, float lr,
                        float b1, float b2, float eps, float bc1, float bc2, uint16_t* wbf) {
@@ -291,6 +306,12 @@ int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* pa
 int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(count, 256)), dim3(256), 0, s, part, S, count,
                      out);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+int launch_sum_all(const float* x, int n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(256), 0, s, x, n, out);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

@@ -490,7 +490,7 @@ int bihip_mlp_train(bihip_mlp* m, const float* X_dev, long ldx, const float* T_d
                                   s)) ||
         (rc = forward(m, batch, s)) ||
         (rc = output_error(m, T_dev, ldt, idx, batch, batch, &nb, s)) ||
-        (rc = launch_sum_splits(m->lossp, nb, 1, loss_d + it, s)) || (rc = backward(m, batch, s)))
+        (rc = launch_sum_all(m->lossp, nb, loss_d + it, s)) || (rc = backward(m, batch, s)))
       break;
     ++m->adam_t;
     const float bc1 = 1.0f - std::pow(b1, (float)m->adam_t);

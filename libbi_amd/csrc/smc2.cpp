@@ -1,4 +1,5 @@
-// SMC^2 over the batched inner filters (bihip_smc2_*, declared in include/bihip.h).
+// SMC^2 and PMMH over the batched inner filters (bihip_smc2_*, bihip_pmmh_*,
+// declared in include/bihip.h).
 //
 // MarginalSIR::sample (share/src/bi/sampler/MarginalSIR.hpp:281-320): init
 // (:322-344), then for every observation a move (:395-488), a step of every
@@ -358,6 +359,156 @@ int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* 
   if (accepts_host) std::copy(s->accepts.begin(), s->accepts.end(), accepts_host);
   if (ess_host) std::copy(s->ess.begin(), s->ess.end(), ess_host);
   if (resampled_host) std::copy(s->resampled.begin(), s->resampled.end(), resampled_host);
+  return BIHIP_OK;
+}
+
+// ---------------------------------------------------------------- PMMH
+// MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-266) for B
+// independent chains: every iteration filters all B proposals in one batch
+// (one persistent launch per observation) and accepts on the host.  Keyed
+// draws as SMC^2's: prior normals {j, 0, THETA, 0}, proposal normals
+// {j, c, PROPOSAL, 0}, acceptance uniform {j, c, ACCEPT, 0}; the filter of
+// iteration c of chain j draws with batch id B c + j.
+
+}  // extern "C"
+
+struct bihip_pmmh {
+  bihip_smc2_config cfg{};
+  int C = 0;
+  bihip_batch* batch = nullptr;
+  std::vector<double> theta, ll;  // C x B x ntheta, C x B
+  std::vector<int> accepted;
+  bool have_obs = false, done = false;
+};
+
+namespace {
+int pmmh_filter(bihip_pmmh* s, const std::vector<double>& lth, uint32_t base,
+                std::vector<double>& ll) {
+  const int B = s->cfg.B, nt = s->cfg.ntheta;
+  std::vector<double> th((size_t)B * nt);
+  for (size_t q = 0; q < th.size(); ++q) th[q] = std::exp(lth[q]);
+  int rc;
+  if ((rc = bihip_batch_set_theta(s->batch, th.data())) ||
+      (rc = bihip_batch_set_stream_base(s->batch, base)) || (rc = bihip_batch_init(s->batch)) ||
+      (rc = bihip_batch_step(s->batch, s->cfg.T)))
+    return rc;
+  return filter_logliks(s->batch, B, ll);
+}
+}  // namespace
+
+extern "C" {
+
+int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg, int C,
+                      const int* N_host, bihip_pmmh** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (c.B <= 0 || c.T <= 0 || C <= 0) return set_error(BIHIP_EINVAL, "need B, T, C > 0");
+  if (c.ntheta < 1 || c.ntheta > 8) return set_error(BIHIP_EINVAL, "ntheta must lie in [1,8]");
+  for (int i = 0; i < c.ntheta; ++i)
+    if (!(c.prior_sd[i] > 0.0) || !(c.prop_sd[i] >= 0.0))
+      return set_error(BIHIP_EINVAL, "prior_sd must be positive and prop_sd non-negative");
+  if ((uint64_t)c.B * ((uint64_t)C + 1) > (1u << 24))
+    return set_error(BIHIP_EINVAL, "B (C + 1) random-stream ids exceed 2^24");
+  bihip_pmmh* s = new bihip_pmmh();
+  s->cfg = c;
+  s->C = C;
+  s->theta.assign((size_t)C * c.B * c.ntheta, 0.0);
+  s->ll.assign((size_t)C * c.B, 0.0);
+  s->accepted.assign(c.B, 0);
+  bihip_batch_config bc{};
+  bc.B = c.B;
+  bc.T = c.T;
+  bc.resampler = c.resampler;
+  bc.essRel = c.essRel_inner;
+  bc.t0 = c.t0;
+  bc.dt = c.dt;
+  bc.seed = c.seed;
+  bc.use_graph = c.use_graph;
+  bc.block = c.block;
+  std::vector<double> th0((size_t)c.B * c.ntheta, 1.0);
+  int rc = bihip_batch_create(ctx, m, &bc, N_host, th0.data(), c.ntheta, &s->batch);
+  if (rc != BIHIP_OK) {
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return BIHIP_OK;
+}
+
+int bihip_pmmh_destroy(bihip_pmmh* s) {
+  if (s == nullptr) return BIHIP_OK;
+  bihip_batch_destroy(s->batch);
+  delete s;
+  return BIHIP_OK;
+}
+
+int bihip_pmmh_set_observations(bihip_pmmh* s, const double* y_host) {
+  if (s == nullptr || y_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  const int rc = bihip_batch_set_observations(s->batch, y_host);
+  s->have_obs = rc == BIHIP_OK;
+  return rc;
+}
+
+int bihip_pmmh_run(bihip_pmmh* s) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->have_obs) return set_error(BIHIP_ESTATE, "no observations set");
+  const bihip_smc2_config& c = s->cfg;
+  const int B = c.B, nt = c.ntheta;
+  std::vector<double> lth((size_t)B * nt), lth2((size_t)B * nt), ll, ll2;
+  for (int j = 0; j < B; ++j) {
+    double z[8];
+    host_normals(c.seed, (uint32_t)j, 0u, BIHIP_TAG_THETA, 0u, nt, z);
+    for (int i = 0; i < nt; ++i) lth[(size_t)j * nt + i] = c.prior_mean[i] + c.prior_sd[i] * z[i];
+  }
+  int rc = pmmh_filter(s, lth, 0u, ll);
+  if (rc) return rc;
+  std::fill(s->accepted.begin(), s->accepted.end(), 1);
+  for (int it = 0; it < s->C; ++it) {
+    if (it > 0) {
+      for (int j = 0; j < B; ++j) {
+        double z[8];
+        host_normals(c.seed, (uint32_t)j, (uint32_t)it, BIHIP_TAG_PROPOSAL, 0u, nt, z);
+        for (int i = 0; i < nt; ++i)
+          lth2[(size_t)j * nt + i] = lth[(size_t)j * nt + i] + c.prop_sd[i] * z[i];
+      }
+      if ((rc = pmmh_filter(s, lth2, (uint32_t)(B * it), ll2))) return rc;
+      for (int j = 0; j < B; ++j) {
+        bool accept;
+        if (!std::isfinite(ll2[j])) {
+          accept = false;
+        } else if (!std::isfinite(ll[j])) {
+          accept = true;
+        } else {
+          const double logratio = ll2[j] - ll[j] + log_prior(c, &lth2[(size_t)j * nt]) -
+                                  log_prior(c, &lth[(size_t)j * nt]);
+          const double u =
+              host_uniform(c.seed, (uint32_t)j, (uint32_t)it, BIHIP_TAG_ACCEPT, 0u);
+          accept = std::log(u) < logratio;
+        }
+        if (accept) {
+          for (int i = 0; i < nt; ++i) lth[(size_t)j * nt + i] = lth2[(size_t)j * nt + i];
+          ll[j] = ll2[j];
+          ++s->accepted[j];
+        }
+      }
+    }
+    for (int j = 0; j < B; ++j) {
+      for (int i = 0; i < nt; ++i)
+        s->theta[((size_t)it * B + j) * nt + i] = std::exp(lth[(size_t)j * nt + i]);
+      s->ll[(size_t)it * B + j] = ll[j];
+    }
+  }
+  s->done = true;
+  return BIHIP_OK;
+}
+
+int bihip_pmmh_outputs(bihip_pmmh* s, double* theta_host, double* ll_host, int* accepted_host) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->done) return set_error(BIHIP_ESTATE, "bihip_pmmh_run has not completed");
+  if (theta_host) std::copy(s->theta.begin(), s->theta.end(), theta_host);
+  if (ll_host) std::copy(s->ll.begin(), s->ll.end(), ll_host);
+  if (accepted_host) std::copy(s->accepted.begin(), s->accepted.end(), accepted_host);
   return BIHIP_OK;
 }
 

@@ -49,6 +49,9 @@ PINNED = {
     "orc_mlp_loss_grad": "central finite differences of the FP64 loss; bf16 mode: rounding "
                          "points checked against numpy bf16 rounding and torch with the same "
                          "rounded weights",
+    "orc_pmmh_run": "first iterations rebuilt from orc_pf_run and the keyed draws; LG model: "
+                    "posterior mean of lambda from the chains against Gauss-Hermite "
+                    "quadrature of the exact Kalman likelihood",
     "orc_smc2_run": "no moves and no outer resampling: lw_j equals orc_pf_run with theta_j and "
                     "batch j; logZ = logsumexp(lw) - log B; LG model: evidence and posterior "
                     "mean of lambda against Gauss-Hermite quadrature of the Kalman likelihood; "
@@ -164,6 +167,8 @@ def _declare(L):
     L.orc_permute.argtypes = [_I, c.c_int]
     L.orc_gather_rows.argtypes = [_I, c.c_int, c.c_int, _D, c.c_int]
     L.orc_pf_run.argtypes = [c.POINTER(Model), c.POINTER(PFConfig), _D, _D, _D, _D, _D, _I, _I, _D]
+    L.orc_pmmh_run.argtypes = [c.POINTER(Model), c.POINTER(SMC2Config), c.c_int, _I, _D, _D, _D,
+                               _I]
     L.orc_uniform_at.argtypes = [c.c_uint64, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, _D]
     L.orc_smc2_run.argtypes = [c.POINTER(Model), c.POINTER(SMC2Config), _I, _D, _D, _D, _D, _I,
                                _D, _I]
@@ -499,6 +504,19 @@ def bf16(x):
     f = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
     r = ((f + 0x7FFF + ((f >> 16) & 1)) & 0xFFFF0000).astype(np.uint32).view(np.float32)
     return r.astype(np.float64)
+
+
+def pmmh_run(m, cfg, C, N, y):
+    """PMMH (MarginalMH) with cfg.B independent chains of C iterations."""
+    y = _f64(y).reshape(len(y), -1)
+    assert y.shape[0] == cfg.T and len(N) == cfg.B
+    N = _i32(N)
+    theta = np.zeros((C, cfg.B, cfg.ntheta))
+    ll = np.zeros((C, cfg.B))
+    acc = np.zeros(cfg.B, np.int32)
+    rc = lib().orc_pmmh_run(ctypes.byref(m), ctypes.byref(cfg), C, _ip(N), _dp(y), _dp(theta),
+                            _dp(ll), _ip(acc))
+    return dict(rc=rc, theta=theta, ll=ll, accepted=acc)
 
 
 def kalman_loglik_lg(m, dt, y):

@@ -342,3 +342,76 @@ int orc_smc2_run(const orc_model* base, const orc_smc2_cfg* c, const int* N, con
   free(f); free(g); free(lth); free(lth2); free(lw); free(Ws); free(Os); free(as);
   return ORC_OK;
 }
+
+/* ------------------------------------------------------------------ PMMH */
+/*
+ * MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-215) for B
+ * independent chains, each with inner filters of N[j] particles:
+ * init (:217-226) filters theta_0 from the prior, then every iteration
+ * proposes (:228-244) theta' = theta + prop_sd z in log space, filters it over
+ * the whole schedule, and accepts (:246-266) with probability
+ * min(1, exp(ll' - ll + log prior' - log prior)); a non-finite ll' is
+ * rejected, a non-finite ll accepts.  Keyed draws: prior normals
+ * {j, 0, THETA, 0}; proposal normals {j, c, PROPOSAL, 0}; acceptance uniform
+ * {j, c, ACCEPT, 0}; the filter of iteration c of chain j draws with batch id
+ * B c + j.
+ */
+static double pmmh_loglik(const orc_model* base, const orc_smc2_cfg* c, const double* lth,
+                          int N, const double* y, uint32_t batch) {
+  orc_model mj;
+  model_with(base, c, lth, &mj);
+  const int nobs = orc_nobs(base);
+  orc_pf* f = orc_pf_new(&mj, N, c->resampler, c->essRel_inner, c->t0, c->dt, c->seed, batch);
+  orc_pf_init(f, NULL);
+  int rc = ORC_OK;
+  for (int s = 0; s < c->T && rc == ORC_OK; ++s) rc = orc_pf_step(f, y + (size_t)s * nobs, NULL, NULL, NULL);
+  const double ll = rc == ORC_OK ? f->logLikelihood : -INFINITY;
+  orc_pf_free(f);
+  return ll;
+}
+
+int orc_pmmh_run(const orc_model* base, const orc_smc2_cfg* c, int C, const int* N,
+                 const double* y, double* theta_out, double* ll_out, int* accepted_out) {
+  const int B = c->B, nt = c->ntheta;
+  double* lth = (double*)malloc(sizeof(double) * (size_t)B * nt);
+  double* lth2 = (double*)malloc(sizeof(double) * (size_t)nt);
+  double* ll = (double*)malloc(sizeof(double) * (size_t)B);
+  for (int j = 0; j < B; ++j) {
+    double z[8];
+    orc_normals(c->seed, (uint32_t)j, 0, ORC_TAG_THETA, 0, nt, 32, z);
+    for (int i = 0; i < nt; ++i) lth[(size_t)j * nt + i] = c->prior_mean[i] + c->prior_sd[i] * z[i];
+    ll[j] = pmmh_loglik(base, c, lth + (size_t)j * nt, N[j], y, (uint32_t)j);
+    if (accepted_out) accepted_out[j] = 1;
+  }
+  for (int it = 0; it < C; ++it) {
+    if (it > 0) {
+      for (int j = 0; j < B; ++j) {
+        double z[8], u;
+        orc_normals(c->seed, (uint32_t)j, (uint32_t)it, ORC_TAG_PROPOSAL, 0, nt, 32, z);
+        for (int i = 0; i < nt; ++i) lth2[i] = lth[(size_t)j * nt + i] + c->prop_sd[i] * z[i];
+        const double ll2 = pmmh_loglik(base, c, lth2, N[j], y, (uint32_t)(B * it + j));
+        int accept;
+        if (!isfinite(ll2)) accept = 0;
+        else if (!isfinite(ll[j])) accept = 1;
+        else {
+          const double logratio = ll2 - ll[j] + log_prior(c, lth2) - log_prior(c, lth + (size_t)j * nt);
+          orc_uniform_at(c->seed, (uint32_t)j, (uint32_t)it, ORC_TAG_ACCEPT, 0, &u);
+          accept = log(u) < logratio;
+        }
+        if (accept) {
+          memcpy(lth + (size_t)j * nt, lth2, sizeof(double) * nt);
+          ll[j] = ll2;
+          if (accepted_out) ++accepted_out[j];
+        }
+      }
+    }
+    for (int j = 0; j < B; ++j) {
+      if (theta_out)
+        for (int i = 0; i < nt; ++i)
+          theta_out[((size_t)it * B + j) * nt + i] = exp(lth[(size_t)j * nt + i]);
+      if (ll_out) ll_out[(size_t)it * B + j] = ll[j];
+    }
+  }
+  free(lth); free(lth2); free(ll);
+  return ORC_OK;
+}

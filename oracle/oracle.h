@@ -163,6 +163,14 @@ int orc_smc2_run(const orc_model* base, const orc_smc2_cfg* c, const int* N, con
     double* logZ, double* theta_out, double* lw_out, int* accepts_out, double* ess_out,
     int* resampled_out);
 
+/* PMMH (MarginalMH) with B independent chains and C iterations (the first is
+ * the initial draw), reusing orc_smc2_cfg (B, T, ntheta, resampler,
+ * essRel_inner, t0, dt, seed, prior, prop_sd).  Outputs (nullable): theta_out
+ * [C x B x ntheta], ll_out [C x B], accepted_out [B] (counts, the initial
+ * draw included). */
+int orc_pmmh_run(const orc_model* base, const orc_smc2_cfg* c, int C, const int* N,
+    const double* y, double* theta_out, double* ll_out, int* accepted_out);
+
 /* ---- multilayer perceptron (mlp.c): GELU z sigma(1.702 z) hidden layers, linear output,
  * flat parameters [W_l (d_{l+1} x d_l, row-major), b_l] for l = 0..L-1 ---- */
 double orc_bf16(double x);

@@ -88,3 +88,29 @@ def test_batch_resample_filters_then_step(ctx, orc):
     # filters 0 and 1 hold the same particles at step 3 but draw different noise
     assert out["logLikelihood"][0] != out["logLikelihood"][1]
     bf.close()
+
+
+@pytest.mark.parametrize("kind", ["pz", "lg"])
+def test_pmmh_matches_oracle(ctx, orc, kind):
+    T, C = 8, 12
+    y = _data(orc, kind, T)
+    gm = capi.make_model(kind, precision=capi.F64)
+    om = oracle_model(orc, gm)
+    if kind == "pz":
+        N = [64, 300, 17, 1024, 5, 700, 256, 2000]
+        pm, ps, qs = [np.log(0.5), np.log(0.2)], [0.3, 0.3], [0.1, 0.1]
+    else:
+        N = [200, 50, 1000, 128]
+        pm, ps, qs = [np.log(0.5)], [1.0], [0.3]
+    c = orc.smc2_config(len(N), T, pm, ps, qs, seed=17, ess_rel_inner=0.6)
+    ref = orc.pmmh_run(om, c, C, N, y)
+    s = capi.PMMH(ctx, gm, N, T, C, pm, ps, qs, seed=17, ess_rel=0.6, block=256)
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    print("PMMH", kind, "accepted", ref["accepted"], flush=True)
+    np.testing.assert_array_equal(out["accepted"], ref["accepted"])
+    np.testing.assert_allclose(out["theta"], ref["theta"], rtol=1e-15)
+    np.testing.assert_allclose(out["ll"], ref["ll"], rtol=1e-10, atol=1e-10)
+    assert (ref["accepted"] > 1).any()

@@ -1,4 +1,4 @@
-"""Pins the oracle's SMC^2 sampler (MarginalSIR over log theta with inner
+"""Pins the oracle's SMC^2 and PMMH samplers (MarginalSIR over log theta with inner
 bootstrap filters, share/src/bi/sampler/MarginalSIR.hpp:281-488), CPU only:
 
 - without moves or outer resampling every slot is the plain filter run with
@@ -114,3 +114,50 @@ def test_smc2_matches_quadrature_on_lg(orc):
     # 0.04 at these sizes; the means of 8 runs are allowed ~4 standard errors
     assert abs(np.mean(zs) - lZq) < 0.3, (zs, lZq)
     assert abs(np.mean(ps) - post) < 0.06, (ps, post)
+
+
+def test_pmmh_first_iterations_rebuilt(orc):
+    """MarginalMH: filter the prior draw, then propose, filter with batch id
+    B c + j, accept on the keyed uniform (MarginalMH.hpp:217-266)."""
+    B, T, N, seed, C = 4, 8, 300, 9, 3
+    m, y = _data(orc, T)
+    mu, sd, psd = np.log(0.5), 0.7, 0.4
+    c = orc.smc2_config(B, T, [mu], [sd], [psd], seed=seed)
+    r = orc.pmmh_run(m, c, C, [N] * B, y)
+    lth = _prior_theta(orc, seed, B, mu, sd)
+    ll = np.array([orc.pf_run(_with_theta(orc, np.exp(lth[j])), y, N, 1.0, seed, batch=j)
+                   ["logIncrements"].sum() for j in range(B)])
+    np.testing.assert_allclose(r["ll"][0], ll, rtol=1e-12)
+    lp = lambda v: -0.5 * ((v - mu) / sd) ** 2
+    acc = np.ones(B, int)
+    for it in range(1, C):
+        for j in range(B):
+            prop = lth[j] + psd * orc.normals(seed, j, it, orc.TAG_PROPOSAL, 1)[0]
+            l2 = orc.pf_run(_with_theta(orc, np.exp(prop)), y, N, 1.0, seed,
+                            batch=B * it + j)["logIncrements"].sum()
+            u = orc.uniform_at(seed, j, it, orc.TAG_ACCEPT, 0)
+            if np.log(u) < l2 - ll[j] + lp(prop) - lp(lth[j]):
+                lth[j], ll[j] = prop, l2
+                acc[j] += 1
+        np.testing.assert_allclose(r["theta"][it, :, 0], np.exp(lth), rtol=1e-14)
+        np.testing.assert_allclose(r["ll"][it], ll, rtol=1e-12)
+    np.testing.assert_array_equal(r["accepted"], acc)
+
+
+def test_pmmh_matches_quadrature_on_lg(orc):
+    T, B, N, C = 60, 32, 256, 120
+    m, y = _data(orc, T)
+    mu, sd = np.log(0.5), 1.0
+    xs, ws = np.polynomial.hermite.hermgauss(120)
+    ls = mu + np.sqrt(2.0) * sd * xs
+    lik = np.array([orc.kalman_loglik_lg(_with_theta(orc, np.exp(l)), 1.0, y[:, 0]) for l in ls])
+    lZq = logsumexp(lik, b=ws / np.sqrt(np.pi))
+    post = np.sum(ws * np.exp(lik - lZq) * np.exp(ls)) / np.sqrt(np.pi)
+    c = orc.smc2_config(B, T, [mu], [sd], [0.25], seed=3)
+    r = orc.pmmh_run(m, c, C, [N] * B, y)
+    draws = r["theta"][C // 3:, :, 0]
+    est = draws.mean()
+    # chain means are independent across chains: their spread gives the error
+    se = draws.mean(axis=0).std(ddof=1) / np.sqrt(B)
+    assert 0.1 < r["accepted"].mean() / C < 0.9
+    assert abs(est - post) < 4 * se + 0.01, (est, post, se)
