@@ -67,6 +67,7 @@ extern "C" {
 #define BIHIP_TAG_PROPOSAL 8  /* SMC^2 random-walk proposals */
 #define BIHIP_TAG_ACCEPT 9    /* SMC^2 acceptance uniforms */
 #define BIHIP_TAG_OUTER 10    /* SMC^2 outer systematic offset */
+#define BIHIP_TAG_PATH 13     /* path sampling (BootstrapPF::samplePath) */
 
 /*
  * Model description: the runtime counterpart of a generated Model<name>
@@ -235,7 +236,8 @@ typedef struct {
   double t0, dt;     /* schedule */
   uint64_t seed;     /* Philox key */
   uint32_t batch;    /* Philox batch id (theta index under SMC^2) */
-  int record;        /* 1 = keep ancestors of every step on the device */
+  int record;        /* 1 = keep ancestors of every step on the device;
+                        2 = also the state at every time (paths) */
   int use_graph;     /* 1 = replay each step as a captured hipGraph */
 } bihip_pf_config;
 
@@ -259,6 +261,19 @@ int bihip_pf_outputs(bihip_pf* pf, double* logIncrements_host, double* ess_host,
                      int* resampled_host, double* logLikelihood_host);
 /* Ancestors of step k (record=1), or of the last resample when k < 0. */
 int bihip_pf_ancestors(bihip_pf* pf, int k, int* as_host);
+/*
+ * Paths (record = 2; AncestryCache::readPath, share/src/bi/cache/AncestryCache.hpp:328-352):
+ * for each particle idx_host[q] at the current time K, its lineage's state at
+ * times 0..K, paths_host[q][k][c] (n x (K + 1) x d doubles).  All states stay
+ * on the device (no pruning: (T + 1) x d x P values, e.g. 84 GB for
+ * d = 40, N = 2^20, T = 500 in FP32, within one GPU's HBM).
+ */
+int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host);
+/* BootstrapPF::samplePath (share/src/bi/filter/BootstrapPF.hpp:146-154): one
+ * particle drawn in proportion to its weight (keyed uniform {0, K, PATH,
+ * batch}, the first with cumulative weight above u W), and its path
+ * ((K + 1) x d); index_host (nullable) gets the particle. */
+int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host);
 /* Current state buffer (device pointer, leading dimension) and log-weights. */
 int bihip_pf_state(bihip_pf* pf, void** X_dev, int* ld, double** lws_dev);
 /* Steps completed so far. */

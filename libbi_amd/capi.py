@@ -21,7 +21,7 @@ SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
 F32, F64 = 0, 1
 LOSS_MSE, LOSS_GAUSS = 0, 1
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
-TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER = 7, 8, 9, 10
+TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER, TAG_PATH = 7, 8, 9, 10, 13
 
 # exported entry points (must match include/bihip.h)
 EXPORTS = [
@@ -56,7 +56,7 @@ EXPORTS = [
     "bihip_surrogate_flow", "bihip_pf_set_surrogate", "bihip_mlp_set_loss",
     "bihip_guided_propose", "bihip_pf_set_guided",
     "bihip_pmmh_create", "bihip_pmmh_destroy", "bihip_pmmh_set_observations", "bihip_pmmh_run",
-    "bihip_pmmh_outputs",
+    "bihip_pmmh_outputs", "bihip_pf_paths", "bihip_pf_sample_path",
 ]
 
 
@@ -234,6 +234,8 @@ def _declare(L):
         "bihip_pmmh_set_observations": [P, pd],
         "bihip_pmmh_run": [P],
         "bihip_pmmh_outputs": [P, pd, pd, pi],
+        "bihip_pf_paths": [P, pi, i, pd],
+        "bihip_pf_sample_path": [P, pd, pi],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -341,9 +343,11 @@ class ParticleFilter:
 
     def __init__(self, ctx, model, P, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0,
                  batch=0, record=False, use_graph=False):
+        """record: False, True (ancestors) or "paths" (ancestors and states)."""
         self.ctx, self.model = ctx, model
+        rec = 2 if record == "paths" else (1 if record else 0)
         self.cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
-                            seed=seed, batch=batch, record=1 if record else 0,
+                            seed=seed, batch=batch, record=rec,
                             use_graph=1 if use_graph else 0)
         h = ctypes.c_void_p()
         check(lib().bihip_pf_create(ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
@@ -411,6 +415,22 @@ class ParticleFilter:
 
     def position(self):
         return lib().bihip_pf_position(self.h)
+
+    def paths(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        K = self.position()
+        out = np.zeros((len(idx), K + 1, self.model.d))
+        check(lib().bihip_pf_paths(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   len(idx), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def sample_path(self):
+        K = self.position()
+        out = np.zeros((K + 1, self.model.d))
+        p = ctypes.c_int()
+        check(lib().bihip_pf_sample_path(self.h, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)), ctypes.byref(p)))
+        return p.value, out
 
     def state(self):
         """Device copies of the current state (d x ld torch tensor), ld, and the

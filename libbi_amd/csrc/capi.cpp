@@ -592,6 +592,7 @@ struct bihip_pf {
   double* hist_ess = nullptr;
   int* hist_res = nullptr;
   int* hist_anc = nullptr;
+  void* hist_X = nullptr;  // record = 2: the state at every time, (T + 1) x d x ld
   ScanWorkspace ws{};
   int k = 0;
   bool ready = false, have_obs = false;
@@ -740,6 +741,10 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
     return rc;
   }
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
+  if (pf->hist_X != nullptr &&
+      (rc = launch_record_state(T.Xout, pf->ld, pf->model.d, P, pf->elem, pf->dev, pf->hist_X, 0,
+                                s)))
+    return rc;
 
   // ---- BootstrapPF::correct bookkeeping
   if (timed) {
@@ -830,6 +835,11 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   PF_ALLOC(pf->hist_ess, cfg->T);
   PF_ALLOC(pf->hist_res, cfg->T);
   if (cfg->record) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
+  if (cfg->record == 2) {
+    uint8_t* hx = nullptr;
+    PF_ALLOC(hx, (size_t)(cfg->T + 1) * m->d * pf->ld * pf->elem);
+    pf->hist_X = hx;
+  }
 #undef PF_ALLOC
   if ((rc = ws_alloc(&pf->ws, scan_tiles(P))) ||
       (rc = rw_alloc(&pf->rw, std::max(pf->nparts, ceil_div(P, RED_TILE))))) {
@@ -857,6 +867,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->as);
   dfree(pf->EF);
   dfree(pf->partials);
+  dfree(pf->hist_X);
   dfree(pf->out);
   dfree(pf->dev);
   dfree(pf->y);
@@ -892,6 +903,10 @@ int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
                                pf->X[0], pf->ld, mean0_host)))
     return rc;
   BIHIP_HIP(hipMemsetAsync(pf->lws, 0, sizeof(double) * pf->cfg.P, s));
+  if (pf->hist_X != nullptr &&
+      (rc = launch_record_state(pf->X[0], pf->ld, pf->model.d, pf->cfg.P, pf->elem, nullptr,
+                                pf->hist_X, 0, s)))
+    return rc;
   PFDevice d{};
   d.status = BIHIP_OK;
   BIHIP_HIP(hipMemcpyAsync(pf->dev, &d, sizeof(d), hipMemcpyHostToDevice, s));
@@ -1085,6 +1100,69 @@ int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* n
   if (pf->nparts > pf->nparts_exact)
     return set_error(BIHIP_EINVAL, "partials buffer too small for the guided kernel");
   return BIHIP_OK;
+}
+
+int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host) {
+  if (pf == nullptr || (n > 0 && (idx_host == nullptr || paths_host == nullptr)))
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  if (pf->hist_X == nullptr || pf->hist_anc == nullptr)
+    return set_error(BIHIP_ESTATE, "paths need record = 2");
+  if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
+  for (int q = 0; q < n; ++q)
+    if (idx_host[q] < 0 || idx_host[q] >= pf->cfg.P)
+      return set_error(BIHIP_EINVAL, "particle %d outside [0, %d)", idx_host[q], pf->cfg.P);
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  const int K = pf->k, D = pf->model.d;
+  int* idx = nullptr;
+  double* out = nullptr;
+  if ((rc = dalloc(&idx, (size_t)n)) || (rc = dalloc(&out, (size_t)n * (K + 1) * D))) {
+    dfree(idx);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(idx, idx_host, sizeof(int) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess &&
+      (rc = launch_paths(pf->hist_anc, pf->hist_X, pf->ld, D, pf->cfg.P, K, pf->elem, idx, n, out,
+                         s)) == BIHIP_OK)
+    e = hipMemcpyAsync(paths_host, out, sizeof(double) * n * (K + 1) * D, hipMemcpyDeviceToHost,
+                       s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(idx);
+  dfree(out);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_error(BIHIP_EHIP, "paths: %s", hipGetErrorString(e));
+  return BIHIP_OK;
+}
+
+int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host) {
+  if (pf == nullptr || path_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  const int P = pf->cfg.P;
+  std::vector<double> lw(P);
+  BIHIP_HIP(hipMemcpyAsync(lw.data(), pf->lws, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  // one multinomial draw over the final log-weights: the first particle whose
+  // cumulative weight exceeds u W (the multinomial resampler's upper bound)
+  double mx = -INFINITY;
+  for (double v : lw)
+    if (!std::isnan(v) && v > mx) mx = v;
+  std::vector<double> Ws(P);
+  double acc = 0.0;
+  for (int i = 0; i < P; ++i) {
+    acc = acc + (std::isnan(lw[i]) ? 0.0 : std::exp(lw[i] - mx));
+    Ws[i] = acc;
+  }
+  if (!(acc > 0.0)) return set_error(BIHIP_DEGENERATE, "all weights are zero");
+  const double u = bihip_uniform_host(pf->cfg.seed, 0u, (uint32_t)pf->k, BIHIP_TAG_PATH,
+                                      pf->cfg.batch);
+  const double target = acc * u;
+  const int p = (int)(std::upper_bound(Ws.begin(), Ws.end(), target) - Ws.begin());
+  const int pick = p < P ? p : P - 1;
+  if (index_host) *index_host = pick;
+  return bihip_pf_paths(pf, &pick, 1, path_host);
 }
 
 int bihip_pf_set_timing(bihip_pf* pf, int enable) {

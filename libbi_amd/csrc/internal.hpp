@@ -188,6 +188,10 @@ int launch_zero_ints(int* p, int n, const PFDevice* gate, hipStream_t s);
 int launch_ancestors_to_offspring(const int* as, int n, int P, int* os, hipStream_t s);
 int launch_gather_rows(const int* as, int P, int ncols, void* X, int ld, int elem,
                        hipStream_t s);
+int launch_record_state(const void* X, int ld, int D, int P, int elem, const PFDevice* dev,
+                        void* hist, int slot0, hipStream_t s);
+int launch_paths(const int* anc, const void* hist, int ld, int D, int P, int K, int elem,
+                 const int* idx, int n, double* out, hipStream_t s);
 int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
                             hipStream_t s);
 

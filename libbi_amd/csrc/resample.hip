@@ -308,6 +308,61 @@ int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out
   return BIHIP_OK;
 }
 
+// The state after the step into slot k + 1 of the history (slot 0 is the
+// initial state): (T + 1) x D columns of leading dimension ld.
+template <class T>
+__global__ void k_record_state(const T* X, int ld, int D, int P, const PFDevice* dev, T* hist,
+                               int slot0) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)P * D) return;
+  const int c = (int)(g / P), p = (int)(g % P);
+  const int slot = dev != nullptr ? dev->k + 1 : slot0;
+  hist[((size_t)slot * D + c) * ld + p] = X[(size_t)c * ld + p];
+}
+
+int launch_record_state(const void* X, int ld, int D, int P, int elem, const PFDevice* dev,
+                        void* hist, int slot0, hipStream_t s) {
+  const dim3 grid(ceil_div((long)P * D, 256)), block(256);
+  if (elem == 8)
+    hipLaunchKernelGGL(k_record_state<double>, grid, block, 0, s, static_cast<const double*>(X), ld,
+                       D, P, dev, static_cast<double*>(hist), slot0);
+  else
+    hipLaunchKernelGGL(k_record_state<float>, grid, block, 0, s, static_cast<const float*>(X), ld,
+                       D, P, dev, static_cast<float*>(hist), slot0);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+// Paths of particles idx[q] at the current time K, traced back through the
+// recorded ancestors: out[q][k][c] = state c at time k of the lineage
+// (AncestryCache::readPath, share/src/bi/cache/AncestryCache.hpp:328-352).
+template <class T>
+__global__ void k_paths(const int* anc, const T* hist, int ld, int D, int P, int K, const int* idx,
+                        int n, double* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int p = idx[q];
+  for (int k = K; k >= 0; --k) {
+    for (int c = 0; c < D; ++c)
+      out[((size_t)q * (K + 1) + k) * D + c] = (double)hist[((size_t)k * D + c) * ld + p];
+    if (k > 0) p = anc[(size_t)(k - 1) * P + p];
+  }
+}
+
+int launch_paths(const int* anc, const void* hist, int ld, int D, int P, int K, int elem,
+                 const int* idx, int n, double* out, hipStream_t s) {
+  if (n <= 0) return BIHIP_OK;
+  const dim3 grid(ceil_div(n, 64)), block(64);
+  if (elem == 8)
+    hipLaunchKernelGGL(k_paths<double>, grid, block, 0, s, anc, static_cast<const double*>(hist), ld,
+                       D, P, K, idx, n, out);
+  else
+    hipLaunchKernelGGL(k_paths<float>, grid, block, 0, s, anc, static_cast<const float*>(hist), ld,
+                       D, P, K, idx, n, out);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
 __global__ void k_fill_f64(double* x, int n, double v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;

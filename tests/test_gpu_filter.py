@@ -150,3 +150,36 @@ def test_step_beyond_schedule_is_rejected(ctx, orc):
     with pytest.raises(capi.BihipError):
         pf.step(1)
     pf.close()
+
+
+def test_paths_match_oracle_lineages(ctx, orc):
+    """record = 2: particle paths traced through the device ancestry equal the
+    oracle's lineages (AncestryCache::readPath), and samplePath draws the
+    particle the keyed uniform selects from the final weights."""
+    T, P, seed = 10, 512, 42
+    y = _pz_data(orc, T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    pf = capi.ParticleFilter(ctx, gm, P, T, 1.0, seed, ess_rel=0.7, record="paths")
+    pf.set_observations(y)
+    pf.init()
+    pf.step(T)
+    idx = np.array([0, 7, 100, 511, 256], np.int32)
+    got = pf.paths(idx)
+    ref = orc.pf_run(om, y, P, 1.0, seed, ess_rel=0.7, want_ancestors=True)
+    states = [orc.init_samples(om, seed, P)]
+    for k in range(1, T + 1):
+        states.append(orc.pf_run(om, y[:k], P, 1.0, seed, ess_rel=0.7, want_state=True)["X"])
+    for q, p in enumerate(idx):
+        for k in range(T, -1, -1):
+            np.testing.assert_allclose(got[q, k], states[k][:, p], rtol=1e-12, atol=1e-12)
+            if k > 0:
+                p = ref["ancestors"][k - 1][p]
+    pick, path = pf.sample_path()
+    _, _, lws = pf.state()
+    lw = lws.cpu().numpy()
+    Ws = np.cumsum(np.exp(lw - lw.max()))
+    u = orc.uniform_at(seed, 0, T, capi.TAG_PATH, 0)
+    assert pick == min(int(np.searchsorted(Ws, Ws[-1] * u, side="right")), P - 1)
+    np.testing.assert_array_equal(path, pf.paths([pick])[0])
+    pf.close()
