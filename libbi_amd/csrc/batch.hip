@@ -103,10 +103,6 @@ __device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* ld
   return pre + incl - v;
 }
 
-__device__ __forceinline__ double fix_to_double(const u128& s) {
-  return (double)s.hi * 4.0 + (double)s.lo * 0x1.0p-62;
-}
-
 __device__ __forceinline__ int sys_offspring(double Ws, double W, int n, double a) {
 #pragma clang fp contract(off)
   const double v = Ws / W * (double)n + a;
@@ -163,7 +159,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
         u128 v(0);
         if (i < N) {
           const double w = nanexp(a.lws[off + i] - F.mx);
-          v = u128((uint64_t)(w * 0x1.0p62));
+          v = fix_weight(w);
         }
         u128 rowtot;
         const u128 incl = block_incl_scan_u128<BLOCK>(v, l128, rowtot);

@@ -635,37 +635,23 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
     if ((rc = pf_event_pair(pf, 1, &e0, &e1))) return rc;
     BIHIP_HIP(hipEventRecord(e0, s));
   }
-  if ((rc = launch_sumexp_scan(pf->lws, P, &pf->dev->mx, pf->dev, pf->Ws, pf->ws, s))) return rc;
-  if (pf->cfg.resampler == BIHIP_MULTINOMIAL) {
-    if ((rc = launch_zero_ints(pf->os, P, pf->dev, s))) return rc;
-    MultinomialLaunch L{};
-    L.Ws = pf->Ws;
-    L.P = P;
-    L.n = P;
-    L.seed = pf->cfg.seed;
-    L.batch = pf->cfg.batch;
-    L.gate = pf->dev;
-    L.status = pf->dev;
-    L.os = pf->os;
-    if ((rc = launch_multinomial(L, s))) return rc;
-    if ((rc = launch_ancestors_permute(pf->os, false, P, pf->as, pf->EF, pf->ws, pf->dev, s)))
-      return rc;
-  } else {
-    OffspringLaunch L{};
-    L.mode = pf->cfg.resampler;
-    L.Ws = pf->Ws;
-    L.P = P;
-    L.n = P;
-    L.philox = true;
-    L.seed = pf->cfg.seed;
-    L.batch = pf->cfg.batch;
-    L.gate = pf->dev;
-    L.status = pf->dev;
-    L.Os = pf->Os;
-    if ((rc = launch_cumulative_offspring(L, s))) return rc;
-    if ((rc = launch_ancestors_permute(pf->Os, true, P, pf->as, pf->EF, pf->ws, pf->dev, s)))
-      return rc;
-  }
+  ResampleTilesLaunch R{};
+  R.mode = pf->cfg.resampler;
+  R.lws = pf->lws;
+  R.mx = &pf->dev->mx;
+  R.P = P;
+  R.n = P;
+  R.seed = pf->cfg.seed;
+  R.batch = pf->cfg.batch;
+  R.gate = pf->dev;
+  R.status = pf->dev;
+  R.Ws = pf->Ws;
+  R.Os = pf->Os;
+  R.os = pf->os;
+  R.as = pf->as;
+  R.EF = pf->EF;
+  R.ws = pf->ws;
+  if ((rc = launch_resample_tiles(R, s))) return rc;
   if (pf->hist_anc) {
     if ((rc = launch_record_ancestors(pf->as, P, pf->dev, pf->hist_anc, s))) return rc;
   }

@@ -59,20 +59,25 @@ struct PFHistory {
   int* resampled;
 };
 
-// Single-pass scan bookkeeping (ticket and epoch, then per-tile aggregates).
+// Scan bookkeeping: the single-pass scan's ticket, epoch and error flag, and
+// the number of free slots of the tiled resampling chain's last permutation.
 struct ScanState {
   uint32_t ticket;
   uint32_t epoch;
   uint32_t error;
-  uint32_t pad;
+  uint32_t free_slots;
 };
 
 struct ScanWorkspace {
-  ScanState* state;  // device: ticket, epoch, error
-  uint64_t* words;   // device: SCAN_WORDS epoch-tagged words per tile
+  ScanState* state;  // device
+  uint64_t* words;   // device: SCAN_WORDS words per tile (see SCAN_WORDS)
   int max_tiles;
 };
-constexpr int SCAN_WORDS = 4;
+// Words per tile: words [0, 4 max_tiles) hold the single-pass scan's
+// epoch-tagged aggregates; [4 max_tiles, 8 max_tiles) the tiled resampling
+// chain's plain tile sums, kept apart so that no tile sum can pass for a
+// published aggregate.
+constexpr int SCAN_WORDS = 8;
 
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -172,6 +177,26 @@ int launch_cumulative_offspring(const OffspringLaunch& L, hipStream_t s);
 int launch_ancestors_permute(const int* in, bool cumulative, int P, int* as, uint64_t* EF,
                              ScanWorkspace ws, const PFDevice* gate, hipStream_t s);
 int launch_cumulative_to_ancestors(const int* Os, int P, int* as, hipStream_t s);
+// The filter's whole resampling chain in tiles, from the log-weights to the
+// permuted ancestors: systematic / stratified (cumulative offspring into Os)
+// or multinomial (Ws, then the offspring counts into os).  Gated by gate.
+struct ResampleTilesLaunch {
+  int mode;                 // BIHIP_SYSTEMATIC / BIHIP_STRATIFIED / BIHIP_MULTINOMIAL
+  const double* lws;
+  const double* mx;         // device: the maximum log-weight
+  int P, n;
+  uint64_t seed;
+  uint32_t batch;
+  const PFDevice* gate;     // the step number is gate->k
+  PFDevice* status;         // where to flag W <= 0
+  double* Ws;               // multinomial only
+  int* Os;                  // systematic / stratified
+  int* os;                  // multinomial
+  int* as;
+  uint64_t* EF;             // P words of scratch
+  ScanWorkspace ws;
+};
+int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s);
 struct MultinomialLaunch {
   const double* Ws;
   int P, n;

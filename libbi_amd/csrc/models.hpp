@@ -31,6 +31,10 @@ struct ModelArgs {
 // ---------------------------------------------------------------- lane groups
 // Cross-lane moves inside a group.  Groups of four use DPP quad permutations
 // (a VALU operand modifier, no LDS traffic); other sizes use ds_bpermute.
+// A quad permutation reads only lanes of the same group, which are active
+// together, so bound_ctrl (zero for an invalid source lane) never applies; it
+// is set because it lets the compiler fold the move into the consuming
+// instruction's DPP operand.
 constexpr int QP_PREV = 3 | (0 << 2) | (1 << 4) | (2 << 6);  // lane g reads g-1 (mod 4)
 constexpr int QP_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane g reads g+1 (mod 4)
 constexpr int QP_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
@@ -38,13 +42,13 @@ constexpr int QP_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
 
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -99,6 +103,7 @@ template <class T>
 struct ModelPZ {
   static constexpr int D = 2, LPP = 1, DL = 2, NOBS = 1;
   static constexpr bool NOISE_FIRST = true;
+  static constexpr bool HAS_FORCING = false;
   T c, e, ml, mq, alpha;
   double mu, sigma, obs_sd;
   __device__ __forceinline__ explicit ModelPZ(const ModelArgs& a) {
@@ -124,6 +129,8 @@ template <class T, int D_, int LPP_>
 struct ModelL96 {
   static constexpr int D = D_, LPP = LPP_, DL = D_ / LPP_, NOBS = D_ / 2;
   static constexpr bool NOISE_FIRST = false;
+  // the right-hand side is g(x) + F with a constant forcing F (rhs_unforced: g)
+  static constexpr bool HAS_FORCING = true;
   static_assert(D_ % LPP_ == 0 && DL >= 2 && D_ % 2 == 0, "even dimension, lane slice >= 2");
   T F;
   double noise_sd, obs_sd;
@@ -132,8 +139,8 @@ struct ModelL96 {
     noise_sd = a.par[1];
     obs_sd = a.par[2];
   }
-  template <class G>
-  __device__ __forceinline__ void rhs(const G& grp, const T (&x)[DL], T (&k)[DL]) const {
+  template <bool FORCED, class G>
+  __device__ __forceinline__ void eval(const G& grp, const T (&x)[DL], T (&k)[DL]) const {
     T xm2, xm1, xp1;  // x[lo-2], x[lo-1], x[hi]
     if constexpr (LPP == 1) {
       xm2 = x[DL - 2];
@@ -149,9 +156,19 @@ struct ModelL96 {
       const T a = (i + 1 < DL) ? x[i + 1 < DL ? i + 1 : 0] : xp1;
       const T b = (i >= 2) ? x[i >= 2 ? i - 2 : 0] : (i == 1 ? xm1 : xm2);
       const T c = (i >= 1) ? x[i >= 1 ? i - 1 : 0] : xm1;
-      k[i] = (a - b) * c - x[i] + F;
+      if constexpr (FORCED) k[i] = (a - b) * c - x[i] + F;
+      else k[i] = (a - b) * c - x[i];
     }
   }
+  template <class G>
+  __device__ __forceinline__ void rhs(const G& grp, const T (&x)[DL], T (&k)[DL]) const {
+    eval<true>(grp, x, k);
+  }
+  template <class G>
+  __device__ __forceinline__ void rhs_unforced(const G& grp, const T (&x)[DL], T (&k)[DL]) const {
+    eval<false>(grp, x, k);
+  }
+  __device__ __forceinline__ T forcing() const { return F; }
 };
 
 // Scalar linear-Gaussian: dx/dt = -lambda x.
@@ -159,6 +176,7 @@ template <class T>
 struct ModelLG {
   static constexpr int D = 1, LPP = 1, DL = 1, NOBS = 1;
   static constexpr bool NOISE_FIRST = false;
+  static constexpr bool HAS_FORCING = false;
   T lambda;
   double noise_sd, obs_sd;
   __device__ __forceinline__ explicit ModelLG(const ModelArgs& a) {
@@ -233,6 +251,14 @@ __device__ __forceinline__ T err_scale(T err, T h, T atol, T rtol, T a, T b) {
 // h *= clamp(safe e2^-expo errold^beta, facl, facr), errold = sqrt(max(e2, 1e-8)).
 // Its logarithms run in double for double states and in float (native log2 /
 // exp2) for float states.
+// Stage derivative: the model's right-hand side, or without its constant
+// forcing when the integrator carries the forcing itself (FOLD).
+template <bool FOLD, class M, class G, class T, int DL>
+__device__ __forceinline__ void stage_rhs(const M& m, const G& grp, const T (&x)[DL], T (&k)[DL]) {
+  if constexpr (FOLD) m.rhs_unforced(grp, x, k);
+  else m.rhs(grp, x, k);
+}
+
 __device__ __forceinline__ double ctl_log(double x) { return log(x); }
 __device__ __forceinline__ double ctl_exp(double x) { return exp(x); }
 __device__ __forceinline__ float ctl_log(float x) { return __logf(x); }
@@ -248,6 +274,16 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
   const T atol = (T)a.atol, rtol = (T)a.rtol;
   const T invD = T(1.0 / (double)M::D);
   const bool quirk = a.host_quirk != 0;
+  // FP32 states of a model with constant forcing F: the stage derivatives
+  // leave F out (k' = k - F) and each stage adds c_s F once, where c_s, the
+  // sum of the stage's coefficients, is its node (1/5, 3/10, 4/5, 8/9, 1;
+  // 1 for the solution weights; 0 for the error weights).  Algebraically the
+  // same step; it saves one add per component and stage.  FP64 states keep
+  // the reference's arithmetic order.
+  constexpr bool FOLD = M::HAS_FORCING && sizeof(T) == 4;
+  T fc = T(0);
+  if constexpr (FOLD) fc = m.forcing();
+  const T c2F = T(0.2) * fc, c3F = T(0.3) * fc, c4F = T(0.8) * fc, c5F = T(8.0 / 9.0) * fc;
 
   T k1[DL];
   double t = t1, h = a.h0;
@@ -260,22 +296,35 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       if (h <= 0.0) break;
     }
     const T hT = (T)h;
-    if (!k1in) m.rhs(grp, x0, k1);
+    if (!k1in) stage_rhs<FOLD>(m, grp, x0, k1);
     k1in = true;
     T xs[DL], a3[DL], a4[DL], a5[DL], a6[DL], y6[DL], er[DL], k[DL];
     // stage 1
+    if constexpr (FOLD) {
 #pragma unroll
-    for (int i = 0; i < DL; ++i) {
-      xs[i] = hT * (T(0.2) * k1[i]) + x0[i];
-      a3[i] = T(3.0 / 40.0) * k1[i];
-      a4[i] = T(44.0 / 45.0) * k1[i];
-      a5[i] = T(19372.0 / 6561.0) * k1[i];
-      a6[i] = T(9017.0 / 3168.0) * k1[i];
-      y6[i] = T(35.0 / 384.0) * k1[i];
-      er[i] = T(71.0 / 57600.0) * k1[i];
+      for (int i = 0; i < DL; ++i) {
+        xs[i] = hT * (T(0.2) * k1[i] + c2F) + x0[i];
+        a3[i] = T(3.0 / 40.0) * k1[i] + c3F;
+        a4[i] = T(44.0 / 45.0) * k1[i] + c4F;
+        a5[i] = T(19372.0 / 6561.0) * k1[i] + c5F;
+        a6[i] = T(9017.0 / 3168.0) * k1[i] + fc;
+        y6[i] = T(35.0 / 384.0) * k1[i] + fc;
+        er[i] = T(71.0 / 57600.0) * k1[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < DL; ++i) {
+        xs[i] = hT * (T(0.2) * k1[i]) + x0[i];
+        a3[i] = T(3.0 / 40.0) * k1[i];
+        a4[i] = T(44.0 / 45.0) * k1[i];
+        a5[i] = T(19372.0 / 6561.0) * k1[i];
+        a6[i] = T(9017.0 / 3168.0) * k1[i];
+        y6[i] = T(35.0 / 384.0) * k1[i];
+        er[i] = T(71.0 / 57600.0) * k1[i];
+      }
     }
     // stage 2
-    m.rhs(grp, xs, k);
+    stage_rhs<FOLD>(m, grp, xs, k);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
       a3[i] += T(9.0 / 40.0) * k[i];
@@ -285,7 +334,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       xs[i] = hT * a3[i] + x0[i];
     }
     // stage 3
-    m.rhs(grp, xs, k);
+    stage_rhs<FOLD>(m, grp, xs, k);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
       a4[i] += T(32.0 / 9.0) * k[i];
@@ -296,7 +345,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       xs[i] = hT * a4[i] + x0[i];
     }
     // stage 4
-    m.rhs(grp, xs, k);
+    stage_rhs<FOLD>(m, grp, xs, k);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
       a5[i] += T(-212.0 / 729.0) * k[i];
@@ -306,7 +355,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       xs[i] = hT * a5[i] + x0[i];
     }
     // stage 5
-    m.rhs(grp, xs, k);
+    stage_rhs<FOLD>(m, grp, xs, k);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
       a6[i] += T(-5103.0 / 18656.0) * k[i];
@@ -315,7 +364,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       xs[i] = hT * a6[i] + x0[i];
     }
     // stage 6: xs now holds x5
-    m.rhs(grp, xs, k);
+    stage_rhs<FOLD>(m, grp, xs, k);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
       y6[i] += T(11.0 / 84.0) * k[i];
@@ -328,7 +377,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
 #pragma unroll
       for (int i = 0; i < DL; ++i) k7[i] = k[i];
     } else {
-      m.rhs(grp, y6, k7);
+      stage_rhs<FOLD>(m, grp, y6, k7);
     }
     T e2l = T(0);
 #pragma unroll

@@ -108,58 +108,360 @@ int launch_cumulative_offspring(const OffspringLaunch& L, hipStream_t s) {
   return BIHIP_OK;
 }
 
-// ---------------------------------------------------------------- permuted layout
+// ---------------------------------------------------------------- tiled chain
+// The permuted layout and the filter's whole resampling chain run as
+// reduce-then-scan over tiles of SCAN_TILE consecutive particles.  Every
+// dependency between tiles crosses a kernel boundary; no tile waits for
+// another inside a launch:
+//   k_weight_sums      tile sums of the fixed-point weights
+//   k_tiles_exclusive  exclusive scan of tile sums, in one block; only above
+//                      SMALL_TILES tiles (below, every block sums the tile
+//                      sums before its own, which stay in L2)
+//   k_offspring_tiles  the tile's inclusive weight sums Ws_i -> cumulative
+//                      offspring O_i (systematic, stratified) or Ws
+//                      (multinomial); tile sums of (e, f)
+//   k_ef_sums          tile sums of (e, f) for offspring given from outside
+//   k_place_tiles      the tile's exclusive E and F: E_k stored, occupied
+//                      slots keep their particle, free slot k listed at rank F_k
+//   k_place_free       one thread per free slot of rank r: the largest i with
+//                      E_i <= r, by binary search over E
+// All sums are integers, so the results equal a sequential scan's exactly.
+constexpr int SMALL_TILES = 1024;
+
 __device__ __forceinline__ int offspring_at(const int* in, bool cumulative, long i) {
   return cumulative ? in[i] - (i > 0 ? in[i - 1] : 0) : in[i];
 }
 
-struct PermLoader {
-  const int* in;
-  int cumulative;
-  __device__ uint64_t operator()(long i) const {
-    const int o = offspring_at(in, cumulative != 0, i);
-    const uint64_t e = o > 1 ? (uint64_t)(o - 1) : 0u;
-    const uint64_t f = o == 0 ? 1u : 0u;
-    return e | (f << 32);
+// (e, f) = (max(o - 1, 0), [o == 0]) packed as e | f << 32.
+__device__ __forceinline__ uint64_t ef_pair(int o) {
+  const uint64_t e = o > 1 ? (uint64_t)(o - 1) : 0u;
+  const uint64_t f = o == 0 ? 1u : 0u;
+  return e | (f << 32);
+}
+
+// Regions of the workspace's tile-chain words (the upper half of each tile's
+// SCAN_WORDS; the totals follow the tile sums).
+struct TileWork {
+  u128* wsum;           // weight sum per tile
+  uint64_t* efsum;      // (e, f) sum per tile
+  u128* wtotal;         // after k_tiles_exclusive
+  uint64_t* eftotal;
+  uint32_t* free_slots;
+};
+
+static TileWork tile_work(const ScanWorkspace& ws) {
+  const size_t mt = (size_t)ws.max_tiles;
+  uint64_t* base = ws.words + 4 * mt;
+  TileWork w;
+  w.wsum = reinterpret_cast<u128*>(base);  // 2 mt words
+  w.efsum = base + 2 * mt;                 // mt words
+  w.wtotal = reinterpret_cast<u128*>(base + 3 * mt);
+  w.eftotal = base + 3 * mt + 2;           // used only with mt > SMALL_TILES
+  w.free_slots = &ws.state->free_slots;
+  return w;
+}
+
+// Sum of the tile sums before `tile` and of all of them.
+template <class V>
+__device__ __forceinline__ void tile_prefix(const V* sums, const V* total, int nt, int tile,
+                                            V* lds, V& pre, V& tot) {
+  if (nt <= SMALL_TILES) {
+    V p(0), t(0);
+    for (int j = threadIdx.x; j < nt; j += SCAN_BLOCK) {
+      const V v = sums[j];
+      t = t + v;
+      if (j < tile) p = p + v;
+    }
+    pre = block_sum_v(p, lds);
+    tot = block_sum_v(t, lds);
+  } else {
+    pre = sums[tile];
+    tot = *total;
   }
-};
+}
 
-struct StoreExclusive64 {
-  uint64_t* out;
-  __device__ void operator()(long i, uint64_t, uint64_t excl) const { out[i] = excl; }
-};
-
-__global__ __launch_bounds__(RS_BLOCK) void k_place(const int* in, int cumulative, int P,
-                                                    const uint64_t* EF, int* as,
-                                                    const PFDevice* gate) {
+__global__ __launch_bounds__(SCAN_BLOCK) void k_weight_sums(const double* lws, const double* mx,
+                                                            int P, u128* wsum,
+                                                            const PFDevice* gate) {
+  __shared__ u128 lds[SCAN_BLOCK / WAVE];
   if (!gate_open(gate)) return;
-  const int k = blockIdx.x * RS_BLOCK + threadIdx.x;
-  if (k >= P) return;
-  const int o = offspring_at(in, cumulative != 0, k);
-  if (o > 0) {
-    as[k] = k;
+  const double m = *mx;
+  const long t0 = (long)blockIdx.x * SCAN_TILE;
+  u128 s(0);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const long i = t0 + (long)j * SCAN_BLOCK + threadIdx.x;
+    if (i < P) s = s + fix_weight(nanexp(lws[i] - m));
+  }
+  s = block_sum_v(s, lds);
+  if (threadIdx.x == 0) wsum[blockIdx.x] = s;
+}
+
+// In-place exclusive scan of nt tile sums by one block; total receives the sum.
+template <class V>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_tiles_exclusive(V* sums, int nt, V* total,
+                                                                const PFDevice* gate) {
+  __shared__ V lds[SCAN_BLOCK / WAVE];
+  if (!gate_open(gate)) return;
+  V carry(0);
+  for (long c0 = 0; c0 < nt; c0 += SCAN_TILE) {
+    const long base = c0 + (long)threadIdx.x * SCAN_ITEMS;
+    V v[SCAN_ITEMS];
+    V t(0);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+      v[j] = base + j < nt ? sums[base + j] : V(0);
+      t = t + v[j];
+    }
+    V agg;
+    V run = carry + block_exclusive_scan(t, lds, agg);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+      if (base + j < nt) sums[base + j] = run;
+      run = run + v[j];
+    }
+    carry = carry + agg;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+struct OffspringTileArgs {
+  const double* lws;
+  const double* mx;
+  int P, n, nt;
+  const u128* wsum;
+  const u128* wtotal;
+  uint64_t* efsum;
+  uint64_t seed;
+  uint32_t batch;
+  const PFDevice* gate;
+  PFDevice* status;
+  double* Ws;
+  int* Os;
+};
+
+// MODE: BIHIP_SYSTEMATIC or BIHIP_STRATIFIED (cumulative offspring, as
+// k_cumulative_offspring from the Ws of launch_sumexp_scan, value for value),
+// or BIHIP_MULTINOMIAL (the inclusive sums Ws only).
+template <int MODE>
+__global__ __launch_bounds__(SCAN_BLOCK) void k_offspring_tiles(OffspringTileArgs a) {
+  __shared__ u128 lds[SCAN_BLOCK / WAVE];
+  __shared__ uint64_t lds64[SCAN_BLOCK / WAVE];
+  if (!gate_open(a.gate)) return;
+  const int tile = blockIdx.x;
+  u128 pre, tot;
+  tile_prefix(a.wsum, a.wtotal, a.nt, tile, lds, pre, tot);
+  const double W = fix_to_double(tot);
+  if (MODE != BIHIP_MULTINOMIAL && !(W > 0.0)) {
+    if (a.status != nullptr && tile == 0 && threadIdx.x == 0) a.status->status = BIHIP_DEGENERATE;
     return;
   }
-  const uint32_t r = (uint32_t)(EF[k] >> 32);  // rank of this free slot
-  int lo = 0, hi = P;                          // first i with E_i > r
+  const double m = *a.mx;
+  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  u128 v[SCAN_ITEMS];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i)
+    v[i] = base + i < a.P ? fix_weight(nanexp(a.lws[base + i] - m)) : u128(0);
+#pragma unroll
+  for (int i = 1; i < SCAN_ITEMS; ++i) v[i] = v[i - 1] + v[i];
+  u128 agg;
+  const u128 start = pre + block_exclusive_scan(v[SCAN_ITEMS - 1], lds, agg);  // Ws_{base-1}
+  if constexpr (MODE == BIHIP_MULTINOMIAL) {
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+      if (base + i < a.P) a.Ws[base + i] = fix_to_double(start + v[i]);
+    return;
+  } else {
+    const uint32_t step = (uint32_t)a.gate->k;
+    double av = 0.0;
+    if (MODE == BIHIP_SYSTEMATIC) {
+      const u32x4 r = philox_block(a.seed, 0, step, TAG_SYSTEMATIC, a.batch, 0);
+      av = u53(r.x, r.y);
+    }
+    auto cumulative = [&](const u128& incl) -> int {
+      const double Wsi = fix_to_double(incl);
+      if (MODE == BIHIP_SYSTEMATIC) return systematic_offspring(Wsi, W, a.n, av);
+      const double reach = stratified_reach(Wsi, W, a.n);
+      int k = (int)reach;
+      if (k > a.n - 1) k = a.n - 1;
+      const u32x4 r = philox_block(a.seed, (uint32_t)k, step, TAG_STRATIFIED, a.batch, 0);
+      return stratified_offspring(reach, u53(r.x, r.y), a.n);
+    };
+    uint64_t ef = 0;
+    if (base < a.P) {
+      int prev = base > 0 ? cumulative(start) : 0;
+#pragma unroll
+      for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i >= a.P) break;
+        const int O = cumulative(start + v[i]);
+        a.Os[base + i] = O;
+        ef += ef_pair(O - prev);
+        prev = O;
+      }
+    }
+    ef = block_sum_v(ef, lds64);
+    if (threadIdx.x == 0) a.efsum[tile] = ef;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_ef_sums(const int* in, int cumulative, int P,
+                                                        uint64_t* efsum, const PFDevice* gate) {
+  __shared__ uint64_t lds[SCAN_BLOCK / WAVE];
+  if (!gate_open(gate)) return;
+  const long t0 = (long)blockIdx.x * SCAN_TILE;
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const long i = t0 + (long)j * SCAN_BLOCK + threadIdx.x;
+    if (i < P) s += ef_pair(offspring_at(in, cumulative != 0, i));
+  }
+  s = block_sum_v(s, lds);
+  if (threadIdx.x == 0) efsum[blockIdx.x] = s;
+}
+
+struct PlaceTileArgs {
+  const int* in;
+  int cumulative;
+  int P, nt;
+  const uint64_t* efsum;
+  const uint64_t* eftotal;
+  uint32_t* E;
+  uint32_t* slots;
+  int* as;
+  uint32_t* free_slots;
+  const PFDevice* gate;
+};
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
+  __shared__ uint64_t lds[SCAN_BLOCK / WAVE];
+  if (!gate_open(a.gate)) return;
+  const int tile = blockIdx.x;
+  uint64_t pre, tot;
+  tile_prefix(a.efsum, a.eftotal, a.nt, tile, lds, pre, tot);
+  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  int o[SCAN_ITEMS];
+  uint64_t t = 0;
+  int prev = (a.cumulative && base > 0 && base < a.P) ? a.in[base - 1] : 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    o[i] = 1;  // past the end: no extras, not free
+    if (base + i < a.P) {
+      const int c = a.in[base + i];
+      o[i] = a.cumulative ? c - prev : c;
+      prev = c;
+    }
+    t += ef_pair(o[i]);
+  }
+  uint64_t agg;
+  uint64_t run = pre + block_exclusive_scan(t, lds, agg);
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    const long k = base + i;
+    if (k < a.P) {
+      a.E[k] = (uint32_t)run;
+      if (o[i] > 0) a.as[k] = (int)k;
+      else a.slots[(uint32_t)(run >> 32)] = (uint32_t)k;
+    }
+    run += ef_pair(o[i]);
+  }
+  if (tile == a.nt - 1 && threadIdx.x == 0) *a.free_slots = (uint32_t)(tot >> 32);
+}
+
+// Free slot of rank r receives the largest i with E_i <= r (the particle
+// whose extra copies cover rank r).
+__global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, const uint32_t* slots,
+                                                         const uint32_t* free_slots, int P,
+                                                         int* as, const PFDevice* gate) {
+  if (!gate_open(gate)) return;
+  const uint32_t r = blockIdx.x * RS_BLOCK + threadIdx.x;
+  if (r >= *free_slots) return;
+  const uint32_t k = slots[r];
+  int lo = 0, hi = P;  // first i with E_i > r
   while (lo < hi) {
     const int mid = lo + ((hi - lo) >> 1);
-    if ((uint32_t)EF[mid] <= r) lo = mid + 1;
+    if (E[mid] <= r) lo = mid + 1;
     else hi = mid;
   }
   as[k] = lo > 0 ? lo - 1 : 0;
 }
 
+// (e, f) tile sums in place -> E, the free-slot list and the ancestors.
+static int place_from_sums(const int* in, bool cumulative, int P, int nt, int* as, uint64_t* EF,
+                           const TileWork& w, const PFDevice* gate, hipStream_t s) {
+  if (nt > SMALL_TILES) {
+    hipLaunchKernelGGL(k_tiles_exclusive<uint64_t>, dim3(1), dim3(SCAN_BLOCK), 0, s, w.efsum, nt,
+                       w.eftotal, gate);
+    BIHIP_LAUNCH_CHECK();
+  }
+  uint32_t* E = reinterpret_cast<uint32_t*>(EF);  // P words of E, then P of slots
+  PlaceTileArgs a{in, cumulative ? 1 : 0, P, nt, w.efsum, w.eftotal, E, E + P, as,
+                  w.free_slots, gate};
+  hipLaunchKernelGGL(k_place_tiles, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_place_free, dim3(ceil_div(P, RS_BLOCK)), dim3(RS_BLOCK), 0, s, E, E + P,
+                     w.free_slots, P, as, gate);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
 int launch_ancestors_permute(const int* in, bool cumulative, int P, int* as, uint64_t* EF,
                              ScanWorkspace ws, const PFDevice* gate, hipStream_t s) {
   if (P <= 0) return BIHIP_OK;
-  int rc = launch_tile_scan<uint64_t>(PermLoader{in, cumulative ? 1 : 0}, StoreExclusive64{EF},
-                                     PFGate{gate}, P, ws, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_place, dim3(ceil_div(P, RS_BLOCK)), dim3(RS_BLOCK), 0, s, in,
-                     cumulative ? 1 : 0, P, EF, as, gate);
+  const int nt = ceil_div(P, SCAN_TILE);
+  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
+  const TileWork w = tile_work(ws);
+  hipLaunchKernelGGL(k_ef_sums, dim3(nt), dim3(SCAN_BLOCK), 0, s, in, cumulative ? 1 : 0, P,
+                     w.efsum, gate);
   BIHIP_LAUNCH_CHECK();
-  return BIHIP_OK;
+  return place_from_sums(in, cumulative, P, nt, as, EF, w, gate, s);
+}
+
+int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
+  if (L.P <= 0) return BIHIP_OK;
+  if (L.gate == nullptr) return set_error(BIHIP_EINVAL, "the tiled resampling chain needs a gate");
+  const int nt = ceil_div(L.P, SCAN_TILE);
+  if (nt > L.ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
+  const TileWork w = tile_work(L.ws);
+  hipLaunchKernelGGL(k_weight_sums, dim3(nt), dim3(SCAN_BLOCK), 0, s, L.lws, L.mx, L.P, w.wsum,
+                     L.gate);
+  BIHIP_LAUNCH_CHECK();
+  if (nt > SMALL_TILES) {
+    hipLaunchKernelGGL(k_tiles_exclusive<u128>, dim3(1), dim3(SCAN_BLOCK), 0, s, w.wsum, nt,
+                       w.wtotal, L.gate);
+    BIHIP_LAUNCH_CHECK();
+  }
+  OffspringTileArgs a{L.lws, L.mx, L.P, L.n, nt, w.wsum, w.wtotal, w.efsum, L.seed, L.batch,
+                      L.gate, L.status, L.Ws, L.Os};
+  switch (L.mode) {
+    case BIHIP_SYSTEMATIC:
+      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_SYSTEMATIC>, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+      break;
+    case BIHIP_STRATIFIED:
+      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_STRATIFIED>, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+      break;
+    case BIHIP_MULTINOMIAL:
+      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_MULTINOMIAL>, dim3(nt), dim3(SCAN_BLOCK), 0, s,
+                         a);
+      break;
+    default:
+      return set_error(BIHIP_EINVAL, "unknown resampler %d", L.mode);
+  }
+  BIHIP_LAUNCH_CHECK();
+  if (L.mode != BIHIP_MULTINOMIAL)
+    return place_from_sums(L.Os, true, L.P, nt, L.as, L.EF, w, L.gate, s);
+  int rc;
+  if ((rc = launch_zero_ints(L.os, L.P, L.gate, s))) return rc;
+  MultinomialLaunch M{};
+  M.Ws = L.Ws;
+  M.P = L.P;
+  M.n = L.n;
+  M.seed = L.seed;
+  M.batch = L.batch;
+  M.gate = L.gate;
+  M.status = L.status;
+  M.os = L.os;
+  if ((rc = launch_multinomial(M, s))) return rc;
+  return launch_ancestors_permute(L.os, false, L.P, L.as, L.EF, L.ws, L.gate, s);
 }
 
 // as_j = #{i : O_i <= j}: slot j of the unpermuted ancestor vector.

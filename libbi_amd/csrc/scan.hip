@@ -19,31 +19,21 @@ int scan_resident_tiles() {
   return limit;
 }
 
-// Weights w = nanexp(lw - max) lie in [0, 1]; they are summed exactly as
-// 128-bit integers with 62 fractional bits (w * 2^62 is exact in double and
-// truncates to an integer below 2^62) and each prefix is rounded to double
-// once.  Exact integer sums make the scan deterministic and monotone: a
-// floating-point parallel scan rounds differently across tile boundaries and
-// can step backwards, which would give a particle negative offspring.
-constexpr double FIX_SCALE = 0x1.0p62;
-constexpr double FIX_UNIT = 0x1.0p-62;
-
+// Weights w = nanexp(lw - max) lie in [0, 1]; they are summed exactly in
+// 128-bit fixed point (fix_weight) and each prefix is rounded to double once
+// (fix_to_double).  Exact integer sums make the scan deterministic and
+// monotone: a floating-point parallel scan rounds differently across tile
+// boundaries and can step backwards, which would give a particle negative
+// offspring.
 struct SumExpLoader {
   const double* lws;
   const double* mx;
-  __device__ u128 operator()(long i) const {
-    const double w = nanexp(lws[i] - *mx);
-    return u128((uint64_t)(w * FIX_SCALE));
-  }
+  __device__ u128 operator()(long i) const { return fix_weight(nanexp(lws[i] - *mx)); }
 };
 
 struct StoreInclusive {
   double* out;
-  __device__ void operator()(long i, u128 incl, u128) const {
-    // hi * 2^64 * 2^-62 is exact; lo is rounded once; the sum rounds once
-    // (monotone in (hi, lo), including when lo wraps into hi).
-    out[i] = (double)incl.hi * 4.0 + (double)incl.lo * FIX_UNIT;
-  }
+  __device__ void operator()(long i, u128 incl, u128) const { out[i] = fix_to_double(incl); }
 };
 
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,

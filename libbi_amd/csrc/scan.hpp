@@ -140,6 +140,47 @@ __device__ __forceinline__ V block_sum_v(V v, V* lds) {
   return r;
 }
 
+// Exclusive scan of one value per thread across the block, in a fixed order;
+// agg receives the block total.  lds holds SCAN_BLOCK / WAVE values.
+template <class V>
+__device__ __forceinline__ V block_exclusive_scan(V v, V* lds, V& agg) {
+  constexpr int NW = SCAN_BLOCK / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  V incl = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const V u = shfl_up_v(incl, o);
+    if (lane >= o) incl = u + incl;
+  }
+  V excl = shfl_up_v(incl, 1);
+  if (lane == 0) excl = V(0);
+  __syncthreads();  // lds may still be read by an earlier reduction
+  if (lane == WAVE - 1) lds[w] = incl;
+  __syncthreads();
+  V wpre = V(0), a = V(0);
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    if (i < w) wpre = wpre + lds[i];
+    a = a + lds[i];
+  }
+  agg = a;
+  return wpre + excl;
+}
+
+// Weights w in [0, 1] as 128-bit integers with 62 fractional bits (w * 2^62 is
+// exact in double and truncates to an integer below 2^62).  Sums of them are
+// exact, so every association order gives the same monotone prefixes.
+constexpr double FIX_SCALE = 0x1.0p62;
+constexpr double FIX_UNIT = 0x1.0p-62;
+
+__device__ __forceinline__ u128 fix_weight(double w) { return u128((uint64_t)(w * FIX_SCALE)); }
+
+// hi * 2^64 * 2^-62 is exact; lo is rounded once; the sum rounds once
+// (monotone in (hi, lo), including when lo wraps into hi).
+__device__ __forceinline__ double fix_to_double(const u128& v) {
+  return (double)v.hi * 4.0 + (double)v.lo * FIX_UNIT;
+}
+
 // Loader: V operator()(long i) const, valid for i < P.
 // Writer: void operator()(long i, V inclusive, V exclusive) const.
 // Gate:   bool operator()() const (false: the launch does nothing).
@@ -183,25 +224,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
   for (int i = 1; i < ITEMS; ++i) v[i] = v[i - 1] + v[i];
   const V total = v[ITEMS - 1];
 
-  // block exclusive scan of thread totals: wave scan, then wave prefixes
-  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
-  V incl = total;
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const V u = shfl_up_v(incl, o);
-    if (lane >= o) incl = u + incl;
-  }
-  V excl = shfl_up_v(incl, 1);
-  if (lane == 0) excl = V(0);
-  if (lane == WAVE - 1) lds[w] = incl;
-  __syncthreads();
-  V wpre = V(0), agg = V(0);
-#pragma unroll
-  for (int i = 0; i < NW; ++i) {
-    if (i < w) wpre = wpre + lds[i];
-    agg = agg + lds[i];
-  }
-  const V texcl = wpre + excl;
+  // block exclusive scan of thread totals
+  V agg;
+  const V texcl = block_exclusive_scan(total, lds, agg);
 
   if (threadIdx.x == 0) publish(ws.words + (size_t)NC * tile, epoch, agg);
 

@@ -48,6 +48,21 @@ def test_pz_filter_fp64_parity(ctx, orc, resampler):
     np.testing.assert_array_equal(anc, ref["ancestors"])
 
 
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.STRATIFIED, capi.MULTINOMIAL])
+def test_pz_filter_large_population_parity(ctx, orc, resampler):
+    """N = 2^21 + 5: more than 1024 tiles of 2048 particles, so the resampling
+    chain scans its tile sums in a launch of their own; ancestors stay bit-exact."""
+    y = _pz_data(orc, 3)
+    gm = capi.make_model("pz", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    P = (1 << 21) + 5
+    logZ, out, anc = _run_gpu(ctx, gm, y, P, 1.0, 43, resampler, 1.0)
+    ref = orc.pf_run(om, y, P, 1.0, 43, resampler=resampler, ess_rel=1.0, want_ancestors=True)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
 def test_l96_filter_fp64_parity(ctx, orc):
     """Lorenz-96 d=40, DOPRI5 rtol=atol=1e-5, FP64 state, N=4096, T=20, ESS<0.5N."""
     om0 = orc.make_model("l96", d=40)
