@@ -28,14 +28,16 @@ struct u32x4 {
   uint32_t x, y, z, w;
 };
 
-// Philox4x32-10 (Salmon et al., SC'11).  The 32x32->64 products use
-// __umulhi/ordinary multiplication; the key schedule is uniform per launch.
+// Philox4x32-10 (Salmon et al., SC'11).  Each 32x32->64 product is one
+// 64-bit multiply (a single v_mad_u64_u32 rather than a low and a high
+// multiply); the key schedule is uniform per launch.
 __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0;
     const uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0;

@@ -235,12 +235,19 @@ __device__ __forceinline__ int integrate_rk4(const M& m, const G& grp, T (&x0)[M
   return n;
 }
 
+// Scaled error of one component: err h / (atol + rtol max(|a|, |b|)).
+// double: as written.  float: err / den with v_rcp_f32 (1 ulp; the norm only
+// steers h), the factor h applied once to the whole norm by the caller, and
+// the larger magnitude by a compare and select (fmax would first quiet
+// signalling NaNs of both operands; a NaN state already makes err NaN).
 template <class T>
 __device__ __forceinline__ T err_scale(T err, T h, T atol, T rtol, T a, T b) {
-  const T den = atol + rtol * fmax(fabs(a), fabs(b));
   if constexpr (sizeof(T) == 4) {
-    return err * h * __builtin_amdgcn_rcpf(den);  // v_rcp_f32 (1 ulp): the norm only steers h
+    const T fa = fabs(a), fb = fabs(b);
+    const T den = atol + rtol * (fa < fb ? fb : fa);
+    return err * __builtin_amdgcn_rcpf(den);
   } else {
+    const T den = atol + rtol * fmax(fabs(a), fabs(b));
     return err * h / den;
   }
 }
@@ -386,7 +393,8 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       const T e = err_scale(er[i], hT, atol, rtol, x0[i], y6[i]);
       e2l += e * e;
     }
-    const T e2 = grp.sum(e2l) * invD;
+    T e2 = grp.sum(e2l) * invD;
+    if constexpr (sizeof(T) == 4) e2 *= hT * hT;  // err_scale leaves h out for float
     if (e2 <= T(1)) {
       t += h;
 #pragma unroll
