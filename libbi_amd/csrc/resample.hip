@@ -161,6 +161,20 @@ static TileWork tile_work(const ScanWorkspace& ws) {
   return w;
 }
 
+// Tile exchanges through LDS between the striped order (element
+// j * SCAN_BLOCK + tid of the tile: coalesced global access) and the blocked
+// order (element tid * SCAN_ITEMS + j: a thread's consecutive elements, as
+// the scans need them).  One padding element per 128 bytes spreads the
+// blocked accesses over the banks.
+template <class V>
+__device__ __forceinline__ int xpos(int i) {
+  return i + i / (128 / (int)sizeof(V));
+}
+template <class V>
+constexpr int xsize() {
+  return SCAN_TILE + SCAN_TILE / (128 / (int)sizeof(V));
+}
+
 // Sum of the tile sums before `tile` and of all of them.
 template <class V>
 __device__ __forceinline__ void tile_prefix(const V* sums, const V* total, int nt, int tile,
@@ -332,57 +346,117 @@ struct PlaceTileArgs {
   const PFDevice* gate;
 };
 
+constexpr uint32_t OCCUPIED = 0xFFFFFFFFu;
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
   __shared__ uint64_t lds[SCAN_BLOCK / WAVE];
+  __shared__ int xi[xsize<int>()];       // offspring input, then E
+  __shared__ uint32_t xf[xsize<int>()];  // free-slot rank, or OCCUPIED
+  __shared__ int s_before;
   if (!gate_open(a.gate)) return;
   const int tile = blockIdx.x;
+  const long t0 = (long)tile * SCAN_TILE;
   uint64_t pre, tot;
   tile_prefix(a.efsum, a.eftotal, a.nt, tile, lds, pre, tot);
-  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const int e = j * SCAN_BLOCK + threadIdx.x;
+    xi[xpos<int>(e)] = t0 + e < a.P ? a.in[t0 + e] : 0;
+  }
+  if (threadIdx.x == 0) s_before = (a.cumulative && t0 > 0) ? a.in[t0 - 1] : 0;
+  __syncthreads();
+  const int b0 = threadIdx.x * SCAN_ITEMS;
+  const long base = t0 + b0;
   int o[SCAN_ITEMS];
   uint64_t t = 0;
-  int prev = (a.cumulative && base > 0 && base < a.P) ? a.in[base - 1] : 0;
+  int prev = b0 > 0 ? xi[xpos<int>(b0 - 1)] : s_before;
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i) {
     o[i] = 1;  // past the end: no extras, not free
     if (base + i < a.P) {
-      const int c = a.in[base + i];
+      const int c = xi[xpos<int>(b0 + i)];
       o[i] = a.cumulative ? c - prev : c;
       prev = c;
     }
     t += ef_pair(o[i]);
   }
   uint64_t agg;
-  uint64_t run = pre + block_exclusive_scan(t, lds, agg);
+  uint64_t run = pre + block_exclusive_scan(t, lds, agg);  // its barriers end the reads of xi
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i) {
-    const long k = base + i;
-    if (k < a.P) {
-      a.E[k] = (uint32_t)run;
-      if (o[i] > 0) a.as[k] = (int)k;
-      else a.slots[(uint32_t)(run >> 32)] = (uint32_t)k;
-    }
+    xi[xpos<int>(b0 + i)] = (int)(uint32_t)run;
+    xf[xpos<int>(b0 + i)] = o[i] > 0 ? OCCUPIED : (uint32_t)(run >> 32);
     run += ef_pair(o[i]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const int e = j * SCAN_BLOCK + threadIdx.x;
+    const long k = t0 + e;
+    if (k < a.P) {
+      a.E[k] = (uint32_t)xi[xpos<int>(e)];
+      const uint32_t f = xf[xpos<int>(e)];
+      if (f == OCCUPIED) a.as[k] = (int)k;
+      else a.slots[f] = (uint32_t)k;
+    }
   }
   if (tile == a.nt - 1 && threadIdx.x == 0) *a.free_slots = (uint32_t)(tot >> 32);
 }
 
 // Free slot of rank r receives the largest i with E_i <= r (the particle
-// whose extra copies cover rank r).
+// whose extra copies cover rank r).  A wave holds 64 consecutive ranks, whose
+// owners are non-decreasing: it narrows one interval [lo, hi) around all of
+// them by 64-ary search (each lane loads one pivot; a ballot counts the pivots
+// <= the first and the last rank), so a search over 2^20 particles takes
+// four dependent loads instead of twenty.  Once the interval holds at most 64
+// particles every lane finds its own owner among them by shuffles; if the
+// owners are spread too far apart to narrow further, each lane finishes with
+// a binary search inside the interval.
 __global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, const uint32_t* slots,
                                                          const uint32_t* free_slots, int P,
                                                          int* as, const PFDevice* gate) {
   if (!gate_open(gate)) return;
+  const uint32_t Z = *free_slots;
   const uint32_t r = blockIdx.x * RS_BLOCK + threadIdx.x;
-  if (r >= *free_slots) return;
-  const uint32_t k = slots[r];
-  int lo = 0, hi = P;  // first i with E_i > r
-  while (lo < hi) {
-    const int mid = lo + ((hi - lo) >> 1);
-    if (E[mid] <= r) lo = mid + 1;
-    else hi = mid;
+  const uint32_t rA = r & ~(uint32_t)(WAVE - 1);  // the wave's first rank
+  if (rA >= Z) return;                             // uniform across the wave
+  const int lane = threadIdx.x & (WAVE - 1);
+  const bool valid = r < Z;
+  const uint32_t rB = min(rA + (uint32_t)(WAVE - 1), Z - 1);
+  const uint32_t k = valid ? slots[r] : 0u;
+  int lo = 0, hi = P;  // E[lo] <= rA, and the owners of ranks rA..rB lie in [lo, hi)
+  while (hi - lo > WAVE) {
+    const int step = (hi - lo + WAVE - 1) / WAVE;
+    const long idx = (long)lo + (long)lane * step;
+    const uint32_t piv = idx < hi ? E[idx] : 0xFFFFFFFFu;
+    const int cA = __popcll(__ballot(piv <= rA));  // >= 1: pivot 0 is E[lo]
+    const int cB = __popcll(__ballot(piv <= rB));
+    const int nlo = lo + (cA - 1) * step;
+    const int nhi = min(hi, lo + cB * step);
+    if (nhi - nlo >= hi - lo) break;  // spread over every pivot interval
+    lo = nlo;
+    hi = nhi;
   }
-  as[k] = lo > 0 ? lo - 1 : 0;
+  int owner;
+  if (hi - lo <= WAVE) {
+    const uint32_t mine = lo + lane < hi ? E[lo + lane] : 0xFFFFFFFFu;
+    int pos = 0;  // E[lo] <= r for every rank of the wave
+#pragma unroll
+    for (int b = WAVE / 2; b >= 1; b >>= 1) {
+      const uint32_t v = (uint32_t)__shfl((int)mine, pos + b, WAVE);
+      if (v <= r) pos += b;  // padding lanes hold 0xFFFFFFFF and never qualify
+    }
+    owner = lo + pos;
+  } else {
+    int a = lo, b = hi;  // first i in [lo, hi) with E_i > r
+    while (a < b) {
+      const int mid = a + ((b - a) >> 1);
+      if (E[mid] <= r) a = mid + 1;
+      else b = mid;
+    }
+    owner = a - 1;
+  }
+  if (valid) as[k] = owner;
 }
 
 // (e, f) tile sums in place -> E, the free-slot list and the ancestors.

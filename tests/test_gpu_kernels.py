@@ -309,7 +309,8 @@ def _offspring_cases(P, rng):
     yield rng.multinomial(P, rng.dirichlet(np.full(P, 0.05)))
 
 
-@pytest.mark.parametrize("P", [1, 2, 3, 31, 2048, 2049, 100000, 1 << 20, (3 << 20) + 5])
+@pytest.mark.parametrize("P", [1, 2, 3, 31, 2048, 2049, 100000, 1 << 20, (3 << 20) + 5,
+                               (8 << 20) + 5])
 def test_permute_bit_exact(ctx, orc, torch_gpu, P):
     rng = np.random.default_rng(40 + P)
     for o in _offspring_cases(P, rng):
