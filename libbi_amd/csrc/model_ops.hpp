@@ -3,6 +3,8 @@
 // kernels are instantiated on.
 #pragma once
 
+#include <cmath>
+
 #include "models.hpp"
 
 namespace bihip {
@@ -19,6 +21,13 @@ inline ModelArgs to_args(const bihip_model* m) {
   a.atol = m->atol;
   a.rtol = m->rtol;
   for (int i = 0; i < 16; ++i) a.par[i] = m->par[i];
+  a.obs_pre = 0;
+  a.obs_inv = a.obs_c = 0.0;
+  if (m->model == BIHIP_MODEL_L96 && m->par[2] > 0.0) {  // obs_sd of the Lorenz-96 model
+    a.obs_pre = 1;
+    a.obs_inv = 1.0 / m->par[2];
+    a.obs_c = -0.91893853320467274178 - std::log(m->par[2]);
+  }
   return a;
 }
 
@@ -37,10 +46,17 @@ __device__ __forceinline__ double obs_terms(const ModelPZ<T>& m, const T (&x)[2]
 template <class T, int D, int LPP>
 __device__ __forceinline__ double obs_terms(const ModelL96<T, D, LPP>& m,
                                             const T (&x)[ModelL96<T, D, LPP>::DL],
-                                            const double* y, int lo) {
+                                            const double* y, int lo,
+                                            const ModelArgs* pre = nullptr) {
   constexpr int DL = ModelL96<T, D, LPP>::DL;
-  const double inv = 1.0 / m.obs_sd;
-  const double c = -HALF_LOG_TWO_PI - log(m.obs_sd);
+  double inv, c;
+  if (pre != nullptr && pre->obs_pre) {
+    inv = pre->obs_inv;
+    c = pre->obs_c;
+  } else {
+    inv = 1.0 / m.obs_sd;
+    c = -HALF_LOG_TWO_PI - log(m.obs_sd);
+  }
   double s = 0.0;
   if constexpr (DL % 2 == 0) {  // even slices start on an observed dimension
 #pragma unroll
@@ -111,18 +127,19 @@ struct LG : ModelLG<T> {
 
 template <class T>
 __device__ __forceinline__ double obs_of(const PZ<T>& m, const T (&x)[2], const double* y,
-                                         int lo) {
+                                         int lo, const ModelArgs* = nullptr) {
   return obs_terms(static_cast<const ModelPZ<T>&>(m), x, y, lo);
 }
 template <class T, int D, int LPP>
 __device__ __forceinline__ double obs_of(const L96<T, D, LPP>& m,
                                          const T (&x)[ModelL96<T, D, LPP>::DL],
-                                         const double* y, int lo) {
-  return obs_terms(static_cast<const ModelL96<T, D, LPP>&>(m), x, y, lo);
+                                         const double* y, int lo,
+                                         const ModelArgs* pre = nullptr) {
+  return obs_terms(static_cast<const ModelL96<T, D, LPP>&>(m), x, y, lo, pre);
 }
 template <class T>
 __device__ __forceinline__ double obs_of(const LG<T>& m, const T (&x)[1], const double* y,
-                                         int lo) {
+                                         int lo, const ModelArgs* = nullptr) {
   return obs_terms(static_cast<const ModelLG<T>&>(m), x, y, lo);
 }
 

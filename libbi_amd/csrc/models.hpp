@@ -26,6 +26,11 @@ struct ModelArgs {
   int model, d, integrator, nsteps, host_quirk, precision;
   double h0, atol, rtol;
   double par[16];
+  // Gaussian observation constants 1/obs_sd and -log(2 pi)/2 - log(obs_sd),
+  // computed once on the host when obs_pre != 0 (kernels read them from their
+  // arguments at the point of use, so they occupy no vector registers)
+  int obs_pre;
+  double obs_inv, obs_c;
 };
 
 // ---------------------------------------------------------------- lane groups

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
     if (a.nsteps_out != nullptr && grp.g == 0) a.nsteps_out[p] = n;
 
     if (y != nullptr) {
-      const double part = grp.sum(obs_of(m, x, y, lo));
+      const double part = grp.sum(obs_of(m, x, y, lo, &a.m));
       if (grp.g == 0) {
         const double base = lws_const ? lws_base : a.lws[p];
         lw = base + part;
@@ -104,7 +104,9 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
       }
     }
   }
-  if (a.partials != nullptr) block_weight_partial<TRANS_BLOCK>(lw, a.partials + 3 * blockIdx.x);
+  // only group leaders (lanes g == 0) hold a log-weight
+  if (a.partials != nullptr)
+    block_weight_partial<TRANS_BLOCK, LPP>(lw, a.partials + 3 * blockIdx.x);
 }
 
 // ---------------------------------------------------------------- initial states
