@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_transition launch at N=2^20, every step resampled: rocprofv3
 # --pmc FETCH_SIZE (x2.000, the factor the same counter shows on a 1 GiB copy)
 # plus WRITE_SIZE (x1.000), averaged over 7 launches.
-L96_TRAFFIC_BYTES = 410_305_843
+L96_TRAFFIC_BYTES = 410_475_582
 
 
 def parse():
