@@ -94,6 +94,25 @@ def test_l96_d200_filter_fp64_parity(ctx, orc):
     np.testing.assert_array_equal(anc, ref["ancestors"])
 
 
+def test_l96_d200_filter_fp32_mixed_precision(ctx, orc):
+    """d=200 with FP32 state (eight lanes per particle, the forcing carried in the
+    stage constants) and no resampling: the FP64 oracle's logZ and ESS to FP32
+    rounding."""
+    om0 = orc.make_model("l96", d=200)
+    mean0 = l96_spun_up(orc, d=200)
+    _, y = orc.simulate(om0, 7, 5, 0.05, mean0)
+    gm = capi.make_model("l96", d=200, precision=capi.F32)
+    om = oracle_model(orc, gm)
+    P = 2048
+    logZ, out, _ = _run_gpu(ctx, gm, y, P, 0.05, 9, capi.SYSTEMATIC, 0.0, mean0=mean0,
+                            record=False)
+    ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.0, mean0=mean0)
+    assert not out["resampled"].any()
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-5)
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=1e-4, atol=1e-2)
+    np.testing.assert_allclose(out["ess"], ref["ess"], rtol=1e-3)
+
+
 @pytest.mark.parametrize("T", [1, 10])
 def test_l96_filter_fp32_mixed_precision(ctx, orc, T):
     """FP32 transitions with FP64 log-weights and logZ against the FP64 oracle on
