@@ -282,9 +282,14 @@ int bihip_pf_position(bihip_pf* pf);
  * Kernel timing with HIP events on the filter's stream: enable, then query
  * the summed milliseconds and launch count of one kernel class
  * (0 = transition, 1 = resample chain, 2 = weight reduction, 3 = the guided
- * proposal kernel alone, which class 0 also covers).
+ * proposal kernel alone, which class 0 also covers).  mode: 0 off,
+ * BIHIP_TIMING_ALL every class, BIHIP_TIMING_TRANSITION classes 0 and 3 only
+ * (each recorded event costs several microseconds of stream time, so the
+ * narrower mode disturbs the timed steps less).
  */
-int bihip_pf_set_timing(bihip_pf* pf, int enable);
+#define BIHIP_TIMING_ALL 1
+#define BIHIP_TIMING_TRANSITION 2
+int bihip_pf_set_timing(bihip_pf* pf, int mode);
 int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_host);
 
 /* ================================================================ batched inner filters */

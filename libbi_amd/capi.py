@@ -22,6 +22,7 @@ F32, F64 = 0, 1
 LOSS_MSE, LOSS_GAUSS = 0, 1
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
 TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER, TAG_PATH = 7, 8, 9, 10, 13
+TIMING_ALL, TIMING_TRANSITION = 1, 2
 
 # exported entry points (must match include/bihip.h)
 EXPORTS = [
@@ -457,8 +458,11 @@ class ParticleFilter:
         check(lib().bihip_pf_set_guided(self.h, mlp.h if mlp is not None else None,
                                         ctypes.byref(norm) if norm is not None else None))
 
-    def set_timing(self, on):
-        check(lib().bihip_pf_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, on, transition_only=False):
+        """HIP events around the step's phases (kernel_time), or around the
+        transition and proposal kernels only."""
+        mode = (TIMING_TRANSITION if transition_only else TIMING_ALL) if on else 0
+        check(lib().bihip_pf_set_timing(self.h, mode))
 
     def kernel_time(self, kind):
         ms = ctypes.c_double()

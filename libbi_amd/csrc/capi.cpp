@@ -596,8 +596,8 @@ struct bihip_pf {
   ScanWorkspace ws{};
   int k = 0;
   bool ready = false, have_obs = false;
-  // timing
-  bool timing = false;
+  // timing: 0 off, BIHIP_TIMING_ALL, BIHIP_TIMING_TRANSITION
+  int timing = 0;
   std::vector<hipEvent_t> ev[4];
   size_t ev_used[4] = {0, 0, 0, 0};
   // graphs (one per buffer parity)
@@ -626,7 +626,11 @@ static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
 }
 
 // One observation interval: resample (device-gated) -> record -> predict+correct -> reduce.
-static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
+// timing: 0, BIHIP_TIMING_ALL (events around every phase) or
+// BIHIP_TIMING_TRANSITION (around the transition and proposal only).
+static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
+  const bool timed = timing == BIHIP_TIMING_ALL;       // every phase
+  const bool timed_trans = timing != 0;                // transition and proposal
   const int P = pf->cfg.P;
   int rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -658,7 +662,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
 
   // ---- predict + correct (fused, ancestor gather on read)
-  if (timed) {
+  if (timed_trans) {
     if ((rc = pf_event_pair(pf, 0, &e0, &e1))) return rc;
     BIHIP_HIP(hipEventRecord(e0, s));
   }
@@ -699,12 +703,12 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
     G.lws = pf->lws;
     G.partials = pf->partials;
     hipEvent_t g0 = nullptr, g1 = nullptr;
-    if (timed) {
+    if (timed_trans) {
       if ((rc = pf_event_pair(pf, 3, &g0, &g1))) return rc;
       BIHIP_HIP(hipEventRecord(g0, s));
     }
     if ((rc = launch_guided(G, s))) return rc;
-    if (timed) BIHIP_HIP(hipEventRecord(g1, s));
+    if (timed_trans) BIHIP_HIP(hipEventRecord(g1, s));
   } else if (pf->surr) {
     SurrogateLaunch S = pf->sl;
     S.Xin = T.Xin;
@@ -726,7 +730,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, bool timed) {
   } else if ((rc = launch_transition(T, s))) {
     return rc;
   }
-  if (timed) BIHIP_HIP(hipEventRecord(e1, s));
+  if (timed_trans) BIHIP_HIP(hipEventRecord(e1, s));
   if (pf->hist_X != nullptr &&
       (rc = launch_record_state(T.Xout, pf->ld, pf->model.d, P, pf->elem, pf->dev, pf->hist_X, 0,
                                 s)))
@@ -754,7 +758,7 @@ static int pf_capture(bihip_pf* pf, int parity) {
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   int rc = BIHIP_OK;
   if (e == hipSuccess) {
-    rc = pf_enqueue_step(pf, cs, false);
+    rc = pf_enqueue_step(pf, cs, 0);
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (rc == BIHIP_OK && e2 != hipSuccess)
       rc = set_error(BIHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -1151,9 +1155,11 @@ int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host) {
   return bihip_pf_paths(pf, &pick, 1, path_host);
 }
 
-int bihip_pf_set_timing(bihip_pf* pf, int enable) {
+int bihip_pf_set_timing(bihip_pf* pf, int mode) {
   if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
-  pf->timing = enable != 0;
+  if (mode < 0 || mode > BIHIP_TIMING_TRANSITION)
+    return set_error(BIHIP_EINVAL, "timing mode must be 0, BIHIP_TIMING_ALL or BIHIP_TIMING_TRANSITION");
+  pf->timing = mode;
   for (auto& u : pf->ev_used) u = 0;
   return BIHIP_OK;
 }
