@@ -156,8 +156,8 @@ static int batch_tune(bihip_batch* b) {
   };
   const int cand[3] = {256, 512, 1024};
   hipEvent_t e0, e1;
-  BIHIP_HIP(hipEventCreate(&e0));
-  BIHIP_HIP(hipEventCreate(&e1));
+  BIHIP_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+  BIHIP_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
   rc = save();
   double best = 1e300;
   for (int c = 0; c < 3 && rc == BIHIP_OK; ++c) {
@@ -383,7 +383,7 @@ int bihip_batch_step(bihip_batch* b, int n) {
     if (b->timing) {
       while (b->ev.size() < b->ev_used + 2) {
         hipEvent_t e;
-        BIHIP_HIP(hipEventCreate(&e));
+        BIHIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         b->ev.push_back(e);
       }
       BIHIP_HIP(hipEventRecord(b->ev[b->ev_used], s));

@@ -611,12 +611,15 @@ struct bihip_pf {
   GuidedLaunch gl{};
 };
 
+// Timing events only: created without the system-scope fence, so recording one
+// does not write back and invalidate the caches between the kernels it brackets
+// (the stream is synchronized before the times are read).
 static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
   auto& v = pf->ev[kind];
   size_t& u = pf->ev_used[kind];
   while (v.size() < u + 2) {
     hipEvent_t e;
-    BIHIP_HIP(hipEventCreate(&e));
+    BIHIP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     v.push_back(e);
   }
   *a = v[u];
