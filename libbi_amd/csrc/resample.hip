@@ -175,22 +175,41 @@ constexpr int xsize() {
   return SCAN_TILE + SCAN_TILE / (128 / (int)sizeof(V));
 }
 
-// Sum of the tile sums before `tile` and of all of them.
+// Sum of the tile sums before `tile` and of all of them, in two parts so that
+// a kernel can issue the loads before its gate check and its other loads:
+// tile_prefix_load reads this thread's share (at most SMALL_TILES / SCAN_BLOCK
+// sums), tile_prefix_combine adds the shares across the block.
 template <class V>
-__device__ __forceinline__ void tile_prefix(const V* sums, const V* total, int nt, int tile,
-                                            V* lds, V& pre, V& tot) {
+struct TilePrefix {
+  V pre, tot;
+};
+
+template <class V>
+__device__ __forceinline__ TilePrefix<V> tile_prefix_load(const V* sums, const V* total, int nt,
+                                                          int tile) {
+  TilePrefix<V> r{V(0), V(0)};
   if (nt <= SMALL_TILES) {
-    V p(0), t(0);
-    for (int j = threadIdx.x; j < nt; j += SCAN_BLOCK) {
-      const V v = sums[j];
-      t = t + v;
-      if (j < tile) p = p + v;
+#pragma unroll
+    for (int q = 0; q < SMALL_TILES / SCAN_BLOCK; ++q) {
+      const int j = q * SCAN_BLOCK + threadIdx.x;
+      if (j < nt) {
+        const V v = sums[j];
+        r.tot = r.tot + v;
+        if (j < tile) r.pre = r.pre + v;
+      }
     }
-    pre = block_sum_v(p, lds);
-    tot = block_sum_v(t, lds);
   } else {
-    pre = sums[tile];
-    tot = *total;
+    r.pre = sums[tile];
+    r.tot = *total;
+  }
+  return r;
+}
+
+template <class V>
+__device__ __forceinline__ void tile_prefix_combine(TilePrefix<V>& r, int nt, V* lds) {
+  if (nt <= SMALL_TILES) {
+    r.pre = block_sum_v(r.pre, lds);
+    r.tot = block_sum_v(r.tot, lds);
   }
 }
 
@@ -198,15 +217,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_weight_sums(const double* lws, c
                                                             int P, u128* wsum,
                                                             const PFDevice* gate) {
   __shared__ u128 lds[SCAN_BLOCK / WAVE];
-  if (!gate_open(gate)) return;
-  const double m = *mx;
+  // the loads are issued before the gate check (all in bounds) to overlap
+  // their latency with the gate's
   const long t0 = (long)blockIdx.x * SCAN_TILE;
-  u128 s(0);
+  double lw[SCAN_ITEMS];
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
     const long i = t0 + (long)j * SCAN_BLOCK + threadIdx.x;
-    if (i < P) s = s + fix_weight(nanexp(lws[i] - m));
+    lw[j] = i < P ? lws[i] : __builtin_nan("");  // NaN weighs zero
   }
+  const double m = *mx;
+  if (!gate_open(gate)) return;
+  u128 s(0);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) s = s + fix_weight(nanexp(lw[j] - m));
   s = block_sum_v(s, lds);
   if (threadIdx.x == 0) wsum[blockIdx.x] = s;
 }
@@ -261,21 +285,26 @@ template <int MODE>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_offspring_tiles(OffspringTileArgs a) {
   __shared__ u128 lds[SCAN_BLOCK / WAVE];
   __shared__ uint64_t lds64[SCAN_BLOCK / WAVE];
-  if (!gate_open(a.gate)) return;
   const int tile = blockIdx.x;
-  u128 pre, tot;
-  tile_prefix(a.wsum, a.wtotal, a.nt, tile, lds, pre, tot);
-  const double W = fix_to_double(tot);
+  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+  // every load is issued before the gate check (all in bounds)
+  double lw[SCAN_ITEMS];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) lw[i] = base + i < a.P ? a.lws[base + i] : 0.0;
+  TilePrefix<u128> tp = tile_prefix_load(a.wsum, a.wtotal, a.nt, tile);
+  const double m = *a.mx;
+  if (!gate_open(a.gate)) return;
+  tile_prefix_combine(tp, a.nt, lds);
+  const u128 pre = tp.pre;
+  const double W = fix_to_double(tp.tot);
   if (MODE != BIHIP_MULTINOMIAL && !(W > 0.0)) {
     if (a.status != nullptr && tile == 0 && threadIdx.x == 0) a.status->status = BIHIP_DEGENERATE;
     return;
   }
-  const double m = *a.mx;
-  const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
   u128 v[SCAN_ITEMS];
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i)
-    v[i] = base + i < a.P ? fix_weight(nanexp(a.lws[base + i] - m)) : u128(0);
+    v[i] = base + i < a.P ? fix_weight(nanexp(lw[i] - m)) : u128(0);
 #pragma unroll
   for (int i = 1; i < SCAN_ITEMS; ++i) v[i] = v[i - 1] + v[i];
   u128 agg;
@@ -353,17 +382,23 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
   __shared__ int xi[xsize<int>()];       // offspring input, then E
   __shared__ uint32_t xf[xsize<int>()];  // free-slot rank, or OCCUPIED
   __shared__ int s_before;
-  if (!gate_open(a.gate)) return;
   const int tile = blockIdx.x;
   const long t0 = (long)tile * SCAN_TILE;
-  uint64_t pre, tot;
-  tile_prefix(a.efsum, a.eftotal, a.nt, tile, lds, pre, tot);
+  // every load is issued before the gate check (all in bounds)
+  int c[SCAN_ITEMS];
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
     const int e = j * SCAN_BLOCK + threadIdx.x;
-    xi[xpos<int>(e)] = t0 + e < a.P ? a.in[t0 + e] : 0;
+    c[j] = t0 + e < a.P ? a.in[t0 + e] : 0;
   }
-  if (threadIdx.x == 0) s_before = (a.cumulative && t0 > 0) ? a.in[t0 - 1] : 0;
+  const int before = (a.cumulative && t0 > 0 && threadIdx.x == 0) ? a.in[t0 - 1] : 0;
+  TilePrefix<uint64_t> tp = tile_prefix_load(a.efsum, a.eftotal, a.nt, tile);
+  if (!gate_open(a.gate)) return;
+  tile_prefix_combine(tp, a.nt, lds);
+  const uint64_t pre = tp.pre, tot = tp.tot;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) xi[xpos<int>(j * SCAN_BLOCK + threadIdx.x)] = c[j];
+  if (threadIdx.x == 0) s_before = before;
   __syncthreads();
   const int b0 = threadIdx.x * SCAN_ITEMS;
   const long base = t0 + b0;
@@ -415,15 +450,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
 __global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, const uint32_t* slots,
                                                          const uint32_t* free_slots, int P,
                                                          int* as, const PFDevice* gate) {
-  if (!gate_open(gate)) return;
-  const uint32_t Z = *free_slots;
   const uint32_t r = blockIdx.x * RS_BLOCK + threadIdx.x;
+  // slots[r] is loaded with the gate and the count (r < P: in bounds; used
+  // only for r < Z)
+  const uint32_t kr = r < (uint32_t)P ? slots[r] : 0u;
+  const uint32_t Z = *free_slots;
+  if (!gate_open(gate)) return;
   const uint32_t rA = r & ~(uint32_t)(WAVE - 1);  // the wave's first rank
   if (rA >= Z) return;                             // uniform across the wave
   const int lane = threadIdx.x & (WAVE - 1);
   const bool valid = r < Z;
   const uint32_t rB = min(rA + (uint32_t)(WAVE - 1), Z - 1);
-  const uint32_t k = valid ? slots[r] : 0u;
+  const uint32_t k = valid ? kr : 0u;
   int lo = 0, hi = P;  // E[lo] <= rA, and the owners of ranks rA..rB lie in [lo, hi)
   while (hi - lo > WAVE) {
     const int step = (hi - lo + WAVE - 1) / WAVE;
