@@ -56,6 +56,8 @@ class DeviceBootstrapPF {
     check(bihip_pf_ancestors(pf, k, a.data()));
     return a;
   }
+  /** The recorded history (record = 2) in LibBi's ParticleFilter NetCDF schema. */
+  void writeNetCDF(const std::string& file) { check(bihip_pf_write_netcdf(pf, file.c_str())); }
   bihip_pf* handle() { return pf; }
 
  private:

@@ -466,6 +466,42 @@ class ParticleFilterMemoryBuffer {
   double logLikelihood = 0.0;
 };
 
+/**
+ * share/src/bi/netcdf/ParticleFilterNetCDFBuffer.hpp behind ParticleFilterBuffer::write /
+ * writeT (share/src/bi/buffer/ParticleFilterBuffer.hpp:69-82): every output time's state,
+ * ancestors and log-weights, then the log-likelihood, in LibBi's ParticleFilter NetCDF
+ * schema (bihip_pfout_*).  T is the number of output times.
+ */
+class ParticleFilterNetCDFBuffer {
+ public:
+  template <class B>
+  ParticleFilterNetCDFBuffer(const B& m, const int P, const int T, const std::string& file)
+      : o(nullptr) {
+    amd::check(bihip_pfout_create(file.c_str(), &m.description(), T, P, &o));
+  }
+  ~ParticleFilterNetCDFBuffer() { bihip_pfout_close(o); }
+  ParticleFilterNetCDFBuffer(const ParticleFilterNetCDFBuffer&) = delete;
+  ParticleFilterNetCDFBuffer& operator=(const ParticleFilterNetCDFBuffer&) = delete;
+
+  template <class S1>
+  void write(const int k, const double t, const S1& s) {
+    amd::check(bihip_pfout_write_time(o, k, t));
+    const auto X = s.to_host();
+    const std::vector<int> as = s.ancestors().to_host();
+    amd::check(bihip_pfout_write_state(o, k, X.data(), s.lead(), as.data()));
+    const std::vector<double> lws = s.logWeights().to_host();
+    amd::check(bihip_pfout_write_log_weights(o, k, lws.data()));
+  }
+  template <class S1>
+  void writeT(const S1& s) {
+    amd::check(bihip_pfout_write_log_likelihood(o, s.logLikelihood));
+    amd::check(bihip_pfout_write_clock(o, (double)s.clock));
+  }
+
+ private:
+  bihip_pfout* o;
+};
+
 // ================================================================ filter
 /**
  * Bootstrap particle filter (share/src/bi/filter/BootstrapPF.hpp:26-196),

@@ -40,6 +40,7 @@ extern "C" {
 #define BIHIP_EINVAL 2     /* invalid argument */
 #define BIHIP_EHIP 3       /* HIP runtime error */
 #define BIHIP_ESTATE 4     /* object used out of order */
+#define BIHIP_EIO 5        /* file input/output error */
 
 /* ---- model, integrator, resampler and precision identifiers ---- */
 #define BIHIP_MODEL_PZ 0  /* predator-prey, docs/src/users.tex:36-85 */
@@ -291,6 +292,42 @@ int bihip_pf_position(bihip_pf* pf);
 #define BIHIP_TIMING_TRANSITION 2
 int bihip_pf_set_timing(bihip_pf* pf, int mode);
 int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_host);
+
+/* ================================================================ NetCDF output */
+/*
+ * ParticleFilterNetCDFBuffer (share/src/bi/netcdf/ParticleFilterNetCDFBuffer.cpp:19-38,
+ * with SimulatorNetCDFBuffer::create, share/src/bi/netcdf/SimulatorNetCDFBuffer.cpp:23-83):
+ * LibBi's DEFAULT particle-filter schema -- dimensions nr, the model's (L96: n), np;
+ * variables time(nr), the state variables (PZ: P, Z; L96: x(nr, n, np); LG: x),
+ * ancestor(nr, np), logweight(nr, np), loglikelihood, clock; attributes libbi_schema =
+ * "ParticleFilter", libbi_schema_version = 1 -- written without libnetcdf as a NetCDF
+ * classic file with 64-bit offsets (CDF-2), which nc_open reads like the reference's
+ * NetCDF-4 files.  clock is NC_DOUBLE microseconds (CDF-2 has no 64-bit integer); only
+ * the state variables, written last, may exceed 4 GiB.  All buffers are host memory.
+ */
+typedef struct bihip_pfout bihip_pfout;
+int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
+                       bihip_pfout** out);
+/* SimulatorNetCDFBuffer::writeTime (SimulatorNetCDFBuffer.cpp:264) */
+int bihip_pfout_write_time(bihip_pfout* o, int k, double t);
+/* ParticleFilterNetCDFBuffer::writeState(k, X, as) (ParticleFilterNetCDFBuffer.hpp:43-44):
+ * X is d x ld column-major in the model's precision. */
+int bihip_pfout_write_state(bihip_pfout* o, int k, const void* X_host, int ld,
+                            const int* as_host);
+/* ParticleFilterNetCDFBuffer::writeLogWeights (ParticleFilterNetCDFBuffer.hpp:54-55) */
+int bihip_pfout_write_log_weights(bihip_pfout* o, int k, const double* lws_host);
+/* ParticleFilterNetCDFBuffer::writeLogLikelihood (ParticleFilterNetCDFBuffer.cpp:87-89) */
+int bihip_pfout_write_log_likelihood(bihip_pfout* o, double ll);
+/* SimulatorNetCDFBuffer::writeClock (SimulatorNetCDFBuffer.cpp:277) */
+int bihip_pfout_write_clock(bihip_pfout* o, double clock_us);
+int bihip_pfout_close(bihip_pfout* o);
+/*
+ * The whole recorded history of a filter created with record = 2, as Filter::filter's
+ * outputs write it (share/src/bi/filter/Filter.hpp:56-71 with ParticleFilterBuffer::write):
+ * output 0 at t0 (the initial samples, identity ancestors, zero log-weights), output k at
+ * t0 + k dt after step k, for the steps taken so far; then the log-likelihood.
+ */
+int bihip_pf_write_netcdf(bihip_pf* pf, const char* path);
 
 /* ================================================================ batched inner filters */
 /*

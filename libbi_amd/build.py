@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libbihip.so")
 SOURCES = ["transition.hip", "reduce.hip", "scan.hip", "resample.hip", "batch.hip", "capi.cpp",
            "batch_capi.cpp", "smc2.cpp",
            "mlp.hip", "mlp_capi.cpp", "surrogate.hip",
-           "guided.hip"]
+           "guided.hip", "netcdf.cpp"]
 ARCH = os.environ.get("BIHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

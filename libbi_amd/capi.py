@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libbihip.so")
 
-OK, DEGENERATE, EINVAL, EHIP, ESTATE = 0, 1, 2, 3, 4
+OK, DEGENERATE, EINVAL, EHIP, ESTATE, EIO = 0, 1, 2, 3, 4, 5
 MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
 RK4, DOPRI5 = 0, 1
 SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
@@ -58,6 +58,9 @@ EXPORTS = [
     "bihip_guided_propose", "bihip_pf_set_guided",
     "bihip_pmmh_create", "bihip_pmmh_destroy", "bihip_pmmh_set_observations", "bihip_pmmh_run",
     "bihip_pmmh_outputs", "bihip_pf_paths", "bihip_pf_sample_path",
+    "bihip_pfout_create", "bihip_pfout_write_time", "bihip_pfout_write_state",
+    "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
+    "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
 ]
 
 
@@ -237,6 +240,14 @@ def _declare(L):
         "bihip_pmmh_outputs": [P, pd, pd, pi],
         "bihip_pf_paths": [P, pi, i, pd],
         "bihip_pf_sample_path": [P, pd, pi],
+        "bihip_pfout_create": [c.c_char_p, pm, i, i, c.POINTER(P)],
+        "bihip_pfout_write_time": [P, i, d],
+        "bihip_pfout_write_state": [P, i, P, i, pi],
+        "bihip_pfout_write_log_weights": [P, i, pd],
+        "bihip_pfout_write_log_likelihood": [P, d],
+        "bihip_pfout_write_clock": [P, d],
+        "bihip_pfout_close": [P],
+        "bihip_pf_write_netcdf": [P, c.c_char_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -457,6 +468,10 @@ class ParticleFilter:
         the bootstrap step."""
         check(lib().bihip_pf_set_guided(self.h, mlp.h if mlp is not None else None,
                                         ctypes.byref(norm) if norm is not None else None))
+
+    def write_netcdf(self, path):
+        """The recorded history (record="paths") in LibBi's ParticleFilter NetCDF schema."""
+        check(lib().bihip_pf_write_netcdf(self.h, os.fsencode(path)))
 
     def set_timing(self, on, transition_only=False):
         """HIP events around the step's phases (kernel_time), or around the
@@ -786,3 +801,43 @@ class MLP:
                                          noise_sd, obs_sd, inc.data_ptr()))
         self.ctx.sync()
         return inc
+
+
+class PFOutput:
+    """bihip_pfout: LibBi's ParticleFilter NetCDF schema written from host arrays
+    (ParticleFilterNetCDFBuffer's writeTime / writeState / writeLogWeights /
+    writeLogLikelihood / writeClock)."""
+
+    def __init__(self, path, model, nr, np_):
+        self.h = ctypes.c_void_p()
+        self.model, self.nr, self.np = model, nr, np_
+        check(lib().bihip_pfout_create(os.fsencode(path), ctypes.byref(model), nr, np_,
+                                       ctypes.byref(self.h)))
+
+    def write_time(self, k, t):
+        check(lib().bihip_pfout_write_time(self.h, k, float(t)))
+
+    def write_state(self, k, X, ancestors):
+        """X: d x ld (ld >= np), float32 or float64 by the model's precision."""
+        dt = np.float32 if self.model.precision == F32 else np.float64
+        X = np.ascontiguousarray(X, dt)
+        a = np.ascontiguousarray(ancestors, np.int32)
+        check(lib().bihip_pfout_write_state(self.h, k, X.ctypes.data_as(ctypes.c_void_p),
+                                            X.shape[-1],
+                                            a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def write_log_weights(self, k, lws):
+        lw = np.ascontiguousarray(lws, np.float64)
+        check(lib().bihip_pfout_write_log_weights(
+            self.h, k, lw.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def write_log_likelihood(self, ll):
+        check(lib().bihip_pfout_write_log_likelihood(self.h, float(ll)))
+
+    def write_clock(self, us):
+        check(lib().bihip_pfout_write_clock(self.h, float(us)))
+
+    def close(self):
+        if self.h:
+            check(lib().bihip_pfout_close(self.h))
+            self.h = ctypes.c_void_p()

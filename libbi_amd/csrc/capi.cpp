@@ -593,6 +593,7 @@ struct bihip_pf {
   int* hist_res = nullptr;
   int* hist_anc = nullptr;
   void* hist_X = nullptr;  // record = 2: the state at every time, (T + 1) x d x ld
+  double* hist_lw = nullptr;  // record = 2: the log-weights at every time, (T + 1) x P
   ScanWorkspace ws{};
   int k = 0;
   bool ready = false, have_obs = false;
@@ -735,8 +736,9 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   }
   if (timed_trans) BIHIP_HIP(hipEventRecord(e1, s));
   if (pf->hist_X != nullptr &&
-      (rc = launch_record_state(T.Xout, pf->ld, pf->model.d, P, pf->elem, pf->dev, pf->hist_X, 0,
-                                s)))
+      ((rc = launch_record_state(T.Xout, pf->ld, pf->model.d, P, pf->elem, pf->dev, pf->hist_X, 0,
+                                 s)) ||
+       (rc = launch_record_state(pf->lws, P, 1, P, 8, pf->dev, pf->hist_lw, 0, s))))
     return rc;
 
   // ---- BootstrapPF::correct bookkeeping
@@ -832,6 +834,7 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
     uint8_t* hx = nullptr;
     PF_ALLOC(hx, (size_t)(cfg->T + 1) * m->d * pf->ld * pf->elem);
     pf->hist_X = hx;
+    PF_ALLOC(pf->hist_lw, (size_t)(cfg->T + 1) * P);
   }
 #undef PF_ALLOC
   if ((rc = ws_alloc(&pf->ws, scan_tiles(P))) ||
@@ -861,6 +864,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->EF);
   dfree(pf->partials);
   dfree(pf->hist_X);
+  dfree(pf->hist_lw);
   dfree(pf->out);
   dfree(pf->dev);
   dfree(pf->y);
@@ -897,8 +901,10 @@ int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
     return rc;
   BIHIP_HIP(hipMemsetAsync(pf->lws, 0, sizeof(double) * pf->cfg.P, s));
   if (pf->hist_X != nullptr &&
-      (rc = launch_record_state(pf->X[0], pf->ld, pf->model.d, pf->cfg.P, pf->elem, nullptr,
-                                pf->hist_X, 0, s)))
+      ((rc = launch_record_state(pf->X[0], pf->ld, pf->model.d, pf->cfg.P, pf->elem, nullptr,
+                                 pf->hist_X, 0, s)) ||
+       (rc = launch_record_state(pf->lws, pf->cfg.P, 1, pf->cfg.P, 8, nullptr, pf->hist_lw, 0,
+                                 s))))
     return rc;
   PFDevice d{};
   d.status = BIHIP_OK;
@@ -1156,6 +1162,52 @@ int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host) {
   const int pick = p < P ? p : P - 1;
   if (index_host) *index_host = pick;
   return bihip_pf_paths(pf, &pick, 1, path_host);
+}
+
+int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {
+  if (pf == nullptr || path == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (pf->hist_X == nullptr || pf->hist_lw == nullptr || pf->hist_anc == nullptr)
+    return set_error(BIHIP_ESTATE, "the NetCDF output needs a filter created with record = 2");
+  if (!pf->ready) return set_error(BIHIP_ESTATE, "bihip_pf_init has not been called");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  hipStream_t s = pf->ctx->stream;
+  const int P = pf->cfg.P, nr = pf->k + 1;
+  const size_t xbytes = (size_t)pf->model.d * pf->ld * pf->elem;
+  std::vector<uint8_t> X(xbytes);
+  std::vector<double> lw(P);
+  std::vector<int> as(P);
+  bihip_pfout* o = nullptr;
+  if ((rc = bihip_pfout_create(path, &pf->model, nr, P, &o))) return rc;
+  for (int k = 0; k < nr && rc == BIHIP_OK; ++k) {
+    const uint8_t* hx = static_cast<const uint8_t*>(pf->hist_X) + (size_t)k * xbytes;
+    hipError_t e = hipMemcpyAsync(X.data(), hx, xbytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(lw.data(), pf->hist_lw + (size_t)k * P, sizeof(double) * P,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && k > 0)
+      e = hipMemcpyAsync(as.data(), pf->hist_anc + (size_t)(k - 1) * P, sizeof(int) * P,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      rc = set_error(BIHIP_EHIP, "copy of output %d: %s", k, hipGetErrorString(e));
+      break;
+    }
+    if (k == 0)  // the initial samples have themselves as ancestors
+      for (int p = 0; p < P; ++p) as[p] = p;
+    if ((rc = bihip_pfout_write_time(o, k, pf->cfg.t0 + k * pf->cfg.dt)) ||
+        (rc = bihip_pfout_write_state(o, k, X.data(), pf->ld, as.data())) ||
+        (rc = bihip_pfout_write_log_weights(o, k, lw.data())))
+      break;
+  }
+  if (rc == BIHIP_OK) {
+    PFDevice d{};
+    BIHIP_HIP(hipMemcpy(&d, pf->dev, sizeof(d), hipMemcpyDeviceToHost));
+    rc = bihip_pfout_write_log_likelihood(o, d.logLikelihood);
+  }
+  if (rc == BIHIP_OK) rc = bihip_pfout_write_clock(o, 0.0);
+  const int rc2 = bihip_pfout_close(o);
+  return rc != BIHIP_OK ? rc : rc2;
 }
 
 int bihip_pf_set_timing(bihip_pf* pf, int mode) {

@@ -19,7 +19,7 @@ def test_client_built_and_prints_usage():
     assert r.returncode == 2 and "usage" in r.stderr
 
 
-def _run(tmp_path, model, res, P, dt, seed, ess_rel, y, mean0=None, fused=False):
+def _run(tmp_path, model, res, P, dt, seed, ess_rel, y, mean0=None, fused=False, extra=()):
     obs = tmp_path / "obs.bin"
     out = tmp_path / "out.bin"
     y = np.asarray(y, np.float64)
@@ -31,6 +31,7 @@ def _run(tmp_path, model, res, P, dt, seed, ess_rel, y, mean0=None, fused=False)
         args.append(str(m0))
     if fused:
         args.append("fused")
+    args.extend(extra)
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     T = y.shape[0]
@@ -66,3 +67,34 @@ def test_shim_l96_filter_matches_oracle(tmp_path, orc, torch_gpu):
     ref = orc.pf_run(om, y, P, 0.05, 9, ess_rel=0.5, mean0=mean0, want_ancestors=True)
     assert logZ == pytest.approx(ref["logZ"], rel=1e-9)
     np.testing.assert_array_equal(anc[1:], ref["ancestors"])
+
+
+@pytest.mark.gpu
+def test_shim_netcdf_outputs_agree(tmp_path, orc, torch_gpu):
+    """ParticleFilterNetCDFBuffer behind the shim filter and the device filter's
+    recorded history write the same ParticleFilter NetCDF file: times, states,
+    ancestors (as the client's own output), log-weights and logZ."""
+    from scipy.io import netcdf_file
+    m = orc.make_model("pz")
+    x0 = orc.init_samples(m, 100, 1)[:, 0]
+    _, y = orc.simulate(m, 100, 12, 1.0, x0)
+    P, T = 512, 12
+    files = {}
+    for fused in (False, True):
+        nc = tmp_path / f"out_{int(fused)}.nc"
+        sub = tmp_path / f"run_{int(fused)}"
+        sub.mkdir()
+        logZ, li, anc = _run(sub, "pz", 0, P, 1.0, 7, 0.6, y, fused=fused, extra=[f"nc={nc}"])
+        f = netcdf_file(str(nc), "r", mmap=False)
+        assert f.dimensions == {"nr": T + 1, "np": P}
+        v = {k: np.array(x.data) for k, x in f.variables.items()}
+        f.close()
+        np.testing.assert_array_equal(v["ancestor"], anc)
+        assert float(v["loglikelihood"]) == logZ
+        np.testing.assert_array_equal(v["time"], np.arange(T + 1, dtype=float))
+        files[fused] = v
+    a, b = files[False], files[True]
+    np.testing.assert_array_equal(a["ancestor"], b["ancestor"])
+    for name in ("P", "Z", "logweight"):
+        np.testing.assert_allclose(a[name], b[name], rtol=1e-12, atol=1e-12)
+    assert float(a["loglikelihood"]) == pytest.approx(float(b["loglikelihood"]), rel=1e-12)

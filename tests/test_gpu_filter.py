@@ -217,3 +217,61 @@ def test_paths_match_oracle_lineages(ctx, orc):
     assert pick == min(int(np.searchsorted(Ws, Ws[-1] * u, side="right")), P - 1)
     np.testing.assert_array_equal(path, pf.paths([pick])[0])
     pf.close()
+
+
+def test_netcdf_output_of_the_recorded_filter(ctx, orc, tmp_path):
+    """bihip_pf_write_netcdf: the recorded history in LibBi's ParticleFilter
+    schema, read back with scipy's NetCDF reader: times t0 + k dt, the oracle's
+    states and ancestors at every output (output 0: the initial samples with
+    identity ancestors), log-weights whose log-mean-exp after step k is the
+    running log-likelihood (the sum of the log-increments), and logZ."""
+    from scipy.io import netcdf_file
+    T, P, seed = 8, 256, 11
+    y = _pz_data(orc, T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    pf = capi.ParticleFilter(ctx, gm, P, T, 1.0, seed, ess_rel=0.7, record="paths")
+    pf.set_observations(y)
+    pf.init()
+    pf.step(T)
+    path = tmp_path / "pf.nc"
+    pf.write_netcdf(path)
+    out = pf.outputs()
+    logZ = pf.term()
+    _, _, lws = pf.state()
+    pf.close()
+    f = netcdf_file(str(path), "r", mmap=False)
+    assert f.dimensions == {"nr": T + 1, "np": P}
+    assert f._attributes["libbi_schema"] == b"ParticleFilter"
+    v = {k: np.array(x.data) for k, x in f.variables.items()}
+    f.close()
+    np.testing.assert_array_equal(v["time"], np.arange(T + 1, dtype=float))
+    ref = orc.pf_run(om, y, P, 1.0, seed, ess_rel=0.7, want_ancestors=True)
+    np.testing.assert_array_equal(v["ancestor"][0], np.arange(P))
+    np.testing.assert_array_equal(v["ancestor"][1:], ref["ancestors"])
+    states = [orc.init_samples(om, seed, P)]
+    for k in range(1, T + 1):
+        states.append(orc.pf_run(om, y[:k], P, 1.0, seed, ess_rel=0.7, want_state=True)["X"])
+    for k in range(T + 1):
+        np.testing.assert_allclose(v["P"][k], states[k][0], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(v["Z"][k], states[k][1], rtol=1e-12, atol=1e-12)
+    lw = v["logweight"]
+    np.testing.assert_array_equal(lw[0], np.zeros(P))
+    np.testing.assert_array_equal(lw[T], lws.cpu().numpy())
+    running = np.cumsum(out["logIncrements"])
+    for k in range(1, T + 1):
+        m = lw[k].max()
+        lme = m + np.log(np.exp(lw[k] - m).sum()) - np.log(P)
+        assert lme == pytest.approx(running[k - 1], rel=1e-12, abs=1e-9)
+    assert float(v["loglikelihood"]) == logZ
+
+
+def test_netcdf_output_needs_recorded_paths(ctx, orc, tmp_path):
+    y = _pz_data(orc, 3)
+    pf = capi.ParticleFilter(ctx, capi.make_model("pz", precision=capi.F64), 64, 3, 1.0, 1)
+    pf.set_observations(y)
+    pf.init()
+    pf.step(3)
+    with pytest.raises(capi.BihipError):
+        pf.write_netcdf(tmp_path / "x.nc")
+    pf.close()
