@@ -280,6 +280,17 @@ int bihip_pf_state(bihip_pf* pf, void** X_dev, int* ld, double** lws_dev);
 /* Steps completed so far. */
 int bihip_pf_position(bihip_pf* pf);
 /*
+ * Schedule (share/src/bi/state/Schedule.hpp:214-337, restricted to observation and
+ * output times): step k runs from times[k] to times[k+1] (T + 1 increasing times, in
+ * place of t0 + k dt), and observed[k] == 0 (nullable: every step observed) makes it a
+ * prediction without correction, which BootstrapPF::correct
+ * (share/src/bi/filter/BootstrapPF.hpp:171-181) and Resampler::resample
+ * (share/src/bi/resampler/Resampler.hpp:165-182) skip: no weight update, no
+ * log-likelihood increment, no resampling at the next step.  Observation row k is read
+ * only when step k is observed.  Not combined with the learned flow or proposal.
+ */
+int bihip_pf_set_schedule(bihip_pf* pf, const double* times_host, const int* observed_host);
+/*
  * Kernel timing with HIP events on the filter's stream: enable, then query
  * the summed milliseconds and launch count of one kernel class
  * (0 = transition, 1 = resample chain, 2 = weight reduction, 3 = the guided

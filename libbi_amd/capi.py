@@ -61,6 +61,7 @@ EXPORTS = [
     "bihip_pfout_create", "bihip_pfout_write_time", "bihip_pfout_write_state",
     "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
+    "bihip_pf_set_schedule",
 ]
 
 
@@ -248,6 +249,7 @@ def _declare(L):
         "bihip_pfout_write_clock": [P, d],
         "bihip_pfout_close": [P],
         "bihip_pf_write_netcdf": [P, c.c_char_p],
+        "bihip_pf_set_schedule": [P, pd, pi],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -468,6 +470,21 @@ class ParticleFilter:
         the bootstrap step."""
         check(lib().bihip_pf_set_guided(self.h, mlp.h if mlp is not None else None,
                                         ctypes.byref(norm) if norm is not None else None))
+
+    def set_schedule(self, times, observed=None):
+        """Step k runs from times[k] to times[k+1]; observed[k] == 0 skips its
+        correction (and the resampling after it)."""
+        t = np.ascontiguousarray(times, np.float64)
+        if t.shape != (self.T + 1,):
+            raise ValueError(f"need {self.T + 1} times")
+        o = None
+        if observed is not None:
+            o = np.ascontiguousarray(observed, np.int32)
+            if o.shape != (self.T,):
+                raise ValueError(f"need {self.T} observed flags")
+        check(lib().bihip_pf_set_schedule(
+            self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if o is not None else None))
 
     def write_netcdf(self, path):
         """The recorded history (record="paths") in LibBi's ParticleFilter NetCDF schema."""

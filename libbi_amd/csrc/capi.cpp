@@ -610,6 +610,10 @@ struct bihip_pf {
   // learned guided proposal (bihip_pf_set_guided)
   bool guided = false;
   GuidedLaunch gl{};
+  // schedule (bihip_pf_set_schedule): T + 1 times and T observed flags
+  double* times = nullptr;
+  int* obs_mask = nullptr;
+  std::vector<double> times_host;
 };
 
 // Timing events only: created without the system-scope fence, so recording one
@@ -686,6 +690,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   T.dt = pf->cfg.dt;
   T.y_base = pf->y;
   T.nobs = pf->nobs;
+  T.times = pf->times;
+  T.mask = pf->obs_mask;
   if (pf->guided) {
     // exact flow of the gathered particles (no noise, no correction) into Xout,
     // then the proposal and the corrected weights in place
@@ -748,7 +754,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   }
   PFHistory H{pf->hist_li, pf->hist_ess, pf->hist_res};
   if ((rc = launch_pf_correct_finalize(pf->partials, pf->nparts, P, pf->cfg.essRel, pf->dev, H,
-                                       pf->rw, s)))
+                                       pf->rw, s, pf->obs_mask)))
     return rc;
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
   return BIHIP_OK;
@@ -865,6 +871,8 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->partials);
   dfree(pf->hist_X);
   dfree(pf->hist_lw);
+  dfree(pf->times);
+  dfree(pf->obs_mask);
   dfree(pf->out);
   dfree(pf->dev);
   dfree(pf->y);
@@ -1025,6 +1033,37 @@ int bihip_pf_state(bihip_pf* pf, void** X, int* ld, double** lws) {
 
 int bihip_pf_position(bihip_pf* pf) { return pf ? pf->k : -1; }
 
+int bihip_pf_set_schedule(bihip_pf* pf, const double* times_host, const int* observed_host) {
+  if (pf == nullptr || times_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (pf->surr || pf->guided)
+    return set_error(BIHIP_ESTATE, "detach the learned flow or proposal before setting a schedule");
+  const int T = pf->cfg.T;
+  for (int k = 0; k < T; ++k)
+    if (!(times_host[k + 1] > times_host[k]))
+      return set_error(BIHIP_EINVAL, "schedule times must increase (times[%d] = %g, times[%d] = %g)",
+                       k, times_host[k], k + 1, times_host[k + 1]);
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  if (pf->times == nullptr && ((rc = dalloc(&pf->times, (size_t)T + 1)) ||
+                               (rc = dalloc(&pf->obs_mask, (size_t)T))))
+    return rc;
+  std::vector<int> mask(T, 1);
+  if (observed_host != nullptr)
+    for (int k = 0; k < T; ++k) mask[k] = observed_host[k] != 0;
+  pf->times_host.assign(times_host, times_host + T + 1);
+  hipStream_t s = pf->ctx->stream;
+  BIHIP_HIP(hipMemcpyAsync(pf->times, times_host, sizeof(double) * (T + 1), hipMemcpyHostToDevice,
+                           s));
+  BIHIP_HIP(hipMemcpyAsync(pf->obs_mask, mask.data(), sizeof(int) * T, hipMemcpyHostToDevice, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  pf->cfg.t0 = times_host[0];
+  for (int i = 0; i < 2; ++i) {  // captured steps hold the previous arguments
+    if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
+    pf->graph[i] = nullptr;
+  }
+  return BIHIP_OK;
+}
+
 int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_norm* norm) {
   if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
   int rc = ctx_enter(pf->ctx);
@@ -1035,6 +1074,8 @@ int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_n
         pf->model.precision != BIHIP_F32)
       return set_error(BIHIP_EINVAL, "the surrogate flow serves FP32 Lorenz-96 d=40 filters");
     if (pf->guided) return set_error(BIHIP_ESTATE, "detach the guided proposal first");
+    if (pf->times != nullptr)
+      return set_error(BIHIP_ESTATE, "the surrogate flow serves the uniform schedule it was trained on");
     SurrogateLaunch S{};
     S.net = mlp_view(mlp);
     if (S.net.L != 4 || S.net.dims[0] != 41 || S.net.dims[4] != 40)
@@ -1073,6 +1114,8 @@ int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* n
         pf->model.precision != BIHIP_F32)
       return set_error(BIHIP_EINVAL, "the guided proposal serves FP32 Lorenz-96 d=200 filters");
     if (pf->surr) return set_error(BIHIP_ESTATE, "detach the surrogate flow first");
+    if (pf->times != nullptr)
+      return set_error(BIHIP_ESTATE, "the guided proposal serves the uniform schedule it was trained on");
     GuidedLaunch G{};
     G.net = mlp_view(mlp);
     if (G.net.L != 4 || G.net.dims[0] != 301 || G.net.dims[4] != 400)
@@ -1195,7 +1238,8 @@ int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {
     }
     if (k == 0)  // the initial samples have themselves as ancestors
       for (int p = 0; p < P; ++p) as[p] = p;
-    if ((rc = bihip_pfout_write_time(o, k, pf->cfg.t0 + k * pf->cfg.dt)) ||
+    const double tk = pf->times != nullptr ? pf->times_host[k] : pf->cfg.t0 + k * pf->cfg.dt;
+    if ((rc = bihip_pfout_write_time(o, k, tk)) ||
         (rc = bihip_pfout_write_state(o, k, X.data(), pf->ld, as.data())) ||
         (rc = bihip_pfout_write_log_weights(o, k, lw.data())))
       break;

@@ -114,6 +114,10 @@ struct TransLaunch {
   int nobs;
   int flow_only;            // 1: no noise (RK4Integrator / DOPRI5Integrator::update)
   const double* aux;        // flow_only: per-particle noise variable, e.g. PZ alpha
+  // with dev, a schedule (both nullable): times[k], times[k+1] in place of
+  // t0 + k dt, and mask[k] == 0 for a step without observation (no correction)
+  const double* times;
+  const int* mask;
 };
 int transition_blocks(const bihip_model* m, int P);
 int launch_transition(const TransLaunch& L, hipStream_t s);
@@ -148,7 +152,8 @@ int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWo
                     hipStream_t s);
 // Combine partials into the filter scalars and histories.
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
-                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s);
+                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
+                               const int* mask = nullptr);
 
 // ---- scan.hip
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,

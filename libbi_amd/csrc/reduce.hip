@@ -77,7 +77,7 @@ __device__ __forceinline__ void combine(int lo, int hi, Load load, double* lds, 
 __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, int nb, int P,
                                                        double essRel, PFDevice* dev,
                                                        PFHistory hist, double* out,
-                                                       ReduceWork w) {
+                                                       ReduceWork w, const int* mask) {
   __shared__ double lds[COMB_BLOCK / WAVE];
   __shared__ int last;
   __shared__ uint32_t s_epoch;
@@ -129,13 +129,21 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, i
   if (dev != nullptr) {
     const int k = dev->k;
     if (hist.resampled) hist.resampled[k] = dev->resample != 0 && dev->status == BIHIP_OK;
-    if (hist.logIncrements) hist.logIncrements[k] = lW - dev->logLikelihood;
-    if (hist.ess) hist.ess[k] = ess;
-    dev->logLikelihood = lW;
-    dev->ess = ess;
-    dev->mx = M;
-    dev->lW = lW;
-    dev->resample = ess < essRel * (double)P ? 1 : 0;
+    if (mask == nullptr || mask[k] != 0) {
+      if (hist.logIncrements) hist.logIncrements[k] = lW - dev->logLikelihood;
+      if (hist.ess) hist.ess[k] = ess;
+      dev->logLikelihood = lW;
+      dev->ess = ess;
+      dev->mx = M;
+      dev->lW = lW;
+      dev->resample = ess < essRel * (double)P ? 1 : 0;
+    } else {
+      // no observation: nothing to correct, and Resampler::resample acts only
+      // at observed elements (share/src/bi/resampler/Resampler.hpp:168)
+      if (hist.logIncrements) hist.logIncrements[k] = 0.0;
+      if (hist.ess) hist.ess[k] = dev->ess;
+      dev->resample = 0;
+    }
     dev->k = k + 1;
   }
 }
@@ -152,15 +160,16 @@ int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t 
 int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWork w,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
-                     0.0, (PFDevice*)nullptr, PFHistory{}, out, w);
+                     0.0, (PFDevice*)nullptr, PFHistory{}, out, w, (const int*)nullptr);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }
 
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
-                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s) {
+                               PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
+                               const int* mask) {
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
-                     essRel, dev, hist, (double*)nullptr, w);
+                     essRel, dev, hist, (double*)nullptr, w, mask);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

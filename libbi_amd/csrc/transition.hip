@@ -34,6 +34,8 @@ struct TransArgs {
   int nobs;
   int flow_only;        // 1: deterministic flow, no noise (integrator update)
   const double* aux;    // flow_only: per-particle noise variable (PZ alpha) or null
+  const double* times;  // with dev: schedule times (nullable: t0 + k dt)
+  const int* mask;      // with dev: observed flags (nullable: every step observed)
 };
 
 // ---------------------------------------------------------------- the kernel
@@ -58,10 +60,13 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
     lws_const = gather;
     lws_base = a.dev->logLikelihood;
     const int k = a.dev->k;
-    t1 = a.t0 + k * a.dt;
-    t2 = a.t0 + (k + 1) * a.dt;
+    t1 = a.times != nullptr ? a.times[k] : a.t0 + k * a.dt;
+    t2 = a.times != nullptr ? a.times[k + 1] : a.t0 + (k + 1) * a.dt;
     step = (uint32_t)(k + 1);
-    y = a.y_base != nullptr ? a.y_base + (size_t)k * a.nobs : nullptr;  // null: flow only
+    // BootstrapPF::correct applies only at observed elements
+    // (share/src/bi/filter/BootstrapPF.hpp:171-181); null y_base: flow only
+    const bool observed = a.mask == nullptr || a.mask[k] != 0;
+    y = a.y_base != nullptr && observed ? a.y_base + (size_t)k * a.nobs : nullptr;
   }
 
   double lw = __builtin_nan("");
@@ -102,6 +107,8 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
         lw = base + part;
         a.lws[p] = lw;
       }
+    } else if (lws_const && a.lws != nullptr && grp.g == 0) {
+      a.lws[p] = lws_base;  // resampled, then no observation: lws = logLikelihood
     }
   }
   // only group leaders (lanes g == 0) hold a log-weight
@@ -292,6 +299,8 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.dev = L.dev;
   r.a.t0 = L.t0;
   r.a.dt = L.dt;
+  r.a.times = L.times;
+  r.a.mask = L.mask;
   r.a.y_base = L.y_base;
   r.a.nobs = L.nobs;
   r.a.flow_only = L.flow_only;

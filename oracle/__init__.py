@@ -45,6 +45,13 @@ PINNED = {
     "orc_gather_rows": "in-place == out-of-place gather",
     "orc_pf_run": "Kalman filter exact logZ on the LG model (essRel 0, 0.5, 1)",
     "orc_pf_step": "equals orc_pf_run step by step (orc_pf_run is built on it)",
+    "orc_pf_run(schedule)": "uniform times with every step observed reproduce the plain run "
+                            "bit for bit; LG model over irregular times with unobserved steps "
+                            "against the scheduled Kalman log-likelihood; bookkeeping (zero "
+                            "increments, no resampling after an unobserved step)",
+    "orc_kalman_loglik_lg_sched": "the joint Gaussian log-density of the observed y built from "
+                                  "the model's covariances (scipy multivariate_normal); equals "
+                                  "orc_kalman_loglik_lg on a uniform schedule",
     "orc_mlp_forward": "torch float64 forward of the same network (GELU z sigmoid(1.702 z))",
     "orc_mlp_loss_grad": "central finite differences of the FP64 loss; bf16 mode: rounding "
                          "points checked against numpy bf16 rounding and torch with the same "
@@ -84,6 +91,8 @@ class PFConfig(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("seed", ctypes.c_uint64),
         ("batch", ctypes.c_uint32),
+        ("times", ctypes.POINTER(ctypes.c_double)),
+        ("observed", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -181,6 +190,9 @@ def _declare(L):
     L.orc_mlp_loss_grad.restype = c.c_double
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
+    L.orc_kalman_loglik_lg_sched.argtypes = [c.POINTER(Model), _D, c.POINTER(c.c_int), c.c_int,
+                                             _D]
+    L.orc_kalman_loglik_lg_sched.restype = c.c_double
 
 
 def _dp(a):
@@ -417,11 +429,21 @@ def gather_rows(a, X):
 
 # ---------------------------------------------------------------- filter
 def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=None,
-           batch=0, want_ancestors=False, want_state=False):
+           batch=0, want_ancestors=False, want_state=False, times=None, observed=None):
+    """times (T + 1) and observed (T flags), both optional, give a schedule in
+    place of t0 + k dt with every step observed."""
     y = _f64(y).reshape(len(y), -1)
     T = y.shape[0]
     cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt, seed=seed,
                    batch=batch)
+    if times is not None:
+        times = _f64(times)
+        assert times.shape == (T + 1,)
+        cfg.times = _dp(times)
+    if observed is not None:
+        observed = np.ascontiguousarray(observed, np.int32)
+        assert observed.shape == (T,)
+        cfg.observed = _ip(observed)
     logZ = ctypes.c_double()
     li = np.zeros(T)
     ess = np.zeros(T)
@@ -522,3 +544,12 @@ def pmmh_run(m, cfg, C, N, y):
 def kalman_loglik_lg(m, dt, y):
     y = _f64(y).ravel()
     return lib().orc_kalman_loglik_lg(ctypes.byref(m), dt, len(y), _dp(y))
+
+
+def kalman_loglik_lg_sched(m, times, observed, y):
+    """Exact log-likelihood of the LG model over a schedule (times: T + 1,
+    observed: T flags; unobserved steps predict only)."""
+    y = _f64(y).ravel()
+    times = _f64(times)
+    obs = np.ascontiguousarray(observed, np.int32)
+    return lib().orc_kalman_loglik_lg_sched(ctypes.byref(m), _dp(times), _ip(obs), len(y), _dp(y))

@@ -30,6 +30,8 @@ struct orc_pf {
   int *Os, *os, *as, *a0;
   double logLikelihood, ess;
   int k;
+  const double* times;  /* nullable: t0 + k dt */
+  const int* observed;  /* nullable: every step observed */
 };
 
 orc_pf* orc_pf_new(const orc_model* m, int P, int resampler, double essRel, double t0,
@@ -70,14 +72,27 @@ void orc_pf_init(orc_pf* f, const double* mean0) {
   f->k = 0;
 }
 
+void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed) {
+  f->times = times;
+  f->observed = observed;
+}
+
+static double pf_time(const orc_pf* f, int k) {
+  return f->times ? f->times[k] : f->t0 + k * f->dt;
+}
+
+static int pf_observed(const orc_pf* f, int k) { return f->observed ? f->observed[k] != 0 : 1; }
+
 /* One BootstrapPF::step from t_k to t_{k+1} with observation y_{k+1}.
  * resampled/ancestors (nullable) report the resampling at t_k.  Returns
  * ORC_OK or ORC_DEGENERATE; *logIncrement gets lW - logLikelihood. */
 int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled,
                 int* ancestors) {
   const int P = f->P, D = f->m.d, k = f->k;
-  const double t1 = f->t0 + k * f->dt, t2 = f->t0 + (k + 1) * f->dt;
-  const int r = k >= 1 && f->ess < f->essRel * P;
+  const double t1 = pf_time(f, k), t2 = pf_time(f, k + 1);
+  /* Resampler::resample (share/src/bi/resampler/Resampler.hpp:165-182): only at
+   * an observed element, when the ESS fell below the threshold */
+  const int r = k >= 1 && pf_observed(f, k - 1) && f->ess < f->essRel * P;
   int rc = ORC_OK;
   if (r) {
     orc_sumexpu_inclusive_scan(f->lws, P, f->Ws);
@@ -110,11 +125,17 @@ int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled
   if (resampled) *resampled = r;
 
   orc_transition(&f->m, f->seed, f->batch, (uint32_t)(k + 1), t1, t2, P, f->X, P, NULL);
-  orc_obs_logdensity(&f->m, y, P, f->X, P, f->lws);
-  double lW;
-  f->ess = orc_ess_reduce(f->lws, P, &lW);
-  if (logIncrement) *logIncrement = lW - f->logLikelihood;
-  f->logLikelihood = lW;
+  /* BootstrapPF::correct (share/src/bi/filter/BootstrapPF.hpp:171-181): only
+   * at an observed element */
+  if (pf_observed(f, k)) {
+    orc_obs_logdensity(&f->m, y, P, f->X, P, f->lws);
+    double lW;
+    f->ess = orc_ess_reduce(f->lws, P, &lW);
+    if (logIncrement) *logIncrement = lW - f->logLikelihood;
+    f->logLikelihood = lW;
+  } else if (logIncrement) {
+    *logIncrement = 0.0;
+  }
   f->k = k + 1;
   return ORC_OK;
 }
@@ -158,6 +179,7 @@ int orc_pf_run(const orc_model* m, const orc_pf_cfg* cfg, const double* y,
                          cfg->batch);
   int rc = ORC_OK;
   orc_pf_init(f, mean0);
+  orc_pf_set_schedule(f, cfg->times, cfg->observed);
   for (int k = 0; k < T; ++k) {
     double li;
     int r;
@@ -182,6 +204,24 @@ double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* 
   for (int k = 0; k < T; ++k) {
     mean = A * mean;
     var = A * A * var + q * q;
+    const double S = var + r * r, v = y[k] - mean;
+    ll += -0.5 * log(2.0 * M_PI * S) - 0.5 * v * v / S;
+    const double K = var / S;
+    mean = mean + K * v;
+    var = (1.0 - K) * var;
+  }
+  return ll;
+}
+
+double orc_kalman_loglik_lg_sched(const orc_model* m, const double* times, const int* observed,
+                                  int T, const double* y) {
+  const double q = m->par[1], r = m->par[2];
+  double mean = m->par[3], var = m->par[4] * m->par[4], ll = 0.0;
+  for (int k = 0; k < T; ++k) {
+    const double A = exp(-m->par[0] * (times[k + 1] - times[k]));
+    mean = A * mean;
+    var = A * A * var + q * q;
+    if (observed && !observed[k]) continue;
     const double S = var + r * r, v = y[k] - mean;
     ll += -0.5 * log(2.0 * M_PI * S) - 0.5 * v * v / S;
     const double K = var / S;

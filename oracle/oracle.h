@@ -124,6 +124,10 @@ typedef struct {
   double t0, dt;
   uint64_t seed;
   uint32_t batch;
+  /* optional schedule (both nullable): times[0..T] replace t0 + k*dt, and
+   * observed[k] = 0 makes step k a prediction without correction */
+  const double* times;
+  const int* observed;
 } orc_pf_cfg;
 
 /*
@@ -142,6 +146,10 @@ orc_pf* orc_pf_new(const orc_model* m, int P, int resampler, double essRel, doub
     double dt, uint64_t seed, uint32_t batch);
 void orc_pf_free(orc_pf* f);
 void orc_pf_init(orc_pf* f, const double* mean0);
+/* Schedule (Schedule, share/src/bi/state/Schedule.hpp:214-337, restricted to
+ * output and observation times): times[0..T] and observed[0..T-1], both
+ * nullable and kept by reference. */
+void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed);
 int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled, int* ancestors);
 double orc_pf_loglikelihood(const orc_pf* f);
 double orc_pf_ess(const orc_pf* f);
@@ -185,6 +193,10 @@ double orc_mlp_loss_grad(int L, const int* dims, const double* params, const dou
 
 /* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
 double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);
+/* the same over a schedule: interval k is [times[k], times[k+1]], and step k
+ * predicts without an update when observed[k] == 0 */
+double orc_kalman_loglik_lg_sched(const orc_model* m, const double* times, const int* observed,
+    int T, const double* y);
 
 #ifdef __cplusplus
 }

@@ -275,3 +275,93 @@ def test_netcdf_output_needs_recorded_paths(ctx, orc, tmp_path):
     with pytest.raises(capi.BihipError):
         pf.write_netcdf(tmp_path / "x.nc")
     pf.close()
+
+
+def _irregular(T, seed=3, p_unobserved=0.3):
+    rng = np.random.default_rng(seed)
+    times = np.concatenate([[0.0], np.cumsum(rng.uniform(0.3, 1.7, size=T))])
+    observed = (rng.uniform(size=T) > p_unobserved).astype(np.int32)
+    observed[0] = 1
+    return times, observed
+
+
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.MULTINOMIAL])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_scheduled_filter_matches_oracle(ctx, orc, resampler, use_graph):
+    """Irregular observation times with unobserved steps (bihip_pf_set_schedule):
+    bit-exact ancestors and resampling decisions, log-increments (zero where
+    unobserved) and logZ against the oracle's scheduled filter, FP64."""
+    T, P, seed = 40, 4096, 21
+    y = _pz_data(orc, T)
+    times, observed = _irregular(T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    om = oracle_model(orc, gm)
+    pf = capi.ParticleFilter(ctx, gm, P, T, 1.0, seed, resampler=resampler, ess_rel=0.8,
+                             record=True, use_graph=use_graph)
+    pf.set_schedule(times, observed)
+    pf.set_observations(y)
+    pf.init()
+    pf.step(T)
+    logZ, out = pf.term(), pf.outputs()
+    anc = np.array([pf.ancestors(k) for k in range(T)])
+    pf.close()
+    ref = orc.pf_run(om, y, P, 1.0, seed, resampler=resampler, ess_rel=0.8,
+                     want_ancestors=True, times=times, observed=observed)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=1e-9, atol=1e-9)
+    assert np.all(out["logIncrements"][observed == 0] == 0.0)
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+    assert out["resampled"].any()
+
+
+def test_scheduled_lg_filter_matches_kalman(ctx, orc):
+    """The scheduled filter's logZ on the linear-Gaussian model against the exact
+    Kalman log-likelihood over the same schedule (N = 2^20, four seeds)."""
+    T = 30
+    times, observed = _irregular(T, seed=5)
+    om = orc.make_model("lg", lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0, atol=1e-10,
+                        rtol=1e-10, h0=0.25)
+    x0 = orc.init_samples(om, 11, 1)[:, 0]
+    _, y = orc.simulate(om, 11, T, 1.0, x0)
+    exact = orc.kalman_loglik_lg_sched(om, times, observed, y)
+    gm = capi.make_model("lg", precision=capi.F64, lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0,
+                         s0=1.0, atol=1e-10, rtol=1e-10, h0=0.25)
+    zs = []
+    for seed in range(4):
+        pf = capi.ParticleFilter(ctx, gm, 1 << 20, T, 1.0, 100 + seed, ess_rel=0.5)
+        pf.set_schedule(times, observed)
+        pf.set_observations(y)
+        pf.init()
+        pf.step(T)
+        zs.append(pf.term())
+        pf.close()
+    print("scheduled LG logZ", np.mean(zs), np.std(zs), "Kalman", exact, flush=True)
+    assert abs(np.mean(zs) - exact) < 0.02
+
+
+def test_schedule_rejects_bad_times(ctx, orc):
+    pf = capi.ParticleFilter(ctx, capi.make_model("pz", precision=capi.F64), 64, 3, 1.0, 1)
+    with pytest.raises(capi.BihipError):
+        pf.set_schedule([0.0, 1.0, 1.0, 2.0])
+    with pytest.raises(ValueError):
+        pf.set_schedule([0.0, 1.0, 2.0])
+    pf.close()
+
+
+def test_netcdf_times_follow_the_schedule(ctx, orc, tmp_path):
+    from scipy.io import netcdf_file
+    T = 6
+    y = _pz_data(orc, T)
+    times, observed = _irregular(T, seed=9)
+    pf = capi.ParticleFilter(ctx, capi.make_model("pz", precision=capi.F64), 128, T, 1.0, 3,
+                             record="paths")
+    pf.set_schedule(times, observed)
+    pf.set_observations(y)
+    pf.init()
+    pf.step(T)
+    pf.write_netcdf(tmp_path / "s.nc")
+    pf.close()
+    f = netcdf_file(str(tmp_path / "s.nc"), "r", mmap=False)
+    np.testing.assert_array_equal(np.array(f.variables["time"].data), times)
+    f.close()

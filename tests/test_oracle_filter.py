@@ -77,3 +77,86 @@ def test_pz_filter_runs(orc):
     r = orc.pf_run(m, y, 1024, 1.0, 1, ess_rel=1.0)
     assert r["rc"] == orc.OK and np.isfinite(r["logZ"])
     assert r["resampled"][1:].all()
+
+
+# ---------------------------------------------------------------- schedules
+def _schedule(T, seed=3, p_unobserved=0.3):
+    rng = np.random.default_rng(seed)
+    times = np.concatenate([[0.0], np.cumsum(rng.uniform(0.3, 1.7, size=T))])
+    observed = (rng.uniform(size=T) > p_unobserved).astype(np.int32)
+    observed[0] = 1
+    return times, observed
+
+
+def test_uniform_schedule_equals_plain_run(orc):
+    m, y = _data(orc, 20)
+    times = np.arange(21, dtype=float)
+    a = orc.pf_run(m, y, 1024, 1.0, 5, ess_rel=0.5, want_ancestors=True)
+    b = orc.pf_run(m, y, 1024, 1.0, 5, ess_rel=0.5, want_ancestors=True, times=times,
+                   observed=np.ones(20, np.int32))
+    assert a["logZ"] == b["logZ"]
+    np.testing.assert_array_equal(a["logIncrements"], b["logIncrements"])
+    np.testing.assert_array_equal(a["ancestors"], b["ancestors"])
+
+
+def test_scheduled_kalman_is_the_joint_gaussian(orc):
+    """The scheduled Kalman log-likelihood equals the log-density of the observed
+    y under their joint Gaussian (x_{k+1} = A_k x_k + q e, y_k = x_{k+1} + r e,
+    A_k = exp(-lambda (t_{k+1} - t_k))), built directly from the covariances."""
+    from scipy.stats import multivariate_normal
+    m = _lg(orc)
+    lam, q, r, m0, s0 = 0.5, 0.5, 0.5, 0.0, 1.0
+    T = 6
+    times, observed = _schedule(T, seed=8)
+    y = np.random.default_rng(9).normal(size=T)
+    A = np.exp(-lam * np.diff(times))
+    # x_{k+1} = (prod A) x0 + sum_j (prod_{i>j} A_i) q e_j
+    C = np.zeros((T, T + 1))  # x_{k+1} in terms of (x0, e_0 .. e_{T-1})
+    for k in range(T):
+        C[k, 0] = np.prod(A[:k + 1])
+        for j in range(k + 1):
+            C[k, 1 + j] = np.prod(A[j + 1:k + 1]) * q
+    S = np.diag([s0 ** 2] + [1.0] * T)
+    cov_x = C @ S @ C.T
+    mean_x = C[:, 0] * m0
+    idx = np.flatnonzero(observed)
+    cov_y = cov_x[np.ix_(idx, idx)] + r ** 2 * np.eye(len(idx))
+    exact = multivariate_normal(mean_x[idx], cov_y).logpdf(y[idx])
+    got = orc.kalman_loglik_lg_sched(m, times, observed, y)
+    assert got == pytest.approx(exact, rel=1e-10)
+    assert orc.kalman_loglik_lg_sched(m, np.arange(T + 1.0), np.ones(T), y) == pytest.approx(
+        orc.kalman_loglik_lg(m, 1.0, y), rel=1e-12)
+
+
+@pytest.mark.parametrize("ess_rel", [1.0, 0.5])
+def test_scheduled_pf_matches_kalman(orc, ess_rel):
+    """Irregular observation times and unobserved steps: the particle filter's
+    logZ against the scheduled Kalman log-likelihood."""
+    m = _lg(orc)
+    T = 30
+    times, observed = _schedule(T)
+    x0 = orc.init_samples(m, 11, 1)[:, 0]
+    _, y = orc.simulate(m, 11, T, 1.0, x0)
+    exact = orc.kalman_loglik_lg_sched(m, times, observed, y)
+    zs = [orc.pf_run(m, y, 1 << 14, 1.0, seed, ess_rel=ess_rel, times=times,
+                     observed=observed)["logZ"] for seed in (1, 2, 3)]
+    assert abs(np.mean(zs) - exact) < 0.1, (zs, exact)
+
+
+def test_scheduled_pf_bookkeeping(orc):
+    """Unobserved steps add nothing to the log-likelihood and are never followed
+    by a resampling; the ESS carries over."""
+    m, y = _data(orc, 25)
+    times, observed = _schedule(25, seed=4, p_unobserved=0.4)
+    r = orc.pf_run(m, y, 2048, 1.0, 7, ess_rel=0.9, want_ancestors=True, times=times,
+                   observed=observed)
+    assert r["rc"] == orc.OK
+    assert r["logZ"] == pytest.approx(r["logIncrements"].sum(), rel=1e-12)
+    for k in range(25):
+        if not observed[k]:
+            assert r["logIncrements"][k] == 0.0
+            if k > 0:
+                assert r["ess"][k] == r["ess"][k - 1]
+        if k >= 1:
+            assert r["resampled"][k] == (observed[k - 1] and r["ess"][k - 1] < 0.9 * 2048)
+    assert r["resampled"].any()
