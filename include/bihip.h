@@ -291,6 +291,14 @@ int bihip_pf_position(bihip_pf* pf);
  */
 int bihip_pf_set_schedule(bihip_pf* pf, const double* times_host, const int* observed_host);
 /*
+ * Inputs (Forcer, share/src/bi/simulator/Forcer.hpp, for parameters the model reads as
+ * inputs): inputs_host holds the model's 16 parameters for each step, T x 16; row k is
+ * in force over [t_k, t_{k+1}), in the transition and the observation density (the
+ * initial sampling keeps the model's parameters).  NULL removes them.  Not combined with
+ * the learned flow or proposal.
+ */
+int bihip_pf_set_inputs(bihip_pf* pf, const double* inputs_host);
+/*
  * Kernel timing with HIP events on the filter's stream: enable, then query
  * the summed milliseconds and launch count of one kernel class
  * (0 = transition, 1 = resample chain, 2 = weight reduction, 3 = the guided

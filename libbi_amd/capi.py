@@ -61,7 +61,7 @@ EXPORTS = [
     "bihip_pfout_create", "bihip_pfout_write_time", "bihip_pfout_write_state",
     "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
-    "bihip_pf_set_schedule",
+    "bihip_pf_set_schedule", "bihip_pf_set_inputs",
 ]
 
 
@@ -250,6 +250,7 @@ def _declare(L):
         "bihip_pfout_close": [P],
         "bihip_pf_write_netcdf": [P, c.c_char_p],
         "bihip_pf_set_schedule": [P, pd, pi],
+        "bihip_pf_set_inputs": [P, pd],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -280,6 +281,15 @@ def ptr(t):
 
 
 # ---------------------------------------------------------------- models
+# parameter names -> index in bihip_model::par, per model (make_model's order)
+PAR_INDEX = {
+    MODEL_PZ: {n: i for i, n in enumerate(["mu", "sigma", "c", "e", "m_l", "m_q", "obs_sd",
+                                           "P0_logmean", "P0_logsd", "Z0_logmean", "Z0_logsd"])},
+    MODEL_L96: {n: i for i, n in enumerate(["F", "noise_sd", "obs_sd", "init_sd"])},
+    MODEL_LG: {n: i for i, n in enumerate(["lam", "noise_sd", "obs_sd", "m0", "s0"])},
+}
+
+
 def make_model(kind, d=None, integrator=None, precision=F64, h0=None, atol=1e-5, rtol=1e-5,
                nsteps=1000, host_quirk=0, **par):
     """Model description with the same parameter names as the oracle's make_model."""
@@ -485,6 +495,25 @@ class ParticleFilter:
         check(lib().bihip_pf_set_schedule(
             self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             o.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if o is not None else None))
+
+    def set_inputs(self, inputs):
+        """The model's 16 parameters for each step (T x 16), or None to remove
+        them; see inputs_from()."""
+        if inputs is None:
+            check(lib().bihip_pf_set_inputs(self.h, None))
+            return
+        v = np.ascontiguousarray(inputs, np.float64)
+        if v.shape != (self.T, 16):
+            raise ValueError(f"need a {self.T} x 16 array of parameters")
+        check(lib().bihip_pf_set_inputs(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def inputs_from(self, **series):
+        """T x 16 parameters: the model's, with named parameters replaced by
+        per-step series (e.g. F=array of T values for the Lorenz-96 forcing)."""
+        out = np.tile(np.array(self.model.par[:16], np.float64), (self.T, 1))
+        for name, vals in series.items():
+            out[:, PAR_INDEX[self.model.model][name]] = vals
+        return out
 
     def write_netcdf(self, path):
         """The recorded history (record="paths") in LibBi's ParticleFilter NetCDF schema."""

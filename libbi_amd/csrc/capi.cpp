@@ -614,6 +614,8 @@ struct bihip_pf {
   double* times = nullptr;
   int* obs_mask = nullptr;
   std::vector<double> times_host;
+  // inputs (bihip_pf_set_inputs): T x 16 model parameters
+  double* inputs = nullptr;
 };
 
 // Timing events only: created without the system-scope fence, so recording one
@@ -692,6 +694,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   T.nobs = pf->nobs;
   T.times = pf->times;
   T.mask = pf->obs_mask;
+  T.inputs = pf->inputs;
   if (pf->guided) {
     // exact flow of the gathered particles (no noise, no correction) into Xout,
     // then the proposal and the corrected weights in place
@@ -873,6 +876,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->hist_lw);
   dfree(pf->times);
   dfree(pf->obs_mask);
+  dfree(pf->inputs);
   dfree(pf->out);
   dfree(pf->dev);
   dfree(pf->y);
@@ -1064,6 +1068,29 @@ int bihip_pf_set_schedule(bihip_pf* pf, const double* times_host, const int* obs
   return BIHIP_OK;
 }
 
+int bihip_pf_set_inputs(bihip_pf* pf, const double* inputs_host) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  if (pf->surr || pf->guided)
+    return set_error(BIHIP_ESTATE, "detach the learned flow or proposal before setting inputs");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  if (inputs_host == nullptr) {
+    dfree(pf->inputs);
+    pf->inputs = nullptr;
+  } else {
+    const size_t n = (size_t)pf->cfg.T * 16;
+    if (pf->inputs == nullptr && (rc = dalloc(&pf->inputs, n))) return rc;
+    BIHIP_HIP(hipMemcpyAsync(pf->inputs, inputs_host, sizeof(double) * n, hipMemcpyHostToDevice,
+                             pf->ctx->stream));
+    BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  }
+  for (int i = 0; i < 2; ++i) {  // captured steps hold the previous arguments
+    if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
+    pf->graph[i] = nullptr;
+  }
+  return BIHIP_OK;
+}
+
 int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_norm* norm) {
   if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
   int rc = ctx_enter(pf->ctx);
@@ -1074,8 +1101,9 @@ int bihip_pf_set_surrogate(bihip_pf* pf, bihip_mlp* mlp, const bihip_surrogate_n
         pf->model.precision != BIHIP_F32)
       return set_error(BIHIP_EINVAL, "the surrogate flow serves FP32 Lorenz-96 d=40 filters");
     if (pf->guided) return set_error(BIHIP_ESTATE, "detach the guided proposal first");
-    if (pf->times != nullptr)
-      return set_error(BIHIP_ESTATE, "the surrogate flow serves the uniform schedule it was trained on");
+    if (pf->times != nullptr || pf->inputs != nullptr)
+      return set_error(BIHIP_ESTATE,
+                       "the surrogate flow serves the uniform schedule and parameters it was trained on");
     SurrogateLaunch S{};
     S.net = mlp_view(mlp);
     if (S.net.L != 4 || S.net.dims[0] != 41 || S.net.dims[4] != 40)
@@ -1114,8 +1142,9 @@ int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* n
         pf->model.precision != BIHIP_F32)
       return set_error(BIHIP_EINVAL, "the guided proposal serves FP32 Lorenz-96 d=200 filters");
     if (pf->surr) return set_error(BIHIP_ESTATE, "detach the surrogate flow first");
-    if (pf->times != nullptr)
-      return set_error(BIHIP_ESTATE, "the guided proposal serves the uniform schedule it was trained on");
+    if (pf->times != nullptr || pf->inputs != nullptr)
+      return set_error(BIHIP_ESTATE,
+                       "the guided proposal serves the uniform schedule and parameters it was trained on");
     GuidedLaunch G{};
     G.net = mlp_view(mlp);
     if (G.net.L != 4 || G.net.dims[0] != 301 || G.net.dims[4] != 400)

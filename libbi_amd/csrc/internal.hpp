@@ -118,6 +118,9 @@ struct TransLaunch {
   // t0 + k dt, and mask[k] == 0 for a step without observation (no correction)
   const double* times;
   const int* mask;
+  // with dev: the model's 16 parameters for each step (nullable), LibBi's
+  // inputs as set by a Forcer over [t_k, t_{k+1})
+  const double* inputs;
 };
 int transition_blocks(const bihip_model* m, int P);
 int launch_transition(const TransLaunch& L, hipStream_t s);

@@ -36,10 +36,23 @@ struct TransArgs {
   const double* aux;    // flow_only: per-particle noise variable (PZ alpha) or null
   const double* times;  // with dev: schedule times (nullable: t0 + k dt)
   const int* mask;      // with dev: observed flags (nullable: every step observed)
+  const double* inputs; // with dev: 16 model parameters per step (nullable)
 };
 
 // ---------------------------------------------------------------- the kernel
-template <class M, int INTEG>
+// A value the whole wave shares, moved to a scalar register so that the
+// compiler keeps what derives from it out of vector registers.
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// SCHED: the filter step has a schedule or inputs (times, observed flags,
+// per-step parameters); without them the kernel is compiled without that code,
+// which costs the plain step about 7% when present but unused.
+template <class M, int INTEG, bool SCHED = false>
 __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
   using T = typename M::real_t;
   constexpr int LPP = M::LPP, DL = M::DL;
@@ -55,17 +68,21 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
   double t1 = a.t1, t2 = a.t2;
   uint32_t step = a.step;
   const double* y = a.y;
+  // the inputs in force over [t_k, t_{k+1}) (Forcer): the step's parameters
+  const double* in = SCHED && a.dev != nullptr && a.inputs != nullptr
+                         ? a.inputs + (size_t)16 * a.dev->k
+                         : nullptr;
   if (a.dev != nullptr) {
     gather = a.dev->resample != 0 && a.dev->status == BIHIP_OK;
     lws_const = gather;
     lws_base = a.dev->logLikelihood;
     const int k = a.dev->k;
-    t1 = a.times != nullptr ? a.times[k] : a.t0 + k * a.dt;
-    t2 = a.times != nullptr ? a.times[k + 1] : a.t0 + (k + 1) * a.dt;
+    t1 = SCHED && a.times != nullptr ? a.times[k] : a.t0 + k * a.dt;
+    t2 = SCHED && a.times != nullptr ? a.times[k + 1] : a.t0 + (k + 1) * a.dt;
     step = (uint32_t)(k + 1);
     // BootstrapPF::correct applies only at observed elements
     // (share/src/bi/filter/BootstrapPF.hpp:171-181); null y_base: flow only
-    const bool observed = a.mask == nullptr || a.mask[k] != 0;
+    const bool observed = !SCHED || a.mask == nullptr || a.mask[k] != 0;
     y = a.y_base != nullptr && observed ? a.y_base + (size_t)k * a.nobs : nullptr;
   }
 
@@ -77,7 +94,14 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
 #pragma unroll
     for (int i = 0; i < DL; ++i) x[i] = Xin[(size_t)(lo + i) * a.ld + src];
 
-    M m(a.m);
+    auto model = [&]() {
+      if (!SCHED || in == nullptr) return M(a.m);
+      ModelArgs ma = a.m;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) ma.par[i] = uniform(in[i]);
+      return M(ma);
+    };
+    M m = model();
     if constexpr (M::NOISE_FIRST) {
       if (!a.flow_only) {
         noise_first(m, a.seed, a.batch, p, step);
@@ -101,13 +125,15 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
     if (a.nsteps_out != nullptr && grp.g == 0) a.nsteps_out[p] = n;
 
     if (y != nullptr) {
-      const double part = grp.sum(obs_of(m, x, y, lo, &a.m));
+      // observation constants from the host, or from the step's inputs
+      const double part =
+          grp.sum(obs_of(m, x, y, lo, !SCHED || in == nullptr ? &a.m : nullptr));
       if (grp.g == 0) {
         const double base = lws_const ? lws_base : a.lws[p];
         lw = base + part;
         a.lws[p] = lw;
       }
-    } else if (lws_const && a.lws != nullptr && grp.g == 0) {
+    } else if (SCHED && lws_const && a.lws != nullptr && grp.g == 0) {
       a.lws[p] = lws_base;  // resampled, then no observation: lws = logLikelihood
     }
   }
@@ -267,11 +293,21 @@ struct TransitionRunner {
   int integ;
   template <class M>
   int run() {
+    const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
+                                            a.inputs != nullptr);
     if (integ == BIHIP_RK4) {
-      hipLaunchKernelGGL((k_transition<M, BIHIP_RK4>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, a);
+      if (sched)
+        hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, true>), dim3(blocks), dim3(TRANS_BLOCK), 0,
+                           s, a);
+      else
+        hipLaunchKernelGGL((k_transition<M, BIHIP_RK4>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, a);
     } else {
-      hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5>), dim3(blocks), dim3(TRANS_BLOCK), 0, s,
-                         a);
+      if (sched)
+        hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5, true>), dim3(blocks), dim3(TRANS_BLOCK),
+                           0, s, a);
+      else
+        hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5>), dim3(blocks), dim3(TRANS_BLOCK), 0, s,
+                           a);
     }
     BIHIP_LAUNCH_CHECK();
     return BIHIP_OK;
@@ -301,6 +337,7 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.dt = L.dt;
   r.a.times = L.times;
   r.a.mask = L.mask;
+  r.a.inputs = L.inputs;
   r.a.y_base = L.y_base;
   r.a.nobs = L.nobs;
   r.a.flow_only = L.flow_only;

@@ -49,6 +49,9 @@ PINNED = {
                             "bit for bit; LG model over irregular times with unobserved steps "
                             "against the scheduled Kalman log-likelihood; bookkeeping (zero "
                             "increments, no resampling after an unobserved step)",
+    "orc_pf_run(inputs)": "inputs equal to the model's parameters reproduce the plain run bit "
+                          "for bit; LG model with a per-step decay rate against the Kalman "
+                          "likelihood with the same per-step transition coefficients",
     "orc_kalman_loglik_lg_sched": "the joint Gaussian log-density of the observed y built from "
                                   "the model's covariances (scipy multivariate_normal); equals "
                                   "orc_kalman_loglik_lg on a uniform schedule",
@@ -93,6 +96,7 @@ class PFConfig(ctypes.Structure):
         ("batch", ctypes.c_uint32),
         ("times", ctypes.POINTER(ctypes.c_double)),
         ("observed", ctypes.POINTER(ctypes.c_int)),
+        ("inputs", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -429,9 +433,11 @@ def gather_rows(a, X):
 
 # ---------------------------------------------------------------- filter
 def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=None,
-           batch=0, want_ancestors=False, want_state=False, times=None, observed=None):
+           batch=0, want_ancestors=False, want_state=False, times=None, observed=None,
+           inputs=None):
     """times (T + 1) and observed (T flags), both optional, give a schedule in
-    place of t0 + k dt with every step observed."""
+    place of t0 + k dt with every step observed; inputs (T x 16, optional) the
+    model's parameters during each step."""
     y = _f64(y).reshape(len(y), -1)
     T = y.shape[0]
     cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt, seed=seed,
@@ -444,6 +450,10 @@ def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=N
         observed = np.ascontiguousarray(observed, np.int32)
         assert observed.shape == (T,)
         cfg.observed = _ip(observed)
+    if inputs is not None:
+        inputs = _f64(inputs)
+        assert inputs.shape == (T, 16)
+        cfg.inputs = _dp(inputs)
     logZ = ctypes.c_double()
     li = np.zeros(T)
     ess = np.zeros(T)

@@ -32,6 +32,7 @@ struct orc_pf {
   int k;
   const double* times;  /* nullable: t0 + k dt */
   const int* observed;  /* nullable: every step observed */
+  const double* inputs; /* nullable: T x 16 parameters per step */
 };
 
 orc_pf* orc_pf_new(const orc_model* m, int P, int resampler, double essRel, double t0,
@@ -76,6 +77,8 @@ void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed) {
   f->times = times;
   f->observed = observed;
 }
+
+void orc_pf_set_inputs(orc_pf* f, const double* inputs) { f->inputs = inputs; }
 
 static double pf_time(const orc_pf* f, int k) {
   return f->times ? f->times[k] : f->t0 + k * f->dt;
@@ -124,11 +127,13 @@ int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled
   if (ancestors) memcpy(ancestors, f->as, sizeof(int) * (size_t)P);
   if (resampled) *resampled = r;
 
-  orc_transition(&f->m, f->seed, f->batch, (uint32_t)(k + 1), t1, t2, P, f->X, P, NULL);
+  orc_model mk = f->m;  /* the inputs in force over [t_k, t_{k+1}) */
+  if (f->inputs) memcpy(mk.par, f->inputs + (size_t)16 * k, sizeof(double) * 16);
+  orc_transition(&mk, f->seed, f->batch, (uint32_t)(k + 1), t1, t2, P, f->X, P, NULL);
   /* BootstrapPF::correct (share/src/bi/filter/BootstrapPF.hpp:171-181): only
    * at an observed element */
   if (pf_observed(f, k)) {
-    orc_obs_logdensity(&f->m, y, P, f->X, P, f->lws);
+    orc_obs_logdensity(&mk, y, P, f->X, P, f->lws);
     double lW;
     f->ess = orc_ess_reduce(f->lws, P, &lW);
     if (logIncrement) *logIncrement = lW - f->logLikelihood;
@@ -180,6 +185,7 @@ int orc_pf_run(const orc_model* m, const orc_pf_cfg* cfg, const double* y,
   int rc = ORC_OK;
   orc_pf_init(f, mean0);
   orc_pf_set_schedule(f, cfg->times, cfg->observed);
+  orc_pf_set_inputs(f, cfg->inputs);
   for (int k = 0; k < T; ++k) {
     double li;
     int r;

@@ -128,6 +128,9 @@ typedef struct {
    * observed[k] = 0 makes step k a prediction without correction */
   const double* times;
   const int* observed;
+  /* optional inputs (nullable): the model's 16 parameters for each step,
+   * T x 16, in force over [t_k, t_{k+1}) */
+  const double* inputs;
 } orc_pf_cfg;
 
 /*
@@ -150,6 +153,10 @@ void orc_pf_init(orc_pf* f, const double* mean0);
  * output and observation times): times[0..T] and observed[0..T-1], both
  * nullable and kept by reference. */
 void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed);
+/* Inputs (Forcer, share/src/bi/simulator/Forcer.hpp, as parameter values per
+ * step): inputs[16 k .. 16 k + 15] replace the model's parameters during step
+ * k (transition and observation density).  Nullable, kept by reference. */
+void orc_pf_set_inputs(orc_pf* f, const double* inputs);
 int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled, int* ancestors);
 double orc_pf_loglikelihood(const orc_pf* f);
 double orc_pf_ess(const orc_pf* f);

@@ -365,3 +365,49 @@ def test_netcdf_times_follow_the_schedule(ctx, orc, tmp_path):
     f = netcdf_file(str(tmp_path / "s.nc"), "r", mmap=False)
     np.testing.assert_array_equal(np.array(f.variables["time"].data), times)
     f.close()
+
+
+def test_inputs_match_oracle(ctx, orc):
+    """Per-step parameters (bihip_pf_set_inputs, LibBi's inputs): a Lorenz-96
+    forcing F_k that varies over the steps, FP64 d=40 with a schedule too:
+    bit-exact ancestors and logZ against the oracle with the same inputs."""
+    om0 = orc.make_model("l96", d=40)
+    mean0 = l96_spun_up(orc)
+    T, P, seed = 15, 2048, 9
+    _, y = orc.simulate(om0, 7, T, 0.05, mean0)
+    gm = capi.make_model("l96", d=40, precision=capi.F64)
+    om = oracle_model(orc, gm)
+    pf = capi.ParticleFilter(ctx, gm, P, T, 0.05, seed, ess_rel=0.5, record=True)
+    F = 8.0 + 1.5 * np.sin(np.arange(T))
+    inputs = pf.inputs_from(F=F, noise_sd=0.1 + 0.02 * np.arange(T) / T)
+    times = np.concatenate([[0.0], np.cumsum(np.random.default_rng(2).uniform(0.03, 0.08, T))])
+    pf.set_schedule(times)
+    pf.set_inputs(inputs)
+    pf.set_observations(y)
+    pf.init(mean0)
+    pf.step(T)
+    logZ, out = pf.term(), pf.outputs()
+    anc = np.array([pf.ancestors(k) for k in range(T)])
+    pf.close()
+    ref = orc.pf_run(om, y, P, 0.05, seed, ess_rel=0.5, mean0=mean0, want_ancestors=True,
+                     times=times, inputs=inputs)
+    assert logZ == pytest.approx(ref["logZ"], rel=1e-9)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+def test_inputs_equal_to_the_parameters_change_nothing(ctx, orc):
+    y = _pz_data(orc, 20)
+    gm = capi.make_model("pz", precision=capi.F64)
+    res = []
+    for use in (False, True):
+        pf = capi.ParticleFilter(ctx, gm, 1024, 20, 1.0, 4, ess_rel=0.7, record=True)
+        if use:
+            pf.set_inputs(pf.inputs_from())
+        pf.set_observations(y)
+        pf.init()
+        pf.step(20)
+        res.append((pf.term(), np.array([pf.ancestors(k) for k in range(20)])))
+        pf.close()
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][1], res[1][1])

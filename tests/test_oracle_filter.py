@@ -160,3 +160,32 @@ def test_scheduled_pf_bookkeeping(orc):
         if k >= 1:
             assert r["resampled"][k] == (observed[k - 1] and r["ess"][k - 1] < 0.9 * 2048)
     assert r["resampled"].any()
+
+
+# ---------------------------------------------------------------- inputs
+def test_inputs_equal_to_the_parameters_change_nothing(orc):
+    m, y = _data(orc, 15)
+    inputs = np.tile(np.array(m.par[:16]), (15, 1))
+    a = orc.pf_run(m, y, 1024, 1.0, 5, ess_rel=0.5, want_ancestors=True)
+    b = orc.pf_run(m, y, 1024, 1.0, 5, ess_rel=0.5, want_ancestors=True, inputs=inputs)
+    assert a["logZ"] == b["logZ"]
+    np.testing.assert_array_equal(a["ancestors"], b["ancestors"])
+
+
+def test_time_varying_decay_matches_kalman(orc):
+    """A per-step decay rate lambda_k (an input) on unit intervals is the
+    constant-lambda model on intervals lambda_k / lambda: the filter's logZ
+    against the scheduled Kalman likelihood on those intervals."""
+    m = _lg(orc)
+    T = 30
+    lam = 0.5
+    lam_k = lam * np.random.default_rng(6).uniform(0.2, 3.0, size=T)
+    inputs = np.tile(np.array(m.par[:16]), (T, 1))
+    inputs[:, 0] = lam_k
+    x0 = orc.init_samples(m, 11, 1)[:, 0]
+    _, y = orc.simulate(m, 11, T, 1.0, x0)
+    times = np.concatenate([[0.0], np.cumsum(lam_k / lam)])
+    exact = orc.kalman_loglik_lg_sched(m, times, np.ones(T, np.int32), y)
+    zs = [orc.pf_run(m, y, 1 << 14, 1.0, seed, ess_rel=0.5, inputs=inputs)["logZ"]
+          for seed in (1, 2, 3)]
+    assert abs(np.mean(zs) - exact) < 0.1, (zs, exact)
