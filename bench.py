@@ -20,6 +20,9 @@ Other workloads (`--workload`), each printing one JSON line of the same form:
          parameter particles (mu, sigma, c, e from seeded log-normal priors),
          inner N_j = floor(exp(U(log 256, log 16384))), one persistent launch
          per observation interval replayed as a hipGraph, autotuned workgroup;
+  smc2-bi the same on the predator-prey model compiled from the LibBi model
+         file libbi_amd/bi/PZ.bi (theta = mu, sigma; the hand-written model
+         is timed beside it with the same theta);
   copy   a plain 1 GiB device-to-device copy: the achievable HBM ceiling, and
          the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
          counters into bytes;
@@ -63,7 +66,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="l96",
-                    choices=["l96", "pz", "smc2", "copy", "surrogate", "guided"])
+                    choices=["l96", "pz", "smc2", "smc2-bi", "copy", "surrogate", "guided"])
     ap.add_argument("--thetas", type=int, default=64)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=24000)
@@ -214,8 +217,11 @@ def smc2_population(B=SMC2_B, seed=2024):
     return N, np.stack([mu, sigma, c, e], axis=1)
 
 
-def bench_smc2(args):
-    """One observation interval of all 1024 inner filters per step."""
+def bench_smc2(args, generated=False):
+    """One observation interval of all 1024 inner filters per step.  generated:
+    the predator-prey model compiled from libbi_amd/bi/PZ.bi, whose theta is
+    (mu, sigma) (c and e are constants of the file); the hand-written model is
+    then also timed with the same two-parameter theta."""
     from libbi_amd import capi, data
     ctx = capi.Context(0)
     gm = capi.make_model("pz", precision=capi.F64)
@@ -223,6 +229,10 @@ def bench_smc2(args):
     _, y = data.simulate(ctx, gm, [2.0, 2.0], T, 1.0, seed=2024)
     N, theta = smc2_population()
     total = int(N.sum())
+    if generated:
+        theta = np.ascontiguousarray(theta[:, :2])
+        hand_rate = _batch_rate(ctx, gm, N, theta, y, args)
+        gm = capi.generated_model("PZ", precision=capi.F64, h0=gm.h0)
     bf = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, use_graph=True,
                           block=0)
     bf.set_observations(y)
@@ -250,8 +260,12 @@ def bench_smc2(args):
     nseq = 16
     seq_ps, seq_el = 0, 0.0
     for j in range(nseq):
-        m = capi.make_model("pz", precision=capi.F64, mu=theta[j, 0], sigma=theta[j, 1],
-                            c=theta[j, 2], e=theta[j, 3])
+        if generated:
+            m = capi.generated_model("PZ", precision=capi.F64, mu=theta[j, 0],
+                                     sigma=theta[j, 1])
+        else:
+            m = capi.make_model("pz", precision=capi.F64, mu=theta[j, 0], sigma=theta[j, 1],
+                                c=theta[j, 2], e=theta[j, 3])
         pf = capi.ParticleFilter(ctx, m, int(N[j]), T, 1.0, 77, ess_rel=0.5, batch=j,
                                  use_graph=True)
         pf.set_observations(y)
@@ -278,21 +292,41 @@ def bench_smc2(args):
                "kind": "port",
                "sample": f"the first {j} inner filters run one after another over T={T} "
                          f"({cel:.1f} s), OpenMP over particles"}
-    cfg = {"workload": "batched SMC^2 inner filters, predator-prey, 1024 theta-particles, "
-                       "N_j log-uniform in [256,16384], FP64, RK4 h=0.1, systematic at "
-                       "ESS<0.5N_j, one persistent launch per step (hipGraph)",
+    cfg = {"workload": ("batched SMC^2 inner filters, predator-prey "
+                        + ("generated from libbi_amd/bi/PZ.bi (theta = mu, sigma)"
+                           if generated else "(theta = mu, sigma, c, e)")
+                        + ", 1024 theta-particles, N_j log-uniform in [256,16384], FP64, RK4 "
+                          "h=0.1, systematic at ESS<0.5N_j, one persistent launch per step "
+                          "(hipGraph)"),
            "particles": total, "filters": SMC2_B, "block": blk, "grid": grid,
            "tune_ms": [float(v) for v in tune_ms],
            "resampled_fraction": float(res.mean()),
            "sequential_pf_particle_steps_per_s": seq_ps / seq_el,
            "sequential_pf_sample": f"filters 0..{nseq - 1} one after another through bihip_pf "
                                    "(hipGraph per step)"}
-    roof = {"bound": "hbm", "kernel": "k_batch_step<PZ<double>,RK4>", "achieved": achieved,
+    if generated:
+        cfg["handwritten_same_theta_particle_steps_per_s"] = hand_rate
+    kname = "k_batch_step<Gen_PZ<double>,RK4>" if generated else "k_batch_step<PZ<double>,RK4>"
+    roof = {"bound": "hbm", "kernel": kname, "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": None, "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n}
     print(json.dumps(_line(total * args.steps / el, args, 1e3 * el / args.steps, "f64", cfg,
                            roof, cpu)), flush=True)
     ctx.close()
+
+
+def _batch_rate(ctx, gm, N, theta, y, args):
+    """Particle-steps/s of the batched filters (hipGraph replay, tuned block)."""
+    from libbi_amd import capi
+    T = args.warmup + args.steps
+    bf = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, use_graph=True,
+                          block=0)
+    bf.set_observations(y)
+    bf.init()
+    bf.step(args.warmup)
+    el = _timed(ctx, lambda: bf.step(args.steps))
+    bf.close()
+    return int(N.sum()) * args.steps / el
 
 
 def bench_surrogate(args):
@@ -505,6 +539,8 @@ def main():
         return bench_pz(args)
     if args.workload == "smc2":
         return bench_smc2(args)
+    if args.workload == "smc2-bi":
+        return bench_smc2(args, generated=True)
     if args.workload == "copy":
         return bench_copy(args)
     if args.workload == "surrogate":

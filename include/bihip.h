@@ -46,6 +46,9 @@ extern "C" {
 #define BIHIP_MODEL_PZ 0  /* predator-prey, docs/src/users.tex:36-85 */
 #define BIHIP_MODEL_L96 1 /* Lorenz-96 with additive noise */
 #define BIHIP_MODEL_LG 2  /* scalar linear-Gaussian (Kalman-checkable) */
+/* first id of the models generated from LibBi model files (libbi_amd/bi);
+ * look them up by name with bihip_model_generated */
+#define BIHIP_MODEL_GENERATED 64
 
 #define BIHIP_RK4 0    /* share/src/bi/ode/RK4Integrator.hpp */
 #define BIHIP_DOPRI5 1 /* share/src/bi/ode/DOPRI5Integrator.hpp */
@@ -89,6 +92,29 @@ typedef struct {
   double h0, atol, rtol;
   double par[16];
 } bihip_model;
+
+/*
+ * Models generated from LibBi model files (the reference's `libbi
+ * --model-file`, lib/Bi/Parser.pm over share/bi.yp, code from
+ * share/tt/cpp/model.hpp.tt and the block and action templates).  Every
+ * file in libbi_amd/bi is compiled into the library when it is built.
+ * bihip_model_generated fills *m for the model named `name` (its `model`
+ * name in the file): id, state dimension, the ode block's algorithm ('RK4'
+ * or 'RK5(4)'), h, atoler and rtoler; par[] holds the model's `param` then
+ * `input` variables in declaration order, all zero.  The model's constants
+ * are compiled in.
+ */
+#define BIHIP_VAR_PARAM 0
+#define BIHIP_VAR_INPUT 1 /* par slots after the params; per step via bihip_pf_set_inputs */
+#define BIHIP_VAR_STATE 2
+#define BIHIP_VAR_OBS 3   /* order of the values of one observation */
+#define BIHIP_VAR_NOISE 4
+int bihip_model_generated(const char* name, int precision, bihip_model* m);
+/* Name of variable i of a kind (BIHIP_VAR_*) of a generated model; NULL past
+ * the end or for a hand-written model. */
+const char* bihip_model_var_name(const bihip_model* m, int kind, int i);
+/* The model's name for a generated model, NULL otherwise. */
+const char* bihip_model_name(const bihip_model* m);
 
 typedef struct bihip_ctx bihip_ctx;
 typedef struct bihip_pf bihip_pf;

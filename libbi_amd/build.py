@@ -2,8 +2,12 @@
 
 Every HIP source is compiled with packed-FP32 instructions disabled
 (-Xclang -target-feature -Xclang -packed-fp32-ops); objects go to
-libbi_amd/_build/ and the shared library to libbi_amd/lib/.
+libbi_amd/_build/ and the shared library to libbi_amd/lib/.  The LibBi model
+files in libbi_amd/bi/ are turned into device functors first
+(libbi_amd/dsl) and compiled in, as the reference compiles a program per
+model file.
 """
+import glob
 import os
 import subprocess
 import sys
@@ -14,6 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbihip.so")
+MODELS = os.path.join(HERE, "bi")  # LibBi model files compiled into the library
+GEN = os.path.join(OBJ, "gen")
 
 SOURCES = ["transition.hip", "reduce.hip", "scan.hip", "resample.hip", "batch.hip", "capi.cpp",
            "batch_capi.cpp", "smc2.cpp",
@@ -26,13 +32,30 @@ CFLAGS = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
-    "-I", os.path.join(ROOT, "include"),
+    "-I", os.path.join(ROOT, "include"), "-I", GEN,
 ]
+
+
+def generate_models():
+    """Writes the functors of the model files in libbi_amd/bi into the build
+    folder (generated_models.hpp, included by the kernels)."""
+    sys.path.insert(0, ROOT)
+    from libbi_amd.dsl import codegen
+    paths = glob.glob(os.path.join(MODELS, "*.bi"))
+    text = codegen.generate(codegen.load_models(paths))
+    os.makedirs(GEN, exist_ok=True)
+    out = os.path.join(GEN, "generated_models.hpp")
+    old = open(out).read() if os.path.exists(out) else None
+    if old != text:
+        with open(out, "w") as f:
+            f.write(text)
+    return out
 
 
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
+    generate_models()
     procs = []
     objs = []
     for src in SOURCES:

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libbihip.so")
 
 OK, DEGENERATE, EINVAL, EHIP, ESTATE, EIO = 0, 1, 2, 3, 4, 5
 MODEL_PZ, MODEL_L96, MODEL_LG = 0, 1, 2
+MODEL_GENERATED = 64  # first id of the models generated from libbi_amd/bi/*.bi
+VAR_PARAM, VAR_INPUT, VAR_STATE, VAR_OBS, VAR_NOISE = 0, 1, 2, 3, 4
 RK4, DOPRI5 = 0, 1
 SYSTEMATIC, STRATIFIED, MULTINOMIAL = 0, 1, 2
 F32, F64 = 0, 1
@@ -62,6 +64,7 @@ EXPORTS = [
     "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
+    "bihip_model_generated", "bihip_model_var_name", "bihip_model_name",
 ]
 
 
@@ -258,6 +261,11 @@ def _declare(L):
         f.restype = c.c_int
     L.bihip_last_error.argtypes = []
     L.bihip_last_error.restype = c.c_char_p
+    L.bihip_model_generated.argtypes = [c.c_char_p, c.c_int, P]
+    L.bihip_model_var_name.argtypes = [P, c.c_int, c.c_int]
+    L.bihip_model_var_name.restype = c.c_char_p
+    L.bihip_model_name.argtypes = [P]
+    L.bihip_model_name.restype = c.c_char_p
     L.bihip_uniform_host.argtypes = [u64, u32, u32, u32, u32]
     L.bihip_uniform_host.restype = c.c_double
     L.bihip_batch_particles.argtypes = [P]
@@ -320,6 +328,50 @@ def make_model(kind, d=None, integrator=None, precision=F64, h0=None, atol=1e-5,
     m.atol, m.rtol = atol, rtol
     for k, v in enumerate(vals):
         m.par[k] = v
+    return m
+
+
+def model_vars(m, kind):
+    """Variable names of one kind (VAR_*) of a generated model ([] otherwise)."""
+    out = []
+    while True:
+        n = lib().bihip_model_var_name(ctypes.byref(m), kind, len(out))
+        if n is None:
+            return out
+        out.append(n.decode())
+
+
+def par_index(m):
+    """Parameter name -> slot of bihip_model::par for a model description."""
+    if m.model >= MODEL_GENERATED:
+        names = model_vars(m, VAR_PARAM) + model_vars(m, VAR_INPUT)
+        return {n: i for i, n in enumerate(names)}
+    return PAR_INDEX[m.model]
+
+
+def generated_model(name, precision=F64, integrator=None, h0=None, atol=None, rtol=None,
+                    nsteps=None, **par):
+    """Description of a model compiled from a LibBi model file in
+    libbi_amd/bi (the `model` name of the file); named values set its params
+    and inputs, every other slot is zero.  The ode block's settings are the
+    defaults of integrator, h0, atol and rtol."""
+    m = Model()
+    check(lib().bihip_model_generated(os.fsencode(name), precision, ctypes.byref(m)))
+    idx = par_index(m)
+    for k, v in par.items():
+        if k not in idx:
+            raise KeyError(f"{name} has no param or input {k!r} (has {sorted(idx)})")
+        m.par[idx[k]] = v
+    if integrator is not None:
+        m.integrator = integrator
+    if h0 is not None:
+        m.h0 = h0
+    if atol is not None:
+        m.atol = atol
+    if rtol is not None:
+        m.rtol = rtol
+    if nsteps is not None:
+        m.nsteps = nsteps
     return m
 
 
@@ -512,7 +564,7 @@ class ParticleFilter:
         per-step series (e.g. F=array of T values for the Lorenz-96 forcing)."""
         out = np.tile(np.array(self.model.par[:16], np.float64), (self.T, 1))
         for name, vals in series.items():
-            out[:, PAR_INDEX[self.model.model][name]] = vals
+            out[:, par_index(self.model)[name]] = vals
         return out
 
     def write_netcdf(self, path):

@@ -15,6 +15,7 @@
 // single-filter path run with that batch id.
 #include "model_ops.hpp"
 #include "scan.hpp"
+#include "gen_model.hpp"
 
 namespace bihip {
 
@@ -237,13 +238,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
 #pragma unroll
       for (int c = 0; c < DL; ++c) x[c] = Xin[(size_t)c * a.ld + off + src];
       M mm = m;
-      if constexpr (M::NOISE_FIRST) noise_first(mm, a.seed, batch, i, step);
-      if constexpr (INTEG == BIHIP_RK4) {
-        integrate_rk4(mm, grp, x, t1, t2, a.m.h0);
+      if constexpr (is_generated<M>::value) {
+        mm.template transition<INTEG>(grp, x, t1, t2, a.m, a.seed, batch, (uint32_t)i, step);
       } else {
-        integrate_dopri5(mm, grp, x, t1, t2, a.m);
+        if constexpr (M::NOISE_FIRST) noise_first(mm, a.seed, batch, i, step);
+        if constexpr (INTEG == BIHIP_RK4) {
+          integrate_rk4(mm, grp, x, t1, t2, a.m.h0);
+        } else {
+          integrate_dopri5(mm, grp, x, t1, t2, a.m);
+        }
+        if constexpr (!M::NOISE_FIRST) noise_after(mm, a.seed, batch, i, 0, step, x);
       }
-      if constexpr (!M::NOISE_FIRST) noise_after(mm, a.seed, batch, i, 0, step, x);
 #pragma unroll
       for (int c = 0; c < DL; ++c) Xout[(size_t)c * a.ld + off + i] = x[c];
       const double lw = (res ? F.logLikelihood : a.lws[off + i]) + obs_of(mm, x, y, 0);
@@ -308,9 +313,17 @@ __global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, cons
   }
   const int j = lo;
   const int p = (int)(g - off[j]);
+  T* X = static_cast<T*>(Xv);
+  if constexpr (is_generated<M>::value) {
+    ModelArgs mj = ma;
+    for (int i = 0; i < ntheta && i < 16; ++i) mj.par[i] = theta[(size_t)j * ntheta + i];
+    double xd[DL];
+    M(mj).init(xd, seed, stream_base + (uint32_t)j, (uint32_t)p);
+    for (int c = 0; c < DL; ++c) X[(size_t)c * ld + g] = (T)xd[c];
+    return;
+  }
   T z[DL];
   slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, stream_base + (uint32_t)j, 0, z);
-  T* X = static_cast<T*>(Xv);
   for (int c = 0; c < DL; ++c) {
     double par[16];
     for (int i = 0; i < 16; ++i) par[i] = i < ntheta ? theta[(size_t)j * ntheta + i] : ma.par[i];
@@ -404,21 +417,36 @@ struct BatchRunner {
 template <class F>
 static int with_small_model(const bihip_model* m, F& f) {
   const bool f64 = m->precision == BIHIP_F64;
+  if (m->model >= BIHIP_MODEL_GENERATED) return with_generated(m, f);
   if (m->model == BIHIP_MODEL_PZ)
     return f64 ? f.template run<PZ<double>>() : f.template run<PZ<float>>();
   return f64 ? f.template run<LG<double>>() : f.template run<LG<float>>();
 }
 
 template <class F>
+struct OccupancyOf {
+  F& f;
+  template <class M>
+  int run() {
+    return f.template occupancy<M>();
+  }
+};
+
+template <class F>
 static int occupancy_of(const bihip_model* m, F& f) {
   const bool f64 = m->precision == BIHIP_F64;
+  if (m->model >= BIHIP_MODEL_GENERATED) {
+    OccupancyOf<F> o{f};
+    return with_generated(m, o);
+  }
   if (m->model == BIHIP_MODEL_PZ)
     return f64 ? f.template occupancy<PZ<double>>() : f.template occupancy<PZ<float>>();
   return f64 ? f.template occupancy<LG<double>>() : f.template occupancy<LG<float>>();
 }
 
 bool batch_model_supported(const bihip_model* m) {
-  return model_supported(m) && (m->model == BIHIP_MODEL_PZ || m->model == BIHIP_MODEL_LG);
+  return model_supported(m) && (m->model == BIHIP_MODEL_PZ || m->model == BIHIP_MODEL_LG ||
+                                generated_info(m->model) != nullptr);
 }
 
 int batch_nmax() { return BATCH_NMAX; }

@@ -232,6 +232,7 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   if (!(cfg->essRel >= 0.0 && cfg->essRel <= 1.0))
     return set_error(BIHIP_EINVAL, "essRel must lie in [0,1]");
   if (!(cfg->dt > 0.0)) return set_error(BIHIP_EINVAL, "dt must be positive");
+  if (int rc = check_interval(m, cfg->t0, cfg->t0 + cfg->dt)) return rc;
   if (ntheta < 0 || ntheta > 16 || (ntheta > 0 && theta_host == nullptr))
     return set_error(BIHIP_EINVAL, "ntheta must lie in [0,16] with theta given");
   if (cfg->block != 0 && cfg->block != 256 && cfg->block != 512 && cfg->block != 1024)
@@ -242,7 +243,7 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   b->model = *m;
   b->cfg = *cfg;
   b->ntheta = ntheta;
-  b->nobs = m->model == BIHIP_MODEL_L96 ? m->d / 2 : 1;
+  b->nobs = bihip_model_nobs(m);
   b->d = m->d;
   b->elem = m->precision == BIHIP_F64 ? 8 : 4;
   b->block = cfg->block;

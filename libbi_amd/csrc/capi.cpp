@@ -70,15 +70,21 @@ static int check_model(const bihip_model* m) {
   return BIHIP_OK;
 }
 
-// An interval must take a bounded number of fixed RK4 steps.
-static int check_interval(const bihip_model* m, double t1, double t2) {
+// An interval must take a bounded number of fixed RK4 steps and, for a
+// generated model, a bounded number of its discrete-time steps.
+int check_interval(const bihip_model* m, double t1, double t2) {
   if (!(t1 <= t2)) return set_error(BIHIP_EINVAL, "need t1 <= t2");
   if (m->integrator == BIHIP_RK4 && (t2 - t1) / m->h0 > 0.5 * RK4_MAX_STEPS)
     return set_error(BIHIP_EINVAL, "interval needs more than %d RK4 steps", RK4_MAX_STEPS / 2);
+  const GenInfo* g = generated_info(m->model);
+  if (g != nullptr && (t2 - t1) / g->delta > GEN_MAX_DELTAS)
+    return set_error(BIHIP_EINVAL, "interval spans more than %d multiples of the model's delta",
+                     GEN_MAX_DELTAS);
   return BIHIP_OK;
 }
 
 static int nobs_of(const bihip_model* m) {
+  if (const GenInfo* g = generated_info(m->model)) return g->nobs;
   switch (m->model) {
     case BIHIP_MODEL_L96: return m->d / 2;
     default: return 1;
@@ -363,6 +369,8 @@ int bihip_flow(bihip_ctx* ctx, const bihip_model* m, double t1, double t2, int P
   if ((rc = check_model(m))) return rc;
   if (P < 0 || ld < P) return set_error(BIHIP_EINVAL, "need 0 <= P <= ld");
   if ((rc = check_interval(m, t1, t2))) return rc;
+  if (generated_info(m->model) != nullptr)
+    return set_error(BIHIP_EINVAL, "the deterministic flow is defined for the built-in models");
   if (!(t2 - t1 > 0.0)) return BIHIP_OK;  // DOPRI5Integrator.hpp:51: nothing to do
   TransLaunch L{};
   L.model = m;
@@ -1042,12 +1050,14 @@ int bihip_pf_set_schedule(bihip_pf* pf, const double* times_host, const int* obs
   if (pf->surr || pf->guided)
     return set_error(BIHIP_ESTATE, "detach the learned flow or proposal before setting a schedule");
   const int T = pf->cfg.T;
-  for (int k = 0; k < T; ++k)
+  int rc;
+  for (int k = 0; k < T; ++k) {
     if (!(times_host[k + 1] > times_host[k]))
       return set_error(BIHIP_EINVAL, "schedule times must increase (times[%d] = %g, times[%d] = %g)",
                        k, times_host[k], k + 1, times_host[k + 1]);
-  int rc = ctx_enter(pf->ctx);
-  if (rc) return rc;
+    if ((rc = check_interval(&pf->model, times_host[k], times_host[k + 1]))) return rc;
+  }
+  if ((rc = ctx_enter(pf->ctx))) return rc;
   if (pf->times == nullptr && ((rc = dalloc(&pf->times, (size_t)T + 1)) ||
                                (rc = dalloc(&pf->obs_mask, (size_t)T))))
     return rc;

@@ -12,6 +12,26 @@ namespace bihip {
 // Sets the calling thread's last-error string; returns the status code.
 int set_error(int code, const char* fmt, ...);
 
+// Host-side description of a model generated from a LibBi model file (the
+// name lists end with nullptr); generated_info gives it for a model id, or
+// nullptr for the hand-written models and unknown ids.
+struct GenInfo {
+  const char* name;
+  int d, nobs, integrator;
+  double h, atol, rtol, delta;
+  const char* const* params;
+  const char* const* inputs;
+  const char* const* states;
+  const char* const* obs;
+  const char* const* noises;
+};
+const GenInfo* generated_info(int model);
+// Multiples of a generated model's delta one interval may span (each is a
+// loop iteration of its transition).
+constexpr int GEN_MAX_DELTAS = 1 << 16;
+// EINVAL unless [t1, t2] is an interval the model's kernels take in bounded time.
+int check_interval(const bihip_model* m, double t1, double t2);
+
 #define BIHIP_HIP(call)                                                          \
   do {                                                                           \
     hipError_t e_ = (call);                                                      \

@@ -107,6 +107,11 @@ struct StateVar {
 };
 
 std::vector<StateVar> state_vars(const bihip_model* m) {
+  if (const bihip::GenInfo* g = bihip::generated_info(m->model)) {  // the model file's state variables
+    std::vector<StateVar> v;
+    for (int i = 0; g->states[i] != nullptr; ++i) v.push_back({g->states[i], "", 1});
+    return v;
+  }
   switch (m->model) {
     case BIHIP_MODEL_PZ:  // docs/src/users.tex:36-85: state P, Z
       return {{"P", "", 1}, {"Z", "", 1}};

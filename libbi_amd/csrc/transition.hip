@@ -9,9 +9,12 @@
 //   RK4/DOPRI5IntegratorHost       share/src/bi/host/ode/{RK4,DOPRI5}IntegratorHost.hpp
 //   observationLogDensities        share/src/bi/host/updater/SparseStaticLogDensityHost.hpp:46-68
 //   gather_rows (fused read)       share/src/bi/host/primitive/matrix_primitive.hpp:65-75
+#include <cstring>
+
 #include "model_ops.hpp"
 #include "blockreduce.hpp"
 #include "internal.hpp"
+#include "gen_model.hpp"
 
 namespace bihip {
 
@@ -102,21 +105,26 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
       return M(ma);
     };
     M m = model();
-    if constexpr (M::NOISE_FIRST) {
-      if (!a.flow_only) {
-        noise_first(m, a.seed, a.batch, p, step);
-      } else {
-        m.alpha = a.aux != nullptr ? (T)a.aux[p] : (T)m.mu;
-      }
-    }
     int n;
-    if constexpr (INTEG == BIHIP_RK4) {
-      n = integrate_rk4(m, grp, x, t1, t2, a.m.h0);
+    if constexpr (is_generated<M>::value) {
+      // the model's transition block: discrete actions and its ode, in order
+      n = m.template transition<INTEG>(grp, x, t1, t2, a.m, a.seed, a.batch, (uint32_t)p, step);
     } else {
-      n = integrate_dopri5(m, grp, x, t1, t2, a.m);
-    }
-    if constexpr (!M::NOISE_FIRST) {
-      if (!a.flow_only) noise_after(m, a.seed, a.batch, p, lo, step, x);
+      if constexpr (M::NOISE_FIRST) {
+        if (!a.flow_only) {
+          noise_first(m, a.seed, a.batch, p, step);
+        } else {
+          m.alpha = a.aux != nullptr ? (T)a.aux[p] : (T)m.mu;
+        }
+      }
+      if constexpr (INTEG == BIHIP_RK4) {
+        n = integrate_rk4(m, grp, x, t1, t2, a.m.h0);
+      } else {
+        n = integrate_dopri5(m, grp, x, t1, t2, a.m);
+      }
+      if constexpr (!M::NOISE_FIRST) {
+        if (!a.flow_only) noise_after(m, a.seed, a.batch, p, lo, step, x);
+      }
     }
 
     T* Xout = static_cast<T*>(a.Xout);
@@ -155,6 +163,13 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_init(ModelArgs ma, uint64_t see
   const int lo = grp.g * DL;
   if (p >= P) return;
   T* X = static_cast<T*>(Xv);
+  if constexpr (is_generated<M>::value) {
+    double xd[DL];
+    M(ma).init(xd, seed, batch, (uint32_t)p);
+#pragma unroll
+    for (int i = 0; i < DL; ++i) X[(size_t)i * ld + p] = (T)xd[i];
+    return;
+  }
   T z[DL];
   slice_normals<T, DL>(seed, (uint32_t)p, 0, TAG_INIT, batch, lo, z);
 #pragma unroll
@@ -203,6 +218,15 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_obs_sample(ModelArgs ma, uint64
   if (p >= P) return;
   const T* X = static_cast<const T*>(Xv);
   constexpr int NOBS = M::NOBS;
+  if constexpr (is_generated<M>::value) {
+    T x[M::DL];
+#pragma unroll
+    for (int i = 0; i < M::DL; ++i) x[i] = X[(size_t)i * ld + p];
+    double yo[NOBS > 0 ? NOBS : 1];
+    M(ma).obs_sample(x, seed, batch, (uint32_t)p, step, yo);
+    for (int o = 0; o < NOBS; ++o) y[(size_t)p * NOBS + o] = yo[o];
+    return;
+  }
   for (int b = 0; 4 * b < NOBS; ++b) {
     T z[4];
     normals4<T>(seed, (uint32_t)p, step, TAG_OBS, batch, (uint32_t)b, z);
@@ -251,9 +275,15 @@ __global__ void k_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t
 }
 
 // ---------------------------------------------------------------- dispatch
+const GenInfo* generated_info(int model) {
+  const int i = model - BIHIP_MODEL_GENERATED;
+  return i >= 0 && i < GEN_COUNT ? &GEN_MODELS[i] : nullptr;
+}
+
 bool model_supported(const bihip_model* m) {
   if (m->integrator != BIHIP_RK4 && m->integrator != BIHIP_DOPRI5) return false;
   if (m->precision != BIHIP_F32 && m->precision != BIHIP_F64) return false;
+  if (const GenInfo* g = generated_info(m->model)) return m->d == g->d;
   switch (m->model) {
     case BIHIP_MODEL_PZ: return m->d == 2;
     case BIHIP_MODEL_L96: return m->d == 8 || m->d == 40 || m->d == 200;
@@ -266,6 +296,7 @@ bool model_supported(const bihip_model* m) {
 template <class F>
 static int with_model(const bihip_model* m, F& f) {
   const bool f64 = m->precision == BIHIP_F64;
+  if (m->model >= BIHIP_MODEL_GENERATED) return with_generated(m, f);
   switch (m->model) {
     case BIHIP_MODEL_PZ:
       return f64 ? f.template run<PZ<double>>() : f.template run<PZ<float>>();
@@ -452,3 +483,49 @@ int launch_normals(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag, u
 }
 
 }  // namespace bihip
+
+using namespace bihip;
+
+int bihip_model_generated(const char* name, int precision, bihip_model* m) {
+  if (name == nullptr || m == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (precision != BIHIP_F32 && precision != BIHIP_F64)
+    return set_error(BIHIP_EINVAL, "precision must be BIHIP_F32 or BIHIP_F64");
+  for (int i = 0; i < GEN_COUNT; ++i) {
+    const GenInfo& g = GEN_MODELS[i];
+    if (std::strcmp(g.name, name) != 0) continue;
+    *m = bihip_model{};
+    m->model = BIHIP_MODEL_GENERATED + i;
+    m->d = g.d;
+    m->integrator = g.integrator;
+    m->nsteps = 1000;
+    m->host_quirk = 0;
+    m->precision = precision;
+    m->h0 = g.h;
+    m->atol = g.atol;
+    m->rtol = g.rtol;
+    return BIHIP_OK;
+  }
+  return set_error(BIHIP_EINVAL, "no generated model named '%s'", name);
+}
+
+const char* bihip_model_var_name(const bihip_model* m, int kind, int i) {
+  const GenInfo* g = m ? generated_info(m->model) : nullptr;
+  if (g == nullptr || i < 0) return nullptr;
+  const char* const* names = nullptr;
+  switch (kind) {
+    case BIHIP_VAR_PARAM: names = g->params; break;
+    case BIHIP_VAR_INPUT: names = g->inputs; break;
+    case BIHIP_VAR_STATE: names = g->states; break;
+    case BIHIP_VAR_OBS: names = g->obs; break;
+    case BIHIP_VAR_NOISE: names = g->noises; break;
+    default: return nullptr;
+  }
+  for (int k = 0; k <= i; ++k)
+    if (names[k] == nullptr) return nullptr;
+  return names[i];
+}
+
+const char* bihip_model_name(const bihip_model* m) {
+  const GenInfo* g = m ? generated_info(m->model) : nullptr;
+  return g ? g->name : nullptr;
+}

@@ -1,0 +1,232 @@
+"""The model-language front end (libbi_amd/dsl): parsing as share/bi.lex and
+share/bi.yp define it, the checks of the generator, the generated functors'
+form, and the library's description of the generated models (host calls
+only, no GPU)."""
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from libbi_amd import capi
+from libbi_amd.dsl import codegen, parser
+from libbi_amd.dsl.codegen import GenError
+from libbi_amd.dsl.parser import BiSyntaxError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BI = os.path.join(os.path.dirname(HERE), "libbi_amd", "bi")
+
+
+def _gen(text):
+    return codegen.analyse_text(text)
+
+
+def test_tokens_follow_the_lexer():
+    toks = parser.tokenize("x <- a**2 .* b // c\n/* d */ 1.5e-3 .5 2 'RK4' dx/dt")
+    kinds = [(t.kind, t.text) for t in toks[:-1]]
+    assert kinds == [("ident", "x"), ("op", "<-"), ("ident", "a"), ("op", "**"),
+                     ("number", "2"), ("op", ".*"), ("ident", "b"), ("number", "1.5e-3"),
+                     ("number", ".5"), ("number", "2"), ("string", "'RK4'"), ("ident", "dx"),
+                     ("op", "/"), ("keyword", "dt")]
+    assert toks[7].line == 2
+
+
+def test_precedence_and_associativity():
+    """'**' binds tighter than '*', which binds tighter than '+'; binary
+    operators associate to the left (share/bi.yp:330-420)."""
+    m = parser.parse("model M { state x\n sub initial { x <- 1 - 2 - 3 * 4 ** 2 ** 3 } }")
+    e = m.block("initial").items[0].expr
+    # ((1 - 2) - (3 * ((4 ** 2) ** 3)))
+    assert isinstance(e, parser.Binary) and e.op == "-"
+    assert e.left.op == "-" and e.left.left.value == 1 and e.left.right.value == 2
+    r = e.right
+    assert r.op == "*" and r.left.value == 3
+    assert r.right.op == "**" and r.right.left.op == "**" and r.right.right.value == 3
+    # the analyser's constant evaluation agrees with Python on the same tree
+    assert codegen._const_eval(e, {}, 0) == (1 - 2) - 3 * ((4 ** 2) ** 3)
+
+
+def test_statements_without_semicolons():
+    m = parser.parse("""model M {
+      const a = 2, b = 3
+      param p
+      state x, y; noise w
+      sub transition { w ~ gaussian(0, p)
+        x <- x + w
+        y <- -x }
+    }""")
+    assert m.consts[0][0] == "a" and m.consts[1][0] == "b"
+    assert m.names_of("state") == ["x", "y"]
+    acts = m.block("transition").items
+    assert [a.target for a in acts] == ["w", "x", "y"]
+    assert isinstance(acts[2].expr, parser.Unary)
+
+
+def test_syntax_errors_name_the_line():
+    with pytest.raises(BiSyntaxError, match="line 3"):
+        parser.parse("model M {\n state x\n sub initial { x <- ( }\n}")
+    with pytest.raises(BiSyntaxError):
+        parser.parse("model M { state x } extra")
+    with pytest.raises(BiSyntaxError, match="dx/dt"):
+        parser.parse("model M { state x\n sub transition { ode { x/dt = 1 } } }")
+
+
+def test_pz_model_of_the_user_guide():
+    g = codegen.load_models([os.path.join(BI, "PZ.bi")])[0]
+    assert (g.params, g.states, g.noises, g.obs) == (["mu", "sigma"], ["P", "Z"], ["alpha"],
+                                                       ["P_obs"])
+    assert g.consts == {"c": 0.25, "e": 0.3, "m_l": 0.1, "m_q": 0.1}
+    assert g.integrator == capi.RK4 and g.h == 0.1 and g.delta == 1.0
+    assert [s.target for s in g.pre] == ["alpha"] and not g.post
+    assert set(g.ode) == {"P", "Z"}
+    assert [d.target for d in g.observation] == ["P_obs"] and g.observation[0].log
+
+
+def test_generated_expressions_keep_the_tree():
+    """Every operation is parenthesised as parsed, so the device evaluates the
+    same tree; constants are compiled in, functions of constants folded."""
+    g = codegen.load_models([os.path.join(BI, "PZ.bi")])[0]
+    src = codegen.generate_model(g)
+    assert "k[0] = ((w_alpha * x[0]) - ((T(0.25) * x[0]) * x[1]));" in src
+    assert ("k[1] = (((((T(0.3) * T(0.25)) * x[0]) * x[1]) - (T(0.1) * x[1])) - "
+            "((T(0.1) * x[1]) * x[1]));") in src
+    assert "w_alpha = (T(p_mu) + (T(p_sigma) * zq0[0]));" in src
+    # log(2.0) folded to the correctly rounded double
+    assert f"exp(({float(np.log(2.0))!r} + (0.2 * (double)zq0[0])))" in src
+    assert "lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);" in src
+
+
+def test_draw_counters():
+    """The k-th normal draw of a block uses Philox block k/4, word pair k%4;
+    uniforms use their own blocks."""
+    g = _gen("""model M { param a; state x, y, z, u, v; noise w
+      sub initial { x ~ gaussian() y ~ gaussian() z ~ uniform(0, 2) u ~ gaussian()
+                    v ~ normal() }
+      sub transition { w ~ gaussian() } }""")
+    draws = [s.draw for s in g.initial]
+    assert [(d.dist, d.index) for d in draws] == [("gaussian", 0), ("gaussian", 1),
+                                                  ("uniform", 0), ("gaussian", 2),
+                                                  ("gaussian", 3)]
+    assert g.normals["initial"] == 4 and g.uniforms["initial"] == 1
+    src = codegen.generate_model(g)
+    assert "normals4<T>(seed, p, 0u, TAG_INIT, batch, 0u, zq0);" in src
+    assert "zq1" not in src.split("void init")[1].split("};")[0]
+    assert "xd[4] = (0.0 + (1.0 * (double)zq0[3]));" in src
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("model M { dim n(4)\n state x[n] }", "dimensions"),
+    ("model M { state x\n sub transition { x ~ gamma(1.0) } }", "not supported"),
+    ("model M { state x\n sub transition { ode { dx/dt = -x } } }", "RK4\\(3\\)"),
+    ("model M { state x\n sub transition { ode(alg = 'RK4') { dx/dt = y } } }", "unknown name"),
+    ("model M { state x; obs y\n sub transition { x <- y } }", "cannot be read"),
+    ("model M { state x; obs y\n sub observation { x ~ gaussian() } }", "cannot be the target"),
+    ("model M { param " + ", ".join(f"p{i}" for i in range(17)) + "\n state x }", "at most 16"),
+    ("model M { state x\n sub transition { x <- x\n ode(alg='RK4') { dx/dt = 1 }\n"
+     " ode(alg='RK4') { dx/dt = 2 } } }", "one ode block"),
+    ("model M { state x\n sub transition { x <- f(x) } }", "unknown function"),
+    ("model M { noise w }", "no state"),
+])
+def test_unsupported_models_are_rejected(text, msg):
+    with pytest.raises(GenError, match=msg):
+        _gen(text)
+
+
+def test_registry_models_compile_into_the_library():
+    """Every file in libbi_amd/bi generates; the library knows each model by
+    name with the file's variables, ode settings and observation count."""
+    paths = sorted(glob.glob(os.path.join(BI, "*.bi")))
+    gs = codegen.load_models(paths)
+    for g in gs:
+        m = capi.generated_model(g.name)
+        assert m.model >= capi.MODEL_GENERATED
+        assert m.d == len(g.states)
+        assert capi.nobs(m) == len(g.obs)
+        assert capi.model_vars(m, capi.VAR_PARAM) == g.params
+        assert capi.model_vars(m, capi.VAR_STATE) == g.states
+        assert capi.model_vars(m, capi.VAR_OBS) == g.obs
+        assert capi.model_vars(m, capi.VAR_NOISE) == g.noises
+        assert (m.integrator, m.h0, m.atol, m.rtol) == (g.integrator, g.h, g.atol, g.rtol)
+        assert capi.lib().bihip_model_name(ctypes.byref(m)).decode() == g.name
+    with pytest.raises(capi.BihipError, match="no generated model"):
+        capi.generated_model("NoSuchModel")
+    with pytest.raises(KeyError):
+        capi.generated_model("PZ", nonsense=1.0)
+    # hand-written models are not generated ones
+    assert capi.model_vars(capi.make_model("pz"), capi.VAR_STATE) == []
+
+
+def test_netcdf_output_uses_the_model_files_variables(tmp_path):
+    from scipy.io import netcdf_file
+    m = capi.generated_model("PZ", precision=capi.F64)
+    o = capi.PFOutput(tmp_path / "pz.nc", m, 2, 3)
+    for k in range(2):
+        o.write_time(k, float(k))
+        o.write_state(k, np.full((2, 3), float(k)), np.arange(3))
+    o.close()
+    f = netcdf_file(str(tmp_path / "pz.nc"), "r", mmap=False)
+    assert set(f.variables) >= {"P", "Z", "time", "ancestor"}
+    f.close()
+
+
+def test_generated_header_is_standalone_cxx(tmp_path):
+    """The generated text is what build() compiles: it parses as C++ with
+    minimal declarations of what the kernels provide (host syntax check)."""
+    gs = codegen.load_models(sorted(glob.glob(os.path.join(BI, "*.bi"))))
+    stub = r"""
+#include <cmath>
+#include <cstdint>
+#define __device__
+#define __forceinline__ inline
+#define BIHIP_RK4 0
+#define BIHIP_F64 1
+#define BIHIP_EINVAL 2
+#define BIHIP_MODEL_GENERATED 64
+using std::exp; using std::log; using std::sqrt; using std::floor; using std::fmin;
+using std::pow; using std::fabs;
+namespace bihip {
+struct ModelArgs { double h0; double par[16]; };
+struct bihip_model_ { int model, precision; };
+struct u32x4 { uint32_t x, y, z, w; };
+struct GenInfo { const char* name; int d, nobs, integrator; double h, atol, rtol, delta;
+  const char* const* params; const char* const* inputs; const char* const* states;
+  const char* const* obs; const char* const* noises; };
+constexpr double HALF_LOG_TWO_PI = 0.9189385332046727;
+constexpr uint32_t TAG_INIT = 1, TAG_NOISE = 2, TAG_OBS = 6;
+inline int set_error(int c, const char*) { return c; }
+inline double gen_scaled(double t, double d) { return t / d; }
+template <class T> void normals4(uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                 T (&z)[4]) { z[0] = z[1] = z[2] = z[3] = T(0); }
+inline u32x4 philox_block(uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {
+  return {0, 0, 0, 0}; }
+template <class T> T uniform_word(const u32x4&) { return T(0); }
+template <class M, class G, class T> int integrate_rk4(const M&, const G&, T (&)[M::DL],
+                                                       double, double, double) { return 0; }
+template <class M, class G, class T> int integrate_dopri5(const M&, const G&, T (&)[M::DL],
+                                                          double, double, const ModelArgs&) {
+  return 0; }
+}
+using bihip_model = bihip::bihip_model_;
+"""
+    src = tmp_path / "gen.cpp"
+    src.write_text(stub + codegen.generate(gs) + r"""
+namespace bihip {
+struct Runner { template <class M> int run() {
+  M m(ModelArgs{});
+  double x[M::DL] = {}; typename M::real_t xt[M::DL] = {}, k[M::DL];
+  int g = 0;
+  m.rhs(g, xt, k);
+  m.template transition<0>(g, xt, 0.0, 1.0, ModelArgs{}, 1, 0, 0, 1);
+  double y[M::NOBS > 0 ? M::NOBS : 1] = {};
+  (void)m.obs(xt, y);
+  m.obs_sample(xt, 1, 0, 0, 1, y);
+  m.init(x, 1, 0, 0);
+  return 0; } };
+int check(const bihip_model* m) { Runner r; return with_generated(m, r); }
+}
+""")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

@@ -1,0 +1,212 @@
+"""Models generated from LibBi model files (libbi_amd/bi/*.bi, compiled into
+the library by libbi_amd.dsl) run through the same filter kernels as the
+hand-written models.
+
+Pinned three ways: the PZ model of the LibBi user guide
+(docs/src/users.tex:52-84) and the linear-Gaussian model, written in the
+model language, against the oracle's filter with the same equations and
+constants (FP64: log-likelihood to 1e-10, ancestors bit-exact); the same
+files against the hand-written device models (FP32); and the discrete-time
+AR1 model, which no hand-written model covers, against its exact Kalman
+likelihood, also with observations between its delta times."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from libbi_amd import capi
+from gpu_helpers import empty, host, oracle_model
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, m, y, P, dt, seed, resampler=capi.SYSTEMATIC, ess_rel=0.5, record=True,
+         batch=0):
+    T = len(y)
+    pf = capi.ParticleFilter(ctx, m, P, T, dt, seed, resampler=resampler, ess_rel=ess_rel,
+                             record=record, batch=batch)
+    pf.set_observations(y)
+    pf.init()
+    pf.step(T)
+    z = pf.term()
+    out = pf.outputs()
+    anc = np.array([pf.ancestors(k) for k in range(T)]) if record else None
+    pf.close()
+    return z, out, anc
+
+
+def _pz_pair(prec, integrator=None, mu=0.5, sigma=0.2):
+    """The generated PZ model and the hand-written one with the file's constants
+    (and the same integrator settings: the file's ode block gives h = 0.1 and
+    leaves atoler and rtoler at their defaults of 1e-3)."""
+    h = capi.make_model("pz", precision=prec, integrator=integrator, mu=mu, sigma=sigma,
+                        c=0.25, e=0.3, m_l=0.1, m_q=0.1, obs_sd=0.2,
+                        P0_logmean=np.log(2.0), P0_logsd=0.2, Z0_logmean=np.log(2.0),
+                        Z0_logsd=0.1)
+    g = capi.generated_model("PZ", precision=prec, integrator=integrator, mu=mu, sigma=sigma,
+                             h0=h.h0, atol=h.atol, rtol=h.rtol, nsteps=h.nsteps)
+    return g, h
+
+
+def _pz_data(orc, T, seed=100):
+    m = orc.make_model("pz")
+    x0 = orc.init_samples(m, seed, 1)[:, 0]
+    _, y = orc.simulate(m, seed, T, 1.0, x0)
+    return y
+
+
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.STRATIFIED, capi.MULTINOMIAL])
+def test_generated_pz_matches_oracle(ctx, orc, resampler):
+    """PZ.bi, FP64, RK4 h = 0.1, N = 4096, T = 60, resampling at ESS < 0.7 N."""
+    y = _pz_data(orc, 60)
+    g, h = _pz_pair(capi.F64)
+    om = oracle_model(orc, h)
+    P = 4096
+    z, out, anc = _run(ctx, g, y, P, 1.0, 42, resampler, 0.7)
+    ref = orc.pf_run(om, y, P, 1.0, 42, resampler=resampler, ess_rel=0.7, want_ancestors=True)
+    assert z == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+def test_generated_pz_dopri5_matches_oracle(ctx, orc):
+    """The ode block's algorithm replaced at run time by RK5(4) (DOPRI5)."""
+    y = _pz_data(orc, 30)
+    g, h = _pz_pair(capi.F64, integrator=capi.DOPRI5)
+    om = oracle_model(orc, h)
+    z, out, anc = _run(ctx, g, y, 2048, 1.0, 7, ess_rel=0.5)
+    ref = orc.pf_run(om, y, 2048, 1.0, 7, ess_rel=0.5, want_ancestors=True)
+    assert z == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+@pytest.mark.parametrize("integrator", [capi.RK4, capi.DOPRI5])
+def test_generated_pz_fp32_equals_handwritten(ctx, orc, integrator):
+    """FP32 state: the generated functor against the hand-written PZ kernel, no
+    resampling (the particles stay paired), logZ to the FP32 bar of 1e-5."""
+    y = _pz_data(orc, 20)
+    g, h = _pz_pair(capi.F32, integrator=integrator)
+    zg, og, _ = _run(ctx, g, y, 4096, 1.0, 3, ess_rel=0.0, record=False)
+    zh, oh, _ = _run(ctx, h, y, 4096, 1.0, 3, ess_rel=0.0, record=False)
+    assert not og["resampled"].any()
+    assert zg == pytest.approx(zh, rel=1e-5)
+    np.testing.assert_allclose(og["logIncrements"], oh["logIncrements"], rtol=1e-4, atol=1e-4)
+
+
+def test_generated_init_and_obs_samples_equal_handwritten(ctx, orc, torch_gpu):
+    """bihip_init_samples and bihip_obs_sample of PZ.bi equal the hand-written model's."""
+    g, h = _pz_pair(capi.F64)
+    P = 3000
+    L = capi.lib()
+    out = []
+    for m in (g, h):
+        X = empty((2, P), torch_gpu.float64)
+        capi.check(L.bihip_init_samples(ctx.h, ctypes.byref(m), 21, 3, P,
+                                        ctypes.c_void_p(X.data_ptr()), P, None))
+        Y = empty((P, 1), torch_gpu.float64)
+        capi.check(L.bihip_obs_sample(ctx.h, ctypes.byref(m), 21, 3, 5, P,
+                                      ctypes.c_void_p(X.data_ptr()), P,
+                                      ctypes.c_void_p(Y.data_ptr())))
+        out.append((host(X), host(Y)))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-15, atol=0)
+
+
+def test_generated_lg_matches_oracle_and_kalman(ctx, orc):
+    """LinearGaussian.bi (DOPRI5, then additive noise) against the oracle's LG
+    filter, and its likelihood against the Kalman filter's."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, atol=1e-10, rtol=1e-10,
+                             h0=0.25, **par)
+    h = capi.make_model("lg", precision=capi.F64, atol=1e-10, rtol=1e-10, h0=0.25, **par)
+    om = oracle_model(orc, h)
+    x0 = orc.init_samples(om, 11, 1)[:, 0]
+    _, y = orc.simulate(om, 11, 40, 1.0, x0)
+    z, out, anc = _run(ctx, g, y, 4096, 1.0, 1, ess_rel=0.5)
+    ref = orc.pf_run(om, y, 4096, 1.0, 1, ess_rel=0.5, want_ancestors=True)
+    assert z == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+    exact = orc.kalman_loglik_lg(om, 1.0, y)
+    zs = [_run(ctx, g, y, 1 << 16, 1.0, s, record=False)[0] for s in (1, 2)]
+    assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
+
+
+def _ar1_kalman(y, theta, sigma2, obs_sd, fires):
+    """Exact log-likelihood of x <- theta x + w (w ~ N(0, sigma2)) applied at
+    the intervals flagged in fires, y_k ~ N(x, obs_sd^2) after interval k,
+    x_0 ~ N(0, 1)."""
+    m, v, ll = 0.0, 1.0, 0.0
+    for k, yk in enumerate(y):
+        if fires[k]:
+            m, v = theta * m, theta * theta * v + sigma2
+        s = v + obs_sd * obs_sd
+        ll += -0.5 * np.log(2 * np.pi * s) - 0.5 * (yk - m) ** 2 / s
+        gain = v / s
+        m, v = m + gain * (yk - m), (1 - gain) * v
+    return ll
+
+
+def _ar1_data(T, theta, sigma2, obs_sd, fires, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal()
+    y = np.empty(T)
+    for k in range(T):
+        if fires[k]:
+            x = theta * x + np.sqrt(sigma2) * rng.normal()
+        y[k] = x + obs_sd * rng.normal()
+    return y
+
+
+@pytest.mark.parametrize("dt", [1.0, 0.5])
+def test_generated_ar1_matches_kalman(ctx, dt):
+    """AR1.bi: a discrete-time model (no ode block) with delta = 1.  At dt = 0.5
+    its actions fire at every other interval only (times that are multiples of
+    delta, share/src/bi/state/Schedule.hpp:230-292); the Kalman filter then
+    predicts with the identity between them."""
+    theta, sigma2, obs_sd = 0.8, 0.5, 0.5
+    T = 40 if dt == 1.0 else 80
+    fires = [((k * dt) % 1.0) == 0.0 for k in range(T)]
+    y = _ar1_data(T, theta, sigma2, obs_sd, fires, seed=5)
+    exact = _ar1_kalman(y, theta, sigma2, obs_sd, fires)
+    m = capi.generated_model("AR1", precision=capi.F64, theta=theta, sigma2=sigma2,
+                             obs_sd=obs_sd)
+    zs = [_run(ctx, m, y, 1 << 16, dt, s, record=False)[0] for s in (1, 2)]
+    assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
+
+
+def test_generated_batch_equals_single_filter(ctx, orc):
+    """SMC^2 inner filters on PZ.bi: filter j of a batch (theta_j = (mu, sigma))
+    equals bihip_pf run alone with batch id j, and the oracle's filter."""
+    T = 15
+    y = _pz_data(orc, T)
+    g, _ = _pz_pair(capi.F64)
+    Ns = [300, 2048, 7000, 1]
+    rng = np.random.default_rng(4)
+    theta = np.stack([np.exp(rng.normal(np.log(0.5), 0.2, len(Ns))),
+                      np.exp(rng.normal(np.log(0.2), 0.2, len(Ns)))], axis=1)
+    bf = capi.BatchFilter(ctx, g, Ns, theta, T, 1.0, seed=5, ess_rel=0.6, record=True)
+    bf.set_observations(y)
+    bf.init()
+    bf.step(T)
+    out = bf.outputs()
+    assert (out["status"] == 0).all()
+    for j, N in enumerate(Ns):
+        m, h = _pz_pair(capi.F64, mu=theta[j, 0], sigma=theta[j, 1])
+        z, o, _ = _run(ctx, m, y, N, 1.0, 5, ess_rel=0.6, record=False, batch=j)
+        np.testing.assert_allclose(out["logIncrements"][j], o["logIncrements"], rtol=1e-10,
+                                   atol=1e-10)
+        assert out["logLikelihood"][j] == pytest.approx(z, rel=1e-10)
+        ref = orc.pf_run(oracle_model(orc, h), y, N, 1.0, 5, ess_rel=0.6, batch=j,
+                         want_ancestors=True)
+        for k in range(T):
+            np.testing.assert_array_equal(bf.ancestors(j, k), ref["ancestors"][k], f"{j} {k}")
+    bf.close()
+
+
+def test_generated_model_rejects_flow(ctx, torch_gpu):
+    g, _ = _pz_pair(capi.F64)
+    X = empty((2, 4), torch_gpu.float64)
+    rc = capi.lib().bihip_flow(ctx.h, ctypes.byref(g), 0.0, 1.0, 4,
+                               ctypes.c_void_p(X.data_ptr()), 4, None, None)
+    assert rc == capi.EINVAL
