@@ -22,6 +22,8 @@ namespace bihip {
 // Largest inner filter: the permutation's search table E (values below N)
 // is kept in LDS as 16-bit integers.
 constexpr int BATCH_NMAX = 16384;
+// Consecutive particles per thread in the resampling passes.
+constexpr int BITEMS = 4;
 
 struct BatchArgs {
   ModelArgs m;
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
     if (tid == 0)
       s_job = (int)__hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    const int job = s_job;
+    const int job = __builtin_amdgcn_readfirstlane(s_job);  // uniform: filter data in SGPRs
     if (job >= a.B) break;
     const int j = a.order[job];
     const BatchFilter F = a.f[j];
@@ -152,72 +154,105 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
     __syncthreads();
     bool res = s_res != 0;
 
-    // ---- Resampler::resample: exact weight scan, offspring, permuted ancestors
+    // ---- Resampler::resample: exact weight scan, offspring, permuted ancestors.
+    // Each thread takes BITEMS consecutive particles of a tile of BLOCK * BITEMS:
+    // (1) the exact fixed-point total W; (2) per tile, the inclusive prefix of
+    // every particle (thread-local sums plus one block scan), its cumulative
+    // offspring and the one of its predecessor (P_{i-1} = P_i - w_i, exact),
+    // the exclusive (e, f) sums (one more block scan), E into LDS and the free
+    // slots' ranks into as[]; (3) the free slots' ancestors by searches of E
+    // that run in lockstep (a uniform number of halvings) with BITEMS in flight.
     if (res) {
-      u128 carry(0);
-      for (int q = 0; q < N; q += BLOCK) {
-        const int i = q + tid;
-        u128 v(0);
-        if (i < N) {
-          const double w = nanexp(a.lws[off + i] - F.mx);
-          v = fix_weight(w);
-        }
-        u128 rowtot;
-        const u128 incl = block_incl_scan_u128<BLOCK>(v, l128, rowtot);
-        if (i < N) a.Ws[off + i] = fix_to_double(carry + incl);
-        carry = carry + rowtot;
+      u128 part(0);
+      for (int base = tid * BITEMS; base < N; base += BLOCK * BITEMS) {
+#pragma unroll
+        for (int u = 0; u < BITEMS; ++u)
+          if (base + u < N) part = part + fix_weight(nanexp(a.lws[off + base + u] - F.mx));
       }
-      const double W = fix_to_double(carry);
+      u128 Wfix;
+      (void)block_incl_scan_u128<BLOCK>(part, l128, Wfix);
+      const double W = fix_to_double(Wfix);
       if (!(W > 0.0)) {
         if (tid == 0) a.f[j].status = BIHIP_DEGENERATE;
         res = false;
       } else {
-        __syncthreads();  // Ws of the whole filter written
         double av = 0.0;
         if (a.resampler == BIHIP_SYSTEMATIC) {
           const u32x4 r = philox_block(a.seed, 0, (uint32_t)k, TAG_SYSTEMATIC, batch, 0);
           av = u53(r.x, r.y);
         }
+        auto offspring = [&](const u128& P) {  // cumulative offspring at prefix P
+          const double Ws = fix_to_double(P);
+          return a.resampler == BIHIP_SYSTEMATIC
+                     ? sys_offspring(Ws, W, N, av)
+                     : strat_offspring(Ws, W, N, a.seed, (uint32_t)k, batch);
+        };
+        u128 carry(0);
         uint64_t efcarry = 0;
-        for (int q = 0; q < N; q += BLOCK) {
-          const int i = q + tid;
-          int o = 0;
-          if (i < N) {
-            int O1, O0 = 0;
-            if (a.resampler == BIHIP_SYSTEMATIC) {
-              O1 = sys_offspring(a.Ws[off + i], W, N, av);
-              if (i > 0) O0 = sys_offspring(a.Ws[off + i - 1], W, N, av);
-            } else {
-              O1 = strat_offspring(a.Ws[off + i], W, N, a.seed, (uint32_t)k, batch);
-              if (i > 0) O0 = strat_offspring(a.Ws[off + i - 1], W, N, a.seed, (uint32_t)k, batch);
+        for (int t0 = 0; t0 < N; t0 += BLOCK * BITEMS) {
+          const int base = t0 + tid * BITEMS;
+          u128 w[BITEMS];
+          u128 loc(0);
+#pragma unroll
+          for (int u = 0; u < BITEMS; ++u) {
+            w[u] = base + u < N ? fix_weight(nanexp(a.lws[off + base + u] - F.mx)) : u128(0);
+            loc = loc + w[u];
+          }
+          u128 tot;
+          const u128 incl = block_incl_scan_u128<BLOCK>(loc, l128, tot);
+          u128 P = carry + (incl - loc);  // prefix before this thread's first particle
+          int O0 = base > 0 && base < N ? offspring(P) : 0;
+          uint64_t ef[BITEMS];
+          int oo[BITEMS];
+          uint64_t eloc = 0;
+#pragma unroll
+          for (int u = 0; u < BITEMS; ++u) {
+            P = P + w[u];
+            const int O1 = base + u < N ? offspring(P) : O0;
+            const int o = O1 - O0;
+            O0 = O1;
+            oo[u] = o;
+            const uint64_t e = o > 1 ? (uint64_t)(o - 1) : 0u;
+            const uint64_t fz = (base + u < N && o == 0) ? 1u : 0u;
+            ef[u] = eloc;  // exclusive within the thread
+            eloc += e | (fz << 32);
+          }
+          uint64_t eftot;
+          const uint64_t ex0 = efcarry + block_excl_scan_u64<BLOCK>(eloc, l64, eftot);
+#pragma unroll
+          for (int u = 0; u < BITEMS; ++u) {
+            const int i = base + u;
+            if (i < N) {
+              const uint64_t ex = ex0 + ef[u];
+              E[i] = (uint16_t)(uint32_t)ex;
+              a.as[off + i] = oo[u] > 0 ? i : -1 - (int)(uint32_t)(ex >> 32);
             }
-            o = O1 - O0;
           }
-          const uint64_t e = o > 1 ? (uint64_t)(o - 1) : 0u;
-          const uint64_t fz = (i < N && o == 0) ? 1u : 0u;
-          uint64_t rowtot;
-          const uint64_t ex = efcarry + block_excl_scan_u64<BLOCK>(e | (fz << 32), l64, rowtot);
-          if (i < N) {
-            E[i] = (uint16_t)(uint32_t)ex;
-            a.as[off + i] = o > 0 ? i : -1 - (int)(uint32_t)(ex >> 32);
-          }
-          efcarry += rowtot;
+          carry = carry + tot;
+          efcarry += eftot;
         }
         __syncthreads();  // E complete
-        for (int q = 0; q < N; q += BLOCK) {
-          const int i = q + tid;
-          if (i >= N) continue;
-          const int v = a.as[off + i];
-          if (v >= 0) continue;
-          const uint32_t r = (uint32_t)(-1 - v);
-          int lo = 0, hi = N;  // first i' with E[i'] > r
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if ((uint32_t)E[mid] <= r) lo = mid + 1;
-            else hi = mid;
+        // free slot of rank r <- the last i with E_i <= r (E is non-decreasing)
+        const int top = 1 << (31 - __clz(N));  // largest power of two <= N
+        for (int base = tid * BITEMS; base < N; base += BLOCK * BITEMS) {
+          int v[BITEMS], pos[BITEMS];
+#pragma unroll
+          for (int u = 0; u < BITEMS; ++u) {
+            v[u] = base + u < N ? a.as[off + base + u] : 0;
+            pos[u] = -1;
           }
-          a.as[off + i] = lo > 0 ? lo - 1 : 0;
+          for (int s = top; s > 0; s >>= 1) {
+#pragma unroll
+            for (int u = 0; u < BITEMS; ++u) {
+              const int c = pos[u] + s;
+              if (v[u] < 0 && c < N && (uint32_t)E[c] <= (uint32_t)(-1 - v[u])) pos[u] = c;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < BITEMS; ++u)
+            if (base + u < N && v[u] < 0) a.as[off + base + u] = pos[u] > 0 ? pos[u] : 0;
         }
+        __syncthreads();  // ancestors written by other threads than the ones that gather
       }
     }
 
@@ -225,7 +260,12 @@ __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
     M m = make_model<M>(a, j);
     const double t1 = a.t0 + k * a.dt, t2 = a.t0 + (k + 1) * a.dt;
     const uint32_t step = (uint32_t)(k + 1);
-    const double* y = a.y + (size_t)k * a.nobs;
+    // the step's observation in registers: the model's terms that depend on it
+    // alone (the log of a log-normal observation) leave the particle loop
+    constexpr int NY = M::NOBS > 0 ? M::NOBS : 1;
+    double y[NY];
+#pragma unroll
+    for (int o = 0; o < NY; ++o) y[o] = o < M::NOBS ? a.y[(size_t)k * a.nobs + o] : 0.0;
     const T* Xin = static_cast<const T*>(a.Xin);
     T* Xout = static_cast<T*>(a.Xout);
     double tm = -__builtin_inf(), ts1 = 0.0, ts2 = 0.0;  // running (max, sum w, sum w^2)

@@ -33,6 +33,9 @@ struct u128 {
     const uint64_t l = lo + o.lo;
     return u128(l, hi + o.hi + (l < lo ? 1u : 0u));
   }
+  __device__ __forceinline__ u128 operator-(const u128& o) const {
+    return u128(lo - o.lo, hi - o.hi - (lo < o.lo ? 1u : 0u));
+  }
 };
 
 // A value as 32-bit chunks.
