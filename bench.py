@@ -253,7 +253,7 @@ def bench_smc2(args, generated=False):
     bt.close()
     res = out["resampled"][:, args.warmup:T]
     per_fixed = 2 * 2 * 8 + 2 * 8   # state read + write, log-weight read + write
-    per_res = 2 * 8 + 4             # weight scan write + read, ancestor write/read
+    per_res = 2 * 4                 # ancestor index written by the resampler, read by the gather
     bytes_total = per_fixed * total * args.steps + per_res * float((res * N[:, None]).sum())
     achieved = bytes_total / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
     # the same filters one after another through the single-filter path (a sample)

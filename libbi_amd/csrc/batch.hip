@@ -36,7 +36,6 @@ struct BatchArgs {
   void* Xout;
   long ld;              // column stride of the flattened state
   double* lws;
-  double* Ws;           // scratch: cumulative weights
   int* as;              // scratch: ancestors
   int* anc_rec;         // nullable: ancestors of step k at anc_rec[k * ld + off + i]
   double* hist_li;      // B x T
@@ -495,7 +494,7 @@ template <int BLOCK>
 static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
   BatchRunner<BLOCK> r;
   r.a = BatchArgs{to_args(L.model), L.theta, L.ntheta, L.order, L.B, L.f, L.Xin, L.Xout, L.ld,
-                  L.lws, L.Ws, L.as, L.anc_rec, L.hist_li, L.hist_ess, L.hist_res, L.y, L.nobs,
+                  L.lws, L.as, L.anc_rec, L.hist_li, L.hist_ess, L.hist_res, L.y, L.nobs,
                   L.T, L.t0, L.dt, L.seed, L.essRel, L.resampler, L.counter};
   r.integ = L.model->integrator;
   r.s = s;

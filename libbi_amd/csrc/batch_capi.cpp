@@ -94,7 +94,6 @@ static BatchLaunch batch_launch(bihip_batch* b) {
   L.Xout = b->X[1 - b->cur];
   L.ld = b->ld;
   L.lws = b->lws;
-  L.Ws = b->Ws;
   L.as = b->as;
   L.anc_rec = b->anc;
   L.hist_li = b->hist_li;

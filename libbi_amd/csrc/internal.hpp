@@ -260,7 +260,6 @@ struct BatchLaunch {
   void* Xout;
   long ld;
   double* lws;
-  double* Ws;
   int* as;
   int* anc_rec;
   double* hist_li;
