@@ -113,6 +113,10 @@ int bihip_model_generated(const char* name, int precision, bihip_model* m);
 /* Name of variable i of a kind (BIHIP_VAR_*) of a generated model; NULL past
  * the end or for a hand-written model. */
 const char* bihip_model_var_name(const bihip_model* m, int kind, int i);
+/* Its number of elements (the size of its dimension, 1 for a scalar; 0 past
+ * the end).  Vector variables take consecutive par slots, state rows and
+ * observation values. */
+int bihip_model_var_size(const bihip_model* m, int kind, int i);
 /* The model's name for a generated model, NULL otherwise. */
 const char* bihip_model_name(const bihip_model* m);
 

@@ -64,7 +64,7 @@ EXPORTS = [
     "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
-    "bihip_model_generated", "bihip_model_var_name", "bihip_model_name",
+    "bihip_model_generated", "bihip_model_var_name", "bihip_model_name", "bihip_model_var_size",
 ]
 
 
@@ -266,6 +266,7 @@ def _declare(L):
     L.bihip_model_var_name.restype = c.c_char_p
     L.bihip_model_name.argtypes = [P]
     L.bihip_model_name.restype = c.c_char_p
+    L.bihip_model_var_size.argtypes = [P, c.c_int, c.c_int]
     L.bihip_uniform_host.argtypes = [u64, u32, u32, u32, u32]
     L.bihip_uniform_host.restype = c.c_double
     L.bihip_batch_particles.argtypes = [P]
@@ -341,11 +342,21 @@ def model_vars(m, kind):
         out.append(n.decode())
 
 
+def model_var_sizes(m, kind):
+    """Element counts of the variables of one kind of a generated model."""
+    return [lib().bihip_model_var_size(ctypes.byref(m), kind, i)
+            for i in range(len(model_vars(m, kind)))]
+
+
 def par_index(m):
-    """Parameter name -> slot of bihip_model::par for a model description."""
+    """Parameter name -> (first) slot of bihip_model::par for a model description."""
     if m.model >= MODEL_GENERATED:
-        names = model_vars(m, VAR_PARAM) + model_vars(m, VAR_INPUT)
-        return {n: i for i, n in enumerate(names)}
+        out, slot = {}, 0
+        for kind in (VAR_PARAM, VAR_INPUT):
+            for n, k in zip(model_vars(m, kind), model_var_sizes(m, kind)):
+                out[n] = slot
+                slot += k
+        return out
     return PAR_INDEX[m.model]
 
 
@@ -358,10 +369,16 @@ def generated_model(name, precision=F64, integrator=None, h0=None, atol=None, rt
     m = Model()
     check(lib().bihip_model_generated(os.fsencode(name), precision, ctypes.byref(m)))
     idx = par_index(m)
+    sizes = dict(zip(model_vars(m, VAR_PARAM) + model_vars(m, VAR_INPUT),
+                     model_var_sizes(m, VAR_PARAM) + model_var_sizes(m, VAR_INPUT)))
     for k, v in par.items():
         if k not in idx:
             raise KeyError(f"{name} has no param or input {k!r} (has {sorted(idx)})")
-        m.par[idx[k]] = v
+        vals = np.atleast_1d(np.asarray(v, np.float64))
+        if vals.size != sizes[k]:
+            raise ValueError(f"{k} has {sizes[k]} elements, {vals.size} given")
+        for j, x in enumerate(vals):
+            m.par[idx[k] + j] = x
     if integrator is not None:
         m.integrator = integrator
     if h0 is not None:

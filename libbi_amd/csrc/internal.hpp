@@ -19,11 +19,9 @@ struct GenInfo {
   const char* name;
   int d, nobs, integrator;
   double h, atol, rtol, delta;
-  const char* const* params;
-  const char* const* inputs;
-  const char* const* states;
-  const char* const* obs;
-  const char* const* noises;
+  const char* const* names[5];  // by BIHIP_VAR_* kind
+  const int* sizes[5];          // elements of each variable
+  const char* const* state_dims;  // dimension of each state variable ("" for a scalar)
 };
 const GenInfo* generated_info(int model);
 // Multiples of a generated model's delta one interval may span (each is a

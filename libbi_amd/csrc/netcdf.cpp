@@ -109,7 +109,8 @@ struct StateVar {
 std::vector<StateVar> state_vars(const bihip_model* m) {
   if (const bihip::GenInfo* g = bihip::generated_info(m->model)) {  // the model file's state variables
     std::vector<StateVar> v;
-    for (int i = 0; g->states[i] != nullptr; ++i) v.push_back({g->states[i], "", 1});
+    for (int i = 0; g->names[BIHIP_VAR_STATE][i] != nullptr; ++i)
+      v.push_back({g->names[BIHIP_VAR_STATE][i], g->state_dims[i], g->sizes[BIHIP_VAR_STATE][i]});
     return v;
   }
   switch (m->model) {
@@ -141,8 +142,12 @@ int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
 
   // dimensions: nr, the model's, np
   o->dims.push_back({"nr", (uint64_t)nr});
-  for (const StateVar& s : sv)
-    if (s.dim[0] != '\0') o->dims.push_back({s.dim, (uint64_t)s.width});
+  for (const StateVar& s : sv) {  // each model dimension once (variables may share one)
+    if (s.dim[0] == '\0') continue;
+    bool seen = false;
+    for (const auto& d : o->dims) seen = seen || d.first == s.dim;
+    if (!seen) o->dims.push_back({s.dim, (uint64_t)s.width});
+  }
   o->dims.push_back({"np", (uint64_t)np});
   const int nrDim = 0, npDim = (int)o->dims.size() - 1;
   auto dim_id = [&](const char* n) {

@@ -510,19 +510,15 @@ int bihip_model_generated(const char* name, int precision, bihip_model* m) {
 
 const char* bihip_model_var_name(const bihip_model* m, int kind, int i) {
   const GenInfo* g = m ? generated_info(m->model) : nullptr;
-  if (g == nullptr || i < 0) return nullptr;
-  const char* const* names = nullptr;
-  switch (kind) {
-    case BIHIP_VAR_PARAM: names = g->params; break;
-    case BIHIP_VAR_INPUT: names = g->inputs; break;
-    case BIHIP_VAR_STATE: names = g->states; break;
-    case BIHIP_VAR_OBS: names = g->obs; break;
-    case BIHIP_VAR_NOISE: names = g->noises; break;
-    default: return nullptr;
-  }
+  if (g == nullptr || i < 0 || kind < 0 || kind > 4) return nullptr;
+  const char* const* names = g->names[kind];
   for (int k = 0; k <= i; ++k)
     if (names[k] == nullptr) return nullptr;
   return names[i];
+}
+
+int bihip_model_var_size(const bihip_model* m, int kind, int i) {
+  return bihip_model_var_name(m, kind, i) != nullptr ? generated_info(m->model)->sizes[kind][i] : 0;
 }
 
 const char* bihip_model_name(const bihip_model* m) {

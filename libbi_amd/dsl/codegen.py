@@ -7,12 +7,18 @@ the program that runs the filter.  Here a model becomes one device functor
 with the interface of the hand-written models (libbi_amd/csrc/models.hpp):
 `rhs` for the integrators, and `transition`, `obs`, `obs_sample` and `init`
 for the filter kernels, which are instantiated on it when the library is
-built.
+built.  One lane holds one particle's whole state.
 
 What a generated model supports (checked here, with an error that names the
 line otherwise):
-  * scalar variables (no `dim`): `param` and `input` (the model's parameter
-    slots, in that order), `state`, `noise`, `obs`; `const` and `inline`;
+  * `dim` declarations (`dim n(size = 8, boundary = 'cyclic')`; boundaries
+    'none', 'cyclic' and 'extended') and variables over at most one
+    dimension; `param` and `input` (the model's parameter slots, in that
+    order), `state`, `noise`, `obs`; `const` and `inline`;
+  * actions on whole vectors (`x <- x + w`, element by element) or on
+    elements named by a dimension alias (`dx[i]/dt = x[i-1]*(x[i+1] - x[i-2])`),
+    with constant index arithmetic on the aliases; a vector action reads the
+    values from before it (its elements are computed, then stored);
   * `sub initial`: `~` and `<-` actions on state variables;
   * `sub transition(delta)`: `~` and `<-` actions and at most one
     `ode(alg, h, atoler, rtoler)` block of `dx/dt = ...` equations, run in the
@@ -25,10 +31,11 @@ line otherwise):
 Blocks the bootstrap filter does not run (`parameter`, `proposal_*`,
 `lookahead_*`, `bridge`) are parsed and skipped.
 
-Random draws are keyed Philox counters: the k-th normal draw of a block comes
-from block k/4, word pair k%4 of the (particle, step, tag, batch) stream, as
-the hand-written models draw theirs, so a model written here reproduces the
-built-in model with the same equations bit for bit.
+Random draws are keyed Philox counters: the k-th normal draw of a block
+(counting vector elements) comes from block k/4, word pair k%4 of the
+(particle, step, tag, batch) stream, as the hand-written models draw theirs,
+so a model written here reproduces the built-in model with the same
+equations bit for bit.
 """
 import math
 from dataclasses import dataclass, field
@@ -38,10 +45,12 @@ from .parser import (Action, Binary, Block, BiSyntaxError, Call, Model, Num, Ref
                      Unary, parse, parse_file)
 
 MAX_SLOTS = 16  # bihip_model::par
+MAX_STATE = 64  # state elements of one particle (one lane's registers)
 NORMAL_DISTS = {"gaussian": False, "normal": False, "log_gaussian": True, "log_normal": True}
 DIST_ARGS = {"gaussian": ("mean", "std"), "uniform": ("lower", "upper")}
 DIST_DEFAULTS = {"mean": 0.0, "std": 1.0, "lower": 0.0, "upper": 1.0}
 UNIFORM_BLOCK = 0x40000000  # Philox block offset of uniform draws
+BOUNDARIES = ("none", "cyclic", "extended")
 # functions of the model language and their C++ names (overloaded for float and double)
 FUNCS = {"exp": "exp", "log": "log", "sqrt": "sqrt", "pow": "pow", "abs": "fabs", "sin": "sin",
          "cos": "cos", "tan": "tan", "asin": "asin", "acos": "acos", "atan": "atan",
@@ -57,6 +66,7 @@ PY_FUNCS = {"exp": math.exp, "log": math.log, "sqrt": math.sqrt, "pow": math.pow
             "lgamma": math.lgamma, "tgamma": math.gamma, "log10": math.log10,
             "exp10": lambda v: 10.0 ** v, "cbrt": lambda v: math.copysign(abs(v) ** (1 / 3), v)}
 ALGS = {"RK4": 0, "RK5(4)": 1}  # BIHIP_RK4, BIHIP_DOPRI5
+KIND_CODE = {"param": 0, "input": 1, "state": 2, "obs": 3, "noise": 4}  # BIHIP_VAR_*
 
 
 class GenError(ValueError):
@@ -64,39 +74,45 @@ class GenError(ValueError):
 
 
 @dataclass
+class Var:
+    name: str
+    kind: str  # param | input | state | noise | obs
+    dim: Optional[str]  # None for a scalar
+    size: int
+    offset: int  # first element in its layout (par slots, state, obs values, noise)
+
+
+@dataclass
 class Draw:
-    target: str
     dist: str  # gaussian | uniform
     log: bool
     a: object  # mean / lower
     b: object  # std / upper
-    index: int  # among the block's draws of the same kind
-    line: int
+    index: int  # first draw of the block's draws of the same kind (one per element)
 
 
 @dataclass
 class Step:
-    kind: str  # draw | set
+    kind: str  # draw | set | ode
     target: str
+    alias: Optional[str]  # dimension alias bound to the element index (x[i])
     expr: object = None
     draw: Optional[Draw] = None
+    line: int = 0
 
 
 @dataclass
 class GenModel:
     name: str
-    params: List[str]
-    inputs: List[str]
-    states: List[str]
-    noises: List[str]
-    obs: List[str]
+    vars: Dict[str, Var]
+    dims: Dict[str, tuple]  # name -> (size, boundary)
     consts: Dict[str, float]
     inlines: Dict[str, object]
-    initial: List[Step]
-    pre: List[Step]  # transition actions before the ode block
-    ode: Dict[str, object]  # state -> derivative
-    post: List[Step]  # transition actions after the ode block
-    observation: List[Draw]
+    initial: List[Step] = field(default_factory=list)
+    pre: List[Step] = field(default_factory=list)  # transition actions before the ode block
+    ode: List[Step] = field(default_factory=list)
+    post: List[Step] = field(default_factory=list)  # transition actions after the ode block
+    observation: List[Step] = field(default_factory=list)
     delta: float = 1.0
     integrator: int = 0
     h: float = 1.0
@@ -106,9 +122,39 @@ class GenModel:
     normals: Dict[str, int] = field(default_factory=dict)   # block -> normal draws
     uniforms: Dict[str, int] = field(default_factory=dict)  # block -> uniform draws
 
+    def names(self, kind):
+        return [v.name for v in self.vars.values() if v.kind == kind]
+
+    def size(self, kind):
+        return sum(v.size for v in self.vars.values() if v.kind == kind)
+
     @property
-    def slots(self):
-        return self.params + self.inputs
+    def params(self):
+        return self.names("param")
+
+    @property
+    def inputs(self):
+        return self.names("input")
+
+    @property
+    def states(self):
+        return self.names("state")
+
+    @property
+    def noises(self):
+        return self.names("noise")
+
+    @property
+    def obs(self):
+        return self.names("obs")
+
+    @property
+    def D(self):
+        return self.size("state")
+
+    @property
+    def NOBS(self):
+        return self.size("obs")
 
     @property
     def cname(self):
@@ -158,15 +204,15 @@ def _const_eval(e, consts, line):
         return -v if e.op == "-" else (v if e.op == "+" else float(not v))
     if isinstance(e, Binary):
         a, b = _const_eval(e.left, consts, line), _const_eval(e.right, consts, line)
-        ops = {"+": a + b, "-": a - b, "*": a * b, "**": None, "%": None}
         if e.op == "/":
             return a / b
         if e.op == "**":
             return math.pow(a, b)
         if e.op == "%":
             return math.fmod(a, b)
-        if e.op in ops:
-            return ops[e.op]
+        arith = {"+": a + b, "-": a - b, "*": a * b}
+        if e.op in arith:
+            return arith[e.op]
         cmp = {"<": a < b, ">": a > b, "<=": a <= b, ">=": a >= b, "==": a == b, "!=": a != b,
                "&&": bool(a) and bool(b), "||": bool(a) or bool(b)}
         return float(cmp[e.op])
@@ -178,133 +224,201 @@ def _const_eval(e, consts, line):
     _err(line, "a constant expression is required here")
 
 
+def _spec_arg(spec, name, pos, consts, default):
+    if len(spec.args) > pos:
+        v = spec.args[pos]
+    else:
+        v = dict(spec.named).get(name)
+    if v is None:
+        return default
+    if isinstance(v, Str):
+        return v.value
+    return _const_eval(v, consts, spec.line)
+
+
 class _Analyser:
     def __init__(self, m: Model):
         self.m = m
-        if m.dims:
-            _err(m.dims[0].line, "dimensions (dim) are not supported by the generator; "
-                                 "vector models are written by hand (Lorenz-96)")
-        for t, s in m.vars:
-            if s.dims:
-                _err(s.line, f"{t} {s.name} has dimensions, which the generator does not support")
-            if t in ("state_aux_", "param_aux_"):
-                _err(s.line, f"{t} variables are not supported")
-        self.kind = {}
-        for t, s in m.vars:
-            if s.name in self.kind:
-                _err(s.line, f"{s.name} declared twice")
-            self.kind[s.name] = t
         self.consts = {}
         for n, e in m.consts:
             self.consts[n] = _const_eval(e, self.consts, 0)
+        self.dims = {}
+        for d in m.dims:
+            size = _spec_arg(d, "size", 0, self.consts, None)
+            if size is None or size != int(size) or size < 1:
+                _err(d.line, f"dim {d.name} needs a positive integer size")
+            bnd = _spec_arg(d, "boundary", 1, self.consts, "none")
+            if bnd not in BOUNDARIES:
+                _err(d.line, f"dim {d.name}: boundary must be one of {BOUNDARIES}")
+            self.dims[d.name] = (int(size), bnd)
+        self.vars = {}
+        offs = {}
+        for t, s in m.vars:
+            if t in ("state_aux_", "param_aux_"):
+                _err(s.line, f"{t} variables are not supported")
+            if s.name in self.vars or s.name in self.dims:
+                _err(s.line, f"{s.name} declared twice")
+            if len(s.dims) > 1:
+                _err(s.line, f"{s.name}: variables over more than one dimension are not supported")
+            dim = s.dims[0] if s.dims else None
+            if dim is not None and dim not in self.dims:
+                _err(s.line, f"{s.name}: unknown dimension {dim!r}")
+            size = self.dims[dim][0] if dim else 1
+            lay = "par" if t in ("param", "input") else t
+            self.vars[s.name] = Var(s.name, t, dim, size, 0)
+            offs.setdefault(lay, []).append(s.name)
+        # par slots: params then inputs; other kinds in declaration order
+        for lay, kinds in (("par", ("param", "input")), ("state", ("state",)),
+                           ("obs", ("obs",)), ("noise", ("noise",))):
+            o = 0
+            for k in kinds:
+                for v in self.vars.values():
+                    if v.kind == k:
+                        v.offset = o
+                        o += v.size
         self.inlines = dict(m.inlines)
 
-    def draw(self, a: Action, block, counts, local_consts):
-        if not isinstance(a.expr, Call):
-            _err(a.line, "the right side of '~' must be a distribution")
-        name = a.expr.name
-        if name in NORMAL_DISTS:
-            args = _args(a.expr, DIST_ARGS["gaussian"], a.line)
-            log = NORMAL_DISTS[name]
-            for n, v in a.expr.named:
-                if n == "log":
-                    log = bool(_const_eval(v, self.consts, a.line))
-            kind = "gaussian"
-            k = counts.setdefault("normal", 0)
-            counts["normal"] += 1
-            d = Draw(a.target, kind, log, args["mean"], args["std"], k, a.line)
-        elif name == "uniform":
-            args = _args(a.expr, DIST_ARGS["uniform"], a.line)
-            k = counts.setdefault("uniform", 0)
-            counts["uniform"] += 1
-            d = Draw(a.target, "uniform", False, args["lower"], args["upper"], k, a.line)
-        else:
-            _err(a.line, f"distribution {name!r} is not supported "
-                         "(gaussian, normal, log_gaussian, log_normal, uniform)")
-        self.check_refs(d.a, block, a.line, local_consts)
-        self.check_refs(d.b, block, a.line, local_consts)
-        return d
-
-    def check_refs(self, e, block, line, local_consts, seen=()):
-        """Every name resolves, to a variable the block may read (Bi::Block::*::validate)."""
+    # ---- expressions
+    def check_refs(self, e, block, line, lc, aliases, elem_dim, seen=()):
+        """Every name resolves, to a variable the block may read
+        (Bi::Block::*::validate); indices are constant expressions of the aliases."""
         if isinstance(e, Ref):
-            if e.index is not None:
-                _err(line, f"indexed reference {e.name}[...] needs dimensions, not supported")
             n = e.name
-            if n in local_consts or n in self.consts:
+            if n in aliases:
+                if e.index is not None:
+                    _err(line, f"dimension alias {n} cannot be indexed")
+                return
+            if n in lc or n in self.consts:
+                if e.index is not None:
+                    _err(line, f"constant {n} cannot be indexed")
                 return
             if n in self.inlines:
                 if n in seen:
                     _err(line, f"inline {n} refers to itself")
-                return self.check_refs(self.inlines[n], block, line, local_consts, seen + (n,))
-            if n not in self.kind:
+                if e.index is not None:
+                    _err(line, f"inline {n} cannot be indexed")
+                return self.check_refs(self.inlines[n], block, line, lc, aliases, elem_dim,
+                                       seen + (n,))
+            if n not in self.vars:
                 _err(line, f"unknown name {n!r}")
-            t = self.kind[n]
+            v = self.vars[n]
             allowed = {"initial": ("param", "input", "state"),
                        "transition": ("param", "input", "state", "noise"),
                        "ode": ("param", "input", "state", "noise"),
                        "observation": ("param", "input", "state")}[block]
-            if t not in allowed:
-                _err(line, f"{t} variable {n} cannot be read in the {block} block")
+            if v.kind not in allowed:
+                _err(line, f"{v.kind} variable {n} cannot be read in the {block} block")
+            if e.index is not None:
+                if v.dim is None:
+                    _err(line, f"{n} has no dimension to index")
+                if len(e.index) != 1:
+                    _err(line, f"{n} takes one index")
+                env = {**self.consts, **lc, **{a: 0.0 for a in aliases}}
+                _const_eval(e.index[0], env, line)  # a constant expression of the aliases
+            elif v.dim is not None and v.dim != elem_dim:
+                _err(line, f"vector {n} read without an index outside an action over "
+                           f"its dimension {v.dim}")
         elif isinstance(e, Unary):
-            self.check_refs(e.arg, block, line, local_consts, seen)
+            self.check_refs(e.arg, block, line, lc, aliases, elem_dim, seen)
         elif isinstance(e, Binary):
-            self.check_refs(e.left, block, line, local_consts, seen)
-            self.check_refs(e.right, block, line, local_consts, seen)
+            self.check_refs(e.left, block, line, lc, aliases, elem_dim, seen)
+            self.check_refs(e.right, block, line, lc, aliases, elem_dim, seen)
         elif isinstance(e, Ternary):
             for x in (e.cond, e.then, e.other):
-                self.check_refs(x, block, line, local_consts, seen)
+                self.check_refs(x, block, line, lc, aliases, elem_dim, seen)
         elif isinstance(e, Call):
             if e.name not in FUNCS:
                 _err(line, f"unknown function {e.name!r}")
             if e.named:
                 _err(line, f"function {e.name} takes no named arguments")
             for x in e.args:
-                self.check_refs(x, block, line, local_consts, seen)
+                self.check_refs(x, block, line, lc, aliases, elem_dim, seen)
         elif isinstance(e, Str):
             _err(line, "a string cannot appear in an expression")
 
-    def local_decls(self, b: Block, local_consts):
+    def target(self, a: Action, block, targets):
+        """(variable, alias) of an action's left side."""
+        v = self.vars.get(a.target)
+        if v is None or v.kind not in targets:
+            _err(a.line, f"{a.target} cannot be the target of an action in the {block} block")
+        alias = None
+        if a.target_dims is not None:
+            if v.dim is None:
+                _err(a.line, f"{a.target} has no dimension to index")
+            if len(a.target_dims) != 1 or not isinstance(a.target_dims[0], Ref) \
+                    or a.target_dims[0].index is not None:
+                _err(a.line, f"the target {a.target}[...] takes one dimension alias")
+            alias = a.target_dims[0].name
+            if alias in self.vars or alias in self.consts or alias in self.dims:
+                _err(a.line, f"dimension alias {alias} hides a declared name")
+        return v, alias
+
+    def draw(self, a: Action, block, counts, lc, v, alias):
+        if not isinstance(a.expr, Call):
+            _err(a.line, "the right side of '~' must be a distribution")
+        name = a.expr.name
+        aliases = (alias,) if alias else ()
+        if name in NORMAL_DISTS:
+            args = _args(a.expr, DIST_ARGS["gaussian"], a.line)
+            log = NORMAL_DISTS[name]
+            for n, val in a.expr.named:
+                if n == "log":
+                    log = bool(_const_eval(val, self.consts, a.line))
+            kind, dist = "normal", "gaussian"
+        elif name == "uniform":
+            args = _args(a.expr, DIST_ARGS["uniform"], a.line)
+            log, kind, dist = False, "uniform", "uniform"
+        else:
+            _err(a.line, f"distribution {name!r} is not supported "
+                         "(gaussian, normal, log_gaussian, log_normal, uniform)")
+        k = counts.setdefault(kind, 0)
+        counts[kind] += v.size
+        names = DIST_ARGS[dist]
+        d = Draw(dist, log, args[names[0]], args[names[1]], k)
+        for x in (d.a, d.b):
+            self.check_refs(x, block, a.line, lc, aliases, v.dim)
+        return d
+
+    def local_decls(self, b: Block, lc):
         for it in _flatten(b.items):
             if isinstance(it, tuple):
                 kind, n, e = it
                 if kind == "const":
-                    local_consts[n] = _const_eval(e, {**self.consts, **local_consts}, b.line)
+                    lc[n] = _const_eval(e, {**self.consts, **lc}, b.line)
                 else:
                     self.inlines[n] = e
 
-    def steps(self, b: Block, block, counts, local_consts, targets):
+    def step(self, a: Action, block, counts, lc, targets):
+        v, alias = self.target(a, block, targets)
+        aliases = (alias,) if alias else ()
+        if a.op == "~":
+            return Step("draw", v.name, alias, draw=self.draw(a, block, counts, lc, v, alias),
+                        line=a.line)
+        if a.op == "<-":
+            self.check_refs(a.expr, block, a.line, lc, aliases, v.dim)
+            return Step("set", v.name, alias, expr=a.expr, line=a.line)
+        _err(a.line, "an ODE equation must be inside an ode block")
+
+    def steps(self, b: Block, block, counts, lc, targets):
         out = []
         for it in _flatten(b.items):
             if isinstance(it, tuple):
                 continue
             if isinstance(it, Block):
                 _err(it.line, f"nested block {it.spec.name or '{...}'} is not supported here")
-            a = it
-            if a.target_dims is not None:
-                _err(a.line, f"indexed target {a.target}[...] needs dimensions, not supported")
-            t = self.kind.get(a.target)
-            if t not in targets:
-                _err(a.line, f"{a.target} cannot be the target of an action in the {block} block")
-            if a.op == "~":
-                out.append(Step("draw", a.target, draw=self.draw(a, block, counts, local_consts)))
-            elif a.op == "<-":
-                self.check_refs(a.expr, block, a.line, local_consts)
-                out.append(Step("set", a.target, expr=a.expr))
-            else:
-                _err(a.line, "an ODE equation must be inside an ode block")
+            out.append(self.step(it, block, counts, lc, targets))
         return out
 
     def run(self) -> GenModel:
         m = self.m
-        consts_all = dict(self.consts)
-        g = GenModel(m.name, m.names_of("param"), m.names_of("input"), m.names_of("state"),
-                     m.names_of("noise"), m.names_of("obs"), consts_all, self.inlines,
-                     [], [], {}, [], [])
+        g = GenModel(m.name, self.vars, self.dims, dict(self.consts), self.inlines)
         if not g.states:
             raise GenError("the model has no state variables")
-        if len(g.slots) > MAX_SLOTS:
-            raise GenError(f"{len(g.slots)} params and inputs; at most {MAX_SLOTS}")
+        if g.size("param") + g.size("input") > MAX_SLOTS:
+            raise GenError(f"{g.size('param') + g.size('input')} param and input elements; "
+                           f"at most {MAX_SLOTS}")
+        if g.D > MAX_STATE:
+            raise GenError(f"{g.D} state elements; at most {MAX_STATE} per particle")
         for b in m.blocks:
             if b.spec.name not in ("initial", "transition", "observation", "parameter",
                                    "proposal_parameter", "proposal_initial",
@@ -323,9 +437,9 @@ class _Analyser:
         # sub transition(delta = 1.0)
         b = m.block("transition")
         if b is not None:
-            g.delta = self.block_arg(b, "delta", 0, 1.0)
-            if not g.delta > 0:
-                _err(b.line, "delta must be positive")
+            g.delta = _spec_arg(b.spec, "delta", 0, self.consts, 1.0)
+            if not (isinstance(g.delta, float) and g.delta > 0):
+                _err(b.line, "delta must be a positive number")
             lc = {}
             self.local_decls(b, lc)
             counts = {}
@@ -342,8 +456,7 @@ class _Analyser:
                     self.ode_block(it, g, lc)
                     cur = g.post
                     continue
-                cur.extend(self.steps(Block(b.spec, [it], b.line), "transition", counts, lc,
-                                      ("state", "noise")))
+                cur.append(self.step(it, "transition", counts, lc, ("state", "noise")))
             g.consts.update(lc)
             g.normals["transition"] = counts.get("normal", 0)
             g.uniforms["transition"] = counts.get("uniform", 0)
@@ -353,25 +466,18 @@ class _Analyser:
             lc = {}
             self.local_decls(b, lc)
             counts = {}
+            seen = []
             for s in self.steps(b, "observation", counts, lc, ("obs",)):
                 if s.kind != "draw":
-                    _err(0, "only '~' actions belong in the observation block")
-                g.observation.append(s.draw)
+                    _err(s.line, "only '~' actions belong in the observation block")
+                if s.target in seen:
+                    _err(s.line, f"{s.target} is the target of two actions")
+                seen.append(s.target)
+                g.observation.append(s)
             g.consts.update(lc)
             g.normals["observation"] = counts.get("normal", 0)
             g.uniforms["observation"] = counts.get("uniform", 0)
-            seen = [d.target for d in g.observation]
-            if len(set(seen)) != len(seen):
-                _err(b.line, "an obs variable is the target of two actions")
         return g
-
-    def block_arg(self, b, name, pos, default):
-        if len(b.spec.args) > pos:
-            return _const_eval(b.spec.args[pos], self.consts, b.line)
-        for n, v in b.spec.named:
-            if n == name:
-                return _const_eval(v, self.consts, b.line)
-        return default
 
     def ode_block(self, b: Block, g: GenModel, lc):
         names = ("alg", "h", "atoler", "rtoler")
@@ -395,35 +501,37 @@ class _Analyser:
         g.rtol = _const_eval(vals["rtoler"], env, b.line) if "rtoler" in vals else 1e-3
         g.has_ode = True
         self.local_decls(b, lc)
+        done = set()
         for it in _flatten(b.items):
             if isinstance(it, tuple):
                 continue
             if isinstance(it, Block) or it.op != "=":
                 _err(getattr(it, "line", b.line), "an ode block holds only dx/dt = ... equations")
-            if it.target_dims is not None:
-                _err(it.line, "indexed ODE targets need dimensions, not supported")
-            if self.kind.get(it.target) != "state":
-                _err(it.line, f"{it.target} is not a state variable")
-            if it.target in g.ode:
-                _err(it.line, f"two equations for d{it.target}/dt")
-            self.check_refs(it.expr, "ode", it.line, lc)
-            g.ode[it.target] = it.expr
+            v, alias = self.target(it, "ode", ("state",))
+            if v.name in done:
+                _err(it.line, f"two equations for d{v.name}/dt")
+            done.add(v.name)
+            self.check_refs(it.expr, "ode", it.line, lc, (alias,) if alias else (), v.dim)
+            g.ode.append(Step("ode", v.name, alias, expr=it.expr, line=it.line))
 
 
 def analyse(model: Model) -> GenModel:
-    return _Analyser(model).run()
+    g = _Analyser(model).run()
+    generate_model(g)  # index ranges are resolved element by element when emitting
+    return g
 
 
 # ---------------------------------------------------------------- emission
 def _lit(v):
     if math.isinf(v):
         return "__builtin_inf()" if v > 0 else "(-__builtin_inf())"
-    r = repr(float(v))
-    return r
+    return repr(float(v))
 
 
 class _Emitter:
-    """Expressions in one arithmetic type: 'T' (the state type) or 'double'."""
+    """Expressions in one arithmetic type, 'T' (the state type) or 'double', for
+    one element: bind maps dimension aliases to indices, elem is the element of
+    an action over a whole vector (its unindexed vectors read that element)."""
 
     def __init__(self, g: GenModel, ty: str, xname: str, xdouble=False):
         self.g, self.ty, self.x, self.xdouble = g, ty, xname, xdouble
@@ -431,33 +539,55 @@ class _Emitter:
     def num(self, v):
         return f"T({_lit(v)})" if self.ty == "T" else _lit(v)
 
-    def expr(self, e, local=()):
+    def element(self, v: Var, e: Ref, bind, elem, line):
+        if e.index is None:
+            return 0 if v.dim is None else elem
+        env = {**self.g.consts, **{k: float(i) for k, i in bind.items()}}
+        f = _const_eval(e.index[0], env, line)
+        if f != int(f):
+            _err(line, f"index of {v.name} is not an integer")
+        i = int(f)
+        size, bnd = self.g.dims[v.dim]
+        if bnd == "cyclic":
+            return i % size
+        if bnd == "extended":
+            return min(max(i, 0), size - 1)
+        if not 0 <= i < size:
+            _err(line, f"index {i} of {v.name} is outside its dimension {v.dim} (size {size})")
+        return i
+
+    def expr(self, e, bind=None, elem=None, line=0, local=()):
         g = self.g
+        bind = bind or {}
         if isinstance(e, Num):
             return self.num(e.value)
         if isinstance(e, Ref):
             n = e.name
+            if n in bind:
+                return self.num(float(bind[n]))
             if n in g.consts:
                 return self.num(g.consts[n])
             if n in g.inlines and n not in local:
-                return "(" + self.expr(g.inlines[n], local + (n,)) + ")"
-            if n in g.slots:
-                v = f"p_{n}"
-                return f"T({v})" if self.ty == "T" else v
-            if n in g.states:
-                v = f"{self.x}[{g.states.index(n)}]"
-                if self.ty == "double" and not self.xdouble:
-                    return f"(double){v}"
-                return v
-            if n in g.noises:
-                v = f"w_{n}"
-                return f"(double){v}" if self.ty == "double" else v
-            raise GenError(f"cannot read {n!r} here")
+                return "(" + self.expr(g.inlines[n], bind, elem, line, local + (n,)) + ")"
+            v = g.vars.get(n)
+            if v is None:
+                raise GenError(f"line {line}: cannot read {n!r} here")
+            i = self.element(v, e, bind, elem, line)
+            if v.kind in ("param", "input"):
+                r = f"p_{n}" + (f"[{i}]" if v.dim else "")
+                return f"T({r})" if self.ty == "T" else r
+            if v.kind == "state":
+                r = f"{self.x}[{v.offset + i}]"
+                return f"(double){r}" if self.ty == "double" and not self.xdouble else r
+            if v.kind == "noise":
+                r = f"w_{n}" + (f"[{i}]" if v.dim else "")
+                return f"(double){r}" if self.ty == "double" else r
+            raise GenError(f"line {line}: cannot read {v.kind} {n} here")
         if isinstance(e, Unary):
-            a = self.expr(e.arg, local)
-            return f"({e.op}{a})"
+            return f"({e.op}{self.expr(e.arg, bind, elem, line, local)})"
         if isinstance(e, Binary):
-            a, b = self.expr(e.left, local), self.expr(e.right, local)
+            a = self.expr(e.left, bind, elem, line, local)
+            b = self.expr(e.right, bind, elem, line, local)
             if e.op == "**":
                 return f"pow({a}, {b})"
             if e.op == "%":
@@ -466,77 +596,140 @@ class _Emitter:
                 return f"{self.ty}(({a}) {e.op} ({b}))"
             return f"({a} {e.op} {b})"
         if isinstance(e, Ternary):
-            return (f"(({self.expr(e.cond, local)}) != {self.num(0.0)} ? "
-                    f"{self.expr(e.then, local)} : {self.expr(e.other, local)})")
+            c = self.expr(e.cond, bind, elem, line, local)
+            return (f"(({c}) != {self.num(0.0)} ? {self.expr(e.then, bind, elem, line, local)}"
+                    f" : {self.expr(e.other, bind, elem, line, local)})")
         if isinstance(e, Call):
             # a function of constants is evaluated here, correctly rounded in double
             # (the reference evaluates constant expressions when it generates code)
             try:
-                return self.num(_const_eval(e, g.consts, 0))
+                env = {**g.consts, **{k: float(i) for k, i in bind.items()}}
+                return self.num(_const_eval(e, env, line))
             except GenError:
                 pass
-            args = ", ".join(self.expr(a, local) for a in e.args)
+            args = ", ".join(self.expr(a, bind, elem, line, local) for a in e.args)
             return f"{FUNCS[e.name]}({args})"
-        raise GenError(f"cannot emit {e!r}")
+        raise GenError(f"line {line}: cannot emit {e!r}")
 
 
-def _draw_value(em: _Emitter, d: Draw, zname: str, uname: str):
-    """Value of a draw (Bi::Action::gaussian / uniform `sample`, share/tt/cpp/action/*.hpp.tt)."""
-    a, b = em.expr(d.a), em.expr(d.b)
+def _elements(g: GenModel, st: Step):
+    """(element, bind) pairs an action covers."""
+    v = g.vars[st.target]
+    for i in range(v.size):
+        yield i, ({st.alias: i} if st.alias else {})
+
+
+def _zname(k):
+    return f"zq{k // 4}[{k % 4}]"
+
+
+def _draw_value(em: _Emitter, d: Draw, i, bind, line):
+    """Value of element i of a draw (Bi::Action::gaussian / uniform `sample`,
+    share/tt/cpp/action/*.hpp.tt)."""
+    a, b = em.expr(d.a, bind, i, line), em.expr(d.b, bind, i, line)
     if d.dist == "gaussian":
-        z = zname if em.ty == "T" else f"(double){zname}"
+        z = _zname(d.index + i)
+        z = z if em.ty == "T" else f"(double){z}"
         v = f"({a} + ({b} * {z}))"
         return f"exp({v})" if d.log else v
-    u = uname if em.ty == "T" else f"(double){uname}"
+    u = f"u{d.index + i}"
+    u = u if em.ty == "T" else f"(double){u}"
     return f"({a} + (({b} - {a}) * {u}))"
 
 
-def _zname(d):
-    return f"zq{d.index // 4}[{d.index % 4}]"
+def _target(g: GenModel, name, i, xname):
+    v = g.vars[name]
+    if v.kind == "state":
+        return f"{xname}[{v.offset + i}]"
+    return f"w_{name}" + (f"[{i}]" if v.dim else "")
+
+
+def _actions(g, steps, em, xname, ind, tmp_ty):
+    """Statements of discrete actions, in order; a vector action computes all
+    its elements before it stores them."""
+    lines = []
+    for st in steps:
+        v = g.vars[st.target]
+        vals = []
+        for i, bind in _elements(g, st):
+            if st.kind == "draw":
+                vals.append(_draw_value(em, st.draw, i, bind, st.line))
+            else:
+                vals.append(em.expr(st.expr, bind, i, st.line))
+        if v.dim is None:
+            lines.append(f"{ind}{_target(g, st.target, 0, xname)} = {vals[0]};")
+        else:
+            lines.append(f"{ind}{{")
+            lines.append(f"{ind}  {tmp_ty} t_[{v.size}];")
+            for i, val in enumerate(vals):
+                lines.append(f"{ind}  t_[{i}] = {val};")
+            for i in range(v.size):
+                lines.append(f"{ind}  {_target(g, st.target, i, xname)} = t_[{i}];")
+            lines.append(f"{ind}}}")
+    return lines
+
+
+def _draw_buffers(g, block, step, tag, ind, w, member=False):
+    nn, nu = g.normals.get(block, 0), g.uniforms.get(block, 0)
+    for b in range((nn + 3) // 4):
+        if not member:
+            w(f"{ind}T zq{b}[4];")
+        w(f"{ind}normals4<T>(seed, p, {step}, {tag}, batch, {b}u, zq{b});")
+    for k in range(nu):
+        w(f"{ind}{'' if member else 'const T '}u{k} = uniform_word<T>(philox_block(seed, p, "
+          f"{step}, {tag}, batch, {UNIFORM_BLOCK:#x}u + {k}u));")
 
 
 def generate_model(g: GenModel) -> str:
     """The device functor of one model."""
-    S = len(g.states)
+    S, NO = g.D, g.NOBS
     out = []
     w = out.append
     w(f"// ---- model {g.name}: generated from its .bi file")
     w("template <class T>")
     w(f"struct {g.cname} {{")
     w("  using real_t = T;")
-    w(f"  static constexpr int D = {S}, LPP = 1, DL = {S}, NOBS = {len(g.obs)};")
+    w(f"  static constexpr int D = {S}, LPP = 1, DL = {S}, NOBS = {NO};")
     w("  static constexpr bool NOISE_FIRST = false, HAS_FORCING = false, GENERATED = true;")
     w(f"  static constexpr double DELTA = {_lit(g.delta)};")
-    for n in g.slots:
-        w(f"  double p_{n};")
-    for n in g.noises:
-        w(f"  T w_{n};")
-    init = ", ".join([f"p_{n}(a.par[{i}])" for i, n in enumerate(g.slots)]
-                     + [f"w_{n}(T(0))" for n in g.noises])
-    w(f"  __device__ __forceinline__ explicit {g.cname}(const ModelArgs& a)"
-      + (f" : {init}" if init else "") + " {}")
-    # rhs: the ode block (share/tt/cpp/action/ode_.hpp.tt); states without an equation stay put
-    em = _Emitter(g, "T", "x")
-    w("  template <class G>")
-    w("  __device__ __forceinline__ void rhs(const G&, const T (&x)[DL], T (&k)[DL]) const {")
-    for j, s in enumerate(g.states):
-        w(f"    k[{j}] = {em.expr(g.ode[s]) if s in g.ode else 'T(0)'};")
-    w("  }")
-
-    # discrete part of the transition: actions in the order written
-    def discrete(steps, ind):
-        lines = []
-        for st in steps:
-            tgt = (f"x[{g.states.index(st.target)}]" if st.target in g.states
-                   else f"w_{st.target}")
-            if st.kind == "draw":
-                lines.append(f"{ind}{tgt} = {_draw_value(em, st.draw, _zname(st.draw), f'u{st.draw.index}')};")
-            else:
-                lines.append(f"{ind}{tgt} = {em.expr(st.expr)};")
-        return lines
-
+    slots = [v for v in g.vars.values() if v.kind in ("param", "input")]
+    noises = [v for v in g.vars.values() if v.kind == "noise"]
+    for v in slots:
+        w(f"  double p_{v.name}" + (f"[{v.size}];" if v.dim else ";"))
+    for v in noises:
+        w(f"  T w_{v.name}" + (f"[{v.size}];" if v.dim else ";"))
     nn_t, nu_t = g.normals.get("transition", 0), g.uniforms.get("transition", 0)
     nbt = (nn_t + 3) // 4
+    # draw buffers of the transition: members, shared by its parts before and after the ode
+    for b in range(nbt):
+        w(f"  T zq{b}[4];")
+    for k in range(nu_t):
+        w(f"  T u{k};")
+    init = [f"p_{v.name}(a.par[{v.offset}])" for v in slots if not v.dim]
+    init += [f"w_{v.name}(T(0))" for v in noises if not v.dim]
+    w(f"  __device__ __forceinline__ explicit {g.cname}(const ModelArgs& a)"
+      + (f" : {', '.join(init)}" if init else "") + " {")
+    for v in slots:
+        if v.dim:
+            w(f"    for (int i = 0; i < {v.size}; ++i) p_{v.name}[i] = a.par[{v.offset} + i];")
+    for v in noises:
+        if v.dim:
+            w(f"    for (int i = 0; i < {v.size}; ++i) w_{v.name}[i] = T(0);")
+    w("  }")
+
+    # rhs: the ode block (share/tt/cpp/action/ode_.hpp.tt); states without an equation stay put
+    em = _Emitter(g, "T", "x")
+    rhs = ["T(0)"] * S
+    for st in g.ode:
+        v = g.vars[st.target]
+        for i, bind in _elements(g, st):
+            rhs[v.offset + i] = em.expr(st.expr, bind, i, st.line)
+    w("  template <class G>")
+    w("  __device__ __forceinline__ void rhs(const G&, const T (&x)[DL], T (&k)[DL]) const {")
+    for j in range(S):
+        w(f"    k[{j}] = {rhs[j]};")
+    w("  }")
+
     w("  // one interval [t1, t2]: discrete actions at multiples of DELTA, the ode in between")
     w("  template <int INTEG, class G>")
     w("  __device__ __forceinline__ int transition(const G& grp, T (&x)[DL], double t1, double t2,")
@@ -551,15 +744,14 @@ def generate_model(g: GenModel) -> str:
     w("      const double sb = fmin(floor(sa) + 1.0, s2);")
     w("      const double tb = sb == s2 ? t2 : sb * DELTA;")
     w("      const bool on = sa == floor(sa);")
-    has_discrete = bool(g.pre or g.post)
-    if has_discrete:
+    if g.pre or g.post:
         w("      if (on) {")
         for b in range(nbt):
             w(f"        normals4<T>(seed, p, step, TAG_NOISE, batch, fire * {nbt}u + {b}u, zq{b});")
         for k in range(nu_t):
             w(f"        u{k} = uniform_word<T>(philox_block(seed, p, step, TAG_NOISE, batch, "
               f"{UNIFORM_BLOCK:#x}u + fire * {nu_t}u + {k}u));")
-        for ln in discrete(g.pre, "        "):
+        for ln in _actions(g, g.pre, em, "x", "        ", "T"):
             w(ln)
         w("      }")
     if g.has_ode:
@@ -567,7 +759,7 @@ def generate_model(g: GenModel) -> str:
         w("      else n += integrate_dopri5(*this, grp, x, ta, tb, ma);")
     if g.post:
         w("      if (on) {")
-        for ln in discrete(g.post, "        "):
+        for ln in _actions(g, g.post, em, "x", "        ", "T"):
             w(ln)
         w("      }")
     w("      fire += on ? 1u : 0u;")
@@ -576,76 +768,59 @@ def generate_model(g: GenModel) -> str:
     w("    }")
     w("    return n;")
     w("  }")
-    # the draw buffers are members, shared by the discrete parts before and after the ode
-    decl = [f"  T zq{b}[4];" for b in range(nbt)] + [f"  T u{k};" for k in range(nu_t)]
-    idx = out.index("  // one interval [t1, t2]: discrete actions at multiples of DELTA, the ode in between")
-    out[idx:idx] = decl
 
     # observation log-density (share/tt/cpp/action/gaussian.hpp.tt:48-73, uniform.hpp.tt:38-52)
     eo = _Emitter(g, "double", "x")
     w("  __device__ __forceinline__ double obs(const T (&x)[DL], const double* y) const {")
     w("    double lp = 0.0;")
-    for d in g.observation:
-        k = g.obs.index(d.target)
-        a, b = eo.expr(d.a), eo.expr(d.b)
-        w("    {")
-        w(f"      const double xy = y[{k}];")
-        if d.dist == "gaussian":
-            w(f"      const double mu = {a};")
-            w(f"      const double sg = {b};")
-            if d.log:
-                w("      const double z = (log(xy) - mu) / sg;")
-                w("      lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);")
+    for st in g.observation:
+        d, v = st.draw, g.vars[st.target]
+        for i, bind in _elements(g, st):
+            a, b = eo.expr(d.a, bind, i, st.line), eo.expr(d.b, bind, i, st.line)
+            w("    {")
+            w(f"      const double xy = y[{v.offset + i}];")
+            if d.dist == "gaussian":
+                w(f"      const double mu = {a};")
+                w(f"      const double sg = {b};")
+                if d.log:
+                    w("      const double z = (log(xy) - mu) / sg;")
+                    w("      lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);")
+                else:
+                    w("      if (sg == 0.0) {")
+                    w("        lp = xy == mu ? __builtin_inf() : -__builtin_inf();")
+                    w("      } else {")
+                    w("        const double z = (xy - mu) / sg;")
+                    w("        lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg);")
+                    w("      }")
             else:
-                w("      if (sg == 0.0) {")
-                w("        lp = xy == mu ? __builtin_inf() : -__builtin_inf();")
-                w("      } else {")
-                w("        const double z = (xy - mu) / sg;")
-                w("        lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg);")
-                w("      }")
-        else:
-            w(f"      const double mn = {a}, mx = {b};")
-            w("      if (xy >= mn && xy <= mx) lp += -log(mx - mn);")
-            w("      else lp = -__builtin_inf();")
-        w("    }")
+                w(f"      const double mn = {a}, mx = {b};")
+                w("      if (xy >= mn && xy <= mx) lp += -log(mx - mn);")
+                w("      else lp = -__builtin_inf();")
+            w("    }")
     w("    return lp;")
     w("  }")
 
     # observation samples (Model::observationSamples), y has NOBS entries
-    no, nuo = g.normals.get("observation", 0), g.uniforms.get("observation", 0)
     w("  __device__ __forceinline__ void obs_sample(const T (&x)[DL], uint64_t seed, uint32_t batch,")
     w("                                             uint32_t p, uint32_t step, double* y) const {")
-    for b in range((no + 3) // 4):
-        w(f"    T zq{b}[4];")
-        w(f"    normals4<T>(seed, p, step, TAG_OBS, batch, {b}u, zq{b});")
-    for k in range(nuo):
-        w(f"    const T u{k} = uniform_word<T>(philox_block(seed, p, step, TAG_OBS, batch, "
-          f"{UNIFORM_BLOCK:#x}u + {k}u));")
-    for j in range(len(g.obs)):
+    _draw_buffers(g, "observation", "step", "TAG_OBS", "    ", w)
+    for j in range(NO):
         w(f"    y[{j}] = __builtin_nan(\"\");")
-    for d in g.observation:
-        w(f"    y[{g.obs.index(d.target)}] = {_draw_value(eo, d, _zname(d), f'u{d.index}')};")
+    for st in g.observation:
+        v = g.vars[st.target]
+        for i, bind in _elements(g, st):
+            w(f"    y[{v.offset + i}] = {_draw_value(eo, st.draw, i, bind, st.line)};")
     w("  }")
 
     # initial states, in double (share/tt/cpp/block/initial.hpp.tt)
-    ni, nui = g.normals.get("initial", 0), g.uniforms.get("initial", 0)
     ei = _Emitter(g, "double", "xd", xdouble=True)
     w("  __device__ __forceinline__ void init(double (&xd)[DL], uint64_t seed, uint32_t batch,")
     w("                                       uint32_t p) const {")
     for j in range(S):
         w(f"    xd[{j}] = 0.0;")
-    for b in range((ni + 3) // 4):
-        w(f"    T zq{b}[4];")
-        w(f"    normals4<T>(seed, p, 0u, TAG_INIT, batch, {b}u, zq{b});")
-    for k in range(nui):
-        w(f"    const T u{k} = uniform_word<T>(philox_block(seed, p, 0u, TAG_INIT, batch, "
-          f"{UNIFORM_BLOCK:#x}u + {k}u));")
-    for st in g.initial:
-        j = g.states.index(st.target)
-        if st.kind == "draw":
-            w(f"    xd[{j}] = {_draw_value(ei, st.draw, _zname(st.draw), f'u{st.draw.index}')};")
-        else:
-            w(f"    xd[{j}] = {ei.expr(st.expr)};")
+    _draw_buffers(g, "initial", "0u", "TAG_INIT", "    ", w)
+    for ln in _actions(g, g.initial, ei, "xd", "    ", "double"):
+        w(ln)
     w("  }")
     w("};")
     return "\n".join(out)
@@ -653,6 +828,10 @@ def generate_model(g: GenModel) -> str:
 
 def _cstr_list(names):
     return "{" + ", ".join(f'"{n}"' for n in names) + (", " if names else "") + "nullptr}"
+
+
+def _int_list(vals):
+    return "{" + ", ".join(str(v) for v in vals) + (", " if vals else "") + "0}"
 
 
 def generate(models: List[GenModel]) -> str:
@@ -665,21 +844,28 @@ def generate(models: List[GenModel]) -> str:
         out.append("")
     out.append(f"constexpr int GEN_COUNT = {len(models)};")
     out.append("static const char* const GEN_NONE[] = {nullptr};")
+    out.append("static const int GEN_NONE_SIZES[] = {0};")
     for i, g in enumerate(models):
-        out.append(f"static const char* const GEN_{i}_PARAMS[] = {_cstr_list(g.params)};")
-        out.append(f"static const char* const GEN_{i}_INPUTS[] = {_cstr_list(g.inputs)};")
-        out.append(f"static const char* const GEN_{i}_STATES[] = {_cstr_list(g.states)};")
-        out.append(f"static const char* const GEN_{i}_OBS[] = {_cstr_list(g.obs)};")
-        out.append(f"static const char* const GEN_{i}_NOISES[] = {_cstr_list(g.noises)};")
+        for kind in KIND_CODE:
+            vs = [v for v in g.vars.values() if v.kind == kind]
+            out.append(f"static const char* const GEN_{i}_{kind.upper()}[] = "
+                       f"{_cstr_list([v.name for v in vs])};")
+            out.append(f"static const int GEN_{i}_{kind.upper()}_SIZES[] = "
+                       f"{_int_list([v.size for v in vs])};")
+        sv = [v for v in g.vars.values() if v.kind == "state"]
+        out.append(f"static const char* const GEN_{i}_STATE_DIMS[] = "
+                   f"{_cstr_list([v.dim or '' for v in sv])};")
     out.append("static const GenInfo GEN_MODELS[] = {")
     for i, g in enumerate(models):
-        out.append(f'    {{"{g.name}", {len(g.states)}, {len(g.obs)}, {g.integrator}, '
+        names = ", ".join(f"GEN_{i}_{k.upper()}" for k in KIND_CODE)
+        sizes = ", ".join(f"GEN_{i}_{k.upper()}_SIZES" for k in KIND_CODE)
+        out.append(f'    {{"{g.name}", {g.D}, {g.NOBS}, {g.integrator}, '
                    f"{_lit(g.h)}, {_lit(g.atol)}, {_lit(g.rtol)}, {_lit(g.delta)}, "
-                   f"GEN_{i}_PARAMS, GEN_{i}_INPUTS, GEN_{i}_STATES, GEN_{i}_OBS, "
-                   f"GEN_{i}_NOISES}},")
+                   f"{{{names}}}, {{{sizes}}}, GEN_{i}_STATE_DIMS}},")
     if not models:
-        out.append('    {"", 0, 0, 0, 0.0, 0.0, 0.0, 1.0, GEN_NONE, GEN_NONE, GEN_NONE, GEN_NONE, '
-                   "GEN_NONE},")
+        none = ", ".join(["GEN_NONE"] * len(KIND_CODE))
+        nsz = ", ".join(["GEN_NONE_SIZES"] * len(KIND_CODE))
+        out.append(f'    {{"", 0, 0, 0, 0.0, 0.0, 0.0, 1.0, {{{none}}}, {{{nsz}}}, GEN_NONE}},')
     out.append("};")
     out.append("")
     out.append("// F::template run<M>() for the generated model of m")

@@ -80,8 +80,8 @@ def test_pz_model_of_the_user_guide():
     assert g.consts == {"c": 0.25, "e": 0.3, "m_l": 0.1, "m_q": 0.1}
     assert g.integrator == capi.RK4 and g.h == 0.1 and g.delta == 1.0
     assert [s.target for s in g.pre] == ["alpha"] and not g.post
-    assert set(g.ode) == {"P", "Z"}
-    assert [d.target for d in g.observation] == ["P_obs"] and g.observation[0].log
+    assert {s.target for s in g.ode} == {"P", "Z"}
+    assert [s.target for s in g.observation] == ["P_obs"] and g.observation[0].draw.log
 
 
 def test_generated_expressions_keep_the_tree():
@@ -117,7 +117,12 @@ def test_draw_counters():
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("model M { dim n(4)\n state x[n] }", "dimensions"),
+    ("model M { dim n(4)\n dim k(2)\n state x[n, k] }", "more than one dimension"),
+    ("model M { dim n(4)\n state x[n]\n sub initial { x[i] <- x[i+1] } }", "outside"),
+    ("model M { dim n(4)\n state x[n], y\n sub initial { y <- x } }", "without an index"),
+    ("model M { dim n(4, boundary = 'odd')\n state x[n] }", "boundary"),
+    ("model M { dim n(2.5)\n state x[n] }", "positive integer"),
+    ("model M { dim n(40)\n dim k(30)\n state x[n], z[k] }", "at most 64"),
     ("model M { state x\n sub transition { x ~ gamma(1.0) } }", "not supported"),
     ("model M { state x\n sub transition { ode { dx/dt = -x } } }", "RK4\\(3\\)"),
     ("model M { state x\n sub transition { ode(alg = 'RK4') { dx/dt = y } } }", "unknown name"),
@@ -142,8 +147,8 @@ def test_registry_models_compile_into_the_library():
     for g in gs:
         m = capi.generated_model(g.name)
         assert m.model >= capi.MODEL_GENERATED
-        assert m.d == len(g.states)
-        assert capi.nobs(m) == len(g.obs)
+        assert m.d == g.D == sum(capi.model_var_sizes(m, capi.VAR_STATE))
+        assert capi.nobs(m) == g.NOBS
         assert capi.model_vars(m, capi.VAR_PARAM) == g.params
         assert capi.model_vars(m, capi.VAR_STATE) == g.states
         assert capi.model_vars(m, capi.VAR_OBS) == g.obs
@@ -191,8 +196,7 @@ struct ModelArgs { double h0; double par[16]; };
 struct bihip_model_ { int model, precision; };
 struct u32x4 { uint32_t x, y, z, w; };
 struct GenInfo { const char* name; int d, nobs, integrator; double h, atol, rtol, delta;
-  const char* const* params; const char* const* inputs; const char* const* states;
-  const char* const* obs; const char* const* noises; };
+  const char* const* names[5]; const int* sizes[5]; const char* const* state_dims; };
 constexpr double HALF_LOG_TWO_PI = 0.9189385332046727;
 constexpr uint32_t TAG_INIT = 1, TAG_NOISE = 2, TAG_OBS = 6;
 inline int set_error(int c, const char*) { return c; }
@@ -230,3 +234,61 @@ int check(const bihip_model* m) { Runner r; return with_generated(m, r); }
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_vectors_over_dimensions():
+    """Dimensions with cyclic, extended and plain boundaries; actions over
+    whole vectors and over aliased elements, with index arithmetic."""
+    g = _gen("""model M {
+      const K = 5
+      dim n(size = K, boundary = 'cyclic')
+      dim e(size = 3, boundary = 'extended')
+      dim h(size = 2)
+      param a[h], b
+      state x[n], u[e], s
+      noise w[n]
+      obs y[h]
+      sub initial { x[i] ~ gaussian(i, b)  u <- 1.0  s <- a[1] }
+      sub transition(delta = 0.5) {
+        w ~ gaussian()
+        x[i] <- x[i-1] + w[i]
+        u[k] <- u[k+1] + u[k-1]
+        ode(alg = 'RK4', h = 0.1) { dx[i]/dt = x[i+1] - x[i-2] }
+      }
+      sub observation { y[j] ~ gaussian(x[2*j+1], a[j]) }
+    }""")
+    assert (g.D, g.NOBS, g.delta) == (9, 2, 0.5)
+    assert [(v.name, v.offset, v.size) for v in g.vars.values() if v.kind == "state"] == \
+        [("x", 0, 5), ("u", 5, 3), ("s", 8, 1)]
+    assert g.normals["transition"] == 5 and g.normals["initial"] == 5
+    src = codegen.generate_model(g)
+    # initial: x[i] ~ gaussian(i, b), the alias as a number; s from the second element of a
+    assert "t_[3] = (3.0 + (p_b * (double)zq0[3]));" in src
+    assert "t_[4] = (4.0 + (p_b * (double)zq1[0]));" in src
+    assert "xd[8] = p_a[1];" in src
+    # cyclic wrap of x[i-1] and x[i-2]; a vector action reads the values from before it
+    assert "t_[0] = (x[4] + w_w[0]);" in src and "x[0] = t_[0];" in src
+    assert "k[0] = (x[1] - x[3]);" in src and "k[4] = (x[0] - x[2]);" in src
+    # extended boundary clamps u[k-1] at 0 and u[k+1] at 2 (state offset 5)
+    assert "t_[0] = (x[6] + x[5]);" in src and "t_[2] = (x[7] + x[6]);" in src
+    # states without an equation keep their value in the ode
+    assert "k[5] = T(0);" in src and "k[8] = T(0);" in src
+    # observations of x[1] and x[3] with the vector param's elements as sd
+    assert "const double mu = (double)x[3];" in src and "const double sg = p_a[1];" in src
+    assert "y[1] = ((double)x[3] + (p_a[1] * (double)zq0[1]));" in src
+
+
+def test_netcdf_output_of_a_vector_model(tmp_path):
+    """A generated model's state over a dimension is written as (nr, dim, np)."""
+    from scipy.io import netcdf_file
+    m = capi.generated_model("Lorenz96", precision=capi.F32)
+    o = capi.PFOutput(tmp_path / "l96.nc", m, 2, 3)
+    X = np.arange(8 * 3, dtype=np.float32).reshape(8, 3)
+    for k in range(2):
+        o.write_time(k, 0.05 * k)
+        o.write_state(k, X + k, np.arange(3))
+    o.close()
+    f = netcdf_file(str(tmp_path / "l96.nc"), "r", mmap=False)
+    assert f.dimensions["n"] == 8 and f.variables["x"].dimensions == ("nr", "n", "np")
+    np.testing.assert_array_equal(np.array(f.variables["x"].data)[1], X + 1)
+    f.close()
