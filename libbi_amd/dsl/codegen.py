@@ -47,8 +47,18 @@ from .parser import (Action, Binary, Block, BiSyntaxError, Call, Model, Num, Ref
 MAX_SLOTS = 16  # bihip_model::par
 MAX_STATE = 64  # state elements of one particle (one lane's registers)
 NORMAL_DISTS = {"gaussian": False, "normal": False, "log_gaussian": True, "log_normal": True}
-DIST_ARGS = {"gaussian": ("mean", "std"), "uniform": ("lower", "upper")}
-DIST_DEFAULTS = {"mean": 0.0, "std": 1.0, "lower": 0.0, "upper": 1.0}
+# (argument name, default) in position order (lib/Bi/Action/*.pm)
+DIST_ARGS = {"gaussian": (("mean", 0.0), ("std", 1.0)),
+             "uniform": (("lower", 0.0), ("upper", 1.0)),
+             "exponential": (("lambda", 1.0),),
+             "gamma": (("shape", 1.0), ("scale", 1.0)),
+             "inverse_gamma": (("shape", 1.0), ("scale", 1.0)),
+             "beta": (("alpha", 1.0), ("beta", 1.0)),
+             "poisson": (("rate", 1.0),),
+             "negbin": (("mean", 1.0), ("shape", 1.0)),
+             "binomial": (("size", 1.0), ("prob", 0.5))}
+# distributions with a sampler here; the others are log-densities of the observation block
+SAMPLED = ("gaussian", "uniform", "exponential")
 UNIFORM_BLOCK = 0x40000000  # Philox block offset of uniform draws
 BOUNDARIES = ("none", "cyclic", "extended")
 # functions of the model language and their C++ names (overloaded for float and double)
@@ -84,10 +94,9 @@ class Var:
 
 @dataclass
 class Draw:
-    dist: str  # gaussian | uniform
-    log: bool
-    a: object  # mean / lower
-    b: object  # std / upper
+    dist: str  # a key of DIST_ARGS
+    log: bool  # log_gaussian / log_normal
+    args: list  # argument expressions in position order
     index: int  # first draw of the block's draws of the same kind (one per element)
 
 
@@ -166,8 +175,9 @@ def _err(line, msg):
     raise GenError(f"line {line}: {msg}")
 
 
-def _args(call, names, line):
+def _args(call, spec, line):
     """Positional then named arguments of an action (Bi::Action::process_args)."""
+    names = [n for n, _ in spec]
     out = {}
     if len(call.args) > len(names):
         _err(line, f"{call.name} takes at most {len(names)} positional arguments")
@@ -181,9 +191,9 @@ def _args(call, names, line):
         if n in out:
             _err(line, f"argument {n!r} of {call.name} given twice")
         out[n] = a
-    for n in names:
-        out.setdefault(n, Num(DIST_DEFAULTS[n], repr(DIST_DEFAULTS[n]), False))
-    return out
+    for n, d in spec:
+        out.setdefault(n, Num(d, repr(d), False))
+    return [out[n] for n in names]
 
 
 def _flatten(items):
@@ -358,24 +368,28 @@ class _Analyser:
             _err(a.line, "the right side of '~' must be a distribution")
         name = a.expr.name
         aliases = (alias,) if alias else ()
+        log = False
         if name in NORMAL_DISTS:
-            args = _args(a.expr, DIST_ARGS["gaussian"], a.line)
+            dist = "gaussian"
             log = NORMAL_DISTS[name]
             for n, val in a.expr.named:
                 if n == "log":
                     log = bool(_const_eval(val, self.consts, a.line))
-            kind, dist = "normal", "gaussian"
-        elif name == "uniform":
-            args = _args(a.expr, DIST_ARGS["uniform"], a.line)
-            log, kind, dist = False, "uniform", "uniform"
+        elif name in DIST_ARGS:
+            dist = name
         else:
             _err(a.line, f"distribution {name!r} is not supported "
-                         "(gaussian, normal, log_gaussian, log_normal, uniform)")
+                         f"({', '.join(sorted(set(DIST_ARGS) | set(NORMAL_DISTS)))})")
+        if dist not in SAMPLED and block != "observation":
+            _err(a.line, f"sampling from {dist} is not built ({', '.join(SAMPLED)} are); "
+                         f"{dist} is supported as an observation density")
+        args = _args(a.expr, DIST_ARGS[dist], a.line)
+        kind = "normal" if dist == "gaussian" else "uniform"
         k = counts.setdefault(kind, 0)
-        counts[kind] += v.size
-        names = DIST_ARGS[dist]
-        d = Draw(dist, log, args[names[0]], args[names[1]], k)
-        for x in (d.a, d.b):
+        if dist in SAMPLED:
+            counts[kind] += v.size
+        d = Draw(dist, log, args, k)
+        for x in d.args:
             self.check_refs(x, block, a.line, lc, aliases, v.dim)
         return d
 
@@ -612,6 +626,68 @@ class _Emitter:
         raise GenError(f"line {line}: cannot emit {e!r}")
 
 
+def _logdensity(d: Draw, a):
+    """Statements adding the log-density of xy to lp, in double
+    (share/tt/cpp/action/*.hpp.tt `logdensity`, share/src/bi/pdf/functor.hpp)."""
+    if d.dist == "gaussian" and d.log:
+        return [f"const double mu = {a[0]};", f"const double sg = {a[1]};",
+                "const double z = (log(xy) - mu) / sg;",
+                "lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);"]
+    if d.dist == "gaussian":
+        return [f"const double mu = {a[0]};", f"const double sg = {a[1]};",
+                "if (sg == 0.0) {",
+                "  lp = xy == mu ? __builtin_inf() : -__builtin_inf();",
+                "} else {",
+                "  const double z = (xy - mu) / sg;",
+                "  lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg);",
+                "}"]
+    if d.dist == "uniform":
+        return [f"const double mn = {a[0]}, mx = {a[1]};",
+                "if (xy >= mn && xy <= mx) lp += -log(mx - mn);",
+                "else lp = -__builtin_inf();"]
+    if d.dist == "exponential":
+        return [f"const double la = {a[0]};", "lp += log(la) - la * xy;"]
+    if d.dist == "gamma":
+        return [f"const double sh = {a[0]}, sc = {a[1]};",
+                "lp += (sh - 1.0) * log(xy) - xy / sc - (lgamma(sh) + sh * log(sc));"]
+    if d.dist == "inverse_gamma":  # the gamma log-density of 1/xy with scale 1/sc, - 2 log xy
+        return [f"const double sh = {a[0]}, sc = 1.0 / ({a[1]});",
+                "const double ix = 1.0 / xy;",
+                "lp += (sh - 1.0) * log(ix) - ix / sc - (lgamma(sh) + sh * log(sc)) - "
+                "2.0 * log(xy);"]
+    if d.dist == "beta":
+        return [f"const double al = {a[0]}, be = {a[1]};",
+                "lp += (al - 1.0) * log(xy) + (be - 1.0) * log(1.0 - xy) - "
+                "(lgamma(al) + lgamma(be) - lgamma(al + be));"]
+    if d.dist == "poisson":
+        return [f"const double la = {a[0]};",
+                "if (la == 0.0) lp += xy == 0.0 ? 0.0 : -__builtin_inf();",
+                "else lp += xy * log(la) - la - lgamma(xy + 1.0);"]
+    if d.dist == "negbin":
+        return [f"const double mu = {a[0]}, k = {a[1]};",
+                "if (xy == 0.0 && k == 0.0) {",
+                "} else if (xy == 0.0) {",
+                "  lp += k * (k < mu ? log(k / (k + mu)) : log1p(-mu / (k + mu)));",
+                "} else if (xy < 1e-10 * k) {",
+                "  const double q = k < mu ? log(k / (1.0 + k / mu)) : log(mu / (1.0 + mu / k));",
+                "  lp += xy * q - mu - lgamma(xy + 1.0) + log1p(xy * (xy - 1.0) / (2.0 * k));",
+                "} else {",
+                "  double r = lgamma(k + xy + 1.0) - lgamma(k + 1.0) - lgamma(xy + 1.0);",
+                "  if (k > 0.0) r += k * log(k / (k + mu));",
+                "  r += xy * log(mu / (k + mu));",
+                "  lp += r + log(k / (k + xy));",
+                "}"]
+    if d.dist == "binomial":
+        return [f"const double n = {a[0]}, pr = {a[1]};",
+                "if (!(xy == 0.0 && n == 0.0)) {",
+                "  double r = lgamma(n + 1.0) - lgamma(xy + 1.0) - lgamma(n - xy + 1.0);",
+                "  if (xy > 0.0) r += xy * log(pr);",
+                "  if (xy < n) r += (n - xy) * log1p(-pr);",
+                "  lp += r;",
+                "}"]
+    raise GenError(f"no log-density for {d.dist}")
+
+
 def _elements(g: GenModel, st: Step):
     """(element, bind) pairs an action covers."""
     v = g.vars[st.target]
@@ -626,15 +702,20 @@ def _zname(k):
 def _draw_value(em: _Emitter, d: Draw, i, bind, line):
     """Value of element i of a draw (Bi::Action::gaussian / uniform `sample`,
     share/tt/cpp/action/*.hpp.tt)."""
-    a, b = em.expr(d.a, bind, i, line), em.expr(d.b, bind, i, line)
+    if d.dist not in SAMPLED:
+        return '__builtin_nan("")'
+    a = [em.expr(x, bind, i, line) for x in d.args]
     if d.dist == "gaussian":
         z = _zname(d.index + i)
         z = z if em.ty == "T" else f"(double){z}"
-        v = f"({a} + ({b} * {z}))"
+        v = f"({a[0]} + ({a[1]} * {z}))"
         return f"exp({v})" if d.log else v
     u = f"u{d.index + i}"
     u = u if em.ty == "T" else f"(double){u}"
-    return f"({a} + (({b} - {a}) * {u}))"
+    if d.dist == "exponential":  # inverse distribution function
+        one = em.num(1.0)
+        return f"((-log({one} - {u})) / {a[0]})"
+    return f"({a[0]} + (({a[1]} - {a[0]}) * {u}))"
 
 
 def _target(g: GenModel, name, i, xname):
@@ -776,26 +857,10 @@ def generate_model(g: GenModel) -> str:
     for st in g.observation:
         d, v = st.draw, g.vars[st.target]
         for i, bind in _elements(g, st):
-            a, b = eo.expr(d.a, bind, i, st.line), eo.expr(d.b, bind, i, st.line)
             w("    {")
             w(f"      const double xy = y[{v.offset + i}];")
-            if d.dist == "gaussian":
-                w(f"      const double mu = {a};")
-                w(f"      const double sg = {b};")
-                if d.log:
-                    w("      const double z = (log(xy) - mu) / sg;")
-                    w("      lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);")
-                else:
-                    w("      if (sg == 0.0) {")
-                    w("        lp = xy == mu ? __builtin_inf() : -__builtin_inf();")
-                    w("      } else {")
-                    w("        const double z = (xy - mu) / sg;")
-                    w("        lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg);")
-                    w("      }")
-            else:
-                w(f"      const double mn = {a}, mx = {b};")
-                w("      if (xy >= mn && xy <= mx) lp += -log(mx - mn);")
-                w("      else lp = -__builtin_inf();")
+            for ln in _logdensity(d, [eo.expr(x, bind, i, st.line) for x in d.args]):
+                w("      " + ln)
             w("    }")
     w("    return lp;")
     w("  }")

@@ -123,7 +123,7 @@ def test_draw_counters():
     ("model M { dim n(4, boundary = 'odd')\n state x[n] }", "boundary"),
     ("model M { dim n(2.5)\n state x[n] }", "positive integer"),
     ("model M { dim n(40)\n dim k(30)\n state x[n], z[k] }", "at most 64"),
-    ("model M { state x\n sub transition { x ~ gamma(1.0) } }", "not supported"),
+    ("model M { state x\n sub transition { x ~ wishart(1.0) } }", "not supported"),
     ("model M { state x\n sub transition { ode { dx/dt = -x } } }", "RK4\\(3\\)"),
     ("model M { state x\n sub transition { ode(alg = 'RK4') { dx/dt = y } } }", "unknown name"),
     ("model M { state x; obs y\n sub transition { x <- y } }", "cannot be read"),
@@ -176,13 +176,12 @@ def test_netcdf_output_uses_the_model_files_variables(tmp_path):
     f.close()
 
 
-def test_generated_header_is_standalone_cxx(tmp_path):
-    """The generated text is what build() compiles: it parses as C++ with
-    minimal declarations of what the kernels provide (host syntax check)."""
-    gs = codegen.load_models(sorted(glob.glob(os.path.join(BI, "*.bi"))))
-    stub = r"""
+# Minimal host declarations of what the kernels provide, so that generated
+# functors compile with g++ (syntax checks and host evaluation of densities).
+HOST_STUB = r"""
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #define __device__
 #define __forceinline__ inline
 #define BIHIP_RK4 0
@@ -190,14 +189,14 @@ def test_generated_header_is_standalone_cxx(tmp_path):
 #define BIHIP_EINVAL 2
 #define BIHIP_MODEL_GENERATED 64
 using std::exp; using std::log; using std::sqrt; using std::floor; using std::fmin;
-using std::pow; using std::fabs;
+using std::pow; using std::fabs; using std::lgamma; using std::log1p;
 namespace bihip {
 struct ModelArgs { double h0; double par[16]; };
 struct bihip_model_ { int model, precision; };
 struct u32x4 { uint32_t x, y, z, w; };
 struct GenInfo { const char* name; int d, nobs, integrator; double h, atol, rtol, delta;
   const char* const* names[5]; const int* sizes[5]; const char* const* state_dims; };
-constexpr double HALF_LOG_TWO_PI = 0.9189385332046727;
+constexpr double HALF_LOG_TWO_PI = 0.91893853320467274178;
 constexpr uint32_t TAG_INIT = 1, TAG_NOISE = 2, TAG_OBS = 6;
 inline int set_error(int c, const char*) { return c; }
 inline double gen_scaled(double t, double d) { return t / d; }
@@ -214,8 +213,26 @@ template <class M, class G, class T> int integrate_dopri5(const M&, const G&, T 
 }
 using bihip_model = bihip::bihip_model_;
 """
+
+
+def _gxx(tmp_path, text, run=False):
     src = tmp_path / "gen.cpp"
-    src.write_text(stub + codegen.generate(gs) + r"""
+    src.write_text(HOST_STUB + text)
+    exe = tmp_path / "gen"
+    args = ["g++", "-std=c++17", "-O1", "-Wall", str(src), "-o", str(exe)]
+    if not run:
+        args = ["g++", "-std=c++17", "-fsyntax-only", "-Wall", str(src)]
+    r = subprocess.run(args, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    if run:
+        return subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+
+
+def test_generated_header_is_standalone_cxx(tmp_path):
+    """The generated text is what build() compiles: it parses as C++ with
+    minimal declarations of what the kernels provide (host syntax check)."""
+    gs = codegen.load_models(sorted(glob.glob(os.path.join(BI, "*.bi"))))
+    _gxx(tmp_path, codegen.generate(gs) + r"""
 namespace bihip {
 struct Runner { template <class M> int run() {
   M m(ModelArgs{});
@@ -231,51 +248,94 @@ struct Runner { template <class M> int run() {
 int check(const bihip_model* m) { Runner r; return with_generated(m, r); }
 }
 """)
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", str(src)],
-                       capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr[-3000:]
 
 
-def test_vectors_over_dimensions():
-    """Dimensions with cyclic, extended and plain boundaries; actions over
-    whole vectors and over aliased elements, with index arithmetic."""
-    g = _gen("""model M {
-      const K = 5
-      dim n(size = K, boundary = 'cyclic')
-      dim e(size = 3, boundary = 'extended')
-      dim h(size = 2)
-      param a[h], b
-      state x[n], u[e], s
-      noise w[n]
-      obs y[h]
-      sub initial { x[i] ~ gaussian(i, b)  u <- 1.0  s <- a[1] }
-      sub transition(delta = 0.5) {
-        w ~ gaussian()
-        x[i] <- x[i-1] + w[i]
-        u[k] <- u[k+1] + u[k-1]
-        ode(alg = 'RK4', h = 0.1) { dx[i]/dt = x[i+1] - x[i-2] }
-      }
-      sub observation { y[j] ~ gaussian(x[2*j+1], a[j]) }
-    }""")
-    assert (g.D, g.NOBS, g.delta) == (9, 2, 0.5)
-    assert [(v.name, v.offset, v.size) for v in g.vars.values() if v.kind == "state"] == \
-        [("x", 0, 5), ("u", 5, 3), ("s", 8, 1)]
-    assert g.normals["transition"] == 5 and g.normals["initial"] == 5
+# one observation of each distribution, given a state x and params a, b
+DENSITIES = {
+    "poisson": "poisson(x)",
+    "negbin": "negbin(x, a)",
+    "binomial": "binomial(b, x/(1.0 + x))",
+    "gamma": "gamma(a, x)",
+    "inverse_gamma": "inverse_gamma(a, x)",
+    "exponential": "exponential(x)",
+    "beta": "beta(a, x)",
+    "gaussian": "gaussian(x, a)",
+    "log_normal": "log_normal(x, a)",
+    "uniform": "uniform(x, x + a)",
+}
+
+
+def _scipy_logpdf(name, yv, x, a, b):
+    from scipy import stats
+    if name == "poisson":
+        return stats.poisson.logpmf(yv, x)
+    if name == "negbin":  # mean x, shape a: n = a, p = a / (a + x)
+        return stats.nbinom.logpmf(yv, a, a / (a + x))
+    if name == "binomial":
+        return stats.binom.logpmf(yv, b, x / (1 + x))
+    if name == "gamma":
+        return stats.gamma.logpdf(yv, a, scale=x)
+    if name == "inverse_gamma":
+        return stats.invgamma.logpdf(yv, a, scale=x)
+    if name == "exponential":
+        return stats.expon.logpdf(yv, scale=1 / x)
+    if name == "beta":
+        return stats.beta.logpdf(yv, a, x)
+    if name == "gaussian":
+        return stats.norm.logpdf(yv, x, a)
+    if name == "log_normal":
+        return stats.lognorm.logpdf(yv, a, scale=np.exp(x))
+    if name == "uniform":
+        return stats.uniform.logpdf(yv, x, a)
+    raise KeyError(name)
+
+
+def test_observation_densities_against_scipy(tmp_path):
+    """The generated log-densities (share/src/bi/pdf/functor.hpp formulas),
+    compiled for the host and evaluated on a grid, against scipy.stats."""
+    models = []
+    for name, dist in DENSITIES.items():
+        models.append(codegen.analyse_text(
+            f"model D_{name} {{ param a, b\n state x\n obs y\n"
+            f" sub observation {{ y ~ {dist} }} }}"))
+    cases = {"poisson": ([0, 1, 3, 17], [0.5, 4.0]), "negbin": ([0, 1, 5, 40], [0.7, 12.0]),
+             "binomial": ([0, 2, 7], [0.3, 2.5]), "gamma": ([0.2, 1.5, 6.0], [0.8, 3.0]),
+             "inverse_gamma": ([0.2, 1.5, 6.0], [0.8, 3.0]),
+             "exponential": ([0.0, 0.4, 3.0], [0.5, 2.0]), "beta": ([0.1, 0.5, 0.93], [0.6, 2.5]),
+             "gaussian": ([-1.0, 0.3, 2.0], [0.1, 1.2]), "log_normal": ([0.2, 1.0, 4.0], [-0.3, 0.9]),
+             "uniform": ([0.35, 0.9, 2.0], [0.3, 0.8])}
+    a, b = 1.7, 7.0
+    body = ["int main() {", "  bihip::ModelArgs ma{}; ma.par[0] = %r; ma.par[1] = %r;" % (a, b)]
+    for g in models:
+        name = g.name[2:]
+        ys, xs = cases[name]
+        for xv in xs:
+            for yv in ys:
+                body.append(f"  {{ bihip::{g.cname}<double> m(ma); double x[1] = {{{xv!r}}};"
+                            f" double y[1] = {{{float(yv)!r}}};"
+                            f' std::printf("{name} %.17g %.17g %.17g\\n", x[0], y[0],'
+                            f" m.obs(x, y)); }}")
+    body.append("  return 0;\n}")
+    src = "\n".join(codegen.generate_model(g) for g in models)
+    out = _gxx(tmp_path, "namespace bihip {\n" + src + "\n}\n" + "\n".join(body), run=True)
+    n = 0
+    for line in out.splitlines():
+        name, xv, yv, lp = line.split()
+        ref = _scipy_logpdf(name, float(yv), float(xv), a, b)
+        assert float(lp) == pytest.approx(ref, rel=1e-11, abs=1e-11), (name, xv, yv)
+        n += 1
+    assert n == sum(len(y) * len(x) for y, x in cases.values())
+
+
+def test_sampling_only_where_built():
+    with pytest.raises(GenError, match="sampling from gamma"):
+        _gen("model M { state x\n noise w\n sub transition { w ~ gamma(2.0) } }")
+    g = _gen("model M { param l\n state x\n noise w\n obs c\n"
+             " sub transition { w ~ exponential(l)\n x <- x + w }\n"
+             " sub observation { c ~ poisson(x) } }")
     src = codegen.generate_model(g)
-    # initial: x[i] ~ gaussian(i, b), the alias as a number; s from the second element of a
-    assert "t_[3] = (3.0 + (p_b * (double)zq0[3]));" in src
-    assert "t_[4] = (4.0 + (p_b * (double)zq1[0]));" in src
-    assert "xd[8] = p_a[1];" in src
-    # cyclic wrap of x[i-1] and x[i-2]; a vector action reads the values from before it
-    assert "t_[0] = (x[4] + w_w[0]);" in src and "x[0] = t_[0];" in src
-    assert "k[0] = (x[1] - x[3]);" in src and "k[4] = (x[0] - x[2]);" in src
-    # extended boundary clamps u[k-1] at 0 and u[k+1] at 2 (state offset 5)
-    assert "t_[0] = (x[6] + x[5]);" in src and "t_[2] = (x[7] + x[6]);" in src
-    # states without an equation keep their value in the ode
-    assert "k[5] = T(0);" in src and "k[8] = T(0);" in src
-    # observations of x[1] and x[3] with the vector param's elements as sd
-    assert "const double mu = (double)x[3];" in src and "const double sg = p_a[1];" in src
-    assert "y[1] = ((double)x[3] + (p_a[1] * (double)zq0[1]));" in src
+    assert "w_w = ((-log(T(1.0) - u0)) / T(p_l));" in src
+    assert 'y[0] = __builtin_nan("");' in src  # no Poisson sampler: NaN observation samples
 
 
 def test_netcdf_output_of_a_vector_model(tmp_path):

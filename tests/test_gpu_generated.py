@@ -252,3 +252,86 @@ def test_generated_lorenz96_fp32_equals_handwritten(ctx, orc):
     zg, og, _ = _run(ctx, g, y, P, 0.05, 4, ess_rel=0.0, record=False)
     assert not og["resampled"].any()
     assert zg == pytest.approx(zh, rel=1e-5)
+
+
+def _lg_kalman(y, lam, noise_sd, obs_sd, m0, s0, dt, fires):
+    """Exact log-likelihood of dx/dt = -lam x over every interval of length dt,
+    plus N(0, noise_sd^2) at the intervals flagged in fires, y_k ~ N(x, obs_sd^2)."""
+    a = np.exp(-lam * dt)
+    m, v, ll = m0, s0 * s0, 0.0
+    for k, yk in enumerate(y):
+        m, v = a * m, a * a * v + (noise_sd ** 2 if fires[k] else 0.0)
+        s = v + obs_sd * obs_sd
+        ll += -0.5 * np.log(2 * np.pi * s) - 0.5 * (yk - m) ** 2 / s
+        gain = v / s
+        m, v = m + gain * (yk - m), (1 - gain) * v
+    return ll
+
+
+def test_generated_ode_with_delta_between_observations(ctx):
+    """LinearGaussian.bi (delta = 1) observed every 0.5: the ode runs over every
+    interval, the noise fires only on those starting at whole times."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, atol=1e-10, rtol=1e-10,
+                             h0=0.25, **par)
+    T, dt = 60, 0.5
+    fires = [(k % 2) == 0 for k in range(T)]
+    rng = np.random.default_rng(8)
+    x, y = rng.normal(), np.empty(T)
+    for k in range(T):
+        x = np.exp(-0.5 * dt) * x + (0.5 * rng.normal() if fires[k] else 0.0)
+        y[k] = x + 0.5 * rng.normal()
+    exact = _lg_kalman(y, 0.5, 0.5, 0.5, 0.0, 1.0, dt, fires)
+    zs = [_run(ctx, g, y, 1 << 16, dt, s, record=False)[0] for s in (1, 2)]
+    assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
+
+
+def _sir_counts(T, beta, gamma, k, sigma, seed):
+    """Incidence counts simulated from SIR.bi's equations in numpy (RK4 h = 0.1)."""
+    rng = np.random.default_rng(seed)
+    N, x = 1000.0, np.array([990.0, 10.0, 0.0, 0.0])
+
+    def f(x, w):
+        inf = beta * np.exp(w) * x[0] * x[1] / N
+        return np.array([-inf, inf - gamma * x[1], gamma * x[1], inf])
+
+    y = np.empty(T)
+    for t in range(T):
+        w = sigma * rng.normal()
+        x[3] = 0.0
+        for _ in range(10):
+            k1 = f(x, w)
+            k2 = f(x + 0.05 * k1, w)
+            k3 = f(x + 0.05 * k2, w)
+            k4 = f(x + 0.1 * k3, w)
+            x = x + 0.1 / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        y[t] = rng.negative_binomial(k, k / (k + x[3]))
+    return y
+
+
+def test_generated_sir_negbin_counts(ctx, torch_gpu):
+    """SIR.bi: the negative-binomial count density on the device against
+    scipy.stats.nbinom (mean Z, shape k), then the filter on counts simulated
+    from the model: two independent estimates of the log-likelihood agree."""
+    from scipy import stats
+    par = dict(beta=0.6, gamma=0.25, k=8.0, sigma=0.1)
+    m = capi.generated_model("SIR", precision=capi.F64, **par)
+    rng = np.random.default_rng(2)
+    P = 2000
+    X0 = np.stack([rng.uniform(100, 900, P), rng.uniform(1, 300, P), rng.uniform(0, 500, P),
+                   rng.uniform(0.5, 60.0, P)])
+    X = torch_gpu.from_numpy(X0.copy()).cuda()
+    for yv in (0.0, 3.0, 25.0, 80.0):
+        lws = torch_gpu.zeros(P, dtype=torch_gpu.float64, device="cuda")
+        yd = torch_gpu.tensor([yv], dtype=torch_gpu.float64, device="cuda")
+        capi.check(capi.lib().bihip_obs_logdensity(ctx.h, ctypes.byref(m),
+                                                   ctypes.c_void_p(yd.data_ptr()), P,
+                                                   ctypes.c_void_p(X.data_ptr()), P,
+                                                   ctypes.c_void_p(lws.data_ptr())))
+        ref = stats.nbinom.logpmf(yv, 8.0, 8.0 / (8.0 + X0[3]))
+        np.testing.assert_allclose(host(lws), ref, rtol=1e-12, atol=1e-12)
+    y = _sir_counts(30, seed=11, **par)
+    zs = [_run(ctx, m, y, 1 << 16, 1.0, s, record=False) for s in (1, 2)]
+    assert all(np.isfinite(z) for z, _, _ in zs)
+    assert abs(zs[0][0] - zs[1][0]) < 0.5, [z for z, _, _ in zs]
+    assert (zs[0][1]["ess"] > 100).all()
