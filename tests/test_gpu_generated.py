@@ -335,3 +335,54 @@ def test_generated_sir_negbin_counts(ctx, torch_gpu):
     assert all(np.isfinite(z) for z, _, _ in zs)
     assert abs(zs[0][0] - zs[1][0]) < 0.5, [z for z, _, _ in zs]
     assert (zs[0][1]["ess"] > 100).all()
+
+
+def test_generated_model_with_schedule_and_inputs(ctx):
+    """LinearGaussian.bi over an irregular schedule with unobserved steps and
+    per-step inputs: the ode decays over each interval, the noise fires at the
+    whole times inside it (delta = 1), the decay rate of step k is lam_k; the
+    likelihood is the Kalman filter's over the same pieces."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, atol=1e-10, rtol=1e-10,
+                             h0=0.25, **par)
+    rng = np.random.default_rng(13)
+    T = 50
+    times = np.concatenate([[0.0], np.cumsum(rng.uniform(0.3, 1.6, T))])
+    observed = (rng.uniform(size=T) > 0.25).astype(np.int32)
+    lams = rng.uniform(0.2, 0.9, T)
+
+    def pieces(t1, t2):  # [a, b) parts of [t1, t2) split at whole times; fires at whole starts
+        out, a = [], t1
+        while a < t2:
+            b = min(np.floor(a) + 1.0, t2)
+            out.append((a, b, a == np.floor(a)))
+            a = b
+        return out
+
+    # truth and observations, then the exact likelihood
+    x, y = rng.normal(), np.zeros(T)
+    for k in range(T):
+        for a, b, on in pieces(times[k], times[k + 1]):
+            x = np.exp(-lams[k] * (b - a)) * x + (0.5 * rng.normal() if on else 0.0)
+        y[k] = x + 0.5 * rng.normal()
+    m, v, exact = 0.0, 1.0, 0.0
+    for k in range(T):
+        for a, b, on in pieces(times[k], times[k + 1]):
+            e = np.exp(-lams[k] * (b - a))
+            m, v = e * m, e * e * v + (0.25 if on else 0.0)
+        if observed[k]:
+            s = v + 0.25
+            exact += -0.5 * np.log(2 * np.pi * s) - 0.5 * (y[k] - m) ** 2 / s
+            gain = v / s
+            m, v = m + gain * (y[k] - m), (1 - gain) * v
+    zs = []
+    for seed in (1, 2):
+        pf = capi.ParticleFilter(ctx, g, 1 << 16, T, 1.0, seed, ess_rel=0.5)
+        pf.set_schedule(times, observed)
+        pf.set_inputs(pf.inputs_from(lam=lams))
+        pf.set_observations(y)
+        pf.init()
+        pf.step(T)
+        zs.append(pf.term())
+        pf.close()
+    assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
