@@ -117,6 +117,16 @@ const char* bihip_model_var_name(const bihip_model* m, int kind, int i);
  * the end).  Vector variables take consecutive par slots, state rows and
  * observation values. */
 int bihip_model_var_size(const bihip_model* m, int kind, int i);
+/* The prior of the first n param elements of a generated model, from its
+ * parameter block (Model::parameterSamples / parameterLogDensities,
+ * share/tt/cpp/model.hpp.tt): a draw with the keyed stream {index, 0, THETA, 0}
+ * (host), and the log-density of theta_host (-inf outside the support).
+ * Gaussian, log-gaussian, uniform, exponential, gamma, inverse gamma and
+ * beta priors with constant arguments. */
+int bihip_model_prior_sample(const bihip_model* m, uint64_t seed, uint32_t index, int n,
+                             double* theta_host);
+int bihip_model_prior_logdensity(const bihip_model* m, int n, const double* theta_host,
+                                 double* lp);
 /* The model's name for a generated model, NULL otherwise. */
 const char* bihip_model_name(const bihip_model* m);
 
@@ -475,6 +485,10 @@ typedef struct {
   double prior_mean[8], prior_sd[8], prop_sd[8];
   int use_graph;  /* inner steps replayed as hipGraphs */
   int block;      /* inner workgroup size, 0 = autotune */
+  /* 1: the prior of theta is the parameter block of the generated model's file
+   * (each of its first ntheta params needs a prior with positive support;
+   * prior_mean and prior_sd are unused); the random walk stays on log theta */
+  int prior_from_model;
 } bihip_smc2_config;
 
 int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,

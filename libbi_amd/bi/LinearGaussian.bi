@@ -8,6 +8,14 @@ model LinearGaussian {
   noise w
   obs y
 
+  sub parameter {
+    lam ~ gamma(2.0, 0.25)
+    noise_sd ~ inverse_gamma(3.0, 1.0)
+    obs_sd ~ log_normal(log(0.5), 0.2)
+    m0 ~ gaussian(0.0, 1.0)
+    s0 ~ uniform(0.5, 2.0)
+  }
+
   sub initial {
     x ~ gaussian(m0, s0)
   }

@@ -11,6 +11,13 @@ model SIR {
   noise w
   obs incidence
 
+  sub parameter {
+    beta ~ gamma(4.0, 0.15)
+    gamma ~ beta(2.0, 6.0)
+    k ~ exponential(0.1)
+    sigma ~ uniform(0.0, 0.3)
+  }
+
   sub initial {
     S <- N - 10.0
     I <- 10.0

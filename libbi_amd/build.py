@@ -24,7 +24,7 @@ GEN = os.path.join(OBJ, "gen")
 SOURCES = ["transition.hip", "reduce.hip", "scan.hip", "resample.hip", "batch.hip", "capi.cpp",
            "batch_capi.cpp", "smc2.cpp",
            "mlp.hip", "mlp_capi.cpp", "surrogate.hip",
-           "guided.hip", "netcdf.cpp"]
+           "guided.hip", "netcdf.cpp", "prior.cpp"]
 ARCH = os.environ.get("BIHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -38,18 +38,21 @@ CFLAGS = [
 
 def generate_models():
     """Writes the functors of the model files in libbi_amd/bi into the build
-    folder (generated_models.hpp, included by the kernels)."""
+    folder (generated_models.hpp, included by the kernels) and the priors of
+    their parameter blocks (generated_priors.hpp, host data)."""
     sys.path.insert(0, ROOT)
     from libbi_amd.dsl import codegen
     paths = glob.glob(os.path.join(MODELS, "*.bi"))
-    text = codegen.generate(codegen.load_models(paths))
+    models = codegen.load_models(paths)
     os.makedirs(GEN, exist_ok=True)
-    out = os.path.join(GEN, "generated_models.hpp")
-    old = open(out).read() if os.path.exists(out) else None
-    if old != text:
-        with open(out, "w") as f:
-            f.write(text)
-    return out
+    for name, text in (("generated_models.hpp", codegen.generate(models)),
+                       ("generated_priors.hpp", codegen.generate_priors(models))):
+        out = os.path.join(GEN, name)
+        old = open(out).read() if os.path.exists(out) else None
+        if old != text:
+            with open(out, "w") as f:
+                f.write(text)
+    return GEN
 
 
 def build(verbose=False):

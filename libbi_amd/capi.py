@@ -65,6 +65,7 @@ EXPORTS = [
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
     "bihip_model_generated", "bihip_model_var_name", "bihip_model_name", "bihip_model_var_size",
+    "bihip_model_prior_sample", "bihip_model_prior_logdensity",
 ]
 
 
@@ -105,6 +106,7 @@ class SMC2Config(ctypes.Structure):
         ("t0", ctypes.c_double), ("dt", ctypes.c_double), ("seed", ctypes.c_uint64),
         ("prior_mean", ctypes.c_double * 8), ("prior_sd", ctypes.c_double * 8),
         ("prop_sd", ctypes.c_double * 8), ("use_graph", ctypes.c_int), ("block", ctypes.c_int),
+        ("prior_from_model", ctypes.c_int),
     ]
 
 
@@ -267,6 +269,8 @@ def _declare(L):
     L.bihip_model_name.argtypes = [P]
     L.bihip_model_name.restype = c.c_char_p
     L.bihip_model_var_size.argtypes = [P, c.c_int, c.c_int]
+    L.bihip_model_prior_sample.argtypes = [P, c.c_uint64, c.c_uint32, c.c_int, P]
+    L.bihip_model_prior_logdensity.argtypes = [P, c.c_int, P, P]
     L.bihip_uniform_host.argtypes = [u64, u32, u32, u32, u32]
     L.bihip_uniform_host.restype = c.c_double
     L.bihip_batch_particles.argtypes = [P]
@@ -390,6 +394,23 @@ def generated_model(name, precision=F64, integrator=None, h0=None, atol=None, rt
     if nsteps is not None:
         m.nsteps = nsteps
     return m
+
+
+def prior_sample(m, n, seed, index):
+    """Draw `index` of the first n param elements from the model file's prior."""
+    th = np.zeros(n)
+    check(lib().bihip_model_prior_sample(ctypes.byref(m), seed, index, n,
+                                         th.ctypes.data_as(ctypes.c_void_p)))
+    return th
+
+
+def prior_logdensity(m, theta):
+    th = np.ascontiguousarray(theta, np.float64)
+    lp = ctypes.c_double()
+    check(lib().bihip_model_prior_logdensity(ctypes.byref(m), len(th),
+                                             th.ctypes.data_as(ctypes.c_void_p),
+                                             ctypes.byref(lp)))
+    return lp.value
 
 
 def nobs(m):
@@ -724,15 +745,20 @@ class SMC2:
     def __init__(self, ctx, model, N, T, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
                  nmoves=1, resampler=SYSTEMATIC, ess_rel_inner=0.5, ess_rel_outer=0.5, t0=0.0,
                  use_graph=False, block=0):
-        nt = len(prior_mean)
+        """prior_mean = prior_sd = None: the prior of theta (the first len(prop_sd)
+        params) is the parameter block of the generated model's file."""
+        from_model = prior_mean is None
+        nt = len(prop_sd)
         self.N = np.ascontiguousarray(N, dtype=np.int32)
         B = len(self.N)
         cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=nmoves, resampler=resampler,
                          essRel_inner=ess_rel_inner, essRel_outer=ess_rel_outer, t0=t0, dt=dt,
-                         seed=seed, use_graph=1 if use_graph else 0, block=block)
+                         seed=seed, use_graph=1 if use_graph else 0, block=block,
+                         prior_from_model=1 if from_model else 0)
         for i in range(nt):
-            cfg.prior_mean[i] = prior_mean[i]
-            cfg.prior_sd[i] = prior_sd[i]
+            if not from_model:
+                cfg.prior_mean[i] = prior_mean[i]
+                cfg.prior_sd[i] = prior_sd[i]
             cfg.prop_sd[i] = prop_sd[i]
         self.cfg = cfg
         h = ctypes.c_void_p()
@@ -781,15 +807,19 @@ class PMMH:
 
     def __init__(self, ctx, model, N, T, C, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
                  resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, use_graph=False, block=0):
-        nt = len(prior_mean)
+        """prior_mean = prior_sd = None: the model file's parameter block is the prior."""
+        from_model = prior_mean is None
+        nt = len(prop_sd)
         self.N = np.ascontiguousarray(N, dtype=np.int32)
         B = len(self.N)
         cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=0, resampler=resampler,
                          essRel_inner=ess_rel, essRel_outer=0.0, t0=t0, dt=dt, seed=seed,
-                         use_graph=1 if use_graph else 0, block=block)
+                         use_graph=1 if use_graph else 0, block=block,
+                         prior_from_model=1 if from_model else 0)
         for k in range(nt):
-            cfg.prior_mean[k] = prior_mean[k]
-            cfg.prior_sd[k] = prior_sd[k]
+            if not from_model:
+                cfg.prior_mean[k] = prior_mean[k]
+                cfg.prior_sd[k] = prior_sd[k]
             cfg.prop_sd[k] = prop_sd[k]
         self.cfg = cfg
         h = ctypes.c_void_p()

@@ -389,6 +389,20 @@ int launch_guided(const GuidedLaunch& L, hipStream_t s);
 void philox_host_block(uint64_t seed, uint32_t index, uint32_t step, uint32_t tag,
                        uint32_t batch, uint32_t block, uint32_t out[4]);
 
+// ---- priors of a generated model's parameter block (prior.cpp), for the
+// first n param elements: prior_check returns BIHIP_OK when every one has a
+// prior with positive support (SMC^2 and PMMH move in log space);
+// prior_sample draws them with the keyed stream {index, 0, THETA, 0};
+// prior_logdensity is their log-density (-inf outside the support).
+struct GenPrior {
+  int dist;  // 0 gaussian, 1 log-gaussian, 2 uniform, 3 exponential, 4 gamma,
+             // 5 inverse gamma, 6 beta; -1 none
+  double a, b;
+};
+int prior_check(const bihip_model* m, int n, bool positive);
+void prior_sample(const bihip_model* m, int n, uint64_t seed, uint32_t index, double* theta);
+double prior_logdensity(const bihip_model* m, int n, const double* theta);
+
 // ---- context access for the other host files (capi.cpp)
 int ctx_enter_device(bihip_ctx* ctx);     // makes the context's device current
 hipStream_t ctx_stream_of(bihip_ctx* ctx);

@@ -137,13 +137,50 @@ bool outer_ancestors(const std::vector<double>& lw, double a, std::vector<int>& 
   return true;
 }
 
-double log_prior(const bihip_smc2_config& c, const double* lth) {
+// log prior density of lth (up to a constant): the Gaussian on lth, or the
+// model file's prior of theta = exp(lth) times the Jacobian exp(sum lth)
+double log_prior(const bihip_model& m, const bihip_smc2_config& c, const double* lth) {
   double lp = 0.0;
+  if (c.prior_from_model) {
+    double th[8];
+    for (int i = 0; i < c.ntheta; ++i) {
+      th[i] = std::exp(lth[i]);
+      lp += lth[i];
+    }
+    return prior_logdensity(&m, c.ntheta, th) + lp;
+  }
   for (int i = 0; i < c.ntheta; ++i) {
     const double z = (lth[i] - c.prior_mean[i]) / c.prior_sd[i];
     lp += -0.5 * z * z;
   }
   return lp;
+}
+
+// theta-particle j's draw from the prior, as lth
+void prior_lth(const bihip_model& m, const bihip_smc2_config& c, int j, double* lth) {
+  const int nt = c.ntheta;
+  if (c.prior_from_model) {
+    double th[8];
+    prior_sample(&m, nt, c.seed, (uint32_t)j, th);
+    for (int i = 0; i < nt; ++i) lth[i] = std::log(th[i]);
+    return;
+  }
+  double z[8];
+  host_normals(c.seed, (uint32_t)j, 0u, BIHIP_TAG_THETA, 0u, nt, z);
+  for (int i = 0; i < nt; ++i) lth[i] = c.prior_mean[i] + c.prior_sd[i] * z[i];
+}
+
+int check_prior(const bihip_model* m, const bihip_smc2_config& c) {
+  if (c.prior_from_model) {
+    if (int rc = prior_check(m, c.ntheta, true)) return rc;
+    for (int i = 0; i < c.ntheta; ++i)
+      if (!(c.prop_sd[i] >= 0.0)) return set_error(BIHIP_EINVAL, "prop_sd must be non-negative");
+    return BIHIP_OK;
+  }
+  for (int i = 0; i < c.ntheta; ++i)
+    if (!(c.prior_sd[i] > 0.0) || !(c.prop_sd[i] >= 0.0))
+      return set_error(BIHIP_EINVAL, "prior_sd must be positive and prop_sd non-negative");
+  return BIHIP_OK;
 }
 
 int filter_logliks(bihip_batch* b, int B, std::vector<double>& ll) {
@@ -190,8 +227,8 @@ int smc2_move(bihip_smc2* s, int k, int* nacc) {
       } else {
         const double u =
             host_uniform(c.seed, (uint32_t)j, (uint32_t)k, BIHIP_TAG_ACCEPT, (uint32_t)m);
-        const double logratio = ll2[j] - ll1[j] + log_prior(c, &s->lth2[(size_t)j * nt]) -
-                                log_prior(c, &s->lth[(size_t)j * nt]);
+        const double logratio = ll2[j] - ll1[j] + log_prior(s->model, c, &s->lth2[(size_t)j * nt]) -
+                                log_prior(s->model, c, &s->lth[(size_t)j * nt]);
         accept = std::log(u) < logratio;
       }
       acc[j] = accept ? 1 : 0;
@@ -219,9 +256,7 @@ int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   if (c.nmoves < 0) return set_error(BIHIP_EINVAL, "nmoves must be >= 0");
   if (!(c.essRel_outer >= 0.0 && c.essRel_outer <= 1.0))
     return set_error(BIHIP_EINVAL, "essRel_outer must lie in [0,1]");
-  for (int i = 0; i < c.ntheta; ++i)
-    if (!(c.prior_sd[i] > 0.0) || !(c.prop_sd[i] >= 0.0))
-      return set_error(BIHIP_EINVAL, "prior_sd must be positive and prop_sd non-negative");
+  if (int rc = check_prior(m, c)) return rc;
   if ((uint64_t)c.B * (1 + (uint64_t)c.nmoves * (c.T + 1)) > (1u << 24))
     return set_error(BIHIP_EINVAL, "B (1 + nmoves (T + 1)) random-stream ids exceed 2^24");
   int maxN = 0;
@@ -288,11 +323,9 @@ int bihip_smc2_run(bihip_smc2* s) {
   int rc;
   // init (:322-344): theta from the prior, every inner filter from its initial law
   for (int j = 0; j < B; ++j) {
-    double z[8];
-    host_normals(c.seed, (uint32_t)j, 0u, BIHIP_TAG_THETA, 0u, nt, z);
+    prior_lth(s->model, c, j, &s->lth[(size_t)j * nt]);
     for (int i = 0; i < nt; ++i) {
       const size_t q = (size_t)j * nt + i;
-      s->lth[q] = c.prior_mean[i] + c.prior_sd[i] * z[i];
       s->theta[q] = std::exp(s->lth[q]);
     }
   }
@@ -373,6 +406,7 @@ int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* 
 }  // extern "C"
 
 struct bihip_pmmh {
+  bihip_model model{};
   bihip_smc2_config cfg{};
   int C = 0;
   bihip_batch* batch = nullptr;
@@ -405,12 +439,11 @@ int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   const bihip_smc2_config& c = *cfg;
   if (c.B <= 0 || c.T <= 0 || C <= 0) return set_error(BIHIP_EINVAL, "need B, T, C > 0");
   if (c.ntheta < 1 || c.ntheta > 8) return set_error(BIHIP_EINVAL, "ntheta must lie in [1,8]");
-  for (int i = 0; i < c.ntheta; ++i)
-    if (!(c.prior_sd[i] > 0.0) || !(c.prop_sd[i] >= 0.0))
-      return set_error(BIHIP_EINVAL, "prior_sd must be positive and prop_sd non-negative");
+  if (int rc = check_prior(m, c)) return rc;
   if ((uint64_t)c.B * ((uint64_t)C + 1) > (1u << 24))
     return set_error(BIHIP_EINVAL, "B (C + 1) random-stream ids exceed 2^24");
   bihip_pmmh* s = new bihip_pmmh();
+  s->model = *m;
   s->cfg = c;
   s->C = C;
   s->theta.assign((size_t)C * c.B * c.ntheta, 0.0);
@@ -456,11 +489,7 @@ int bihip_pmmh_run(bihip_pmmh* s) {
   const bihip_smc2_config& c = s->cfg;
   const int B = c.B, nt = c.ntheta;
   std::vector<double> lth((size_t)B * nt), lth2((size_t)B * nt), ll, ll2;
-  for (int j = 0; j < B; ++j) {
-    double z[8];
-    host_normals(c.seed, (uint32_t)j, 0u, BIHIP_TAG_THETA, 0u, nt, z);
-    for (int i = 0; i < nt; ++i) lth[(size_t)j * nt + i] = c.prior_mean[i] + c.prior_sd[i] * z[i];
-  }
+  for (int j = 0; j < B; ++j) prior_lth(s->model, c, j, &lth[(size_t)j * nt]);
   int rc = pmmh_filter(s, lth, 0u, ll);
   if (rc) return rc;
   std::fill(s->accepted.begin(), s->accepted.end(), 1);
@@ -480,8 +509,8 @@ int bihip_pmmh_run(bihip_pmmh* s) {
         } else if (!std::isfinite(ll[j])) {
           accept = true;
         } else {
-          const double logratio = ll2[j] - ll[j] + log_prior(c, &lth2[(size_t)j * nt]) -
-                                  log_prior(c, &lth[(size_t)j * nt]);
+          const double logratio = ll2[j] - ll[j] + log_prior(s->model, c, &lth2[(size_t)j * nt]) -
+                                  log_prior(s->model, c, &lth[(size_t)j * nt]);
           const double u =
               host_uniform(c.seed, (uint32_t)j, (uint32_t)it, BIHIP_TAG_ACCEPT, 0u);
           accept = std::log(u) < logratio;

@@ -24,12 +24,18 @@ line otherwise):
     `ode(alg, h, atoler, rtoler)` block of `dx/dt = ...` equations, run in the
     order written; discrete actions fire at multiples of delta
     (share/src/bi/state/Schedule.hpp:230-292), the ODE covers every interval;
-  * `sub observation`: `~` actions on obs variables;
-  * distributions gaussian/normal, log_gaussian/log_normal and uniform, with
-    positional or named arguments (lib/Bi/Action/gaussian.pm, uniform.pm);
+  * `sub observation`: `~` actions on obs variables, with the log-densities of
+    gaussian/normal, log_gaussian/log_normal, uniform, exponential, gamma,
+    inverse_gamma, beta, poisson, negbin and binomial (lib/Bi/Action/*.pm
+    arguments, positional or named);
+  * draws (initial, transition, observation samples) from gaussian,
+    log_gaussian, uniform and exponential;
+  * `sub parameter`: priors with constant arguments (gaussian, log_gaussian,
+    uniform, exponential, gamma, inverse_gamma, beta), written as a host table
+    for SMC^2 and PMMH (generate_priors);
   * the ODE algorithms 'RK4' and 'RK5(4)' (share/tt/cpp/block/ode.hpp.tt).
-Blocks the bootstrap filter does not run (`parameter`, `proposal_*`,
-`lookahead_*`, `bridge`) are parsed and skipped.
+Blocks the samplers here do not run (`proposal_*`, `lookahead_*`, `bridge`)
+are parsed and skipped.
 
 Random draws are keyed Philox counters: the k-th normal draw of a block
 (counting vector elements) comes from block k/4, word pair k%4 of the
@@ -59,6 +65,11 @@ DIST_ARGS = {"gaussian": (("mean", 0.0), ("std", 1.0)),
              "binomial": (("size", 1.0), ("prob", 0.5))}
 # distributions with a sampler here; the others are log-densities of the observation block
 SAMPLED = ("gaussian", "uniform", "exponential")
+# priors of the parameter block, sampled and evaluated on the host (libbi_amd/csrc/prior.cpp)
+PRIOR_CODE = {"gaussian": 0, "log_gaussian": 1, "uniform": 2, "exponential": 3, "gamma": 4,
+              "inverse_gamma": 5, "beta": 6}
+PRIOR_NAMES = ("gaussian", "log_gaussian", "uniform", "exponential", "gamma", "inverse_gamma",
+               "beta")
 UNIFORM_BLOCK = 0x40000000  # Philox block offset of uniform draws
 BOUNDARIES = ("none", "cyclic", "extended")
 # functions of the model language and their C++ names (overloaded for float and double)
@@ -130,6 +141,8 @@ class GenModel:
     has_ode: bool = False
     normals: Dict[str, int] = field(default_factory=dict)   # block -> normal draws
     uniforms: Dict[str, int] = field(default_factory=dict)  # block -> uniform draws
+    # sub parameter: (distribution, first argument, second argument) per param element
+    priors: Dict[int, tuple] = field(default_factory=dict)
 
     def names(self, kind):
         return [v.name for v in self.vars.values() if v.kind == kind]
@@ -312,7 +325,8 @@ class _Analyser:
             if n not in self.vars:
                 _err(line, f"unknown name {n!r}")
             v = self.vars[n]
-            allowed = {"initial": ("param", "input", "state"),
+            allowed = {"parameter": (),
+                       "initial": ("param", "input", "state"),
                        "transition": ("param", "input", "state", "noise"),
                        "ode": ("param", "input", "state", "noise"),
                        "observation": ("param", "input", "state")}[block]
@@ -380,7 +394,7 @@ class _Analyser:
         else:
             _err(a.line, f"distribution {name!r} is not supported "
                          f"({', '.join(sorted(set(DIST_ARGS) | set(NORMAL_DISTS)))})")
-        if dist not in SAMPLED and block != "observation":
+        if dist not in SAMPLED and block not in ("observation", "parameter"):
             _err(a.line, f"sampling from {dist} is not built ({', '.join(SAMPLED)} are); "
                          f"{dist} is supported as an observation density")
         args = _args(a.expr, DIST_ARGS[dist], a.line)
@@ -438,6 +452,29 @@ class _Analyser:
                                    "proposal_parameter", "proposal_initial",
                                    "lookahead_transition", "lookahead_observation", "bridge"):
                 _err(b.line, f"unknown block {b.spec.name!r}")
+        # sub parameter: the prior of the params, constant arguments (SMC^2 and PMMH)
+        b = m.block("parameter")
+        if b is not None:
+            lc = {}
+            self.local_decls(b, lc)
+            for it in _flatten(b.items):
+                if isinstance(it, tuple):
+                    continue
+                if isinstance(it, Block) or it.op != "~":
+                    _err(getattr(it, "line", b.line),
+                         "the parameter block takes '~' actions only")
+                v, alias = self.target(it, "parameter", ("param",))
+                d = self.draw(it, "parameter", {}, lc, v, alias)
+                if d.dist not in PRIOR_CODE:
+                    _err(it.line, f"{d.dist} is not a prior here ({', '.join(PRIOR_NAMES)})")
+                for i in range(v.size):
+                    env = {**self.consts, **lc, **({alias: float(i)} if alias else {})}
+                    args = [_const_eval(x, env, it.line) for x in d.args]
+                    code = PRIOR_CODE["log_gaussian" if d.dist == "gaussian" and d.log
+                                      else d.dist]
+                    if v.offset + i in g.priors:
+                        _err(it.line, f"{v.name} has two priors")
+                    g.priors[v.offset + i] = (code, args[0], args[1] if len(args) > 1 else 0.0)
         # sub initial
         b = m.block("initial")
         if b is not None:
@@ -945,6 +982,30 @@ def generate(models: List[GenModel]) -> str:
     out.append("  }")
     out.append('  return set_error(BIHIP_EINVAL, "unknown generated model");')
     out.append("}")
+    out.append("")
+    out.append("}  // namespace bihip")
+    return "\n".join(out) + "\n"
+
+
+def generate_priors(models: List[GenModel]) -> str:
+    """Host data for the priors of the parameter blocks: per model, one
+    {distribution, a, b} per param element (distribution -1: no prior)."""
+    out = ["// Generated by libbi_amd.dsl from the model files in libbi_amd/bi; do not edit.",
+           "#pragma once", "", "namespace bihip {", ""]
+    for i, g in enumerate(models):
+        n = g.size("param")
+        rows = []
+        for k in range(n):
+            c, a, b = g.priors.get(k, (-1, 0.0, 0.0))
+            rows.append(f"{{{c}, {_lit(a)}, {_lit(b)}}}")
+        if not rows:
+            rows = ["{-1, 0.0, 0.0}"]
+        out.append(f"static const GenPrior GEN_{i}_PRIOR[] = {{{', '.join(rows)}}};")
+    out.append("static const GenPrior* const GEN_PRIORS[] = {"
+               + ", ".join(f"GEN_{i}_PRIOR" for i in range(len(models))) + ("" if models else "nullptr") + "};")
+    out.append("static const int GEN_PRIOR_COUNT[] = {"
+               + ", ".join(str(g.size("param")) for g in models) + ("" if models else "0") + "};")
+    out.append(f"constexpr int GEN_PRIOR_MODELS = {len(models)};")
     out.append("")
     out.append("}  // namespace bihip")
     return "\n".join(out) + "\n"

@@ -352,3 +352,55 @@ def test_netcdf_output_of_a_vector_model(tmp_path):
     assert f.dimensions["n"] == 8 and f.variables["x"].dimensions == ("nr", "n", "np")
     np.testing.assert_array_equal(np.array(f.variables["x"].data)[1], X + 1)
     f.close()
+
+
+def _scipy_prior(name, a, b):
+    from scipy import stats
+    return {"gaussian": lambda: stats.norm(a, b), "log_normal": lambda: stats.lognorm(b, scale=np.exp(a)),
+            "uniform": lambda: stats.uniform(a, b - a), "exponential": lambda: stats.expon(scale=1 / a),
+            "gamma": lambda: stats.gamma(a, scale=b), "inverse_gamma": lambda: stats.invgamma(a, scale=b),
+            "beta": lambda: stats.beta(a, b)}[name]()
+
+
+PRIORS = {  # the parameter blocks of the model files, per param
+    "LinearGaussian": [("gamma", 2.0, 0.25), ("inverse_gamma", 3.0, 1.0),
+                       ("log_normal", np.log(0.5), 0.2), ("gaussian", 0.0, 1.0),
+                       ("uniform", 0.5, 2.0)],
+    "SIR": [("gamma", 4.0, 0.15), ("beta", 2.0, 6.0), ("exponential", 0.1, 0.0),
+            ("uniform", 0.0, 0.3)],
+    "PZ": [("uniform", 0.0, 1.0), ("uniform", 0.0, 0.5)],
+}
+
+
+@pytest.mark.parametrize("model", sorted(PRIORS))
+def test_parameter_block_priors_against_scipy(model):
+    """Draws from a model file's parameter block (host, keyed) pass a
+    Kolmogorov-Smirnov test against scipy's distribution, and the log-density
+    equals scipy's."""
+    from scipy import stats
+    m = capi.generated_model(model)
+    spec = PRIORS[model]
+    n = len(spec)
+    draws = np.array([capi.prior_sample(m, n, 11, j) for j in range(4000)])
+    for i, (name, a, b) in enumerate(spec):
+        p = stats.kstest(draws[:, i], _scipy_prior(name, a, b).cdf).pvalue
+        assert p > 1e-3, (model, i, name, p)
+    # deterministic: the same key gives the same draw
+    np.testing.assert_array_equal(capi.prior_sample(m, n, 11, 5), draws[5])
+    for th in draws[:20]:
+        ref = sum(_scipy_prior(name, a, b).logpdf(v) for (name, a, b), v in zip(spec, th))
+        assert capi.prior_logdensity(m, th) == pytest.approx(ref, rel=1e-12, abs=1e-12)
+    # outside the support
+    bad = draws[0].copy()
+    bad[0] = -1.0 if spec[0][0] != "gaussian" else bad[0]
+    if spec[0][0] != "gaussian":
+        assert capi.prior_logdensity(m, bad) == -np.inf
+
+
+def test_priors_must_exist_and_be_positive_for_the_samplers():
+    with pytest.raises(capi.BihipError, match="no prior"):
+        capi.prior_sample(capi.generated_model("Lorenz96"), 1, 1, 0)
+    with pytest.raises(capi.BihipError, match="not generated"):
+        capi.prior_sample(capi.make_model("pz"), 1, 1, 0)
+    with pytest.raises(GenError, match="cannot be read"):
+        _gen("model M { param a, b\n state x\n sub parameter { a ~ gaussian(b, 1.0) } }")

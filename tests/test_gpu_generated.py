@@ -386,3 +386,67 @@ def test_generated_model_with_schedule_and_inputs(ctx):
         zs.append(pf.term())
         pf.close()
     assert abs(np.mean(zs) - exact) < 0.05, (zs, exact)
+
+
+def test_smc2_with_the_model_files_prior_matches_quadrature(ctx, orc):
+    """SMC^2 over lam of LinearGaussian.bi with its parameter block's prior
+    (lam ~ gamma(2, 0.25)): the evidence and the posterior mean of lam against
+    quadrature of the exact Kalman likelihood times that prior."""
+    from scipy import stats
+    from scipy.special import logsumexp
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, atol=1e-10, rtol=1e-10,
+                             h0=0.25, **par)
+    h = capi.make_model("lg", precision=capi.F64, atol=1e-10, rtol=1e-10, h0=0.25, **par)
+    om = oracle_model(orc, h)
+    T = 40
+    x0 = orc.init_samples(om, 21, 1)[:, 0]
+    _, y = orc.simulate(om, 21, T, 1.0, x0)
+    y = np.asarray(y).reshape(T, 1)
+    # quadrature over u = log lam
+    u = np.linspace(-7.0, 2.0, 1500)
+    lam = np.exp(u)
+    ll = []
+    for v in lam:
+        om.par[0] = v
+        ll.append(orc.kalman_loglik_lg(om, 1.0, y[:, 0]))
+    lf = np.array(ll) + stats.gamma(2.0, scale=0.25).logpdf(lam) + u
+    du = u[1] - u[0]
+    lZq = logsumexp(lf) + np.log(du)
+    post = np.sum(np.exp(lf - lZq) * lam) * du
+    zs, ps = [], []
+    for seed in (1, 2, 3, 4):
+        s = capi.SMC2(ctx, g, [512] * 512, T, None, None, [0.3], seed=seed, nmoves=1,
+                      ess_rel_outer=0.8, block=256)
+        s.set_observations(y)
+        s.run()
+        out = s.outputs()
+        s.close()
+        w = np.exp(out["lw"] - out["lw"].max())
+        zs.append(out["logZ"])
+        ps.append((w * out["theta"][:, 0]).sum() / w.sum())
+        assert out["accepts"].sum() > 100
+    assert abs(np.mean(zs) - lZq) < 0.15, (zs, lZq)
+    assert abs(np.mean(ps) - post) < 0.03, (ps, post)
+
+
+def test_pmmh_with_the_model_files_prior(ctx, orc):
+    """PMMH chains over lam of LinearGaussian.bi start from the keyed draws of
+    its parameter block's prior, stay in its support and move."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, **par)
+    h = capi.make_model("lg", precision=capi.F64, **par)
+    om = oracle_model(orc, h)
+    T, B, C = 20, 64, 60
+    x0 = orc.init_samples(om, 5, 1)[:, 0]
+    _, y = orc.simulate(om, 5, T, 1.0, x0)
+    p = capi.PMMH(ctx, g, [512] * B, T, C, None, None, [0.3], seed=3)
+    p.set_observations(np.asarray(y).reshape(T, 1))
+    p.run()
+    out = p.outputs()
+    p.close()
+    th = out["theta"][:, :, 0]
+    first = np.array([capi.prior_sample(g, 1, 3, j)[0] for j in range(B)])
+    np.testing.assert_allclose(th[0], first, rtol=1e-15)
+    assert (th > 0).all() and np.isfinite(out["ll"]).all()
+    assert out["accepted"].sum() > B
