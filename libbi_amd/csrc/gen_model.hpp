@@ -12,6 +12,7 @@
 
 #include "internal.hpp"
 #include "model_ops.hpp"
+#include "gen_samplers.hpp"
 
 namespace bihip {
 

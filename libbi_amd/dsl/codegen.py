@@ -28,8 +28,10 @@ line otherwise):
     gaussian/normal, log_gaussian/log_normal, uniform, exponential, gamma,
     inverse_gamma, beta, poisson, negbin and binomial (lib/Bi/Action/*.pm
     arguments, positional or named);
-  * draws (initial, transition, observation samples) from gaussian,
-    log_gaussian, uniform and exponential;
+  * draws (initial, transition, observation samples) from all of them:
+    gaussian from normal words, uniform and exponential from a uniform, the
+    others by the rejection samplers of libbi_amd/csrc/gen_samplers.hpp, each
+    draw with a keyed Philox stream of its own;
   * `sub parameter`: priors with constant arguments (gaussian, log_gaussian,
     uniform, exponential, gamma, inverse_gamma, beta), written as a host table
     for SMC^2 and PMMH (generate_priors);
@@ -63,8 +65,12 @@ DIST_ARGS = {"gaussian": (("mean", 0.0), ("std", 1.0)),
              "poisson": (("rate", 1.0),),
              "negbin": (("mean", 1.0), ("shape", 1.0)),
              "binomial": (("size", 1.0), ("prob", 0.5))}
-# distributions with a sampler here; the others are log-densities of the observation block
-SAMPLED = ("gaussian", "uniform", "exponential")
+# samplers: normal words (gaussian), one uniform word pair (uniform, exponential), or a
+# keyed stream of words per draw (the rejection samplers of gen_samplers.hpp)
+SAMPLED = ("gaussian", "uniform", "exponential", "gamma", "inverse_gamma", "beta", "poisson",
+           "negbin", "binomial")
+STREAM_DISTS = ("gamma", "inverse_gamma", "beta", "poisson", "negbin", "binomial")
+STREAM_BLOCK = 0x20000000  # Philox block of the first stream draw; 4096 blocks per draw
 # priors of the parameter block, sampled and evaluated on the host (libbi_amd/csrc/prior.cpp)
 PRIOR_CODE = {"gaussian": 0, "log_gaussian": 1, "uniform": 2, "exponential": 3, "gamma": 4,
               "inverse_gamma": 5, "beta": 6}
@@ -141,6 +147,7 @@ class GenModel:
     has_ode: bool = False
     normals: Dict[str, int] = field(default_factory=dict)   # block -> normal draws
     uniforms: Dict[str, int] = field(default_factory=dict)  # block -> uniform draws
+    streams: Dict[str, int] = field(default_factory=dict)   # block -> stream draws
     # sub parameter: (distribution, first argument, second argument) per param element
     priors: Dict[int, tuple] = field(default_factory=dict)
 
@@ -394,14 +401,11 @@ class _Analyser:
         else:
             _err(a.line, f"distribution {name!r} is not supported "
                          f"({', '.join(sorted(set(DIST_ARGS) | set(NORMAL_DISTS)))})")
-        if dist not in SAMPLED and block not in ("observation", "parameter"):
-            _err(a.line, f"sampling from {dist} is not built ({', '.join(SAMPLED)} are); "
-                         f"{dist} is supported as an observation density")
         args = _args(a.expr, DIST_ARGS[dist], a.line)
-        kind = "normal" if dist == "gaussian" else "uniform"
+        kind = ("normal" if dist == "gaussian" else
+                "stream" if dist in STREAM_DISTS else "uniform")
         k = counts.setdefault(kind, 0)
-        if dist in SAMPLED:
-            counts[kind] += v.size
+        counts[kind] += v.size
         d = Draw(dist, log, args, k)
         for x in d.args:
             self.check_refs(x, block, a.line, lc, aliases, v.dim)
@@ -485,6 +489,7 @@ class _Analyser:
             g.consts.update(lc)
             g.normals["initial"] = counts.get("normal", 0)
             g.uniforms["initial"] = counts.get("uniform", 0)
+            g.streams["initial"] = counts.get("stream", 0)
         # sub transition(delta = 1.0)
         b = m.block("transition")
         if b is not None:
@@ -511,6 +516,7 @@ class _Analyser:
             g.consts.update(lc)
             g.normals["transition"] = counts.get("normal", 0)
             g.uniforms["transition"] = counts.get("uniform", 0)
+            g.streams["transition"] = counts.get("stream", 0)
         # sub observation
         b = m.block("observation")
         if b is not None:
@@ -528,6 +534,7 @@ class _Analyser:
             g.consts.update(lc)
             g.normals["observation"] = counts.get("normal", 0)
             g.uniforms["observation"] = counts.get("uniform", 0)
+            g.streams["observation"] = counts.get("stream", 0)
         return g
 
     def ode_block(self, b: Block, g: GenModel, lc):
@@ -736,12 +743,17 @@ def _zname(k):
     return f"zq{k // 4}[{k % 4}]"
 
 
-def _draw_value(em: _Emitter, d: Draw, i, bind, line):
-    """Value of element i of a draw (Bi::Action::gaussian / uniform `sample`,
-    share/tt/cpp/action/*.hpp.tt)."""
-    if d.dist not in SAMPLED:
-        return '__builtin_nan("")'
+def _draw_value(em: _Emitter, d: Draw, i, bind, line, sctx=None):
+    """Value of element i of a draw (the actions' `sample`,
+    share/tt/cpp/action/*.hpp.tt).  sctx = (step, tag, first stream index) of
+    the block's keyed streams for the rejection samplers."""
     a = [em.expr(x, bind, i, line) for x in d.args]
+    if d.dist in STREAM_DISTS:
+        step, tag, first = sctx
+        ad = ", ".join(f"(double)({x})" for x in a)
+        v = (f"gen_{d.dist}({ad}, DrawStream(seed, p, {step}, {tag}, batch, "
+             f"{STREAM_BLOCK:#x}u + (({first} + {d.index + i}u) << 12)))")
+        return f"T({v})" if em.ty == "T" else v
     if d.dist == "gaussian":
         z = _zname(d.index + i)
         z = z if em.ty == "T" else f"(double){z}"
@@ -762,7 +774,7 @@ def _target(g: GenModel, name, i, xname):
     return f"w_{name}" + (f"[{i}]" if v.dim else "")
 
 
-def _actions(g, steps, em, xname, ind, tmp_ty):
+def _actions(g, steps, em, xname, ind, tmp_ty, sctx=None):
     """Statements of discrete actions, in order; a vector action computes all
     its elements before it stores them."""
     lines = []
@@ -771,7 +783,7 @@ def _actions(g, steps, em, xname, ind, tmp_ty):
         vals = []
         for i, bind in _elements(g, st):
             if st.kind == "draw":
-                vals.append(_draw_value(em, st.draw, i, bind, st.line))
+                vals.append(_draw_value(em, st.draw, i, bind, st.line, sctx))
             else:
                 vals.append(em.expr(st.expr, bind, i, st.line))
         if v.dim is None:
@@ -848,6 +860,8 @@ def generate_model(g: GenModel) -> str:
         w(f"    k[{j}] = {rhs[j]};")
     w("  }")
 
+    nst = g.streams.get("transition", 0)
+    tsctx = ("step", "TAG_NOISE", f"fire * {nst}u")
     w("  // one interval [t1, t2]: discrete actions at multiples of DELTA, the ode in between")
     w("  template <int INTEG, class G>")
     w("  __device__ __forceinline__ int transition(const G& grp, T (&x)[DL], double t1, double t2,")
@@ -869,7 +883,7 @@ def generate_model(g: GenModel) -> str:
         for k in range(nu_t):
             w(f"        u{k} = uniform_word<T>(philox_block(seed, p, step, TAG_NOISE, batch, "
               f"{UNIFORM_BLOCK:#x}u + fire * {nu_t}u + {k}u));")
-        for ln in _actions(g, g.pre, em, "x", "        ", "T"):
+        for ln in _actions(g, g.pre, em, "x", "        ", "T", tsctx):
             w(ln)
         w("      }")
     if g.has_ode:
@@ -877,7 +891,7 @@ def generate_model(g: GenModel) -> str:
         w("      else n += integrate_dopri5(*this, grp, x, ta, tb, ma);")
     if g.post:
         w("      if (on) {")
-        for ln in _actions(g, g.post, em, "x", "        ", "T"):
+        for ln in _actions(g, g.post, em, "x", "        ", "T", tsctx):
             w(ln)
         w("      }")
     w("      fire += on ? 1u : 0u;")
@@ -911,7 +925,8 @@ def generate_model(g: GenModel) -> str:
     for st in g.observation:
         v = g.vars[st.target]
         for i, bind in _elements(g, st):
-            w(f"    y[{v.offset + i}] = {_draw_value(eo, st.draw, i, bind, st.line)};")
+            w(f"    y[{v.offset + i}] = "
+              f"{_draw_value(eo, st.draw, i, bind, st.line, ('step', 'TAG_OBS', '0u'))};")
     w("  }")
 
     # initial states, in double (share/tt/cpp/block/initial.hpp.tt)
@@ -921,7 +936,7 @@ def generate_model(g: GenModel) -> str:
     for j in range(S):
         w(f"    xd[{j}] = 0.0;")
     _draw_buffers(g, "initial", "0u", "TAG_INIT", "    ", w)
-    for ln in _actions(g, g.initial, ei, "xd", "    ", "double"):
+    for ln in _actions(g, g.initial, ei, "xd", "    ", "double", ("0u", "TAG_INIT", "0u")):
         w(ln)
     w("  }")
     w("};")

@@ -189,7 +189,7 @@ HOST_STUB = r"""
 #define BIHIP_EINVAL 2
 #define BIHIP_MODEL_GENERATED 64
 using std::exp; using std::log; using std::sqrt; using std::floor; using std::fmin;
-using std::pow; using std::fabs; using std::lgamma; using std::log1p;
+using std::pow; using std::fabs; using std::lgamma; using std::log1p; using std::cos;
 namespace bihip {
 struct ModelArgs { double h0; double par[16]; };
 struct bihip_model_ { int model, precision; };
@@ -202,8 +202,22 @@ inline int set_error(int c, const char*) { return c; }
 inline double gen_scaled(double t, double d) { return t / d; }
 template <class T> void normals4(uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t,
                                  T (&z)[4]) { z[0] = z[1] = z[2] = z[3] = T(0); }
-inline u32x4 philox_block(uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t) {
-  return {0, 0, 0, 0}; }
+// host stand-in for the keyed Philox words (splitmix64 of the counter): good
+// enough words for statistical tests of the samplers
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline u32x4 philox_block(uint64_t seed, uint32_t i, uint32_t s, uint32_t t, uint32_t b,
+                          uint32_t blk) {
+  uint64_t h = mix64(seed ^ mix64(((uint64_t)i << 32 | s) ^ mix64(((uint64_t)t << 32 | b) ^
+                                                                     mix64(blk))));
+  uint64_t h2 = mix64(h);
+  return {(uint32_t)h, (uint32_t)(h >> 32), (uint32_t)h2, (uint32_t)(h2 >> 32)};
+}
+inline double cospi(double x) { return std::cos(3.14159265358979323846 * x); }
 template <class T> T uniform_word(const u32x4&) { return T(0); }
 template <class M, class G, class T> int integrate_rk4(const M&, const G&, T (&)[M::DL],
                                                        double, double, double) { return 0; }
@@ -211,13 +225,15 @@ template <class M, class G, class T> int integrate_dopri5(const M&, const G&, T 
                                                           double, double, const ModelArgs&) {
   return 0; }
 }
+#include "GEN_SAMPLERS"
 using bihip_model = bihip::bihip_model_;
 """
 
 
 def _gxx(tmp_path, text, run=False):
     src = tmp_path / "gen.cpp"
-    src.write_text(HOST_STUB + text)
+    samplers = os.path.join(os.path.dirname(HERE), "libbi_amd", "csrc", "gen_samplers.hpp")
+    src.write_text(HOST_STUB.replace("GEN_SAMPLERS", samplers) + text)
     exe = tmp_path / "gen"
     args = ["g++", "-std=c++17", "-O1", "-Wall", str(src), "-o", str(exe)]
     if not run:
@@ -327,15 +343,18 @@ def test_observation_densities_against_scipy(tmp_path):
     assert n == sum(len(y) * len(x) for y, x in cases.values())
 
 
-def test_sampling_only_where_built():
-    with pytest.raises(GenError, match="sampling from gamma"):
-        _gen("model M { state x\n noise w\n sub transition { w ~ gamma(2.0) } }")
-    g = _gen("model M { param l\n state x\n noise w\n obs c\n"
-             " sub transition { w ~ exponential(l)\n x <- x + w }\n"
+def test_draws_from_every_distribution():
+    """Exponential draws invert a uniform; the rejection samplers take a keyed
+    stream per draw element (fire-dependent in the transition)."""
+    g = _gen("model M { param l\n state x\n noise w, v[n2]\n dim n2(2)\n obs c\n"
+             " sub transition { w ~ exponential(l)\n v ~ gamma(2.0, l)\n x <- x + w }\n"
              " sub observation { c ~ poisson(x) } }")
     src = codegen.generate_model(g)
     assert "w_w = ((-log(T(1.0) - u0)) / T(p_l));" in src
-    assert 'y[0] = __builtin_nan("");' in src  # no Poisson sampler: NaN observation samples
+    assert ("t_[1] = T(gen_gamma((double)(T(2.0)), (double)(T(p_l)), DrawStream(seed, p, step, "
+            "TAG_NOISE, batch, 0x20000000u + ((fire * 2u + 1u) << 12))));") in src
+    assert ("y[0] = gen_poisson((double)((double)x[0]), DrawStream(seed, p, step, TAG_OBS, "
+            "batch, 0x20000000u + ((0u + 0u) << 12)));") in src
 
 
 def test_netcdf_output_of_a_vector_model(tmp_path):
@@ -404,3 +423,55 @@ def test_priors_must_exist_and_be_positive_for_the_samplers():
         capi.prior_sample(capi.make_model("pz"), 1, 1, 0)
     with pytest.raises(GenError, match="cannot be read"):
         _gen("model M { param a, b\n state x\n sub parameter { a ~ gaussian(b, 1.0) } }")
+
+
+SAMPLER_CASES = [  # (distribution, expression of the draw, scipy distribution)
+    ("gamma", "gen_gamma(2.5, 1.5, w)", lambda st: st.gamma(2.5, scale=1.5)),
+    ("gamma_small_shape", "gen_gamma(0.4, 2.0, w)", lambda st: st.gamma(0.4, scale=2.0)),
+    ("inverse_gamma", "gen_inverse_gamma(3.0, 2.0, w)", lambda st: st.invgamma(3.0, scale=2.0)),
+    ("beta", "gen_beta(2.0, 5.0, w)", lambda st: st.beta(2.0, 5.0)),
+    ("poisson_small", "gen_poisson(3.7, w)", lambda st: st.poisson(3.7)),
+    ("poisson_large", "gen_poisson(57.3, w)", lambda st: st.poisson(57.3)),
+    ("negbin", "gen_negbin(12.0, 3.0, w)", lambda st: st.nbinom(3.0, 3.0 / 15.0)),
+    ("negbin_large", "gen_negbin(140.0, 20.0, w)", lambda st: st.nbinom(20.0, 20.0 / 160.0)),
+    ("binomial_small", "gen_binomial(20.0, 0.3, w)", lambda st: st.binom(20, 0.3)),
+    ("binomial_large", "gen_binomial(400.0, 0.35, w)", lambda st: st.binom(400, 0.35)),
+    ("binomial_flip", "gen_binomial(90.0, 0.8, w)", lambda st: st.binom(90, 0.8)),
+]
+
+
+def test_rejection_samplers_against_scipy(tmp_path):
+    """The draws of gen_samplers.hpp (compiled for the host with stand-in
+    random words) follow their distributions: Kolmogorov-Smirnov for the
+    continuous ones, chi-square over the support for the counts."""
+    from scipy import stats
+    n = 20000
+    body = ["int main() {"]
+    for name, expr, _ in SAMPLER_CASES:
+        body.append(f"  for (uint32_t j = 0; j < {n}u; ++j) {{ bihip::DrawStream w(7, j, 1, 2, 0, "
+                    f"0x20000000u); std::printf(\"{name} %.17g\\n\", bihip::{expr.replace(', w)', ', w)')}); }}")
+    body.append("  return 0;\n}")
+    out = _gxx(tmp_path, "\n".join(body), run=True)
+    draws = {}
+    for line in out.splitlines():
+        k, v = line.split()
+        draws.setdefault(k, []).append(float(v))
+    for name, _, dist in SAMPLER_CASES:
+        x = np.array(draws[name])
+        d = dist(stats)
+        if name.startswith(("poisson", "negbin", "binomial")):
+            assert np.all(x == np.floor(x)) and x.min() >= 0
+            lo, hi = int(d.ppf(1e-4)), int(d.ppf(1 - 1e-4))
+            ks = np.arange(lo, hi + 1)
+            obs = np.array([np.sum(x == k) for k in ks], float)
+            obs[0] += np.sum(x < lo)
+            obs[-1] += np.sum(x > hi)
+            exp = d.pmf(ks) * n
+            exp[0] += d.cdf(lo - 1) * n
+            exp[-1] += d.sf(hi) * n
+            keep = exp > 5
+            chi = np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep])
+            p = stats.chi2.sf(chi, keep.sum() - 1)
+        else:
+            p = stats.kstest(x, d.cdf).pvalue
+        assert p > 1e-4, (name, p)

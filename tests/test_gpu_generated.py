@@ -450,3 +450,70 @@ def test_pmmh_with_the_model_files_prior(ctx, orc):
     np.testing.assert_allclose(th[0], first, rtol=1e-15)
     assert (th > 0).all() and np.isfinite(out["ll"]).all()
     assert out["accepted"].sum() > B
+
+
+def _chi2_counts(x, d):
+    """p-value of a chi-square test of integer draws x against scipy distribution d."""
+    from scipy import stats
+    n = len(x)
+    lo, hi = int(d.ppf(1e-4)), int(d.ppf(1 - 1e-4))
+    ks = np.arange(lo, hi + 1)
+    obs = np.array([np.sum(x == k) for k in ks], float)
+    obs[0] += np.sum(x < lo)
+    obs[-1] += np.sum(x > hi)
+    exp = d.pmf(ks) * n
+    exp[0] += d.cdf(lo - 1) * n
+    exp[-1] += d.sf(hi) * n
+    keep = exp > 5
+    return stats.chi2.sf(np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep]), keep.sum() - 1)
+
+
+def test_generated_count_draws_on_the_device(ctx, torch_gpu):
+    """Rejection samplers in device code: SIR.bi's negative-binomial and
+    SIRStochastic.bi's binomial observation samples, and SIRStochastic.bi's
+    binomial infections and recoveries over one step, against scipy."""
+    from scipy import stats
+    L, P = capi.lib(), 20000
+    c = lambda t: ctypes.c_void_p(t.data_ptr())
+
+    def obs_samples(m, X0):
+        X = torch_gpu.from_numpy(np.ascontiguousarray(X0)).cuda()
+        Y = torch_gpu.empty((P, 1), dtype=torch_gpu.float64, device="cuda")
+        capi.check(L.bihip_obs_sample(ctx.h, ctypes.byref(m), 9, 0, 3, P, c(X), P, c(Y)))
+        return host(Y)[:, 0]
+
+    sir = capi.generated_model("SIR", precision=capi.F64, beta=0.6, gamma=0.25, k=8.0, sigma=0.1)
+    X0 = np.tile(np.array([[700.0], [100.0], [200.0], [30.0]]), (1, P))
+    y = obs_samples(sir, X0)
+    assert _chi2_counts(y, stats.nbinom(8.0, 8.0 / 38.0)) > 1e-4
+    st = capi.generated_model("SIRStochastic", precision=capi.F64, beta=0.5, gamma=0.3, rho=0.4)
+    X0 = np.tile(np.array([[900.0], [60.0], [40.0], [80.0]]), (1, P))
+    y = obs_samples(st, X0)
+    assert _chi2_counts(y, stats.binom(80, 0.4)) > 1e-4
+    Xin = torch_gpu.from_numpy(np.ascontiguousarray(X0)).cuda()
+    Xout = torch_gpu.empty_like(Xin)
+    capi.check(L.bihip_transition(ctx.h, ctypes.byref(st), 5, 0, 4, 3.0, 4.0, P, c(Xin),
+                                  c(Xout), P, None, None))
+    X1 = host(Xout)
+    inf, rec = 900.0 - X1[0], X1[2] - 40.0
+    np.testing.assert_array_equal(X1[3], inf)
+    np.testing.assert_array_equal(X1[0] + X1[1] + X1[2], np.full(P, 1000.0))
+    assert _chi2_counts(inf, stats.binom(900, 1 - np.exp(-0.5 * 60 / 1000))) > 1e-4
+    assert _chi2_counts(rec, stats.binom(60, 1 - np.exp(-0.3))) > 1e-4
+
+
+def test_generated_stochastic_sir_filter(ctx):
+    """SIRStochastic.bi filtered on cases simulated from its own law in numpy:
+    two independent estimates of the log-likelihood agree."""
+    rng = np.random.default_rng(17)
+    beta, gamma, rho, T = 0.5, 0.3, 0.4, 30
+    S, I, R = 990.0, 10.0, 0.0
+    y = np.empty(T)
+    for t in range(T):
+        inf = rng.binomial(int(S), 1 - np.exp(-beta * I / 1000.0))
+        rec = rng.binomial(int(I), 1 - np.exp(-gamma))
+        S, I, R = S - inf, I + inf - rec, R + rec
+        y[t] = rng.binomial(inf, rho)
+    m = capi.generated_model("SIRStochastic", precision=capi.F64, beta=beta, gamma=gamma, rho=rho)
+    zs = [_run(ctx, m, y, 1 << 16, 1.0, s, record=False)[0] for s in (1, 2)]
+    assert all(np.isfinite(zs)) and abs(zs[0] - zs[1]) < 0.5, zs
