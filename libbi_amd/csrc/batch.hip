@@ -357,7 +357,7 @@ __global__ void k_batch_init(ModelArgs ma, const double* theta, int ntheta, cons
     ModelArgs mj = ma;
     for (int i = 0; i < ntheta && i < 16; ++i) mj.par[i] = theta[(size_t)j * ntheta + i];
     double xd[DL];
-    M(mj).init(xd, seed, stream_base + (uint32_t)j, (uint32_t)p);
+    M(mj).init(xd, seed, stream_base + (uint32_t)j, (uint32_t)p, 0);
     for (int c = 0; c < DL; ++c) X[(size_t)c * ld + g] = (T)xd[c];
     return;
   }
@@ -426,7 +426,9 @@ struct BatchRunner {
   hipStream_t s;
   template <class M>
   int run() {
-    if (integ == BIHIP_RK4) {
+    if constexpr (M::LPP != 1) {
+      return set_error(BIHIP_EINVAL, "batched filters take models of one lane per particle");
+    } else if (integ == BIHIP_RK4) {
       hipLaunchKernelGGL((k_batch_step<M, BIHIP_RK4, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
     } else {
       hipLaunchKernelGGL((k_batch_step<M, BIHIP_DOPRI5, BLOCK>), dim3(grid), dim3(BLOCK), 0, s,
@@ -438,7 +440,9 @@ struct BatchRunner {
   template <class M>
   int occupancy() {
     int n = 0;
-    if (integ == BIHIP_RK4) {
+    if constexpr (M::LPP != 1) {
+      return 1;
+    } else if (integ == BIHIP_RK4) {
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
               &n, reinterpret_cast<const void*>(&k_batch_step<M, BIHIP_RK4, BLOCK>), BLOCK, 0) !=
           hipSuccess)
@@ -484,8 +488,9 @@ static int occupancy_of(const bihip_model* m, F& f) {
 }
 
 bool batch_model_supported(const bihip_model* m) {
+  const GenInfo* g = generated_info(m->model);
   return model_supported(m) && (m->model == BIHIP_MODEL_PZ || m->model == BIHIP_MODEL_LG ||
-                                generated_info(m->model) != nullptr);
+                                (g != nullptr && g->lpp == 1));
 }
 
 int batch_nmax() { return BATCH_NMAX; }
@@ -535,10 +540,14 @@ struct BatchInitRunner {
   hipStream_t s;
   template <class M>
   int run() {
-    hipLaunchKernelGGL((k_batch_init<M>), dim3(ceil_div(total, 256)), dim3(256), 0, s, ma, theta,
-                       ntheta, off, B, total, seed, stream_base, X, ld);
-    BIHIP_LAUNCH_CHECK();
-    return BIHIP_OK;
+    if constexpr (M::LPP != 1) {
+      return set_error(BIHIP_EINVAL, "batched filters take models of one lane per particle");
+    } else {
+      hipLaunchKernelGGL((k_batch_init<M>), dim3(ceil_div(total, 256)), dim3(256), 0, s, ma,
+                         theta, ntheta, off, B, total, seed, stream_base, X, ld);
+      BIHIP_LAUNCH_CHECK();
+      return BIHIP_OK;
+    }
   }
   template <class M>
   int occupancy() {

@@ -43,8 +43,8 @@ __device__ __forceinline__ float uniform_word<float>(const u32x4& r) {
 
 template <class M, std::enable_if_t<is_generated<M>::value, int> = 0>
 __device__ __forceinline__ double obs_of(const M& m, const typename M::real_t (&x)[M::DL],
-                                         const double* y, int, const ModelArgs* = nullptr) {
-  return m.obs(x, y);
+                                         const double* y, int lo, const ModelArgs* = nullptr) {
+  return m.obs(x, y, lo);  // the terms of the lane's state elements [lo, lo + DL)
 }
 
 }  // namespace bihip

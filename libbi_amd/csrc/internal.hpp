@@ -17,7 +17,7 @@ int set_error(int code, const char* fmt, ...);
 // nullptr for the hand-written models and unknown ids.
 struct GenInfo {
   const char* name;
-  int d, nobs, integrator;
+  int d, nobs, integrator, lpp;  // lpp: lanes per particle
   double h, atol, rtol, delta;
   const char* const* names[5];  // by BIHIP_VAR_* kind
   const int* sizes[5];          // elements of each variable

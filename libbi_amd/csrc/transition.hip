@@ -55,8 +55,17 @@ __device__ __forceinline__ double uniform(double v) {
 // SCHED: the filter step has a schedule or inputs (times, observed flags,
 // per-step parameters); without them the kernel is compiled without that code,
 // which costs the plain step about 7% when present but unused.
+// Waves per SIMD the compiler must leave room for: a generated functor names
+// its own (MIN_WAVES), the hand-written ones fit their budget unaided.
+template <class M>
+constexpr int trans_min_waves() {
+  if constexpr (is_generated<M>::value) return M::MIN_WAVES;
+  else return 1;
+}
+
 template <class M, int INTEG, bool SCHED = false>
-__global__ __launch_bounds__(TRANS_BLOCK) void k_transition(TransArgs a) {
+__global__ __launch_bounds__(TRANS_BLOCK) __attribute__((amdgpu_waves_per_eu(trans_min_waves<M>())))
+void k_transition(TransArgs a) {
   using T = typename M::real_t;
   constexpr int LPP = M::LPP, DL = M::DL;
   const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
@@ -165,9 +174,9 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_init(ModelArgs ma, uint64_t see
   T* X = static_cast<T*>(Xv);
   if constexpr (is_generated<M>::value) {
     double xd[DL];
-    M(ma).init(xd, seed, batch, (uint32_t)p);
+    M(ma).init(xd, seed, batch, (uint32_t)p, lo);
 #pragma unroll
-    for (int i = 0; i < DL; ++i) X[(size_t)i * ld + p] = (T)xd[i];
+    for (int i = 0; i < DL; ++i) X[(size_t)(lo + i) * ld + p] = (T)xd[i];
     return;
   }
   T z[DL];
@@ -218,10 +227,10 @@ __global__ __launch_bounds__(TRANS_BLOCK) void k_obs_sample(ModelArgs ma, uint64
   if (p >= P) return;
   const T* X = static_cast<const T*>(Xv);
   constexpr int NOBS = M::NOBS;
-  if constexpr (is_generated<M>::value) {
-    T x[M::DL];
+  if constexpr (is_generated<M>::value) {  // the whole state of the particle
+    T x[M::D];
 #pragma unroll
-    for (int i = 0; i < M::DL; ++i) x[i] = X[(size_t)i * ld + p];
+    for (int i = 0; i < M::D; ++i) x[i] = X[(size_t)i * ld + p];
     double yo[NOBS > 0 ? NOBS : 1];
     M(ma).obs_sample(x, seed, batch, (uint32_t)p, step, yo);
     for (int o = 0; o < NOBS; ++o) y[(size_t)p * NOBS + o] = yo[o];
@@ -314,6 +323,7 @@ static int with_model(const bihip_model* m, F& f) {
 int transition_blocks(const bihip_model* m, int P) {
   int lpp = 1;
   if (m->model == BIHIP_MODEL_L96) lpp = m->d == 200 ? 8 : (m->d == 40 ? 4 : 2);
+  if (const GenInfo* g = generated_info(m->model)) lpp = g->lpp;
   return ceil_div((long)P * lpp, TRANS_BLOCK);
 }
 

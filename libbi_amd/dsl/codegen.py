@@ -150,6 +150,7 @@ class GenModel:
     streams: Dict[str, int] = field(default_factory=dict)   # block -> stream draws
     # sub parameter: (distribution, first argument, second argument) per param element
     priors: Dict[int, tuple] = field(default_factory=dict)
+    lpp: int = 1  # lanes per particle
 
     def names(self, kind):
         return [v.name for v in self.vars.values() if v.kind == kind]
@@ -575,8 +576,138 @@ class _Analyser:
 
 def analyse(model: Model) -> GenModel:
     g = _Analyser(model).run()
+    g.lpp = _lane_groups(g)
     generate_model(g)  # index ranges are resolved element by element when emitting
     return g
+
+
+# ---------------------------------------------------------------- lane groups
+class _NotLanes(Exception):
+    pass
+
+
+def _index_affine(g, e, alias, line):
+    """(slope, intercept) of an index expression in the alias (alias absent: slope 0)."""
+    f = lambda v: _const_eval(e, {**g.consts, **({alias: float(v)} if alias else {})}, line)
+    b = f(0)
+    if alias is None:
+        return 0, int(b)
+    a = f(1) - b
+    if f(2) != 2 * a + b or a != int(a) or b != int(b):
+        raise _NotLanes()
+    return int(a), int(b)
+
+
+def _refs(e, out):
+    """Every Ref in an expression tree (inlines expanded by the caller)."""
+    if isinstance(e, Ref):
+        out.append(e)
+    elif isinstance(e, Unary):
+        _refs(e.arg, out)
+    elif isinstance(e, Binary):
+        _refs(e.left, out)
+        _refs(e.right, out)
+    elif isinstance(e, Ternary):
+        for x in (e.cond, e.then, e.other):
+            _refs(x, out)
+    elif isinstance(e, Call):
+        for x in e.args:
+            _refs(x, out)
+    return out
+
+
+def _expand(g, e, seen=()):
+    """The expression with inlines substituted."""
+    if isinstance(e, Ref) and e.index is None and e.name in g.inlines and e.name not in seen:
+        return _expand(g, g.inlines[e.name], seen + (e.name,))
+    if isinstance(e, Unary):
+        return Unary(e.op, _expand(g, e.arg, seen))
+    if isinstance(e, Binary):
+        return Binary(e.op, _expand(g, e.left, seen), _expand(g, e.right, seen))
+    if isinstance(e, Ternary):
+        return Ternary(*[_expand(g, x, seen) for x in (e.cond, e.then, e.other)])
+    if isinstance(e, Call):
+        return Call(e.name, [_expand(g, x, seen) for x in e.args], e.named)
+    return e
+
+
+def _step_exprs(st):
+    return [st.expr] if st.kind != "draw" else list(st.draw.args)
+
+
+def _lane_groups(g: GenModel) -> int:
+    """Lanes per particle.  A model whose state is one vector over a cyclic
+    dimension of at least 16 elements is split over a group of 8, 4 or 2
+    lanes (the largest that leaves slices of at least 8), as the hand-written
+    Lorenz-96 functor is, when its ODE reads the state at constant offsets of
+    its alias (at most a slice away: the halo comes from the neighbouring
+    lanes), its other actions work element by element, and each observation
+    element reads the state at one index affine in its own alias.  Otherwise
+    one lane holds the whole state."""
+    states = [v for v in g.vars.values() if v.kind == "state"]
+    if len(states) != 1 or states[0].dim is None:
+        return 1
+    x = states[0]
+    D = x.size
+    if g.dims[x.dim][1] != "cyclic" or D < 16:
+        return 1
+    lpp = next((n for n in (8, 4, 2) if D % n == 0 and D // n >= 8), 1)
+    if lpp == 1:
+        return 1
+    DL = D // lpp
+    try:
+        # the ode: x at offsets |c| <= DL of the alias; noise element by element
+        for st in g.ode + g.pre + g.post + g.initial:
+            v = g.vars[st.target]
+            ode = st.kind == "ode"
+            if v.dim is not None and v.dim != x.dim:
+                raise _NotLanes()
+            for e in _step_exprs(st):
+                for r in _refs(_expand(g, e), []):
+                    u = g.vars.get(r.name)
+                    if u is None:
+                        continue  # an alias or a constant
+                    if u.kind in ("param", "input"):
+                        if u.dim is not None:
+                            raise _NotLanes()
+                        continue
+                    if v.dim is None:  # a scalar target reads no element of a vector
+                        if u.dim is not None:
+                            raise _NotLanes()
+                        continue
+                    if u.dim is None:
+                        continue
+                    if u.dim != x.dim:
+                        raise _NotLanes()
+                    if r.index is None:
+                        continue
+                    a, c = _index_affine(g, r.index[0], st.alias, st.line)
+                    if a != 1 or (c != 0 and not (ode and u.kind == "state" and abs(c) <= DL)):
+                        raise _NotLanes()
+        # observations: one state index affine in the obs element, inside the state
+        for st in g.observation:
+            v = g.vars[st.target]
+            idx = set()
+            for e in st.draw.args:
+                for r in _refs(_expand(g, e), []):
+                    u = g.vars.get(r.name)
+                    if u is None or u.kind in ("param", "input"):
+                        if u is not None and u.dim is not None:
+                            raise _NotLanes()
+                        continue
+                    if r.index is None:
+                        raise _NotLanes()
+                    idx.add(_index_affine(g, r.index[0], st.alias, st.line))
+            if len(idx) > 1:
+                raise _NotLanes()
+            a, b = idx.pop() if idx else (0, 0)
+            if a < 0 or not all(0 <= a * j + b < D for j in range(v.size)):
+                raise _NotLanes()
+            if v.dim is not None and st.alias is None and idx:
+                raise _NotLanes()
+    except (_NotLanes, GenError):
+        return 1
+    return lpp
 
 
 # ---------------------------------------------------------------- emission
@@ -590,6 +721,8 @@ class _Emitter:
     """Expressions in one arithmetic type, 'T' (the state type) or 'double', for
     one element: bind maps dimension aliases to indices, elem is the element of
     an action over a whole vector (its unindexed vectors read that element)."""
+
+    fold_bind = True  # aliases are numbers known when generating
 
     def __init__(self, g: GenModel, ty: str, xname: str, xdouble=False):
         self.g, self.ty, self.x, self.xdouble = g, ty, xname, xdouble
@@ -621,26 +754,34 @@ class _Emitter:
             return self.num(e.value)
         if isinstance(e, Ref):
             n = e.name
-            if n in bind:
-                return self.num(float(bind[n]))
             if n in g.consts:
                 return self.num(g.consts[n])
-            if n in g.inlines and n not in local:
+            if n in g.inlines and n not in local and n not in bind:
                 return "(" + self.expr(g.inlines[n], bind, elem, line, local + (n,)) + ")"
-            v = g.vars.get(n)
-            if v is None:
-                raise GenError(f"line {line}: cannot read {n!r} here")
-            i = self.element(v, e, bind, elem, line)
-            if v.kind in ("param", "input"):
-                r = f"p_{n}" + (f"[{i}]" if v.dim else "")
-                return f"T({r})" if self.ty == "T" else r
-            if v.kind == "state":
-                r = f"{self.x}[{v.offset + i}]"
-                return f"(double){r}" if self.ty == "double" and not self.xdouble else r
-            if v.kind == "noise":
-                r = f"w_{n}" + (f"[{i}]" if v.dim else "")
-                return f"(double){r}" if self.ty == "double" else r
-            raise GenError(f"line {line}: cannot read {v.kind} {n} here")
+            return self.ref(e, bind, elem, line)
+        return self.other(e, bind, elem, line, local)
+
+    def ref(self, e, bind, elem, line):
+        g, n = self.g, e.name
+        if n in bind:
+            return self.num(float(bind[n]))
+        v = g.vars.get(n)
+        if v is None:
+            raise GenError(f"line {line}: cannot read {n!r} here")
+        i = self.element(v, e, bind, elem, line)
+        if v.kind in ("param", "input"):
+            r = f"p_{n}" + (f"[{i}]" if v.dim else "")
+            return f"pt_{n}" + (f"[{i}]" if v.dim else "") if self.ty == "T" else r
+        if v.kind == "state":
+            r = f"{self.x}[{v.offset + i}]"
+            return f"(double){r}" if self.ty == "double" and not self.xdouble else r
+        if v.kind == "noise":
+            r = f"w_{n}" + (f"[{i}]" if v.dim else "")
+            return f"(double){r}" if self.ty == "double" else r
+        raise GenError(f"line {line}: cannot read {v.kind} {n} here")
+
+    def other(self, e, bind, elem, line, local):
+        g = self.g
         if isinstance(e, Unary):
             return f"({e.op}{self.expr(e.arg, bind, elem, line, local)})"
         if isinstance(e, Binary):
@@ -661,7 +802,9 @@ class _Emitter:
             # a function of constants is evaluated here, correctly rounded in double
             # (the reference evaluates constant expressions when it generates code)
             try:
-                env = {**g.consts, **{k: float(i) for k, i in bind.items()}}
+                env = dict(g.consts)
+                if self.fold_bind:
+                    env.update({k: float(i) for k, i in bind.items()})
                 return self.num(_const_eval(e, env, line))
             except GenError:
                 pass
@@ -670,9 +813,10 @@ class _Emitter:
         raise GenError(f"line {line}: cannot emit {e!r}")
 
 
-def _logdensity(d: Draw, a):
+def _logdensity(d: Draw, a, lsg=None):
     """Statements adding the log-density of xy to lp, in double
-    (share/tt/cpp/action/*.hpp.tt `logdensity`, share/src/bi/pdf/functor.hpp)."""
+    (share/tt/cpp/action/*.hpp.tt `logdensity`, share/src/bi/pdf/functor.hpp);
+    lsg names log(sd) of a gaussian when it is the same for every element."""
     if d.dist == "gaussian" and d.log:
         return [f"const double mu = {a[0]};", f"const double sg = {a[1]};",
                 "const double z = (log(xy) - mu) / sg;",
@@ -683,7 +827,7 @@ def _logdensity(d: Draw, a):
                 "  lp = xy == mu ? __builtin_inf() : -__builtin_inf();",
                 "} else {",
                 "  const double z = (xy - mu) / sg;",
-                "  lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg);",
+                f"  lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - {lsg or 'log(sg)'};",
                 "}"]
     if d.dist == "uniform":
         return [f"const double mn = {a[0]}, mx = {a[1]};",
@@ -730,6 +874,22 @@ def _logdensity(d: Draw, a):
                 "  lp += r;",
                 "}"]
     raise GenError(f"no log-density for {d.dist}")
+
+
+def _hoist(args_by_elem, key, w, ind):
+    """Arguments that are the same for every element are computed once: returns
+    the argument strings to use and the name of log(sd) when the second
+    argument of a gaussian is invariant (None otherwise)."""
+    first = args_by_elem[0]
+    same = [all(a[i] == first[i] for a in args_by_elem) for i in range(len(first))]
+    if len(args_by_elem) < 2:
+        return first, [False] * len(first)
+    names = []
+    for i, (arg, inv) in enumerate(zip(first, same)):
+        if inv:
+            w(f"{ind}const double h{key}_{i} = {arg};")
+        names.append(f"h{key}_{i}" if inv else None)
+    return names, same
 
 
 def _elements(g: GenModel, st: Step):
@@ -810,8 +970,335 @@ def _draw_buffers(g, block, step, tag, ind, w, member=False):
           f"{step}, {tag}, batch, {UNIFORM_BLOCK:#x}u + {k}u));")
 
 
+class _LaneEmitter(_Emitter):
+    """Expressions of local element l of a lane's slice: the state at offset c
+    of the alias is x[l + c] or a halo value from a neighbouring lane (named
+    by halo()), vectors over the state's dimension are the lane's slices, an
+    alias used as a number is the element's global index lo + l.  In an
+    observation the alias is the runtime obs index j_ and the state element
+    is the lane's x[l]."""
+    fold_bind = False
+
+    def __init__(self, g, ty, xname, DL, halo=None, obs=False, xdouble=False):
+        super().__init__(g, ty, xname, xdouble)
+        self.DL, self.halo, self.obs = DL, halo, obs
+
+    def ref(self, e, bind, elem, line):
+        g, n = self.g, e.name
+        l = elem
+        if n in bind:
+            v = "j_" if self.obs else f"(lo + {l})"
+            return f"T({v})" if self.ty == "T" else f"(double)({v})"
+        v = g.vars.get(n)
+        if v is None:
+            raise GenError(f"line {line}: cannot read {n!r} here")
+        if v.kind in ("param", "input"):
+            return f"pt_{n}" if self.ty == "T" else f"p_{n}"
+        if v.kind == "noise":
+            r = f"w_{n}" + (f"[{l}]" if v.dim else "")
+            return f"(double){r}" if self.ty == "double" else r
+        # the state
+        if self.obs:
+            q = 0
+        elif e.index is None:
+            q = l
+        else:
+            alias = next(iter(bind)) if bind else None
+            _, c = _index_affine(g, e.index[0], alias, line)
+            q = l + c
+        if 0 <= q < self.DL or self.obs:
+            r = f"{self.x}[{l if self.obs else q}]"
+        else:
+            r = self.halo(q)
+        return f"(double){r}" if self.ty == "double" and not self.xdouble else r
+
+
+def _generate_lanes(g: GenModel) -> str:
+    """The device functor of a model split over lane groups (g.lpp lanes per
+    particle, DL = D / lpp state elements per lane)."""
+    x = next(v for v in g.vars.values() if v.kind == "state")
+    D, L = x.size, g.lpp
+    DL = D // L
+    out = []
+    w = out.append
+    w(f"// ---- model {g.name}: generated from its .bi file ({L} lanes per particle)")
+    w("template <class T>")
+    w(f"struct {g.cname} {{")
+    w("  using real_t = T;")
+    w(f"  static constexpr int D = {D}, LPP = {L}, DL = {DL}, NOBS = {g.NOBS};")
+    w("  static constexpr bool NOISE_FIRST = false, GENERATED = true;")
+    w(f"  static constexpr bool HAS_FORCING = {'true' if _forcing(g) else 'false'};")
+    w(f"  static constexpr int MIN_WAVES = sizeof(T) == 8 ? {_min_waves(g, DL)} : 1;")
+    w(f"  static constexpr double DELTA = {_lit(g.delta)};")
+    slots = [v for v in g.vars.values() if v.kind in ("param", "input")]
+    noises = [v for v in g.vars.values() if v.kind == "noise"]
+    for v in slots:
+        w(f"  double p_{v.name};")
+        w(f"  T pt_{v.name};")
+    for v in noises:
+        w(f"  T w_{v.name}" + (f"[{DL}];" if v.dim else ";"))
+    nn_t, nu_t = g.normals.get("transition", 0), g.uniforms.get("transition", 0)
+    nbt = (nn_t + 3) // 4
+    for b in sorted({st.draw.index // 4 for st in g.pre + g.post
+                     if st.kind == "draw" and st.draw.dist == "gaussian"
+                     and g.vars[st.target].dim is None}):
+        w(f"  T zq{b}[4];")
+    init = [f"p_{v.name}(a.par[{v.offset}]), pt_{v.name}((T)a.par[{v.offset}])" for v in slots]
+    init += [f"w_{v.name}(T(0))" for v in noises if not v.dim]
+    w(f"  __device__ __forceinline__ explicit {g.cname}(const ModelArgs& a)"
+      + (f" : {', '.join(init)}" if init else "") + " {")
+    for v in noises:
+        if v.dim:
+            w(f"    for (int i = 0; i < DL; ++i) w_{v.name}[i] = T(0);")
+    w("  }")
+
+    # rhs with the halo from the neighbouring lanes of the group
+    need = {}
+
+    def halo(q):
+        if q < 0:
+            name = f"hp{DL + q}"
+            need[name] = f"grp.prev(x[{DL + q}])"
+        else:
+            name = f"hn{q - DL}"
+            need[name] = f"grp.next(x[{q - DL}])"
+        return name
+
+    em = _LaneEmitter(g, "T", "x", DL, halo)
+    forced = _forcing(g)
+    for fname, forms in (("rhs", None), ("rhs_unforced", forced)):
+        if fname == "rhs_unforced" and forced is None:
+            break
+        need.clear()
+        rhs = ["T(0)"] * DL
+        for n, st in enumerate(g.ode):
+            e = st.expr if forms is None else forms[1][n]
+            for l in range(DL):
+                rhs[l] = em.expr(e, {st.alias: l} if st.alias else {}, l, st.line)
+        w("  template <class G>")
+        w(f"  __device__ __forceinline__ void {fname}(const G& grp, const T (&x)[DL], "
+          "T (&k)[DL]) const {")
+        w("    const int lo = grp.g * DL;")
+        w("    (void)lo;")
+        for name, val in sorted(need.items()):
+            w(f"    const T {name} = {val};")
+        for l in range(DL):
+            w(f"    k[{l}] = {rhs[l]};")
+        w("  }")
+    if forced is not None:
+        w(f"  __device__ __forceinline__ T forcing() const {{ return pt_{forced[0]}; }}")
+
+    # discrete actions on the lane's elements (vector actions read the values from before them)
+    def actions(steps, ind, ty, xname, sctx):
+        lines = []
+        el = _LaneEmitter(g, ty, xname, DL, xdouble=(ty == "double"))
+        for st in steps:
+            v = g.vars[st.target]
+            n = DL if v.dim else 1
+            vals = []
+            for l in range(n):
+                bind = {st.alias: l} if st.alias else {}
+                if st.kind == "draw":
+                    vals.append(_lane_draw(el, st.draw, l, bind, st.line, sctx, v.dim is not None))
+                else:
+                    vals.append(el.expr(st.expr, bind, l, st.line))
+            tgt = (lambda l: f"{xname}[{l}]") if v.kind == "state" else (
+                (lambda l: f"w_{v.name}[{l}]") if v.dim else (lambda l: f"w_{v.name}"))
+            if v.dim is None:
+                lines.append(f"{ind}{tgt(0)} = {vals[0]};")
+            else:
+                lines.append(f"{ind}{{")
+                if st.kind == "draw" and st.draw.dist == "gaussian":
+                    step, tag, _, _, nfirst, _ = sctx
+                    lines.append(f"{ind}  T zs_[{DL}];  // normals {st.draw.index} + lo .. of the firing")
+                    lines.append(f"{ind}  slice_normals<T, {DL}>(seed, p, {step}, {tag}, batch, "
+                                 f"{nfirst} + {st.draw.index}u + (uint32_t)lo, zs_);")
+                lines.append(f"{ind}  {ty} t_[{DL}];")
+                for l, val in enumerate(vals):
+                    lines.append(f"{ind}  t_[{l}] = {val};")
+                for l in range(DL):
+                    lines.append(f"{ind}  {tgt(l)} = t_[{l}];")
+                lines.append(f"{ind}}}")
+        return lines
+
+    nst = g.streams.get("transition", 0)
+    tctx = ("step", "TAG_NOISE", "TAG_NOISE", f"fire * {nst}u", f"fire * {4 * nbt}u",
+            f"fire * {nu_t}u")
+    w("  template <int INTEG, class G>")
+    w("  __device__ __forceinline__ int transition(const G& grp, T (&x)[DL], double t1, double t2,")
+    w("                                            const ModelArgs& ma, uint64_t seed,")
+    w("                                            uint32_t batch, uint32_t p, uint32_t step) {")
+    w("    const int lo = grp.g * DL;")
+    w("    (void)lo;")
+    w("    int n = 0;")
+    w("    double sa = gen_scaled(t1, DELTA);")
+    w("    const double s2 = gen_scaled(t2, DELTA);")
+    w("    double ta = t1;")
+    w("    uint32_t fire = 0;")
+    w("    while (sa < s2) {")
+    w("      const double sb = fmin(floor(sa) + 1.0, s2);")
+    w("      const double tb = sb == s2 ? t2 : sb * DELTA;")
+    w("      const bool on = sa == floor(sa);")
+    # the normal words of scalar draws (vector draws take their slice of the firing's normals)
+    sblocks = sorted({st.draw.index // 4 for st in g.pre + g.post
+                      if st.kind == "draw" and st.draw.dist == "gaussian"
+                      and g.vars[st.target].dim is None})
+    if g.pre or g.post:
+        w("      if (on) {")
+        for b in sblocks:
+            w(f"        normals4<T>(seed, p, step, TAG_NOISE, batch, fire * {nbt}u + {b}u, zq{b});")
+        for ln in actions(g.pre, "        ", "T", "x", tctx):
+            w(ln)
+        w("      }")
+    if g.has_ode:
+        w("      if constexpr (INTEG == BIHIP_RK4) n += integrate_rk4(*this, grp, x, ta, tb, ma.h0);")
+        w("      else n += integrate_dopri5(*this, grp, x, ta, tb, ma);")
+    if g.post:
+        w("      if (on) {")
+        for ln in actions(g.post, "        ", "T", "x", tctx):
+            w(ln)
+        w("      }")
+    w("      fire += on ? 1u : 0u;")
+    w("      sa = sb;")
+    w("      ta = tb;")
+    w("    }")
+    w("    return n;")
+    w("  }")
+
+    # observation terms of the lane's state elements: element s = lo + l is read
+    # by obs element j when s = a j + b
+    eo = _LaneEmitter(g, "double", "x", DL, obs=True)
+    w("  __device__ __forceinline__ double obs(const T (&x)[DL], const double* y, int lo) const {")
+    w("    double lp = 0.0;")
+    for k_, st in enumerate(g.observation):
+        d, v = st.draw, g.vars[st.target]
+        idx = set()
+        for e in d.args:
+            for r in _refs(_expand(g, e), []):
+                u = g.vars.get(r.name)
+                if u is not None and u.kind == "state":
+                    idx.add(_index_affine(g, r.index[0], st.alias, st.line))
+        a, b = idx.pop() if idx else (0, 0)
+        bind = {st.alias: 0} if st.alias else {}
+        per = [[eo.expr(x_, bind, l, st.line) for x_ in d.args] for l in range(DL)]
+        names, same = _hoist(per, k_, w, "    ")
+        lsg = None
+        if d.dist == "gaussian" and not d.log and same[1]:
+            lsg = f"l{k_}"
+            w(f"    const double {lsg} = log({names[1]});")
+        for l in range(DL):
+            args = [nm if sm else a_ for nm, sm, a_ in zip(names, same, per[l])]
+            w("    {")
+            w(f"      const int s_ = lo + {l};")
+            if a == 0:
+                w(f"      const int j_ = 0;")
+                w(f"      if (s_ == {b}) {{")
+            else:
+                w(f"      const int j_ = (s_ - {b}) / {a};")
+                w(f"      if ((s_ - {b}) % {a} == 0 && j_ >= 0 && j_ < {v.size}) {{")
+            w(f"        const double xy = y[{v.offset} + j_];")
+            for ln in _logdensity(d, args, lsg):
+                w("        " + ln)
+            w("      }")
+            w("    }")
+    w("    return lp;")
+    w("  }")
+    _emit_obs_sample(g, w)
+
+    # initial states of the lane's elements, in double
+    w("  __device__ __forceinline__ void init(double (&xd)[DL], uint64_t seed, uint32_t batch,")
+    w("                                       uint32_t p, int lo) const {")
+    w("    for (int i = 0; i < DL; ++i) xd[i] = 0.0;")
+    ictx = ("0u", "TAG_INIT", "TAG_INIT", "0u", "0u", "0u")
+    for ln in actions(g.initial, "    ", "double", "xd", ictx):
+        w(ln)
+    w("  }")
+    w("};")
+    return "\n".join(out)
+
+
+def _lane_draw(em, d: Draw, l, bind, line, sctx, vector):
+    """Draw of local element l (vector: global element lo + l; scalar: the same
+    draw in every lane).  sctx = (step, tag, tag, first stream, first normal,
+    first uniform) of the firing."""
+    step, tag, _, sfirst, nfirst, ufirst = sctx
+    a = [em.expr(x_, bind, l, line) for x_ in d.args]
+    k = f"{d.index}u + (uint32_t)(lo + {l})" if vector else f"{d.index}u"
+    if d.dist in STREAM_DISTS:
+        ad = ", ".join(f"(double)({x_})" for x_ in a)
+        v = (f"gen_{d.dist}({ad}, DrawStream(seed, p, {step}, {tag}, batch, "
+             f"{STREAM_BLOCK:#x}u + (({sfirst} + {k}) << 12)))")
+        return f"T({v})" if em.ty == "T" else v
+    if d.dist == "gaussian":
+        z = f"zs_[{l}]" if vector else _zname(d.index)
+        z = z if em.ty == "T" else f"(double){z}"
+        v = f"({a[0]} + ({a[1]} * {z}))"
+        return f"exp({v})" if d.log else v
+    u = (f"uniform_word<T>(philox_block(seed, p, {step}, {tag}, batch, {UNIFORM_BLOCK:#x}u + "
+         f"{ufirst} + {k}))")
+    u = u if em.ty == "T" else f"(double){u}"
+    if d.dist == "exponential":
+        one = em.num(1.0)
+        return f"((-log({one} - {u})) / {a[0]})"
+    return f"({a[0]} + (({a[1]} - {a[0]}) * {u}))"
+
+
+def _forcing(g: GenModel):
+    """(F, {equation index: g}) when every state element's equation is
+    g + F with F one scalar param or input: the integrators then carry the
+    constant forcing themselves in FP32 (HAS_FORCING, as the hand-written
+    Lorenz-96 functor does); otherwise None."""
+    if not g.ode or sum(g.vars[st.target].size for st in g.ode) != g.D:
+        return None
+    name, rest = None, {}
+    for n, st in enumerate(g.ode):
+        e = st.expr
+        if not (isinstance(e, Binary) and e.op == "+" and isinstance(e.right, Ref)
+                and e.right.index is None):
+            return None
+        v = g.vars.get(e.right.name)
+        if v is None or v.kind not in ("param", "input") or v.dim is not None:
+            return None
+        if name not in (None, v.name):
+            return None
+        name, rest[n] = v.name, e.left
+    return name, rest
+
+
+def _min_waves(g: GenModel, DL: int) -> int:
+    """Waves per SIMD asked of the compiler for the double instance.  A
+    double slice of DL elements through DOPRI5 holds about nine arrays of DL
+    (two registers each) plus some 70 registers of controller and addresses,
+    RK4 about six: when that estimate falls just above 256 (at most 300), a
+    cap at 256 registers costs a few spills and doubles the waves per SIMD
+    (the 40-element Lorenz-96 slice of 10: 268 registers uncapped, 256 with 8
+    spills, 206 -> 136 us per step).  Float instances keep the compiler's
+    choice (capped at 128 the same model spilled 14 and slowed)."""
+    arrays = 9 if g.integrator == ALGS["RK5(4)"] else 6
+    return 2 if 2 * arrays * DL + 70 <= 300 else 1
+
+
+def _emit_obs_sample(g: GenModel, w):
+    """Observation samples (Model::observationSamples) from a particle's whole
+    state x[D] (one thread per particle); y has NOBS entries."""
+    eo = _Emitter(g, "double", "x")
+    w("  __device__ __forceinline__ void obs_sample(const T (&x)[D], uint64_t seed, uint32_t batch,")
+    w("                                             uint32_t p, uint32_t step, double* y) const {")
+    _draw_buffers(g, "observation", "step", "TAG_OBS", "    ", w)
+    for j in range(g.NOBS):
+        w(f"    y[{j}] = __builtin_nan(\"\");")
+    for st in g.observation:
+        v = g.vars[st.target]
+        for i, bind in _elements(g, st):
+            w(f"    y[{v.offset + i}] = "
+              f"{_draw_value(eo, st.draw, i, bind, st.line, ('step', 'TAG_OBS', '0u'))};")
+    w("  }")
+
+
 def generate_model(g: GenModel) -> str:
     """The device functor of one model."""
+    if g.lpp > 1:
+        return _generate_lanes(g)
     S, NO = g.D, g.NOBS
     out = []
     w = out.append
@@ -820,12 +1307,15 @@ def generate_model(g: GenModel) -> str:
     w(f"struct {g.cname} {{")
     w("  using real_t = T;")
     w(f"  static constexpr int D = {S}, LPP = 1, DL = {S}, NOBS = {NO};")
-    w("  static constexpr bool NOISE_FIRST = false, HAS_FORCING = false, GENERATED = true;")
+    w("  static constexpr bool NOISE_FIRST = false, GENERATED = true;")
+    w(f"  static constexpr bool HAS_FORCING = {'true' if _forcing(g) else 'false'};")
+    w(f"  static constexpr int MIN_WAVES = sizeof(T) == 8 ? {_min_waves(g, S)} : 1;")
     w(f"  static constexpr double DELTA = {_lit(g.delta)};")
     slots = [v for v in g.vars.values() if v.kind in ("param", "input")]
     noises = [v for v in g.vars.values() if v.kind == "noise"]
-    for v in slots:
+    for v in slots:  # in double, and in the state type for the transition's arithmetic
         w(f"  double p_{v.name}" + (f"[{v.size}];" if v.dim else ";"))
+        w(f"  T pt_{v.name}" + (f"[{v.size}];" if v.dim else ";"))
     for v in noises:
         w(f"  T w_{v.name}" + (f"[{v.size}];" if v.dim else ";"))
     nn_t, nu_t = g.normals.get("transition", 0), g.uniforms.get("transition", 0)
@@ -835,13 +1325,15 @@ def generate_model(g: GenModel) -> str:
         w(f"  T zq{b}[4];")
     for k in range(nu_t):
         w(f"  T u{k};")
-    init = [f"p_{v.name}(a.par[{v.offset}])" for v in slots if not v.dim]
+    init = [f"p_{v.name}(a.par[{v.offset}]), pt_{v.name}((T)a.par[{v.offset}])"
+            for v in slots if not v.dim]
     init += [f"w_{v.name}(T(0))" for v in noises if not v.dim]
     w(f"  __device__ __forceinline__ explicit {g.cname}(const ModelArgs& a)"
       + (f" : {', '.join(init)}" if init else "") + " {")
     for v in slots:
         if v.dim:
             w(f"    for (int i = 0; i < {v.size}; ++i) p_{v.name}[i] = a.par[{v.offset} + i];")
+            w(f"    for (int i = 0; i < {v.size}; ++i) pt_{v.name}[i] = (T)a.par[{v.offset} + i];")
     for v in noises:
         if v.dim:
             w(f"    for (int i = 0; i < {v.size}; ++i) w_{v.name}[i] = T(0);")
@@ -849,16 +1341,24 @@ def generate_model(g: GenModel) -> str:
 
     # rhs: the ode block (share/tt/cpp/action/ode_.hpp.tt); states without an equation stay put
     em = _Emitter(g, "T", "x")
-    rhs = ["T(0)"] * S
-    for st in g.ode:
-        v = g.vars[st.target]
-        for i, bind in _elements(g, st):
-            rhs[v.offset + i] = em.expr(st.expr, bind, i, st.line)
-    w("  template <class G>")
-    w("  __device__ __forceinline__ void rhs(const G&, const T (&x)[DL], T (&k)[DL]) const {")
-    for j in range(S):
-        w(f"    k[{j}] = {rhs[j]};")
-    w("  }")
+    forced = _forcing(g)
+    for fname, forms in (("rhs", None), ("rhs_unforced", forced)):
+        if fname == "rhs_unforced" and forced is None:
+            break
+        rhs = ["T(0)"] * S
+        for n, st in enumerate(g.ode):
+            v = g.vars[st.target]
+            e = st.expr if forms is None else forms[1][n]
+            for i, bind in _elements(g, st):
+                rhs[v.offset + i] = em.expr(e, bind, i, st.line)
+        w("  template <class G>")
+        w(f"  __device__ __forceinline__ void {fname}(const G&, const T (&x)[DL], "
+          "T (&k)[DL]) const {")
+        for j in range(S):
+            w(f"    k[{j}] = {rhs[j]};")
+        w("  }")
+    if forced is not None:
+        w(f"  __device__ __forceinline__ T forcing() const {{ return pt_{forced[0]}; }}")
 
     nst = g.streams.get("transition", 0)
     tsctx = ("step", "TAG_NOISE", f"fire * {nst}u")
@@ -903,36 +1403,33 @@ def generate_model(g: GenModel) -> str:
 
     # observation log-density (share/tt/cpp/action/gaussian.hpp.tt:48-73, uniform.hpp.tt:38-52)
     eo = _Emitter(g, "double", "x")
-    w("  __device__ __forceinline__ double obs(const T (&x)[DL], const double* y) const {")
+    w("  __device__ __forceinline__ double obs(const T (&x)[DL], const double* y, int) const {")
     w("    double lp = 0.0;")
-    for st in g.observation:
+    for ka, st in enumerate(g.observation):
         d, v = st.draw, g.vars[st.target]
-        for i, bind in _elements(g, st):
+        elems = list(_elements(g, st))
+        per = [[eo.expr(x, bind, i, st.line) for x in d.args] for i, bind in elems]
+        names, same = _hoist(per, ka, w, "    ")
+        lsg = None
+        if d.dist == "gaussian" and not d.log and len(same) > 1 and same[1]:
+            lsg = f"l{ka}"
+            w(f"    const double {lsg} = log({names[1]});")
+        for (i, bind), args in zip(elems, per):
+            args = [nm if sm else a for nm, sm, a in zip(names, same, args)]
             w("    {")
             w(f"      const double xy = y[{v.offset + i}];")
-            for ln in _logdensity(d, [eo.expr(x, bind, i, st.line) for x in d.args]):
+            for ln in _logdensity(d, args, lsg):
                 w("      " + ln)
             w("    }")
     w("    return lp;")
     w("  }")
 
-    # observation samples (Model::observationSamples), y has NOBS entries
-    w("  __device__ __forceinline__ void obs_sample(const T (&x)[DL], uint64_t seed, uint32_t batch,")
-    w("                                             uint32_t p, uint32_t step, double* y) const {")
-    _draw_buffers(g, "observation", "step", "TAG_OBS", "    ", w)
-    for j in range(NO):
-        w(f"    y[{j}] = __builtin_nan(\"\");")
-    for st in g.observation:
-        v = g.vars[st.target]
-        for i, bind in _elements(g, st):
-            w(f"    y[{v.offset + i}] = "
-              f"{_draw_value(eo, st.draw, i, bind, st.line, ('step', 'TAG_OBS', '0u'))};")
-    w("  }")
+    _emit_obs_sample(g, w)
 
     # initial states, in double (share/tt/cpp/block/initial.hpp.tt)
     ei = _Emitter(g, "double", "xd", xdouble=True)
     w("  __device__ __forceinline__ void init(double (&xd)[DL], uint64_t seed, uint32_t batch,")
-    w("                                       uint32_t p) const {")
+    w("                                       uint32_t p, int) const {")
     for j in range(S):
         w(f"    xd[{j}] = 0.0;")
     _draw_buffers(g, "initial", "0u", "TAG_INIT", "    ", w)
@@ -976,13 +1473,13 @@ def generate(models: List[GenModel]) -> str:
     for i, g in enumerate(models):
         names = ", ".join(f"GEN_{i}_{k.upper()}" for k in KIND_CODE)
         sizes = ", ".join(f"GEN_{i}_{k.upper()}_SIZES" for k in KIND_CODE)
-        out.append(f'    {{"{g.name}", {g.D}, {g.NOBS}, {g.integrator}, '
+        out.append(f'    {{"{g.name}", {g.D}, {g.NOBS}, {g.integrator}, {g.lpp}, '
                    f"{_lit(g.h)}, {_lit(g.atol)}, {_lit(g.rtol)}, {_lit(g.delta)}, "
                    f"{{{names}}}, {{{sizes}}}, GEN_{i}_STATE_DIMS}},")
     if not models:
         none = ", ".join(["GEN_NONE"] * len(KIND_CODE))
         nsz = ", ".join(["GEN_NONE_SIZES"] * len(KIND_CODE))
-        out.append(f'    {{"", 0, 0, 0, 0.0, 0.0, 0.0, 1.0, {{{none}}}, {{{nsz}}}, GEN_NONE}},')
+        out.append(f'    {{"", 0, 0, 0, 1, 0.0, 0.0, 0.0, 1.0, {{{none}}}, {{{nsz}}}, GEN_NONE}},')
     out.append("};")
     out.append("")
     out.append("// F::template run<M>() for the generated model of m")

@@ -92,7 +92,7 @@ def test_generated_expressions_keep_the_tree():
     assert "k[0] = ((w_alpha * x[0]) - ((T(0.25) * x[0]) * x[1]));" in src
     assert ("k[1] = (((((T(0.3) * T(0.25)) * x[0]) * x[1]) - (T(0.1) * x[1])) - "
             "((T(0.1) * x[1]) * x[1]));") in src
-    assert "w_alpha = (T(p_mu) + (T(p_sigma) * zq0[0]));" in src
+    assert "w_alpha = (pt_mu + (pt_sigma * zq0[0]));" in src
     # log(2.0) folded to the correctly rounded double
     assert f"exp(({float(np.log(2.0))!r} + (0.2 * (double)zq0[0])))" in src
     assert "lp += -0.5 * (z * z) - HALF_LOG_TWO_PI - log(sg * xy);" in src
@@ -194,7 +194,7 @@ namespace bihip {
 struct ModelArgs { double h0; double par[16]; };
 struct bihip_model_ { int model, precision; };
 struct u32x4 { uint32_t x, y, z, w; };
-struct GenInfo { const char* name; int d, nobs, integrator; double h, atol, rtol, delta;
+struct GenInfo { const char* name; int d, nobs, integrator, lpp; double h, atol, rtol, delta;
   const char* const* names[5]; const int* sizes[5]; const char* const* state_dims; };
 constexpr double HALF_LOG_TWO_PI = 0.91893853320467274178;
 constexpr uint32_t TAG_INIT = 1, TAG_NOISE = 2, TAG_OBS = 6;
@@ -219,6 +219,11 @@ inline u32x4 philox_block(uint64_t seed, uint32_t i, uint32_t s, uint32_t t, uin
 }
 inline double cospi(double x) { return std::cos(3.14159265358979323846 * x); }
 template <class T> T uniform_word(const u32x4&) { return T(0); }
+template <class T, int DL> void slice_normals(uint64_t, uint32_t, uint32_t, uint32_t, uint32_t,
+                                              int, T (&z)[DL]) { for (int i = 0; i < DL; ++i) z[i] = T(0); }
+struct Grp { int g = 0;
+  template <class T> T prev(T v) const { return v; }
+  template <class T> T next(T v) const { return v; } };
 template <class M, class G, class T> int integrate_rk4(const M&, const G&, T (&)[M::DL],
                                                        double, double, double) { return 0; }
 template <class M, class G, class T> int integrate_dopri5(const M&, const G&, T (&)[M::DL],
@@ -252,14 +257,14 @@ def test_generated_header_is_standalone_cxx(tmp_path):
 namespace bihip {
 struct Runner { template <class M> int run() {
   M m(ModelArgs{});
-  double x[M::DL] = {}; typename M::real_t xt[M::DL] = {}, k[M::DL];
-  int g = 0;
+  double x[M::DL] = {}; typename M::real_t xt[M::DL] = {}, k[M::DL], xf[M::D] = {};
+  Grp g;
   m.rhs(g, xt, k);
   m.template transition<0>(g, xt, 0.0, 1.0, ModelArgs{}, 1, 0, 0, 1);
   double y[M::NOBS > 0 ? M::NOBS : 1] = {};
-  (void)m.obs(xt, y);
-  m.obs_sample(xt, 1, 0, 0, 1, y);
-  m.init(x, 1, 0, 0);
+  (void)m.obs(xt, y, 0);
+  m.obs_sample(xf, 1, 0, 0, 1, y);
+  m.init(x, 1, 0, 0, 0);
   return 0; } };
 int check(const bihip_model* m) { Runner r; return with_generated(m, r); }
 }
@@ -330,7 +335,7 @@ def test_observation_densities_against_scipy(tmp_path):
                 body.append(f"  {{ bihip::{g.cname}<double> m(ma); double x[1] = {{{xv!r}}};"
                             f" double y[1] = {{{float(yv)!r}}};"
                             f' std::printf("{name} %.17g %.17g %.17g\\n", x[0], y[0],'
-                            f" m.obs(x, y)); }}")
+                            f" m.obs(x, y, 0)); }}")
     body.append("  return 0;\n}")
     src = "\n".join(codegen.generate_model(g) for g in models)
     out = _gxx(tmp_path, "namespace bihip {\n" + src + "\n}\n" + "\n".join(body), run=True)
@@ -350,8 +355,8 @@ def test_draws_from_every_distribution():
              " sub transition { w ~ exponential(l)\n v ~ gamma(2.0, l)\n x <- x + w }\n"
              " sub observation { c ~ poisson(x) } }")
     src = codegen.generate_model(g)
-    assert "w_w = ((-log(T(1.0) - u0)) / T(p_l));" in src
-    assert ("t_[1] = T(gen_gamma((double)(T(2.0)), (double)(T(p_l)), DrawStream(seed, p, step, "
+    assert "w_w = ((-log(T(1.0) - u0)) / pt_l);" in src
+    assert ("t_[1] = T(gen_gamma((double)(T(2.0)), (double)(pt_l), DrawStream(seed, p, step, "
             "TAG_NOISE, batch, 0x20000000u + ((fire * 2u + 1u) << 12))));") in src
     assert ("y[0] = gen_poisson((double)((double)x[0]), DrawStream(seed, p, step, TAG_OBS, "
             "batch, 0x20000000u + ((0u + 0u) << 12)));") in src

@@ -517,3 +517,55 @@ def test_generated_stochastic_sir_filter(ctx):
     m = capi.generated_model("SIRStochastic", precision=capi.F64, beta=beta, gamma=gamma, rho=rho)
     zs = [_run(ctx, m, y, 1 << 16, 1.0, s, record=False)[0] for s in (1, 2)]
     assert all(np.isfinite(zs)) and abs(zs[0] - zs[1]) < 0.5, zs
+
+
+def test_generated_lorenz96_d40_lane_groups_match_oracle(ctx, orc):
+    """Lorenz96d40.bi runs four lanes per particle with the stencil's
+    neighbours exchanged between lanes: FP64 against the oracle's Lorenz-96
+    filter (N = 4096, T = 20, ESS < 0.5 N), ancestors bit-exact."""
+    from gpu_helpers import l96_spun_up
+    par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+    g = capi.generated_model("Lorenz96d40", precision=capi.F64, **par)
+    h = capi.make_model("l96", d=40, precision=capi.F64, atol=g.atol, rtol=g.rtol, h0=g.h0, **par)
+    om = oracle_model(orc, h)
+    _, y = orc.simulate(om, 7, 20, 0.05, l96_spun_up(orc))
+    z, out, anc = _run(ctx, g, y, 4096, 0.05, 9, ess_rel=0.5)
+    ref = orc.pf_run(om, y, 4096, 0.05, 9, ess_rel=0.5, mean0=np.zeros(40), want_ancestors=True)
+    assert out["resampled"].any()
+    assert z == pytest.approx(ref["logZ"], rel=1e-10)
+    np.testing.assert_allclose(out["logIncrements"], ref["logIncrements"], rtol=1e-9, atol=1e-8)
+    np.testing.assert_array_equal(anc, ref["ancestors"])
+
+
+def test_generated_lorenz96_d40_fp32_and_samples(ctx, orc, torch_gpu):
+    """FP32 against the hand-written four-lane kernel (no resampling, logZ to
+    1e-5); initial and observation samples of the lane layout equal the
+    hand-written model's."""
+    from gpu_helpers import l96_spun_up
+    par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+    g = capi.generated_model("Lorenz96d40", precision=capi.F32, **par)
+    h = capi.make_model("l96", d=40, precision=capi.F32, atol=g.atol, rtol=g.rtol, h0=g.h0, **par)
+    om = oracle_model(orc, h)
+    _, y = orc.simulate(om, 7, 10, 0.05, l96_spun_up(orc))
+    P, T = 4096, 10
+    pf = capi.ParticleFilter(ctx, h, P, T, 0.05, 4, ess_rel=0.0)
+    pf.set_observations(y)
+    pf.init(np.zeros(40))
+    pf.step(T)
+    zh = pf.term()
+    pf.close()
+    zg, og, _ = _run(ctx, g, y, P, 0.05, 4, ess_rel=0.0, record=False)
+    assert not og["resampled"].any()
+    assert zg == pytest.approx(zh, rel=1e-5)
+    L, c = capi.lib(), (lambda t: ctypes.c_void_p(t.data_ptr()))
+    m0 = np.zeros(40)
+    outs = []
+    for m in (g, h):
+        X = empty((40, P), torch_gpu.float32)
+        capi.check(L.bihip_init_samples(ctx.h, ctypes.byref(m), 21, 3, P, c(X), P,
+                                        m0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        Y = empty((P, 20), torch_gpu.float64)
+        capi.check(L.bihip_obs_sample(ctx.h, ctypes.byref(m), 21, 3, 5, P, c(X), P, c(Y)))
+        outs.append((host(X), host(Y)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-6, atol=1e-6)
