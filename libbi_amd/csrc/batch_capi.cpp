@@ -224,7 +224,8 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   if (m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
     return set_error(BIHIP_EINVAL, "NULL argument");
   if (!batch_model_supported(m))
-    return set_error(BIHIP_EINVAL, "batched filters support the PZ and LG models");
+    return set_error(BIHIP_EINVAL,
+                     "batched filters support the PZ and LG models and one-lane generated models");
   if (cfg->B <= 0 || cfg->T <= 0) return set_error(BIHIP_EINVAL, "need B > 0 and T > 0");
   if (cfg->resampler != BIHIP_SYSTEMATIC && cfg->resampler != BIHIP_STRATIFIED)
     return set_error(BIHIP_EINVAL, "batched filters resample systematically or stratified");

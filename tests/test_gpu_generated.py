@@ -569,3 +569,13 @@ def test_generated_lorenz96_d40_fp32_and_samples(ctx, orc, torch_gpu):
         outs.append((host(X), host(Y)))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_generated_lane_group_model_rejected_by_batch(ctx):
+    """The batched inner filters take one-lane models only: a lane-group model
+    is refused at create time with an error, not run."""
+    g = capi.generated_model("Lorenz96d40", precision=capi.F64, F=8.0, noise_sd=0.1,
+                             obs_sd=1.0, init_sd=1.0)
+    with pytest.raises(capi.BihipError):
+        capi.BatchFilter(ctx, g, [256, 256], np.zeros((2, 0)), 5, 0.05, 1)
