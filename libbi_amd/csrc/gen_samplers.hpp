@@ -147,7 +147,10 @@ __device__ __forceinline__ double gen_binomial(double n, double p, DrawStream w)
     const double s = pp / q, a = (n + 1.0) * s;
     double r = pow(q, n), u = w.uniform();
     x = 0.0;
-    while (u > r && x < n) {
+    // the probabilities fall faster than geometrically past the mean, so r
+    // underflows within a few hundred terms: the walk ends there even when
+    // rounding leaves u above their sum
+    while (u > r && x < n && r > 0.0) {
       u -= r;
       x += 1.0;
       r *= a / x - s;

@@ -442,6 +442,8 @@ SAMPLER_CASES = [  # (distribution, expression of the draw, scipy distribution)
     ("binomial_small", "gen_binomial(20.0, 0.3, w)", lambda st: st.binom(20, 0.3)),
     ("binomial_large", "gen_binomial(400.0, 0.35, w)", lambda st: st.binom(400, 0.35)),
     ("binomial_flip", "gen_binomial(90.0, 0.8, w)", lambda st: st.binom(90, 0.8)),
+    # inversion over a count of 1e9: the walk stops at the tail, not at n
+    ("binomial_huge_n", "gen_binomial(1.0e9, 3.0e-9, w)", lambda st: st.binom(10**9, 3.0e-9)),
 ]
 
 
