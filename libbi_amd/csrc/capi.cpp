@@ -583,6 +583,7 @@ struct bihip_pf {
   bihip_model model{};
   bihip_pf_config cfg{};
   int ld = 0, elem = 4, nobs = 1, nparts = 0;
+  bool one_delta = false;  // gen_one_delta over the run's steps
   void* X[2] = {nullptr, nullptr};
   int cur = 0;
   double* lws = nullptr;
@@ -703,6 +704,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   T.times = pf->times;
   T.mask = pf->obs_mask;
   T.inputs = pf->inputs;
+  T.one_delta = pf->one_delta ? 1 : 0;
   if (pf->guided) {
     // exact flow of the gathered particles (no noise, no correction) into Xout,
     // then the proposal and the corrected weights in place
@@ -816,6 +818,7 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   pf->ctx = ctx;
   pf->model = *m;
   pf->cfg = *cfg;
+  pf->one_delta = gen_one_delta(m, cfg->t0, cfg->dt, cfg->T);
   const int P = cfg->P;
   pf->ld = (P + 63) / 64 * 64;
   pf->elem = bihip_model_elem_size(m);

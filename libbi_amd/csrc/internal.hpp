@@ -139,8 +139,15 @@ struct TransLaunch {
   // with dev: the model's 16 parameters for each step (nullable), LibBi's
   // inputs as set by a Forcer over [t_k, t_{k+1})
   const double* inputs;
+  // with dev and no schedule: every step's [t1, t2] is one sub-interval of a
+  // generated model's delta starting at a multiple of it (gen_one_delta);
+  // without dev, launch_transition decides from t1 and t2 itself
+  int one_delta;
 };
 int transition_blocks(const bihip_model* m, int P);
+// A generated model whose steps [t0 + k dt, t0 + (k+1) dt], k < steps, are each
+// one sub-interval of its delta starting at a multiple of it.
+bool gen_one_delta(const bihip_model* m, double t0, double dt, int steps);
 int launch_transition(const TransLaunch& L, hipStream_t s);
 int launch_init_samples(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X,
                         int ld, const double* mean0_dev, hipStream_t s);

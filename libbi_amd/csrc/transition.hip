@@ -63,7 +63,9 @@ constexpr int trans_min_waves() {
   else return 1;
 }
 
-template <class M, int INTEG, bool SCHED = false>
+// ONE: a generated model's interval is one sub-interval of its delta that
+// starts at a multiple of it (its transition runs without the sub-interval loop).
+template <class M, int INTEG, bool SCHED = false, bool ONE = false>
 __global__ __launch_bounds__(TRANS_BLOCK) __attribute__((amdgpu_waves_per_eu(trans_min_waves<M>())))
 void k_transition(TransArgs a) {
   using T = typename M::real_t;
@@ -117,7 +119,8 @@ void k_transition(TransArgs a) {
     int n;
     if constexpr (is_generated<M>::value) {
       // the model's transition block: discrete actions and its ode, in order
-      n = m.template transition<INTEG>(grp, x, t1, t2, a.m, a.seed, a.batch, (uint32_t)p, step);
+      n = m.template transition<INTEG, ONE>(grp, x, t1, t2, a.m, a.seed, a.batch, (uint32_t)p,
+                                            step);
     } else {
       if constexpr (M::NOISE_FIRST) {
         if (!a.flow_only) {
@@ -327,15 +330,45 @@ int transition_blocks(const bihip_model* m, int P) {
   return ceil_div((long)P * lpp, TRANS_BLOCK);
 }
 
+// [t1, t2] is one sub-interval of delta starting at a multiple of it, with a
+// margin (1e-6 of delta) far inside the device's rounding window (1e-4,
+// gen_scaled), so that the device's own rounding of t1 and t2 decides alike.
+static bool one_interval(double t1, double t2, double delta) {
+  const double s1 = t1 / delta, s2 = t2 / delta;
+  const double r1 = std::round(s1), r2 = std::round(s2);
+  return std::fabs(s1 - r1) < 1e-6 && std::fabs(s2 - r2) < 1e-6 && r2 == r1 + 1.0;
+}
+
+bool gen_one_delta(const bihip_model* m, double t0, double dt, int steps) {
+  const GenInfo* g = generated_info(m->model);
+  if (g == nullptr || steps <= 0) return false;
+  for (int k = 0; k < steps; ++k)
+    if (!one_interval(t0 + k * dt, t0 + (k + 1) * dt, g->delta)) return false;
+  return true;
+}
+
 struct TransitionRunner {
   TransArgs a;
   int blocks;
   hipStream_t s;
   int integ;
+  bool one;
   template <class M>
   int run() {
     const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
                                             a.inputs != nullptr);
+    if constexpr (is_generated<M>::value) {
+      if (one && !sched) {
+        if (integ == BIHIP_RK4)
+          hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, false, true>), dim3(blocks),
+                             dim3(TRANS_BLOCK), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5, false, true>), dim3(blocks),
+                             dim3(TRANS_BLOCK), 0, s, a);
+        BIHIP_LAUNCH_CHECK();
+        return BIHIP_OK;
+      }
+    }
     if (integ == BIHIP_RK4) {
       if (sched)
         hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, true>), dim3(blocks), dim3(TRANS_BLOCK), 0,
@@ -386,6 +419,10 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.blocks = transition_blocks(L.model, L.P);
   r.s = s;
   r.integ = L.model->integrator;
+  if (const GenInfo* g = generated_info(L.model->model))
+    r.one = L.dev != nullptr ? L.one_delta != 0 : one_interval(L.t1, L.t2, g->delta);
+  else
+    r.one = false;
   return with_model(L.model, r);
 }
 

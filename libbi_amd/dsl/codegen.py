@@ -1124,21 +1124,15 @@ def _generate_lanes(g: GenModel) -> str:
     nst = g.streams.get("transition", 0)
     tctx = ("step", "TAG_NOISE", "TAG_NOISE", f"fire * {nst}u", f"fire * {4 * nbt}u",
             f"fire * {nu_t}u")
-    w("  template <int INTEG, class G>")
+    w("  template <int INTEG, bool ONE = false, class G>")
     w("  __device__ __forceinline__ int transition(const G& grp, T (&x)[DL], double t1, double t2,")
     w("                                            const ModelArgs& ma, uint64_t seed,")
     w("                                            uint32_t batch, uint32_t p, uint32_t step) {")
     w("    const int lo = grp.g * DL;")
     w("    (void)lo;")
     w("    int n = 0;")
-    w("    double sa = gen_scaled(t1, DELTA);")
-    w("    const double s2 = gen_scaled(t2, DELTA);")
-    w("    double ta = t1;")
-    w("    uint32_t fire = 0;")
-    w("    while (sa < s2) {")
-    w("      const double sb = fmin(floor(sa) + 1.0, s2);")
-    w("      const double tb = sb == s2 ? t2 : sb * DELTA;")
-    w("      const bool on = sa == floor(sa);")
+    outer, body = w, []
+    w = body.append
     # the normal words of scalar draws (vector draws take their slice of the firing's normals)
     sblocks = sorted({st.draw.index // 4 for st in g.pre + g.post
                       if st.kind == "draw" and st.draw.dist == "gaussian"
@@ -1158,12 +1152,8 @@ def _generate_lanes(g: GenModel) -> str:
         for ln in actions(g.post, "        ", "T", "x", tctx):
             w(ln)
         w("      }")
-    w("      fire += on ? 1u : 0u;")
-    w("      sa = sb;")
-    w("      ta = tb;")
-    w("    }")
-    w("    return n;")
-    w("  }")
+    w = outer
+    _emit_interval_loop(w, body)
 
     # observation terms of the lane's state elements: element s = lo + l is read
     # by obs element j when s = a j + b
@@ -1265,6 +1255,42 @@ def _forcing(g: GenModel):
     return name, rest
 
 
+def _emit_interval_loop(w, body):
+    """The rest of transition(): `body` (the actions and the ode of one
+    sub-interval [ta, tb], firing when `on`, the firing's count `fire`) run
+    over the sub-intervals of [t1, t2] that the multiples of DELTA cut, or
+    once when the caller knows [t1, t2] to be one sub-interval that starts at
+    a multiple (ONE): the loop's state then costs no registers."""
+    w("    if constexpr (ONE) {")
+    w("      const bool on = true;")
+    w("      const uint32_t fire = 0;")
+    w("      const double ta = t1, tb = t2;")
+    w("      (void)on;")
+    w("      (void)fire;")
+    w("      (void)ta;")
+    w("      (void)tb;")
+    for ln in body:
+        w(ln)
+    w("      return n;")
+    w("    }")
+    w("    double sa = gen_scaled(t1, DELTA);")
+    w("    const double s2 = gen_scaled(t2, DELTA);")
+    w("    double ta = t1;")
+    w("    uint32_t fire = 0;")
+    w("    while (sa < s2) {")
+    w("      const double sb = fmin(floor(sa) + 1.0, s2);")
+    w("      const double tb = sb == s2 ? t2 : sb * DELTA;")
+    w("      const bool on = sa == floor(sa);")
+    for ln in body:
+        w(ln)
+    w("      fire += on ? 1u : 0u;")
+    w("      sa = sb;")
+    w("      ta = tb;")
+    w("    }")
+    w("    return n;")
+    w("  }")
+
+
 def _min_waves(g: GenModel, DL: int) -> int:
     """Waves per SIMD asked of the compiler for the double instance.  A
     double slice of DL elements through DOPRI5 holds about nine arrays of DL
@@ -1363,19 +1389,13 @@ def generate_model(g: GenModel) -> str:
     nst = g.streams.get("transition", 0)
     tsctx = ("step", "TAG_NOISE", f"fire * {nst}u")
     w("  // one interval [t1, t2]: discrete actions at multiples of DELTA, the ode in between")
-    w("  template <int INTEG, class G>")
+    w("  template <int INTEG, bool ONE = false, class G>")
     w("  __device__ __forceinline__ int transition(const G& grp, T (&x)[DL], double t1, double t2,")
     w("                                            const ModelArgs& ma, uint64_t seed,")
     w("                                            uint32_t batch, uint32_t p, uint32_t step) {")
     w("    int n = 0;")
-    w("    double sa = gen_scaled(t1, DELTA);")
-    w("    const double s2 = gen_scaled(t2, DELTA);")
-    w("    double ta = t1;")
-    w("    uint32_t fire = 0;")
-    w("    while (sa < s2) {")
-    w("      const double sb = fmin(floor(sa) + 1.0, s2);")
-    w("      const double tb = sb == s2 ? t2 : sb * DELTA;")
-    w("      const bool on = sa == floor(sa);")
+    outer, body = w, []
+    w = body.append
     if g.pre or g.post:
         w("      if (on) {")
         for b in range(nbt):
@@ -1394,12 +1414,8 @@ def generate_model(g: GenModel) -> str:
         for ln in _actions(g, g.post, em, "x", "        ", "T", tsctx):
             w(ln)
         w("      }")
-    w("      fire += on ? 1u : 0u;")
-    w("      sa = sb;")
-    w("      ta = tb;")
-    w("    }")
-    w("    return n;")
-    w("  }")
+    w = outer
+    _emit_interval_loop(w, body)
 
     # observation log-density (share/tt/cpp/action/gaussian.hpp.tt:48-73, uniform.hpp.tt:38-52)
     eo = _Emitter(g, "double", "x")
