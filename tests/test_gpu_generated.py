@@ -579,3 +579,33 @@ def test_generated_lane_group_model_rejected_by_batch(ctx):
                              obs_sd=1.0, init_sd=1.0)
     with pytest.raises(capi.BihipError):
         capi.BatchFilter(ctx, g, [256, 256], np.zeros((2, 0)), 5, 0.05, 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [capi.F32, capi.F64])
+def test_generated_one_delta_path_equals_sub_interval_loop(ctx, orc, prec):
+    """Steps of one delta run the generated transition without its
+    sub-interval loop; the same times given as a schedule take the loop.
+    Both give the same log-likelihood, states and ancestors, bit for bit
+    (Lorenz96d40.bi over lane groups, FP32 and FP64)."""
+    from gpu_helpers import l96_spun_up
+    par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+    g = capi.generated_model("Lorenz96d40", precision=prec, **par)
+    h = capi.make_model("l96", d=40, precision=capi.F64, atol=g.atol, rtol=g.rtol, h0=g.h0, **par)
+    _, y = orc.simulate(oracle_model(orc, h), 7, 12, 0.05, l96_spun_up(orc))
+    P, T = 2048, 12
+    res = []
+    for sched in (False, True):
+        pf = capi.ParticleFilter(ctx, g, P, T, 0.05, 5, ess_rel=0.5, record=True)
+        pf.set_observations(y)
+        if sched:
+            pf.set_schedule(0.05 * np.arange(T + 1))
+        pf.init()
+        pf.step(T)
+        X, _, lw = pf.state()
+        res.append((pf.term(), host(X), host(lw),
+                    np.array([pf.ancestors(k) for k in range(T)])))
+        pf.close()
+    assert res[0][0] == res[1][0]
+    for a, b in zip(res[0][1:], res[1][1:]):
+        np.testing.assert_array_equal(a, b)
