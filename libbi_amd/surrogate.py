@@ -8,7 +8,7 @@ BF16 MFMA with FP32 accumulation) is trained on the GPU by bihip_mlp_train
 to predict the normalised increment (Phi_dt(x) - x) / dx_scale from
 ((x - x_shift) / x_scale, (F - F0) / F_scale).  The trained network then
 runs inside the filter step (bihip_pf_set_surrogate), and the exact
-integrator stays available as the judge of its log-likelihood error.
+integrator stays available to measure its log-likelihood error.
 
 Data preparation (normalising the pairs) uses torch elementwise operations on
 the device tensors; every training and inference computation runs in the
