@@ -20,6 +20,9 @@ Other workloads (`--workload`), each printing one JSON line of the same form:
          parameter particles (mu, sigma, c, e from seeded log-normal priors),
          inner N_j = floor(exp(U(log 256, log 16384))), one persistent launch
          per observation interval replayed as a hipGraph, autotuned workgroup;
+  l96-bi the default workload on the Lorenz-96 model compiled from the LibBi
+         model file libbi_amd/bi/Lorenz96d40.bi (initial x ~ N(0,1); the
+         hand-written model is timed beside it from the same start);
   smc2-bi the same on the predator-prey model compiled from the LibBi model
          file libbi_amd/bi/PZ.bi (theta = mu, sigma; the hand-written model
          is timed beside it with the same theta);
@@ -66,7 +69,8 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="l96",
-                    choices=["l96", "pz", "smc2", "smc2-bi", "copy", "surrogate", "guided"])
+                    choices=["l96", "l96-bi", "pz", "smc2", "smc2-bi", "copy", "surrogate",
+                             "guided"])
     ap.add_argument("--thetas", type=int, default=64)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=24000)
@@ -557,12 +561,87 @@ def main():
     gm = capi.make_model("l96", d=D, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
                          init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
     _, y = data.simulate(ctx, gm, mean0, T_SCHEDULE, DT, seed=2024)
+    P = args.particles
+    generated = args.workload == "l96-bi"
+    if generated:
+        # the same filter on the functor compiled from the model file; its
+        # initial block draws x ~ N(0, init_sd), so the hand-written model is
+        # timed beside it from the same initial distribution (mean 0)
+        bm = capi.generated_model("Lorenz96d40", precision=capi.F32, F=8.0, noise_sd=0.1,
+                                  obs_sd=1.0, init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+        hand = run_l96(args, ctx, gm, y, np.zeros(D))
+        fig = run_l96(args, ctx, bm, y, None)
+    else:
+        fig = run_l96(args, ctx, gm, y, mean0)
+    elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first = fig
+    bytes_total = sum(transition_bytes(P, bool(r)) for r in resampled)
+    achieved = bytes_total / (trans_ms * 1e-3) / 1e9 if trans_ms > 0 else 0.0
+
+    value = P * args.steps / elapsed
+    line = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (truth trajectory and observations simulated by the library)",
+        "config": {
+            "workload": ("Lorenz-96 d=40 bootstrap PF step" +
+                         (" on the model compiled from libbi_amd/bi/Lorenz96d40.bi (four lanes "
+                          "per particle), initial x ~ N(0,1)" if generated else "") +
+                         ", N=2^20 FP32 state / FP64 weights, "
+                         "DOPRI5 rtol=atol=1e-5, obs every 0.05 on even dims N(0,1), "
+                         f"{args.resampler} resampling at ESS<0.5N"),
+            "particles": P,
+            "resampled_steps": int(resampled.sum()),
+            "log_weights": "f64",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": ("k_transition<Gen_Lorenz96d40<float>,DOPRI5,ONE>" if generated
+                       else "k_transition<L96<float,40,4>,DOPRI5>"),
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": L96_TRAFFIC_BYTES if P == P_DEFAULT and not generated else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch, scaled by "
+                              "a 1 GiB copy (factors 2.0 / 1.0)",
+            "algorithmic_bytes_per_launch": bytes_total / max(len(resampled), 1),
+            "avg_launch_ms": trans_ms / max(trans_n, 1),
+            "launches": trans_n,
+            "resample_chain_ms_per_step": res_ms / max(res_n, 1),
+            "reduce_ms_per_step": red_ms / max(red_n, 1),
+            "phase_timing_sample": f"warm-up steps {first}..{args.warmup - 1}",
+        },
+    }
+    if generated:
+        line["config"]["handwritten_same_init_particle_steps_per_s"] = P * args.steps / hand[0]
+        line["config"]["handwritten_same_init_transition_ms"] = hand[1] / max(hand[2], 1)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args, y, mean0)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def run_l96(args, ctx, m, y, init_mean):
+    """The timed filter of the Lorenz-96 workloads: warm-up steps (the later
+    ones with events around every phase), then args.steps timed steps with
+    events around the transition only, since each event costs stream time."""
+    import torch
+    from libbi_amd import capi
     res = {"systematic": capi.SYSTEMATIC, "stratified": capi.STRATIFIED,
            "multinomial": capi.MULTINOMIAL}[args.resampler]
     P = args.particles
-    pf = capi.ParticleFilter(ctx, gm, P, T_SCHEDULE, DT, seed=77, resampler=res, ess_rel=0.5)
+    pf = capi.ParticleFilter(ctx, m, P, T_SCHEDULE, DT, seed=77, resampler=res, ess_rel=0.5)
     pf.set_observations(y)
-    pf.init(mean0)
+    pf.init(init_mean)
 
     # warm-up: the first steps untimed, the rest with events around every phase
     # (resampling chain and reduction figures); the timed steps carry events
@@ -585,54 +664,8 @@ def main():
     out = pf.outputs()
     k0, k1 = args.warmup, args.warmup + args.steps
     resampled = out["resampled"][k0:k1]
-    bytes_total = sum(transition_bytes(P, bool(r)) for r in resampled)
-    achieved = bytes_total / (trans_ms * 1e-3) / 1e9 if trans_ms > 0 else 0.0
-
-    value = P * args.steps / elapsed
-    line = {
-        "metric": "particle-steps/s",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (truth trajectory and observations simulated by the library)",
-        "config": {
-            "workload": "Lorenz-96 d=40 bootstrap PF step, N=2^20 FP32 state / FP64 weights, "
-                        "DOPRI5 rtol=atol=1e-5, obs every 0.05 on even dims N(0,1), "
-                        f"{args.resampler} resampling at ESS<0.5N",
-            "particles": P,
-            "resampled_steps": int(resampled.sum()),
-            "log_weights": "f64",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_transition<L96<float,40,4>,DOPRI5>",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": L96_TRAFFIC_BYTES if P == P_DEFAULT else None,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch, scaled by "
-                              "a 1 GiB copy (factors 2.0 / 1.0)",
-            "algorithmic_bytes_per_launch": bytes_total / max(len(resampled), 1),
-            "avg_launch_ms": trans_ms / max(trans_n, 1),
-            "launches": trans_n,
-            "resample_chain_ms_per_step": res_ms / max(res_n, 1),
-            "reduce_ms_per_step": red_ms / max(red_n, 1),
-            "phase_timing_sample": f"warm-up steps {first}..{args.warmup - 1}",
-        },
-    }
     pf.close()
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args, y, mean0)
-    print(json.dumps(line), flush=True)
-    ctx.close()
+    return elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first
 
 
 if __name__ == "__main__":
