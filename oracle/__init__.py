@@ -52,6 +52,13 @@ PINNED = {
     "orc_pf_run(inputs)": "inputs equal to the model's parameters reproduce the plain run bit "
                           "for bit; LG model with a per-step decay rate against the Kalman "
                           "likelihood with the same per-step transition coefficients",
+    "orc_pf_run(imposed)": "the filter's own resampling history imposed reproduces its run bit "
+                           "for bit (three resamplers); a history without resampling equals "
+                           "the essRel = 0 run; one ancestor row laid out differently changes "
+                           "the estimate from that step on and not before",
+    "orc_pf_set_state/orc_pf_step_imposed": "restarting from the run's own state after k steps "
+                                            "and imposing its resampling continues the run bit "
+                                            "for bit",
     "orc_kalman_loglik_lg_sched": "the joint Gaussian log-density of the observed y built from "
                                   "the model's covariances (scipy multivariate_normal); equals "
                                   "orc_kalman_loglik_lg on a uniform schedule",
@@ -97,6 +104,8 @@ class PFConfig(ctypes.Structure):
         ("times", ctypes.POINTER(ctypes.c_double)),
         ("observed", ctypes.POINTER(ctypes.c_int)),
         ("inputs", ctypes.POINTER(ctypes.c_double)),
+        ("imposed_resampled", ctypes.POINTER(ctypes.c_int)),
+        ("imposed_ancestors", ctypes.POINTER(ctypes.c_int)),
     ]
 
 
@@ -192,6 +201,22 @@ def _declare(L):
     L.orc_mlp_forward.argtypes = [c.c_int, _I, _D, _D, c.c_int, c.c_int, _D]
     L.orc_mlp_loss_grad.argtypes = [c.c_int, _I, _D, _D, _D, c.c_int, c.c_int, c.c_int, _D]
     L.orc_mlp_loss_grad.restype = c.c_double
+    L.orc_pf_new.argtypes = [c.POINTER(Model), c.c_int, c.c_int, c.c_double, c.c_double,
+                             c.c_double, c.c_uint64, c.c_uint32]
+    L.orc_pf_new.restype = c.c_void_p
+    L.orc_pf_free.argtypes = [c.c_void_p]
+    L.orc_pf_init.argtypes = [c.c_void_p, _D]
+    L.orc_pf_impose.argtypes = [c.c_void_p, _I, _I]
+    L.orc_pf_set_state.argtypes = [c.c_void_p, c.c_int, _D, _D, c.c_double, c.c_double]
+    L.orc_pf_get_state.argtypes = [c.c_void_p, _D, _D]
+    L.orc_pf_step.argtypes = [c.c_void_p, _D, _D, _I, _I]
+    L.orc_pf_step_imposed.argtypes = [c.c_void_p, _D, c.c_int, _I, _D]
+    L.orc_pf_loglikelihood.argtypes = [c.c_void_p]
+    L.orc_pf_loglikelihood.restype = c.c_double
+    L.orc_pf_ess.argtypes = [c.c_void_p]
+    L.orc_pf_ess.restype = c.c_double
+    L.orc_pf_term.argtypes = [c.c_void_p]
+    L.orc_pf_term.restype = c.c_double
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
     L.orc_kalman_loglik_lg_sched.argtypes = [c.POINTER(Model), _D, c.POINTER(c.c_int), c.c_int,
@@ -434,14 +459,24 @@ def gather_rows(a, X):
 # ---------------------------------------------------------------- filter
 def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=None,
            batch=0, want_ancestors=False, want_state=False, times=None, observed=None,
-           inputs=None):
+           inputs=None, imposed_resampled=None, imposed_ancestors=None):
     """times (T + 1) and observed (T flags), both optional, give a schedule in
     place of t0 + k dt with every step observed; inputs (T x 16, optional) the
-    model's parameters during each step."""
+    model's parameters during each step.  imposed_resampled (T flags) with
+    imposed_ancestors (T x P) re-run another filter's resampling history:
+    step k resamples iff the flag is set, with those ancestors."""
     y = _f64(y).reshape(len(y), -1)
     T = y.shape[0]
     cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt, seed=seed,
                    batch=batch)
+    if (imposed_resampled is None) != (imposed_ancestors is None):
+        raise ValueError("imposed_resampled and imposed_ancestors go together")
+    if imposed_resampled is not None:
+        imposed_resampled = _i32(imposed_resampled)
+        imposed_ancestors = _i32(imposed_ancestors)
+        assert imposed_resampled.shape == (T,) and imposed_ancestors.shape == (T, P)
+        cfg.imposed_resampled = _ip(imposed_resampled)
+        cfg.imposed_ancestors = _ip(imposed_ancestors)
     if times is not None:
         times = _f64(times)
         assert times.shape == (T + 1,)
@@ -465,6 +500,66 @@ def pf_run(m, y, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, mean0=N
                           ctypes.byref(logZ), _dp(li), _dp(ess), _ip(res), _ip(anc), _dp(Xo))
     return dict(rc=rc, logZ=logZ.value, logIncrements=li, ess=ess, resampled=res,
                 ancestors=anc, X=Xo)
+
+
+class StepFilter:
+    """The step-wise oracle filter (orc_pf_*): one BootstrapPF stepped from
+    Python, whose state can be replaced between steps (set_state) and whose
+    resampling can follow another filter's (step with ancestors)."""
+
+    def __init__(self, m, P, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, batch=0):
+        self.m, self.P = m, P
+        self.h = lib().orc_pf_new(ctypes.byref(m), P, resampler, ess_rel, t0, dt, seed, batch)
+
+    def close(self):
+        if self.h:
+            lib().orc_pf_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, mean0=None):
+        mean = _f64(mean0) if mean0 is not None else np.zeros(self.m.d)
+        lib().orc_pf_init(self.h, _dp(mean))
+
+    def set_state(self, k, X, lws, log_likelihood, ess):
+        """X: d x P (any float type, converted to float64)."""
+        X = _f64(X)
+        lws = _f64(lws)
+        assert X.shape == (self.m.d, self.P) and lws.shape == (self.P,)
+        lib().orc_pf_set_state(self.h, k, _dp(X), _dp(lws), log_likelihood, ess)
+
+    def state(self):
+        X = np.empty((self.m.d, self.P))
+        lws = np.empty(self.P)
+        lib().orc_pf_get_state(self.h, _dp(X), _dp(lws))
+        return X, lws
+
+    def step(self, y, resampled=None, ancestors=None):
+        """One step with observation y; with resampled (0/1) the resampling at
+        the step's start is imposed, with the P ancestors when set.  Returns
+        (rc, logIncrement)."""
+        y = _f64(y).ravel()
+        li = ctypes.c_double()
+        if resampled is None:
+            rc = lib().orc_pf_step(self.h, _dp(y), ctypes.byref(li), None, None)
+        else:
+            a = _i32(ancestors) if resampled else None
+            if resampled:
+                assert a.shape == (self.P,)
+            rc = lib().orc_pf_step_imposed(self.h, _dp(y), 1 if resampled else 0, _ip(a),
+                                           ctypes.byref(li))
+        return rc, li.value
+
+    def log_likelihood(self):
+        return lib().orc_pf_loglikelihood(self.h)
+
+    def ess(self):
+        return lib().orc_pf_ess(self.h)
+
+    def term(self):
+        return lib().orc_pf_term(self.h)
 
 
 def uniform_at(seed, index, step, tag, batch=0):

@@ -33,6 +33,8 @@ struct orc_pf {
   const double* times;  /* nullable: t0 + k dt */
   const int* observed;  /* nullable: every step observed */
   const double* inputs; /* nullable: T x 16 parameters per step */
+  const int* imp_r;     /* nullable: imposed resampling flags per step */
+  const int* imp_a;     /* nullable: imposed ancestors, T x P */
 };
 
 orc_pf* orc_pf_new(const orc_model* m, int P, int resampler, double essRel, double t0,
@@ -80,6 +82,25 @@ void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed) {
 
 void orc_pf_set_inputs(orc_pf* f, const double* inputs) { f->inputs = inputs; }
 
+void orc_pf_impose(orc_pf* f, const int* resampled, const int* ancestors) {
+  f->imp_r = resampled;
+  f->imp_a = ancestors;
+}
+
+void orc_pf_set_state(orc_pf* f, int k, const double* X, const double* lws,
+                      double logLikelihood, double ess) {
+  memcpy(f->X, X, sizeof(double) * (size_t)f->m.d * f->P);
+  memcpy(f->lws, lws, sizeof(double) * (size_t)f->P);
+  f->logLikelihood = logLikelihood;
+  f->ess = ess;
+  f->k = k;
+}
+
+void orc_pf_get_state(const orc_pf* f, double* X, double* lws) {
+  if (X) memcpy(X, f->X, sizeof(double) * (size_t)f->m.d * f->P);
+  if (lws) memcpy(lws, f->lws, sizeof(double) * (size_t)f->P);
+}
+
 static double pf_time(const orc_pf* f, int k) {
   return f->times ? f->times[k] : f->t0 + k * f->dt;
 }
@@ -87,17 +108,27 @@ static double pf_time(const orc_pf* f, int k) {
 static int pf_observed(const orc_pf* f, int k) { return f->observed ? f->observed[k] != 0 : 1; }
 
 /* One BootstrapPF::step from t_k to t_{k+1} with observation y_{k+1}.
- * resampled/ancestors (nullable) report the resampling at t_k.  Returns
- * ORC_OK or ORC_DEGENERATE; *logIncrement gets lW - logLikelihood. */
-int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled,
-                int* ancestors) {
+ * imp_r < 0: the filter decides its resampling itself; otherwise imp_r (0/1)
+ * imposes it, with the P ancestors imp_a when set.  resampled/ancestors
+ * (nullable) report the resampling at t_k.  Returns ORC_OK or
+ * ORC_DEGENERATE; *logIncrement gets lW - logLikelihood. */
+static int pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled,
+                   int* ancestors, int imp_r, const int* imp_a) {
   const int P = f->P, D = f->m.d, k = f->k;
   const double t1 = pf_time(f, k), t2 = pf_time(f, k + 1);
   /* Resampler::resample (share/src/bi/resampler/Resampler.hpp:165-182): only at
    * an observed element, when the ESS fell below the threshold */
-  const int r = k >= 1 && pf_observed(f, k - 1) && f->ess < f->essRel * P;
+  const int imposed = imp_r >= 0;
+  const int r = imposed ? imp_r != 0
+                        : k >= 1 && pf_observed(f, k - 1) && f->ess < f->essRel * P;
   int rc = ORC_OK;
-  if (r) {
+  if (r && imposed) {
+    /* another filter's resampling at t_k: its ancestors, then the same
+     * gather and reset of the log-weights */
+    memcpy(f->as, imp_a, sizeof(int) * (size_t)P);
+    orc_gather_rows(f->as, P, D, f->X, P);
+    for (int i = 0; i < P; ++i) f->lws[i] = f->logLikelihood;
+  } else if (r) {
     orc_sumexpu_inclusive_scan(f->lws, P, f->Ws);
     if (f->resampler == ORC_SYSTEMATIC) {
       double a;
@@ -145,6 +176,19 @@ int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled
   return ORC_OK;
 }
 
+int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled,
+                int* ancestors) {
+  if (f->imp_r != NULL && f->imp_a != NULL)
+    return pf_step(f, y, logIncrement, resampled, ancestors, f->imp_r[f->k] != 0,
+                   f->imp_a + (size_t)f->k * f->P);
+  return pf_step(f, y, logIncrement, resampled, ancestors, -1, NULL);
+}
+
+int orc_pf_step_imposed(orc_pf* f, const double* y, int resampled, const int* ancestors,
+                        double* logIncrement) {
+  return pf_step(f, y, logIncrement, NULL, NULL, resampled != 0, ancestors);
+}
+
 double orc_pf_loglikelihood(const orc_pf* f) { return f->logLikelihood; }
 double orc_pf_ess(const orc_pf* f) { return f->ess; }
 int orc_pf_size(const orc_pf* f) { return f->P; }
@@ -186,6 +230,7 @@ int orc_pf_run(const orc_model* m, const orc_pf_cfg* cfg, const double* y,
   orc_pf_init(f, mean0);
   orc_pf_set_schedule(f, cfg->times, cfg->observed);
   orc_pf_set_inputs(f, cfg->inputs);
+  orc_pf_impose(f, cfg->imposed_resampled, cfg->imposed_ancestors);
   for (int k = 0; k < T; ++k) {
     double li;
     int r;

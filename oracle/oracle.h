@@ -131,6 +131,14 @@ typedef struct {
   /* optional inputs (nullable): the model's 16 parameters for each step,
    * T x 16, in force over [t_k, t_{k+1}) */
   const double* inputs;
+  /* optional imposed resampling (nullable; both or neither): at step k the
+   * filter resamples iff imposed_resampled[k] != 0, and then takes its
+   * ancestors from imposed_ancestors[k P .. k P + P - 1] in place of the ones
+   * its own weights and uniforms would give (the gather and lws[:] =
+   * logLikelihood follow as usual).  Re-runs another filter's resampling
+   * history in FP64, e.g. that of an FP32 run. */
+  const int* imposed_resampled;
+  const int* imposed_ancestors;
 } orc_pf_cfg;
 
 /*
@@ -157,6 +165,19 @@ void orc_pf_set_schedule(orc_pf* f, const double* times, const int* observed);
  * step): inputs[16 k .. 16 k + 15] replace the model's parameters during step
  * k (transition and observation density).  Nullable, kept by reference. */
 void orc_pf_set_inputs(orc_pf* f, const double* inputs);
+/* Imposed resampling history (see orc_pf_cfg), both nullable, kept by reference. */
+void orc_pf_impose(orc_pf* f, const int* resampled, const int* ancestors);
+/* One step whose resampling at its start is imposed: resampled (0/1) and,
+ * when set, the P ancestors.  Otherwise as orc_pf_step. */
+int orc_pf_step_imposed(orc_pf* f, const double* y, int resampled, const int* ancestors,
+    double* logIncrement);
+/* Replace the filter's state after k steps: X (d x P column-major, ld = P),
+ * log-weights, running log-likelihood and the ESS of the last correction, so
+ * that the next orc_pf_step continues from another filter's state. */
+void orc_pf_set_state(orc_pf* f, int k, const double* X, const double* lws,
+    double logLikelihood, double ess);
+/* The current state (d x P column-major, ld = P) and log-weights (nullable). */
+void orc_pf_get_state(const orc_pf* f, double* X, double* lws);
 int orc_pf_step(orc_pf* f, const double* y, double* logIncrement, int* resampled, int* ancestors);
 double orc_pf_loglikelihood(const orc_pf* f);
 double orc_pf_ess(const orc_pf* f);

@@ -189,3 +189,71 @@ def test_time_varying_decay_matches_kalman(orc):
     zs = [orc.pf_run(m, y, 1 << 14, 1.0, seed, ess_rel=0.5, inputs=inputs)["logZ"]
           for seed in (1, 2, 3)]
     assert abs(np.mean(zs) - exact) < 0.1, (zs, exact)
+
+
+# ---------------------------------------------------------------- imposed resampling
+@pytest.mark.parametrize("resampler", [0, 1, 2])
+def test_imposed_own_history_reproduces_the_run(orc, resampler):
+    """The filter re-run on its own resampling history (flags and ancestors)
+    reproduces its log-likelihood, increments and ESS bit for bit."""
+    om = orc.make_model("l96", d=8)
+    mean0 = np.full(8, 8.0)
+    _, y = orc.simulate(om, 3, 15, 0.05, mean0)
+    P = 512
+    r = orc.pf_run(om, y, P, 0.05, 9, resampler=resampler, ess_rel=0.7, mean0=mean0,
+                   want_ancestors=True)
+    assert 3 <= r["resampled"].sum() < 15
+    s = orc.pf_run(om, y, P, 0.05, 9, resampler=resampler, ess_rel=0.7, mean0=mean0,
+                   imposed_resampled=r["resampled"], imposed_ancestors=r["ancestors"])
+    assert s["logZ"] == r["logZ"]
+    np.testing.assert_array_equal(s["logIncrements"], r["logIncrements"])
+    np.testing.assert_array_equal(s["ess"], r["ess"])
+    np.testing.assert_array_equal(s["resampled"], r["resampled"])
+
+
+def test_imposed_history_is_followed(orc):
+    """Imposed flags and ancestors replace the filter's own: no resampling at
+    all equals the essRel = 0 run, and another valid ancestor vector at one
+    step changes the estimate (the imposed rows are read)."""
+    om = orc.make_model("l96", d=8)
+    mean0 = np.full(8, 8.0)
+    _, y = orc.simulate(om, 3, 12, 0.05, mean0)
+    P, T = 256, 12
+    none = orc.pf_run(om, y, P, 0.05, 4, ess_rel=0.7, mean0=mean0,
+                      imposed_resampled=np.zeros(T, np.int32),
+                      imposed_ancestors=np.zeros((T, P), np.int32))
+    plain = orc.pf_run(om, y, P, 0.05, 4, ess_rel=0.0, mean0=mean0)
+    assert none["logZ"] == plain["logZ"]
+    r = orc.pf_run(om, y, P, 0.05, 4, ess_rel=0.7, mean0=mean0, want_ancestors=True)
+    k = int(np.nonzero(r["resampled"])[0][0])
+    anc = r["ancestors"].copy()
+    anc[k] = np.roll(anc[k], 1)  # the same multiset, another layout
+    moved = orc.pf_run(om, y, P, 0.05, 4, ess_rel=0.7, mean0=mean0,
+                       imposed_resampled=r["resampled"], imposed_ancestors=anc)
+    # the layout changes which noise each copy draws, hence the estimate
+    assert moved["logZ"] != r["logZ"]
+    np.testing.assert_array_equal(moved["logIncrements"][:k], r["logIncrements"][:k])
+
+
+def test_step_filter_restarts_from_a_given_state(orc):
+    """StepFilter.set_state with a run's own state after k steps, then steps
+    with the run's resampling imposed, continues the run bit for bit."""
+    om = orc.make_model("l96", d=8)
+    mean0 = np.full(8, 8.0)
+    _, y = orc.simulate(om, 3, 10, 0.05, mean0)
+    P, k0 = 300, 4
+    full = orc.pf_run(om, y, P, 0.05, 6, ess_rel=0.8, mean0=mean0, want_ancestors=True)
+    head = orc.StepFilter(om, P, 0.05, 6, ess_rel=0.8)
+    head.init(mean0)
+    for k in range(k0):
+        head.step(y[k])
+    X, lws = head.state()
+    ll, ess = head.log_likelihood(), head.ess()
+    head.close()
+    f = orc.StepFilter(om, P, 0.05, 6, ess_rel=0.8)
+    f.set_state(k0, X, lws, ll, ess)
+    for k in range(k0, 10):
+        rc, li = f.step(y[k], full["resampled"][k], full["ancestors"][k])
+        assert rc == orc.OK and li == full["logIncrements"][k]
+    assert f.term() == full["logZ"]
+    f.close()
