@@ -128,8 +128,12 @@ class State<B, ON_DEVICE> {
   typedef device_vector<int> temp_int_vector_type;
 
   State(const B& m, int P)
-      : P(P), ld((P + 63) / 64 * 64), d(m.getDynSize()), X(ld * d), Y(ld * d), clock(0) {}
+      : P(P), ld((P + 63) / 64 * 64), d(m.getDynSize()), X(ld * d), Y(ld * d), clock(0),
+        desc(m.description()) {}
   int size() const { return P; }
+  /** The device description of the model the state belongs to (the
+   * reference resolves it from the type B at compile time). */
+  const bihip_model& model() const { return desc; }
   int lead() const { return ld; }
   real_type* buf() { return X.buf(); }
   /** Swap in the result of an out-of-place update. */
@@ -147,6 +151,9 @@ class State<B, ON_DEVICE> {
 
  public:
   long clock;
+
+ private:
+  bihip_model desc;
 };
 
 /**
@@ -389,29 +396,72 @@ class Resampler : public R {
 };
 
 // ================================================================ integrators
-/** share/src/bi/ode/RK4Integrator.hpp:21-75: the ode block alone, no noise. */
+namespace amd {
+/** The ode block's settings (share/src/bi/ode/IntegratorConstants.hpp:29-44),
+ * per host thread: the generated ode block sets them before it integrates
+ * (share/tt/cpp/block/ode.hpp.tt:40-43). */
+struct OdeSettings {
+  bool set = false;
+  double h0 = 0.0, atoler = 0.0, rtoler = 0.0;
+};
+inline OdeSettings& ode_settings() {
+  thread_local OdeSettings o;
+  return o;
+}
+
+/** The deterministic flow of the state s over [t1, t2] with integrator alg:
+ * the state's model description, with the ode settings of this thread when
+ * bi_ode_set has been called. */
+template <class B>
+inline void flow(int alg, double t1, double t2, State<B, ON_DEVICE>& s) {
+  if (!(std::fabs(t2 - t1) > 0.0)) return;  // as the reference's update
+  if (t1 > t2) throw DeviceError(BIHIP_EINVAL, "bihip: integrator update needs t1 <= t2");
+  bihip_model d = s.model();
+  d.integrator = alg;
+  const OdeSettings& o = ode_settings();
+  if (o.set) {
+    d.h0 = o.h0;
+    d.atol = o.atoler;
+    d.rtol = o.rtoler;
+  }
+  check(bihip_flow(ctx(), &d, t1, t2, s.size(), s.buf(), s.lead(), nullptr, nullptr));
+}
+}  // namespace amd
+
+/** share/src/bi/ode/RK4Integrator.hpp:21-75: the ode block alone, no noise,
+ * called as the generated ode block calls it (share/tt/cpp/block/ode.hpp.tt:47-48). */
 template <class B, class S = void>
 class RK4Integrator {
  public:
   template <class T1>
-  static void update(const B& m, const T1 t1, const T1 t2, State<B, ON_DEVICE>& s) {
-    bihip_model d = m.description();
-    d.integrator = BIHIP_RK4;
-    amd::check(bihip_flow(amd::ctx(), &d, t1, t2, s.size(), s.buf(), s.lead(), nullptr, nullptr));
+  static void update(const T1 t1, const T1 t2, State<B, ON_DEVICE>& s) {
+    amd::flow(BIHIP_RK4, (double)t1, (double)t2, s);
   }
 };
 
-/** share/src/bi/ode/DOPRI5Integrator.hpp:22-76 */
+/** share/src/bi/ode/DOPRI5Integrator.hpp:22-76 (share/tt/cpp/block/ode.hpp.tt:49-50). */
 template <class B, class S = void>
 class DOPRI5Integrator {
  public:
   template <class T1>
-  static void update(const B& m, const T1 t1, const T1 t2, State<B, ON_DEVICE>& s) {
-    bihip_model d = m.description();
-    d.integrator = BIHIP_DOPRI5;
-    amd::check(bihip_flow(amd::ctx(), &d, t1, t2, s.size(), s.buf(), s.lead(), nullptr, nullptr));
+  static void update(const T1 t1, const T1 t2, State<B, ON_DEVICE>& s) {
+    amd::flow(BIHIP_DOPRI5, (double)t1, (double)t2, s);
   }
 };
+
+}  // namespace bi
+
+/** share/src/bi/ode/IntegratorConstants.hpp:29-44: initial step size and
+ * tolerances of the next integrator updates on this host thread. */
+inline void bi_ode_set(const double h0, const double atoler, const double rtoler) {
+  bi::amd::OdeSettings& o = bi::amd::ode_settings();
+  o.set = true;
+  o.h0 = h0;
+  o.atoler = atoler;
+  o.rtoler = rtoler;
+}
+
+namespace bi {
 
 // ================================================================ observer, forcer, output
 /** Observations of T times, nobs values each, on the device (Observer.hpp:115-126). */

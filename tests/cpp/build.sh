@@ -1,5 +1,5 @@
 #!/bin/sh
-# Builds the C++ shim client (tests/cpp/_build/shim_filter) with the host
+# Builds the C++ shim clients (tests/cpp/_build/shim_filter, shim_ode) with the host
 # compiler only: the shims reach the GPU through include/bihip.h and
 # libbi_amd/lib/libbihip.so.
 set -e
@@ -10,3 +10,7 @@ ${CXX:-g++} -std=c++17 -O2 -Wall -Wextra -I "$ROOT/include" "$HERE/shim_filter.c
   -L "$ROOT/libbi_amd/lib" -lbihip -Wl,-rpath,'$ORIGIN/../../../libbi_amd/lib' \
   -o "$HERE/_build/shim_filter.tmp"
 mv "$HERE/_build/shim_filter.tmp" "$HERE/_build/shim_filter"
+${CXX:-g++} -std=c++17 -O2 -Wall -Wextra -I "$ROOT/include" "$HERE/shim_ode.cpp" \
+  -L "$ROOT/libbi_amd/lib" -lbihip -Wl,-rpath,'$ORIGIN/../../../libbi_amd/lib' \
+  -o "$HERE/_build/shim_ode.tmp"
+mv "$HERE/_build/shim_ode.tmp" "$HERE/_build/shim_ode"

@@ -98,3 +98,60 @@ def test_shim_netcdf_outputs_agree(tmp_path, orc, torch_gpu):
     for name in ("P", "Z", "logweight"):
         np.testing.assert_allclose(a[name], b[name], rtol=1e-12, atol=1e-12)
     assert float(a["loglikelihood"]) == pytest.approx(float(b["loglikelihood"]), rel=1e-12)
+
+
+ODE_CLIENT = os.path.join(HERE, "cpp", "_build", "shim_ode")
+
+
+def test_ode_client_built_and_prints_usage():
+    """The generated ode block's call form, bi_ode_set(h, atoler, rtoler) then
+    bi::{RK4,DOPRI5}Integrator<Model, action_typelist>::update(t1, t2, s)
+    (share/tt/cpp/block/ode.hpp.tt:40-52), compiles against the shims."""
+    assert os.path.exists(ODE_CLIENT), "run __graft_entry__.build() first"
+    r = subprocess.run([ODE_CLIENT], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg,model,t2,h,tol", [("dopri5", "l96", 0.05, 0.05, 1e-5),
+                                                ("rk4", "pz", 1.0, 0.1, 1e-5)])
+def test_shim_integrator_update_matches_flow(tmp_path, orc, ctx, alg, model, t2, h, tol):
+    """The static update(t1, t2, s) of the shims runs the ode block with the
+    constants bi_ode_set gave: it equals bihip_flow of the same model and the
+    oracle's flow (FP64, 1e-10), and the constants reach the integrator (a
+    DOPRI5 update at atol = rtol = tol differs from one at 1e-3)."""
+    import ctypes
+    from libbi_amd import capi
+    from gpu_helpers import dev, host
+    P = 777
+    rng = np.random.default_rng(3)
+    if model == "l96":
+        om = orc.make_model("l96", d=40, noise_sd=0.0, atol=tol, rtol=tol, h0=h)
+        X0 = l96_spun_up(orc)[:, None] + rng.normal(size=(40, P))
+        gm = capi.make_model("l96", d=40, precision=capi.F64, noise_sd=0.0, atol=tol, rtol=tol,
+                             h0=h)
+    else:
+        om = orc.make_model("pz", h0=h, sigma=0.0)
+        X0 = np.exp(rng.normal(np.log(2.0), 0.2, size=(2, P)))
+        gm = capi.make_model("pz", precision=capi.F64, h0=h)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    X0.astype(np.float64).tofile(inp)
+
+    def client(atol, rtol):
+        r = subprocess.run([ODE_CLIENT, alg, model, str(P), "0.0", repr(t2), repr(h), repr(atol),
+                            repr(rtol), str(inp), str(out)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        return np.fromfile(out, np.float64).reshape(X0.shape)
+
+    got = client(tol, tol)
+    X = dev(X0)
+    capi.check(capi.lib().bihip_flow(ctx.h, ctypes.byref(gm), 0.0, t2, P,
+                                     ctypes.c_void_p(X.data_ptr()), P, None, None))
+    np.testing.assert_array_equal(got, host(X))
+    ref = orc.transition(om, 1, 1, 0.0, t2, X0)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10)
+    if alg == "dopri5":
+        loose = client(1e-3, 1e-3)
+        assert np.abs(loose - got).max() > 1e-6
+
