@@ -46,6 +46,7 @@ so a model written here reproduces the built-in model with the same
 equations bit for bit.
 """
 import math
+import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -876,12 +877,15 @@ def _logdensity(d: Draw, a, lsg=None):
     raise GenError(f"no log-density for {d.dist}")
 
 
-def _hoist(args_by_elem, key, w, ind):
+def _hoist(args_by_elem, key, w, ind, local=None):
     """Arguments that are the same for every element are computed once: returns
     the argument strings to use and the name of log(sd) when the second
-    argument of a gaussian is invariant (None otherwise)."""
+    argument of a gaussian is invariant (None otherwise).  An argument that
+    names the variable `local` (declared per element) is never hoisted."""
     first = args_by_elem[0]
-    same = [all(a[i] == first[i] for a in args_by_elem) for i in range(len(first))]
+    same = [all(a[i] == first[i] for a in args_by_elem) and
+            (local is None or not re.search(rf"\b{local}\b", first[i]))
+            for i in range(len(first))]
     if len(args_by_elem) < 2:
         return first, [False] * len(first)
     names = []
@@ -1171,7 +1175,7 @@ def _generate_lanes(g: GenModel) -> str:
         a, b = idx.pop() if idx else (0, 0)
         bind = {st.alias: 0} if st.alias else {}
         per = [[eo.expr(x_, bind, l, st.line) for x_ in d.args] for l in range(DL)]
-        names, same = _hoist(per, k_, w, "    ")
+        names, same = _hoist(per, k_, w, "    ", local="j_")
         lsg = None
         if d.dist == "gaussian" and not d.log and same[1]:
             lsg = f"l{k_}"
@@ -1181,14 +1185,18 @@ def _generate_lanes(g: GenModel) -> str:
             w("    {")
             w(f"      const int s_ = lo + {l};")
             if a == 0:
-                w(f"      const int j_ = 0;")
+                # every element j reads the same state element b (or none):
+                # the lane that holds b adds all of them
                 w(f"      if (s_ == {b}) {{")
+                w(f"       for (int j_ = 0; j_ < {v.size}; ++j_) {{")
             else:
                 w(f"      const int j_ = (s_ - {b}) / {a};")
                 w(f"      if ((s_ - {b}) % {a} == 0 && j_ >= 0 && j_ < {v.size}) {{")
             w(f"        const double xy = y[{v.offset} + j_];")
             for ln in _logdensity(d, args, lsg):
                 w("        " + ln)
+            if a == 0:
+                w("       }")
             w("      }")
             w("    }")
     w("    return lp;")

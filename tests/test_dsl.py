@@ -482,3 +482,73 @@ def test_rejection_samplers_against_scipy(tmp_path):
         else:
             p = stats.kstest(x, d.cdf).pvalue
         assert p > 1e-4, (name, p)
+
+
+LANE_OBS_MODEL = """
+model LaneObs {
+  const N = 40
+  dim n(size = N, boundary = 'cyclic')
+  dim m(size = N/2)
+  param F, obs_sd
+  state x[n]
+  obs y[m]
+  obs z[m]
+  obs c[m]
+  sub transition(delta = 0.05) {
+    ode(alg = 'RK5(4)', h = 0.05, atoler = 1.0e-5, rtoler = 1.0e-5) {
+      dx[i]/dt = (x[i+1] - x[i-2])*x[i-1] - x[i] + F
+    }
+  }
+  sub observation {
+    y[j] ~ gaussian(x[2*j], obs_sd)
+    z[j] ~ gaussian(x[3], obs_sd + 0.01*j)
+    c[j] ~ gaussian(F, obs_sd)
+  }
+}
+"""
+
+
+def test_lane_group_observations_equal_one_lane(tmp_path):
+    """A vector observation whose state index does not move with its element
+    (z[j] ~ gaussian(x[3], ...)), or that reads no state (c[j] ~ gaussian(F,
+    ...)), adds every element's term in the lane group layout: the lanes'
+    obs() summed over a group equal the one-lane functor's obs()."""
+    import copy
+    g = codegen.analyse_text(LANE_OBS_MODEL)
+    assert g.lpp == 4
+    g1 = copy.deepcopy(g)
+    g1.lpp = 1
+    g1.name = g.name + "One"
+    src = codegen.generate_model(g) + "\n" + codegen.generate_model(g1)
+    rng = np.random.default_rng(5)
+    x = rng.normal(8.0, 2.0, size=40)
+    y = rng.normal(size=60)
+    xs = ", ".join(repr(float(v)) for v in x)
+    ys = ", ".join(repr(float(v)) for v in y)
+    body = f"""
+int main() {{
+  bihip::ModelArgs ma{{}}; ma.par[0] = 8.0; ma.par[1] = 0.7;
+  const double x[40] = {{{xs}}};
+  const double y[60] = {{{ys}}};
+  bihip::{g.cname}<double> m(ma);
+  bihip::{g1.cname}<double> m1(ma);
+  double lanes = 0.0;
+  for (int l = 0; l < 4; ++l) {{
+    double xl[10];
+    for (int i = 0; i < 10; ++i) xl[i] = x[10 * l + i];
+    lanes += m.obs(xl, y, 10 * l);
+  }}
+  double one[40];
+  for (int i = 0; i < 40; ++i) one[i] = x[i];
+  std::printf("%.17g %.17g\\n", lanes, m1.obs(one, y, 0));
+  return 0;
+}}
+"""
+    out = _gxx(tmp_path, "namespace bihip {\n" + src + "\n}\n" + body, run=True)
+    lanes, one = (float(v) for v in out.split())
+    from scipy import stats
+    ref = (stats.norm.logpdf(y[:20], x[0:40:2], 0.7).sum() +
+           stats.norm.logpdf(y[20:40], x[3], 0.7 + 0.01 * np.arange(20)).sum() +
+           stats.norm.logpdf(y[40:60], 8.0, 0.7).sum())
+    assert one == pytest.approx(ref, rel=1e-12)
+    assert lanes == pytest.approx(ref, rel=1e-12)
