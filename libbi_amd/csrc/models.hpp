@@ -243,13 +243,20 @@ __device__ __forceinline__ int integrate_rk4(const M& m, const G& grp, T (&x0)[M
 // Scaled error of one component: err h / (atol + rtol max(|a|, |b|)).
 // double: as written.  float: err / den with v_rcp_f32 (1 ulp; the norm only
 // steers h), the factor h applied once to the whole norm by the caller, and
-// the larger magnitude by a compare and select (fmax would first quiet
-// signalling NaNs of both operands; a NaN state already makes err NaN).
+// the larger magnitude by one v_max_f32 with absolute-value source modifiers
+// (a NaN state already makes err, hence the norm, NaN).
+__device__ __forceinline__ float max_abs(float a, float b) {
+  // one instruction with the operands' absolute values (the compiler's fmaxf
+  // first canonicalizes operands it cannot prove quiet, two more instructions)
+  float r;
+  asm("v_max_f32_e64 %0, |%1|, |%2|" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 template <class T>
 __device__ __forceinline__ T err_scale(T err, T h, T atol, T rtol, T a, T b) {
   if constexpr (sizeof(T) == 4) {
-    const T fa = fabs(a), fb = fabs(b);
-    const T den = atol + rtol * (fa < fb ? fb : fa);
+    const T den = atol + rtol * max_abs(a, b);
     return err * __builtin_amdgcn_rcpf(den);
   } else {
     const T den = atol + rtol * fmax(fabs(a), fabs(b));
@@ -271,21 +278,26 @@ __device__ __forceinline__ void stage_rhs(const M& m, const G& grp, const T (&x)
   else m.rhs(grp, x, k);
 }
 
+// The controller's logarithms: natural in double; base 2 in float, where
+// v_log_f32 and v_exp_f32 compute log2 and exp2 natively (every logarithm of
+// the controller is then in base 2: the same factors, with the constants
+// log2 0.9 and log2 1e-4).
 __device__ __forceinline__ double ctl_log(double x) { return log(x); }
 __device__ __forceinline__ double ctl_exp(double x) { return exp(x); }
-__device__ __forceinline__ float ctl_log(float x) { return __logf(x); }
-__device__ __forceinline__ float ctl_exp(float x) { return __expf(x); }
+__device__ __forceinline__ float ctl_log(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float ctl_exp(float x) { return __builtin_amdgcn_exp2f(x); }
 
-template <class M, class T, class G>
-__device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0)[M::DL],
-                                                double t1, double t2, const ModelArgs& a) {
+template <bool QUIRK, class M, class T, class G>
+__device__ __forceinline__ int dopri5_loop(const M& m, const G& grp, T (&x0)[M::DL],
+                                           double t1, double t2, const ModelArgs& a) {
   constexpr int DL = M::DL;
+  constexpr bool F32 = sizeof(T) == 4;
   const T facl = T(0.2), facr = T(10.0), beta = T(0.04);
-  const T logsafe = T(-0.10536051565782630123);  // log(0.9)
+  // log(0.9), in base 2 for float (see ctl_log)
+  const T logsafe = F32 ? T(-0.15200309344504995) : T(-0.10536051565782630123);
   const T expo = T(0.5 * (0.2 - 0.04 * 0.75));
   const T atol = (T)a.atol, rtol = (T)a.rtol;
   const T invD = T(1.0 / (double)M::D);
-  const bool quirk = a.host_quirk != 0;
   // FP32 states of a model with constant forcing F: the stage derivatives
   // leave F out (k' = k - F) and each stage adds c_s F once, where c_s, the
   // sum of the stage's coefficients, is its node (1/5, 3/10, 4/5, 8/9, 1;
@@ -299,7 +311,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
 
   T k1[DL];
   double t = t1, h = a.h0;
-  T logfacold = T(-9.2103403719761836);  // log(1e-4)
+  T logfacold = F32 ? T(-13.287712379549449) : T(-9.2103403719761836);  // log(1e-4)
   bool k1in = false;
   int n = 0;
   while (t < t2 && n < a.nsteps) {
@@ -383,18 +395,13 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
       er[i] += T(22.0 / 525.0) * k[i];
       y6[i] = hT * y6[i] + x0[i];
     }
-    // stage 7 (FSAL): k7 at x6, or at x5 as the reference host does
-    T k7[DL];
-    if (quirk) {
-#pragma unroll
-      for (int i = 0; i < DL; ++i) k7[i] = k[i];
-    } else {
-      stage_rhs<FOLD>(m, grp, y6, k7);
-    }
+    // stage 7 (FSAL): k7 at x6 into k; the reference host evaluates it at
+    // x5, which is k6 = f(x5) already in k
+    if constexpr (!QUIRK) stage_rhs<FOLD>(m, grp, y6, k);
     T e2l = T(0);
 #pragma unroll
     for (int i = 0; i < DL; ++i) {
-      er[i] += T(-1.0 / 40.0) * k7[i];
+      er[i] += T(-1.0 / 40.0) * k[i];
       const T e = err_scale(er[i], hT, atol, rtol, x0[i], y6[i]);
       e2l += e * e;
     }
@@ -405,7 +412,7 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
 #pragma unroll
       for (int i = 0; i < DL; ++i) {
         x0[i] = y6[i];
-        k1[i] = k7[i];
+        k1[i] = k[i];
       }
     }
     if (t < t2) {
@@ -424,12 +431,52 @@ __device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0
   return n;
 }
 
+// The reference host's k7 convention is a launch-wide setting: the loop is
+// compiled for each, so neither carries the other's register moves.
+template <class M, class T, class G>
+__device__ __forceinline__ int integrate_dopri5(const M& m, const G& grp, T (&x0)[M::DL],
+                                                double t1, double t2, const ModelArgs& a) {
+  if (a.host_quirk != 0) return dopri5_loop<true>(m, grp, x0, t1, t2, a);
+  return dopri5_loop<false>(m, grp, x0, t1, t2, a);
+}
+
 // Normals for the lane's slice [lo, lo+DL) of a D-vector drawn with the
 // layout "normal j = word pair (j%4)/2 of Philox block j/4".
 template <class T, int DL>
 __device__ __forceinline__ void slice_normals(uint64_t seed, uint32_t index, uint32_t step,
                                               uint32_t tag, uint32_t batch, int lo,
                                               T (&z)[DL]) {
+  if constexpr (DL % 2 == 0) {
+    // Slices of an even length start at even elements (lo = g DL), so the
+    // slice is DL/2 whole Box-Muller pairs: global pairs q0 .. q0 + DL/2 - 1,
+    // q0 = lo/2, pair q being word pair q%2 of block q/2.  Each pair is
+    // transformed once, from words chosen by the parity of q0.
+    constexpr int NP = DL / 2;
+    constexpr int NB = NP / 2 + 1;  // blocks an odd q0 touches
+    const int q0 = lo >> 1, b0 = q0 >> 1;
+    const bool odd = (q0 & 1) != 0;
+    uint32_t w[4 * NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      // an even q0 with an even number of pairs does not reach the last block
+      if (NP % 2 == 1 || b < NB - 1 || odd) {
+        const u32x4 r = philox_block(seed, index, step, tag, batch, (uint32_t)(b0 + b));
+        w[4 * b] = r.x;
+        w[4 * b + 1] = r.y;
+        w[4 * b + 2] = r.z;
+        w[4 * b + 3] = r.w;
+      } else {
+        w[4 * b] = w[4 * b + 1] = w[4 * b + 2] = w[4 * b + 3] = 0u;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const uint32_t r0 = odd ? w[2 * i + 2] : w[2 * i];
+      const uint32_t r1 = odd ? w[2 * i + 3] : w[2 * i + 1];
+      Normal<T>::pair(r0, r1, z[2 * i], z[2 * i + 1]);
+    }
+    return;
+  }
   constexpr int NB = (DL + 3) / 4 + 1;
   T Z[4 * NB];
   const int b0 = lo >> 2, s = lo & 3;
