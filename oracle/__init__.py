@@ -217,6 +217,9 @@ def _declare(L):
     L.orc_pf_ess.restype = c.c_double
     L.orc_pf_term.argtypes = [c.c_void_p]
     L.orc_pf_term.restype = c.c_double
+    L.orc_guided_step.argtypes = [c.c_int, _I, _D, c.c_int, _D, c.c_double, c.c_int, c.c_int,
+                                  _D, _D, c.c_uint64, c.c_uint32, c.c_uint32, c.c_double,
+                                  c.c_double, c.c_int, _D, _D]
     L.orc_kalman_loglik_lg.argtypes = [c.POINTER(Model), c.c_double, c.c_int, _D]
     L.orc_kalman_loglik_lg.restype = c.c_double
     L.orc_kalman_loglik_lg_sched.argtypes = [c.POINTER(Model), _D, c.POINTER(c.c_int), c.c_int,
@@ -624,6 +627,25 @@ def mlp_loss_grad(dims, params, X, T, bf16=False, loss=LOSS_MSE):
     val = lib().orc_mlp_loss_grad(len(d) - 1, _ip(d), _dp(_f64(params)), _dp(X), _dp(T),
                                   X.shape[0], loss, 1 if bf16 else 0, _dp(g))
     return val, g
+
+
+def guided_step(dims, params, norm, F, Phi, y, seed, step, q=0.1, obs_sd=1.0, batch=0,
+                bf16=True, noise_bits=24):
+    """The learned guided proposal's step on flowed states Phi (d x P):
+    returns (X', increments); norm = (x_shift, x_scale, innov_scale,
+    par_shift, par_scale)."""
+    d_ = _i32(dims)
+    Phi = _f64(Phi)
+    d, P = Phi.shape
+    Xo = np.empty_like(Phi)
+    inc = np.empty(P)
+    nm = _f64(norm)
+    rc = lib().orc_guided_step(len(d_) - 1, _ip(d_), _dp(_f64(params)), 1 if bf16 else 0,
+                               _dp(nm), F, d, P, _dp(Phi), _dp(_f64(y)), seed, batch, step, q,
+                               obs_sd, noise_bits, _dp(Xo), _dp(inc))
+    if rc != OK:
+        raise ValueError("guided_step: dims do not fit the state")
+    return Xo, inc
 
 
 def bf16(x):

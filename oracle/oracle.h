@@ -219,6 +219,19 @@ enum { ORC_LOSS_MSE = 0, ORC_LOSS_GAUSS = 1 };
 double orc_mlp_loss_grad(int L, const int* dims, const double* params, const double* X,
     const double* T, int n, int loss_kind, int bf16, double* grad);
 
+/* The learned guided proposal's step (guided.c) over P flowed states Phi
+ * (d x P column-major, ld = P; d even, the even components observed in y):
+ * inputs u = ((Phi - norm[0]) / norm[1], (y - H Phi) / norm[2],
+ * (F - norm[3]) / norm[4]), (m, s) = MLP(u) (dims[0] = d + d/2 + 1,
+ * dims[L] = 2 d), x' = Phi + q (m + e^{s/2} eps) with the transition-noise
+ * normals of (seed, p, step, batch), and the log-weight increment
+ * log p(y | x') + log N(x'; Phi, q^2 I) - log N(x'; Phi + q m, q^2 e^s).
+ * Outputs Xout (d x P) and inc (P). */
+int orc_guided_step(int L, const int* dims, const double* params, int bf16,
+    const double* norm, double F, int d, int P, const double* Phi, const double* y,
+    uint64_t seed, uint32_t batch, uint32_t step, double q, double obs_sd, int noise_bits,
+    double* Xout, double* inc);
+
 /* scalar linear-Gaussian model: exact log-likelihood by the Kalman filter */
 double orc_kalman_loglik_lg(const orc_model* m, double dt, int T, const double* y);
 /* the same over a schedule: interval k is [times[k], times[k+1]], and step k

@@ -142,3 +142,41 @@ def test_gaussian_loss_gradient_matches_oracle(ctx, torch_gpu, orc):
     assert loss == pytest.approx(rl, rel=1e-4)
     assert rel < 5e-3
     net.close()
+
+
+def test_proposal_kernel_matches_oracle(ctx, torch_gpu, orc):
+    """The fused proposal kernel (bihip_guided_propose) against the FP64
+    oracle's guided step (orc_guided_step, bf16 rounding points of the GPU
+    network) on the same 301-256-256-256-400 network, flowed states,
+    observations and noise: the proposed states and the log-weight increments
+    (a sum of 300 terms) agree to the network's bf16 precision."""
+    torch = torch_gpu
+    P, seed, step = 2000, 9, 4
+    net = capi.MLP(ctx, guided.DIMS, seed=17)
+    p = net.get_params()
+    last = guided.DIMS[-1] * guided.DIMS[-2] + guided.DIMS[-1]
+    p[-last:] *= 0.3
+    net.set_params(p)
+    prm = net.get_params().astype(np.float64)
+    norm = _norm()
+    rng = np.random.default_rng(4)
+    Phi = (2.3 + 3.6 * rng.normal(size=(D, P))).astype(np.float32)
+    y = rng.normal(size=NOBS) + 2.0
+    X = torch.from_numpy(Phi.copy()).cuda()
+    inc = net.guided_propose(norm, 8.0, X, y, seed, step).cpu().numpy()
+    xg = X.cpu().numpy().astype(np.float64)
+    nv = (norm.x_shift, norm.x_scale, norm.innov_scale, norm.par_shift, norm.par_scale)
+    xo, io = orc.guided_step(guided.DIMS, prm, nv, 8.0, Phi.astype(np.float64), y, seed, step,
+                             q=0.1, obs_sd=1.0)
+    dx = np.abs(xg - xo)
+    di = np.abs(inc - io)
+    scale = np.abs(io).mean()
+    print(f"guided vs oracle: max |dx| {dx.max():.3e} (mean {dx.mean():.3e}); "
+          f"max |dinc| {di.max():.3e} (mean {di.mean():.3e}), mean |inc| {scale:.1f}", flush=True)
+    # x' = Phi + 0.1 (m + e^{s/2} eps): bf16 network outputs, FP32 state
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=2e-4)
+    assert dx.mean() < 2e-5
+    # increments: relative to their size, the bf16 network's precision
+    assert di.max() < 1e-3 * scale
+    assert di.mean() < 2e-4 * scale
+    net.close()
