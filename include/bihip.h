@@ -315,6 +315,16 @@ int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host)
  * batch}, the first with cumulative weight above u W), and its path
  * ((K + 1) x d); index_host (nullable) gets the particle. */
 int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host);
+/* Impose another filter's resampling history: step k resamples iff
+ * resampled_host[k] != 0, gathering through the P ancestors
+ * ancestors_host[k P .. k P + P - 1] (checked to lie in [0, P)), in place of
+ * the filter's own ESS test and draws; outputs report the imposed flags.
+ * Both arrays are copied (T flags; T x P ancestors kept on the device).
+ * NULL resampled_host removes the history.  No counterpart in the reference:
+ * it runs two filters on one genealogy (an FP32 filter against the FP64
+ * oracle's, a learned flow against the exact one).  Not with use_graph. */
+int bihip_pf_impose_resampling(bihip_pf* pf, const int* resampled_host,
+                               const int* ancestors_host);
 /* Current state buffer (device pointer, leading dimension) and log-weights. */
 int bihip_pf_state(bihip_pf* pf, void** X_dev, int* ld, double** lws_dev);
 /* Steps completed so far. */

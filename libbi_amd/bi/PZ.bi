@@ -1,18 +1,23 @@
 /**
- * Phytoplankton-zooplankton model of the LibBi user guide
- * (docs/src/users.tex:52-84), with the ode block's algorithm named: the
- * guide's default RK4(3) integrator is not built, RK4 with h = 0.1 is.
+ * Predator-prey plankton model: the equations, constants and distributions
+ * of the LibBi user guide's first example (docs/src/users.tex:52-84), with
+ * an explicit integrator (RK4, step 0.1) in the ode block, since the guide's
+ * default RK4(3) is not built here.
+ *
+ * P grows at a daily random rate alpha and is grazed by Z; Z converts a
+ * fraction e of what it grazes into growth and dies at linear and quadratic
+ * rates.  Only log P is observed, with noise of standard deviation 0.2.
  */
 model PZ {
-  const c = 0.25   // zooplankton clearance rate
-  const e = 0.3    // zooplankton growth efficiency
-  const m_l = 0.1  // zooplankton linear mortality
-  const m_q = 0.1  // zooplankton quadratic mortality
+  const c = 0.25    // grazing (clearance) rate of Z on P
+  const e = 0.3     // fraction of grazed P that becomes Z
+  const m_l = 0.1   // linear death rate of Z
+  const m_q = 0.1   // quadratic death rate of Z
 
-  param mu, sigma  // mean and std. dev. of phytoplankton growth
-  state P, Z       // phytoplankton, zooplankton
-  noise alpha      // stochastic phytoplankton growth rate
-  obs P_obs        // observations of phytoplankton
+  param mu, sigma   // daily growth rate of P: its mean and its spread
+  state P, Z        // prey (phytoplankton) and predator (zooplankton)
+  noise alpha       // the day's growth rate of P
+  obs P_obs         // measured P
 
   sub parameter {
     mu ~ uniform(0.0, 1.0)

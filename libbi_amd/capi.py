@@ -45,7 +45,7 @@ EXPORTS = [
     "bihip_pf_create", "bihip_pf_destroy", "bihip_pf_set_observations", "bihip_pf_init",
     "bihip_pf_step", "bihip_pf_term", "bihip_pf_status", "bihip_pf_outputs",
     "bihip_pf_ancestors", "bihip_pf_state", "bihip_pf_position", "bihip_pf_set_timing",
-    "bihip_pf_kernel_time",
+    "bihip_pf_kernel_time", "bihip_pf_impose_resampling",
     "bihip_batch_create", "bihip_batch_destroy", "bihip_batch_set_observations",
     "bihip_batch_init", "bihip_batch_step", "bihip_batch_outputs", "bihip_batch_ancestors",
     "bihip_batch_tuning", "bihip_batch_set_timing", "bihip_batch_kernel_time",
@@ -256,6 +256,7 @@ def _declare(L):
         "bihip_pf_write_netcdf": [P, c.c_char_p],
         "bihip_pf_set_schedule": [P, pd, pi],
         "bihip_pf_set_inputs": [P, pd],
+        "bihip_pf_impose_resampling": [P, pi, pi],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -519,6 +520,20 @@ class ParticleFilter:
         a = np.zeros(self.P, np.int32)
         check(lib().bihip_pf_ancestors(self.h, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return a
+
+    def impose_resampling(self, resampled, ancestors):
+        """Follow another filter's resampling history: step k resamples iff
+        resampled[k], with ancestors[k] (T x P); None removes it."""
+        if resampled is None:
+            check(lib().bihip_pf_impose_resampling(self.h, None, None))
+            return
+        r = np.ascontiguousarray(resampled, np.int32)
+        a = np.ascontiguousarray(ancestors, np.int32)
+        if r.shape != (self.T,) or a.shape != (self.T, self.P):
+            raise ValueError(f"need {self.T} flags and {self.T} x {self.P} ancestors")
+        self._imposed = (r, a)
+        I = ctypes.POINTER(ctypes.c_int)
+        check(lib().bihip_pf_impose_resampling(self.h, r.ctypes.data_as(I), a.ctypes.data_as(I)))
 
     def state_ptr(self):
         X = ctypes.c_void_p()

@@ -625,6 +625,9 @@ struct bihip_pf {
   std::vector<double> times_host;
   // inputs (bihip_pf_set_inputs): T x 16 model parameters
   double* inputs = nullptr;
+  // imposed resampling (bihip_pf_impose_resampling): T flags, T x P ancestors
+  int* imp_r = nullptr;
+  int* imp_as = nullptr;
 };
 
 // Timing events only: created without the system-scope fence, so recording one
@@ -658,6 +661,9 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
     if ((rc = pf_event_pair(pf, 1, &e0, &e1))) return rc;
     BIHIP_HIP(hipEventRecord(e0, s));
   }
+  // an imposed history replaces the filter's own resampling: no chain, the
+  // step gathers through the imposed ancestors of step k when flagged
+  const int* step_as = pf->imp_r != nullptr ? pf->imp_as + (size_t)pf->k * P : pf->as;
   ResampleTilesLaunch R{};
   R.mode = pf->cfg.resampler;
   R.lws = pf->lws;
@@ -674,9 +680,11 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   R.as = pf->as;
   R.EF = pf->EF;
   R.ws = pf->ws;
-  if ((rc = launch_resample_tiles(R, s))) return rc;
-  if (pf->hist_anc) {
-    if ((rc = launch_record_ancestors(pf->as, P, pf->dev, pf->hist_anc, s))) return rc;
+  if (pf->imp_r == nullptr) {
+    if ((rc = launch_resample_tiles(R, s))) return rc;
+    if (pf->hist_anc) {
+      if ((rc = launch_record_ancestors(pf->as, P, pf->dev, pf->hist_anc, s))) return rc;
+    }
   }
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
 
@@ -691,7 +699,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   T.Xout = pf->X[1 - pf->cur];
   T.ld = pf->ld;
   T.P = P;
-  T.as = pf->as;
+  T.as = step_as;
+  T.imp_r = pf->imp_r;
   T.lws = pf->lws;
   T.seed = pf->cfg.seed;
   T.batch = pf->cfg.batch;
@@ -725,6 +734,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
     G.nobs = pf->nobs;
     G.lws = pf->lws;
     G.partials = pf->partials;
+    G.imp_r = pf->imp_r;
     hipEvent_t g0 = nullptr, g1 = nullptr;
     if (timed_trans) {
       if ((rc = pf_event_pair(pf, 3, &g0, &g1))) return rc;
@@ -739,7 +749,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
     S.ld = pf->ld;
     S.P = P;
     S.step = true;
-    S.as = pf->as;
+    S.as = step_as;
+    S.imp_r = pf->imp_r;
     S.lws = pf->lws;
     S.partials = pf->partials;
     S.dev = pf->dev;
@@ -765,7 +776,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
     if ((rc = pf_event_pair(pf, 2, &e0, &e1))) return rc;
     BIHIP_HIP(hipEventRecord(e0, s));
   }
-  PFHistory H{pf->hist_li, pf->hist_ess, pf->hist_res};
+  // with an imposed history the recorded flags are the imposed ones
+  PFHistory H{pf->hist_li, pf->hist_ess, pf->imp_r != nullptr ? nullptr : pf->hist_res};
   if ((rc = launch_pf_correct_finalize(pf->partials, pf->nparts, P, pf->cfg.essRel, pf->dev, H,
                                        pf->rw, s, pf->obs_mask)))
     return rc;
@@ -888,6 +900,8 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->times);
   dfree(pf->obs_mask);
   dfree(pf->inputs);
+  dfree(pf->imp_r);
+  dfree(pf->imp_as);
   dfree(pf->out);
   dfree(pf->dev);
   dfree(pf->y);
@@ -935,6 +949,9 @@ int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
   BIHIP_HIP(hipMemsetAsync(pf->hist_li, 0, sizeof(double) * pf->cfg.T, s));
   BIHIP_HIP(hipMemsetAsync(pf->hist_ess, 0, sizeof(double) * pf->cfg.T, s));
   BIHIP_HIP(hipMemsetAsync(pf->hist_res, 0, sizeof(int) * pf->cfg.T, s));
+  if (pf->imp_r != nullptr)  // the recorded flags of an imposed history are its own
+    BIHIP_HIP(hipMemcpyAsync(pf->hist_res, pf->imp_r, sizeof(int) * pf->cfg.T,
+                             hipMemcpyDeviceToDevice, s));
   BIHIP_HIP(hipStreamSynchronize(s));
   for (auto& u : pf->ev_used) u = 0;
   pf->ready = true;
@@ -1101,6 +1118,54 @@ int bihip_pf_set_inputs(bihip_pf* pf, const double* inputs_host) {
     if (pf->graph[i]) (void)hipGraphExecDestroy(pf->graph[i]);
     pf->graph[i] = nullptr;
   }
+  return BIHIP_OK;
+}
+
+int bihip_pf_impose_resampling(bihip_pf* pf, const int* resampled_host,
+                               const int* ancestors_host) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  if (resampled_host == nullptr) {
+    BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+    dfree(pf->imp_r);
+    dfree(pf->imp_as);
+    pf->imp_r = nullptr;
+    pf->imp_as = nullptr;
+    return BIHIP_OK;
+  }
+  if (ancestors_host == nullptr) return set_error(BIHIP_EINVAL, "NULL ancestors");
+  if (pf->cfg.use_graph)
+    return set_error(BIHIP_ESTATE, "an imposed history needs a filter without graph replay");
+  const int T = pf->cfg.T, P = pf->cfg.P;
+  // every ancestor of a flagged step is checked here, before any kernel reads it
+  for (int k = 0; k < T; ++k) {
+    if (!resampled_host[k]) continue;
+    const int* a = ancestors_host + (size_t)k * P;
+    for (int i = 0; i < P; ++i)
+      if (a[i] < 0 || a[i] >= P)
+        return set_error(BIHIP_EINVAL, "ancestor %d of step %d is %d, outside [0, %d)", i, k,
+                         a[i], P);
+  }
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  if (pf->imp_r == nullptr) {
+    if ((rc = dalloc(&pf->imp_r, (size_t)T))) return rc;
+    if ((rc = dalloc(&pf->imp_as, (size_t)T * P))) {
+      dfree(pf->imp_r);
+      pf->imp_r = nullptr;
+      return rc;
+    }
+  }
+  std::vector<int> flags(T);
+  for (int k = 0; k < T; ++k) flags[k] = resampled_host[k] != 0;
+  BIHIP_HIP(hipMemcpyAsync(pf->imp_r, flags.data(), sizeof(int) * T, hipMemcpyHostToDevice,
+                           pf->ctx->stream));
+  if (pf->hist_res != nullptr)
+    BIHIP_HIP(hipMemcpyAsync(pf->hist_res, flags.data(), sizeof(int) * T,
+                             hipMemcpyHostToDevice, pf->ctx->stream));
+  BIHIP_HIP(hipMemcpyAsync(pf->imp_as, ancestors_host, sizeof(int) * (size_t)T * P,
+                           hipMemcpyHostToDevice, pf->ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
   return BIHIP_OK;
 }
 

@@ -47,6 +47,7 @@ struct GuidedArgs {
   double* partials;
   // standalone mode: log-weight increment per particle
   double* inc;
+  const int* imp_r;  // filter mode: imposed resampling flags per step (nullable)
 };
 
 namespace {
@@ -73,7 +74,8 @@ __global__ __launch_bounds__(GP_THREADS) void k_guided(GuidedArgs a) {
   bool lws_const = false;
   double lws_base = 0.0;
   if constexpr (FILTER) {
-    lws_const = a.dev->resample != 0 && a.dev->status == BIHIP_OK;
+    lws_const = a.imp_r != nullptr ? a.imp_r[a.dev->k] != 0
+                                   : a.dev->resample != 0 && a.dev->status == BIHIP_OK;
     lws_base = a.dev->logLikelihood;
     step = (uint32_t)(a.dev->k + 1);
     y = a.y_base + (size_t)a.dev->k * a.nobs;
@@ -206,6 +208,7 @@ int launch_guided(const GuidedLaunch& L, hipStream_t s) {
   a.lws = L.lws;
   a.partials = L.partials;
   a.inc = L.inc;
+  a.imp_r = L.imp_r;
   const dim3 grid(guided_blocks(L.P)), block(GP_THREADS);
   if (L.dev != nullptr) {
     if (L.lws == nullptr || L.partials == nullptr || L.y_base == nullptr)

@@ -143,6 +143,9 @@ struct TransLaunch {
   // generated model's delta starting at a multiple of it (gen_one_delta);
   // without dev, launch_transition decides from t1 and t2 itself
   int one_delta;
+  // with dev (nullable): imposed resampling flags, one per step; the step
+  // gathers through `as` iff imp_r[k] != 0, in place of dev->resample
+  const int* imp_r;
 };
 int transition_blocks(const bihip_model* m, int P);
 // A generated model whose steps [t0 + k dt, t0 + (k+1) dt], k < steps, are each
@@ -364,6 +367,7 @@ struct SurrogateLaunch {
   int nobs;
   float noise_sd;
   double obs_sd;
+  const int* imp_r;                  // step: imposed resampling flags (nullable)
 };
 int surrogate_blocks(int P);
 int launch_surrogate(const SurrogateLaunch& L, hipStream_t s);
@@ -388,6 +392,7 @@ struct GuidedLaunch {
   double* lws;
   double* partials;
   double* inc;        // standalone: log-weight increments
+  const int* imp_r;   // filter mode: imposed resampling flags (nullable)
 };
 int guided_blocks(int P);
 int launch_guided(const GuidedLaunch& L, hipStream_t s);

@@ -44,6 +44,7 @@ struct SurrArgs {
   int nobs;
   float noise_sd;
   double obs_sd;
+  const int* imp_r;  // imposed resampling flags per step (nullable)
 };
 
 namespace {
@@ -67,7 +68,8 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
   uint32_t step = 0;
   const double* y = nullptr;
   if constexpr (STEP) {
-    gather = a.dev->resample != 0 && a.dev->status == BIHIP_OK;
+    gather = a.imp_r != nullptr ? a.imp_r[a.dev->k] != 0
+                                : a.dev->resample != 0 && a.dev->status == BIHIP_OK;
     lws_const = gather;
     lws_base = a.dev->logLikelihood;
     const int k = a.dev->k;
@@ -204,6 +206,7 @@ int launch_surrogate(const SurrogateLaunch& L, hipStream_t s) {
   a.nobs = L.nobs;
   a.noise_sd = L.noise_sd;
   a.obs_sd = L.obs_sd;
+  a.imp_r = L.imp_r;
   const dim3 grid(surrogate_blocks(L.P)), block(SG_THREADS);
   if (L.step) {
     if (L.dev == nullptr || L.lws == nullptr || L.partials == nullptr || L.y_base == nullptr)

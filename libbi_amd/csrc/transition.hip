@@ -40,6 +40,7 @@ struct TransArgs {
   const double* times;  // with dev: schedule times (nullable: t0 + k dt)
   const int* mask;      // with dev: observed flags (nullable: every step observed)
   const double* inputs; // with dev: 16 model parameters per step (nullable)
+  const int* imp_r;     // with dev: imposed resampling flags per step (nullable)
 };
 
 // ---------------------------------------------------------------- the kernel
@@ -87,10 +88,11 @@ void k_transition(TransArgs a) {
                          ? a.inputs + (size_t)16 * a.dev->k
                          : nullptr;
   if (a.dev != nullptr) {
-    gather = a.dev->resample != 0 && a.dev->status == BIHIP_OK;
+    const int k = a.dev->k;
+    gather = a.imp_r != nullptr ? a.imp_r[k] != 0
+                                : a.dev->resample != 0 && a.dev->status == BIHIP_OK;
     lws_const = gather;
     lws_base = a.dev->logLikelihood;
-    const int k = a.dev->k;
     t1 = SCHED && a.times != nullptr ? a.times[k] : a.t0 + k * a.dt;
     t2 = SCHED && a.times != nullptr ? a.times[k + 1] : a.t0 + (k + 1) * a.dt;
     step = (uint32_t)(k + 1);
@@ -416,6 +418,7 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.nobs = L.nobs;
   r.a.flow_only = L.flow_only;
   r.a.aux = L.aux;
+  r.a.imp_r = L.imp_r;
   r.blocks = transition_blocks(L.model, L.P);
   r.s = s;
   r.integ = L.model->integrator;

@@ -133,3 +133,67 @@ def test_l96_fp32_local_error_vs_fp64_oracle(ctx, orc):
     pf.close()
     of.close()
     assert total / abs(logZ) < 1e-7
+
+
+def test_imposed_own_history_reproduces_the_device_filter(ctx, orc):
+    """bihip_pf_impose_resampling with the filter's own flags and ancestors
+    reproduces the run bit for bit (the imposed path gathers exactly as the
+    resampling chain does), and reports the imposed flags."""
+    P, T = 1 << 16, 20
+    mean0, y = _data(orc, T)
+    gm = _fp32_model()
+    runs = []
+    for imposed in (None, "own"):
+        pf = capi.ParticleFilter(ctx, gm, P, T, DT, seed=77, ess_rel=0.5, record=True)
+        pf.set_observations(y)
+        if imposed is not None:
+            pf.impose_resampling(runs[0][1]["resampled"], runs[0][2])
+        pf.init(mean0)
+        pf.step(T)
+        out = pf.outputs()
+        anc = np.array([pf.ancestors(k) for k in range(T)]) if imposed is None else None
+        runs.append((pf.term(), out, anc))
+        pf.close()
+    (z0, o0, _), (z1, o1, _) = runs
+    assert o0["resampled"].sum() >= 5
+    assert z1 == z0
+    np.testing.assert_array_equal(o1["logIncrements"], o0["logIncrements"])
+    np.testing.assert_array_equal(o1["resampled"], o0["resampled"])
+
+
+def test_imposed_history_rejects_out_of_range_ancestors(ctx, orc):
+    P, T = 256, 4
+    mean0, y = _data(orc, T)
+    pf = capi.ParticleFilter(ctx, _fp32_model(), P, T, DT, seed=1, ess_rel=0.5)
+    anc = np.tile(np.arange(P, dtype=np.int32), (T, 1))
+    anc[2, 17] = P
+    with pytest.raises(capi.BihipError):
+        pf.impose_resampling(np.array([0, 1, 1, 0]), anc)
+    pf.close()
+
+
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.MULTINOMIAL])
+def test_fp32_filter_on_the_fp64_oracle_history(ctx, orc, resampler):
+    """The reverse direction: the FP64 oracle filters with its own resampling,
+    and the FP32 device filter follows the oracle's history (N = 2^16,
+    T = 50)."""
+    P, T = 1 << 16, 50
+    mean0, y = _data(orc, T)
+    gm = _fp32_model()
+    om = oracle_model(orc, gm)
+    ref = orc.pf_run(om, y, P, DT, 77, resampler=resampler, ess_rel=0.5, mean0=mean0,
+                     want_ancestors=True)
+    pf = capi.ParticleFilter(ctx, gm, P, T, DT, seed=77, resampler=resampler, ess_rel=0.5)
+    pf.set_observations(y)
+    pf.impose_resampling(ref["resampled"], ref["ancestors"])
+    pf.init(mean0)
+    pf.step(T)
+    logZ = pf.term()
+    out = pf.outputs()
+    pf.close()
+    rel = abs(logZ - ref["logZ"]) / abs(ref["logZ"])
+    inc = np.abs(out["logIncrements"] - ref["logIncrements"])
+    print(f"FP32 on the oracle's history, resampler {resampler}: logZ {logZ:.8f} fp64 "
+          f"{ref['logZ']:.8f} rel {rel:.3e}; max |increment diff| {inc.max():.3e}", flush=True)
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    assert rel < 2e-5
