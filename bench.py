@@ -81,7 +81,9 @@ def parse():
     ap.add_argument("--resampler", default="systematic",
                     choices=["systematic", "stratified", "multinomial"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=1 << 18)
+    ap.add_argument("--cpu-particles", type=int, default=1 << 16)
+    ap.add_argument("--precision-steps", type=int, default=20,
+                    help="timed steps re-run by the FP64 oracle for the logZ error")
     ap.add_argument("--cpu-steps", type=int, default=60)
     return ap.parse_args()
 
@@ -95,25 +97,130 @@ def transition_bytes(P, resampled):
     return P * per
 
 
-def cpu_baseline(args, y, mean0):
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(args, y, mean0, check=None):
     """The FP64 OpenMP restatement of the reference host filter (the oracle),
-    timed on a bounded sample of the same workload."""
+    timed on a bounded sample of the same workload: the median of five runs.
+
+    With `check` (the device filter's state at the start of the timed steps
+    and its resampling history over them, from precision_history), the same
+    oracle also re-runs those timed steps in FP64 at full N on that history,
+    and the line gets the log-likelihood error of the FP32 run against it."""
     import oracle
     threads = len(os.sched_getaffinity(0))
     m = oracle.make_model("l96", d=D)
     P, T = args.cpu_particles, args.cpu_steps
-    t0 = time.perf_counter()
-    r = oracle.pf_run(m, y[:T], P, DT, 1234, resampler=oracle.SYSTEMATIC, ess_rel=0.5,
-                      mean0=mean0)
-    el = time.perf_counter() - t0
-    return {
+    times = []
+    for rep in range(5):
+        t0 = time.perf_counter()
+        r = oracle.pf_run(m, y[:T], P, DT, 1234 + rep, resampler=oracle.SYSTEMATIC, ess_rel=0.5,
+                          mean0=mean0)
+        times.append(time.perf_counter() - t0)
+    el = float(np.median(times))
+    cpu = {
         "value": P * T / el,
         "unit": "particle-steps/s",
         "cores": int(os.environ.get("OMP_NUM_THREADS", threads)),
         "kind": "port",
-        "sample": f"L96 d=40 DOPRI5 FP64 bootstrap PF, N={P}, {T} observation intervals "
-                  f"({el:.1f} s), OpenMP over particles, serial resampling",
+        "sample": f"L96 d=40 DOPRI5 FP64 bootstrap PF, N={P}, {T} observation intervals, "
+                  f"median of 5 runs ({', '.join(f'{t:.2f}' for t in times)} s), OpenMP over "
+                  "particles, serial resampling",
+        "cpu_model": cpu_model(),
         "logZ": r["logZ"],
+    }
+    if check is not None:
+        cpu["precision"] = precision_check(oracle, args, y, check)
+    return cpu
+
+
+def oracle_model_of(oracle, gm):
+    """The oracle's model matching a device model (FP32 kernels draw 24-bit normals)."""
+    m = oracle.Model()
+    m.model, m.d, m.integrator = gm.model, gm.d, gm.integrator
+    m.nsteps, m.host_quirk = gm.nsteps, gm.host_quirk
+    from libbi_amd import capi
+    m.noise_bits = 24 if gm.precision == capi.F32 else 32
+    m.h0, m.atol, m.rtol = gm.h0, gm.atol, gm.rtol
+    for i in range(16):
+        m.par[i] = gm.par[i]
+    return m
+
+
+def precision_history(args, ctx, gm, y, init_mean, timed):
+    """Re-runs the timed filter (untimed, same seed: the device filter is
+    deterministic) recording its ancestors, and returns its state at the start
+    of the timed steps and its resampling history over the first
+    args.precision_steps of them.  The re-run's increments must equal the
+    timed run's bit for bit."""
+    from libbi_amd import capi
+    res = {"systematic": capi.SYSTEMATIC, "stratified": capi.STRATIFIED,
+           "multinomial": capi.MULTINOMIAL}[args.resampler]
+    P, W = args.particles, args.warmup
+    K = min(args.steps, args.precision_steps)
+    pf = capi.ParticleFilter(ctx, gm, P, T_SCHEDULE, DT, seed=77, resampler=res, ess_rel=0.5,
+                             record=True)
+    pf.set_observations(y)
+    pf.init(init_mean)
+    pf.step(W)
+    out = pf.outputs()
+    X, _, lws = pf.state()
+    X = X[:, :P].cpu().numpy()
+    lws = lws.cpu().numpy()
+    ll_W, ess_prev = out["logLikelihood"], (out["ess"][W - 1] if W > 0 else 0.0)
+    pf.step(K)
+    out = pf.outputs()
+    anc = np.array([pf.ancestors(k) for k in range(W, W + K)])
+    pf.close()
+    same = bool(np.array_equal(out["logIncrements"][W:W + K], timed["logIncrements"][W:W + K]))
+    return dict(model=gm, resampler=res, X=X, lws=lws, ll=ll_W, ess=ess_prev, K=K,
+                resampled=out["resampled"][W:W + K], ancestors=anc,
+                increments=out["logIncrements"][W:W + K], rerun_identical=same,
+                logZ=timed["logLikelihood"])
+
+
+def precision_check(oracle, args, y, h):
+    """The FP64 oracle restarted from the FP32 run's state at the first timed
+    step (FP32 values are exact in FP64) and stepped on the FP32 run's
+    resampling history: the log-likelihood over those steps, FP32 against
+    FP64.  Lorenz-96's chaotic growth (about exp(1.7 t)) of the FP32 rounding
+    is part of the measured error."""
+    P, W, K = args.particles, args.warmup, h["K"]
+    om = oracle_model_of(oracle, h["model"])
+    of = oracle.StepFilter(om, P, DT, 77, resampler=h["resampler"], ess_rel=0.5)
+    t0 = time.perf_counter()
+    of.set_state(W, h["X"], h["lws"], h["ll"], h["ess"])
+    li = np.zeros(K)
+    for j in range(K):
+        r = int(h["resampled"][j])
+        rc, li[j] = of.step(y[W + j], r, h["ancestors"][j] if r else None)
+        if rc != oracle.OK:
+            raise RuntimeError(f"oracle step {W + j} failed ({rc})")
+    el = time.perf_counter() - t0
+    of.close()
+    z32 = float(np.sum(h["increments"]))
+    z64 = float(np.sum(li))
+    return {
+        "logZ": h["logZ"],
+        "timed_steps_checked": f"{W}..{W + K - 1}",
+        "loglik_fp32": z32,
+        "loglik_fp64": z64,
+        "logZ_rel_err_vs_fp64": abs(z32 - z64) / abs(z64),
+        "max_step_increment_diff": float(np.max(np.abs(h["increments"] - li))),
+        "rerun_identical": h["rerun_identical"],
+        "oracle_seconds": el,
+        "method": "FP64 oracle restarted from the FP32 state at the first timed step, on the "
+                  "FP32 run's resampling flags and ancestors, same Philox noise",
     }
 
 
@@ -573,7 +680,7 @@ def main():
         fig = run_l96(args, ctx, bm, y, None)
     else:
         fig = run_l96(args, ctx, gm, y, mean0)
-    elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first = fig
+    elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first, timed_out = fig
     bytes_total = sum(transition_bytes(P, bool(r)) for r in resampled)
     achieved = bytes_total / (trans_ms * 1e-3) / 1e9 if trans_ms > 0 else 0.0
 
@@ -624,8 +731,14 @@ def main():
     if generated:
         line["config"]["handwritten_same_init_particle_steps_per_s"] = P * args.steps / hand[0]
         line["config"]["handwritten_same_init_transition_ms"] = hand[1] / max(hand[2], 1)
+    line["logZ"] = timed_out["logLikelihood"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args, y, mean0)
+        check = None
+        if not generated and args.precision_steps > 0:
+            check = precision_history(args, ctx, gm, y, mean0, timed_out)
+        line["cpu_baseline"] = cpu_baseline(args, y, mean0, check)
+        if check is not None:
+            line["logZ_rel_err_vs_fp64"] = line["cpu_baseline"]["precision"]["logZ_rel_err_vs_fp64"]
     print(json.dumps(line), flush=True)
     ctx.close()
 
@@ -665,7 +778,7 @@ def run_l96(args, ctx, m, y, init_mean):
     k0, k1 = args.warmup, args.warmup + args.steps
     resampled = out["resampled"][k0:k1]
     pf.close()
-    return elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first
+    return elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first, out
 
 
 if __name__ == "__main__":

@@ -8,7 +8,9 @@
 // words in order from its own keyed Philox stream (its first block given by
 // the caller), so a rejection consumes words of that draw only; each
 // rejection loop is bounded (256 attempts, far beyond what any of these
-// methods needs at their acceptance rates above 0.85).  Arithmetic is double.
+// methods needs at their acceptance rates above 0.85); a draw that reaches
+// the bound returns NaN, so the failure reaches the weights (a NaN weight is
+// zero) instead of biasing the draw.  Arithmetic is double.
 //
 // Needs u32x4 and philox_block(seed, index, step, tag, batch, block).
 #pragma once
@@ -61,7 +63,7 @@ __device__ __forceinline__ double gen_gamma_unit(double k, DrawStream& w) {
     k += 1.0;
   }
   const double d = k - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
-  double x = d;
+  double x = __builtin_nan("");
   for (int t = 0; t < GEN_MAX_ATTEMPTS; ++t) {
     const double z = w.normal();
     const double v0 = 1.0 + c * z;
@@ -106,7 +108,7 @@ __device__ __forceinline__ double gen_poisson_stream(double lambda, DrawStream& 
   const double slam = sqrt(lambda), loglam = log(lambda);
   const double b = 0.931 + 2.53 * slam, a = -0.059 + 0.02483 * b;
   const double inv_alpha = 1.1239 + 1.1328 / (b - 3.4), vr = 0.9277 - 3.6224 / (b - 2.0);
-  double k = floor(lambda);
+  double k = __builtin_nan("");
   for (int t = 0; t < GEN_MAX_ATTEMPTS; ++t) {
     const double U = w.uniform() - 0.5, V = w.uniform();
     const double us = 0.5 - fabs(U);
@@ -164,7 +166,7 @@ __device__ __forceinline__ double gen_binomial(double n, double p, DrawStream w)
     const double alpha = (2.83 + 5.1 / b) * spq, lpq = log(pp / q);
     const double m = floor((n + 1.0) * pp);
     const double h = lgamma(m + 1.0) + lgamma(n - m + 1.0);
-    x = m;
+    x = __builtin_nan("");
     for (int t = 0; t < GEN_MAX_ATTEMPTS; ++t) {
       const double u = w.uniform() - 0.5, v = w.uniform();
       const double us = 0.5 - fabs(u);
