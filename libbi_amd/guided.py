@@ -88,8 +88,11 @@ def training_tensors(Phi, Xn, Y, norm, F=8.0, q=0.1):
 
 
 def train(ctx, n=1 << 21, steps=(3000, 2000, 1000), lrs=(1e-3, 3e-4, 1e-4), batch=4096, seed=1,
-          F=8.0, data_=None):
-    Phi, Xn, Y = data_ if data_ is not None else triples(ctx, n, F=F, seed=seed)
+          F=8.0, data_=None, burn=20, chunk=1 << 18):
+    """Train the proposal on n triples (an ensemble of `chunk` particles,
+    recorded every interval after `burn` intervals); returns (mlp, norm, report)."""
+    Phi, Xn, Y = data_ if data_ is not None else triples(ctx, n, F=F, seed=seed, burn=burn,
+                                                         chunk=chunk)
     norm = normalisation(Phi, F)
     U, T = training_tensors(Phi, Xn, Y, norm, F)
     net = capi.MLP(ctx, DIMS, seed=seed)

@@ -85,9 +85,13 @@ def test_l96_fp32_filter_vs_fp64_oracle_on_imposed_ancestors(ctx, orc, P, resamp
     # a decision may differ only where the FP64 ESS sits on the threshold
     for k in np.nonzero(own != out["resampled"])[0]:
         assert abs(ref["ess"][k - 1] / (0.5 * P) - 1.0) < 1e-5, (k, ref["ess"][k - 1])
-    # bound measured on MI355X (largest of the four cases, with a margin of 3):
-    # the chaotic growth of the FP32 rounding over 2.5 time units
-    assert rel < 1e-6, rel
+    # Measured on MI355X: below 1e-6 for systematic resampling, 3.3e-6 (N = 2^20)
+    # and 5.1e-6 (N = 2^16) for multinomial.  The bound is the largest with a
+    # margin of 4.  Its cause is the chaotic growth of the FP32 rounding of the
+    # first steps over 2.5 time units (about exp(1.7 * 2.5) = 70), not the
+    # weights: the per-step error that FP32 adds (the local test below) stays
+    # under 1e-7 of |logZ|.
+    assert rel < 2e-5, rel
 
 
 def test_l96_fp32_local_error_vs_fp64_oracle(ctx, orc):
