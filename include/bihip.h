@@ -250,6 +250,18 @@ int bihip_cumulative_offspring_to_ancestors(bihip_ctx* ctx, const int* Os_dev, i
  * copies fill the other slots in ascending order. Bit-exact with the host. */
 int bihip_cumulative_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* Os_dev, int P,
                                                     int* as_dev);
+/* The same permutation without any allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its scratch is a static device array of the library (about 8 MiB per GPU,
+ * held until the process ends), so P may not exceed BIHIP_NOALLOC_MAX_P
+ * (BIHIP_EINVAL beyond it) and a context must have been created on the
+ * device.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, threads or contexts) race and must be serialised
+ * by the caller.  Results equal bihip_cumulative_offspring_to_ancestors_permute's. */
+#define BIHIP_NOALLOC_MAX_P (1 << 20)
+int bihip_cumulative_offspring_to_ancestors_permute_noalloc(bihip_ctx* ctx, const int* Os_dev,
+                                                            int P, int* as_dev);
 /* offspringToAncestorsPermute (misc.hpp:120-128; ResamplerHost.hpp:86-107). */
 int bihip_offspring_to_ancestors_permute(bihip_ctx* ctx, const int* os_dev, int P,
                                          int* as_dev);

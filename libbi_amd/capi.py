@@ -25,6 +25,7 @@ LOSS_MSE, LOSS_GAUSS = 0, 1
 TAG_INIT, TAG_NOISE, TAG_SYSTEMATIC, TAG_STRATIFIED, TAG_MULTINOMIAL, TAG_OBS = 1, 2, 3, 4, 5, 6
 TAG_THETA, TAG_PROPOSAL, TAG_ACCEPT, TAG_OUTER, TAG_PATH = 7, 8, 9, 10, 13
 TIMING_ALL, TIMING_TRANSITION = 1, 2
+NOALLOC_MAX_P = 1 << 20  # BIHIP_NOALLOC_MAX_P: largest P of the allocation-free permutation
 
 # exported entry points (must match include/bihip.h)
 EXPORTS = [
@@ -41,6 +42,7 @@ EXPORTS = [
     "bihip_multinomial_ancestors", "bihip_ancestors_to_offspring",
     "bihip_offspring_to_cumulative", "bihip_cumulative_offspring_to_ancestors",
     "bihip_cumulative_offspring_to_ancestors_permute", "bihip_offspring_to_ancestors_permute",
+    "bihip_cumulative_offspring_to_ancestors_permute_noalloc",
     "bihip_gather_rows",
     "bihip_pf_create", "bihip_pf_destroy", "bihip_pf_set_observations", "bihip_pf_init",
     "bihip_pf_step", "bihip_pf_term", "bihip_pf_status", "bihip_pf_outputs",
@@ -191,6 +193,7 @@ def _declare(L):
         "bihip_cumulative_offspring_to_ancestors": [P, P, i, P],
         "bihip_cumulative_offspring_to_ancestors_permute": [P, P, i, P],
         "bihip_offspring_to_ancestors_permute": [P, P, i, P],
+        "bihip_cumulative_offspring_to_ancestors_permute_noalloc": [P, P, i, P],
         "bihip_gather_rows": [P, P, i, i, P, i, i],
         "bihip_pf_create": [P, pm, pc, c.POINTER(P)],
         "bihip_pf_destroy": [P],

@@ -167,9 +167,13 @@ int bihip_ctx_create(int device, bihip_ctx** out) {
   BIHIP_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_error(BIHIP_EINVAL, "no device %d", device);
   BIHIP_HIP(hipSetDevice(device));
+  // the device's address of the static scratch of the allocation-free entry
+  // point, looked up once per device with the device current
+  int rc = noalloc_scratch_init(device);
+  if (rc) return rc;
   bihip_ctx* c = new bihip_ctx();
   c->device = device;
-  int rc = dalloc(&c->out, 5);
+  rc = dalloc(&c->out, 5);
   if (rc) {
     delete c;
     return rc;

@@ -418,6 +418,8 @@ double prior_logdensity(const bihip_model* m, int n, const double* theta);
 // ---- context access for the other host files (capi.cpp)
 int ctx_enter_device(bihip_ctx* ctx);     // makes the context's device current
 hipStream_t ctx_stream_of(bihip_ctx* ctx);
+// ---- noalloc.hip: looks up the current device's copy of the static scratch
+int noalloc_scratch_init(int device);
 
 // ---- device memory helpers
 template <class T>
