@@ -29,6 +29,9 @@ Other workloads (`--workload`), each printing one JSON line of the same form:
   copy   a plain 1 GiB device-to-device copy: the achievable HBM ceiling, and
          the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE
          counters into bytes;
+  valu   the vector-ALU issue-rate probe (FP32 FMA, integer, FP32 mul/add
+         mixes): the achievable VALU ceiling, and the known-count kernel that
+         scales the SQ_INSTS_VALU counter;
   guided the Lorenz-96 d=200 filter with a learned guided proposal
          (301-256-256-256-400 BF16 MFMA network, trained on the GPU before the
          timed region) and the exact importance-weight correction, N=2^18
@@ -59,10 +62,20 @@ P_DEFAULT = 1 << 20
 T_SCHEDULE = 500
 DT = 0.05
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_transition launch at N=2^20, every step resampled: rocprofv3
-# --pmc FETCH_SIZE (x2.000, the factor the same counter shows on a 1 GiB copy)
-# plus WRITE_SIZE (x1.000), averaged over 7 launches.
-L96_TRAFFIC_BYTES = 410_475_582
+# Per-launch figures of k_transition<L96<float,40,4>,DOPRI5> on the command
+# `bench.py --gpus 1 --steps 20 --warmup 5` (N = 2^20, the 20 timed launches,
+# intervals 55..74, every one resampled), from rocprofv3 --pmc runs of that
+# command with --no-cpu-baseline:
+#  - HBM bytes: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the factors the same
+#    counters show for a 1 GiB device copy, bench.py --workload copy);
+#  - VALU wave-instructions (SQ_INSTS_VALU, which reads exactly the known count
+#    of the issue-rate probe) and the share of them in the FP32-rate classes
+#    (SQ_INSTS_VALU_FMA_F32, _ADD_F32, _MUL_F32, _TRANS_F32, _CVT, _INT64); the
+#    rest (SQ_INSTS_VALU_INT32 and the moves, DPP, compares, selects and bit
+#    operations no class counter covers) issue at the integer probe's rate.
+L96_TRAFFIC_BYTES = 396_513_232
+L96_VALU_PER_LAUNCH = 1.289e8
+L96_VALU_FP32_SHARE = 0.735
 
 
 def parse():
@@ -70,13 +83,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="l96",
                     choices=["l96", "l96-bi", "pz", "smc2", "smc2-bi", "copy", "surrogate",
-                             "guided"])
+                             "guided", "valu"])
     ap.add_argument("--thetas", type=int, default=64)
     ap.add_argument("--train-pairs", type=int, default=1 << 22)
     ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--seeds", type=int, default=3)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--settle", type=int, default=50,
+                    help="untimed filter steps before the warm-up (Lorenz-96 workloads): the "
+                         "initial cloud's transient")
     ap.add_argument("--particles", type=int, default=P_DEFAULT)
     ap.add_argument("--resampler", default="systematic",
                     choices=["systematic", "stratified", "multinomial"])
@@ -109,14 +125,9 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(args, y, mean0, check=None):
+def cpu_baseline(args, y, mean0):
     """The FP64 OpenMP restatement of the reference host filter (the oracle),
-    timed on a bounded sample of the same workload: the median of five runs.
-
-    With `check` (the device filter's state at the start of the timed steps
-    and its resampling history over them, from precision_history), the same
-    oracle also re-runs those timed steps in FP64 at full N on that history,
-    and the line gets the log-likelihood error of the FP32 run against it."""
+    timed on a bounded sample of the same workload: the median of five runs."""
     import oracle
     threads = len(os.sched_getaffinity(0))
     m = oracle.make_model("l96", d=D)
@@ -139,8 +150,6 @@ def cpu_baseline(args, y, mean0, check=None):
         "cpu_model": cpu_model(),
         "logZ": r["logZ"],
     }
-    if check is not None:
-        cpu["precision"] = precision_check(oracle, args, y, check)
     return cpu
 
 
@@ -166,7 +175,7 @@ def precision_history(args, ctx, gm, y, init_mean, timed):
     from libbi_amd import capi
     res = {"systematic": capi.SYSTEMATIC, "stratified": capi.STRATIFIED,
            "multinomial": capi.MULTINOMIAL}[args.resampler]
-    P, W = args.particles, args.warmup
+    P, W = args.particles, args.settle + args.warmup
     K = min(args.steps, args.precision_steps)
     pf = capi.ParticleFilter(ctx, gm, P, T_SCHEDULE, DT, seed=77, resampler=res, ess_rel=0.5,
                              record=True)
@@ -195,7 +204,7 @@ def precision_check(oracle, args, y, h):
     resampling history: the log-likelihood over those steps, FP32 against
     FP64.  Lorenz-96's chaotic growth (about exp(1.7 t)) of the FP32 rounding
     is part of the measured error."""
-    P, W, K = args.particles, args.warmup, h["K"]
+    P, W, K = args.particles, args.settle + args.warmup, h["K"]
     om = oracle_model_of(oracle, h["model"])
     of = oracle.StepFilter(om, P, DT, 77, resampler=h["resampler"], ess_rel=0.5)
     t0 = time.perf_counter()
@@ -640,6 +649,49 @@ def bench_copy(args):
                       "frac_of_peak": gbs / HBM_PEAK_GBS}), flush=True)
 
 
+def valu_probe(kinds=(0, 1, 2), trips=4096, reps=3):
+    """Wave-level VALU instructions per second of the issue-rate probe
+    (libbi_amd/lib/libbiprobe.so): eight independent chains per lane, eight
+    waves per SIMD, no memory traffic in the loop.  Returns {kind: rate}."""
+    import ctypes
+    import torch
+    L = ctypes.CDLL(os.path.join(HERE, "libbi_amd", "lib", "libbiprobe.so"))
+    L.biprobe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks = cus * 8  # 4 waves per block: 8 waves per SIMD
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    rates = {}
+    for k in kinds:
+        best = 0.0
+        for r in range(reps + 1):
+            ms, n = ctypes.c_float(), ctypes.c_double()
+            rc = L.biprobe_valu(k, blocks, trips, ctypes.c_void_p(out.data_ptr()),
+                                ctypes.byref(ms), ctypes.byref(n))
+            if rc != 0:
+                raise RuntimeError(f"VALU probe kind {k} failed ({rc})")
+            if r > 0:
+                best = max(best, n.value / (ms.value * 1e-3))
+        rates[k] = best
+    return rates, blocks, trips
+
+
+def bench_valu(args):
+    import torch
+    rates, blocks, trips = valu_probe()
+    props = torch.cuda.get_device_properties(0)
+    print(json.dumps({"metric": "VALU wave-instructions/s", "value": rates[0],
+                      "unit": "wave-instr/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "VALU issue-rate probe: 8 independent chains per "
+                                             "lane, 8 waves per SIMD, 64 VALU instructions per "
+                                             "loop trip, no memory traffic",
+                                 "blocks": blocks, "trips": trips, "cus": props.multi_processor_count},
+                      "rates": {"v_fma_f32": rates[0], "v_xor_b32+v_add_u32": rates[1],
+                                "v_mul_f32+v_add_f32": rates[2]}}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -658,11 +710,13 @@ def main():
         return bench_surrogate(args)
     if args.workload == "guided":
         return bench_guided(args)
+    if args.workload == "valu":
+        return bench_valu(args)
     import torch
     from libbi_amd import capi, data
 
-    if args.warmup + args.steps > T_SCHEDULE:
-        sys.exit(f"warmup + steps must be at most {T_SCHEDULE}")
+    if args.settle + args.warmup + args.steps > T_SCHEDULE:
+        sys.exit(f"settle + warmup + steps must be at most {T_SCHEDULE}")
     ctx = capi.Context(0)
     mean0 = data.l96_spinup(ctx, d=D)
     gm = capi.make_model("l96", d=D, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
@@ -708,6 +762,9 @@ def main():
             "particles": P,
             "resampled_steps": int(resampled.sum()),
             "log_weights": "f64",
+            "timed_intervals": f"{args.settle + args.warmup}..{args.settle + args.warmup + args.steps - 1} "
+                               f"of the T={T_SCHEDULE} schedule, after {args.settle} untimed settling "
+                               f"and {args.warmup} warm-up intervals",
         },
         "roofline": {
             "bound": "hbm",
@@ -718,16 +775,38 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": L96_TRAFFIC_BYTES if P == P_DEFAULT and not generated else None,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch, scaled by "
-                              "a 1 GiB copy (factors 2.0 / 1.0)",
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per launch of this "
+                              "command's timed window (bench.py --gpus 1 --steps 20 --warmup 5), "
+                              "scaled by a 1 GiB copy (factors 2.0 / 1.0)",
             "algorithmic_bytes_per_launch": bytes_total / max(len(resampled), 1),
             "avg_launch_ms": trans_ms / max(trans_n, 1),
             "launches": trans_n,
             "resample_chain_ms_per_step": res_ms / max(res_n, 1),
             "reduce_ms_per_step": red_ms / max(red_n, 1),
-            "phase_timing_sample": f"warm-up steps {first}..{args.warmup - 1}",
+            "phase_timing_sample": f"warm-up steps {args.settle + first}..{args.settle + args.warmup - 1}",
         },
     }
+    if not generated and P == P_DEFAULT:
+        # what bounds the kernel: its VALU issue rate against the ceiling that
+        # the issue-rate probe measures now, on this GPU, for its instruction mix
+        rates, _, _ = valu_probe(kinds=(0, 1))
+        launch_s = trans_ms * 1e-3 / max(trans_n, 1)
+        rate = L96_VALU_PER_LAUNCH / launch_s
+        ceiling = 1.0 / (L96_VALU_FP32_SHARE / rates[0] + (1.0 - L96_VALU_FP32_SHARE) / rates[1])
+        line["roofline"]["limiter"] = "valu"
+        line["roofline"]["valu_frac"] = rate / ceiling
+        line["roofline"]["valu"] = {
+            "wave_instr_per_launch": L96_VALU_PER_LAUNCH,
+            "issue_rate": rate,
+            "ceiling": ceiling,
+            "unit": "wave-instr/s",
+            "probe_fp32_fma": rates[0],
+            "probe_int32": rates[1],
+            "fp32_rate_share": L96_VALU_FP32_SHARE,
+            "source": "SQ_INSTS_VALU and its class counters per launch (rocprofv3 --pmc, this "
+                      "command's timed window) / this run's average launch time; ceiling = the "
+                      "issue-rate probes (bench.py --workload valu) weighted by the mix",
+        }
     if generated:
         line["config"]["handwritten_same_init_particle_steps_per_s"] = P * args.steps / hand[0]
         line["config"]["handwritten_same_init_transition_ms"] = hand[1] / max(hand[2], 1)
@@ -736,9 +815,13 @@ def main():
         check = None
         if not generated and args.precision_steps > 0:
             check = precision_history(args, ctx, gm, y, mean0, timed_out)
-        line["cpu_baseline"] = cpu_baseline(args, y, mean0, check)
+        line["cpu_baseline"] = cpu_baseline(args, y, mean0)
         if check is not None:
-            line["logZ_rel_err_vs_fp64"] = line["cpu_baseline"]["precision"]["logZ_rel_err_vs_fp64"]
+            # the same oracle re-runs the first timed steps in FP64 at full N
+            # on the timed run's resampling history
+            import oracle
+            line["precision"] = precision_check(oracle, args, y, check)
+            line["logZ_rel_err_vs_fp64"] = line["precision"]["logZ_rel_err_vs_fp64"]
     print(json.dumps(line), flush=True)
     ctx.close()
 
@@ -756,6 +839,12 @@ def run_l96(args, ctx, m, y, init_mean):
     pf.set_observations(y)
     pf.init(init_mean)
 
+    # settling: the initial cloud (x0 ~ N(mean0, I)) is wider than the
+    # filtering distribution for the first intervals, and DOPRI5 takes more
+    # steps on it (about 245 us per transition over intervals 0-24, 190 us
+    # from interval 50 on, 193 us averaged over the T = 500 run); the timed
+    # window starts after it, in the phase that makes up most of the run
+    pf.step(args.settle)
     # warm-up: the first steps untimed, the rest with events around every phase
     # (resampling chain and reduction figures); the timed steps carry events
     # around the transition only, since each event costs stream time
@@ -775,7 +864,7 @@ def run_l96(args, ctx, m, y, init_mean):
     pf.status()
     trans_ms, trans_n = pf.kernel_time(0)
     out = pf.outputs()
-    k0, k1 = args.warmup, args.warmup + args.steps
+    k0, k1 = args.settle + args.warmup, args.settle + args.warmup + args.steps
     resampled = out["resampled"][k0:k1]
     pf.close()
     return elapsed, trans_ms, trans_n, res_ms, res_n, red_ms, red_n, resampled, first, out

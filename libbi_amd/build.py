@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libbihip.so")
+PROBE_LIB = os.path.join(LIBDIR, "libbiprobe.so")
 MODELS = os.path.join(HERE, "bi")  # LibBi model files compiled into the library
 GEN = os.path.join(OBJ, "gen")
 
@@ -86,6 +87,15 @@ def build(verbose=False):
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the VALU issue-rate probe that bench.py --workload valu times (not part
+    # of the C ABI of bihip.h)
+    tmp = PROBE_LIB + ".tmp"
+    p = subprocess.run([HIPCC] + CFLAGS + ["-x", "hip", "-shared", os.path.join(CSRC, "probe.hip"),
+                                           "-o", tmp], capture_output=True, text=True)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout + p.stderr)
+        raise RuntimeError("hipcc failed for probe.hip")
+    os.replace(tmp, PROBE_LIB)
     return LIB
 
 

@@ -115,6 +115,20 @@ __device__ __forceinline__ double nan_max(double a, double b) {
   return a < b ? b : a;
 }
 
+// ---------------------------------------------------------------- XCD-aware blocks
+// The chip's 8 XCDs each have their own L2, and consecutive workgroups are
+// dispatched to them in turn (block b runs on XCD b % 8).  xcd_block maps the
+// dispatched block b of nb to a logical block so that each XCD takes one
+// contiguous range of logical blocks: neighbouring particles (for instance
+// the consecutive copies of one ancestor after a resample) then share one
+// L2 instead of being fetched into eight.  A bijection on [0, nb).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb / NUM_XCD, r = nb % NUM_XCD;
+  const int k = b % NUM_XCD, j = b / NUM_XCD;
+  return k * q + (k < r ? k : r) + j;
+}
+
 // ---------------------------------------------------------------- wave helpers
 constexpr int WAVE = 64;
 

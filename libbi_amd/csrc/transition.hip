@@ -71,7 +71,8 @@ __global__ __launch_bounds__(TRANS_BLOCK) __attribute__((amdgpu_waves_per_eu(tra
 void k_transition(TransArgs a) {
   using T = typename M::real_t;
   constexpr int LPP = M::LPP, DL = M::DL;
-  const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = blk * TRANS_BLOCK + threadIdx.x;
   const int p = tid / LPP;
   const Group<LPP> grp;
   const int lo = grp.g * DL;
@@ -161,7 +162,7 @@ void k_transition(TransArgs a) {
   }
   // only group leaders (lanes g == 0) hold a log-weight
   if (a.partials != nullptr)
-    block_weight_partial<TRANS_BLOCK, LPP>(lw, a.partials + 3 * blockIdx.x);
+    block_weight_partial<TRANS_BLOCK, LPP>(lw, a.partials + 3 * blk);
 }
 
 // ---------------------------------------------------------------- initial states
