@@ -290,7 +290,8 @@ typedef struct {
   uint64_t seed;     /* Philox key */
   uint32_t batch;    /* Philox batch id (theta index under SMC^2) */
   int record;        /* 1 = keep ancestors of every step on the device;
-                        2 = also the state at every time (paths) */
+                        2 = also the state at every time (paths);
+                        3 = the pruned ancestry tree only (paths, no full history) */
   int use_graph;     /* 1 = replay each step as a captured hipGraph */
 } bihip_pf_config;
 
@@ -322,6 +323,17 @@ int bihip_pf_ancestors(bihip_pf* pf, int k, int* as_host);
  * d = 40, N = 2^20, T = 500 in FP32, within one GPU's HBM).
  */
 int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host);
+/* record = 3: AncestryCache<ON_DEVICE> (share/src/bi/cache/AncestryCache.hpp:58-470,
+ * prune kernel share/src/bi/cuda/cache/AncestryCacheKernel.cuh:29-52): after
+ * every step the particles become a generation of nodes of a tree, leaves
+ * without children are freed with every ancestor left without surviving
+ * children, and only lineages that reach the current particles stay, so
+ * bihip_pf_paths and bihip_pf_sample_path work on histories whose full record
+ * would not fit in HBM.  The node store starts at 4 P nodes and doubles when a
+ * step could overflow it (the only host waits of such a filter).  Not with
+ * use_graph.  bihip_pf_cache_nodes: the live nodes and the node capacity
+ * (AncestryCache::report, AncestryCache.hpp:438-444); synchronises. */
+int bihip_pf_cache_nodes(bihip_pf* pf, long* live, long* capacity);
 /* BootstrapPF::samplePath (share/src/bi/filter/BootstrapPF.hpp:146-154): one
  * particle drawn in proportion to its weight (keyed uniform {0, K, PATH,
  * batch}, the first with cumulative weight above u W), and its path

@@ -65,6 +65,7 @@ EXPORTS = [
     "bihip_pfout_create", "bihip_pfout_write_time", "bihip_pfout_write_state",
     "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
+    "bihip_pf_cache_nodes",
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
     "bihip_model_generated", "bihip_model_var_name", "bihip_model_name", "bihip_model_var_size",
     "bihip_model_prior_sample", "bihip_model_prior_logdensity",
@@ -194,6 +195,7 @@ def _declare(L):
         "bihip_cumulative_offspring_to_ancestors_permute": [P, P, i, P],
         "bihip_offspring_to_ancestors_permute": [P, P, i, P],
         "bihip_cumulative_offspring_to_ancestors_permute_noalloc": [P, P, i, P],
+        "bihip_pf_cache_nodes": [P, pl, pl],
         "bihip_gather_rows": [P, P, i, i, P, i, i],
         "bihip_pf_create": [P, pm, pc, c.POINTER(P)],
         "bihip_pf_destroy": [P],
@@ -461,9 +463,11 @@ class ParticleFilter:
 
     def __init__(self, ctx, model, P, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0,
                  batch=0, record=False, use_graph=False):
-        """record: False, True (ancestors) or "paths" (ancestors and states)."""
+        """record: False, True (ancestors), "paths" (ancestors and states at
+        every time) or "tree" (the pruned ancestry tree: paths without the
+        full history)."""
         self.ctx, self.model = ctx, model
-        rec = 2 if record == "paths" else (1 if record else 0)
+        rec = {"paths": 2, "tree": 3}.get(record, 1 if record else 0)
         self.cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
                             seed=seed, batch=batch, record=rec,
                             use_graph=1 if use_graph else 0)
@@ -555,6 +559,12 @@ class ParticleFilter:
         check(lib().bihip_pf_paths(self.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                    len(idx), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def cache_nodes(self):
+        """(live nodes, node capacity) of the ancestry tree (record="tree")."""
+        live, cap = ctypes.c_long(), ctypes.c_long()
+        check(lib().bihip_pf_cache_nodes(self.h, ctypes.byref(live), ctypes.byref(cap)))
+        return live.value, cap.value
 
     def sample_path(self):
         K = self.position()

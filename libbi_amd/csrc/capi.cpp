@@ -632,6 +632,9 @@ struct bihip_pf {
   // imposed resampling (bihip_pf_impose_resampling): T flags, T x P ancestors
   int* imp_r = nullptr;
   int* imp_as = nullptr;
+  // record = 3: the pruned ancestry tree (AncestryCache)
+  AncestryCache cache{};
+  bool use_cache = false;
 };
 
 // Timing events only: created without the system-scope fence, so recording one
@@ -654,11 +657,30 @@ static int pf_event_pair(bihip_pf* pf, int kind, hipEvent_t* a, hipEvent_t* b) {
 // One observation interval: resample (device-gated) -> record -> predict+correct -> reduce.
 // timing: 0, BIHIP_TIMING_ALL (events around every phase) or
 // BIHIP_TIMING_TRANSITION (around the transition and proposal only).
+// Room for one more generation in the ancestry tree: the host keeps an upper
+// bound of its live nodes (P more per step) and reads the true count, and
+// grows the store, only when that bound leaves no room.
+static int pf_cache_room(bihip_pf* pf, hipStream_t s) {
+  AncestryCache& c = pf->cache;
+  const long P = pf->cfg.P;
+  if (c.live_ub + P <= c.cap) return BIHIP_OK;
+  int live = 0;
+  BIHIP_HIP(hipMemcpyAsync(&live, c.live, sizeof(int), hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  c.live_ub = live;
+  if (live + P <= c.cap) return BIHIP_OK;
+  const long want = std::max(2 * c.cap, (long)live + 2 * P);
+  if (want >= (1L << 31))
+    return set_error(BIHIP_EINVAL, "the ancestry tree would exceed 2^31 nodes");
+  return ancestry_enlarge(&c, want, s);
+}
+
 static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   const bool timed = timing == BIHIP_TIMING_ALL;       // every phase
   const bool timed_trans = timing != 0;                // transition and proposal
   const int P = pf->cfg.P;
   int rc;
+  if (pf->use_cache && (rc = pf_cache_room(pf, s))) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // ---- resample (Resampler::resample, gated by dev->resample)
   if (timed) {
@@ -769,6 +791,14 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
     return rc;
   }
   if (timed_trans) BIHIP_HIP(hipEventRecord(e1, s));
+  // the new generation into the ancestry tree (AncestryCache::writeState,
+  // share/src/bi/cache/AncestryCache.hpp:400-436), before the reduction moves
+  // the resampling decision on
+  if (pf->use_cache) {
+    if ((rc = ancestry_write(&pf->cache, T.Xout, pf->ld, step_as, pf->dev, pf->imp_r, s)))
+      return rc;
+    pf->cache.live_ub += P;
+  }
   if (pf->hist_X != nullptr &&
       ((rc = launch_record_state(T.Xout, pf->ld, pf->model.d, P, pf->elem, pf->dev, pf->hist_X, 0,
                                  s)) ||
@@ -829,6 +859,10 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   if (!(cfg->essRel >= 0.0 && cfg->essRel <= 1.0))
     return set_error(BIHIP_EINVAL, "essRel must lie in [0,1]");
   if (!(cfg->dt > 0.0)) return set_error(BIHIP_EINVAL, "dt must be positive");
+  if (cfg->record < 0 || cfg->record > 3) return set_error(BIHIP_EINVAL, "record must be 0..3");
+  if (cfg->record == 3 && cfg->use_graph)
+    return set_error(BIHIP_EINVAL, "the ancestry tree (record = 3) needs a filter without graph "
+                                   "replay");
   if ((rc = check_interval(m, cfg->t0, cfg->t0 + cfg->dt))) return rc;
   bihip_pf* pf = new bihip_pf();
   pf->ctx = ctx;
@@ -865,7 +899,7 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   PF_ALLOC(pf->hist_li, cfg->T);
   PF_ALLOC(pf->hist_ess, cfg->T);
   PF_ALLOC(pf->hist_res, cfg->T);
-  if (cfg->record) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
+  if (cfg->record == 1 || cfg->record == 2) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
   if (cfg->record == 2) {
     uint8_t* hx = nullptr;
     PF_ALLOC(hx, (size_t)(cfg->T + 1) * m->d * pf->ld * pf->elem);
@@ -873,6 +907,15 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
     PF_ALLOC(pf->hist_lw, (size_t)(cfg->T + 1) * P);
   }
 #undef PF_ALLOC
+  if (cfg->record == 3) {
+    // room for four generations to start with; grown when a step could overflow it
+    if ((rc = ancestry_alloc(&pf->cache, P, m->d, pf->elem, (long)4 * P, ctx->stream))) {
+      ancestry_free(&pf->cache);
+      bihip_pf_destroy(pf);
+      return rc;
+    }
+    pf->use_cache = true;
+  }
   if ((rc = ws_alloc(&pf->ws, scan_tiles(P))) ||
       (rc = rw_alloc(&pf->rw, std::max(pf->nparts, ceil_div(P, RED_TILE))))) {
     bihip_pf_destroy(pf);
@@ -913,6 +956,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->hist_ess);
   dfree(pf->hist_res);
   dfree(pf->hist_anc);
+  ancestry_free(&pf->cache);
   rw_free(&pf->rw);
   ws_free(&pf->ws);
   delete pf;
@@ -947,6 +991,10 @@ int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
        (rc = launch_record_state(pf->lws, pf->cfg.P, 1, pf->cfg.P, 8, nullptr, pf->hist_lw, 0,
                                  s))))
     return rc;
+  if (pf->use_cache) {
+    if ((rc = ancestry_init(&pf->cache, pf->X[0], pf->ld, s))) return rc;
+    pf->cache.live_ub = pf->cfg.P;
+  }
   PFDevice d{};
   d.status = BIHIP_OK;
   BIHIP_HIP(hipMemcpyAsync(pf->dev, &d, sizeof(d), hipMemcpyHostToDevice, s));
@@ -1258,8 +1306,8 @@ int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* n
 int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host) {
   if (pf == nullptr || (n > 0 && (idx_host == nullptr || paths_host == nullptr)))
     return set_error(BIHIP_EINVAL, "NULL argument");
-  if (pf->hist_X == nullptr || pf->hist_anc == nullptr)
-    return set_error(BIHIP_ESTATE, "paths need record = 2");
+  if (!pf->use_cache && (pf->hist_X == nullptr || pf->hist_anc == nullptr))
+    return set_error(BIHIP_ESTATE, "paths need record = 2 or 3");
   if (n < 0) return set_error(BIHIP_EINVAL, "n < 0");
   for (int q = 0; q < n; ++q)
     if (idx_host[q] < 0 || idx_host[q] >= pf->cfg.P)
@@ -1276,8 +1324,9 @@ int bihip_pf_paths(bihip_pf* pf, const int* idx_host, int n, double* paths_host)
   }
   hipError_t e = hipMemcpyAsync(idx, idx_host, sizeof(int) * n, hipMemcpyHostToDevice, s);
   if (e == hipSuccess &&
-      (rc = launch_paths(pf->hist_anc, pf->hist_X, pf->ld, D, pf->cfg.P, K, pf->elem, idx, n, out,
-                         s)) == BIHIP_OK)
+      (rc = pf->use_cache ? ancestry_paths(&pf->cache, K, idx, n, out, s)
+                          : launch_paths(pf->hist_anc, pf->hist_X, pf->ld, D, pf->cfg.P, K,
+                                         pf->elem, idx, n, out, s)) == BIHIP_OK)
     e = hipMemcpyAsync(paths_host, out, sizeof(double) * n * (K + 1) * D, hipMemcpyDeviceToHost,
                        s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1316,6 +1365,20 @@ int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host) {
   const int pick = p < P ? p : P - 1;
   if (index_host) *index_host = pick;
   return bihip_pf_paths(pf, &pick, 1, path_host);
+}
+
+int bihip_pf_cache_nodes(bihip_pf* pf, long* live, long* capacity) {
+  if (pf == nullptr) return set_error(BIHIP_EINVAL, "NULL filter");
+  if (!pf->use_cache) return set_error(BIHIP_ESTATE, "the filter keeps no ancestry tree (record != 3)");
+  int rc = ctx_enter(pf->ctx);
+  if (rc) return rc;
+  int n = 0;
+  BIHIP_HIP(hipMemcpyAsync(&n, pf->cache.live, sizeof(int), hipMemcpyDeviceToHost,
+                           pf->ctx->stream));
+  BIHIP_HIP(hipStreamSynchronize(pf->ctx->stream));
+  if (live) *live = n;
+  if (capacity) *capacity = pf->cache.cap;
+  return BIHIP_OK;
 }
 
 int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {

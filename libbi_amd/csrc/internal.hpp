@@ -421,6 +421,30 @@ hipStream_t ctx_stream_of(bihip_ctx* ctx);
 // ---- noalloc.hip: looks up the current device's copy of the static scratch
 int noalloc_scratch_init(int device);
 
+// ---- ancestry.hip: the pruned ancestry tree of a filter (record = 3)
+struct AncestryCache {
+  int P = 0, D = 0, elem = 4, cur = 0;
+  long cap = 0;                       // node slots
+  void* Xs = nullptr;                 // node states, column-major, leading dimension cap
+  int* node_as = nullptr;             // parent slot (-1: first generation)
+  int* node_os = nullptr;             // surviving children (0: free slot)
+  int* ls[2] = {nullptr, nullptr};    // slot of each current particle (double-buffered)
+  int* cnt = nullptr;                 // children of each leaf in the last step
+  int* slots = nullptr;               // free slots taken by the new generation
+  int* tile_free = nullptr;           // compaction: free slots per tile, then tile offsets
+  long* nfree = nullptr;              // free slots found by the last write
+  int* live = nullptr;                // live nodes (device)
+  long live_ub = 0;                   // host: upper bound of *live since it was last read
+};
+int ancestry_alloc(AncestryCache* c, int P, int D, int elem, long cap, hipStream_t s);
+void ancestry_free(AncestryCache* c);
+int ancestry_enlarge(AncestryCache* c, long new_cap, hipStream_t s);
+int ancestry_init(AncestryCache* c, const void* X, int ld, hipStream_t s);
+int ancestry_write(AncestryCache* c, const void* X, int ld, const int* as, const PFDevice* dev,
+                   const int* imp_r, hipStream_t s);
+int ancestry_paths(const AncestryCache* c, int K, const int* idx, int n, double* out,
+                   hipStream_t s);
+
 // ---- device memory helpers
 template <class T>
 inline int dalloc(T** p, size_t n) {
