@@ -360,10 +360,12 @@ def bench_smc2(args, generated=False):
     bf.step(args.warmup)
     el = _timed(ctx, lambda: bf.step(args.steps))
     blk, grid, tune_ms = bf.tuning()
+    items, oversub, table = bf.tuning_table()
     out = bf.outputs()
     bf.close()
-    # kernel duration with events (direct launches at the tuned size, same steps)
-    bt = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, block=blk)
+    # kernel duration with events (direct launches at the tuned configuration, same steps)
+    bt = capi.BatchFilter(ctx, gm, N, theta, T, 1.0, seed=77, ess_rel=0.5, block=blk,
+                          items=items, oversub=oversub)
     bt.set_observations(y)
     bt.init()
     bt.step(args.warmup)
@@ -420,6 +422,7 @@ def bench_smc2(args, generated=False):
                           "(hipGraph)"),
            "particles": total, "filters": SMC2_B, "block": blk, "grid": grid,
            "tune_ms": [float(v) for v in tune_ms],
+           "tile_items": items, "grid_oversub": oversub, "configs_timed": len(table),
            "resampled_fraction": float(res.mean()),
            "sequential_pf_particle_steps_per_s": seq_ps / seq_el,
            "sequential_pf_sample": f"filters 0..{nseq - 1} one after another through bihip_pf "

@@ -450,6 +450,10 @@ typedef struct {
   int block;      /* workgroup size 256/512/1024, or 0 = autotune at the first step */
   long capacity;  /* particles the layout may grow to (0 = the initial total) */
   uint32_t stream_base; /* Philox batch id of filter 0; ids stay below 2^24 */
+  int items;      /* with block != 0: particles per thread in the resampling passes
+                     (0 = 4; 2 and 8 exist for PZ FP64 RK4 only) */
+  int oversub;    /* with block != 0: persistent grid of this many times the resident
+                     workgroups per CU (0 = 1) */
 } bihip_batch_config;
 
 int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_config* cfg,
@@ -470,6 +474,13 @@ int bihip_batch_ancestors(bihip_batch* b, int j, int k, int* as_host);
 /* The autotuner's choice (workgroup size, persistent grid) and its timings
  * of one step at 256 / 512 / 1024 threads (ms). */
 int bihip_batch_tuning(bihip_batch* b, int* block, int* grid, double* ms3_host);
+/* The autotuner's other choices (resampling tile width, grid oversubscription)
+ * and every configuration it timed: on entry *n is the capacity of cfg_host
+ * (n x 3 ints: block, items, oversub) and ms_host (n doubles), on return the
+ * number timed.  The reference has no autotuner (its kernels use fixed launch
+ * shapes, share/src/bi/cuda/cuda.hpp). */
+int bihip_batch_tuning_table(bihip_batch* b, int* items, int* oversub, int* n, int* cfg_host,
+                             double* ms_host);
 /* HIP-event timing of each fused step launch. */
 int bihip_batch_set_timing(bihip_batch* b, int enable);
 int bihip_batch_kernel_time(bihip_batch* b, double* ms_host, long* count_host);

@@ -50,7 +50,8 @@ EXPORTS = [
     "bihip_pf_kernel_time", "bihip_pf_impose_resampling",
     "bihip_batch_create", "bihip_batch_destroy", "bihip_batch_set_observations",
     "bihip_batch_init", "bihip_batch_step", "bihip_batch_outputs", "bihip_batch_ancestors",
-    "bihip_batch_tuning", "bihip_batch_set_timing", "bihip_batch_kernel_time",
+    "bihip_batch_tuning", "bihip_batch_tuning_table", "bihip_batch_set_timing",
+    "bihip_batch_kernel_time",
     "bihip_batch_particles", "bihip_batch_set_theta", "bihip_batch_set_sizes",
     "bihip_batch_set_stream_base", "bihip_batch_increments", "bihip_batch_resample_filters",
     "bihip_batch_accept",
@@ -98,6 +99,7 @@ class BatchConfig(ctypes.Structure):
         ("essRel", ctypes.c_double), ("t0", ctypes.c_double), ("dt", ctypes.c_double),
         ("seed", ctypes.c_uint64), ("record", ctypes.c_int), ("use_graph", ctypes.c_int),
         ("block", ctypes.c_int), ("capacity", ctypes.c_long), ("stream_base", ctypes.c_uint32),
+        ("items", ctypes.c_int), ("oversub", ctypes.c_int),
     ]
 
 
@@ -218,6 +220,7 @@ def _declare(L):
         "bihip_batch_outputs": [P, pd, pd, pd, pi, pi],
         "bihip_batch_ancestors": [P, i, i, pi],
         "bihip_batch_tuning": [P, pi, pi, pd],
+        "bihip_batch_tuning_table": [P, pi, pi, pi, pi, pd],
         "bihip_batch_set_timing": [P, i],
         "bihip_batch_kernel_time": [P, pd, pl],
         "bihip_batch_set_theta": [P, pd],
@@ -654,7 +657,8 @@ class BatchFilter:
     """bihip_batch: B inner bootstrap filters with their own N_j and theta_j."""
 
     def __init__(self, ctx, model, N, theta, T, dt, seed, resampler=SYSTEMATIC, ess_rel=0.5,
-                 t0=0.0, record=False, use_graph=False, block=0, capacity=0, stream_base=0):
+                 t0=0.0, record=False, use_graph=False, block=0, capacity=0, stream_base=0,
+                 items=0, oversub=0):
         self.ctx, self.model = ctx, model
         self.N = np.ascontiguousarray(N, dtype=np.int32)
         self.theta = np.ascontiguousarray(theta, dtype=np.float64)
@@ -663,7 +667,7 @@ class BatchFilter:
         self.cfg = BatchConfig(B=B, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
                                seed=seed, record=1 if record else 0,
                                use_graph=1 if use_graph else 0, block=block, capacity=capacity,
-                               stream_base=stream_base)
+                               stream_base=stream_base, items=items, oversub=oversub)
         h = ctypes.c_void_p()
         check(lib().bihip_batch_create(
             ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
@@ -719,6 +723,19 @@ class BatchFilter:
         check(lib().bihip_batch_tuning(self.h, ctypes.byref(blk), ctypes.byref(grid),
                                        ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return blk.value, grid.value, ms
+
+    def tuning_table(self):
+        """(items, oversub) chosen, and every timed configuration as a list of
+        ((block, items, oversub), ms)."""
+        it, ov, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(64)
+        cfg = np.zeros((64, 3), np.int32)
+        ms = np.zeros(64)
+        check(lib().bihip_batch_tuning_table(
+            self.h, ctypes.byref(it), ctypes.byref(ov), ctypes.byref(n),
+            cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        m = min(n.value, 64)
+        return it.value, ov.value, [(tuple(int(v) for v in cfg[i]), float(ms[i])) for i in range(m)]
 
     def set_timing(self, on):
         check(lib().bihip_batch_set_timing(self.h, 1 if on else 0))

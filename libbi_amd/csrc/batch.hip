@@ -13,6 +13,8 @@
 // with its own batch id (stream base + j at initialisation, kept when the
 // filter's content is replaced by a copy), so each inner filter equals the
 // single-filter path run with that batch id.
+#include <type_traits>
+
 #include "model_ops.hpp"
 #include "scan.hpp"
 #include "gen_model.hpp"
@@ -22,8 +24,11 @@ namespace bihip {
 // Largest inner filter: the permutation's search table E (values below N)
 // is kept in LDS as 16-bit integers.
 constexpr int BATCH_NMAX = 16384;
-// Consecutive particles per thread in the resampling passes.
-constexpr int BITEMS = 4;
+// Consecutive particles per thread in the resampling passes: the kernel's
+// template parameter ITEMS, 4 unless the autotuner picks another width.  The
+// hand-written predator-prey model in FP64 with RK4 (the SMC^2 workload) is
+// compiled with widths 2, 4 and 8; every other model with 4.
+constexpr int BITEMS_DEFAULT = 4;
 
 struct BatchArgs {
   ModelArgs m;
@@ -124,7 +129,7 @@ __device__ __forceinline__ int strat_offspring(double Ws, double W, int n, uint6
   return o < n ? o : n;
 }
 
-template <class M, int INTEG, int BLOCK>
+template <class M, int INTEG, int BLOCK, int BITEMS = BITEMS_DEFAULT>
 __global__ __launch_bounds__(BLOCK) void k_batch_step(BatchArgs a) {
   using T = typename M::real_t;
   constexpr int DL = M::DL;
@@ -419,23 +424,48 @@ int launch_batch_copy(const long* dst_off, const long* src_off, const int* len, 
 }
 
 // ---------------------------------------------------------------- dispatch
+// Tile widths compiled for a model and integrator (batch_items_for).
+template <class M, int INTEG>
+constexpr bool many_widths() {
+  return std::is_same<M, PZ<double>>::value && INTEG == BIHIP_RK4;
+}
+
 template <int BLOCK>
 struct BatchRunner {
   BatchArgs a;
-  int grid, integ;
+  int grid, integ, items;
   hipStream_t s;
+  template <class M, int INTEG>
+  int launch() {
+    if constexpr (many_widths<M, INTEG>()) {
+      switch (items) {
+        case 2:
+          hipLaunchKernelGGL((k_batch_step<M, INTEG, BLOCK, 2>), dim3(grid), dim3(BLOCK), 0, s, a);
+          BIHIP_LAUNCH_CHECK();
+          return BIHIP_OK;
+        case 8:
+          hipLaunchKernelGGL((k_batch_step<M, INTEG, BLOCK, 8>), dim3(grid), dim3(BLOCK), 0, s, a);
+          BIHIP_LAUNCH_CHECK();
+          return BIHIP_OK;
+        default:
+          break;
+      }
+    }
+    if (items != BITEMS_DEFAULT)
+      return set_error(BIHIP_EINVAL, "tile width %d not compiled for this model", items);
+    hipLaunchKernelGGL((k_batch_step<M, INTEG, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
   template <class M>
   int run() {
     if constexpr (M::LPP != 1) {
       return set_error(BIHIP_EINVAL, "batched filters take models of one lane per particle");
     } else if (integ == BIHIP_RK4) {
-      hipLaunchKernelGGL((k_batch_step<M, BIHIP_RK4, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, a);
+      return launch<M, BIHIP_RK4>();
     } else {
-      hipLaunchKernelGGL((k_batch_step<M, BIHIP_DOPRI5, BLOCK>), dim3(grid), dim3(BLOCK), 0, s,
-                         a);
+      return launch<M, BIHIP_DOPRI5>();
     }
-    BIHIP_LAUNCH_CHECK();
-    return BIHIP_OK;
   }
   template <class M>
   int occupancy() {
@@ -496,8 +526,10 @@ bool batch_model_supported(const bihip_model* m) {
 int batch_nmax() { return BATCH_NMAX; }
 
 template <int BLOCK>
-static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
+static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out, int items,
+                        int oversub) {
   BatchRunner<BLOCK> r;
+  r.items = items;
   r.a = BatchArgs{to_args(L.model), L.theta, L.ntheta, L.order, L.B, L.f, L.Xin, L.Xout, L.ld,
                   L.lws, L.as, L.anc_rec, L.hist_li, L.hist_ess, L.hist_res, L.y, L.nobs,
                   L.T, L.t0, L.dt, L.seed, L.essRel, L.resampler, L.counter};
@@ -509,7 +541,7 @@ static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
   // Workgroups beyond what is resident wait in the dispatcher and then take
   // whatever the queue still holds (or leave at once); no workgroup waits on
   // another, so oversubscription is safe and lets the hardware fill each CU.
-  const int per_cu = std::max(8, occupancy_of(L.model, r));
+  const int per_cu = std::max(8, occupancy_of(L.model, r)) * std::max(1, oversub);
   r.grid = std::max(1, std::min(L.B, cus * per_cu));
   if (grid_out) *grid_out = r.grid;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, L.counter);
@@ -517,13 +549,26 @@ static int launch_block(const BatchLaunch& L, hipStream_t s, int* grid_out) {
   return with_small_model(L.model, r);
 }
 
-int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out) {
+int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out, int items,
+                      int oversub) {
   switch (block) {
-    case 256: return launch_block<256>(L, s, grid_out);
-    case 512: return launch_block<512>(L, s, grid_out);
-    case 1024: return launch_block<1024>(L, s, grid_out);
+    case 256: return launch_block<256>(L, s, grid_out, items, oversub);
+    case 512: return launch_block<512>(L, s, grid_out, items, oversub);
+    case 1024: return launch_block<1024>(L, s, grid_out, items, oversub);
   }
   return set_error(BIHIP_EINVAL, "batch block size must be 256, 512 or 1024");
+}
+
+// The tile widths compiled for a model (the autotuner's candidates).
+int batch_items_for(const bihip_model* m, int* items) {
+  if (m->model == BIHIP_MODEL_PZ && m->precision == BIHIP_F64 && m->integrator == BIHIP_RK4) {
+    items[0] = 2;
+    items[1] = 4;
+    items[2] = 8;
+    return 3;
+  }
+  items[0] = BITEMS_DEFAULT;
+  return 1;
 }
 
 struct BatchInitRunner {

@@ -39,7 +39,11 @@ struct bihip_batch {
   int k = 0;
   bool ready = false, have_obs = false;
   int block = 0, grid = 0;
-  double tune_ms[3] = {0, 0, 0};
+  int items = 4, oversub = 1;  // resampling tile width, grid oversubscription
+  double tune_ms[3] = {0, 0, 0};  // best step time per workgroup size
+  // every configuration the autotuner timed: (block, items, oversub) and ms
+  std::vector<int> tune_cfg;
+  std::vector<double> tune_all;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -111,8 +115,11 @@ static BatchLaunch batch_launch(bihip_batch* b) {
   return L;
 }
 
-// The autotuner: one step from the current state with each workgroup size,
-// on a snapshot that is restored afterwards; the fastest size is kept.
+// The autotuner: one step from the current state with each configuration
+// (workgroup size 256 / 512 / 1024, resampling tile width of the widths
+// compiled for the model, persistent grid of one or two times the resident
+// workgroups per CU), on a snapshot that is restored afterwards; the fastest
+// configuration is kept.
 static int batch_tune(bihip_batch* b) {
   hipStream_t s = ctx_stream_of(b->ctx);
   const int B = b->cfg.B;
@@ -154,30 +161,42 @@ static int batch_tune(bihip_batch* b) {
     return BIHIP_OK;
   };
   const int cand[3] = {256, 512, 1024};
+  int widths[3];
+  const int nw = batch_items_for(&b->model, widths);
   hipEvent_t e0, e1;
   BIHIP_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
   BIHIP_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
   rc = save();
   double best = 1e300;
-  for (int c = 0; c < 3 && rc == BIHIP_OK; ++c) {
-    // two runs per size: the first also loads the code object
-    for (int rep = 0; rep < 2 && rc == BIHIP_OK; ++rep) {
-      if ((rc = restore())) break;
-      BIHIP_HIP(hipEventRecord(e0, s));
-      int grid = 0;
-      if ((rc = launch_batch_step(batch_launch(b), cand[c], s, &grid))) break;
-      BIHIP_HIP(hipEventRecord(e1, s));
-      BIHIP_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      BIHIP_HIP(hipEventElapsedTime(&ms, e0, e1));
-      b->tune_ms[c] = ms;
-      if (rep == 1 && ms < best) {
-        best = ms;
-        b->block = cand[c];
-        b->grid = grid;
+  b->tune_cfg.clear();
+  b->tune_all.clear();
+  for (int c = 0; c < 3; ++c) b->tune_ms[c] = 1e300;
+  for (int c = 0; c < 3 && rc == BIHIP_OK; ++c)
+    for (int w = 0; w < nw && rc == BIHIP_OK; ++w)
+      for (int ov = 1; ov <= 2 && rc == BIHIP_OK; ++ov) {
+        // two runs per configuration: the first also loads the code object
+        for (int rep = 0; rep < 2 && rc == BIHIP_OK; ++rep) {
+          if ((rc = restore())) break;
+          BIHIP_HIP(hipEventRecord(e0, s));
+          int grid = 0;
+          if ((rc = launch_batch_step(batch_launch(b), cand[c], s, &grid, widths[w], ov))) break;
+          BIHIP_HIP(hipEventRecord(e1, s));
+          BIHIP_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          BIHIP_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (rep == 0) continue;
+          b->tune_cfg.insert(b->tune_cfg.end(), {cand[c], widths[w], ov});
+          b->tune_all.push_back(ms);
+          b->tune_ms[c] = std::min(b->tune_ms[c], (double)ms);
+          if (ms < best) {
+            best = ms;
+            b->block = cand[c];
+            b->items = widths[w];
+            b->oversub = ov;
+            b->grid = grid;
+          }
+        }
       }
-    }
-  }
   if (rc == BIHIP_OK) rc = restore();
   (void)hipStreamSynchronize(s);
   (void)hipEventDestroy(e0);
@@ -196,7 +215,7 @@ static int batch_capture(bihip_batch* b, int parity) {
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     int grid = 0;
-    rc = launch_batch_step(batch_launch(b), b->block, cs, &grid);
+    rc = launch_batch_step(batch_launch(b), b->block, cs, &grid, b->items, b->oversub);
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (rc == BIHIP_OK && e2 != hipSuccess)
       rc = set_error(BIHIP_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
@@ -237,6 +256,13 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
     return set_error(BIHIP_EINVAL, "ntheta must lie in [0,16] with theta given");
   if (cfg->block != 0 && cfg->block != 256 && cfg->block != 512 && cfg->block != 1024)
     return set_error(BIHIP_EINVAL, "block must be 0 (autotune), 256, 512 or 1024");
+  if (cfg->items != 0) {
+    int widths[3];
+    const int nw = batch_items_for(m, widths);
+    if (std::find(widths, widths + nw, cfg->items) == widths + nw)
+      return set_error(BIHIP_EINVAL, "tile width %d not compiled for this model", cfg->items);
+  }
+  if (cfg->oversub < 0 || cfg->oversub > 8) return set_error(BIHIP_EINVAL, "oversub must lie in [0, 8]");
   const int B = cfg->B;
   bihip_batch* b = new bihip_batch();
   b->ctx = ctx;
@@ -247,6 +273,8 @@ int bihip_batch_create(bihip_ctx* ctx, const bihip_model* m, const bihip_batch_c
   b->d = m->d;
   b->elem = m->precision == BIHIP_F64 ? 8 : 4;
   b->block = cfg->block;
+  b->items = cfg->items > 0 ? cfg->items : 4;
+  b->oversub = cfg->oversub > 0 ? cfg->oversub : 1;
   b->stream_base = cfg->stream_base;
   if ((rc = check_sizes(N_host, B)) != BIHIP_OK) {
     delete b;
@@ -389,7 +417,8 @@ int bihip_batch_step(bihip_batch* b, int n) {
       }
       BIHIP_HIP(hipEventRecord(b->ev[b->ev_used], s));
       int grid = 0;
-      if ((rc = launch_batch_step(batch_launch(b), b->block, s, &grid))) return rc;
+      if ((rc = launch_batch_step(batch_launch(b), b->block, s, &grid, b->items, b->oversub)))
+        return rc;
       b->grid = grid;
       BIHIP_HIP(hipEventRecord(b->ev[b->ev_used + 1], s));
       b->ev_used += 2;
@@ -398,7 +427,8 @@ int bihip_batch_step(bihip_batch* b, int n) {
       BIHIP_HIP(hipGraphLaunch(b->graph[b->cur], s));
     } else {
       int grid = 0;
-      if ((rc = launch_batch_step(batch_launch(b), b->block, s, &grid))) return rc;
+      if ((rc = launch_batch_step(batch_launch(b), b->block, s, &grid, b->items, b->oversub)))
+        return rc;
       b->grid = grid;
     }
     b->cur = 1 - b->cur;
@@ -448,7 +478,25 @@ int bihip_batch_tuning(bihip_batch* b, int* block, int* grid, double* ms3) {
   if (block) *block = b->block;
   if (grid) *grid = b->grid;
   if (ms3)
-    for (int i = 0; i < 3; ++i) ms3[i] = b->tune_ms[i];
+    for (int i = 0; i < 3; ++i) ms3[i] = b->tune_ms[i] < 1e300 ? b->tune_ms[i] : 0.0;
+  return BIHIP_OK;
+}
+
+int bihip_batch_tuning_table(bihip_batch* b, int* items, int* oversub, int* n, int* cfg_host,
+                             double* ms_host) {
+  if (b == nullptr) return set_error(BIHIP_EINVAL, "NULL batch");
+  if (items) *items = b->items;
+  if (oversub) *oversub = b->oversub;
+  const int m = (int)b->tune_all.size();
+  if (n) {
+    const int cap = *n;
+    *n = m;
+    for (int i = 0; i < m && i < cap; ++i) {
+      if (cfg_host)
+        for (int q = 0; q < 3; ++q) cfg_host[3 * i + q] = b->tune_cfg[3 * i + q];
+      if (ms_host) ms_host[i] = b->tune_all[i];
+    }
+  }
   return BIHIP_OK;
 }
 

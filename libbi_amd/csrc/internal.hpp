@@ -283,7 +283,13 @@ struct BatchLaunch {
 };
 bool batch_model_supported(const bihip_model* m);
 int batch_nmax();
-int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out);
+// One fused step of every filter: workgroups of `block` threads, `items`
+// consecutive particles per thread in the resampling passes (a width that
+// batch_items_for lists for the model), and `oversub` times the resident
+// workgroups per CU in the persistent grid.
+int launch_batch_step(const BatchLaunch& L, int block, hipStream_t s, int* grid_out, int items,
+                      int oversub);
+int batch_items_for(const bihip_model* m, int* items);  // returns how many (at most 3)
 int launch_batch_init(const bihip_model* m, const double* theta, int ntheta, const long* off,
                       int B, long total, uint64_t seed, uint32_t stream_base, void* X, long ld,
                       hipStream_t s);

@@ -100,20 +100,58 @@ def test_batch_autotune_and_graph(ctx, orc):
     Ns = np.floor(np.exp(rng.uniform(np.log(256), np.log(16384), 64))).astype(int)
     theta = _thetas(64, seed=2)
     res = []
-    block = 0
+    block, items, oversub = 0, 0, 0
     for use_graph in (False, True):
         bf = capi.BatchFilter(ctx, gm, Ns, theta, T, 1.0, seed=8, ess_rel=0.5,
-                              use_graph=use_graph, block=block)
+                              use_graph=use_graph, block=block, items=items, oversub=oversub)
         bf.set_observations(y)
         bf.init()
         bf.step(T)
         blk, grid, ms = bf.tuning()
         if block == 0:
             assert blk in (256, 512, 1024) and grid > 0 and (ms > 0).all()
-            block = blk
+            it, ov, table = bf.tuning_table()
+            # workgroup size x tile width (2, 4, 8 for PZ FP64 RK4) x grid factor (1, 2)
+            assert len(table) == 18
+            assert {c[1] for c, _ in table} == {2, 4, 8} and {c[2] for c, _ in table} == {1, 2}
+            best = min(table, key=lambda r: r[1])
+            assert best[0] == (blk, it, ov)
+            print("autotuner:", sorted(table, key=lambda r: r[1])[:4], flush=True)
+            block, items, oversub = blk, it, ov
         res.append(bf.outputs())
         bf.close()
     np.testing.assert_array_equal(res[0]["logIncrements"], res[1]["logIncrements"])
+
+
+@pytest.mark.parametrize("items,oversub", [(2, 1), (8, 2), (4, 2)])
+def test_batch_tile_widths_give_the_same_bits(ctx, orc, items, oversub):
+    """The resampling passes' tile width and the grid's oversubscription change
+    no result: the weight sums are exact integers and each filter's floating
+    point sums run in the same order within its workgroup."""
+    T = 10
+    y = _pz_data(orc, T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    theta = _thetas(len(NS))
+    outs = []
+    for it, ov in ((4, 1), (items, oversub)):
+        bf = capi.BatchFilter(ctx, gm, NS, theta, T, 1.0, seed=42, ess_rel=0.7, record=True,
+                              block=512, items=it, oversub=ov)
+        bf.set_observations(y)
+        bf.init()
+        bf.step(T)
+        outs.append((bf.outputs(), [bf.ancestors(j, T - 1) for j in range(len(NS))]))
+        bf.close()
+    (o0, a0), (o1, a1) = outs
+    np.testing.assert_array_equal(o0["logIncrements"], o1["logIncrements"])
+    for j in range(len(NS)):
+        np.testing.assert_array_equal(a0[j], a1[j])
+
+
+def test_batch_rejects_uncompiled_tile_width(ctx):
+    gm = capi.make_model("lg", precision=capi.F64)
+    with pytest.raises(capi.BihipError):
+        capi.BatchFilter(ctx, gm, [100, 200], np.zeros((2, 0)), 4, 1.0, seed=1, block=256,
+                         items=8)
 
 
 def test_batch_rejects_oversized_filter(ctx):
