@@ -160,34 +160,89 @@ __global__ __launch_bounds__(256) void k_gauss_grad(const float* Y, int ldy, con
 }
 
 // part[s][c] = sum of bf16 dZ[r][c] over rows r in [256 s, 256 s + 256), in a
-// fixed order: four row phases per column, then their sum.
+// fixed order.  A workgroup takes 64 columns of one split: each thread reads
+// eight columns of eight rows (rows rs, rs + 32, ..., rs + 224 of the split,
+// rs = thread / 8) as 16-byte loads issued together, sums its rows in order,
+// and the 32 row slots are then added in order through LDS.  Needs cols and
+// ld to be multiples of 8 (the padded layer widths are multiples of 64).
 __global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* dZ, int ld, int rows,
                                                      int cols, float* part) {
-  __shared__ float red[4][64];
-  const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  __shared__ float red[32][64 + 1];
+  const int cg = threadIdx.x & 7, rs = threadIdx.x >> 3;
+  const int c0 = blockIdx.x * 64 + cg * 8;
   const int r0 = blockIdx.y * 256;
-  float s = 0.0f;
-  if (c < cols) {
-#pragma unroll 8
-    for (int i = g; i < 256; i += 4) {
-      const int r = r0 + i;
-      if (r < rows) s += bf2f(dZ[(size_t)r * ld + c]);
+  float acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0f;
+  if (c0 < cols) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + rs + 32 * u;
+      v[u] = r < rows ? *reinterpret_cast<const uint4*>(dZ + (size_t)r * ld + c0)
+                      : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[2 * q] += bf2f((uint16_t)(w[q] & 0xFFFFu));
+        acc[2 * q + 1] += bf2f((uint16_t)(w[q] >> 16));
+      }
     }
   }
-  red[g][threadIdx.x & 63] = s;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) red[rs][cg * 8 + j] = acc[j];
   __syncthreads();
-  if (g == 0 && c < cols)
-    part[(size_t)blockIdx.y * cols + c] =
-        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if (threadIdx.x < 64) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    float t = 0.0f;
+    for (int q = 0; q < 32; ++q) t += red[q][threadIdx.x];
+    if (c < cols) part[(size_t)blockIdx.y * cols + c] = t;
+  }
 }
 
-// out[i] = sum_s part[s][i] (fixed order)
+// out[i] = sum_s part[s][i], added in the order s = 0, 1, ..., S-1.  Each
+// thread takes four consecutive outputs (one 16-byte load per split when
+// count is a multiple of 4) and issues the loads of eight splits before
+// adding them, so the split loads overlap instead of each waiting for the
+// previous add; the order of the additions is unchanged.
 __global__ void k_sum_splits(const float* part, int S, long count, float* out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i >= count) return;
-  float s = 0.0f;
-  for (int z = 0; z < S; ++z) s += part[(size_t)z * count + i];
-  out[i] = s;
+  if ((count & 3) == 0) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int z = 0;
+    for (; z + 8 <= S; z += 8) {
+      float4 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        v[u] = *reinterpret_cast<const float4*>(part + (size_t)(z + u) * count + i);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        acc.x += v[u].x;
+        acc.y += v[u].y;
+        acc.z += v[u].z;
+        acc.w += v[u].w;
+      }
+    }
+    for (; z < S; ++z) {
+      const float4 v = *reinterpret_cast<const float4*>(part + (size_t)z * count + i);
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+    *reinterpret_cast<float4*>(out + i) = acc;
+    return;
+  }
+  const int n = count - i < 4 ? (int)(count - i) : 4;
+  for (int q = 0; q < n; ++q) {
+    float s = 0.0f;
+    for (int z = 0; z < S; ++z) s += part[(size_t)z * count + i + q];
+    out[i + q] = s;
+  }
 }
 
 // out[0] = sum of x[0 .. n) by one workgroup: strided partial sums, then a
@@ -293,18 +348,20 @@ int launch_gauss_grad(const float* Y, int ldy, const float* T, long ldt, const l
 
 int launch_colsum_bf16(const uint16_t* dZ, int ld, int rows, int cols, float* part, float* out,
                        hipStream_t s) {
+  if (cols % 8 || ld % 8 || ((uintptr_t)dZ & 15))
+    return set_error(BIHIP_EINVAL, "column sums need 8-aligned columns (cols %d, ld %d)", cols, ld);
   const int S = ceil_div(rows, 256);
   hipLaunchKernelGGL(k_colsum_bf16, dim3(ceil_div(cols, 64), S), dim3(256), 0, s, dZ, ld, rows,
                      cols, part);
   BIHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(cols, 256)), dim3(256), 0, s, part, S,
+  hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(cols, 1024)), dim3(256), 0, s, part, S,
                      (long)cols, out);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }
 
 int launch_sum_splits(const float* part, int S, long count, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(count, 256)), dim3(256), 0, s, part, S, count,
+  hipLaunchKernelGGL(k_sum_splits, dim3(ceil_div(count, 1024)), dim3(256), 0, s, part, S, count,
                      out);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
