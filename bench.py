@@ -85,7 +85,7 @@ def parse():
                     choices=["l96", "l96-bi", "pz", "smc2", "smc2-bi", "copy", "surrogate",
                              "guided", "valu"])
     ap.add_argument("--thetas", type=int, default=64)
-    ap.add_argument("--train-pairs", type=int, default=1 << 22)
+    ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=24000)
     ap.add_argument("--seeds", type=int, default=3)
     ap.add_argument("--steps", type=int, default=100)
@@ -458,7 +458,11 @@ def bench_surrogate(args):
     from libbi_amd import capi, data, surrogate
     ctx = capi.Context(0)
     t0 = time.perf_counter()
-    X, Y = surrogate.flow_pairs(ctx, args.train_pairs + (1 << 16), seed=3)
+    # pairs from the model's prior predictive: an ensemble of 2^15 drawn around
+    # the filter's initial mean, recorded at every interval from the first
+    # (so the pairs cover the states the filter visits over its run)
+    X, Y = surrogate.flow_pairs(ctx, args.train_pairs + (1 << 16), seed=3, burn=0,
+                                chunk=1 << 15)
     n = args.train_pairs
     q = args.train_steps // 4
     net, norm, rep = surrogate.train(ctx, steps=(q, q, q, args.train_steps - 3 * q),
@@ -503,6 +507,16 @@ def bench_surrogate(args):
     pf.step(args.steps)
     k_ms, k_n = pf.kernel_time(0)
     pf.close()
+    # the exact transition's kernel time on the same steps
+    pf = capi.ParticleFilter(ctx, gm, P, T, DT, seed=100, ess_rel=0.5)
+    pf.set_observations(y)
+    pf.init(mean0)
+    pf.step(args.warmup)
+    pf.set_timing(True, transition_only=True)
+    pf.step(args.steps)
+    x_ms, x_n = pf.kernel_time(0)
+    pf.close()
+    local = surrogate_local_error(ctx, gm, net, norm, y, mean0, P, min(T, 40))
     flops = 2.0 * sum(a * b for a, b in zip(surrogate.DIMS[:-1], surrogate.DIMS[1:])) * P
     achieved = flops * k_n / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     v_sur = P * args.steps * args.seeds / el["surrogate"]
@@ -518,7 +532,13 @@ def bench_surrogate(args):
            "logZ_diff_mean": float(diff.mean()),
            "logZ_diff_sd": float(diff.std(ddof=1)) if len(diff) > 1 else None,
            "logZ_exact_sd_over_seeds": float(np.std(res["exact"], ddof=1))
-           if len(diff) > 1 else None}
+           if len(diff) > 1 else None,
+           "loglik_error_per_interval": local,
+           # what the network could gain at best: its FLOPs at the full BF16
+           # MFMA rate against the exact DOPRI5 transition's measured time
+           "exact_transition_ms": x_ms / max(x_n, 1),
+           "network_ms_at_mfma_peak": flops / 2.5e15 * 1e3,
+           "speedup_bound_at_mfma_peak": (x_ms / max(x_n, 1)) / (flops / 2.5e15 * 1e3)}
     roof = {"bound": "mfma", "kernel": "k_surrogate<40,256,4,step>", "achieved": achieved,
             "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
             "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n,
@@ -528,6 +548,48 @@ def bench_surrogate(args):
     ctx.close()
 
 
+def surrogate_local_error(ctx, gm, net, norm, y, mean0, P, K, seed=8):
+    """Log-likelihood error of the surrogate per observation interval: from
+    the exact filter's particles before each of the first K steps (resampled
+    to equal weights), one interval with the exact FP32 flow and one with the
+    surrogate, the same noise, and the observation density; the difference of
+    the two log-mean-weights.  Free of Monte Carlo error between the two runs
+    (common particles and noise) and of the chaotic growth of the flow error
+    along a lineage."""
+    import ctypes
+    import torch
+    from libbi_amd import capi
+    pf = capi.ParticleFilter(ctx, gm, P, K, DT, seed=seed, ess_rel=0.5)
+    pf.set_observations(y[:K])
+    pf.init(mean0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1000 + seed)
+    diffs = []
+    for k in range(K):
+        if k > 0:
+            pf.step(1)
+        X, _, lws = pf.state()
+        w = torch.exp(lws - lws.max())
+        C = X[:, :P][:, torch.multinomial(w, P, replacement=True, generator=g)].contiguous()
+        Zx = C.clone()
+        capi.check(capi.lib().bihip_flow(ctx.h, ctypes.byref(gm), k * DT, (k + 1) * DT, P,
+                                         ctypes.c_void_p(Zx.data_ptr()), P, None, None))
+        Zs = net.surrogate_flow(norm, 8.0, C)
+        eps = 0.1 * torch.randn(C.shape, generator=g, device="cuda", dtype=torch.float32)
+        yk = torch.as_tensor(y[k], device="cuda")
+
+        def ll(Z):
+            inc = (-0.5 * (yk[:, None] - (Z + eps).double()[0::2]) ** 2).sum(0)
+            return float(torch.logsumexp(inc, 0)) - np.log(P)
+        diffs.append(ll(Zs) - ll(Zx))
+    pf.close()
+    d = np.array(diffs)
+    return {"intervals": K, "mean_abs_nats": float(np.abs(d).mean()),
+            "max_abs_nats": float(np.abs(d).max()), "signed_sum_nats": float(d.sum()),
+            "method": "exact filter's clouds, common particles and noise, exact FP32 flow "
+                      "against the surrogate over one interval"}
+
+
 def bench_guided(args):
     """Guided-proposal filters for 64 forcing values against bootstrap filters."""
     import torch
@@ -535,7 +597,10 @@ def bench_guided(args):
     ctx = capi.Context(0)
     t0 = time.perf_counter()
     Fs_train = [7.5, 7.8333, 8.1667, 8.5]
-    chunks = [guided.triples(ctx, (1 << 21) // len(Fs_train), F=F, seed=11 + i)
+    # triples from each forcing value's prior predictive (an ensemble drawn
+    # around the initial mean, recorded at every interval from the first)
+    chunks = [guided.triples(ctx, (1 << 21) // len(Fs_train), F=F, seed=11 + i, burn=0,
+                             chunk=1 << 12)
               for i, F in enumerate(Fs_train)]
     Phi = torch.cat([c[0] for c in chunks])
     norm = guided.normalisation(Phi)
