@@ -614,7 +614,29 @@ struct MultinomialArgs {
 };
 
 // a_j = #{i : Ws_i <= W U_j}: upper bound, never a zero-weight particle.
-__global__ __launch_bounds__(RS_BLOCK) void k_multinomial(MultinomialArgs a) {
+// A workgroup of MN_BLOCK threads takes MN_DRAWS consecutive draws, MN_ITEMS
+// per thread (draw j = block * MN_DRAWS + q * MN_BLOCK + thread, q < MN_ITEMS).
+// The search runs in two levels: the block loads the last Ws of each of
+// MN_FENCES equal chunks into LDS and finds the chunk of u there, then
+// searches that chunk in global memory (11 dependent loads at 2^20 instead of
+// 20), the thread's MN_ITEMS searches in lockstep so their loads overlap.
+// The offspring counts are gathered per block in an LDS hash table: one
+// device-wide atomic per distinct ancestor of the block's MN_DRAWS draws
+// (plus one per draw whose slot another ancestor took), so a particle that
+// takes thousands of draws is hit by one atomic per block, not one per draw.
+// Counts are integers: the result does not depend on the order of the
+// additions.
+constexpr int MN_FENCES = 512;
+constexpr int MN_BLOCK = 1024;
+constexpr int MN_ITEMS = 4;
+constexpr int MN_DRAWS = MN_BLOCK * MN_ITEMS;
+constexpr int MN_HASH_BITS = 12;
+constexpr int MN_HASH = 1 << MN_HASH_BITS;
+
+__global__ __launch_bounds__(MN_BLOCK) void k_multinomial(MultinomialArgs a) {
+  __shared__ double fence[MN_FENCES];
+  __shared__ int hkey[MN_HASH];
+  __shared__ int hcnt[MN_HASH];
   if (!gate_open(a.gate)) return;
   const double W = a.Ws[a.P - 1];
   if (!(W > 0.0)) {
@@ -622,32 +644,85 @@ __global__ __launch_bounds__(RS_BLOCK) void k_multinomial(MultinomialArgs a) {
       a.status->status = BIHIP_DEGENERATE;
     return;
   }
-  const int j = blockIdx.x * RS_BLOCK + threadIdx.x;
-  if (j >= a.n) return;
-  double U;
-  if (a.U != nullptr) {
-    U = a.U[j];
-  } else {
-    const uint32_t step = a.gate != nullptr ? (uint32_t)a.gate->k : a.step;
-    const u32x4 r = philox_block(a.seed, (uint32_t)j, step, TAG_MULTINOMIAL, a.batch, 0);
-    U = u53(r.x, r.y);
+  const int stride = (a.P + MN_FENCES - 1) / MN_FENCES;
+  const int nch = (a.P + stride - 1) / stride;  // chunks, at most MN_FENCES
+  for (int c = threadIdx.x; c < MN_FENCES; c += MN_BLOCK) {
+    const int last = (c + 1) * stride < a.P ? (c + 1) * stride - 1 : a.P - 1;
+    fence[c] = c < nch ? a.Ws[last] : 0.0;
   }
-  const double u = W * U;
-  int lo = 0, hi = a.P;
-  while (lo < hi) {
-    const int mid = lo + ((hi - lo) >> 1);
-    if (a.Ws[mid] <= u) lo = mid + 1;
-    else hi = mid;
+  for (int h = threadIdx.x; h < MN_HASH; h += MN_BLOCK) {
+    hkey[h] = -1;
+    hcnt[h] = 0;
   }
-  if (lo >= a.P) lo = a.P - 1;
-  if (a.as != nullptr) a.as[j] = lo;
-  if (a.os != nullptr) atomicAdd(&a.os[lo], 1);
+  __syncthreads();
+  const uint32_t step = a.gate != nullptr ? (uint32_t)a.gate->k : a.step;
+  int anc[MN_ITEMS], lo[MN_ITEMS], top[MN_ITEMS];
+  double u[MN_ITEMS];
+#pragma unroll
+  for (int q = 0; q < MN_ITEMS; ++q) {
+    const int j = blockIdx.x * MN_DRAWS + q * MN_BLOCK + threadIdx.x;
+    anc[q] = -1;
+    lo[q] = 0;
+    top[q] = 0;
+    u[q] = 0.0;
+    if (j < a.n) {
+      double U;
+      if (a.U != nullptr) {
+        U = a.U[j];
+      } else {
+        const u32x4 r = philox_block(a.seed, (uint32_t)j, step, TAG_MULTINOMIAL, a.batch, 0);
+        U = u53(r.x, r.y);
+      }
+      u[q] = W * U;
+      // chunks lying wholly at or below u (Ws is non-decreasing)
+      int f = 0, hi = nch;
+      while (f < hi) {
+        const int mid = f + ((hi - f) >> 1);
+        if (fence[mid] <= u[q]) f = mid + 1;
+        else hi = mid;
+      }
+      lo[q] = f * stride;
+      top[q] = f < nch ? (lo[q] + stride < a.P ? lo[q] + stride : a.P) : lo[q];
+      anc[q] = 0;  // valid draw
+    }
+  }
+  // the searches inside the chunks, in lockstep
+  for (bool more = true; more;) {
+    more = false;
+#pragma unroll
+    for (int q = 0; q < MN_ITEMS; ++q) {
+      if (lo[q] < top[q]) {
+        const int mid = lo[q] + ((top[q] - lo[q]) >> 1);
+        if (a.Ws[mid] <= u[q]) lo[q] = mid + 1;
+        else top[q] = mid;
+        more = more || lo[q] < top[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MN_ITEMS; ++q) {
+    if (anc[q] < 0) continue;
+    anc[q] = lo[q] < a.P ? lo[q] : a.P - 1;
+    if (a.as != nullptr) a.as[blockIdx.x * MN_DRAWS + q * MN_BLOCK + threadIdx.x] = anc[q];
+  }
+  if (a.os == nullptr) return;  // uniform: the launch either counts or it does not
+#pragma unroll
+  for (int q = 0; q < MN_ITEMS; ++q) {
+    if (anc[q] < 0) continue;
+    const int h = (int)(((uint32_t)anc[q] * 2654435761u) >> (32 - MN_HASH_BITS));
+    const int prev = atomicCAS(&hkey[h], -1, anc[q]);
+    if (prev == -1 || prev == anc[q]) atomicAdd(&hcnt[h], 1);
+    else atomicAdd(&a.os[anc[q]], 1);  // its slot holds another ancestor
+  }
+  __syncthreads();
+  for (int h = threadIdx.x; h < MN_HASH; h += MN_BLOCK)
+    if (hkey[h] >= 0) atomicAdd(&a.os[hkey[h]], hcnt[h]);
 }
 
 int launch_multinomial(const MultinomialLaunch& L, hipStream_t s) {
   if (L.n <= 0) return BIHIP_OK;
   MultinomialArgs a{L.Ws, L.P, L.n, L.U, L.seed, L.step, L.batch, L.gate, L.status, L.as, L.os};
-  hipLaunchKernelGGL(k_multinomial, dim3(ceil_div(L.n, RS_BLOCK)), dim3(RS_BLOCK), 0, s, a);
+  hipLaunchKernelGGL(k_multinomial, dim3(ceil_div(L.n, MN_DRAWS)), dim3(MN_BLOCK), 0, s, a);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }
