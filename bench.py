@@ -308,7 +308,7 @@ def bench_pz(args):
         cel = time.perf_counter() - t0
         cpu = {"value": PZ_P * PZ_T * reps / cel, "unit": "particle-steps/s",
                "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": f"PZ FP64 RK4 bootstrap PF N={PZ_P}, T={PZ_T}, {reps} runs ({cel:.1f} s)"}
     cfg = {"workload": "predator-prey bootstrap PF, N=4096 FP64, RK4 h=0.1, systematic "
                        "resampling every step, T=100 daily observations of log P",
@@ -411,7 +411,7 @@ def bench_smc2(args, generated=False):
         cel = time.perf_counter() - t0
         cpu = {"value": cps / cel, "unit": "particle-steps/s",
                "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(),
                "sample": f"the first {j} inner filters run one after another over T={T} "
                          f"({cel:.1f} s), OpenMP over particles"}
     cfg = {"workload": ("batched SMC^2 inner filters, predator-prey "
