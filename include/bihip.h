@@ -660,6 +660,52 @@ int bihip_guided_propose(bihip_ctx* ctx, bihip_mlp* mlp, const bihip_guided_norm
  * attached. */
 int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* norm);
 
+/* ================================================================ filter banks */
+/*
+ * B bootstrap particle filters of P particles each, one parameter vector
+ * theta_j per filter, advanced by shared launches: every launch of a step
+ * (the resampling chain, the transition or learned flow, the correct step)
+ * covers all B filters, filter j on row j of the grid's workgroups.  This is
+ * the inner loop of MarginalSIR::step (share/src/bi/sampler/MarginalSIR.hpp:
+ * 346-368), which the reference runs serially over theta, for inner filters
+ * larger than one workgroup (bihip_batch takes N_j <= 16384) and models of
+ * any lane width (Lorenz-96 d = 200).  Filter j equals bihip_pf with Philox
+ * batch id cfg->batch + j and bihip_model::par[i] = theta_host[j * ntheta + i]
+ * (i < ntheta), step for step: the same ancestors and the same log-weights.
+ * Built-in models; cfg->resampler systematic or stratified; cfg->record 0 or
+ * 1 (ancestors); cfg->use_graph replays a captured step.
+ */
+typedef struct bihip_pfbank bihip_pfbank;
+int bihip_pfbank_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config* cfg, int B,
+                        const double* theta_host, int ntheta, bihip_pfbank** bank);
+int bihip_pfbank_destroy(bihip_pfbank* bank);
+/* Observations shared by every filter, T x nobs row-major. */
+int bihip_pfbank_set_observations(bihip_pfbank* bank, const double* y_host);
+/* Initial particles of every filter (Lorenz-96: around mean0_host); lws = 0. */
+int bihip_pfbank_init(bihip_pfbank* bank, const double* mean0_host);
+/* n observation intervals of every filter (asynchronous). */
+int bihip_pfbank_step(bihip_pfbank* bank, int n);
+/* Host copies (any may be NULL): logLikelihood[B], logIncrements[B x T],
+ * ess[B x T], resampled[B x T], status[B]. */
+int bihip_pfbank_outputs(bihip_pfbank* bank, double* logLikelihood_host,
+                         double* logIncrements_host, double* ess_host, int* resampled_host,
+                         int* status_host);
+/* Ancestors of filter j at step k (record = 1): P ints. */
+int bihip_pfbank_ancestors(bihip_pfbank* bank, int j, int k, int* as_host);
+/* Device state of filter j: d x P of leading dimension ld, and its log-weights. */
+int bihip_pfbank_state(bihip_pfbank* bank, int j, void** X_dev, int* ld, double** lws_dev);
+/* Steps completed so far. */
+int bihip_pfbank_position(bihip_pfbank* bank);
+/* The learned surrogate flow (bihip_pf_set_surrogate) or guided proposal
+ * (bihip_pf_set_guided) for every filter, each with its own par[0] as the
+ * network's parameter input; the filters may differ only in par[0]. */
+int bihip_pfbank_set_surrogate(bihip_pfbank* bank, bihip_mlp* mlp,
+                               const bihip_surrogate_norm* norm);
+int bihip_pfbank_set_guided(bihip_pfbank* bank, bihip_mlp* mlp, const bihip_guided_norm* norm);
+/* Event timing of the bank's launches, as bihip_pf_set_timing / bihip_pf_kernel_time. */
+int bihip_pfbank_set_timing(bihip_pfbank* bank, int mode);
+int bihip_pfbank_kernel_time(bihip_pfbank* bank, int kernel, double* ms_host, long* count_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,7 @@ GEN = os.path.join(OBJ, "gen")
 SOURCES = ["transition.hip", "reduce.hip", "scan.hip", "resample.hip", "batch.hip", "capi.cpp",
            "batch_capi.cpp", "smc2.cpp",
            "mlp.hip", "mlp_capi.cpp", "surrogate.hip",
-           "guided.hip", "netcdf.cpp", "prior.cpp", "noalloc.hip", "ancestry.hip"]
+           "guided.hip", "netcdf.cpp", "prior.cpp", "noalloc.hip", "ancestry.hip", "bank.cpp"]
 ARCH = os.environ.get("BIHIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

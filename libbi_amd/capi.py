@@ -70,6 +70,10 @@ EXPORTS = [
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
     "bihip_model_generated", "bihip_model_var_name", "bihip_model_name", "bihip_model_var_size",
     "bihip_model_prior_sample", "bihip_model_prior_logdensity",
+    "bihip_pfbank_create", "bihip_pfbank_destroy", "bihip_pfbank_set_observations",
+    "bihip_pfbank_init", "bihip_pfbank_step", "bihip_pfbank_outputs", "bihip_pfbank_ancestors",
+    "bihip_pfbank_state", "bihip_pfbank_position", "bihip_pfbank_set_surrogate",
+    "bihip_pfbank_set_guided", "bihip_pfbank_set_timing", "bihip_pfbank_kernel_time",
 ]
 
 
@@ -265,6 +269,19 @@ def _declare(L):
         "bihip_pf_set_schedule": [P, pd, pi],
         "bihip_pf_set_inputs": [P, pd],
         "bihip_pf_impose_resampling": [P, pi, pi],
+        "bihip_pfbank_create": [P, pm, pc, i, pd, i, c.POINTER(P)],
+        "bihip_pfbank_destroy": [P],
+        "bihip_pfbank_set_observations": [P, pd],
+        "bihip_pfbank_init": [P, pd],
+        "bihip_pfbank_step": [P, i],
+        "bihip_pfbank_outputs": [P, pd, pd, pd, pi, pi],
+        "bihip_pfbank_ancestors": [P, i, i, pi],
+        "bihip_pfbank_state": [P, i, c.POINTER(P), pi, c.POINTER(P)],
+        "bihip_pfbank_position": [P],
+        "bihip_pfbank_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
+        "bihip_pfbank_set_guided": [P, P, c.POINTER(GuidedNorm)],
+        "bihip_pfbank_set_timing": [P, i],
+        "bihip_pfbank_kernel_time": [P, i, pd, pl],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -650,6 +667,110 @@ class ParticleFilter:
         ms = ctypes.c_double()
         n = ctypes.c_long()
         check(lib().bihip_pf_kernel_time(self.h, kind, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class FilterBank:
+    """bihip_pfbank: B bootstrap filters of P particles with parameters theta_j
+    (B x ntheta, column i replacing par[i]), advanced by shared launches;
+    filter j equals ParticleFilter(batch=batch + j) with those parameters."""
+
+    def __init__(self, ctx, model, P, T, dt, seed, theta, resampler=SYSTEMATIC, ess_rel=0.5,
+                 t0=0.0, batch=0, record=False, use_graph=False):
+        self.ctx, self.model = ctx, model
+        th = np.ascontiguousarray(theta, np.float64)
+        if th.ndim == 1:
+            th = th[:, None]
+        self.B, self.ntheta = th.shape
+        self.theta = th
+        self.cfg = PFConfig(P=P, T=T, resampler=resampler, essRel=ess_rel, t0=t0, dt=dt,
+                            seed=seed, batch=batch, record=1 if record else 0,
+                            use_graph=1 if use_graph else 0)
+        h = ctypes.c_void_p()
+        check(lib().bihip_pfbank_create(ctx.h, ctypes.byref(model), ctypes.byref(self.cfg),
+                                        self.B, th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        self.ntheta, ctypes.byref(h)))
+        self.h = h
+        ctx.adopt(self)
+        self.P, self.T, self.nobs = P, T, nobs(model)
+
+    def close(self):
+        if self.h:
+            lib().bihip_pfbank_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_observations(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(self.T, self.nobs)
+        check(lib().bihip_pfbank_set_observations(self.h, y.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def init(self, mean0=None):
+        m = None
+        if mean0 is not None:
+            self._mean0 = np.ascontiguousarray(mean0, dtype=np.float64)
+            m = self._mean0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(lib().bihip_pfbank_init(self.h, m))
+
+    def step(self, n=1):
+        check(lib().bihip_pfbank_step(self.h, n))
+
+    def position(self):
+        return lib().bihip_pfbank_position(self.h)
+
+    def outputs(self):
+        B, T = self.B, self.T
+        ll, li, ess = np.zeros(B), np.zeros((B, T)), np.zeros((B, T))
+        res, st = np.zeros((B, T), np.int32), np.zeros(B, np.int32)
+        D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        check(lib().bihip_pfbank_outputs(self.h, ll.ctypes.data_as(D), li.ctypes.data_as(D),
+                                         ess.ctypes.data_as(D), res.ctypes.data_as(I),
+                                         st.ctypes.data_as(I)))
+        return dict(logLikelihood=ll, logIncrements=li, ess=ess, resampled=res, status=st)
+
+    def ancestors(self, j, k):
+        a = np.zeros(self.P, np.int32)
+        check(lib().bihip_pfbank_ancestors(self.h, j, k,
+                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return a
+
+    def state(self, j):
+        """Device copies of filter j's state (d x ld torch tensor), ld and
+        log-weights (P doubles)."""
+        import torch
+        X, ld, lws = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_void_p()
+        check(lib().bihip_pfbank_state(self.h, j, ctypes.byref(X), ctypes.byref(ld),
+                                       ctypes.byref(lws)))
+        dt = torch.float32 if self.model.precision == F32 else torch.float64
+        Xt = torch.empty((self.model.d, ld.value), dtype=dt, device="cuda")
+        lw = torch.empty(self.P, dtype=torch.float64, device="cuda")
+        check(lib().bihip_memcpy_device(self.ctx.h, Xt.data_ptr(), X.value,
+                                        Xt.numel() * Xt.element_size()))
+        check(lib().bihip_memcpy_device(self.ctx.h, lw.data_ptr(), lws.value, 8 * self.P))
+        self.ctx.sync()
+        return Xt, ld.value, lw
+
+    def set_surrogate(self, mlp, norm):
+        check(lib().bihip_pfbank_set_surrogate(self.h, mlp.h if mlp is not None else None,
+                                               ctypes.byref(norm) if norm is not None else None))
+
+    def set_guided(self, mlp, norm):
+        check(lib().bihip_pfbank_set_guided(self.h, mlp.h if mlp is not None else None,
+                                            ctypes.byref(norm) if norm is not None else None))
+
+    def set_timing(self, on, transition_only=False):
+        mode = (TIMING_TRANSITION if transition_only else TIMING_ALL) if on else 0
+        check(lib().bihip_pfbank_set_timing(self.h, mode))
+
+    def kernel_time(self, kind):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        check(lib().bihip_pfbank_kernel_time(self.h, kind, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
 

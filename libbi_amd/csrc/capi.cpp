@@ -56,7 +56,7 @@ static void ws_free(ScanWorkspace* ws) {
   *ws = ScanWorkspace{};
 }
 
-static int check_model(const bihip_model* m) {
+int check_model(const bihip_model* m) {
   if (m == nullptr) return set_error(BIHIP_EINVAL, "model is NULL");
   if (!model_supported(m))
     return set_error(BIHIP_EINVAL,
@@ -83,7 +83,7 @@ int check_interval(const bihip_model* m, double t1, double t2) {
   return BIHIP_OK;
 }
 
-static int nobs_of(const bihip_model* m) {
+int nobs_of(const bihip_model* m) {
   if (const GenInfo* g = generated_info(m->model)) return g->nobs;
   switch (m->model) {
     case BIHIP_MODEL_L96: return m->d / 2;

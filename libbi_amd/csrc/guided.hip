@@ -48,6 +48,8 @@ struct GuidedArgs {
   // standalone mode: log-weight increment per particle
   double* inc;
   const int* imp_r;  // filter mode: imposed resampling flags per step (nullable)
+  BankStrides bk;    // filter mode: filter blockIdx.y of a bank at these strides
+  const float* par_u_bank;  // bank: the parameter input per filter (nullable)
 };
 
 namespace {
@@ -64,6 +66,15 @@ __global__ __launch_bounds__(GP_THREADS) void k_guided(GuidedArgs a) {
   constexpr int LDS_ELEMS = GH * LDSK_H > 128 * LDSK_0 ? GH * LDSK_H : 128 * LDSK_0;
   __shared__ __attribute__((aligned(16))) uint16_t Ws[LDS_ELEMS];
 
+  if constexpr (FILTER) {
+    const long f = blockIdx.y;
+    a.X = bank_shift(a.X, f * a.bk.x * (long)sizeof(float));
+    a.lws = bank_shift(a.lws, f * a.bk.p * (long)sizeof(double));
+    a.partials = bank_shift(a.partials, f * (long)gridDim.x * 3 * (long)sizeof(double));
+    a.dev = a.dev + f;
+    a.batch += (uint32_t)f;
+    if (a.par_u_bank != nullptr) a.par_u = a.par_u_bank[f];
+  }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5;
   const int p = blockIdx.x * GP_TILE + w * 32 + (lane & 31);
   const bool active = p < a.P;
@@ -209,7 +220,12 @@ int launch_guided(const GuidedLaunch& L, hipStream_t s) {
   a.partials = L.partials;
   a.inc = L.inc;
   a.imp_r = L.imp_r;
-  const dim3 grid(guided_blocks(L.P)), block(GP_THREADS);
+  a.bk = L.bk;
+  a.par_u_bank = L.par_u_bank;
+  const int banks = L.banks > 1 ? L.banks : 1;
+  if (banks > 1 && L.dev == nullptr)
+    return set_error(BIHIP_EINVAL, "a bank runs the guided proposal as a filter step");
+  const dim3 grid(guided_blocks(L.P), banks), block(GP_THREADS);
   if (L.dev != nullptr) {
     if (L.lws == nullptr || L.partials == nullptr || L.y_base == nullptr)
       return set_error(BIHIP_EINVAL, "the guided filter step needs the filter's state");

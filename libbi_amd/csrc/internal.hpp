@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/bihip.h"
 
 namespace bihip {
@@ -29,6 +31,8 @@ const GenInfo* generated_info(int model);
 constexpr int GEN_MAX_DELTAS = 1 << 16;
 // EINVAL unless [t1, t2] is an interval the model's kernels take in bounded time.
 int check_interval(const bihip_model* m, double t1, double t2);
+int check_model(const bihip_model* m);
+int nobs_of(const bihip_model* m);
 
 #define BIHIP_HIP(call)                                                          \
   do {                                                                           \
@@ -111,6 +115,31 @@ constexpr int RK4_MAX_STEPS = 1 << 20;
 __host__ __device__ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ---- transition.hip
+// Filters of a bank advanced by shared launches, filter f = blockIdx.y:
+// its state at f * x elements, its per-particle vectors (as, lws, Os, EF
+// words) at f * p elements, its resampling tile words at f * words bytes,
+// its scan state at f * state bytes, its PFDevice at dev + f, its partials
+// after the f gridDim.x triples of the filters before it, and its Philox
+// batch id batch + f.  A lone filter: one y-block and zero strides.
+struct BankStrides {
+  long x = 0, p = 0, words = 0, state = 0;
+};
+
+// ptr advanced by `bytes`, null kept null.
+template <class T>
+__host__ __device__ __forceinline__ T* bank_shift(T* ptr, long bytes) {
+  return ptr != nullptr ? reinterpret_cast<T*>(reinterpret_cast<
+                              typename std::conditional<std::is_const<T>::value, const char,
+                                                        char>::type*>(ptr) + bytes)
+                        : ptr;
+}
+
+// Parameter row of one filter of a bank: the model's 16 parameters, then the
+// observation constants that the host computes from them (1 / obs_sd and
+// -log(2 pi)/2 - log obs_sd), so that a bank filter computes exactly what a
+// lone filter with those parameters computes.
+constexpr int BANK_ROW = 18;
+
 struct TransLaunch {
   const bihip_model* model;
   const void* Xin;
@@ -146,6 +175,11 @@ struct TransLaunch {
   // with dev (nullable): imposed resampling flags, one per step; the step
   // gathers through `as` iff imp_r[k] != 0, in place of dev->resample
   const int* imp_r;
+  // a bank of `banks` filters (0 or 1: a lone filter) with parameter rows
+  // bank_par (BANK_ROW doubles per filter; with dev only)
+  int banks;
+  BankStrides bk;
+  const double* bank_par;
 };
 int transition_blocks(const bihip_model* m, int P);
 // A generated model whose steps [t0 + k dt, t0 + (k+1) dt], k < steps, are each
@@ -154,6 +188,13 @@ bool gen_one_delta(const bihip_model* m, double t0, double dt, int steps);
 int launch_transition(const TransLaunch& L, hipStream_t s);
 int launch_init_samples(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X,
                         int ld, const double* mean0_dev, hipStream_t s);
+// A bank's initial samples: filter f with parameter row bank_par + BANK_ROW f,
+// batch id batch + f, state at X + f * x elements.
+int launch_init_bank(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X, int ld,
+                     const double* mean0_dev, int banks, long x, const double* bank_par,
+                     hipStream_t s);
+// The parameter row of a model (its par and observation constants).
+void bank_row(const bihip_model* m, double* row);
 int launch_obs_logdensity(const bihip_model* m, const double* y, int P, const void* X, int ld,
                           double* lws, hipStream_t s);
 int launch_obs_sample(const bihip_model* m, uint64_t seed, uint32_t batch, uint32_t step, int P,
@@ -182,6 +223,10 @@ int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t 
 int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWork w,
                     hipStream_t s);
 // Combine partials into the filter scalars and histories.
+// The correct step of each filter of a bank: filter f's nb partials at
+// partials + 3 nb f, its PFDevice at dev + f, its histories at f * T.
+int launch_correct_bank(const double* partials, int nb, int P, double essRel, PFDevice* dev,
+                        PFHistory hist, int T, int banks, hipStream_t s);
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
                                PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
                                const int* mask = nullptr);
@@ -231,6 +276,8 @@ struct ResampleTilesLaunch {
   int* as;
   uint64_t* EF;             // P words of scratch
   ScanWorkspace ws;
+  int banks;                // 0 or 1: a lone filter; otherwise gate, status and mx of
+  BankStrides bk;           // filter f at + f PFDevice
 };
 int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s);
 struct MultinomialLaunch {
@@ -253,8 +300,9 @@ int launch_record_state(const void* X, int ld, int D, int P, int elem, const PFD
                         void* hist, int slot0, hipStream_t s);
 int launch_paths(const int* anc, const void* hist, int ld, int D, int P, int K, int elem,
                  const int* idx, int n, double* out, hipStream_t s);
+// banks > 1: filter f's ancestors at as + f bp, its history at out + f hs.
 int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
-                            hipStream_t s);
+                            hipStream_t s, int banks = 1, long bp = 0, long hs = 0);
 
 // ---- batch.hip
 struct BatchLaunch {
@@ -374,6 +422,9 @@ struct SurrogateLaunch {
   float noise_sd;
   double obs_sd;
   const int* imp_r;                  // step: imposed resampling flags (nullable)
+  int banks;                         // step: a bank of filters (0 or 1: a lone filter)
+  BankStrides bk;
+  const float* par_u_bank;           // bank: the normalised parameter input per filter
 };
 int surrogate_blocks(int P);
 int launch_surrogate(const SurrogateLaunch& L, hipStream_t s);
@@ -399,6 +450,9 @@ struct GuidedLaunch {
   double* partials;
   double* inc;        // standalone: log-weight increments
   const int* imp_r;   // filter mode: imposed resampling flags (nullable)
+  int banks;          // filter mode: a bank of filters (0 or 1: a lone filter)
+  BankStrides bk;
+  const float* par_u_bank;  // bank: the normalised parameter input per filter
 };
 int guided_blocks(int P);
 int launch_guided(const GuidedLaunch& L, hipStream_t s);

@@ -65,6 +65,44 @@ __device__ __forceinline__ void combine(int lo, int hi, Load load, double* lds, 
   S2 = block_sum<COMB_BLOCK>(s2, lds);
 }
 
+// BootstrapPF::correct's bookkeeping (share/src/bi/filter/BootstrapPF.hpp:171-181)
+// from the totals (M, S1, S2) of the step's log-weights, with Resampler::
+// resample's decision (Resampler.hpp:168) for the next time; with out, the
+// totals {max, sum w, sum w^2, ess, lW}.  One thread.
+__device__ __forceinline__ void pf_bookkeeping(double M, double S1, double S2, int P,
+                                               double essRel, PFDevice* dev, PFHistory hist,
+                                               double* out, const int* mask) {
+  const double ess = S1 * S1 / S2;
+  const double lW = M + log(S1) - log((double)P);
+  if (out != nullptr) {
+    out[0] = M;
+    out[1] = S1;
+    out[2] = S2;
+    out[3] = ess;
+    out[4] = lW;
+  }
+  if (dev != nullptr) {
+    const int k = dev->k;
+    if (hist.resampled) hist.resampled[k] = dev->resample != 0 && dev->status == BIHIP_OK;
+    if (mask == nullptr || mask[k] != 0) {
+      if (hist.logIncrements) hist.logIncrements[k] = lW - dev->logLikelihood;
+      if (hist.ess) hist.ess[k] = ess;
+      dev->logLikelihood = lW;
+      dev->ess = ess;
+      dev->mx = M;
+      dev->lW = lW;
+      dev->resample = ess < essRel * (double)P ? 1 : 0;
+    } else {
+      // no observation: nothing to correct, and Resampler::resample acts only
+      // at observed elements (share/src/bi/resampler/Resampler.hpp:168)
+      if (hist.logIncrements) hist.logIncrements[k] = 0.0;
+      if (hist.ess) hist.ess[k] = dev->ess;
+      dev->resample = 0;
+    }
+    dev->k = k + 1;
+  }
+}
+
 // Two-level combination of nb partials: block b combines partials
 // [b*COMB_BLOCK, (b+1)*COMB_BLOCK) into a level-2 triple, published as
 // epoch-tagged words; the block that takes the last ticket combines the
@@ -117,35 +155,43 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_reduce(const double* partials, i
   if (threadIdx.x != 0) return;
   __hip_atomic_store(&w.state->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(&w.state->epoch, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const double ess = S1 * S1 / S2;
-  const double lW = M + log(S1) - log((double)P);
-  if (out != nullptr) {
-    out[0] = M;
-    out[1] = S1;
-    out[2] = S2;
-    out[3] = ess;
-    out[4] = lW;
-  }
-  if (dev != nullptr) {
-    const int k = dev->k;
-    if (hist.resampled) hist.resampled[k] = dev->resample != 0 && dev->status == BIHIP_OK;
-    if (mask == nullptr || mask[k] != 0) {
-      if (hist.logIncrements) hist.logIncrements[k] = lW - dev->logLikelihood;
-      if (hist.ess) hist.ess[k] = ess;
-      dev->logLikelihood = lW;
-      dev->ess = ess;
-      dev->mx = M;
-      dev->lW = lW;
-      dev->resample = ess < essRel * (double)P ? 1 : 0;
-    } else {
-      // no observation: nothing to correct, and Resampler::resample acts only
-      // at observed elements (share/src/bi/resampler/Resampler.hpp:168)
-      if (hist.logIncrements) hist.logIncrements[k] = 0.0;
-      if (hist.ess) hist.ess[k] = dev->ess;
-      dev->resample = 0;
+  pf_bookkeeping(M, S1, S2, P, essRel, dev, hist, out, mask);
+}
+
+// One workgroup per filter of a bank (filter blockIdx.x): the filter's nb
+// partials (after the nb of each filter before it), thread t taking partials
+// t, t + BANK_RED, ... in a fixed order -- maximum first, then the rescaled
+// sums -- and the filter's bookkeeping, with its PFDevice at dev + f and its
+// histories at f * T.
+constexpr int BANK_RED = 1024;
+
+__global__ __launch_bounds__(BANK_RED) void k_correct_bank(const double* partials, int nb, int P,
+                                                           double essRel, PFDevice* dev,
+                                                           PFHistory hist, int T) {
+  __shared__ double lds[BANK_RED / WAVE];
+  const long f = blockIdx.x;
+  const double* q = partials + 3 * f * nb;
+  double m = __builtin_nan("");
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nb; i += BANK_RED) m = nan_max(m, q[3 * i]);
+  const double M = block_max<BANK_RED>(m, lds);
+  double s1 = 0.0, s2 = 0.0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nb; i += BANK_RED) {
+    const double a = q[3 * i], b = q[3 * i + 1], c = q[3 * i + 2];
+    if (b != 0.0) {
+      const double e = exp(a - M);
+      s1 += b * e;
+      s2 += c * (e * e);
     }
-    dev->k = k + 1;
   }
+  const double S1 = block_sum<BANK_RED>(s1, lds);
+  const double S2 = block_sum<BANK_RED>(s2, lds);
+  if (threadIdx.x != 0) return;
+  PFHistory h{hist.logIncrements ? hist.logIncrements + f * T : nullptr,
+              hist.ess ? hist.ess + f * T : nullptr,
+              hist.resampled ? hist.resampled + f * T : nullptr};
+  pf_bookkeeping(M, S1, S2, P, essRel, dev + f, h, nullptr, nullptr);
 }
 
 int reduce_blocks(int nb) { return ceil_div(nb, COMB_BLOCK); }
@@ -170,6 +216,14 @@ int launch_pf_correct_finalize(const double* partials, int nb, int P, double ess
                                const int* mask) {
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
                      essRel, dev, hist, (double*)nullptr, w, mask);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+int launch_correct_bank(const double* partials, int nb, int P, double essRel, PFDevice* dev,
+                        PFHistory hist, int T, int banks, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct_bank, dim3(banks), dim3(BANK_RED), 0, s, partials, nb, P, essRel,
+                     dev, hist, T);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

@@ -213,10 +213,20 @@ __device__ __forceinline__ void tile_prefix_combine(TilePrefix<V>& r, int nt, V*
   }
 }
 
+// The chain's kernels run filter blockIdx.y of a bank at the strides bk (a
+// lone filter: y = 0).
+__device__ __forceinline__ long bank_f() { return (long)blockIdx.y; }
+constexpr long DEV_BYTES = (long)sizeof(PFDevice);
+
 __global__ __launch_bounds__(SCAN_BLOCK) void k_weight_sums(const double* lws, const double* mx,
                                                             int P, u128* wsum,
-                                                            const PFDevice* gate) {
+                                                            const PFDevice* gate, BankStrides bk) {
   __shared__ u128 lds[SCAN_BLOCK / WAVE];
+  const long f = bank_f();
+  lws = bank_shift(lws, f * bk.p * 8);
+  mx = bank_shift(mx, f * DEV_BYTES);
+  wsum = bank_shift(wsum, f * bk.words);
+  gate = bank_shift(gate, f * DEV_BYTES);
   // the loads are issued before the gate check (all in bounds) to overlap
   // their latency with the gate's
   const long t0 = (long)blockIdx.x * SCAN_TILE;
@@ -238,8 +248,13 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_weight_sums(const double* lws, c
 // In-place exclusive scan of nt tile sums by one block; total receives the sum.
 template <class V>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_tiles_exclusive(V* sums, int nt, V* total,
-                                                                const PFDevice* gate) {
+                                                                const PFDevice* gate,
+                                                                BankStrides bk) {
   __shared__ V lds[SCAN_BLOCK / WAVE];
+  const long f = bank_f();
+  sums = bank_shift(sums, f * bk.words);
+  total = bank_shift(total, f * bk.words);
+  gate = bank_shift(gate, f * DEV_BYTES);
   if (!gate_open(gate)) return;
   V carry(0);
   for (long c0 = 0; c0 < nt; c0 += SCAN_TILE) {
@@ -276,6 +291,7 @@ struct OffspringTileArgs {
   PFDevice* status;
   double* Ws;
   int* Os;
+  BankStrides bk;
 };
 
 // MODE: BIHIP_SYSTEMATIC or BIHIP_STRATIFIED (cumulative offspring, as
@@ -285,6 +301,19 @@ template <int MODE>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_offspring_tiles(OffspringTileArgs a) {
   __shared__ u128 lds[SCAN_BLOCK / WAVE];
   __shared__ uint64_t lds64[SCAN_BLOCK / WAVE];
+  {
+    const long f = bank_f();
+    a.lws = bank_shift(a.lws, f * a.bk.p * 8);
+    a.mx = bank_shift(a.mx, f * DEV_BYTES);
+    a.wsum = bank_shift(a.wsum, f * a.bk.words);
+    a.wtotal = bank_shift(a.wtotal, f * a.bk.words);
+    a.efsum = bank_shift(a.efsum, f * a.bk.words);
+    a.batch += (uint32_t)f;
+    a.gate = bank_shift(a.gate, f * DEV_BYTES);
+    a.status = bank_shift(a.status, f * DEV_BYTES);
+    a.Ws = bank_shift(a.Ws, f * a.bk.p * 8);
+    a.Os = bank_shift(a.Os, f * a.bk.p * 4);
+  }
   const int tile = blockIdx.x;
   const long base = (long)tile * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
   // every load is issued before the gate check (all in bounds)
@@ -373,6 +402,7 @@ struct PlaceTileArgs {
   int* as;
   uint32_t* free_slots;
   const PFDevice* gate;
+  BankStrides bk;
 };
 
 constexpr uint32_t OCCUPIED = 0xFFFFFFFFu;
@@ -382,6 +412,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
   __shared__ int xi[xsize<int>()];       // offspring input, then E
   __shared__ uint32_t xf[xsize<int>()];  // free-slot rank, or OCCUPIED
   __shared__ int s_before;
+  {
+    const long f = bank_f();
+    a.in = bank_shift(a.in, f * a.bk.p * 4);
+    a.efsum = bank_shift(a.efsum, f * a.bk.words);
+    a.eftotal = bank_shift(a.eftotal, f * a.bk.words);
+    a.E = bank_shift(a.E, f * a.bk.p * 8);      // E and the slots share the filter's EF words
+    a.slots = bank_shift(a.slots, f * a.bk.p * 8);
+    a.as = bank_shift(a.as, f * a.bk.p * 4);
+    a.free_slots = bank_shift(a.free_slots, f * a.bk.state);
+    a.gate = bank_shift(a.gate, f * DEV_BYTES);
+  }
   const int tile = blockIdx.x;
   const long t0 = (long)tile * SCAN_TILE;
   // every load is issued before the gate check (all in bounds)
@@ -449,7 +490,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_place_tiles(PlaceTileArgs a) {
 // a binary search inside the interval.
 __global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, const uint32_t* slots,
                                                          const uint32_t* free_slots, int P,
-                                                         int* as, const PFDevice* gate) {
+                                                         int* as, const PFDevice* gate,
+                                                         BankStrides bk) {
+  {
+    const long f = bank_f();
+    E = bank_shift(E, f * bk.p * 8);
+    slots = bank_shift(slots, f * bk.p * 8);
+    free_slots = bank_shift(free_slots, f * bk.state);
+    as = bank_shift(as, f * bk.p * 4);
+    gate = bank_shift(gate, f * DEV_BYTES);
+  }
   const uint32_t r = blockIdx.x * RS_BLOCK + threadIdx.x;
   // slots[r] is loaded with the gate and the count (r < P: in bounds; used
   // only for r < Z)
@@ -498,20 +548,22 @@ __global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, cons
 }
 
 // (e, f) tile sums in place -> E, the free-slot list and the ancestors.
+// banks filters at the strides bk (1 and zero strides: a lone filter).
 static int place_from_sums(const int* in, bool cumulative, int P, int nt, int* as, uint64_t* EF,
-                           const TileWork& w, const PFDevice* gate, hipStream_t s) {
+                           const TileWork& w, const PFDevice* gate, hipStream_t s, int banks = 1,
+                           BankStrides bk = {}) {
   if (nt > SMALL_TILES) {
-    hipLaunchKernelGGL(k_tiles_exclusive<uint64_t>, dim3(1), dim3(SCAN_BLOCK), 0, s, w.efsum, nt,
-                       w.eftotal, gate);
+    hipLaunchKernelGGL(k_tiles_exclusive<uint64_t>, dim3(1, banks), dim3(SCAN_BLOCK), 0, s,
+                       w.efsum, nt, w.eftotal, gate, bk);
     BIHIP_LAUNCH_CHECK();
   }
   uint32_t* E = reinterpret_cast<uint32_t*>(EF);  // P words of E, then P of slots
   PlaceTileArgs a{in, cumulative ? 1 : 0, P, nt, w.efsum, w.eftotal, E, E + P, as,
-                  w.free_slots, gate};
-  hipLaunchKernelGGL(k_place_tiles, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+                  w.free_slots, gate, bk};
+  hipLaunchKernelGGL(k_place_tiles, dim3(nt, banks), dim3(SCAN_BLOCK), 0, s, a);
   BIHIP_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_place_free, dim3(ceil_div(P, RS_BLOCK)), dim3(RS_BLOCK), 0, s, E, E + P,
-                     w.free_slots, P, as, gate);
+  hipLaunchKernelGGL(k_place_free, dim3(ceil_div(P, RS_BLOCK), banks), dim3(RS_BLOCK), 0, s, E,
+                     E + P, w.free_slots, P, as, gate, bk);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }
@@ -534,22 +586,27 @@ int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
   const int nt = ceil_div(L.P, SCAN_TILE);
   if (nt > L.ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
   const TileWork w = tile_work(L.ws);
-  hipLaunchKernelGGL(k_weight_sums, dim3(nt), dim3(SCAN_BLOCK), 0, s, L.lws, L.mx, L.P, w.wsum,
-                     L.gate);
+  const int banks = L.banks > 1 ? L.banks : 1;
+  if (banks > 1 && L.mode == BIHIP_MULTINOMIAL)
+    return set_error(BIHIP_EINVAL, "filter banks resample systematically or stratified");
+  hipLaunchKernelGGL(k_weight_sums, dim3(nt, banks), dim3(SCAN_BLOCK), 0, s, L.lws, L.mx, L.P,
+                     w.wsum, L.gate, L.bk);
   BIHIP_LAUNCH_CHECK();
   if (nt > SMALL_TILES) {
-    hipLaunchKernelGGL(k_tiles_exclusive<u128>, dim3(1), dim3(SCAN_BLOCK), 0, s, w.wsum, nt,
-                       w.wtotal, L.gate);
+    hipLaunchKernelGGL(k_tiles_exclusive<u128>, dim3(1, banks), dim3(SCAN_BLOCK), 0, s, w.wsum,
+                       nt, w.wtotal, L.gate, L.bk);
     BIHIP_LAUNCH_CHECK();
   }
   OffspringTileArgs a{L.lws, L.mx, L.P, L.n, nt, w.wsum, w.wtotal, w.efsum, L.seed, L.batch,
-                      L.gate, L.status, L.Ws, L.Os};
+                      L.gate, L.status, L.Ws, L.Os, L.bk};
   switch (L.mode) {
     case BIHIP_SYSTEMATIC:
-      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_SYSTEMATIC>, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_SYSTEMATIC>, dim3(nt, banks), dim3(SCAN_BLOCK), 0,
+                         s, a);
       break;
     case BIHIP_STRATIFIED:
-      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_STRATIFIED>, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+      hipLaunchKernelGGL(k_offspring_tiles<BIHIP_STRATIFIED>, dim3(nt, banks), dim3(SCAN_BLOCK), 0,
+                         s, a);
       break;
     case BIHIP_MULTINOMIAL:
       hipLaunchKernelGGL(k_offspring_tiles<BIHIP_MULTINOMIAL>, dim3(nt), dim3(SCAN_BLOCK), 0, s,
@@ -560,7 +617,7 @@ int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
   }
   BIHIP_LAUNCH_CHECK();
   if (L.mode != BIHIP_MULTINOMIAL)
-    return place_from_sums(L.Os, true, L.P, nt, L.as, L.EF, w, L.gate, s);
+    return place_from_sums(L.Os, true, L.P, nt, L.as, L.EF, w, L.gate, s, banks, L.bk);
   int rc;
   if ((rc = launch_zero_ints(L.os, L.P, L.gate, s))) return rc;
   MultinomialLaunch M{};
@@ -781,18 +838,25 @@ int launch_gather_rows(const int* as, int P, int ncols, void* X, int ld, int ele
   return BIHIP_OK;
 }
 
-// Ancestry output of the step: as (resampled) or the identity.
-__global__ void k_record_ancestors(const int* as, int P, const PFDevice* dev, int* out) {
+// Ancestry output of the step: as (resampled) or the identity.  Filter
+// blockIdx.y of a bank: its ancestors at y * bp, PFDevice dev + y, history
+// at out + y * hs.
+__global__ void k_record_ancestors(const int* as, int P, const PFDevice* dev, int* out, long bp,
+                                   long hs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
+  const long f = blockIdx.y;
+  as += f * bp;
+  dev += f;
+  out += f * hs;
   const bool r = dev->resample != 0 && dev->status == BIHIP_OK;
   out[(size_t)dev->k * P + i] = r ? as[i] : i;
 }
 
 int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_record_ancestors, dim3(ceil_div(P, 256)), dim3(256), 0, s, as, P, gate,
-                     out);
+                            hipStream_t s, int banks, long bp, long hs) {
+  hipLaunchKernelGGL(k_record_ancestors, dim3(ceil_div(P, 256), banks > 1 ? banks : 1), dim3(256),
+                     0, s, as, P, gate, out, bp, hs);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

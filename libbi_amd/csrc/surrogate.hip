@@ -45,6 +45,8 @@ struct SurrArgs {
   float noise_sd;
   double obs_sd;
   const int* imp_r;  // imposed resampling flags per step (nullable)
+  BankStrides bk;    // STEP: filter blockIdx.y of a bank at these strides
+  const float* par_u_bank;  // bank: the parameter input per filter (nullable)
 };
 
 namespace {
@@ -59,6 +61,17 @@ __global__ __launch_bounds__(SG_THREADS) void k_surrogate(SurrArgs a) {
   static_assert(NL >= 2 && H % 32 == 0 && K0 <= H && DO <= H, "surrogate shape");
   __shared__ __attribute__((aligned(16))) uint16_t Ws[H * LDSK];
 
+  if constexpr (STEP) {
+    const long f = blockIdx.y;
+    a.Xin = bank_shift(a.Xin, f * a.bk.x * (long)sizeof(float));
+    a.Xout = bank_shift(a.Xout, f * a.bk.x * (long)sizeof(float));
+    a.as = bank_shift(a.as, f * a.bk.p * (long)sizeof(int));
+    a.lws = bank_shift(a.lws, f * a.bk.p * (long)sizeof(double));
+    a.partials = bank_shift(a.partials, f * (long)gridDim.x * 3 * (long)sizeof(double));
+    a.dev = a.dev + f;
+    a.batch += (uint32_t)f;
+    if (a.par_u_bank != nullptr) a.par_u = a.par_u_bank[f];
+  }
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, h = lane >> 5;
   const int p = blockIdx.x * SG_TILE + w * 32 + (lane & 31);
   const bool active = p < a.P;
@@ -207,7 +220,12 @@ int launch_surrogate(const SurrogateLaunch& L, hipStream_t s) {
   a.noise_sd = L.noise_sd;
   a.obs_sd = L.obs_sd;
   a.imp_r = L.imp_r;
-  const dim3 grid(surrogate_blocks(L.P)), block(SG_THREADS);
+  a.bk = L.bk;
+  a.par_u_bank = L.par_u_bank;
+  const int banks = L.banks > 1 ? L.banks : 1;
+  if (banks > 1 && !L.step)
+    return set_error(BIHIP_EINVAL, "a bank runs the surrogate as a filter step");
+  const dim3 grid(surrogate_blocks(L.P), banks), block(SG_THREADS);
   if (L.step) {
     if (L.dev == nullptr || L.lws == nullptr || L.partials == nullptr || L.y_base == nullptr)
       return set_error(BIHIP_EINVAL, "the surrogate filter step needs the filter's state");

@@ -41,6 +41,8 @@ struct TransArgs {
   const int* mask;      // with dev: observed flags (nullable: every step observed)
   const double* inputs; // with dev: 16 model parameters per step (nullable)
   const int* imp_r;     // with dev: imposed resampling flags per step (nullable)
+  BankStrides bk;       // BANK: strides of the filters (filter f = blockIdx.y)
+  const double* bank_par;  // BANK: BANK_ROW parameters per filter
 };
 
 // ---------------------------------------------------------------- the kernel
@@ -66,11 +68,29 @@ constexpr int trans_min_waves() {
 
 // ONE: a generated model's interval is one sub-interval of its delta that
 // starts at a multiple of it (its transition runs without the sub-interval loop).
-template <class M, int INTEG, bool SCHED = false, bool ONE = false>
+// BANK: filter blockIdx.y of a bank, with its own state, vectors, PFDevice,
+// Philox batch id and parameter row (then the step is exactly a lone
+// filter's with those parameters and that batch id).
+template <class M, int INTEG, bool SCHED = false, bool ONE = false, bool BANK = false>
 __global__ __launch_bounds__(TRANS_BLOCK) __attribute__((amdgpu_waves_per_eu(trans_min_waves<M>())))
 void k_transition(TransArgs a) {
   using T = typename M::real_t;
   constexpr int LPP = M::LPP, DL = M::DL;
+  if constexpr (BANK) {
+    const long f = blockIdx.y;
+    a.Xin = bank_shift(a.Xin, f * a.bk.x * (long)sizeof(T));
+    a.Xout = bank_shift(a.Xout, f * a.bk.x * (long)sizeof(T));
+    a.as = bank_shift(a.as, f * a.bk.p * (long)sizeof(int));
+    a.lws = bank_shift(a.lws, f * a.bk.p * (long)sizeof(double));
+    a.partials = bank_shift(a.partials, f * (long)gridDim.x * 3 * (long)sizeof(double));
+    a.dev = a.dev + f;
+    a.batch += (uint32_t)f;
+    const double* row = a.bank_par + BANK_ROW * f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a.m.par[i] = uniform(row[i]);
+    a.m.obs_inv = uniform(row[16]);
+    a.m.obs_c = uniform(row[17]);
+  }
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int tid = blk * TRANS_BLOCK + threadIdx.x;
   const int p = tid / LPP;
@@ -166,12 +186,24 @@ void k_transition(TransArgs a) {
 }
 
 // ---------------------------------------------------------------- initial states
+// bank_par (nullable): filter blockIdx.y of a bank, parameter row
+// bank_par + BANK_ROW y, batch id batch + y, state at Xv + y * bx elements.
 template <class M>
 __global__ __launch_bounds__(TRANS_BLOCK) void k_init(ModelArgs ma, uint64_t seed,
                                                       uint32_t batch, int P, void* Xv, int ld,
-                                                      const double* mean0) {
+                                                      const double* mean0,
+                                                      const double* bank_par, long bx) {
   using T = typename M::real_t;
   constexpr int LPP = M::LPP, DL = M::DL;
+  if (bank_par != nullptr) {
+    const long f = blockIdx.y;
+    const double* row = bank_par + BANK_ROW * f;
+    for (int i = 0; i < 16; ++i) ma.par[i] = row[i];
+    ma.obs_inv = row[16];
+    ma.obs_c = row[17];
+    batch += (uint32_t)f;
+    Xv = bank_shift(Xv, f * bx * (long)sizeof(T));
+  }
   const int tid = blockIdx.x * TRANS_BLOCK + threadIdx.x;
   const int p = tid / LPP;
   const Group<LPP> grp;
@@ -356,10 +388,29 @@ struct TransitionRunner {
   hipStream_t s;
   int integ;
   bool one;
+  int banks;
   template <class M>
   int run() {
     const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
                                             a.inputs != nullptr);
+    if (banks > 1) {
+      if constexpr (is_generated<M>::value) {
+        return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
+      } else {
+        if (sched || a.dev == nullptr || a.bank_par == nullptr || a.imp_r != nullptr)
+          return set_error(BIHIP_EINVAL, "a filter bank step takes no schedule, inputs or "
+                                         "imposed history");
+        const dim3 grid(blocks, banks);
+        if (integ == BIHIP_RK4)
+          hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, false, false, true>), grid,
+                             dim3(TRANS_BLOCK), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5, false, false, true>), grid,
+                             dim3(TRANS_BLOCK), 0, s, a);
+        BIHIP_LAUNCH_CHECK();
+        return BIHIP_OK;
+      }
+    }
     if constexpr (is_generated<M>::value) {
       if (one && !sched) {
         if (integ == BIHIP_RK4)
@@ -420,6 +471,9 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.flow_only = L.flow_only;
   r.a.aux = L.aux;
   r.a.imp_r = L.imp_r;
+  r.a.bk = L.bk;
+  r.a.bank_par = L.bank_par;
+  r.banks = L.banks;
   r.blocks = transition_blocks(L.model, L.P);
   r.s = s;
   r.integ = L.model->integrator;
@@ -438,10 +492,13 @@ struct InitRunner {
   void* X;
   const double* mean0;
   hipStream_t s;
+  int banks;
+  const double* bank_par;
+  long bx;
   template <class M>
   int run() {
-    hipLaunchKernelGGL((k_init<M>), dim3(blocks), dim3(TRANS_BLOCK), 0, s, ma, seed, batch, P, X,
-                       ld, mean0);
+    hipLaunchKernelGGL((k_init<M>), dim3(blocks, banks), dim3(TRANS_BLOCK), 0, s, ma, seed, batch,
+                       P, X, ld, mean0, bank_par, bx);
     BIHIP_LAUNCH_CHECK();
     return BIHIP_OK;
   }
@@ -450,8 +507,25 @@ struct InitRunner {
 int launch_init_samples(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X,
                         int ld, const double* mean0_dev, hipStream_t s) {
   if (P <= 0) return BIHIP_OK;
-  InitRunner r{to_args(m), seed, batch, P, ld, transition_blocks(m, P), X, mean0_dev, s};
+  InitRunner r{to_args(m), seed, batch, P, ld, transition_blocks(m, P), X, mean0_dev, s,
+               1, nullptr, 0};
   return with_model(m, r);
+}
+
+int launch_init_bank(const bihip_model* m, uint64_t seed, uint32_t batch, int P, void* X, int ld,
+                     const double* mean0_dev, int banks, long x, const double* bank_par,
+                     hipStream_t s) {
+  if (P <= 0 || banks <= 0) return BIHIP_OK;
+  InitRunner r{to_args(m), seed, batch, P, ld, transition_blocks(m, P), X, mean0_dev, s,
+               banks, bank_par, x};
+  return with_model(m, r);
+}
+
+void bank_row(const bihip_model* m, double* row) {
+  const ModelArgs a = to_args(m);
+  for (int i = 0; i < 16; ++i) row[i] = a.par[i];
+  row[16] = a.obs_inv;
+  row[17] = a.obs_c;
 }
 
 struct ObsRunner {

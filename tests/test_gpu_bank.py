@@ -1,0 +1,228 @@
+"""Filter banks (bihip_pfbank_*): B filters of P particles advanced by shared
+launches.  Filter j of a bank equals bihip_pf run alone with parameters
+theta_j and Philox batch id j -- the same ancestors at every step, bit for
+bit, the same states, and log-likelihood increments equal to rounding (the
+bank's correct step sums the same partials in another order) -- and equals
+the oracle's filter with theta_j and batch id j."""
+import numpy as np
+import pytest
+
+from libbi_amd import capi, data, guided
+from gpu_helpers import oracle_model
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.05
+
+
+def _pz_data(orc, T, seed=100):
+    m = orc.make_model("pz")
+    x0 = orc.init_samples(m, seed, 1)[:, 0]
+    _, y = orc.simulate(m, seed, T, 1.0, x0)
+    return y
+
+
+def _pz_thetas(B, seed=3):
+    rng = np.random.default_rng(seed)
+    return np.stack([np.exp(rng.normal(np.log(0.5), 0.2, B)),
+                     np.exp(rng.normal(np.log(0.2), 0.2, B)),
+                     np.exp(rng.normal(np.log(0.25), 0.1, B)),
+                     np.exp(rng.normal(np.log(0.3), 0.1, B))], axis=1)
+
+
+def _lone(ctx, gm, theta, P, T, dt, seed, j, resampler, ess_rel, y, mean0=None):
+    m = capi.Model.from_buffer_copy(gm)
+    for i, v in enumerate(theta):
+        m.par[i] = v
+    pf = capi.ParticleFilter(ctx, m, P, T, dt, seed, resampler=resampler, ess_rel=ess_rel,
+                             batch=j, record=True)
+    pf.set_observations(y)
+    pf.init(mean0)
+    return pf
+
+
+@pytest.mark.parametrize("resampler", [capi.SYSTEMATIC, capi.STRATIFIED])
+def test_pz_bank_equals_lone_filters_and_oracle(ctx, orc, resampler):
+    B, P, T = 6, 3000, 25  # P not a multiple of the tile or of 64
+    y = _pz_data(orc, T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    theta = _pz_thetas(B)
+    bank = capi.FilterBank(ctx, gm, P, T, 1.0, 42, theta, resampler=resampler, ess_rel=0.7,
+                           record=True)
+    bank.set_observations(y)
+    bank.init()
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    assert out["resampled"].sum() > 20
+    for j in range(B):
+        pf = _lone(ctx, gm, theta[j], P, T, 1.0, 42, j, resampler, 0.7, y)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-13,
+                                   atol=1e-12)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k), f"{j} {k}")
+        Xb, ld, lwb = bank.state(j)
+        Xp, _, lwp = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy())
+        assert np.array_equal(lwb.cpu().numpy(), lwp.cpu().numpy())
+        pf.close()
+        if j in (0, B - 1):
+            om = oracle_model(orc, gm)
+            for i, v in enumerate(theta[j]):
+                om.par[i] = v
+            ref = orc.pf_run(om, y, P, 1.0, 42, resampler=resampler, ess_rel=0.7, batch=j,
+                             want_ancestors=True)
+            np.testing.assert_allclose(out["logIncrements"][j], ref["logIncrements"], rtol=1e-9,
+                                       atol=1e-9)
+            np.testing.assert_array_equal(out["resampled"][j], ref["resampled"])
+            for k in range(T):
+                np.testing.assert_array_equal(bank.ancestors(j, k), ref["ancestors"][k])
+    bank.close()
+
+
+def test_l96_d200_fp32_bank_equals_lone_filters(ctx, torch_gpu):
+    """Eight-lane Lorenz-96 d=200 in FP32, one forcing per filter (the
+    configuration of 64 forcings x 2^18 particles, here 4 x 2^14)."""
+    B, P, T = 4, 1 << 14, 12
+    Fs = np.array([7.7, 7.9, 8.1, 8.3])
+    gm = guided.model(8.0)
+    mean0 = data.l96_spinup(ctx, d=gm.d, F=8.0)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=31)
+    bank = capi.FilterBank(ctx, gm, P, T, DT, 5, Fs, ess_rel=0.5, record=True, batch=100)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    assert out["resampled"].sum() >= 4
+    for j in range(B):
+        pf = _lone(ctx, gm, [Fs[j]], P, T, DT, 5, 100 + j, capi.SYSTEMATIC, 0.5, y, mean0)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-13)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k), f"{j} {k}")
+        Xb, _, _ = bank.state(j)
+        Xp, _, _ = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy()), j
+        pf.close()
+    # the forcing matters: the filters' likelihoods differ
+    assert len(set(np.round(out["logLikelihood"], 6))) == B
+    bank.close()
+
+
+def test_bank_graph_replay_equals_eager(ctx, orc):
+    B, P, T = 3, 5000, 15
+    y = _pz_data(orc, T, seed=7)
+    gm = capi.make_model("pz", precision=capi.F64)
+    theta = _pz_thetas(B, seed=8)
+    outs = []
+    for use_graph in (False, True):
+        bank = capi.FilterBank(ctx, gm, P, T, 1.0, 9, theta, ess_rel=0.8, record=True,
+                               use_graph=use_graph)
+        bank.set_observations(y)
+        bank.init()
+        for _ in range(T):  # one replay at a time, with synchronisations between them
+            bank.step(1)
+            ctx.sync()
+        o = bank.outputs()
+        o["anc"] = np.array([[bank.ancestors(j, k) for k in range(T)] for j in range(B)])
+        outs.append(o)
+        bank.close()
+    for key in ("logIncrements", "resampled", "anc", "logLikelihood"):
+        np.testing.assert_array_equal(outs[0][key], outs[1][key], key)
+
+
+def _guided_net(ctx):
+    net = capi.MLP(ctx, guided.DIMS, seed=23)
+    p = net.get_params()
+    last = guided.DIMS[-1] * guided.DIMS[-2] + guided.DIMS[-1]
+    p[-last:] *= 0.3
+    net.set_params(p)
+    return net
+
+
+def test_guided_bank_equals_lone_guided_filters(ctx, torch_gpu):
+    B, P, T = 3, 1 << 12, 8
+    Fs = np.array([7.6, 8.0, 8.4])
+    gm = guided.model(8.0)
+    mean0 = data.l96_spinup(ctx, d=gm.d, F=8.0)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=13)
+    net = _guided_net(ctx)
+    norm = capi.GuidedNorm(x_shift=2.3, x_scale=3.6, innov_scale=1.0, par_shift=8.0,
+                           par_scale=1.0)
+    bank = capi.FilterBank(ctx, gm, P, T, DT, 3, Fs, ess_rel=0.5, record=True)
+    bank.set_guided(net, norm)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    for j in range(B):
+        pf = _lone(ctx, gm, [Fs[j]], P, T, DT, 3, j, capi.SYSTEMATIC, 0.5, y, mean0)
+        pf.set_guided(net, norm)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-13)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k))
+        Xb, _, _ = bank.state(j)
+        Xp, _, _ = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy()), j
+        pf.close()
+    bank.close()
+    net.close()
+
+
+def test_surrogate_bank_equals_lone_surrogate_filters(ctx, torch_gpu):
+    B, P, T = 3, 1 << 12, 8
+    Fs = np.array([7.5, 8.0, 8.5])
+    gm = capi.make_model("l96", d=40, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                         init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+    mean0 = data.l96_spinup(ctx, d=40)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=17)
+    net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=5)
+    norm = capi.SurrogateNorm(x_shift=2.3, x_scale=3.6, dx_scale=0.05, par_shift=8.0,
+                              par_scale=1.0)
+    bank = capi.FilterBank(ctx, gm, P, T, DT, 4, Fs, ess_rel=0.5, record=True)
+    bank.set_surrogate(net, norm)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    for j in range(B):
+        pf = _lone(ctx, gm, [Fs[j]], P, T, DT, 4, j, capi.SYSTEMATIC, 0.5, y, mean0)
+        pf.set_surrogate(net, norm)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-13)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k))
+        pf.close()
+    bank.close()
+    net.close()
+
+
+def test_bank_rejects_what_it_does_not_run(ctx):
+    gm = capi.make_model("pz", precision=capi.F64)
+    theta = _pz_thetas(2)
+    with pytest.raises(capi.BihipError, match="systematically or stratified"):
+        capi.FilterBank(ctx, gm, 100, 5, 1.0, 1, theta, resampler=capi.MULTINOMIAL)
+    gen = capi.generated_model("PZ", precision=capi.F64)
+    with pytest.raises(capi.BihipError, match="built-in models"):
+        capi.FilterBank(ctx, gen, 100, 5, 1.0, 1, theta)
+    l96 = capi.make_model("l96", d=40, precision=capi.F32)
+    bank = capi.FilterBank(ctx, l96, 256, 5, DT, 1, np.array([[8.0, 0.1], [8.0, 0.2]]))
+    net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=1)
+    norm = capi.SurrogateNorm(x_shift=0.0, x_scale=1.0, dx_scale=0.05, par_shift=8.0,
+                              par_scale=1.0)
+    with pytest.raises(capi.BihipError, match="differ only in par"):
+        bank.set_surrogate(net, norm)
+    bank.close()
+    net.close()
