@@ -517,6 +517,7 @@ def bench_surrogate(args):
     x_ms, x_n = pf.kernel_time(0)
     pf.close()
     local = surrogate_local_error(ctx, gm, net, norm, y, mean0, P, min(T, 40))
+    batched = surrogate_batched(ctx, gm, net, norm, y, mean0, args)
     flops = 2.0 * sum(a * b for a, b in zip(surrogate.DIMS[:-1], surrogate.DIMS[1:])) * P
     achieved = flops * k_n / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     v_sur = P * args.steps * args.seeds / el["surrogate"]
@@ -538,7 +539,8 @@ def bench_surrogate(args):
            # MFMA rate against the exact DOPRI5 transition's measured time
            "exact_transition_ms": x_ms / max(x_n, 1),
            "network_ms_at_mfma_peak": flops / 2.5e15 * 1e3,
-           "speedup_bound_at_mfma_peak": (x_ms / max(x_n, 1)) / (flops / 2.5e15 * 1e3)}
+           "speedup_bound_at_mfma_peak": (x_ms / max(x_n, 1)) / (flops / 2.5e15 * 1e3),
+           "batched_form": batched}
     roof = {"bound": "mfma", "kernel": "k_surrogate<40,256,4,step>", "achieved": achieved,
             "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0, "traffic": None,
             "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n,
@@ -546,6 +548,57 @@ def bench_surrogate(args):
     print(json.dumps(_line(v_sur, args, 1e3 * el["surrogate"] / (args.steps * args.seeds),
                            "bf16", cfg, roof, None)), flush=True)
     ctx.close()
+
+
+def surrogate_batched(ctx, gm, net, norm, y, mean0, args, B=256, N=4096, lone=16):
+    """The batched form: B filters of N particles over forcing values
+    F in [7.5, 8.5] (SMC^2's inner filters for the Lorenz-96 forcing) as one
+    filter bank, with the exact DOPRI5 flow and with the surrogate, against the
+    plain GPU filter (bihip_pf, hipGraph replay) run for one forcing after
+    another, timed over `lone` of them.  The exact bank's filters equal the
+    plain filters bit for bit (same log-likelihood error); the surrogate bank
+    carries the surrogate's error per interval."""
+    from libbi_amd import capi
+    T = args.warmup + args.steps
+    Fs = np.linspace(7.5, 8.5, B)
+
+    def bank_rate(use_surrogate):
+        b = capi.FilterBank(ctx, gm, N, T, DT, 100, Fs, ess_rel=0.5, use_graph=True)
+        if use_surrogate:
+            b.set_surrogate(net, norm)
+        b.set_observations(y)
+        b.init(mean0)
+        b.step(args.warmup)
+        el = _timed(ctx, lambda: b.step(args.steps))
+        b.close()
+        return B * N * args.steps / el
+
+    fl = []
+    for j in range(lone):
+        m = capi.Model.from_buffer_copy(gm)
+        m.par[0] = Fs[j]
+        pf = capi.ParticleFilter(ctx, m, N, T, DT, seed=100, ess_rel=0.5, batch=j, use_graph=True)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(args.warmup)
+        fl.append(pf)
+
+    def run():
+        for pf in fl:
+            pf.step(args.steps)
+
+    plain = lone * N * args.steps / _timed(ctx, run)
+    for pf in fl:
+        pf.close()
+    exact = bank_rate(False)
+    sur = bank_rate(True)
+    return {"filters": B, "particles_per_filter": N,
+            "plain_filter_particle_steps_per_s": plain,
+            "exact_bank_particle_steps_per_s": exact,
+            "surrogate_bank_particle_steps_per_s": sur,
+            "exact_bank_vs_plain": exact / plain, "surrogate_bank_vs_plain": sur / plain,
+            "error": "exact bank: the plain filter's (bit-identical filters); surrogate bank: "
+                     "loglik_error_per_interval"}
 
 
 def surrogate_local_error(ctx, gm, net, norm, y, mean0, P, K, seed=8):
@@ -624,47 +677,50 @@ def bench_guided(args):
     mean0 = data.l96_spinup(ctx, d=guided.D)
     _, y = data.simulate(ctx, guided.model(8.0), mean0, Tn, DT, seed=2024)
 
-    def filters(use, seed):
-        out = []
-        for j, F in enumerate(Fs):
-            pf = capi.ParticleFilter(ctx, guided.model(F), P, Tn, DT, seed=seed, ess_rel=0.5,
-                                     batch=j, use_graph=True)
-            pf.set_observations(y)
-            pf.init(mean0)
-            if use:
-                pf.set_guided(net, norm)
-            out.append(pf)
-        return out
+    def bank(use, seed, use_graph=True):
+        # every forcing value's filter in one bank: each launch covers all of them
+        b = capi.FilterBank(ctx, guided.model(8.0), P, Tn, DT, seed, Fs, ess_rel=0.5,
+                            use_graph=use_graph)
+        b.set_observations(y)
+        b.init(mean0)
+        if use:
+            b.set_guided(net, norm)
+        return b
 
     res = {}
     for use in (False, True):
         for seed in (1, 2):
-            fl = filters(use, seed)
-            for pf in fl:
-                pf.step(args.warmup)
+            b = bank(use, seed)
+            b.step(args.warmup)
+            el = _timed(ctx, lambda: b.step(args.steps))
+            out = b.outputs()
+            b.close()
+            res[(use, seed)] = (el, out["logLikelihood"], out["ess"][:, args.warmup:Tn])
+    # the same bootstrap filters one after another (bihip_pf, hipGraph replay)
+    lone = []
+    for j, F in enumerate(Fs):
+        pf = capi.ParticleFilter(ctx, guided.model(F), P, Tn, DT, seed=1, ess_rel=0.5, batch=j,
+                                 use_graph=True)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(args.warmup)
+        lone.append(pf)
 
-            def run():
-                for _ in range(args.steps):
-                    for pf in fl:
-                        pf.step(1)
+    def run_lone():
+        for pf in lone:
+            pf.step(args.steps)
 
-            el = _timed(ctx, run)
-            zs = np.array([pf.term() for pf in fl])
-            ess = np.array([pf.outputs()["ess"][args.warmup:Tn] for pf in fl])
-            for pf in fl:
-                pf.close()
-            res[(use, seed)] = (el, zs, ess)
-    # kernel duration of the proposal (events on direct launches of one filter)
-    pf = capi.ParticleFilter(ctx, guided.model(8.0), P, Tn, DT, seed=1, ess_rel=0.5)
-    pf.set_observations(y)
-    pf.init(mean0)
-    pf.set_guided(net, norm)
-    pf.set_timing(True)
-    pf.step(args.steps)
-    k_ms, k_n = pf.kernel_time(3)
-    step_ms, _ = pf.kernel_time(0)
-    pf.close()
-    flops = 2.0 * sum(a * b for a, b in zip(guided.DIMS[:-1], guided.DIMS[1:])) * P
+    el_lone = _timed(ctx, run_lone)
+    for pf in lone:
+        pf.close()
+    # kernel durations of the bank's guided steps (events on direct launches)
+    b = bank(True, 1, use_graph=False)
+    b.set_timing(True)
+    b.step(args.steps)
+    k_ms, k_n = b.kernel_time(3)
+    step_ms, _ = b.kernel_time(0)
+    b.close()
+    flops = 2.0 * sum(a * b_ for a, b_ in zip(guided.DIMS[:-1], guided.DIMS[1:])) * P * len(Fs)
     ps = P * args.thetas * args.steps
     v_g = 2 * ps / (res[(True, 1)][0] + res[(True, 2)][0])
     v_b = 2 * ps / (res[(False, 1)][0] + res[(False, 2)][0])
@@ -672,16 +728,20 @@ def bench_guided(args):
     ess = {u: float(np.mean([res[(u, s_)][2].mean() for s_ in (1, 2)])) / P for u in (False, True)}
     cfg = {"workload": "Lorenz-96 d=200 filters with a learned guided proposal (301-256-256-256-400 "
                        "BF16 MFMA) and exact weight correction, N=2^18 FP32 state / FP64 weights "
-                       f"for each of {args.thetas} forcing values F in [7.5, 8.5], DOPRI5 "
-                       "rtol=atol=1e-5, systematic resampling at ESS<0.5N",
+                       f"for each of {args.thetas} forcing values F in [7.5, 8.5] as one filter "
+                       "bank (shared launches), DOPRI5 rtol=atol=1e-5, systematic resampling at "
+                       "ESS<0.5N",
            "particles_per_filter": P, "filters": args.thetas, "training_seconds": train_s,
            "final_minibatch_loss": float(losses[-1][-100:].mean()),
            "bootstrap_particle_steps_per_s": v_b,
+           "bootstrap_lone_filters_particle_steps_per_s": ps / el_lone,
+           "bank_vs_lone_filters": v_b / (ps / el_lone),
            "mean_ess_fraction_guided": ess[True], "mean_ess_fraction_bootstrap": ess[False],
            "logZ_seed_sd_guided": spread[True], "logZ_seed_sd_bootstrap": spread[False],
            "logZ_mean_guided_minus_bootstrap": float(np.mean(
                (res[(True, 1)][1] + res[(True, 2)][1]) / 2 - (res[(False, 1)][1] + res[(False, 2)][1]) / 2))}
-    roof = {"bound": "mfma", "kernel": "k_guided<filter>", "achieved": flops / (k_ms / k_n * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "k_guided<filter> (one launch: every filter of the bank)",
+            "achieved": flops / (k_ms / k_n * 1e-3) / 1e12
             if k_ms > 0 else 0.0, "peak": 2500.0, "unit": "TFLOP/s", "traffic": None,
             "avg_launch_ms": k_ms / max(k_n, 1), "launches": k_n, "flop_per_launch": flops,
             "flow_plus_proposal_ms_per_step": step_ms / max(k_n, 1)}
