@@ -69,6 +69,8 @@ struct Group {
   __device__ __forceinline__ T prev(T v) const {
     if constexpr (LPP == 1) {
       return v;
+    } else if constexpr (LPP == 2) {
+      return dpp<QP_XOR1>(v);
     } else if constexpr (LPP == 4) {
       return dpp<QP_PREV>(v);
     } else {
@@ -80,6 +82,8 @@ struct Group {
   __device__ __forceinline__ T next(T v) const {
     if constexpr (LPP == 1) {
       return v;
+    } else if constexpr (LPP == 2) {
+      return dpp<QP_XOR1>(v);
     } else if constexpr (LPP == 4) {
       return dpp<QP_NEXT>(v);
     } else {
@@ -90,7 +94,9 @@ struct Group {
   // butterfly sum: every lane of the group gets the same bits
   template <class T>
   __device__ __forceinline__ T sum(T v) const {
-    if constexpr (LPP == 4) {
+    if constexpr (LPP == 2) {
+      return v + dpp<QP_XOR1>(v);
+    } else if constexpr (LPP == 4) {
       v += dpp<QP_XOR1>(v);
       v += dpp<QP_XOR2>(v);
       return v;

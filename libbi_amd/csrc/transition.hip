@@ -339,6 +339,14 @@ bool model_supported(const bihip_model* m) {
   return false;
 }
 
+// Lanes per particle of the Lorenz-96 d=40 model.  Four: ten dimensions per
+// lane, 112 VGPRs.  Two lanes (twenty dimensions each, ten Philox blocks per
+// particle instead of twelve) need 216 VGPRs, two waves per SIMD, and
+// measured 234 us per N = 2^20 FP32 transition against 207-210 us.
+#ifndef L96_D40_LPP
+#define L96_D40_LPP 4
+#endif
+
 // Calls F::template run<M>() for the model type selected by m.
 template <class F>
 static int with_model(const bihip_model* m, F& f) {
@@ -349,7 +357,8 @@ static int with_model(const bihip_model* m, F& f) {
       return f64 ? f.template run<PZ<double>>() : f.template run<PZ<float>>();
     case BIHIP_MODEL_L96:
       if (m->d == 40)
-        return f64 ? f.template run<L96<double, 40, 4>>() : f.template run<L96<float, 40, 4>>();
+        return f64 ? f.template run<L96<double, 40, L96_D40_LPP>>()
+                   : f.template run<L96<float, 40, L96_D40_LPP>>();
       if (m->d == 200)
         return f64 ? f.template run<L96<double, 200, 8>>() : f.template run<L96<float, 200, 8>>();
       return f64 ? f.template run<L96<double, 8, 2>>() : f.template run<L96<float, 8, 2>>();
@@ -360,7 +369,7 @@ static int with_model(const bihip_model* m, F& f) {
 
 int transition_blocks(const bihip_model* m, int P) {
   int lpp = 1;
-  if (m->model == BIHIP_MODEL_L96) lpp = m->d == 200 ? 8 : (m->d == 40 ? 4 : 2);
+  if (m->model == BIHIP_MODEL_L96) lpp = m->d == 200 ? 8 : (m->d == 40 ? L96_D40_LPP : 2);
   if (const GenInfo* g = generated_info(m->model)) lpp = g->lpp;
   return ceil_div((long)P * lpp, TRANS_BLOCK);
 }
