@@ -702,6 +702,27 @@ int bihip_pfbank_position(bihip_pfbank* bank);
 int bihip_pfbank_set_surrogate(bihip_pfbank* bank, bihip_mlp* mlp,
                                const bihip_surrogate_norm* norm);
 int bihip_pfbank_set_guided(bihip_pfbank* bank, bihip_mlp* mlp, const bihip_guided_norm* norm);
+/* The outer loop of SMC^2 over a bank (MarginalSIR.hpp:281-488, as bihip_batch_set_theta,
+ * _set_stream_base, _increments, _resample_filters and _accept do for bihip_batch):
+ * new parameters (B x ntheta) for the following steps or initialisation; the Philox batch
+ * id of filter 0 from the next step on; the log-likelihood increments of step k (B
+ * doubles); filter j takes the particles, log-weights, running log-likelihood and
+ * parameters of filter anc_host[j]; filter j of dst takes those of filter j of src where
+ * accept_host[j] != 0 (banks of one shape at the same step). */
+int bihip_pfbank_set_theta(bihip_pfbank* bank, const double* theta_host);
+int bihip_pfbank_set_stream_base(bihip_pfbank* bank, uint32_t base);
+int bihip_pfbank_increments(bihip_pfbank* bank, int k, double* li_host);
+int bihip_pfbank_resample_filters(bihip_pfbank* bank, const int* anc_host);
+int bihip_pfbank_accept(bihip_pfbank* dst, const bihip_pfbank* src, const int* accept_host);
+/* SMC^2 (bihip_smc2_*) whose inner filters are filter banks of P particles each (the
+ * theta-particles' filters and a move's proposals) in place of bihip_batch: any inner size,
+ * lane-group models (Lorenz-96), the learned surrogate flow.  Built-in models; the
+ * bihip_smc2_config fields block and prior_from_model are unused; Lorenz-96 filters start
+ * around mean0_host.  Outputs and keyed draws as bihip_smc2_create's. */
+int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                           int P, const double* mean0_host, bihip_smc2** s);
+/* The learned surrogate flow in every inner filter (bihip_pfbank_set_surrogate). */
+int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
 /* Event timing of the bank's launches, as bihip_pf_set_timing / bihip_pf_kernel_time. */
 int bihip_pfbank_set_timing(bihip_pfbank* bank, int mode);
 int bihip_pfbank_kernel_time(bihip_pfbank* bank, int kernel, double* ms_host, long* count_host);

@@ -74,6 +74,9 @@ EXPORTS = [
     "bihip_pfbank_init", "bihip_pfbank_step", "bihip_pfbank_outputs", "bihip_pfbank_ancestors",
     "bihip_pfbank_state", "bihip_pfbank_position", "bihip_pfbank_set_surrogate",
     "bihip_pfbank_set_guided", "bihip_pfbank_set_timing", "bihip_pfbank_kernel_time",
+    "bihip_pfbank_set_theta", "bihip_pfbank_set_stream_base", "bihip_pfbank_increments",
+    "bihip_pfbank_resample_filters", "bihip_pfbank_accept", "bihip_smc2_create_bank",
+    "bihip_smc2_set_surrogate",
 ]
 
 
@@ -282,6 +285,13 @@ def _declare(L):
         "bihip_pfbank_set_guided": [P, P, c.POINTER(GuidedNorm)],
         "bihip_pfbank_set_timing": [P, i],
         "bihip_pfbank_kernel_time": [P, i, pd, pl],
+        "bihip_pfbank_set_theta": [P, pd],
+        "bihip_pfbank_set_stream_base": [P, u32],
+        "bihip_pfbank_increments": [P, i, pd],
+        "bihip_pfbank_resample_filters": [P, pi],
+        "bihip_pfbank_accept": [P, P, pi],
+        "bihip_smc2_create_bank": [P, pm, c.POINTER(SMC2Config), i, pd, c.POINTER(P)],
+        "bihip_smc2_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -763,6 +773,31 @@ class FilterBank:
         check(lib().bihip_pfbank_set_guided(self.h, mlp.h if mlp is not None else None,
                                             ctypes.byref(norm) if norm is not None else None))
 
+    def set_theta(self, theta):
+        th = np.ascontiguousarray(theta, np.float64).reshape(self.B, self.ntheta)
+        self.theta = th
+        check(lib().bihip_pfbank_set_theta(self.h, th.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+
+    def set_stream_base(self, base):
+        check(lib().bihip_pfbank_set_stream_base(self.h, base))
+
+    def increments(self, k):
+        li = np.zeros(self.B)
+        check(lib().bihip_pfbank_increments(self.h, k, li.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+        return li
+
+    def resample_filters(self, anc):
+        a = np.ascontiguousarray(anc, np.int32)
+        check(lib().bihip_pfbank_resample_filters(self.h, a.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+
+    def accept(self, src, acc):
+        a = np.ascontiguousarray(acc, np.int32)
+        check(lib().bihip_pfbank_accept(self.h, src.h, a.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int))))
+
     def set_timing(self, on, transition_only=False):
         mode = (TIMING_TRANSITION if transition_only else TIMING_ALL) if on else 0
         check(lib().bihip_pfbank_set_timing(self.h, mode))
@@ -910,12 +945,17 @@ class SMC2:
 
     def __init__(self, ctx, model, N, T, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
                  nmoves=1, resampler=SYSTEMATIC, ess_rel_inner=0.5, ess_rel_outer=0.5, t0=0.0,
-                 use_graph=False, block=0):
+                 use_graph=False, block=0, bank=None, mean0=None):
         """prior_mean = prior_sd = None: the prior of theta (the first len(prop_sd)
-        params) is the parameter block of the generated model's file."""
+        params) is the parameter block of the generated model's file.  bank=(B, P):
+        inner filters as filter banks of P particles (N unused; mean0 for Lorenz-96)."""
         from_model = prior_mean is None
         nt = len(prop_sd)
-        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        if bank is not None:
+            B, Pb = bank
+            self.N = np.full(B, Pb, np.int32)
+        else:
+            self.N = np.ascontiguousarray(N, dtype=np.int32)
         B = len(self.N)
         cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=nmoves, resampler=resampler,
                          essRel_inner=ess_rel_inner, essRel_outer=ess_rel_outer, t0=t0, dt=dt,
@@ -928,9 +968,17 @@ class SMC2:
             cfg.prop_sd[i] = prop_sd[i]
         self.cfg = cfg
         h = ctypes.c_void_p()
-        check(lib().bihip_smc2_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg),
-                                      self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                      ctypes.byref(h)))
+        if bank is not None:
+            self._mean0 = (np.ascontiguousarray(mean0, np.float64) if mean0 is not None
+                           else None)
+            m0 = (self._mean0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                  if self._mean0 is not None else None)
+            check(lib().bihip_smc2_create_bank(ctx.h, ctypes.byref(model), ctypes.byref(cfg),
+                                               int(bank[1]), m0, ctypes.byref(h)))
+        else:
+            check(lib().bihip_smc2_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg),
+                                          self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          ctypes.byref(h)))
         self.h = h
         ctx.adopt(self)
         self.B, self.T, self.nt, self.nobs = B, T, nt, nobs(model)
@@ -953,6 +1001,11 @@ class SMC2:
 
     def run(self):
         check(lib().bihip_smc2_run(self.h))
+
+    def set_surrogate(self, mlp, norm):
+        """The learned surrogate flow in every inner filter (bank=... only)."""
+        check(lib().bihip_smc2_set_surrogate(self.h, mlp.h if mlp is not None else None,
+                                             ctypes.byref(norm) if norm is not None else None))
 
     def outputs(self):
         D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)

@@ -46,6 +46,9 @@ struct bihip_pfbank {
   double* par = nullptr;     // B x BANK_ROW
   float* par_u = nullptr;    // B: normalised parameter input of a learned flow
   std::vector<double> rows;  // host copy of par
+  int ntheta = 0;
+  int* idx = nullptr;        // B ints: filter indices of whole-filter copies
+  int* mask = nullptr;       // B ints: which filters a copy takes
   ScanState* ws_state = nullptr;
   uint64_t* ws_words = nullptr;
   int k = 0;
@@ -262,6 +265,33 @@ int set_par_u(bihip_pfbank* b, double shift, double scale) {
   return BIHIP_OK;
 }
 
+// Parameter rows of theta (B x ntheta over the model's parameters), on the
+// device, with the learned flow's parameter inputs.
+int upload_rows(bihip_pfbank* b, const double* theta) {
+  for (int j = 0; j < b->B; ++j) {
+    bihip_model mj = b->model;
+    for (int i = 0; i < b->ntheta; ++i) mj.par[i] = theta[(size_t)j * b->ntheta + i];
+    if (b->model.model == BIHIP_MODEL_L96 && (b->model.par[2] > 0.0) != (mj.par[2] > 0.0))
+      return set_error(BIHIP_EINVAL, "filter %d: the observation sd must stay positive", j);
+    bank_row(&mj, &b->rows[(size_t)j * BANK_ROW]);
+  }
+  return BIHIP_OK;
+}
+
+int sync_rows(bihip_pfbank* b) {
+  BIHIP_HIP(hipMemcpy(b->par, b->rows.data(), sizeof(double) * b->rows.size(),
+                      hipMemcpyHostToDevice));
+  if (b->guided) return set_par_u(b, b->gl.par_shift, b->gl.par_scale);
+  if (b->surr) return set_par_u(b, b->sl.par_shift, b->sl.par_scale);
+  return BIHIP_OK;
+}
+
+bool same_shape(const bihip_pfbank* a, const bihip_pfbank* b) {
+  return a->B == b->B && a->cfg.P == b->cfg.P && a->ld == b->ld && a->elem == b->elem &&
+         a->model.model == b->model.model && a->model.d == b->model.d &&
+         a->model.precision == b->model.precision;
+}
+
 }  // namespace
 
 extern "C" {
@@ -303,20 +333,16 @@ int bihip_pfbank_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_con
   b->tiles = scan_tiles(cfg->P);
   b->nparts = transition_blocks(m, cfg->P);
   b->nparts_cap = b->nparts;
+  b->ntheta = ntheta;
   const bool l96f32 = m->model == BIHIP_MODEL_L96 && m->precision == BIHIP_F32;
   if (l96f32 && m->d == 200) b->nparts_cap = std::max(b->nparts_cap, guided_blocks(cfg->P));
   if (l96f32 && m->d == 40) b->nparts_cap = std::max(b->nparts_cap, surrogate_blocks(cfg->P));
   // parameter rows: the model's, with theta's columns in place
   b->rows.resize((size_t)B * BANK_ROW);
-  for (int j = 0; j < B; ++j) {
-    bihip_model mj = *m;
-    for (int i = 0; i < ntheta; ++i) mj.par[i] = theta_host[(size_t)j * ntheta + i];
-    // the precomputed observation constants of Lorenz-96 (1 / obs_sd, log obs_sd)
-    if (m->model == BIHIP_MODEL_L96 && (m->par[2] > 0.0) != (mj.par[2] > 0.0)) {
-      delete b;
-      return set_error(BIHIP_EINVAL, "filter %d: the observation sd must stay positive", j);
-    }
-    bank_row(&mj, &b->rows[(size_t)j * BANK_ROW]);
+  // the precomputed observation constants of Lorenz-96 (1 / obs_sd, log obs_sd)
+  if ((rc = upload_rows(b, theta_host))) {
+    delete b;
+    return rc;
   }
   const size_t xb = (size_t)B * ld * m->d * b->elem;
   const size_t n = (size_t)B * ld;
@@ -347,6 +373,8 @@ int bihip_pfbank_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_con
   BK_ALLOC(b->par_u, (size_t)B);
   BK_ALLOC(b->ws_state, (size_t)B);
   BK_ALLOC(b->ws_words, (size_t)B * SCAN_WORDS * b->tiles);
+  BK_ALLOC(b->idx, (size_t)B);
+  BK_ALLOC(b->mask, (size_t)B);
 #undef BK_ALLOC
   BIHIP_HIP(hipMemcpy(b->par, b->rows.data(), sizeof(double) * b->rows.size(),
                       hipMemcpyHostToDevice));
@@ -381,6 +409,8 @@ int bihip_pfbank_destroy(bihip_pfbank* b) {
   dfree(b->par_u);
   dfree(b->ws_state);
   dfree(b->ws_words);
+  dfree(b->idx);
+  dfree(b->mask);
   delete b;
   return BIHIP_OK;
 }
@@ -565,6 +595,110 @@ int bihip_pfbank_set_guided(bihip_pfbank* b, bihip_mlp* mlp, const bihip_guided_
   b->guided = mlp != nullptr;
   b->nparts = b->guided ? guided_blocks(b->cfg.P) : transition_blocks(&b->model, b->cfg.P);
   return BIHIP_OK;
+}
+
+int bihip_pfbank_set_theta(bihip_pfbank* b, const double* theta_host) {
+  if (b == nullptr || (b->ntheta > 0 && theta_host == nullptr))
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(ctx_stream_of(b->ctx)));
+  std::vector<double> keep = b->rows;
+  if ((rc = upload_rows(b, theta_host))) {
+    b->rows = keep;
+    return rc;
+  }
+  if ((b->guided || b->surr) && (rc = learned_rows_ok(b))) {
+    b->rows = keep;
+    return rc;
+  }
+  return sync_rows(b);
+}
+
+int bihip_pfbank_set_stream_base(bihip_pfbank* b, uint32_t base) {
+  if (b == nullptr) return set_error(BIHIP_EINVAL, "NULL bank");
+  if ((uint64_t)base + (uint64_t)b->B > (1u << 24))
+    return set_error(BIHIP_EINVAL, "Philox batch ids must stay below 2^24");
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(ctx_stream_of(b->ctx)));
+  if (b->cfg.batch != base) drop_graphs(b);  // the captured launches hold the batch id
+  b->cfg.batch = base;
+  return BIHIP_OK;
+}
+
+int bihip_pfbank_increments(bihip_pfbank* b, int k, double* li_host) {
+  if (b == nullptr || li_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (k < 0 || k >= b->cfg.T) return set_error(BIHIP_EINVAL, "step %d out of range", k);
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  BIHIP_HIP(hipStreamSynchronize(ctx_stream_of(b->ctx)));
+  BIHIP_HIP(hipMemcpy2D(li_host, sizeof(double), b->hist_li + k, sizeof(double) * b->cfg.T,
+                        sizeof(double), b->B, hipMemcpyDeviceToHost));
+  return BIHIP_OK;
+}
+
+int bihip_pfbank_resample_filters(bihip_pfbank* b, const int* anc_host) {
+  if (b == nullptr || anc_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (!b->ready) return set_error(BIHIP_ESTATE, "bihip_pfbank_init has not been called");
+  for (int j = 0; j < b->B; ++j)
+    if (anc_host[j] < 0 || anc_host[j] >= b->B)
+      return set_error(BIHIP_EINVAL, "ancestor %d of filter %d out of range", anc_host[j], j);
+  int rc = ctx_enter_device(b->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(b->ctx);
+  BIHIP_HIP(hipMemcpyAsync(b->idx, anc_host, sizeof(int) * b->B, hipMemcpyHostToDevice, s));
+  const long xb = (long)b->model.d * b->ld * b->elem, lb = (long)b->ld * sizeof(double);
+  // particles into the other state buffer, log-weights through the EF scratch
+  if ((rc = launch_filter_copy(b->X[b->cur], b->X[1 - b->cur], xb, b->B, b->idx, nullptr, s)) ||
+      (rc = launch_filter_copy(b->lws, b->EF, lb, b->B, b->idx, nullptr, s)) ||
+      (rc = launch_filter_copy(b->EF, b->lws, lb, b->B, nullptr, nullptr, s)))
+    return rc;
+  std::vector<PFDevice> d(b->B), d2(b->B);
+  BIHIP_HIP(hipMemcpyAsync(d.data(), b->dev, sizeof(PFDevice) * b->B, hipMemcpyDeviceToHost, s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  std::vector<double> r2(b->rows.size());
+  for (int j = 0; j < b->B; ++j) {
+    d2[j] = d[anc_host[j]];
+    std::copy(&b->rows[(size_t)anc_host[j] * BANK_ROW],
+              &b->rows[(size_t)anc_host[j] * BANK_ROW] + BANK_ROW, &r2[(size_t)j * BANK_ROW]);
+  }
+  BIHIP_HIP(hipMemcpy(b->dev, d2.data(), sizeof(PFDevice) * b->B, hipMemcpyHostToDevice));
+  b->rows = r2;
+  b->cur = 1 - b->cur;
+  return sync_rows(b);
+}
+
+int bihip_pfbank_accept(bihip_pfbank* dst, const bihip_pfbank* src, const int* accept_host) {
+  if (dst == nullptr || src == nullptr || accept_host == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  if (!same_shape(dst, src)) return set_error(BIHIP_EINVAL, "banks of different shapes");
+  if (!dst->ready || !src->ready || dst->k != src->k)
+    return set_error(BIHIP_ESTATE, "both banks must be initialised and at the same step");
+  int rc = ctx_enter_device(dst->ctx);
+  if (rc) return rc;
+  hipStream_t s = ctx_stream_of(dst->ctx);
+  BIHIP_HIP(hipMemcpyAsync(dst->mask, accept_host, sizeof(int) * dst->B, hipMemcpyHostToDevice,
+                           s));
+  const long xb = (long)dst->model.d * dst->ld * dst->elem, lb = (long)dst->ld * sizeof(double);
+  if ((rc = launch_filter_copy(src->X[src->cur], dst->X[dst->cur], xb, dst->B, nullptr,
+                               dst->mask, s)) ||
+      (rc = launch_filter_copy(src->lws, dst->lws, lb, dst->B, nullptr, dst->mask, s)))
+    return rc;
+  std::vector<PFDevice> d(dst->B), e(dst->B);
+  BIHIP_HIP(hipMemcpyAsync(d.data(), dst->dev, sizeof(PFDevice) * dst->B, hipMemcpyDeviceToHost,
+                           s));
+  BIHIP_HIP(hipMemcpyAsync(e.data(), src->dev, sizeof(PFDevice) * dst->B, hipMemcpyDeviceToHost,
+                           s));
+  BIHIP_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < dst->B; ++j) {
+    if (!accept_host[j]) continue;
+    d[j] = e[j];
+    std::copy(&src->rows[(size_t)j * BANK_ROW], &src->rows[(size_t)j * BANK_ROW] + BANK_ROW,
+              &dst->rows[(size_t)j * BANK_ROW]);
+  }
+  BIHIP_HIP(hipMemcpy(dst->dev, d.data(), sizeof(PFDevice) * dst->B, hipMemcpyHostToDevice));
+  return sync_rows(dst);
 }
 
 int bihip_pfbank_set_timing(bihip_pfbank* b, int mode) {

@@ -300,6 +300,10 @@ int launch_record_state(const void* X, int ld, int D, int P, int elem, const PFD
                         void* hist, int slot0, hipStream_t s);
 int launch_paths(const int* anc, const void* hist, int ld, int D, int P, int K, int elem,
                  const int* idx, int n, double* out, hipStream_t s);
+// Whole-filter copies: filter j of dst (bytes per filter) takes filter
+// from[j] of src (from null: j) where mask is null or mask[j] != 0.
+int launch_filter_copy(const void* src, void* dst, long bytes, int banks, const int* from,
+                       const int* mask, hipStream_t s);
 // banks > 1: filter f's ancestors at as + f bp, its history at out + f hs.
 int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out,
                             hipStream_t s, int banks = 1, long bp = 0, long hs = 0);

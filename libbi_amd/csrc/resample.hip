@@ -17,6 +17,8 @@
 // slot k keeps particle k when o_k > 0, and otherwise receives the particle
 // i whose extra copies cover rank F_k, i.e. the largest i with E_i <= F_k.
 // This is the host's serial two-pointer layout, bit for bit.
+#include <algorithm>
+
 #include "scan.hpp"
 
 namespace bihip {
@@ -857,6 +859,32 @@ int launch_record_ancestors(const int* as, int P, const PFDevice* gate, int* out
                             hipStream_t s, int banks, long bp, long hs) {
   hipLaunchKernelGGL(k_record_ancestors, dim3(ceil_div(P, 256), banks > 1 ? banks : 1), dim3(256),
                      0, s, as, P, gate, out, bp, hs);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+// Whole-filter copies between the filters of banks: filter j of dst takes
+// filter from[j] of src (from null: filter j) where mask is null or
+// mask[j] != 0; n16 16-byte words per filter, filter j on grid row j.
+__global__ void k_filter_copy(const uint4* src, uint4* dst, long n16, const int* from,
+                              const int* mask) {
+  const long j = blockIdx.y;
+  if (mask != nullptr && mask[j] == 0) return;
+  const long f = from != nullptr ? from[j] : j;
+  const uint4* s = src + f * n16;
+  uint4* d = dst + j * n16;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (long)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+
+int launch_filter_copy(const void* src, void* dst, long bytes, int banks, const int* from,
+                       const int* mask, hipStream_t s) {
+  if (bytes % 16 != 0) return set_error(BIHIP_EINVAL, "filter copies move 16-byte words");
+  const long n16 = bytes / 16;
+  const int gx = (int)std::min<long>(ceil_div(n16, 256L), 512L);
+  hipLaunchKernelGGL(k_filter_copy, dim3(gx > 0 ? gx : 1, banks), dim3(256), 0, s,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16, from, mask);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

@@ -28,12 +28,74 @@
 
 using namespace bihip;
 
+namespace {
+
+// The inner filters of the B theta-particles: one bihip_batch (ragged N_j up to
+// 16384, one workgroup per filter) or one filter bank (equal N of any size,
+// shared launches).  Both equal bihip_pf with batch id stream_base + j.
+struct Inner {
+  virtual ~Inner() = default;
+  virtual int set_observations(const double* y) = 0;
+  virtual int set_sizes(const int* N) = 0;
+  virtual int set_theta(const double* theta) = 0;
+  virtual int set_stream_base(uint32_t base) = 0;
+  virtual int init() = 0;
+  virtual int step(int n) = 0;
+  virtual int outputs(double* ll, int* status) = 0;
+  virtual int increments(int k, double* li) = 0;
+  virtual int resample_filters(const int* anc) = 0;
+  virtual int accept(Inner* src, const int* acc) = 0;
+};
+
+struct BatchInner : Inner {
+  bihip_batch* b = nullptr;
+  ~BatchInner() override { bihip_batch_destroy(b); }
+  int set_observations(const double* y) override { return bihip_batch_set_observations(b, y); }
+  int set_sizes(const int* N) override { return bihip_batch_set_sizes(b, N); }
+  int set_theta(const double* th) override { return bihip_batch_set_theta(b, th); }
+  int set_stream_base(uint32_t base) override { return bihip_batch_set_stream_base(b, base); }
+  int init() override { return bihip_batch_init(b); }
+  int step(int n) override { return bihip_batch_step(b, n); }
+  int outputs(double* ll, int* st) override {
+    return bihip_batch_outputs(b, ll, nullptr, nullptr, nullptr, st);
+  }
+  int increments(int k, double* li) override { return bihip_batch_increments(b, k, li); }
+  int resample_filters(const int* anc) override { return bihip_batch_resample_filters(b, anc); }
+  int accept(Inner* src, const int* acc) override {
+    return bihip_batch_accept(b, static_cast<BatchInner*>(src)->b, acc);
+  }
+};
+
+struct BankInner : Inner {
+  bihip_pfbank* b = nullptr;
+  int P = 0;
+  std::vector<double> mean0;
+  ~BankInner() override { bihip_pfbank_destroy(b); }
+  int set_observations(const double* y) override { return bihip_pfbank_set_observations(b, y); }
+  int set_sizes(const int* N) override { return BIHIP_OK; }  // every filter keeps P
+  int set_theta(const double* th) override { return bihip_pfbank_set_theta(b, th); }
+  int set_stream_base(uint32_t base) override { return bihip_pfbank_set_stream_base(b, base); }
+  int init() override { return bihip_pfbank_init(b, mean0.empty() ? nullptr : mean0.data()); }
+  int step(int n) override { return bihip_pfbank_step(b, n); }
+  int outputs(double* ll, int* st) override {
+    return bihip_pfbank_outputs(b, ll, nullptr, nullptr, nullptr, st);
+  }
+  int increments(int k, double* li) override { return bihip_pfbank_increments(b, k, li); }
+  int resample_filters(const int* anc) override { return bihip_pfbank_resample_filters(b, anc); }
+  int accept(Inner* src, const int* acc) override {
+    return bihip_pfbank_accept(b, static_cast<BankInner*>(src)->b, acc);
+  }
+};
+
+}  // namespace
+
 struct bihip_smc2 {
   bihip_ctx* ctx = nullptr;
   bihip_model model{};
   bihip_smc2_config cfg{};
-  bihip_batch* slots = nullptr;
-  bihip_batch* prop = nullptr;
+  Inner* slots = nullptr;
+  Inner* prop = nullptr;
+  bool bank = false;
   std::vector<int> N0, N;  // sizes at creation, current sizes per slot
   std::vector<double> lth, lth2, lw, theta;
   std::vector<int> accepts, resampled;
@@ -183,10 +245,10 @@ int check_prior(const bihip_model* m, const bihip_smc2_config& c) {
   return BIHIP_OK;
 }
 
-int filter_logliks(bihip_batch* b, int B, std::vector<double>& ll) {
+int filter_logliks(Inner* b, int B, std::vector<double>& ll) {
   std::vector<int> st(B);
   ll.resize(B);
-  int rc = bihip_batch_outputs(b, ll.data(), nullptr, nullptr, nullptr, st.data());
+  int rc = b->outputs(ll.data(), st.data());
   if (rc) return rc;
   for (int j = 0; j < B; ++j)
     if (st[j] != BIHIP_OK) ll[j] = -INFINITY;
@@ -212,10 +274,9 @@ int smc2_move(bihip_smc2* s, int k, int* nacc) {
       }
     }
     const uint32_t base = (uint32_t)(B * (1 + m + c.nmoves * k));
-    if ((rc = bihip_batch_set_sizes(s->prop, s->N.data())) ||
-        (rc = bihip_batch_set_theta(s->prop, th.data())) ||
-        (rc = bihip_batch_set_stream_base(s->prop, base)) || (rc = bihip_batch_init(s->prop)) ||
-        (rc = bihip_batch_step(s->prop, k)) || (rc = filter_logliks(s->prop, B, ll2)) ||
+    if ((rc = s->prop->set_sizes(s->N.data())) || (rc = s->prop->set_theta(th.data())) ||
+        (rc = s->prop->set_stream_base(base)) || (rc = s->prop->init()) ||
+        (rc = s->prop->step(k)) || (rc = filter_logliks(s->prop, B, ll2)) ||
         (rc = filter_logliks(s->slots, B, ll1)))
       return rc;
     for (int j = 0; j < B; ++j) {
@@ -237,7 +298,7 @@ int smc2_move(bihip_smc2* s, int k, int* nacc) {
         ++*nacc;
       }
     }
-    if ((rc = bihip_batch_accept(s->slots, s->prop, acc.data()))) return rc;
+    if ((rc = s->slots->accept(s->prop, acc.data()))) return rc;
   }
   return BIHIP_OK;
 }
@@ -246,11 +307,7 @@ int smc2_move(bihip_smc2* s, int k, int* nacc) {
 
 extern "C" {
 
-int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
-                      const int* N_host, bihip_smc2** out) {
-  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
-    return set_error(BIHIP_EINVAL, "NULL argument");
-  const bihip_smc2_config& c = *cfg;
+static int smc2_check(const bihip_model* m, const bihip_smc2_config& c) {
   if (c.B <= 0 || c.T <= 0) return set_error(BIHIP_EINVAL, "need B > 0 and T > 0");
   if (c.ntheta < 1 || c.ntheta > 8) return set_error(BIHIP_EINVAL, "ntheta must lie in [1,8]");
   if (c.nmoves < 0) return set_error(BIHIP_EINVAL, "nmoves must be >= 0");
@@ -259,14 +316,14 @@ int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   if (int rc = check_prior(m, c)) return rc;
   if ((uint64_t)c.B * (1 + (uint64_t)c.nmoves * (c.T + 1)) > (1u << 24))
     return set_error(BIHIP_EINVAL, "B (1 + nmoves (T + 1)) random-stream ids exceed 2^24");
-  int maxN = 0;
-  for (int j = 0; j < c.B; ++j) maxN = N_host[j] > maxN ? N_host[j] : maxN;
+  return BIHIP_OK;
+}
+
+static bihip_smc2* smc2_new(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config& c) {
   bihip_smc2* s = new bihip_smc2();
   s->ctx = ctx;
   s->model = *m;
   s->cfg = c;
-  s->N0.assign(N_host, N_host + c.B);
-  s->N = s->N0;
   s->lth.assign((size_t)c.B * c.ntheta, 0.0);
   s->lth2 = s->lth;
   s->theta = s->lth;
@@ -274,6 +331,20 @@ int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   s->accepts.assign(c.T + 1, 0);
   s->resampled.assign(c.T, 0);
   s->ess.assign(c.T, 0.0);
+  return s;
+}
+
+int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                      const int* N_host, bihip_smc2** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (int rc = smc2_check(m, c)) return rc;
+  int maxN = 0;
+  for (int j = 0; j < c.B; ++j) maxN = N_host[j] > maxN ? N_host[j] : maxN;
+  bihip_smc2* s = smc2_new(ctx, m, c);
+  s->N0.assign(N_host, N_host + c.B);
+  s->N = s->N0;
   bihip_batch_config bc{};
   bc.B = c.B;
   bc.T = c.T;
@@ -287,30 +358,85 @@ int bihip_smc2_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   bc.block = c.block;
   bc.capacity = (long)c.B * maxN;  // outer resampling can repeat the largest filter
   bc.stream_base = 0;
-  int rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &s->slots);
+  auto* slots = new BatchInner();
+  auto* prop = new BatchInner();
+  s->slots = slots;
+  s->prop = prop;
+  int rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &slots->b);
   if (rc == BIHIP_OK)
-    rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &s->prop);
+    rc = bihip_batch_create(ctx, m, &bc, N_host, s->theta.data(), c.ntheta, &prop->b);
   if (rc != BIHIP_OK) {
-    bihip_batch_destroy(s->slots);
-    delete s;
+    bihip_smc2_destroy(s);
     return rc;
   }
   *out = s;
   return BIHIP_OK;
 }
 
+int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                           int P, const double* mean0_host, bihip_smc2** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (int rc = smc2_check(m, c)) return rc;
+  if (c.prior_from_model) return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
+  if (P <= 0) return set_error(BIHIP_EINVAL, "need P > 0");
+  if (m->model == BIHIP_MODEL_L96 && mean0_host == nullptr)
+    return set_error(BIHIP_EINVAL, "Lorenz-96 needs mean0");
+  bihip_smc2* s = smc2_new(ctx, m, c);
+  s->bank = true;
+  s->N0.assign(c.B, P);
+  s->N = s->N0;
+  bihip_pf_config pc{};
+  pc.P = P;
+  pc.T = c.T;
+  pc.resampler = c.resampler;
+  pc.essRel = c.essRel_inner;
+  pc.t0 = c.t0;
+  pc.dt = c.dt;
+  pc.seed = c.seed;
+  pc.batch = 0;
+  pc.record = 0;
+  pc.use_graph = c.use_graph;
+  auto* slots = new BankInner();
+  auto* prop = new BankInner();
+  s->slots = slots;
+  s->prop = prop;
+  for (BankInner* b : {slots, prop}) {
+    b->P = P;
+    if (mean0_host != nullptr) b->mean0.assign(mean0_host, mean0_host + m->d);
+  }
+  int rc = bihip_pfbank_create(ctx, m, &pc, c.B, s->theta.data(), c.ntheta, &slots->b);
+  if (rc == BIHIP_OK)
+    rc = bihip_pfbank_create(ctx, m, &pc, c.B, s->theta.data(), c.ntheta, &prop->b);
+  if (rc != BIHIP_OK) {
+    bihip_smc2_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return BIHIP_OK;
+}
+
+int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->bank) return set_error(BIHIP_ESTATE, "the surrogate flow runs in filter banks");
+  int rc = bihip_pfbank_set_surrogate(static_cast<BankInner*>(s->slots)->b, mlp, norm);
+  if (rc == BIHIP_OK) rc = bihip_pfbank_set_surrogate(static_cast<BankInner*>(s->prop)->b, mlp, norm);
+  return rc;
+}
+
 int bihip_smc2_destroy(bihip_smc2* s) {
   if (s == nullptr) return BIHIP_OK;
-  bihip_batch_destroy(s->prop);
-  bihip_batch_destroy(s->slots);
+  delete s->prop;
+  delete s->slots;
   delete s;
   return BIHIP_OK;
 }
 
 int bihip_smc2_set_observations(bihip_smc2* s, const double* y_host) {
   if (s == nullptr || y_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
-  int rc = bihip_batch_set_observations(s->slots, y_host);
-  if (rc == BIHIP_OK) rc = bihip_batch_set_observations(s->prop, y_host);
+  int rc = s->slots->set_observations(y_host);
+  if (rc == BIHIP_OK) rc = s->prop->set_observations(y_host);
   s->have_obs = rc == BIHIP_OK;
   return rc;
 }
@@ -330,9 +456,8 @@ int bihip_smc2_run(bihip_smc2* s) {
     }
   }
   s->N = s->N0;  // a second run starts from the sizes given at creation
-  if ((rc = bihip_batch_set_sizes(s->slots, s->N.data())) ||
-      (rc = bihip_batch_set_theta(s->slots, s->theta.data())) ||
-      (rc = bihip_batch_set_stream_base(s->slots, 0u)) || (rc = bihip_batch_init(s->slots)))
+  if ((rc = s->slots->set_sizes(s->N.data())) || (rc = s->slots->set_theta(s->theta.data())) ||
+      (rc = s->slots->set_stream_base(0u)) || (rc = s->slots->init()))
     return rc;
   std::fill(s->lw.begin(), s->lw.end(), 0.0);
   std::fill(s->accepts.begin(), s->accepts.end(), 0);
@@ -348,9 +473,8 @@ int bihip_smc2_run(bihip_smc2* s) {
     s->accepts[k] = nacc;
     if (k == T) break;  // the final move (:301-303) follows the last observation
     // step (:346-368): every inner filter, lw += logIncrement
-    if ((rc = bihip_batch_step(s->slots, 1)) ||
-        (rc = bihip_batch_increments(s->slots, k, li.data())) ||
-        (rc = bihip_batch_outputs(s->slots, nullptr, nullptr, nullptr, nullptr, st.data())))
+    if ((rc = s->slots->step(1)) || (rc = s->slots->increments(k, li.data())) ||
+        (rc = s->slots->outputs(nullptr, st.data())))
       return rc;
     for (int j = 0; j < B; ++j) s->lw[j] += st[j] == BIHIP_OK ? li[j] : -INFINITY;
     // interact (:370-393): outer ESS and marginal likelihood, then resampling
@@ -364,7 +488,7 @@ int bihip_smc2_run(bihip_smc2* s) {
       const double a = host_uniform(c.seed, 0u, (uint32_t)k, BIHIP_TAG_OUTER, 0u);
       if (!outer_ancestors(s->lw, a, as))
         return set_error(BIHIP_DEGENERATE, "outer weights are all zero at step %d", k);
-      if ((rc = bihip_batch_resample_filters(s->slots, as.data()))) return rc;
+      if ((rc = s->slots->resample_filters(as.data()))) return rc;
       std::vector<double> l2(s->lth.size());
       std::vector<int> N2(B);
       for (int j = 0; j < B; ++j) {
@@ -426,7 +550,13 @@ int pmmh_filter(bihip_pmmh* s, const std::vector<double>& lth, uint32_t base,
       (rc = bihip_batch_set_stream_base(s->batch, base)) || (rc = bihip_batch_init(s->batch)) ||
       (rc = bihip_batch_step(s->batch, s->cfg.T)))
     return rc;
-  return filter_logliks(s->batch, B, ll);
+  std::vector<int> st(B);
+  ll.resize(B);
+  if ((rc = bihip_batch_outputs(s->batch, ll.data(), nullptr, nullptr, nullptr, st.data())))
+    return rc;
+  for (int j = 0; j < B; ++j)
+    if (st[j] != BIHIP_OK) ll[j] = -INFINITY;
+  return BIHIP_OK;
 }
 }  // namespace
 

@@ -226,3 +226,99 @@ def test_bank_rejects_what_it_does_not_run(ctx):
         bank.set_surrogate(net, norm)
     bank.close()
     net.close()
+
+
+def test_bank_whole_filter_operations_equal_batch(ctx, orc):
+    """Outer resampling (filter j takes filter anc[j]) and accepted moves
+    (filter j of one bank takes filter j of another where accepted) on a bank
+    equal the same operations on bihip_batch filters of the same size, theta
+    and streams, step for step afterwards."""
+    B, P, T = 5, 1000, 12
+    y = _pz_data(orc, T, seed=3)
+    gm = capi.make_model("pz", precision=capi.F64)
+    th1, th2 = _pz_thetas(B, seed=1), _pz_thetas(B, seed=2)
+    anc = np.array([4, 4, 0, 2, 1], np.int32)
+    acc = np.array([1, 0, 0, 1, 1], np.int32)
+    runs = []
+    for kind in ("bank", "batch"):
+        def make(th, base):
+            if kind == "bank":
+                f = capi.FilterBank(ctx, gm, P, T, 1.0, 8, th, ess_rel=0.6)
+                f.set_stream_base(base)
+            else:
+                f = capi.BatchFilter(ctx, gm, [P] * B, th, T, 1.0, seed=8, ess_rel=0.6, block=256,
+                                     stream_base=base)
+            f.set_observations(y)
+            f.init()
+            return f
+        a, b = make(th1, 0), make(th2, 100)
+        a.step(4)
+        b.step(4)
+        a.resample_filters(anc)
+        a.step(2)
+        b.step(2)
+        a.accept(b, acc)
+        a.step(T - 6)
+        o = a.outputs()
+        runs.append((o["logLikelihood"], np.array([a.increments(k) for k in range(T)])))
+        a.close()
+        b.close()
+    np.testing.assert_allclose(runs[0][0], runs[1][0], rtol=1e-12)
+    np.testing.assert_allclose(runs[0][1], runs[1][1], rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("er_out,nmoves,use_graph", [(0.5, 1, False), (0.9, 2, True)])
+def test_smc2_over_banks_matches_oracle(ctx, orc, er_out, nmoves, use_graph):
+    """SMC^2 whose inner filters are filter banks against the oracle's
+    MarginalSIR with equal inner sizes: same decisions, same evidence."""
+    B, P, T = 16, 1500, 8
+    y = _pz_data(orc, T, seed=13)
+    gm = capi.make_model("pz", precision=capi.F64)
+    pm, ps, pp = [np.log(0.5), np.log(0.2)], [0.3, 0.3], [0.1, 0.1]
+    c = orc.smc2_config(B, T, pm, ps, pp, seed=31, nmoves=nmoves, resampler=capi.SYSTEMATIC,
+                        ess_rel_inner=0.6, ess_rel_outer=er_out)
+    ref = orc.smc2_run(oracle_model(orc, gm), c, [P] * B, y)
+    s = capi.SMC2(ctx, gm, None, T, pm, ps, pp, seed=31, nmoves=nmoves, ess_rel_inner=0.6,
+                  ess_rel_outer=er_out, use_graph=use_graph, bank=(B, P))
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    print("SMC2 over banks: resampled", ref["resampled"].sum(), "accepts",
+          ref["accepts"].sum(), flush=True)
+    assert ref["resampled"].sum() >= 1 and ref["accepts"].sum() >= 1
+    np.testing.assert_array_equal(out["resampled"], ref["resampled"])
+    np.testing.assert_array_equal(out["accepts"], ref["accepts"])
+    np.testing.assert_allclose(out["theta"], ref["theta"], rtol=1e-15)
+    np.testing.assert_allclose(out["lw"], ref["lw"], rtol=1e-9, atol=1e-9)
+    assert out["logZ"] == pytest.approx(ref["logZ"], rel=1e-10, abs=1e-10)
+
+
+def test_smc2_over_surrogate_banks_runs_lorenz96(ctx, torch_gpu):
+    """SMC^2 over the Lorenz-96 forcing with the surrogate flow in every inner
+    filter: the outer loop runs (resamplings and accepted moves) and keeps the
+    forcing inside the prior's support."""
+    B, P, T = 16, 1024, 10
+    gm = capi.make_model("l96", d=40, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                         init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+    mean0 = data.l96_spinup(ctx, d=40)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=5)
+    net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=5)
+    p = net.get_params()
+    p *= 0.1  # a near-zero flow correction: x' close to x
+    net.set_params(p)
+    norm = capi.SurrogateNorm(x_shift=2.3, x_scale=3.6, dx_scale=0.05, par_shift=8.0,
+                              par_scale=1.0)
+    s = capi.SMC2(ctx, gm, None, T, [np.log(8.0)], [0.05], [0.02], dt=DT, seed=3,
+                  ess_rel_outer=0.8, use_graph=True, bank=(B, P), mean0=mean0)
+    s.set_surrogate(net, norm)
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    net.close()
+    print("surrogate SMC2: logZ", out["logZ"], "resampled", out["resampled"].sum(), "accepts",
+          out["accepts"].sum(), flush=True)
+    assert np.isfinite(out["logZ"])
+    assert out["resampled"].sum() >= 1
+    assert (out["theta"] > 6.0).all() and (out["theta"] < 10.5).all()
