@@ -517,7 +517,7 @@ def bench_surrogate(args):
     x_ms, x_n = pf.kernel_time(0)
     pf.close()
     local = surrogate_local_error(ctx, gm, net, norm, y, mean0, P, min(T, 40))
-    batched = surrogate_batched(ctx, gm, net, norm, y, mean0, args)
+    batched = surrogate_batched(ctx, gm, y, mean0, args)
     flops = 2.0 * sum(a * b for a, b in zip(surrogate.DIMS[:-1], surrogate.DIMS[1:])) * P
     achieved = flops * k_n / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     v_sur = P * args.steps * args.seeds / el["surrogate"]
@@ -550,17 +550,27 @@ def bench_surrogate(args):
     ctx.close()
 
 
-def surrogate_batched(ctx, gm, net, norm, y, mean0, args, B=256, N=4096, lone=16):
+def surrogate_batched(ctx, gm, y, mean0, args, B=256, N=4096, lone=16):
     """The batched form: B filters of N particles over forcing values
     F in [7.5, 8.5] (SMC^2's inner filters for the Lorenz-96 forcing) as one
-    filter bank, with the exact DOPRI5 flow and with the surrogate, against the
-    plain GPU filter (bihip_pf, hipGraph replay) run for one forcing after
-    another, timed over `lone` of them.  The exact bank's filters equal the
-    plain filters bit for bit (same log-likelihood error); the surrogate bank
-    carries the surrogate's error per interval."""
-    from libbi_amd import capi
+    filter bank, with the exact DOPRI5 flow and with a surrogate trained over
+    that forcing range, against the plain GPU filter (bihip_pf, hipGraph
+    replay) run for one forcing after another, timed over `lone` of them; then
+    the whole SMC^2 over F on exact and on surrogate banks.  The exact bank's
+    filters equal the plain filters bit for bit (same log-likelihood error);
+    the surrogate's error per interval is measured at two forcings."""
+    from libbi_amd import capi, surrogate
     T = args.warmup + args.steps
     Fs = np.linspace(7.5, 8.5, B)
+    t0 = time.perf_counter()
+    net, norm, rep = surrogate.train_over_forcings(ctx, np.linspace(7.5, 8.5, 5),
+                                                   args.train_pairs // 5, seed=13)
+    train_s = time.perf_counter() - t0
+    local = {}
+    for F in (7.75, 8.25):
+        mF = capi.Model.from_buffer_copy(gm)
+        mF.par[0] = F
+        local[str(F)] = surrogate_local_error(ctx, mF, net, norm, y, mean0, 1 << 18, 20, F=F)
 
     def bank_rate(use_surrogate):
         b = capi.FilterBank(ctx, gm, N, T, DT, 100, Fs, ess_rel=0.5, use_graph=True)
@@ -592,16 +602,51 @@ def surrogate_batched(ctx, gm, net, norm, y, mean0, args, B=256, N=4096, lone=16
         pf.close()
     exact = bank_rate(False)
     sur = bank_rate(True)
+
+    def smc2(use_surrogate, Ts=20, seed=7):
+        # the whole SMC^2 over the forcing (prior log F ~ N(log 8, 0.03^2), one
+        # random-walk move after each outer resampling) on banks; every inner
+        # particle-step it runs, proposals included, over its wall time
+        s = capi.SMC2(ctx, gm, None, Ts, [np.log(8.0)], [0.03], [0.01], dt=DT, seed=seed,
+                      ess_rel_outer=0.5, use_graph=True, bank=(B, N), mean0=mean0)
+        if use_surrogate:
+            s.set_surrogate(net, norm)
+        s.set_observations(y[:Ts])
+        el = _timed(ctx, s.run)
+        out = s.outputs()
+        s.close()
+        moved = [k + 1 for k in range(Ts) if out["resampled"][k]]
+        steps = B * N * (Ts + sum(k for k in moved if k <= Ts))
+        return {"logZ": out["logZ"], "outer_resamplings": int(out["resampled"].sum()),
+                "accepted_moves": int(out["accepts"].sum()), "seconds": el,
+                "inner_particle_steps_per_s": steps / el,
+                "F_mean": float(out["theta"].mean())}
+
+    seeds = (7, 8, 9)
+    runs_ex = [smc2(False, seed=q) for q in seeds]
+    runs_su = [smc2(True, seed=q) for q in seeds]
+    s_ex, s_su = runs_ex[0], runs_su[0]
+    z_ex = np.array([r["logZ"] for r in runs_ex])
+    z_su = np.array([r["logZ"] for r in runs_su])
     return {"filters": B, "particles_per_filter": N,
             "plain_filter_particle_steps_per_s": plain,
             "exact_bank_particle_steps_per_s": exact,
             "surrogate_bank_particle_steps_per_s": sur,
             "exact_bank_vs_plain": exact / plain, "surrogate_bank_vs_plain": sur / plain,
             "error": "exact bank: the plain filter's (bit-identical filters); surrogate bank: "
-                     "loglik_error_per_interval"}
+                     "loglik_error_per_interval",
+            "smc2_exact_banks": s_ex, "smc2_surrogate_banks": s_su,
+            "smc2_surrogate_vs_plain": s_su["inner_particle_steps_per_s"] / plain,
+            "smc2_logZ_exact_by_seed": z_ex.tolist(), "smc2_logZ_surrogate_by_seed": z_su.tolist(),
+            "smc2_logZ_seed_sd_exact": float(z_ex.std(ddof=1)),
+            "smc2_logZ_mean_surrogate_minus_exact": float(z_su.mean() - z_ex.mean()),
+            "smc2_F_mean_exact": float(np.mean([r["F_mean"] for r in runs_ex])),
+            "smc2_F_mean_surrogate": float(np.mean([r["F_mean"] for r in runs_su])),
+            "forcing_range_network": {"training_seconds": train_s, **rep,
+                                      "loglik_error_per_interval": local}}
 
 
-def surrogate_local_error(ctx, gm, net, norm, y, mean0, P, K, seed=8):
+def surrogate_local_error(ctx, gm, net, norm, y, mean0, P, K, seed=8, F=8.0):
     """Log-likelihood error of the surrogate per observation interval: from
     the exact filter's particles before each of the first K steps (resampled
     to equal weights), one interval with the exact FP32 flow and one with the
@@ -627,7 +672,7 @@ def surrogate_local_error(ctx, gm, net, norm, y, mean0, P, K, seed=8):
         Zx = C.clone()
         capi.check(capi.lib().bihip_flow(ctx.h, ctypes.byref(gm), k * DT, (k + 1) * DT, P,
                                          ctypes.c_void_p(Zx.data_ptr()), P, None, None))
-        Zs = net.surrogate_flow(norm, 8.0, C)
+        Zs = net.surrogate_flow(norm, F, C)
         eps = 0.1 * torch.randn(C.shape, generator=g, device="cuda", dtype=torch.float32)
         yk = torch.as_tensor(y[k], device="cuda")
 

@@ -98,6 +98,39 @@ def train(ctx, n_pairs=1 << 22, steps=(4000, 3000, 2000), lrs=(2e-3, 5e-4, 1e-4)
                        "final_minibatch_loss": float(losses[-100:].mean()) if len(losses) else None}
 
 
+def train_over_forcings(ctx, Fs, pairs_per_F, steps=(6000, 6000, 6000, 6000),
+                        lrs=(2e-3, 1e-3, 3e-4, 1e-4), batch=4096, seed=1, chunk=1 << 15):
+    """Train one surrogate for a range of forcings (the network's parameter
+    input), from each forcing's prior predictive; returns (mlp, norm, report).
+    This is the network SMC^2 over F needs: trained at one F it ignores its
+    parameter input."""
+    import torch
+    Fs = [float(F) for F in Fs]
+    pairs = [flow_pairs(ctx, pairs_per_F, F=F, seed=seed + 11 * i, burn=0, chunk=chunk)
+             for i, F in enumerate(Fs)]
+    X = torch.cat([x for x, _ in pairs])
+    Y = torch.cat([y for _, y in pairs])
+    F0 = 0.5 * (min(Fs) + max(Fs))
+    norm = normalisation(X, Y, F0)
+    norm.par_scale = max(0.5 * (max(Fs) - min(Fs)), 1e-3)
+    del X, Y
+    Us, Ts = [], []
+    for (x, y), F in zip(pairs, Fs):
+        U, T = training_tensors(x, y, norm, F)
+        Us.append(U)
+        Ts.append(T)
+    del pairs
+    U, T = torch.cat(Us), torch.cat(Ts)
+    del Us, Ts
+    net = capi.MLP(ctx, DIMS, seed=seed)
+    losses = []
+    for k, (s, lr) in enumerate(zip(steps, lrs)):
+        losses.append(net.train(U, T, batch=batch, steps=s, lr=lr, seed=seed + 7 * k))
+    losses = np.concatenate(losses) if losses else np.zeros(0)
+    return net, norm, {"forcings": Fs, "pairs": int(U.shape[0]), "steps": int(sum(steps)),
+                       "final_minibatch_loss": float(losses[-100:].mean()) if len(losses) else None}
+
+
 def flow_error(ctx, net, norm, X, Y, F=8.0):
     """RMS error of the surrogate flow against the exact flow on pairs (X, Y) (n x d)."""
     Xc = X.T.contiguous()
