@@ -83,7 +83,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="l96",
                     choices=["l96", "l96-bi", "pz", "smc2", "smc2-bi", "copy", "surrogate",
-                             "guided", "valu"])
+                             "guided", "valu", "primitives"])
     ap.add_argument("--thetas", type=int, default=64)
     ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=24000)
@@ -822,6 +822,98 @@ def bench_copy(args):
                       "frac_of_peak": gbs / HBM_PEAK_GBS}), flush=True)
 
 
+def bench_primitives(args, N=1 << 24, reps=20):
+    """Every standalone primitive of the path at N = 2^24, timed by HIP events
+    on the library's stream (the context is bound to torch's current stream):
+    calls per second and algorithmic GB/s against the HBM peak.  Entry points
+    that return a host value (the reductions) synchronise inside each call, so
+    their event time includes that round trip; rocprofv3 --kernel-trace --stats
+    of this workload gives every kernel's own duration."""
+    import ctypes
+    import torch
+    from libbi_amd import capi
+    ctx = capi.Context(0)
+    L = capi.lib()
+    c = lambda t: ctypes.c_void_p(t.data_ptr())
+    dev = "cuda"
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    lws = 2.0 * torch.randn(N, dtype=torch.float64, device=dev, generator=g)
+    Ws = torch.empty(N, dtype=torch.float64, device=dev)
+    Os = torch.empty(N, dtype=torch.int32, device=dev)
+    os_ = torch.empty(N, dtype=torch.int32, device=dev)
+    as_ = torch.empty(N, dtype=torch.int32, device=dev)
+    U = torch.rand(N, dtype=torch.float64, device=dev, generator=g)
+    alphas = torch.rand(N, dtype=torch.float64, device=dev, generator=g)
+    zf = torch.empty(N, dtype=torch.float32, device=dev)
+    ud = torch.empty(N, dtype=torch.float64, device=dev)
+    d_cols = 40
+    X = torch.randn((d_cols, N // 16), dtype=torch.float32, device=dev, generator=g)
+    out1, out2 = ctypes.c_double(), ctypes.c_double()
+    capi.check(L.bihip_sumexpu_inclusive_scan(ctx.h, c(lws), N, c(Ws), None))
+    capi.check(L.bihip_systematic_cumulative_offspring(ctx.h, c(Ws), N, N, 0.37, c(Os)))
+    capi.check(L.bihip_cumulative_offspring_to_ancestors_permute(ctx.h, c(Os), N, c(as_)))
+    # ancestors of a 2^20-particle resampling (for the gather of 40 FP32 columns)
+    Ng = N // 16
+    Wg = torch.empty(Ng, dtype=torch.float64, device=dev)
+    Og = torch.empty(Ng, dtype=torch.int32, device=dev)
+    ag = torch.empty(Ng, dtype=torch.int32, device=dev)
+    capi.check(L.bihip_sumexpu_inclusive_scan(ctx.h, c(lws[:Ng]), Ng, c(Wg), None))
+    capi.check(L.bihip_systematic_cumulative_offspring(ctx.h, c(Wg), Ng, Ng, 0.37, c(Og)))
+    capi.check(L.bihip_cumulative_offspring_to_ancestors_permute(ctx.h, c(Og), Ng, c(ag)))
+    moved = int((ag != torch.arange(Ng, device=dev, dtype=torch.int32)).sum())
+    Xg = X.clone()
+    items = [
+        ("ess_reduce", lambda: L.bihip_ess_reduce(ctx.h, c(lws), N, ctypes.byref(out1),
+                                                  ctypes.byref(out2)), 8 * N, True),
+        ("logsumexp_reduce", lambda: L.bihip_logsumexp_reduce(ctx.h, c(lws), N,
+                                                              ctypes.byref(out1)), 8 * N, True),
+        ("sumexpu_inclusive_scan", lambda: L.bihip_sumexpu_inclusive_scan(
+            ctx.h, c(lws), N, c(Ws), None), 16 * N, False),
+        ("systematic_cumulative_offspring", lambda: L.bihip_systematic_cumulative_offspring(
+            ctx.h, c(Ws), N, N, 0.37, c(Os)), 12 * N, False),
+        ("stratified_cumulative_offspring", lambda: L.bihip_stratified_cumulative_offspring(
+            ctx.h, c(Ws), N, N, c(alphas), c(Os)), 20 * N, False),
+        ("multinomial_ancestors", lambda: L.bihip_multinomial_ancestors(
+            ctx.h, c(Ws), N, N, c(U), c(as_)), 20 * N, False),
+        ("cumulative_offspring_to_ancestors_permute",
+         lambda: L.bihip_cumulative_offspring_to_ancestors_permute(ctx.h, c(Os), N, c(as_)),
+         8 * N, False),
+        ("gather_rows (2^20 x 40 FP32, in place)", lambda: L.bihip_gather_rows(
+            ctx.h, c(ag), Ng, d_cols, c(Xg), Ng, 4), 4 * Ng + 2 * 4 * d_cols * moved, False),
+        ("normals (FP32)", lambda: L.bihip_normals(ctx.h, 5, 0, 1, 2, 0, N, capi.F32, c(zf)),
+         4 * N, False),
+        ("uniforms", lambda: L.bihip_uniforms(ctx.h, 5, 1, 3, 0, N, c(ud)), 8 * N, False),
+    ]
+    res = {}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, fn, nbytes, host in items:
+        for _ in range(3):
+            capi.check(fn())
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            capi.check(fn())
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms_per_call": ms, "algorithmic_bytes": nbytes,
+                     "GB_s": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "host_round_trip_in_call": host}
+    ctx.close()
+    key = "systematic_cumulative_offspring"
+    print(json.dumps({"metric": "GB/s of the standalone primitives", "value": res[key]["GB_s"],
+                      "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 3,
+                      "ms_per_step": res[key]["ms_per_call"], "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic (log-weights 2 N(0,1), uniforms, normals)",
+                      "config": {"workload": f"standalone primitives at N = 2^24 "
+                                             f"(value: {key})", "N": N,
+                                 "gather_moved_rows": moved},
+                      "primitives": res}), flush=True)
+
+
 def valu_probe(kinds=(0, 1, 2), trips=4096, reps=3):
     """Wave-level VALU instructions per second of the issue-rate probe
     (libbi_amd/lib/libbiprobe.so): eight independent chains per lane, eight
@@ -883,6 +975,8 @@ def main():
         return bench_surrogate(args)
     if args.workload == "guided":
         return bench_guided(args)
+    if args.workload == "primitives":
+        return bench_primitives(args)
     if args.workload == "valu":
         return bench_valu(args)
     import torch

@@ -96,10 +96,10 @@ struct ScanWorkspace {
   int max_tiles;
 };
 // Words per tile: words [0, 4 max_tiles) hold the single-pass scan's
-// epoch-tagged aggregates; [4 max_tiles, 8 max_tiles) the tiled resampling
-// chain's plain tile sums, kept apart so that no tile sum can pass for a
-// published aggregate.
-constexpr int SCAN_WORDS = 8;
+// epoch-tagged tile aggregates and [4 max_tiles, 8 max_tiles) its inclusive
+// prefixes; [8 max_tiles, 12 max_tiles) the tiled resampling chain's plain
+// tile sums, kept apart so that no tile sum can pass for a published value.
+constexpr int SCAN_WORDS = 12;
 
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -235,6 +235,11 @@ int launch_pf_correct_finalize(const double* partials, int nb, int P, double ess
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,
                        double* Ws, ScanWorkspace ws, hipStream_t s);
 int launch_count_scan(const int* os, int P, int* Os, ScanWorkspace ws, hipStream_t s);
+// The same inclusive scan of nanexp(lws - *mx) by the tiled chain's kernels.
+int launch_sumexp_scan_tiles(const double* lws, int P, const double* mx, double* Ws,
+                             ScanWorkspace ws, hipStream_t s);
+// Tiles that are certainly resident together (two per CU).
+int scan_resident_tiles();
 int scan_tiles(int P);
 
 // ---- resample.hip

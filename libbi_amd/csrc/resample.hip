@@ -153,7 +153,7 @@ struct TileWork {
 
 static TileWork tile_work(const ScanWorkspace& ws) {
   const size_t mt = (size_t)ws.max_tiles;
-  uint64_t* base = ws.words + 4 * mt;
+  uint64_t* base = ws.words + 8 * mt;
   TileWork w;
   w.wsum = reinterpret_cast<u128*>(base);  // 2 mt words
   w.efsum = base + 2 * mt;                 // mt words
@@ -550,6 +550,32 @@ __global__ __launch_bounds__(RS_BLOCK) void k_place_free(const uint32_t* E, cons
 }
 
 // (e, f) tile sums in place -> E, the free-slot list and the ancestors.
+// Ws = the inclusive scan of nanexp(lws - *mx) over P elements, by the tiled
+// chain's kernels: exact 128-bit tile sums, (above SMALL_TILES tiles) their
+// exclusive scan by one block, then each tile's inclusive prefixes.  The
+// sums are exact integers, so the result equals the single-pass scan's bit
+// for bit; no tile waits on another (no tickets), whatever the grid size.
+int launch_sumexp_scan_tiles(const double* lws, int P, const double* mx, double* Ws,
+                             ScanWorkspace ws, hipStream_t s) {
+  if (P <= 0) return BIHIP_OK;
+  const int nt = ceil_div(P, SCAN_TILE);
+  if (nt > ws.max_tiles) return set_error(BIHIP_EINVAL, "scan workspace too small");
+  const TileWork w = tile_work(ws);
+  hipLaunchKernelGGL(k_weight_sums, dim3(nt), dim3(SCAN_BLOCK), 0, s, lws, mx, P, w.wsum,
+                     (const PFDevice*)nullptr, BankStrides{});
+  BIHIP_LAUNCH_CHECK();
+  if (nt > SMALL_TILES) {
+    hipLaunchKernelGGL(k_tiles_exclusive<u128>, dim3(1), dim3(SCAN_BLOCK), 0, s, w.wsum, nt,
+                       w.wtotal, (const PFDevice*)nullptr, BankStrides{});
+    BIHIP_LAUNCH_CHECK();
+  }
+  OffspringTileArgs a{lws, mx, P, P, nt, w.wsum, w.wtotal, w.efsum, 0, 0,
+                      nullptr, nullptr, Ws, nullptr, BankStrides{}};
+  hipLaunchKernelGGL(k_offspring_tiles<BIHIP_MULTINOMIAL>, dim3(nt), dim3(SCAN_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
 // banks filters at the strides bk (1 and zero strides: a lone filter).
 static int place_from_sums(const int* in, bool cumulative, int P, int nt, int* as, uint64_t* EF,
                            const TileWork& w, const PFDevice* gate, hipStream_t s, int banks = 1,

@@ -231,47 +231,59 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(Loader load, Writer wr
   V agg;
   const V texcl = block_exclusive_scan(total, lds, agg);
 
-  if (threadIdx.x == 0) publish(ws.words + (size_t)NC * tile, epoch, agg);
-
-  // prefix = sum of the aggregates of tiles [0, tile) in a fixed order:
-  // thread t adds tiles t, t+256, t+512, ...; the words of up to four tiles
-  // are loaded at once, and only words not yet published are polled again
-  V part = V(0);
-  for (uint32_t j0 = threadIdx.x; j0 < tile; j0 += 4 * SCAN_BLOCK) {
-    uint64_t wd[4][NC];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t j = j0 + q * SCAN_BLOCK;
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        wd[q][c] = j < tile ? __hip_atomic_load(ws.words + (size_t)NC * j + c, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT)
-                            : ((uint64_t)epoch << 32);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t j = j0 + q * SCAN_BLOCK;
-      if (j >= tile) break;
-      uint32_t c32[NC];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        uint64_t x = wd[q][c];
+  // publish the tile's aggregate at once; then decoupled look-back by wave
+  // 0: lane l reads predecessor hi - 1 - l, taking its inclusive prefix when
+  // that is published and its aggregate otherwise (waiting, bounded, for one
+  // of the two); the window ends at the nearest inclusive prefix, else moves
+  // 64 tiles back.  The sums are exact integers, so the order of the
+  // additions cannot change the result.
+  uint64_t* agg_words = ws.words;                              // NC words per tile
+  uint64_t* incl_words = ws.words + 4 * (size_t)ws.max_tiles;  // NC words per tile
+  if (threadIdx.x == 0) publish(agg_words + (size_t)NC * tile, epoch, agg);
+  __shared__ V s_prefix;
+  if (threadIdx.x < WAVE) {
+    const int lane = threadIdx.x;
+    V run = V(0);
+    long hi = tile;
+    while (hi > 0) {
+      const long j = hi - 1 - lane;
+      bool incl = j < 0;  // before tile 0: an inclusive prefix of zero
+      V val = V(0);
+      if (j >= 0) {
+        const uint64_t* iw = incl_words + (size_t)NC * j;
+        const uint64_t* aw = agg_words + (size_t)NC * j;
         uint32_t spins = 0;
-        while ((x >> 32) != epoch) {
+        for (;;) {
+          if ((__hip_atomic_load(iw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) == epoch) {
+            incl = true;
+            break;
+          }
+          if ((__hip_atomic_load(aw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) == epoch)
+            break;
           if (++spins > SCAN_SPIN_LIMIT) {
             __hip_atomic_store(&ws.state->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          x = __hip_atomic_load(ws.words + (size_t)NC * j + c, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
         }
-        c32[c] = (uint32_t)x;
+        val = await_value<V>(const_cast<uint64_t*>(incl ? iw : aw), epoch, &ws.state->error);
       }
-      part = part + Chunks<V>::join(c32);
+      const uint64_t found = __ballot(incl);
+      const int stop = found != 0 ? __ffsll((unsigned long long)found) - 1 : WAVE - 1;
+      V mine = lane <= stop ? val : V(0);
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) mine = mine + shfl_xor_v(mine, o);
+      run = run + mine;
+      if (found != 0) break;
+      hi -= WAVE;
+    }
+    if (lane == 0) {
+      s_prefix = run;
+      publish(incl_words + (size_t)NC * tile, epoch, run + agg);
     }
   }
-  const V prefix = block_sum_v(part, lds);
+  __syncthreads();
+  const V prefix = s_prefix;
 
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {

@@ -228,7 +228,7 @@ def test_reductions(ctx, orc, torch_gpu, P, kind):
     assert mx.value == orc.max_reduce(w)
 
 
-@pytest.mark.parametrize("P", [1, 7, 2048, 2049, 65537, 1 << 20])
+@pytest.mark.parametrize("P", [1, 7, 2048, 2049, 65537, 1 << 20, (1 << 20) + 5, (1 << 22) + 3])
 def test_sumexp_scan_parity_and_determinism(ctx, orc, torch_gpu, P):
     rng = np.random.default_rng(P + 1)
     w = rng.normal(scale=2.0, size=P)
@@ -245,6 +245,22 @@ def test_sumexp_scan_parity_and_determinism(ctx, orc, torch_gpu, P):
     for _ in range(3):  # bit-identical on every run
         capi.check(L().bihip_sumexpu_inclusive_scan(ctx.h, _c(d), P, _c(Ws), None))
         np.testing.assert_array_equal(host(Ws), got)
+
+
+def test_sumexp_scan_paths_agree_bit_for_bit(ctx, torch_gpu):
+    """A grid resident at once runs the single-pass look-back scan, a larger
+    one the tiled reduce-then-scan; with the maximum in the first 2^20
+    elements both scan those elements to the same bits."""
+    P, Q = (1 << 22) + 3, 1 << 20
+    rng = np.random.default_rng(77)
+    w = rng.normal(scale=2.0, size=P)
+    w[0] = w.max() + 1.0
+    d = dev(w)
+    full = empty(P, torch_gpu.float64)
+    head = empty(Q, torch_gpu.float64)
+    capi.check(L().bihip_sumexpu_inclusive_scan(ctx.h, _c(d), P, _c(full), None))
+    capi.check(L().bihip_sumexpu_inclusive_scan(ctx.h, _c(d), Q, _c(head), None))
+    np.testing.assert_array_equal(host(full)[:Q], host(head))
 
 
 # ---------------------------------------------------------------- resamplers (bit-exact)
