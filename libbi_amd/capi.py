@@ -64,7 +64,7 @@ EXPORTS = [
     "bihip_pmmh_create", "bihip_pmmh_destroy", "bihip_pmmh_set_observations", "bihip_pmmh_run",
     "bihip_pmmh_outputs", "bihip_pf_paths", "bihip_pf_sample_path",
     "bihip_pfout_create", "bihip_pfout_write_time", "bihip_pfout_write_state",
-    "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood",
+    "bihip_pfout_write_log_weights", "bihip_pfout_write_log_likelihood", "bihip_pfout_write_noise",
     "bihip_pfout_write_clock", "bihip_pfout_close", "bihip_pf_write_netcdf",
     "bihip_pf_cache_nodes",
     "bihip_pf_set_schedule", "bihip_pf_set_inputs",
@@ -265,6 +265,7 @@ def _declare(L):
         "bihip_pfout_write_time": [P, i, d],
         "bihip_pfout_write_state": [P, i, P, i, pi],
         "bihip_pfout_write_log_weights": [P, i, pd],
+        "bihip_pfout_write_noise": [P, i, P, i],
         "bihip_pfout_write_log_likelihood": [P, d],
         "bihip_pfout_write_clock": [P, d],
         "bihip_pfout_close": [P],
@@ -1189,6 +1190,13 @@ class PFOutput:
         check(lib().bihip_pfout_write_state(self.h, k, X.ctypes.data_as(ctypes.c_void_p),
                                             X.shape[-1],
                                             a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def write_noise(self, k, Z):
+        """Z: noise_width x ld (ld >= np) noise variables, by the model's precision."""
+        dt = np.float32 if self.model.precision == F32 else np.float64
+        Z = np.ascontiguousarray(Z, dt)
+        check(lib().bihip_pfout_write_noise(self.h, k, Z.ctypes.data_as(ctypes.c_void_p),
+                                            Z.shape[-1]))
 
     def write_log_weights(self, k, lws):
         lw = np.ascontiguousarray(lws, np.float64)

@@ -616,6 +616,8 @@ struct bihip_pf {
   int* hist_anc = nullptr;
   void* hist_X = nullptr;  // record = 2: the state at every time, (T + 1) x d x ld
   double* hist_lw = nullptr;  // record = 2: the log-weights at every time, (T + 1) x P
+  void* hist_noise = nullptr;  // record = 2: the noise variables, (T + 1) x noise_width x ld
+  int noise_width = 0;
   ScanWorkspace ws{};
   int k = 0;
   bool ready = false, have_obs = false;
@@ -749,6 +751,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   T.mask = pf->obs_mask;
   T.inputs = pf->inputs;
   T.one_delta = pf->one_delta ? 1 : 0;
+  T.noise_hist = pf->hist_noise;
+  T.noise_width = pf->noise_width;
   if (pf->guided) {
     // exact flow of the gathered particles (no noise, no correction) into Xout,
     // then the proposal and the corrected weights in place
@@ -914,6 +918,12 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
     PF_ALLOC(hx, (size_t)(cfg->T + 1) * m->d * pf->ld * pf->elem);
     pf->hist_X = hx;
     PF_ALLOC(pf->hist_lw, (size_t)(cfg->T + 1) * P);
+    pf->noise_width = noise_width_of(m);
+    if (pf->noise_width > 0) {
+      uint8_t* hn = nullptr;
+      PF_ALLOC(hn, (size_t)(cfg->T + 1) * pf->noise_width * pf->ld * pf->elem);
+      pf->hist_noise = hn;
+    }
   }
 #undef PF_ALLOC
   if (cfg->record == 3) {
@@ -953,6 +963,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->partials);
   dfree(pf->hist_X);
   dfree(pf->hist_lw);
+  dfree(pf->hist_noise);
   dfree(pf->times);
   dfree(pf->obs_mask);
   dfree(pf->inputs);
@@ -1000,6 +1011,9 @@ int bihip_pf_init(bihip_pf* pf, const double* mean0_host) {
        (rc = launch_record_state(pf->lws, pf->cfg.P, 1, pf->cfg.P, 8, nullptr, pf->hist_lw, 0,
                                  s))))
     return rc;
+  if (pf->hist_noise != nullptr)
+    BIHIP_HIP(hipMemsetAsync(pf->hist_noise, 0,
+                             (size_t)(pf->cfg.T + 1) * pf->noise_width * pf->ld * pf->elem, s));
   if (pf->use_cache) {
     if ((rc = ancestry_init(&pf->cache, pf->X[0], pf->ld, s))) return rc;
     pf->cache.live_ub = pf->cfg.P;
@@ -1401,6 +1415,8 @@ int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {
   const int P = pf->cfg.P, nr = pf->k + 1;
   const size_t xbytes = (size_t)pf->model.d * pf->ld * pf->elem;
   std::vector<uint8_t> X(xbytes);
+  const size_t zbytes = (size_t)pf->noise_width * pf->ld * pf->elem;
+  std::vector<uint8_t> Z(zbytes);
   std::vector<double> lw(P);
   std::vector<int> as(P);
   bihip_pfout* o = nullptr;
@@ -1414,6 +1430,9 @@ int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {
     if (e == hipSuccess && k > 0)
       e = hipMemcpyAsync(as.data(), pf->hist_anc + (size_t)(k - 1) * P, sizeof(int) * P,
                          hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && pf->hist_noise != nullptr)
+      e = hipMemcpyAsync(Z.data(), static_cast<const uint8_t*>(pf->hist_noise) + k * zbytes,
+                         zbytes, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       rc = set_error(BIHIP_EHIP, "copy of output %d: %s", k, hipGetErrorString(e));
@@ -1424,6 +1443,7 @@ int bihip_pf_write_netcdf(bihip_pf* pf, const char* path) {
     const double tk = pf->times != nullptr ? pf->times_host[k] : pf->cfg.t0 + k * pf->cfg.dt;
     if ((rc = bihip_pfout_write_time(o, k, tk)) ||
         (rc = bihip_pfout_write_state(o, k, X.data(), pf->ld, as.data())) ||
+        (pf->hist_noise != nullptr && (rc = bihip_pfout_write_noise(o, k, Z.data(), pf->ld))) ||
         (rc = bihip_pfout_write_log_weights(o, k, lw.data())))
       break;
   }

@@ -180,7 +180,14 @@ struct TransLaunch {
   int banks;
   BankStrides bk;
   const double* bank_par;
+  // with dev (nullable): a (T + 1) x noise_width x ld history of the noise
+  // variables (in the model's precision) that the step fills at slot k + 1
+  void* noise_hist;
+  int noise_width;
 };
+// Noise variables per particle of a built-in model (PZ: alpha; Lorenz-96:
+// w[d]; LG: w), 0 for a generated one.
+int noise_width_of(const bihip_model* m);
 int transition_blocks(const bihip_model* m, int P);
 // A generated model whose steps [t0 + k dt, t0 + (k+1) dt], k < steps, are each
 // one sub-interval of its delta starting at a multiple of it.

@@ -97,16 +97,23 @@ __device__ __forceinline__ void noise_first(ModelPZ<T>& m, uint64_t seed, uint32
   m.alpha = (T)m.mu + (T)m.sigma * z[0];
 }
 
+// zout (nullable): the standard normals w of the lane's slice, the model
+// file's noise variable (x <- x + noise_sd w), at zout[(lo + i) * ld + p].
 template <class M>
 __device__ __forceinline__ void noise_after(const M& m, uint64_t seed, uint32_t batch, int p,
                                             int lo, uint32_t step,
-                                            typename M::real_t (&x)[M::DL]) {
+                                            typename M::real_t (&x)[M::DL],
+                                            typename M::real_t* zout = nullptr, int ld = 0) {
   using T = typename M::real_t;
   T z[M::DL];
   slice_normals<T, M::DL>(seed, (uint32_t)p, step, TAG_NOISE, batch, lo, z);
   const T sd = (T)m.noise_sd;
 #pragma unroll
   for (int i = 0; i < M::DL; ++i) x[i] = x[i] + sd * z[i];
+  if (zout != nullptr) {
+#pragma unroll
+    for (int i = 0; i < M::DL; ++i) zout[(size_t)(lo + i) * ld + p] = z[i];
+  }
 }
 
 template <class T>

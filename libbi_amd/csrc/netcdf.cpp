@@ -4,7 +4,8 @@
 //   SimulatorNetCDFBuffer::createVar    share/src/bi/netcdf/SimulatorNetCDFBuffer.cpp:167-215
 //   ParticleFilterBuffer::write / writeT share/src/bi/buffer/ParticleFilterBuffer.hpp:69-82
 // Dimensions nr (output times), the model's dimensions, np (particles);
-// variables time(nr), each state variable (nr, [model dims reversed], np),
+// variables time(nr), each noise variable and each state variable (nr, [model
+// dims reversed], np) -- the R_VAR and D_VAR types that create() writes --,
 // ancestor(nr, np) int, logweight(nr, np), loglikelihood(), clock(); global
 // attributes libbi_schema = "ParticleFilter", libbi_schema_version = 1,
 // libbi_version.
@@ -79,6 +80,9 @@ struct bihip_pfout {
   std::vector<int> v_state;           // one per state variable
   std::vector<uint64_t> state_width;  // components per state variable and particle
   std::vector<int> state_first;       // first state component of each variable
+  std::vector<int> v_noise;           // the same for the noise variables
+  std::vector<uint64_t> noise_width;
+  std::vector<int> noise_first;
   std::vector<uint8_t> scratch;
 };
 
@@ -123,6 +127,49 @@ std::vector<StateVar> state_vars(const bihip_model* m) {
   }
 }
 
+// The noise variables of a built-in model (the model files' declarations:
+// PZ.bi's alpha, Lorenz96.bi's w[n], LinearGaussian.bi's w); a generated
+// model's noise is not recorded, so none are written for it.
+std::vector<StateVar> noise_vars(const bihip_model* m) {
+  if (bihip::generated_info(m->model) != nullptr) return {};
+  switch (m->model) {
+    case BIHIP_MODEL_PZ:
+      return {{"alpha", "", 1}};
+    case BIHIP_MODEL_L96:
+      return {{"w", "n", m->d}};
+    default:
+      return {{"w", "", 1}};
+  }
+}
+
+// Variables v (each w components of ld-strided rows from `first` on) of
+// output k, big-endian.
+int write_rows(bihip_pfout* o, int k, const void* X_host, int ld, const std::vector<int>& vars,
+               const std::vector<uint64_t>& width, const std::vector<int>& first) {
+  const int esz = type_size(o->state_type);
+  const uint64_t np = (uint64_t)o->np;
+  for (size_t v = 0; v < vars.size(); ++v) {
+    const uint64_t w = width[v];
+    o->scratch.resize(w * np * esz);
+    for (uint64_t c = 0; c < w; ++c) {
+      const uint64_t row = (uint64_t)first[v] + c;
+      uint8_t* dst = o->scratch.data() + c * np * esz;
+      if (esz == 4) {
+        const float* src = static_cast<const float*>(X_host) + row * (uint64_t)ld;
+        for (uint64_t p = 0; p < np; ++p) put_be<float>(dst + 4 * p, src[p]);
+      } else {
+        const double* src = static_cast<const double*>(X_host) + row * (uint64_t)ld;
+        for (uint64_t p = 0; p < np; ++p) put_be<double>(dst + 8 * p, src[p]);
+      }
+    }
+    const Var& var = o->vars[vars[v]];
+    int rc = write_at(o, var.begin + (uint64_t)k * w * np * esz, o->scratch.data(),
+                      o->scratch.size());
+    if (rc) return rc;
+  }
+  return BIHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -139,10 +186,13 @@ int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
   o->d = m->d;
   o->state_type = m->precision == BIHIP_F32 ? NC_FLOAT : NC_DOUBLE;
   const std::vector<StateVar> sv = state_vars(m);
+  const std::vector<StateVar> nv = noise_vars(m);
+  std::vector<StateVar> all = sv;
+  all.insert(all.end(), nv.begin(), nv.end());
 
   // dimensions: nr, the model's, np
   o->dims.push_back({"nr", (uint64_t)nr});
-  for (const StateVar& s : sv) {  // each model dimension once (variables may share one)
+  for (const StateVar& s : all) {  // each model dimension once (variables may share one)
     if (s.dim[0] == '\0') continue;
     bool seen = false;
     for (const auto& d : o->dims) seen = seen || d.first == s.dim;
@@ -172,6 +222,16 @@ int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
   o->v_ll = add("loglikelihood", {}, NC_DOUBLE);
   o->v_anc = add("ancestor", {nrDim, npDim}, NC_INT);
   o->v_lw = add("logweight", {nrDim, npDim}, NC_DOUBLE);
+  int nfirst = 0;
+  for (const StateVar& s : nv) {
+    std::vector<int> dims{nrDim};
+    if (s.dim[0] != '\0') dims.push_back(dim_id(s.dim));
+    dims.push_back(npDim);
+    o->v_noise.push_back(add(s.name, dims, o->state_type));
+    o->noise_width.push_back((uint64_t)s.width);
+    o->noise_first.push_back(nfirst);
+    nfirst += s.width;
+  }
   int first = 0;
   for (const StateVar& s : sv) {
     std::vector<int> dims{nrDim};
@@ -281,30 +341,22 @@ int bihip_pfout_write_state(bihip_pfout* o, int k, const void* X_host, int ld,
   if (o == nullptr || k < 0 || k >= o->nr) return set_error(BIHIP_EINVAL, "bad output index");
   if (X_host == nullptr || as_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
   if (ld < o->np) return set_error(BIHIP_EINVAL, "ld < np");
-  const int esz = type_size(o->state_type);
   const uint64_t np = (uint64_t)o->np;
-  for (size_t v = 0; v < o->v_state.size(); ++v) {
-    const uint64_t w = o->state_width[v];
-    o->scratch.resize(w * np * esz);
-    for (uint64_t c = 0; c < w; ++c) {
-      const uint64_t row = (uint64_t)o->state_first[v] + c;
-      uint8_t* dst = o->scratch.data() + c * np * esz;
-      if (esz == 4) {
-        const float* src = static_cast<const float*>(X_host) + row * (uint64_t)ld;
-        for (uint64_t p = 0; p < np; ++p) put_be<float>(dst + 4 * p, src[p]);
-      } else {
-        const double* src = static_cast<const double*>(X_host) + row * (uint64_t)ld;
-        for (uint64_t p = 0; p < np; ++p) put_be<double>(dst + 8 * p, src[p]);
-      }
-    }
-    const Var& var = o->vars[o->v_state[v]];
-    int rc = write_at(o, var.begin + (uint64_t)k * w * np * esz, o->scratch.data(),
-                      o->scratch.size());
-    if (rc) return rc;
-  }
+  int rc = write_rows(o, k, X_host, ld, o->v_state, o->state_width, o->state_first);
+  if (rc) return rc;
   o->scratch.resize(np * 4);
   for (uint64_t p = 0; p < np; ++p) put_be<int32_t>(o->scratch.data() + 4 * p, as_host[p]);
   return write_at(o, o->vars[o->v_anc].begin + (uint64_t)k * np * 4, o->scratch.data(), np * 4);
+}
+
+// SimulatorNetCDFBuffer::writeState's noise part (R_VAR, SimulatorNetCDFBuffer.hpp:341-352):
+// the noise variables drawn in the step to output k, in the state's layout.
+int bihip_pfout_write_noise(bihip_pfout* o, int k, const void* Z_host, int ld) {
+  if (o == nullptr || k < 0 || k >= o->nr) return set_error(BIHIP_EINVAL, "bad output index");
+  if (Z_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
+  if (o->v_noise.empty()) return set_error(BIHIP_EINVAL, "the model has no recorded noise variables");
+  if (ld < o->np) return set_error(BIHIP_EINVAL, "ld < np");
+  return write_rows(o, k, Z_host, ld, o->v_noise, o->noise_width, o->noise_first);
 }
 
 // ParticleFilterNetCDFBuffer::writeLogWeights

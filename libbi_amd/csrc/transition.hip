@@ -43,6 +43,9 @@ struct TransArgs {
   const int* imp_r;     // with dev: imposed resampling flags per step (nullable)
   BankStrides bk;       // BANK: strides of the filters (filter f = blockIdx.y)
   const double* bank_par;  // BANK: BANK_ROW parameters per filter
+  void* noise_hist;     // SCHED, with dev (nullable): the step's noise variables into
+                        // slot k + 1 of a (T + 1) x noise_width x ld history
+  int noise_width;
 };
 
 // ---------------------------------------------------------------- the kernel
@@ -145,9 +148,16 @@ void k_transition(TransArgs a) {
       n = m.template transition<INTEG, ONE>(grp, x, t1, t2, a.m, a.seed, a.batch, (uint32_t)p,
                                             step);
     } else {
+      T* zout = nullptr;  // the noise variables of this step (recorded histories only)
+      if constexpr (SCHED) {
+        if (a.noise_hist != nullptr && a.dev != nullptr && !a.flow_only)
+          zout = static_cast<T*>(a.noise_hist) +
+                 (size_t)(a.dev->k + 1) * (size_t)a.noise_width * (size_t)a.ld;
+      }
       if constexpr (M::NOISE_FIRST) {
         if (!a.flow_only) {
           noise_first(m, a.seed, a.batch, p, step);
+          if (SCHED && zout != nullptr && grp.g == 0) zout[p] = m.alpha;  // PZ's alpha
         } else {
           m.alpha = a.aux != nullptr ? (T)a.aux[p] : (T)m.mu;
         }
@@ -158,7 +168,7 @@ void k_transition(TransArgs a) {
         n = integrate_dopri5(m, grp, x, t1, t2, a.m);
       }
       if constexpr (!M::NOISE_FIRST) {
-        if (!a.flow_only) noise_after(m, a.seed, a.batch, p, lo, step, x);
+        if (!a.flow_only) noise_after(m, a.seed, a.batch, p, lo, step, x, zout, a.ld);
       }
     }
 
@@ -401,7 +411,7 @@ struct TransitionRunner {
   template <class M>
   int run() {
     const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
-                                            a.inputs != nullptr);
+                                            a.inputs != nullptr || a.noise_hist != nullptr);
     if (banks > 1) {
       if constexpr (is_generated<M>::value) {
         return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
@@ -482,6 +492,8 @@ int launch_transition(const TransLaunch& L, hipStream_t s) {
   r.a.imp_r = L.imp_r;
   r.a.bk = L.bk;
   r.a.bank_par = L.bank_par;
+  r.a.noise_hist = L.noise_hist;
+  r.a.noise_width = L.noise_width;
   r.banks = L.banks;
   r.blocks = transition_blocks(L.model, L.P);
   r.s = s;
@@ -528,6 +540,15 @@ int launch_init_bank(const bihip_model* m, uint64_t seed, uint32_t batch, int P,
   InitRunner r{to_args(m), seed, batch, P, ld, transition_blocks(m, P), X, mean0_dev, s,
                banks, bank_par, x};
   return with_model(m, r);
+}
+
+int noise_width_of(const bihip_model* m) {
+  if (generated_info(m->model) != nullptr) return 0;
+  switch (m->model) {
+    case BIHIP_MODEL_PZ: return 1;    // alpha
+    case BIHIP_MODEL_L96: return m->d;  // w[n]
+    default: return 1;                // w
+  }
 }
 
 void bank_row(const bihip_model* m, double* row) {

@@ -264,6 +264,48 @@ def test_netcdf_output_of_the_recorded_filter(ctx, orc, tmp_path):
         lme = m + np.log(np.exp(lw[k] - m).sum()) - np.log(P)
         assert lme == pytest.approx(running[k - 1], rel=1e-12, abs=1e-9)
     assert float(v["loglikelihood"]) == logZ
+    # the noise variable alpha (R_VAR): zero before the first step, then the
+    # draw alpha = mu + sigma z of each step, z the particle's first noise normal
+    np.testing.assert_array_equal(v["alpha"][0], np.zeros(P))
+    mu, sigma = gm.par[0], gm.par[1]
+    for k in (1, 4, T):
+        z = np.array([orc.normals(seed, p, k, orc.TAG_NOISE, 1, bits=32)[0] for p in range(P)])
+        np.testing.assert_allclose(v["alpha"][k], mu + sigma * z, rtol=1e-14, atol=1e-14)
+
+
+def test_recorded_noise_of_lorenz96_and_unchanged_results(ctx, orc, tmp_path):
+    """Lorenz-96 d=8 FP64 with record="paths": the NetCDF file's w[n] holds each
+    step's standard normals (x <- x + noise_sd w), and recording changes no
+    result of the filter (the same increments and states, bit for bit)."""
+    from scipy.io import netcdf_file
+    T, P, seed, dt = 6, 300, 5, 0.05
+    gm = capi.make_model("l96", d=8, precision=capi.F64)
+    om = oracle_model(orc, gm)
+    mean0 = l96_spun_up(orc, d=8)
+    x0 = orc.init_samples(om, 3, 1, mean0=mean0)[:, 0]
+    _, y = orc.simulate(om, 3, T, dt, x0)
+    runs = {}
+    for rec in (False, "paths"):
+        pf = capi.ParticleFilter(ctx, gm, P, T, dt, seed, ess_rel=0.5, record=rec)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(T)
+        if rec:
+            pf.write_netcdf(tmp_path / "l96.nc")
+        X, _, lws = pf.state()
+        runs[rec] = (pf.outputs()["logIncrements"], X[:, :P].cpu().numpy(), lws.cpu().numpy())
+        pf.close()
+    for a, b in zip(runs[False], runs["paths"]):
+        np.testing.assert_array_equal(a, b)
+    f = netcdf_file(str(tmp_path / "l96.nc"), "r", mmap=False)
+    assert f.variables["w"].dimensions == ("nr", "n", "np")
+    w = np.array(f.variables["w"].data)
+    f.close()
+    np.testing.assert_array_equal(w[0], np.zeros((8, P)))
+    for k in (1, T):
+        for p in (0, 1, P // 2, P - 1):
+            z = orc.normals(seed, p, k, orc.TAG_NOISE, 8, bits=32)
+            np.testing.assert_allclose(w[k][:, p], z, rtol=1e-14, atol=1e-14)
 
 
 def test_netcdf_output_needs_recorded_paths(ctx, orc, tmp_path):

@@ -20,10 +20,10 @@ def _read(path):
     return out
 
 
-def _fill(o, model, nr, np_, ld, rng):
+def _fill(o, model, nr, np_, ld, rng, nw=0):
     d = model.d
     dt = np.float32 if model.precision == capi.F32 else np.float64
-    Xs, As, Ls, ts = [], [], [], []
+    Xs, As, Ls, ts, Zs = [], [], [], [], []
     for k in range(nr):
         X = rng.normal(size=(d, ld)).astype(dt)
         a = rng.integers(0, np_, size=np_).astype(np.int32)
@@ -32,13 +32,18 @@ def _fill(o, model, nr, np_, ld, rng):
         o.write_time(k, t)
         o.write_state(k, X, a)
         o.write_log_weights(k, lw)
+        if nw:
+            Z = rng.normal(size=(nw, ld)).astype(dt)
+            o.write_noise(k, Z)
+            Zs.append(Z[:, :np_])
         Xs.append(X[:, :np_])
         As.append(a)
         Ls.append(lw)
         ts.append(t)
     o.write_log_likelihood(-123.456)
     o.write_clock(789.0)
-    return np.array(Xs), np.array(As), np.array(Ls), np.array(ts)
+    out = np.array(Xs), np.array(As), np.array(Ls), np.array(ts)
+    return out + (np.array(Zs),) if nw else out
 
 
 @pytest.mark.parametrize("prec", [capi.F64, capi.F32])
@@ -47,14 +52,17 @@ def test_l96_schema_and_values(tmp_path, prec):
     nr, np_, ld = 4, 9, 12
     path = tmp_path / "pf.nc"
     o = capi.PFOutput(path, m, nr, np_)
-    X, A, L, t = _fill(o, m, nr, np_, ld, np.random.default_rng(1))
+    X, A, L, t, Z = _fill(o, m, nr, np_, ld, np.random.default_rng(1), nw=6)
     o.close()
     r = _read(path)
     assert r["dims"] == {"nr": nr, "n": 6, "np": np_}
     assert r["attrs"]["libbi_schema"] == b"ParticleFilter"
     assert int(r["attrs"]["libbi_schema_version"]) == 1
     v = r["vars"]
-    assert set(v) == {"time", "x", "ancestor", "logweight", "loglikelihood", "clock"}
+    assert set(v) == {"time", "x", "w", "ancestor", "logweight", "loglikelihood", "clock"}
+    # the noise variable w[n] (R_VAR) takes the state's layout
+    assert v["w"][0] == ("nr", "n", "np")
+    np.testing.assert_array_equal(v["w"][2], Z)
     # state variables take (nr, model dims, np), in the model's precision
     assert v["x"][0] == ("nr", "n", "np")
     assert v["x"][1] == np.dtype(">f4" if prec == capi.F32 else ">f8")
@@ -68,17 +76,19 @@ def test_l96_schema_and_values(tmp_path, prec):
     assert float(v["clock"][2]) == 789.0
 
 
-def test_pz_variables_are_p_and_z(tmp_path):
+def test_pz_variables_are_p_z_and_alpha(tmp_path):
     m = capi.make_model("pz", precision=capi.F64)
     nr, np_ = 3, 5
     path = tmp_path / "pz.nc"
     o = capi.PFOutput(path, m, nr, np_)
-    X, A, L, t = _fill(o, m, nr, np_, np_, np.random.default_rng(2))
+    X, A, L, t, Z = _fill(o, m, nr, np_, np_, np.random.default_rng(2), nw=1)
     o.close()
     r = _read(path)
     assert r["dims"] == {"nr": nr, "np": np_}
     v = r["vars"]
     assert v["P"][0] == ("nr", "np") and v["Z"][0] == ("nr", "np")
+    assert v["alpha"][0] == ("nr", "np")
+    np.testing.assert_array_equal(v["alpha"][2], Z[:, 0, :])
     np.testing.assert_array_equal(v["P"][2], X[:, 0, :])
     np.testing.assert_array_equal(v["Z"][2], X[:, 1, :])
 
