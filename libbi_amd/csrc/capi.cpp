@@ -107,6 +107,8 @@ struct bihip_ctx {
   ScanWorkspace ws{};
   uint64_t* EF = nullptr;
   int EF_cap = 0;
+  uint8_t* mc = nullptr;  // the chunked multinomial's scratch
+  size_t mc_cap = 0;
 };
 
 static int ctx_enter(bihip_ctx* ctx) {
@@ -157,6 +159,20 @@ static int ctx_EF(bihip_ctx* ctx, int P) {
   return BIHIP_OK;
 }
 
+static int ctx_mc(bihip_ctx* ctx, int P, int n, McWork* w) {
+  const size_t bytes = McWork::bytes(P, n, true);
+  if (bytes > ctx->mc_cap) {
+    BIHIP_HIP(hipStreamSynchronize(ctx->stream));
+    dfree(ctx->mc);
+    ctx->mc_cap = 0;
+    int rc = dalloc(&ctx->mc, bytes);
+    if (rc) return rc;
+    ctx->mc_cap = bytes;
+  }
+  w->carve(ctx->mc, P, n, true);
+  return BIHIP_OK;
+}
+
 extern "C" {
 
 const char* bihip_last_error(void) { return g_error.c_str(); }
@@ -189,6 +205,7 @@ int bihip_ctx_destroy(bihip_ctx* ctx) {
   dfree(ctx->partials);
   dfree(ctx->out);
   dfree(ctx->EF);
+  dfree(ctx->mc);
   rw_free(&ctx->rw);
   ws_free(&ctx->ws);
   delete ctx;
@@ -535,6 +552,11 @@ int bihip_multinomial_ancestors(bihip_ctx* ctx, const double* Ws, int P, int n, 
   L.n = n;
   L.U = U;
   L.as = as;
+  if (n > 0 && multinomial_chunks_fit(P)) {
+    McWork w;
+    if ((rc = ctx_mc(ctx, P, n, &w))) return rc;
+    return launch_multinomial_chunks(L, w, ctx->stream);
+  }
   return launch_multinomial(L, ctx->stream);
 }
 
@@ -646,6 +668,9 @@ struct bihip_pf {
   // record = 3: the pruned ancestry tree (AncestryCache)
   AncestryCache cache{};
   bool use_cache = false;
+  // multinomial resampling: the chunked search's scratch
+  uint8_t* mc_buf = nullptr;
+  McWork mc{};
 };
 
 // Timing events only: created without the system-scope fence, so recording one
@@ -717,6 +742,7 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   R.as = pf->as;
   R.EF = pf->EF;
   R.ws = pf->ws;
+  R.mc = pf->mc_buf != nullptr ? &pf->mc : nullptr;
   if (pf->imp_r == nullptr) {
     if ((rc = launch_resample_tiles(R, s))) return rc;
     if (pf->hist_anc) {
@@ -912,6 +938,10 @@ int bihip_pf_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config*
   PF_ALLOC(pf->hist_li, cfg->T);
   PF_ALLOC(pf->hist_ess, cfg->T);
   PF_ALLOC(pf->hist_res, cfg->T);
+  if (cfg->resampler == BIHIP_MULTINOMIAL && multinomial_chunks_fit(P)) {
+    PF_ALLOC(pf->mc_buf, McWork::bytes(P, P, false));
+    pf->mc.carve(pf->mc_buf, P, P, false);
+  }
   if (cfg->record == 1 || cfg->record == 2) PF_ALLOC(pf->hist_anc, (size_t)cfg->T * P);
   if (cfg->record == 2) {
     uint8_t* hx = nullptr;
@@ -964,6 +994,7 @@ int bihip_pf_destroy(bihip_pf* pf) {
   dfree(pf->hist_X);
   dfree(pf->hist_lw);
   dfree(pf->hist_noise);
+  dfree(pf->mc_buf);
   dfree(pf->times);
   dfree(pf->obs_mask);
   dfree(pf->inputs);

@@ -273,6 +273,7 @@ int launch_cumulative_to_ancestors(const int* Os, int P, int* as, hipStream_t s)
 // The filter's whole resampling chain in tiles, from the log-weights to the
 // permuted ancestors: systematic / stratified (cumulative offspring into Os)
 // or multinomial (Ws, then the offspring counts into os).  Gated by gate.
+struct McWork;
 struct ResampleTilesLaunch {
   int mode;                 // BIHIP_SYSTEMATIC / BIHIP_STRATIFIED / BIHIP_MULTINOMIAL
   const double* lws;
@@ -290,6 +291,7 @@ struct ResampleTilesLaunch {
   ScanWorkspace ws;
   int banks;                // 0 or 1: a lone filter; otherwise gate, status and mx of
   BankStrides bk;           // filter f at + f PFDevice
+  const McWork* mc;         // multinomial (nullable): the chunked search's scratch
 };
 int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s);
 struct MultinomialLaunch {
@@ -304,6 +306,22 @@ struct MultinomialLaunch {
   int* os;                  // nullable: offspring counts (atomic histogram; must be zeroed)
 };
 int launch_multinomial(const MultinomialLaunch& L, hipStream_t s);
+// Scratch of the chunked multinomial: per-block chunk counts, chunk offsets,
+// the segment -> chunk map and the draws sorted by chunk (targets and, for
+// ancestors out, draw indices).
+struct McWork {
+  uint64_t* cstart = nullptr;
+  double* st = nullptr;
+  uint32_t* H = nullptr;
+  int* segmap = nullptr;
+  int* sj = nullptr;
+  static size_t bytes(int P, int n, bool draws);
+  void carve(void* base, int P, int n, bool draws);
+};
+bool multinomial_chunks_fit(int P);
+// The same draws and results as launch_multinomial, sorted by chunk of the
+// cumulative weights and searched in LDS (L.os: counts, zeroed here).
+int launch_multinomial_chunks(const MultinomialLaunch& L, const McWork& w, hipStream_t s);
 int launch_zero_ints(int* p, int n, const PFDevice* gate, hipStream_t s);
 int launch_ancestors_to_offspring(const int* as, int n, int P, int* os, hipStream_t s);
 int launch_gather_rows(const int* as, int P, int ncols, void* X, int ld, int elem,

@@ -647,7 +647,6 @@ int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
   if (L.mode != BIHIP_MULTINOMIAL)
     return place_from_sums(L.Os, true, L.P, nt, L.as, L.EF, w, L.gate, s, banks, L.bk);
   int rc;
-  if ((rc = launch_zero_ints(L.os, L.P, L.gate, s))) return rc;
   MultinomialLaunch M{};
   M.Ws = L.Ws;
   M.P = L.P;
@@ -657,7 +656,11 @@ int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
   M.gate = L.gate;
   M.status = L.status;
   M.os = L.os;
-  if ((rc = launch_multinomial(M, s))) return rc;
+  if (L.mc != nullptr) {
+    if ((rc = launch_multinomial_chunks(M, *L.mc, s))) return rc;  // zeroes os itself
+  } else if ((rc = launch_zero_ints(L.os, L.P, L.gate, s)) || (rc = launch_multinomial(M, s))) {
+    return rc;
+  }
   return launch_ancestors_permute(L.os, false, L.P, L.as, L.EF, L.ws, L.gate, s);
 }
 
@@ -808,6 +811,258 @@ int launch_multinomial(const MultinomialLaunch& L, hipStream_t s) {
   if (L.n <= 0) return BIHIP_OK;
   MultinomialArgs a{L.Ws, L.P, L.n, L.U, L.seed, L.step, L.batch, L.gate, L.status, L.as, L.os};
   hipLaunchKernelGGL(k_multinomial, dim3(ceil_div(L.n, MN_DRAWS)), dim3(MN_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
+// ---------------------------------------------------------------- multinomial by chunks
+// The same draws and ancestors (a_j = #{i : Ws_i <= W U_j}), searched in LDS,
+// with offspring counts written by the workgroup that owns their chunk:
+//   k_mc_bin     each draw's chunk of MC_CHUNK particles (upper bound over the
+//                chunks' last Ws, staged in LDS); per binning block, draws per
+//                chunk (chunk-major matrix H); os zeroed
+//   k_mc_offsets per chunk, the exclusive scan of its column of H over the
+//                binning blocks, its total and its segments of at most
+//                MC_SEG draws (one for an empty chunk), packed in one word;
+//                k_tiles_exclusive over the words gives each chunk's first
+//                sorted draw and first segment
+//   k_mc_scatter the draws again, each target (and draw index) to its
+//                chunk's next sorted position; the segment -> chunk map
+//   k_mc_search  one workgroup per segment: the chunk's Ws in LDS (each
+//                chunk read once), the segment's draws searched there and
+//                counted in LDS, the counts stored (one-segment chunks: plain
+//                stores of every particle's count) or added (a chunk of
+//                several segments: one atomic per particle drawn).
+// Counts are integers and each draw's ancestor depends on its target only, so
+// the results equal k_multinomial's bit for bit whatever the order of the
+// sorted draws within a chunk.
+constexpr int MC_CHUNK = 2048;
+constexpr int MC_MAX_CHUNKS = 8192;
+constexpr int MC_BLOCK = 1024;
+constexpr int MC_BIN_BLOCKS = 256;
+constexpr int MC_SEG = 4096;
+
+struct MCArgs {
+  const double* Ws;
+  int P, n;
+  const double* U;  // nullable: Philox draws
+  uint64_t seed;
+  uint32_t step, batch;
+  const PFDevice* gate;
+  PFDevice* status;
+  int nch, per_block, nb;  // chunks; draws per binning block; binning blocks
+  uint32_t* H;             // nch x MC_BIN_BLOCKS, chunk-major
+  uint64_t* cstart;        // nch + 1: (first segment << 32) | first sorted draw
+  int* segmap;             // segments: their chunk
+  double* st;              // n sorted targets
+  int* sj;                 // n (nullable): their draws
+  int* as;                 // nullable: the ancestor of each draw
+  int* os;                 // nullable: offspring counts
+};
+
+__device__ __forceinline__ double mc_target(const MCArgs& a, double W, long j) {
+  const uint32_t step = a.gate != nullptr ? (uint32_t)a.gate->k : a.step;
+  double U;
+  if (a.U != nullptr) {
+    U = a.U[j];
+  } else {
+    const u32x4 r = philox_block(a.seed, (uint32_t)j, step, TAG_MULTINOMIAL, a.batch, 0);
+    U = u53(r.x, r.y);
+  }
+  return W * U;
+}
+
+// the chunks' last Ws into LDS
+__device__ __forceinline__ void mc_fences(const MCArgs& a, double* fence) {
+  for (int c = threadIdx.x; c < a.nch; c += MC_BLOCK) {
+    const long last = (long)(c + 1) * MC_CHUNK < a.P ? (long)(c + 1) * MC_CHUNK - 1 : a.P - 1;
+    fence[c] = a.Ws[last];
+  }
+}
+
+// chunk of target t: the number of fences <= t, the last chunk for t >= W
+__device__ __forceinline__ int mc_chunk(const double* fence, int nch, double t) {
+  int f = 0, hi = nch;
+  while (f < hi) {
+    const int mid = f + ((hi - f) >> 1);
+    if (fence[mid] <= t) f = mid + 1;
+    else hi = mid;
+  }
+  return f < nch ? f : nch - 1;
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_bin(MCArgs a) {
+  __shared__ double fence[MC_MAX_CHUNKS];
+  __shared__ uint32_t cnt[MC_MAX_CHUNKS];
+  if (!gate_open(a.gate)) return;
+  const double W = a.Ws[a.P - 1];
+  if (!(W > 0.0)) {
+    if (a.status != nullptr && blockIdx.x == 0 && threadIdx.x == 0)
+      a.status->status = BIHIP_DEGENERATE;
+    return;
+  }
+  mc_fences(a, fence);
+  for (int c = threadIdx.x; c < a.nch; c += MC_BLOCK) cnt[c] = 0;
+  if (a.os != nullptr)
+    for (long i = (long)blockIdx.x * MC_BLOCK + threadIdx.x; i < a.P;
+         i += (long)gridDim.x * MC_BLOCK)
+      a.os[i] = 0;
+  __syncthreads();
+  const long j0 = (long)blockIdx.x * a.per_block;
+  const long j1 = j0 + a.per_block < a.n ? j0 + a.per_block : a.n;
+  for (long j = j0 + threadIdx.x; j < j1; j += MC_BLOCK)
+    atomicAdd(&cnt[mc_chunk(fence, a.nch, mc_target(a, W, j))], 1u);
+  __syncthreads();
+  for (int c = threadIdx.x; c < a.nch; c += MC_BLOCK)
+    a.H[(size_t)c * MC_BIN_BLOCKS + blockIdx.x] = cnt[c];
+}
+
+// one block of SCAN_BLOCK (>= MC_BIN_BLOCKS) threads per chunk
+__global__ __launch_bounds__(SCAN_BLOCK) void k_mc_offsets(MCArgs a) {
+  __shared__ uint64_t lds[SCAN_BLOCK / WAVE];
+  if (!gate_open(a.gate) || !(a.Ws[a.P - 1] > 0.0)) return;
+  const int c = blockIdx.x, b = threadIdx.x;
+  uint32_t* col = a.H + (size_t)c * MC_BIN_BLOCKS;
+  const uint64_t v = b < a.nb ? col[b] : 0u;
+  uint64_t total;
+  const uint64_t ex = block_exclusive_scan(v, lds, total);
+  if (b < a.nb) col[b] = (uint32_t)ex;
+  if (b == 0) {
+    const uint64_t segs = total > 0 ? (total + MC_SEG - 1) / MC_SEG : 1;
+    a.cstart[c] = (segs << 32) | total;
+  }
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_scatter(MCArgs a) {
+  __shared__ double fence[MC_MAX_CHUNKS];
+  __shared__ uint32_t cur[MC_MAX_CHUNKS];
+  if (!gate_open(a.gate)) return;
+  const double W = a.Ws[a.P - 1];
+  if (!(W > 0.0)) return;
+  mc_fences(a, fence);
+  for (int c = threadIdx.x; c < a.nch; c += MC_BLOCK)
+    cur[c] = (uint32_t)a.cstart[c] + a.H[(size_t)c * MC_BIN_BLOCKS + blockIdx.x];
+  // the segment -> chunk map
+  for (long c = (long)blockIdx.x * MC_BLOCK + threadIdx.x; c < a.nch;
+       c += (long)gridDim.x * MC_BLOCK) {
+    const uint32_t s0 = (uint32_t)(a.cstart[c] >> 32), s1 = (uint32_t)(a.cstart[c + 1] >> 32);
+    for (uint32_t q = s0; q < s1; ++q) a.segmap[q] = (int)c;
+  }
+  __syncthreads();
+  const long j0 = (long)blockIdx.x * a.per_block;
+  const long j1 = j0 + a.per_block < a.n ? j0 + a.per_block : a.n;
+  for (long j = j0 + threadIdx.x; j < j1; j += MC_BLOCK) {
+    const double t = mc_target(a, W, j);
+    const uint32_t pos = atomicAdd(&cur[mc_chunk(fence, a.nch, t)], 1u);
+    a.st[pos] = t;
+    if (a.sj != nullptr) a.sj[pos] = (int)j;
+  }
+}
+
+__global__ __launch_bounds__(MC_BLOCK) void k_mc_search(MCArgs a) {
+  __shared__ double wsl[MC_CHUNK];
+  __shared__ uint32_t cnt[MC_CHUNK];
+  if (!gate_open(a.gate) || !(a.Ws[a.P - 1] > 0.0)) return;
+  const uint32_t nseg = (uint32_t)(a.cstart[a.nch] >> 32);
+  if (blockIdx.x >= nseg) return;
+  const int c = a.segmap[blockIdx.x];
+  const uint64_t w0 = a.cstart[c], w1 = a.cstart[c + 1];
+  const uint32_t seg = blockIdx.x - (uint32_t)(w0 >> 32);
+  const bool whole = (uint32_t)(w1 >> 32) - (uint32_t)(w0 >> 32) == 1u;
+  const long c0 = (long)(uint32_t)w0, c1 = (long)(uint32_t)w1;
+  const long lo = c0 + (long)seg * MC_SEG;
+  const long hi = lo + MC_SEG < c1 ? lo + MC_SEG : c1;
+  const long base = (long)c * MC_CHUNK;
+  const int len = base + MC_CHUNK < a.P ? MC_CHUNK : (int)(a.P - base);
+  for (int i = threadIdx.x; i < MC_CHUNK; i += MC_BLOCK) {
+    wsl[i] = i < len ? a.Ws[base + i] : 0.0;
+    cnt[i] = 0;
+  }
+  __syncthreads();
+  for (long pos = lo + threadIdx.x; pos < hi; pos += MC_BLOCK) {
+    const double t = a.st[pos];
+    int f = 0, h = len;  // the first i with wsl[i] > t
+    while (f < h) {
+      const int mid = f + ((h - f) >> 1);
+      if (wsl[mid] <= t) f = mid + 1;
+      else h = mid;
+    }
+    const int k = base + f < a.P ? f : (int)(a.P - 1 - base);
+    if (a.as != nullptr) a.as[a.sj[pos]] = (int)(base + k);
+    if (a.os != nullptr) atomicAdd(&cnt[k], 1u);
+  }
+  if (a.os == nullptr) return;
+  __syncthreads();
+  for (int i = threadIdx.x; i < len; i += MC_BLOCK) {
+    if (whole) a.os[base + i] = (int)cnt[i];
+    else if (cnt[i] != 0) atomicAdd(&a.os[base + i], (int)cnt[i]);
+  }
+}
+
+bool multinomial_chunks_fit(int P) { return P > 0 && ceil_div(P, MC_CHUNK) <= MC_MAX_CHUNKS; }
+
+static long mc_segments_max(int P, int n) {
+  return (long)ceil_div(P, MC_CHUNK) + ceil_div(n, MC_SEG) + 1;
+}
+
+size_t McWork::bytes(int P, int n, bool draws) {
+  const size_t nch = (size_t)ceil_div(P, MC_CHUNK);
+  return nch * MC_BIN_BLOCKS * 4 + (nch + 1) * 8 + (size_t)mc_segments_max(P, n) * 4 +
+         (size_t)n * 8 + (draws ? (size_t)n * 4 : 0) + 64 * 6;
+}
+
+void McWork::carve(void* base, int P, int n, bool draws) {
+  const size_t nch = (size_t)ceil_div(P, MC_CHUNK);
+  uint8_t* b = static_cast<uint8_t*>(base);
+  auto take = [&](size_t bytes) {
+    uint8_t* r = b;
+    b += (bytes + 63) / 64 * 64;
+    return r;
+  };
+  cstart = reinterpret_cast<uint64_t*>(take((nch + 1) * 8));
+  st = reinterpret_cast<double*>(take((size_t)n * 8));
+  H = reinterpret_cast<uint32_t*>(take(nch * MC_BIN_BLOCKS * 4));
+  segmap = reinterpret_cast<int*>(take((size_t)mc_segments_max(P, n) * 4));
+  sj = draws ? reinterpret_cast<int*>(take((size_t)n * 4)) : nullptr;
+}
+
+int launch_multinomial_chunks(const MultinomialLaunch& L, const McWork& w, hipStream_t s) {
+  if (L.n <= 0) return BIHIP_OK;
+  if (!multinomial_chunks_fit(L.P)) return set_error(BIHIP_EINVAL, "too many particles");
+  if (L.as != nullptr && w.sj == nullptr)
+    return set_error(BIHIP_EINVAL, "the ancestors of the draws need the draw index buffer");
+  MCArgs a{};
+  a.Ws = L.Ws;
+  a.P = L.P;
+  a.n = L.n;
+  a.U = L.U;
+  a.seed = L.seed;
+  a.step = L.step;
+  a.batch = L.batch;
+  a.gate = L.gate;
+  a.status = L.status;
+  a.nch = ceil_div(L.P, MC_CHUNK);
+  a.nb = std::min(MC_BIN_BLOCKS, ceil_div(L.n, MC_BLOCK * 4));
+  a.per_block = ceil_div(ceil_div(L.n, a.nb), MC_BLOCK) * MC_BLOCK;
+  a.nb = ceil_div(L.n, a.per_block);
+  a.H = w.H;
+  a.cstart = w.cstart;
+  a.segmap = w.segmap;
+  a.st = w.st;
+  a.sj = L.as != nullptr ? w.sj : nullptr;
+  a.as = L.as;
+  a.os = L.os;
+  hipLaunchKernelGGL(k_mc_bin, dim3(a.nb), dim3(MC_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mc_offsets, dim3(a.nch), dim3(SCAN_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tiles_exclusive<uint64_t>, dim3(1), dim3(SCAN_BLOCK), 0, s, a.cstart,
+                     a.nch, a.cstart + a.nch, L.gate, BankStrides{});
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mc_scatter, dim3(a.nb), dim3(MC_BLOCK), 0, s, a);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_mc_search, dim3((unsigned)mc_segments_max(L.P, L.n)), dim3(MC_BLOCK), 0,
+                     s, a);
   BIHIP_LAUNCH_CHECK();
   return BIHIP_OK;
 }

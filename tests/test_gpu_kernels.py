@@ -306,6 +306,41 @@ def test_multinomial_bit_exact(ctx, orc, torch_gpu, P):
     np.testing.assert_array_equal(host(a), ref)
 
 
+def _ws_cases(P, rng):
+    """Cumulative weights of every shape the chunked search treats differently:
+    spread, one particle holding everything, a few heavy particles among
+    light ones (chunks with few draws next to chunks with many), and weights
+    concentrated at the end."""
+    w = rng.random(P)
+    yield "spread", np.cumsum(w)
+    w = np.zeros(P)
+    w[P // 3] = 1.0
+    yield "one", np.cumsum(w)
+    w = np.full(P, 1e-9)
+    w[rng.choice(P, min(P, 5), replace=False)] = 1.0
+    yield "few heavy", np.cumsum(w)
+    w = np.exp(np.linspace(-30.0, 0.0, P))
+    yield "rising", np.cumsum(w)
+
+
+@pytest.mark.parametrize("P,n", [(1, 1000), (7, 5), (2048, 2048), (2049, 70000),
+                                 ((1 << 20) + 3, 1 << 20), ((1 << 22) + 7, 300000)])
+def test_multinomial_chunked_search_bit_exact(ctx, orc, torch_gpu, P, n):
+    """Draws sorted by chunk of the cumulative weights and searched in LDS or
+    in global memory give the oracle's ancestors draw for draw, and the
+    filter's offspring counts (through the C ABI's ancestors) for every shape
+    of weights, including n != P and the last uniform below 1."""
+    rng = np.random.default_rng(P + n)
+    for name, Ws in _ws_cases(P, rng):
+        U = rng.random(n)
+        U[0] = 0.0
+        U[-1] = 1.0 - 2.0 ** -53
+        a = empty(n, torch_gpu.int32)
+        capi.check(L().bihip_multinomial_ancestors(ctx.h, _c(dev(Ws)), P, n, _c(dev(U)), _c(a)))
+        ref, _ = orc.multinomial_ancestors(Ws, U)
+        np.testing.assert_array_equal(host(a), ref, name)
+
+
 def test_degenerate_weights_raise(ctx, torch_gpu):
     Ws = dev(np.zeros(10))
     O = empty(10, torch_gpu.int32)
