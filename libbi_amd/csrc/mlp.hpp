@@ -17,21 +17,26 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32v4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16, round to nearest even (NaN stays NaN)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// float -> bfloat16 in the high half of a word, round to nearest even; a NaN
+// stays a NaN (quiet, sign and payload kept).  Branch-free: the NaN case is a
+// select, so the fused kernels carry no exec-mask branches per activation.
+__device__ __forceinline__ uint32_t f2bf_hi(float f) {
+  const uint32_t u = __float_as_uint(f);
+  const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return __builtin_isnan(f) ? (u | 0x400000u) : r;
 }
+
+// float -> bfloat16, round to nearest even (NaN stays NaN)
+__device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)(f2bf_hi(f) >> 16); }
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// two floats -> packed bfloat16 pair (low half = a), round to nearest even.
-// (The integer form keeps the fused kernels' register allocation free of
-// spills, which the compiler's v_cvt_pk_bf16_f32 lowering did not.)
+// two floats -> packed bfloat16 pair (low half = a), round to nearest even:
+// the two high halves joined by one byte permute.  (The integer form keeps
+// the fused kernels' register allocation free of spills, which the
+// compiler's v_cvt_pk_bf16_f32 lowering did not.)
 __device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
-  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+  return __builtin_amdgcn_perm(f2bf_hi(b), f2bf_hi(a), 0x07060302u);
 }
 
 // GELU in its sigmoid form z sigma(1.702 z) (Hendrycks & Gimpel 2016, sec. 2):
