@@ -37,9 +37,11 @@ static int ws_alloc(ScanWorkspace* ws, int tiles) {
 static int rw_alloc(ReduceWork* w, int nb) {
   w->max_blocks = reduce_blocks(nb);
   int rc;
-  if ((rc = dalloc(&w->words, (size_t)6 * w->max_blocks))) return rc;
+  // 6 epoch-tagged words per block (k_reduce), then 3 plain doubles per block
+  // (the two-launch combination)
+  if ((rc = dalloc(&w->words, (size_t)9 * w->max_blocks))) return rc;
   if ((rc = dalloc(&w->state, 1))) return rc;
-  BIHIP_HIP(hipMemset(w->words, 0, sizeof(uint64_t) * 6 * w->max_blocks));
+  BIHIP_HIP(hipMemset(w->words, 0, sizeof(uint64_t) * 9 * w->max_blocks));
   BIHIP_HIP(hipMemset(w->state, 0, sizeof(ScanState)));
   return BIHIP_OK;
 }

@@ -219,7 +219,8 @@ int launch_iota(int* x, int n, hipStream_t s);
 // ---- reduce.hip
 // Level-2 partials and the last-block ticket of the two-level reduction.
 struct ReduceWork {
-  uint64_t* words;    // 6 epoch-tagged words (3 doubles) per combining block
+  uint64_t* words;    // 6 epoch-tagged words (3 doubles) per combining block, then
+                      // 3 plain doubles per block (the two-launch combination)
   ScanState* state;   // ticket (zero between launches), epoch, error
   int max_blocks;
 };

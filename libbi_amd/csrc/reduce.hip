@@ -194,6 +194,71 @@ __global__ __launch_bounds__(BANK_RED) void k_correct_bank(const double* partial
   pf_bookkeeping(M, S1, S2, P, essRel, dev + f, h, nullptr, nullptr);
 }
 
+// The same combination in two launches without tickets or epochs: block b of
+// k_combine stores the triple of partials [b COMB_BLOCK, (b + 1) COMB_BLOCK)
+// with plain stores, then one block of k_finish combines the triples in order
+// and finishes.  The combining order is k_reduce's, so are the results, bit
+// for bit; with nb <= COMB_BLOCK k_finish alone combines the partials (one
+// launch, no device-scope atomics: the small filters' correct step).  At
+// 16384 partials the two launches take 4.7 + 5.0 us against k_reduce's 9 us:
+// each launch's own latency, not the ticket, sets the time.
+__global__ __launch_bounds__(COMB_BLOCK) void k_combine(const double* partials, int nb,
+                                                        double* lvl) {
+  __shared__ double lds[COMB_BLOCK / WAVE];
+  const int lo = blockIdx.x * COMB_BLOCK, hi = min(nb, lo + COMB_BLOCK);
+  double M, S1, S2;
+  combine(lo, hi,
+          [&](int i, double& a, double& b, double& c) {
+            a = partials[3 * i];
+            b = partials[3 * i + 1];
+            c = partials[3 * i + 2];
+          },
+          lds, M, S1, S2);
+  if (threadIdx.x == 0) {
+    lvl[3 * blockIdx.x] = M;
+    lvl[3 * blockIdx.x + 1] = S1;
+    lvl[3 * blockIdx.x + 2] = S2;
+  }
+}
+
+__global__ __launch_bounds__(COMB_BLOCK) void k_finish(const double* trip, int nb, int P,
+                                                       double essRel, PFDevice* dev,
+                                                       PFHistory hist, double* out,
+                                                       const int* mask) {
+  __shared__ double lds[COMB_BLOCK / WAVE];
+  double M, S1, S2;
+  combine(0, nb,
+          [&](int i, double& a, double& b, double& c) {
+            a = trip[3 * i];
+            b = trip[3 * i + 1];
+            c = trip[3 * i + 2];
+          },
+          lds, M, S1, S2);
+  if (threadIdx.x == 0) pf_bookkeeping(M, S1, S2, P, essRel, dev, hist, out, mask);
+}
+
+static int launch_two_level(const double* partials, int nb, int P, double essRel, PFDevice* dev,
+                            PFHistory hist, double* out, const ReduceWork& w, hipStream_t s,
+                            const int* mask) {
+  if (nb <= COMB_BLOCK) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(COMB_BLOCK), 0, s, partials, nb, P, essRel, dev,
+                       hist, out, mask);
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
+  const int nb2 = reduce_blocks(nb);
+  if (nb2 > COMB_BLOCK || nb2 > w.max_blocks)
+    return set_error(BIHIP_EINVAL, "too many partials for two levels");
+  // plain triples after the epoch-tagged words (which k_reduce alone reads)
+  double* lvl = reinterpret_cast<double*>(w.words + 6 * (size_t)w.max_blocks);
+  hipLaunchKernelGGL(k_combine, dim3(nb2), dim3(COMB_BLOCK), 0, s, partials, nb, lvl);
+  BIHIP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(COMB_BLOCK), 0, s, (const double*)lvl, nb2, P,
+                     essRel, dev, hist, out, mask);
+  BIHIP_LAUNCH_CHECK();
+  return BIHIP_OK;
+}
+
 int reduce_blocks(int nb) { return ceil_div(nb, COMB_BLOCK); }
 
 int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t s) {
@@ -205,6 +270,8 @@ int launch_lws_partials(const double* lws, int P, double* partials, hipStream_t 
 
 int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWork w,
                     hipStream_t s) {
+  if (reduce_blocks(nb) <= COMB_BLOCK)
+    return launch_two_level(partials, nb, P, 0.0, nullptr, PFHistory{}, out, w, s, nullptr);
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
                      0.0, (PFDevice*)nullptr, PFHistory{}, out, w, (const int*)nullptr);
   BIHIP_LAUNCH_CHECK();
@@ -214,6 +281,8 @@ int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWo
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
                                PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
                                const int* mask) {
+  if (reduce_blocks(nb) <= COMB_BLOCK)
+    return launch_two_level(partials, nb, P, essRel, dev, hist, nullptr, w, s, mask);
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,
                      essRel, dev, hist, (double*)nullptr, w, mask);
   BIHIP_LAUNCH_CHECK();
