@@ -745,6 +745,10 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   R.EF = pf->EF;
   R.ws = pf->ws;
   R.mc = pf->mc_buf != nullptr ? &pf->mc : nullptr;
+  // the previous correct step stored the tile weight sums (fixed per filter)
+  const bool stored_sums =
+      correct_stores_sums(pf->nparts) && ceil_div(P, SCAN_TILE) <= pf->ws.max_tiles;
+  R.sums_ready = stored_sums ? 1 : 0;
   if (pf->imp_r == nullptr) {
     if ((rc = launch_resample_tiles(R, s))) return rc;
     if (pf->hist_anc) {
@@ -854,7 +858,8 @@ static int pf_enqueue_step(bihip_pf* pf, hipStream_t s, int timing) {
   // with an imposed history the recorded flags are the imposed ones
   PFHistory H{pf->hist_li, pf->hist_ess, pf->imp_r != nullptr ? nullptr : pf->hist_res};
   if ((rc = launch_pf_correct_finalize(pf->partials, pf->nparts, P, pf->cfg.essRel, pf->dev, H,
-                                       pf->rw, s, pf->obs_mask)))
+                                       pf->rw, s, pf->obs_mask, pf->lws,
+                                       stored_sums ? tile_sum_words(pf->ws) : nullptr)))
     return rc;
   if (timed) BIHIP_HIP(hipEventRecord(e1, s));
   return BIHIP_OK;

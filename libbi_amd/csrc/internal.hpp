@@ -235,9 +235,16 @@ int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWo
 // partials + 3 nb f, its PFDevice at dev + f, its histories at f * T.
 int launch_correct_bank(const double* partials, int nb, int P, double essRel, PFDevice* dev,
                         PFHistory hist, int T, int banks, hipStream_t s);
+// With tile_sums (and correct_stores_sums(nb)), the last launch also stores
+// the fixed-point weight sum of each tile of SCAN_TILE log-weights against the
+// step's maximum, which the next step's resampling chain then reads instead of
+// computing them (ResampleTilesLaunch::sums_ready): whenever the step leaves
+// the resampling gate open, and under a mask at every step.
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
                                PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
-                               const int* mask = nullptr);
+                               const int* mask = nullptr, const double* lws = nullptr,
+                               uint64_t* tile_sums = nullptr);
+bool correct_stores_sums(int nb);
 
 // ---- scan.hip
 int launch_sumexp_scan(const double* lws, int P, const double* mx_dev, const PFDevice* gate,
@@ -293,7 +300,11 @@ struct ResampleTilesLaunch {
   int banks;                // 0 or 1: a lone filter; otherwise gate, status and mx of
   BankStrides bk;           // filter f at + f PFDevice
   const McWork* mc;         // multinomial (nullable): the chunked search's scratch
+  int sums_ready;           // the tile weight sums are stored (a lone filter whose
+                            // correct step stores them): no k_weight_sums
 };
+// Where the tiled chain keeps its tile weight sums (2 words per tile).
+uint64_t* tile_sum_words(const ScanWorkspace& ws);
 int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s);
 struct MultinomialLaunch {
   const double* Ws;

@@ -237,6 +237,48 @@ __global__ __launch_bounds__(COMB_BLOCK) void k_finish(const double* trip, int n
   if (threadIdx.x == 0) pf_bookkeeping(M, S1, S2, P, essRel, dev, hist, out, mask);
 }
 
+// k_finish with the next step's tile weight sums, one block per tile of
+// SCAN_TILE log-weights: every block combines the triples in k_finish's
+// order (the same M, S1, S2 in each), block 0 does the bookkeeping, and
+// each block stores its tile's sum of fix_weight(nanexp(lw - M)), the value
+// k_weight_sums computes from dev->mx = M, when the step leaves the gate
+// open: ess < essRel P (the decision pf_bookkeeping takes), or any step under
+// a mask, whose decision needs the step number that block 0 moves on.  The
+// log-weights are loaded first so that their latency overlaps the
+// combination's.
+__global__ __launch_bounds__(COMB_BLOCK) void k_finish_sums(const double* trip, int nb, int P,
+                                                            double essRel, PFDevice* dev,
+                                                            PFHistory hist, const int* mask,
+                                                            const double* lws, u128* wsum) {
+  static_assert(COMB_BLOCK == SCAN_BLOCK, "one thread per SCAN_ITEMS log-weights");
+  __shared__ double lds[COMB_BLOCK / WAVE];
+  __shared__ u128 lds_w[SCAN_BLOCK / WAVE];
+  const long t0 = (long)blockIdx.x * SCAN_TILE;
+  double lw[SCAN_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const long i = t0 + (long)j * SCAN_BLOCK + threadIdx.x;
+    lw[j] = i < P ? lws[i] : __builtin_nan("");  // NaN weighs zero
+  }
+  double M, S1, S2;
+  combine(0, nb,
+          [&](int i, double& a, double& b, double& c) {
+            a = trip[3 * i];
+            b = trip[3 * i + 1];
+            c = trip[3 * i + 2];
+          },
+          lds, M, S1, S2);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    pf_bookkeeping(M, S1, S2, P, essRel, dev, hist, nullptr, mask);
+  const double ess = S1 * S1 / S2;
+  if (mask == nullptr && !(ess < essRel * (double)P)) return;  // uniform across the grid
+  u128 t(0);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) t = t + fix_weight(nanexp(lw[j] - M));
+  t = block_sum_v(t, lds_w);
+  if (threadIdx.x == 0) wsum[blockIdx.x] = t;
+}
+
 static int launch_two_level(const double* partials, int nb, int P, double essRel, PFDevice* dev,
                             PFHistory hist, double* out, const ReduceWork& w, hipStream_t s,
                             const int* mask) {
@@ -278,9 +320,30 @@ int launch_finalize(const double* partials, int nb, int P, double* out, ReduceWo
   return BIHIP_OK;
 }
 
+bool correct_stores_sums(int nb) { return reduce_blocks(nb) <= COMB_BLOCK; }
+
 int launch_pf_correct_finalize(const double* partials, int nb, int P, double essRel,
                                PFDevice* dev, PFHistory hist, ReduceWork w, hipStream_t s,
-                               const int* mask) {
+                               const int* mask, const double* lws, uint64_t* tile_sums) {
+  if (tile_sums != nullptr) {
+    if (!correct_stores_sums(nb) || lws == nullptr || dev == nullptr || P <= 0)
+      return set_error(BIHIP_EINVAL, "the correct step stores tile sums up to %d partials",
+                       COMB_BLOCK * COMB_BLOCK);
+    const double* trip = partials;
+    int n = nb;
+    if (nb > COMB_BLOCK) {
+      n = reduce_blocks(nb);
+      if (n > w.max_blocks) return set_error(BIHIP_EINVAL, "too many partials for two levels");
+      double* lvl = reinterpret_cast<double*>(w.words + 6 * (size_t)w.max_blocks);
+      hipLaunchKernelGGL(k_combine, dim3(n), dim3(COMB_BLOCK), 0, s, partials, nb, lvl);
+      BIHIP_LAUNCH_CHECK();
+      trip = lvl;
+    }
+    hipLaunchKernelGGL(k_finish_sums, dim3(ceil_div(P, SCAN_TILE)), dim3(COMB_BLOCK), 0, s, trip,
+                       n, P, essRel, dev, hist, mask, lws, reinterpret_cast<u128*>(tile_sums));
+    BIHIP_LAUNCH_CHECK();
+    return BIHIP_OK;
+  }
   if (reduce_blocks(nb) <= COMB_BLOCK)
     return launch_two_level(partials, nb, P, essRel, dev, hist, nullptr, w, s, mask);
   hipLaunchKernelGGL(k_reduce, dim3(reduce_blocks(nb)), dim3(COMB_BLOCK), 0, s, partials, nb, P,

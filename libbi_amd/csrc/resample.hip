@@ -151,9 +151,11 @@ struct TileWork {
   uint32_t* free_slots;
 };
 
+uint64_t* tile_sum_words(const ScanWorkspace& ws) { return ws.words + 8 * (size_t)ws.max_tiles; }
+
 static TileWork tile_work(const ScanWorkspace& ws) {
   const size_t mt = (size_t)ws.max_tiles;
-  uint64_t* base = ws.words + 8 * mt;
+  uint64_t* base = tile_sum_words(ws);
   TileWork w;
   w.wsum = reinterpret_cast<u128*>(base);  // 2 mt words
   w.efsum = base + 2 * mt;                 // mt words
@@ -617,9 +619,13 @@ int launch_resample_tiles(const ResampleTilesLaunch& L, hipStream_t s) {
   const int banks = L.banks > 1 ? L.banks : 1;
   if (banks > 1 && L.mode == BIHIP_MULTINOMIAL)
     return set_error(BIHIP_EINVAL, "filter banks resample systematically or stratified");
-  hipLaunchKernelGGL(k_weight_sums, dim3(nt, banks), dim3(SCAN_BLOCK), 0, s, L.lws, L.mx, L.P,
-                     w.wsum, L.gate, L.bk);
-  BIHIP_LAUNCH_CHECK();
+  if (!L.sums_ready) {
+    hipLaunchKernelGGL(k_weight_sums, dim3(nt, banks), dim3(SCAN_BLOCK), 0, s, L.lws, L.mx, L.P,
+                       w.wsum, L.gate, L.bk);
+    BIHIP_LAUNCH_CHECK();
+  } else if (banks > 1) {
+    return set_error(BIHIP_EINVAL, "stored tile sums serve a lone filter");
+  }
   if (nt > SMALL_TILES) {
     hipLaunchKernelGGL(k_tiles_exclusive<u128>, dim3(1, banks), dim3(SCAN_BLOCK), 0, s, w.wsum,
                        nt, w.wtotal, L.gate, L.bk);
