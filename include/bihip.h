@@ -349,7 +349,9 @@ int bihip_pf_sample_path(bihip_pf* pf, double* path_host, int* index_host);
  * oracle's, a learned flow against the exact one).  Not with use_graph. */
 int bihip_pf_impose_resampling(bihip_pf* pf, const int* resampled_host,
                                const int* ancestors_host);
-/* Current state buffer (device pointer, leading dimension) and log-weights. */
+/* Current state buffer (device pointer, leading dimension) and log-weights:
+ * read-only views between steps (the next step resamples with the weight totals
+ * its correct step computed). */
 int bihip_pf_state(bihip_pf* pf, void** X_dev, int* ld, double** lws_dev);
 /* Steps completed so far. */
 int bihip_pf_position(bihip_pf* pf);

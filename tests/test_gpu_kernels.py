@@ -361,7 +361,7 @@ def _offspring_cases(P, rng):
 
 
 @pytest.mark.parametrize("P", [1, 2, 3, 31, 2048, 2049, 100000, 1 << 20, (3 << 20) + 5,
-                               (8 << 20) + 5])
+                               4 << 20, (8 << 20) + 5])
 def test_permute_bit_exact(ctx, orc, torch_gpu, P):
     rng = np.random.default_rng(40 + P)
     for o in _offspring_cases(P, rng):
