@@ -393,8 +393,8 @@ int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_
  * ParticleFilterNetCDFBuffer (share/src/bi/netcdf/ParticleFilterNetCDFBuffer.cpp:19-38,
  * with SimulatorNetCDFBuffer::create, share/src/bi/netcdf/SimulatorNetCDFBuffer.cpp:23-83):
  * LibBi's DEFAULT particle-filter schema -- dimensions nr, the model's (L96: n), np;
- * variables time(nr), the noise variables of a built-in model (PZ: alpha; L96:
- * w(nr, n, np); LG: w) and the state variables (PZ: P, Z; L96: x(nr, n, np); LG: x),
+ * variables time(nr), the noise variables (a generated model's from its file; PZ: alpha;
+ * L96: w(nr, n, np); LG: w) and the state variables (PZ: P, Z; L96: x(nr, n, np); LG: x),
  * ancestor(nr, np), logweight(nr, np), loglikelihood, clock; attributes libbi_schema =
  * "ParticleFilter", libbi_schema_version = 1 -- written without libnetcdf as a NetCDF
  * classic file with 64-bit offsets (CDF-2), which nc_open reads like the reference's
@@ -412,7 +412,8 @@ int bihip_pfout_write_state(bihip_pfout* o, int k, const void* X_host, int ld,
                             const int* as_host);
 /* SimulatorNetCDFBuffer::writeState's noise part (R_VAR, SimulatorNetCDFBuffer.hpp:341-352):
  * the noise variables drawn in the step to output k (PZ: alpha; Lorenz-96: w[n]; LG: w),
- * noise_width x ld column-major in the model's precision.  Generated models have none. */
+ * noise_width x ld column-major in the model's precision; a generated model's noise
+ * variables in their declaration order. */
 int bihip_pfout_write_noise(bihip_pfout* o, int k, const void* Z_host, int ld);
 /* ParticleFilterNetCDFBuffer::writeLogWeights (ParticleFilterNetCDFBuffer.hpp:54-55) */
 int bihip_pfout_write_log_weights(bihip_pfout* o, int k, const double* lws_host);

@@ -24,6 +24,7 @@ struct GenInfo {
   const char* const* names[5];  // by BIHIP_VAR_* kind
   const int* sizes[5];          // elements of each variable
   const char* const* state_dims;  // dimension of each state variable ("" for a scalar)
+  const char* const* noise_dims;  // dimension of each noise variable ("" for a scalar)
 };
 const GenInfo* generated_info(int model);
 // Multiples of a generated model's delta one interval may span (each is a

@@ -127,11 +127,16 @@ std::vector<StateVar> state_vars(const bihip_model* m) {
   }
 }
 
-// The noise variables of a built-in model (the model files' declarations:
-// PZ.bi's alpha, Lorenz96.bi's w[n], LinearGaussian.bi's w); a generated
-// model's noise is not recorded, so none are written for it.
+// The noise variables, in the order of the recorded noise rows: a generated
+// model's from its file, a built-in model's as the model files declare them
+// (PZ.bi's alpha, Lorenz96.bi's w[n], LinearGaussian.bi's w).
 std::vector<StateVar> noise_vars(const bihip_model* m) {
-  if (bihip::generated_info(m->model) != nullptr) return {};
+  if (const bihip::GenInfo* g = bihip::generated_info(m->model)) {
+    std::vector<StateVar> v;
+    for (int i = 0; g->names[BIHIP_VAR_NOISE][i] != nullptr; ++i)
+      v.push_back({g->names[BIHIP_VAR_NOISE][i], g->noise_dims[i], g->sizes[BIHIP_VAR_NOISE][i]});
+    return v;
+  }
   switch (m->model) {
     case BIHIP_MODEL_PZ:
       return {{"alpha", "", 1}};

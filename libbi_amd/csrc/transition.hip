@@ -143,17 +143,20 @@ void k_transition(TransArgs a) {
     };
     M m = model();
     int n;
+    T* zout = nullptr;  // the noise variables of this step (recorded histories only)
+    if constexpr (SCHED) {
+      if (a.noise_hist != nullptr && a.dev != nullptr && !a.flow_only)
+        zout = static_cast<T*>(a.noise_hist) +
+               (size_t)(a.dev->k + 1) * (size_t)a.noise_width * (size_t)a.ld;
+    }
     if constexpr (is_generated<M>::value) {
       // the model's transition block: discrete actions and its ode, in order
       n = m.template transition<INTEG, ONE>(grp, x, t1, t2, a.m, a.seed, a.batch, (uint32_t)p,
                                             step);
-    } else {
-      T* zout = nullptr;  // the noise variables of this step (recorded histories only)
       if constexpr (SCHED) {
-        if (a.noise_hist != nullptr && a.dev != nullptr && !a.flow_only)
-          zout = static_cast<T*>(a.noise_hist) +
-                 (size_t)(a.dev->k + 1) * (size_t)a.noise_width * (size_t)a.ld;
+        if (zout != nullptr) m.noise_out(grp, zout, a.ld, (uint32_t)p);
       }
+    } else {
       if constexpr (M::NOISE_FIRST) {
         if (!a.flow_only) {
           noise_first(m, a.seed, a.batch, p, step);
@@ -543,7 +546,11 @@ int launch_init_bank(const bihip_model* m, uint64_t seed, uint32_t batch, int P,
 }
 
 int noise_width_of(const bihip_model* m) {
-  if (generated_info(m->model) != nullptr) return 0;
+  if (const GenInfo* g = generated_info(m->model)) {  // the model file's noise variables
+    int w = 0;
+    for (int i = 0; g->names[BIHIP_VAR_NOISE][i] != nullptr; ++i) w += g->sizes[BIHIP_VAR_NOISE][i];
+    return w;
+  }
   switch (m->model) {
     case BIHIP_MODEL_PZ: return 1;    // alpha
     case BIHIP_MODEL_L96: return m->d;  // w[n]

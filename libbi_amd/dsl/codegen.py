@@ -1159,6 +1159,20 @@ def _generate_lanes(g: GenModel) -> str:
     w = outer
     _emit_interval_loop(w, body)
 
+    # the noise variables after the transition, into the step's noise rows
+    # (element e of the noise layout at z + e ld): a vector's slice from each
+    # lane, a scalar (the same in every lane) from lane 0
+    w("  template <class G>")
+    w("  __device__ __forceinline__ void noise_out(const G& grp, T* z, long ld, uint32_t p) const {")
+    w("    const int lo = grp.g * DL;")
+    w("    (void)lo;")
+    for v in noises:
+        if v.dim:
+            w(f"    for (int i = 0; i < DL; ++i) z[(long)({v.offset} + lo + i) * ld + p] = w_{v.name}[i];")
+        else:
+            w(f"    if (grp.g == 0) z[(long){v.offset} * ld + p] = w_{v.name};")
+    w("  }")
+
     # observation terms of the lane's state elements: element s = lo + l is read
     # by obs element j when s = a j + b
     eo = _LaneEmitter(g, "double", "x", DL, obs=True)
@@ -1425,6 +1439,17 @@ def generate_model(g: GenModel) -> str:
     w = outer
     _emit_interval_loop(w, body)
 
+    # the noise variables after the transition, into the step's noise rows
+    # (element e of the noise layout at z + e ld)
+    w("  template <class G>")
+    w("  __device__ __forceinline__ void noise_out(const G&, T* z, long ld, uint32_t p) const {")
+    for v in noises:
+        if v.dim:
+            w(f"    for (int i = 0; i < {v.size}; ++i) z[(long)({v.offset} + i) * ld + p] = w_{v.name}[i];")
+        else:
+            w(f"    z[(long){v.offset} * ld + p] = w_{v.name};")
+    w("  }")
+
     # observation log-density (share/tt/cpp/action/gaussian.hpp.tt:48-73, uniform.hpp.tt:38-52)
     eo = _Emitter(g, "double", "x")
     w("  __device__ __forceinline__ double obs(const T (&x)[DL], const double* y, int) const {")
@@ -1493,17 +1518,21 @@ def generate(models: List[GenModel]) -> str:
         sv = [v for v in g.vars.values() if v.kind == "state"]
         out.append(f"static const char* const GEN_{i}_STATE_DIMS[] = "
                    f"{_cstr_list([v.dim or '' for v in sv])};")
+        nv = [v for v in g.vars.values() if v.kind == "noise"]
+        out.append(f"static const char* const GEN_{i}_NOISE_DIMS[] = "
+                   f"{_cstr_list([v.dim or '' for v in nv])};")
     out.append("static const GenInfo GEN_MODELS[] = {")
     for i, g in enumerate(models):
         names = ", ".join(f"GEN_{i}_{k.upper()}" for k in KIND_CODE)
         sizes = ", ".join(f"GEN_{i}_{k.upper()}_SIZES" for k in KIND_CODE)
         out.append(f'    {{"{g.name}", {g.D}, {g.NOBS}, {g.integrator}, {g.lpp}, '
                    f"{_lit(g.h)}, {_lit(g.atol)}, {_lit(g.rtol)}, {_lit(g.delta)}, "
-                   f"{{{names}}}, {{{sizes}}}, GEN_{i}_STATE_DIMS}},")
+                   f"{{{names}}}, {{{sizes}}}, GEN_{i}_STATE_DIMS, GEN_{i}_NOISE_DIMS}},")
     if not models:
         none = ", ".join(["GEN_NONE"] * len(KIND_CODE))
         nsz = ", ".join(["GEN_NONE_SIZES"] * len(KIND_CODE))
-        out.append(f'    {{"", 0, 0, 0, 1, 0.0, 0.0, 0.0, 1.0, {{{none}}}, {{{nsz}}}, GEN_NONE}},')
+        out.append(f'    {{"", 0, 0, 0, 1, 0.0, 0.0, 0.0, 1.0, {{{none}}}, {{{nsz}}}, GEN_NONE, '
+                   'GEN_NONE},')
     out.append("};")
     out.append("")
     out.append("// F::template run<M>() for the generated model of m")

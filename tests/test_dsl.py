@@ -195,7 +195,8 @@ struct ModelArgs { double h0; double par[16]; };
 struct bihip_model_ { int model, precision; };
 struct u32x4 { uint32_t x, y, z, w; };
 struct GenInfo { const char* name; int d, nobs, integrator, lpp; double h, atol, rtol, delta;
-  const char* const* names[5]; const int* sizes[5]; const char* const* state_dims; };
+  const char* const* names[5]; const int* sizes[5]; const char* const* state_dims;
+  const char* const* noise_dims; };
 constexpr double HALF_LOG_TWO_PI = 0.91893853320467274178;
 constexpr uint32_t TAG_INIT = 1, TAG_NOISE = 2, TAG_OBS = 6;
 inline int set_error(int c, const char*) { return c; }

@@ -571,6 +571,49 @@ def test_generated_lorenz96_d40_fp32_and_samples(ctx, orc, torch_gpu):
     np.testing.assert_allclose(outs[0][1], outs[1][1], rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("which", ["PZ", "Lorenz96d40-f32", "Lorenz96d40-f64"])
+def test_generated_noise_recorded_as_handwritten(ctx, orc, tmp_path, which):
+    """With record="paths" a generated model's noise variables reach the NetCDF
+    file under its names: PZ.bi's alpha and Lorenz96d40.bi's w[n] (four lanes
+    per particle, each storing its ten elements) equal the hand-written
+    functors' recorded noise bit for bit (the same keyed Philox draws), and
+    recording changes no result of the generated filter."""
+    from scipy.io import netcdf_file
+    from gpu_helpers import l96_spun_up
+    if which == "PZ":
+        g, h = _pz_pair(capi.F64)
+        y = _pz_data(orc, 6)
+        dt, P, name, dims, mean0 = 1.0, 500, "alpha", ("nr", "np"), None
+    else:
+        prec = capi.F32 if which.endswith("f32") else capi.F64
+        par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+        g = capi.generated_model("Lorenz96d40", precision=prec, **par)
+        h = capi.make_model("l96", d=40, precision=prec, atol=g.atol, rtol=g.rtol, h0=g.h0,
+                            **par)
+        _, y = orc.simulate(oracle_model(orc, h), 7, 6, 0.05, l96_spun_up(orc))
+        dt, P, name, dims, mean0 = 0.05, 1000, "w", ("nr", "n", "np"), np.zeros(40)
+    T = len(y)
+    runs, noise = {}, {}
+    for key, m, rec in (("g0", g, False), ("g", g, "paths"), ("h", h, "paths")):
+        pf = capi.ParticleFilter(ctx, m, P, T, dt, 11, ess_rel=0.5, record=rec)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(T)
+        X, _, lws = pf.state()
+        runs[key] = (pf.outputs()["logIncrements"], X[:, :P].cpu().numpy(), lws.cpu().numpy())
+        if rec:
+            pf.write_netcdf(tmp_path / f"{key}.nc")
+            f = netcdf_file(str(tmp_path / f"{key}.nc"), "r", mmap=False)
+            assert f.variables[name].dimensions == dims
+            noise[key] = np.array(f.variables[name].data)
+            f.close()
+        pf.close()
+    for a, b in zip(runs["g0"], runs["g"]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(noise["g"][1:]).max() > 0
+    np.testing.assert_array_equal(noise["g"], noise["h"])
+
+
 @pytest.mark.gpu
 def test_generated_lane_group_model_rejected_by_batch(ctx):
     """The batched inner filters take one-lane models only: a lane-group model

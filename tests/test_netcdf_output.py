@@ -93,6 +93,33 @@ def test_pz_variables_are_p_z_and_alpha(tmp_path):
     np.testing.assert_array_equal(v["Z"][2], X[:, 1, :])
 
 
+@pytest.mark.parametrize("name,state,noise", [
+    ("PZ", {"P": ("nr", "np"), "Z": ("nr", "np")}, {"alpha": ("nr", "np")}),
+    ("Lorenz96d40", {"x": ("nr", "n", "np")}, {"w": ("nr", "n", "np")}),
+    ("LinearGaussian", {"x": ("nr", "np")}, {"w": ("nr", "np")})])
+def test_generated_model_variables_from_its_file(tmp_path, name, state, noise):
+    """A generated model's state and noise variables take the names and
+    dimensions of its model file; its noise rows follow the declaration order."""
+    m = capi.generated_model(name, precision=capi.F64)
+    nw = sum(capi.model_var_sizes(m, capi.VAR_NOISE))
+    nr, np_, ld = 3, 5, 8
+    path = tmp_path / "gen.nc"
+    o = capi.PFOutput(path, m, nr, np_)
+    X, A, L, t, Z = _fill(o, m, nr, np_, ld, np.random.default_rng(2), nw=nw)
+    o.close()
+    v = _read(path)["vars"]
+    assert set(v) == set(state) | set(noise) | {"time", "ancestor", "logweight",
+                                                "loglikelihood", "clock"}
+    for k, dims in {**state, **noise}.items():
+        assert v[k][0] == dims
+    row = 0
+    for k in noise:  # one noise variable per file here
+        got = v[k][2].reshape(nr, -1, np_)
+        np.testing.assert_array_equal(got, Z[:, row:row + got.shape[1], :])
+        row += got.shape[1]
+    assert row == nw
+
+
 def test_outputs_written_out_of_order(tmp_path):
     """Outputs are written at their index, in any order (the buffer is random access)."""
     m = capi.make_model("lg", precision=capi.F64)
