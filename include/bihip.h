@@ -731,6 +731,13 @@ int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
                            int P, const double* mean0_host, bihip_smc2** s);
 /* The learned surrogate flow in every inner filter (bihip_pfbank_set_surrogate). */
 int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
+/* MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-266) with the B
+ * chains' proposals filtered as one filter bank of B filters of P particles: lane-group
+ * models (Lorenz-96) and inner filters of any size.  Built-in models; the
+ * bihip_smc2_config fields block and prior_from_model are unused; Lorenz-96 filters start
+ * around mean0_host.  Run, outputs and keyed draws as bihip_pmmh_create's. */
+int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                           int C, int P, const double* mean0_host, bihip_pmmh** s);
 /* Event timing of the bank's launches, as bihip_pf_set_timing / bihip_pf_kernel_time. */
 int bihip_pfbank_set_timing(bihip_pfbank* bank, int mode);
 int bihip_pfbank_kernel_time(bihip_pfbank* bank, int kernel, double* ms_host, long* count_host);

@@ -76,7 +76,7 @@ EXPORTS = [
     "bihip_pfbank_set_guided", "bihip_pfbank_set_timing", "bihip_pfbank_kernel_time",
     "bihip_pfbank_set_theta", "bihip_pfbank_set_stream_base", "bihip_pfbank_increments",
     "bihip_pfbank_resample_filters", "bihip_pfbank_accept", "bihip_smc2_create_bank",
-    "bihip_smc2_set_surrogate",
+    "bihip_smc2_set_surrogate", "bihip_pmmh_create_bank",
 ]
 
 
@@ -293,6 +293,7 @@ def _declare(L):
         "bihip_pfbank_accept": [P, P, pi],
         "bihip_smc2_create_bank": [P, pm, c.POINTER(SMC2Config), i, pd, c.POINTER(P)],
         "bihip_smc2_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
+        "bihip_pmmh_create_bank": [P, pm, c.POINTER(SMC2Config), i, i, pd, c.POINTER(P)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -1023,14 +1024,21 @@ class SMC2:
 
 
 class PMMH:
-    """bihip_pmmh: MarginalMH with B independent chains, proposals filtered as one batch."""
+    """bihip_pmmh: MarginalMH with B independent chains, proposals filtered as one batch
+    (or, with bank=(B, P), as one filter bank of B filters of P particles)."""
 
     def __init__(self, ctx, model, N, T, C, prior_mean, prior_sd, prop_sd, dt=1.0, seed=1,
-                 resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, use_graph=False, block=0):
-        """prior_mean = prior_sd = None: the model file's parameter block is the prior."""
+                 resampler=SYSTEMATIC, ess_rel=0.5, t0=0.0, use_graph=False, block=0,
+                 bank=None, mean0=None):
+        """prior_mean = prior_sd = None: the model file's parameter block is the prior.
+        bank=(B, P): the chains' filters run as a filter bank (N is ignored); Lorenz-96
+        filters start around mean0."""
         from_model = prior_mean is None
         nt = len(prop_sd)
-        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        if bank is not None:
+            self.N = np.full(bank[0], bank[1], np.int32)
+        else:
+            self.N = np.ascontiguousarray(N, dtype=np.int32)
         B = len(self.N)
         cfg = SMC2Config(B=B, T=T, ntheta=nt, nmoves=0, resampler=resampler,
                          essRel_inner=ess_rel, essRel_outer=0.0, t0=t0, dt=dt, seed=seed,
@@ -1043,9 +1051,17 @@ class PMMH:
             cfg.prop_sd[k] = prop_sd[k]
         self.cfg = cfg
         h = ctypes.c_void_p()
-        check(lib().bihip_pmmh_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg), C,
-                                      self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                      ctypes.byref(h)))
+        if bank is not None:
+            self._mean0 = (np.ascontiguousarray(mean0, np.float64) if mean0 is not None
+                           else None)
+            m0 = (self._mean0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                  if self._mean0 is not None else None)
+            check(lib().bihip_pmmh_create_bank(ctx.h, ctypes.byref(model), ctypes.byref(cfg), C,
+                                               int(bank[1]), m0, ctypes.byref(h)))
+        else:
+            check(lib().bihip_pmmh_create(ctx.h, ctypes.byref(model), ctypes.byref(cfg), C,
+                                          self.N.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          ctypes.byref(h)))
         self.h = h
         ctx.adopt(self)
         self.B, self.T, self.C, self.nt, self.nobs = B, T, C, nt, nobs(model)

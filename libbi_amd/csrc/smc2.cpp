@@ -521,11 +521,12 @@ int bihip_smc2_outputs(bihip_smc2* s, double* logZ, double* theta_host, double* 
 
 // ---------------------------------------------------------------- PMMH
 // MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-266) for B
-// independent chains: every iteration filters all B proposals in one batch
-// (one persistent launch per observation) and accepts on the host.  Keyed
-// draws as SMC^2's: prior normals {j, 0, THETA, 0}, proposal normals
-// {j, c, PROPOSAL, 0}, acceptance uniform {j, c, ACCEPT, 0}; the filter of
-// iteration c of chain j draws with batch id B c + j.
+// independent chains: every iteration filters all B proposals together (one
+// batch: a persistent launch per observation; or one filter bank: shared
+// launches per step) and accepts on the host.  Keyed draws as SMC^2's: prior
+// normals {j, 0, THETA, 0}, proposal normals {j, c, PROPOSAL, 0}, acceptance
+// uniform {j, c, ACCEPT, 0}; the filter of iteration c of chain j draws with
+// batch id B c + j.
 
 }  // extern "C"
 
@@ -533,7 +534,7 @@ struct bihip_pmmh {
   bihip_model model{};
   bihip_smc2_config cfg{};
   int C = 0;
-  bihip_batch* batch = nullptr;
+  Inner* inner = nullptr;  // the B proposals' filters
   std::vector<double> theta, ll;  // C x B x ntheta, C x B
   std::vector<int> accepted;
   bool have_obs = false, done = false;
@@ -546,14 +547,12 @@ int pmmh_filter(bihip_pmmh* s, const std::vector<double>& lth, uint32_t base,
   std::vector<double> th((size_t)B * nt);
   for (size_t q = 0; q < th.size(); ++q) th[q] = std::exp(lth[q]);
   int rc;
-  if ((rc = bihip_batch_set_theta(s->batch, th.data())) ||
-      (rc = bihip_batch_set_stream_base(s->batch, base)) || (rc = bihip_batch_init(s->batch)) ||
-      (rc = bihip_batch_step(s->batch, s->cfg.T)))
+  if ((rc = s->inner->set_theta(th.data())) || (rc = s->inner->set_stream_base(base)) ||
+      (rc = s->inner->init()) || (rc = s->inner->step(s->cfg.T)))
     return rc;
   std::vector<int> st(B);
   ll.resize(B);
-  if ((rc = bihip_batch_outputs(s->batch, ll.data(), nullptr, nullptr, nullptr, st.data())))
-    return rc;
+  if ((rc = s->inner->outputs(ll.data(), st.data()))) return rc;
   for (int j = 0; j < B; ++j)
     if (st[j] != BIHIP_OK) ll[j] = -INFINITY;
   return BIHIP_OK;
@@ -562,16 +561,16 @@ int pmmh_filter(bihip_pmmh* s, const std::vector<double>& lth, uint32_t base,
 
 extern "C" {
 
-int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg, int C,
-                      const int* N_host, bihip_pmmh** out) {
-  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
-    return set_error(BIHIP_EINVAL, "NULL argument");
-  const bihip_smc2_config& c = *cfg;
+static int pmmh_check(const bihip_model* m, const bihip_smc2_config& c, int C) {
   if (c.B <= 0 || c.T <= 0 || C <= 0) return set_error(BIHIP_EINVAL, "need B, T, C > 0");
   if (c.ntheta < 1 || c.ntheta > 8) return set_error(BIHIP_EINVAL, "ntheta must lie in [1,8]");
   if (int rc = check_prior(m, c)) return rc;
   if ((uint64_t)c.B * ((uint64_t)C + 1) > (1u << 24))
     return set_error(BIHIP_EINVAL, "B (C + 1) random-stream ids exceed 2^24");
+  return BIHIP_OK;
+}
+
+static bihip_pmmh* pmmh_new(const bihip_model* m, const bihip_smc2_config& c, int C) {
   bihip_pmmh* s = new bihip_pmmh();
   s->model = *m;
   s->cfg = c;
@@ -579,6 +578,18 @@ int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   s->theta.assign((size_t)C * c.B * c.ntheta, 0.0);
   s->ll.assign((size_t)C * c.B, 0.0);
   s->accepted.assign(c.B, 0);
+  return s;
+}
+
+int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg, int C,
+                      const int* N_host, bihip_pmmh** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || N_host == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (int rc = pmmh_check(m, c, C)) return rc;
+  bihip_pmmh* s = pmmh_new(m, c, C);
+  auto* inner = new BatchInner();
+  s->inner = inner;
   bihip_batch_config bc{};
   bc.B = c.B;
   bc.T = c.T;
@@ -590,9 +601,45 @@ int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
   bc.use_graph = c.use_graph;
   bc.block = c.block;
   std::vector<double> th0((size_t)c.B * c.ntheta, 1.0);
-  int rc = bihip_batch_create(ctx, m, &bc, N_host, th0.data(), c.ntheta, &s->batch);
+  int rc = bihip_batch_create(ctx, m, &bc, N_host, th0.data(), c.ntheta, &inner->b);
   if (rc != BIHIP_OK) {
-    delete s;
+    bihip_pmmh_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return BIHIP_OK;
+}
+
+int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
+                           int C, int P, const double* mean0_host, bihip_pmmh** out) {
+  if (ctx == nullptr || m == nullptr || cfg == nullptr || out == nullptr)
+    return set_error(BIHIP_EINVAL, "NULL argument");
+  const bihip_smc2_config& c = *cfg;
+  if (int rc = pmmh_check(m, c, C)) return rc;
+  if (c.prior_from_model) return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
+  if (P <= 0) return set_error(BIHIP_EINVAL, "need P > 0");
+  if (m->model == BIHIP_MODEL_L96 && mean0_host == nullptr)
+    return set_error(BIHIP_EINVAL, "Lorenz-96 needs mean0");
+  bihip_pmmh* s = pmmh_new(m, c, C);
+  auto* inner = new BankInner();
+  s->inner = inner;
+  inner->P = P;
+  if (mean0_host != nullptr) inner->mean0.assign(mean0_host, mean0_host + m->d);
+  bihip_pf_config pc{};
+  pc.P = P;
+  pc.T = c.T;
+  pc.resampler = c.resampler;
+  pc.essRel = c.essRel_inner;
+  pc.t0 = c.t0;
+  pc.dt = c.dt;
+  pc.seed = c.seed;
+  pc.batch = 0;
+  pc.record = 0;
+  pc.use_graph = c.use_graph;
+  std::vector<double> th0((size_t)c.B * c.ntheta, 1.0);
+  int rc = bihip_pfbank_create(ctx, m, &pc, c.B, th0.data(), c.ntheta, &inner->b);
+  if (rc != BIHIP_OK) {
+    bihip_pmmh_destroy(s);
     return rc;
   }
   *out = s;
@@ -601,14 +648,14 @@ int bihip_pmmh_create(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_con
 
 int bihip_pmmh_destroy(bihip_pmmh* s) {
   if (s == nullptr) return BIHIP_OK;
-  bihip_batch_destroy(s->batch);
+  delete s->inner;
   delete s;
   return BIHIP_OK;
 }
 
 int bihip_pmmh_set_observations(bihip_pmmh* s, const double* y_host) {
   if (s == nullptr || y_host == nullptr) return set_error(BIHIP_EINVAL, "NULL argument");
-  const int rc = bihip_batch_set_observations(s->batch, y_host);
+  const int rc = s->inner->set_observations(y_host);
   s->have_obs = rc == BIHIP_OK;
   return rc;
 }

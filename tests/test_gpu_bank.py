@@ -294,6 +294,54 @@ def test_smc2_over_banks_matches_oracle(ctx, orc, er_out, nmoves, use_graph):
     assert out["logZ"] == pytest.approx(ref["logZ"], rel=1e-10, abs=1e-10)
 
 
+def test_pmmh_over_banks_matches_oracle(ctx, orc):
+    """PMMH whose chains' proposals are filtered as one filter bank against the
+    oracle's MarginalMH with equal inner sizes: the same acceptances, draws and
+    log-likelihoods."""
+    B, P, T, C = 8, 1500, 8, 12
+    y = _pz_data(orc, T, seed=13)
+    gm = capi.make_model("pz", precision=capi.F64)
+    pm, ps, qs = [np.log(0.5), np.log(0.2)], [0.3, 0.3], [0.1, 0.1]
+    c = orc.smc2_config(B, T, pm, ps, qs, seed=17, ess_rel_inner=0.6)
+    ref = orc.pmmh_run(oracle_model(orc, gm), c, C, [P] * B, y)
+    s = capi.PMMH(ctx, gm, None, T, C, pm, ps, qs, seed=17, ess_rel=0.6, bank=(B, P))
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    print("PMMH over banks: accepted", ref["accepted"], flush=True)
+    assert (ref["accepted"] > 1).any()
+    np.testing.assert_array_equal(out["accepted"], ref["accepted"])
+    np.testing.assert_allclose(out["theta"], ref["theta"], rtol=1e-15)
+    np.testing.assert_allclose(out["ll"], ref["ll"], rtol=1e-10, atol=1e-10)
+
+
+def test_pmmh_over_banks_runs_lorenz96_d40(ctx):
+    """PMMH on the Lorenz-96 d=40 forcing (FP32 state, four lanes per particle,
+    which the batched filters do not run): each chain's first log-likelihood
+    equals the lone filter's with that forcing and batch id, and the chains
+    move inside the prior's support."""
+    B, P, T, C = 8, 2048, 6, 4
+    gm = capi.make_model("l96", d=40, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                         init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+    mean0 = data.l96_spinup(ctx, d=40)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=5)
+    s = capi.PMMH(ctx, gm, None, T, C, [np.log(8.0)], [0.05], [0.02], dt=DT, seed=3,
+                  bank=(B, P), mean0=mean0)
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    assert np.isfinite(out["ll"]).all()
+    assert (out["theta"] > 6.0).all() and (out["theta"] < 10.5).all()
+    for j in (0, B - 1):
+        pf = _lone(ctx, gm, [out["theta"][0, j, 0]], P, T, DT, 3, j, capi.SYSTEMATIC, 0.5, y,
+                   mean0)
+        pf.step(T)
+        assert pf.term() == pytest.approx(out["ll"][0, j], rel=1e-9)
+        pf.close()
+
+
 def test_smc2_over_surrogate_banks_runs_lorenz96(ctx, torch_gpu):
     """SMC^2 over the Lorenz-96 forcing with the surrogate flow in every inner
     filter: the outer loop runs (resamplings and accepted moves) and keeps the
