@@ -680,8 +680,10 @@ int bihip_pf_set_guided(bihip_pf* pf, bihip_mlp* mlp, const bihip_guided_norm* n
  * any lane width (Lorenz-96 d = 200).  Filter j equals bihip_pf with Philox
  * batch id cfg->batch + j and bihip_model::par[i] = theta_host[j * ntheta + i]
  * (i < ntheta), step for step: the same ancestors and the same log-weights.
- * Built-in models; cfg->resampler systematic or stratified; cfg->record 0 or
- * 1 (ancestors); cfg->use_graph replays a captured step.
+ * Built-in and generated models (a generated model's parameters and inputs
+ * are its par slots; no schedule or per-step inputs); cfg->resampler
+ * systematic or stratified; cfg->record 0 or 1 (ancestors); cfg->use_graph
+ * replays a captured step.
  */
 typedef struct bihip_pfbank bihip_pfbank;
 int bihip_pfbank_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_config* cfg, int B,
@@ -724,18 +726,20 @@ int bihip_pfbank_resample_filters(bihip_pfbank* bank, const int* anc_host);
 int bihip_pfbank_accept(bihip_pfbank* dst, const bihip_pfbank* src, const int* accept_host);
 /* SMC^2 (bihip_smc2_*) whose inner filters are filter banks of P particles each (the
  * theta-particles' filters and a move's proposals) in place of bihip_batch: any inner size,
- * lane-group models (Lorenz-96), the learned surrogate flow.  Built-in models; the
- * bihip_smc2_config fields block and prior_from_model are unused; Lorenz-96 filters start
- * around mean0_host.  Outputs and keyed draws as bihip_smc2_create's. */
+ * lane-group models (Lorenz-96, Lorenz96d40.bi), the learned surrogate flow.  The
+ * bihip_smc2_config field block is unused; prior_from_model as bihip_smc2_create's;
+ * Lorenz-96 filters start around mean0_host.  Outputs and keyed draws as
+ * bihip_smc2_create's. */
 int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
                            int P, const double* mean0_host, bihip_smc2** s);
 /* The learned surrogate flow in every inner filter (bihip_pfbank_set_surrogate). */
 int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
 /* MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-266) with the B
  * chains' proposals filtered as one filter bank of B filters of P particles: lane-group
- * models (Lorenz-96) and inner filters of any size.  Built-in models; the
- * bihip_smc2_config fields block and prior_from_model are unused; Lorenz-96 filters start
- * around mean0_host.  Run, outputs and keyed draws as bihip_pmmh_create's. */
+ * models (Lorenz-96, Lorenz96d40.bi) and inner filters of any size.  The
+ * bihip_smc2_config field block is unused; prior_from_model as bihip_pmmh_create's;
+ * Lorenz-96 filters start around mean0_host.  Run, outputs and keyed draws as
+ * bihip_pmmh_create's. */
 int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
                            int C, int P, const double* mean0_host, bihip_pmmh** s);
 /* Event timing of the bank's launches, as bihip_pf_set_timing / bihip_pf_kernel_time. */

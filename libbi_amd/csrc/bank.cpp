@@ -302,8 +302,6 @@ int bihip_pfbank_create(bihip_ctx* ctx, const bihip_model* m, const bihip_pf_con
   int rc = ctx_enter_device(ctx);
   if (rc) return rc;
   if ((rc = check_model(m))) return rc;
-  if (m->model >= BIHIP_MODEL_GENERATED)
-    return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
   if (B <= 0) return set_error(BIHIP_EINVAL, "need B > 0");
   if (ntheta < 0 || ntheta > 16 || (ntheta > 0 && theta_host == nullptr))
     return set_error(BIHIP_EINVAL, "theta is B x ntheta with 0 <= ntheta <= 16");

@@ -379,7 +379,6 @@ int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
     return set_error(BIHIP_EINVAL, "NULL argument");
   const bihip_smc2_config& c = *cfg;
   if (int rc = smc2_check(m, c)) return rc;
-  if (c.prior_from_model) return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
   if (P <= 0) return set_error(BIHIP_EINVAL, "need P > 0");
   if (m->model == BIHIP_MODEL_L96 && mean0_host == nullptr)
     return set_error(BIHIP_EINVAL, "Lorenz-96 needs mean0");
@@ -616,7 +615,6 @@ int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
     return set_error(BIHIP_EINVAL, "NULL argument");
   const bihip_smc2_config& c = *cfg;
   if (int rc = pmmh_check(m, c, C)) return rc;
-  if (c.prior_from_model) return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
   if (P <= 0) return set_error(BIHIP_EINVAL, "need P > 0");
   if (m->model == BIHIP_MODEL_L96 && mean0_host == nullptr)
     return set_error(BIHIP_EINVAL, "Lorenz-96 needs mean0");

@@ -416,22 +416,20 @@ struct TransitionRunner {
     const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
                                             a.inputs != nullptr || a.noise_hist != nullptr);
     if (banks > 1) {
-      if constexpr (is_generated<M>::value) {
-        return set_error(BIHIP_EINVAL, "filter banks run the built-in models");
-      } else {
-        if (sched || a.dev == nullptr || a.bank_par == nullptr || a.imp_r != nullptr)
-          return set_error(BIHIP_EINVAL, "a filter bank step takes no schedule, inputs or "
-                                         "imposed history");
-        const dim3 grid(blocks, banks);
-        if (integ == BIHIP_RK4)
-          hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, false, false, true>), grid,
-                             dim3(TRANS_BLOCK), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5, false, false, true>), grid,
-                             dim3(TRANS_BLOCK), 0, s, a);
-        BIHIP_LAUNCH_CHECK();
-        return BIHIP_OK;
-      }
+      // a generated model runs its general interval loop over delta (the
+      // one-delta form gives the same values)
+      if (sched || a.dev == nullptr || a.bank_par == nullptr || a.imp_r != nullptr)
+        return set_error(BIHIP_EINVAL, "a filter bank step takes no schedule, inputs or "
+                                       "imposed history");
+      const dim3 grid(blocks, banks);
+      if (integ == BIHIP_RK4)
+        hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, false, false, true>), grid,
+                           dim3(TRANS_BLOCK), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_transition<M, BIHIP_DOPRI5, false, false, true>), grid,
+                           dim3(TRANS_BLOCK), 0, s, a);
+      BIHIP_LAUNCH_CHECK();
+      return BIHIP_OK;
     }
     if constexpr (is_generated<M>::value) {
       if (one && !sched) {

@@ -214,9 +214,15 @@ def test_bank_rejects_what_it_does_not_run(ctx):
     theta = _pz_thetas(2)
     with pytest.raises(capi.BihipError, match="systematically or stratified"):
         capi.FilterBank(ctx, gm, 100, 5, 1.0, 1, theta, resampler=capi.MULTINOMIAL)
-    gen = capi.generated_model("PZ", precision=capi.F64)
-    with pytest.raises(capi.BihipError, match="built-in models"):
-        capi.FilterBank(ctx, gen, 100, 5, 1.0, 1, theta)
+    gen = capi.generated_model("Lorenz96d40", precision=capi.F32)
+    gbank = capi.FilterBank(ctx, gen, 256, 5, DT, 1, np.array([[8.0], [8.1]]))
+    net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=1)
+    norm = capi.SurrogateNorm(x_shift=0.0, x_scale=1.0, dx_scale=0.05, par_shift=8.0,
+                              par_scale=1.0)
+    with pytest.raises(capi.BihipError, match="serves FP32 Lorenz-96 d=40"):
+        gbank.set_surrogate(net, norm)  # the learned flow is the built-in model's
+    gbank.close()
+    net.close()
     l96 = capi.make_model("l96", d=40, precision=capi.F32)
     bank = capi.FilterBank(ctx, l96, 256, 5, DT, 1, np.array([[8.0, 0.1], [8.0, 0.2]]))
     net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=1)
@@ -292,6 +298,80 @@ def test_smc2_over_banks_matches_oracle(ctx, orc, er_out, nmoves, use_graph):
     np.testing.assert_allclose(out["theta"], ref["theta"], rtol=1e-15)
     np.testing.assert_allclose(out["lw"], ref["lw"], rtol=1e-9, atol=1e-9)
     assert out["logZ"] == pytest.approx(ref["logZ"], rel=1e-10, abs=1e-10)
+
+
+@pytest.mark.parametrize("which", ["PZ", "Lorenz96d40"])
+def test_generated_model_bank_equals_lone_filters(ctx, orc, which):
+    """A bank on a model compiled from its LibBi file (PZ.bi in FP64;
+    Lorenz96d40.bi in FP32 over four lanes per particle, which the batched
+    filters do not run) equals the lone generated filters with the same
+    parameters and batch ids: the same ancestors, states and log-weights."""
+    if which == "PZ":
+        B, P, T, dt, mean0 = 4, 3000, 10, 1.0, None
+        g = capi.generated_model("PZ", precision=capi.F64, mu=0.5, sigma=0.2)
+        theta = _pz_thetas(B)[:, :2]  # (mu, sigma), the file's first params
+        y = _pz_data(orc, T)
+    else:
+        B, P, T, dt = 3, 2048, 8, DT
+        par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+        g = capi.generated_model("Lorenz96d40", precision=capi.F32, **par)
+        h = capi.make_model("l96", d=40, precision=capi.F32, atol=g.atol, rtol=g.rtol, h0=g.h0,
+                            **par)
+        mean0 = data.l96_spinup(ctx, d=40)
+        _, y = data.simulate(ctx, h, mean0, T, DT, seed=5)
+        theta = np.array([[7.8], [8.0], [8.2]])  # F
+    bank = capi.FilterBank(ctx, g, P, T, dt, 42, theta, ess_rel=0.6, record=True)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    assert out["resampled"].sum() >= 2
+    for j in range(B):
+        pf = _lone(ctx, g, theta[j], P, T, dt, 42, j, capi.SYSTEMATIC, 0.6, y, mean0)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-12,
+                                   atol=1e-10)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k), f"{j} {k}")
+        Xb, _, lwb = bank.state(j)
+        Xp, _, lwp = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy())
+        assert np.array_equal(lwb.cpu().numpy(), lwp.cpu().numpy())
+        pf.close()
+    bank.close()
+
+
+def test_smc2_over_generated_banks_equals_batched(ctx, orc):
+    """SMC^2 over lam of LinearGaussian.bi with the prior of its parameter
+    block, the inner filters as filter banks and as the batched filters of
+    equal sizes: the same resamplings, moves and draws, the same evidence."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, atol=1e-10, rtol=1e-10,
+                             h0=0.25, **par)
+    h = capi.make_model("lg", precision=capi.F64, atol=1e-10, rtol=1e-10, h0=0.25, **par)
+    om = oracle_model(orc, h)
+    B, P, T = 64, 512, 12
+    x0 = orc.init_samples(om, 21, 1)[:, 0]
+    _, y = orc.simulate(om, 21, T, 1.0, x0)
+    y = np.asarray(y).reshape(T, 1)
+    outs = []
+    for bank in (None, (B, P)):
+        s = capi.SMC2(ctx, g, [P] * B, T, None, None, [0.3], seed=3, nmoves=1,
+                      ess_rel_outer=0.8, bank=bank)
+        s.set_observations(y)
+        s.run()
+        outs.append(s.outputs())
+        s.close()
+    a, b = outs
+    assert a["resampled"].sum() >= 1 and a["accepts"].sum() >= 1
+    np.testing.assert_array_equal(b["resampled"], a["resampled"])
+    np.testing.assert_array_equal(b["accepts"], a["accepts"])
+    np.testing.assert_allclose(b["theta"], a["theta"], rtol=1e-15)
+    np.testing.assert_allclose(b["lw"], a["lw"], rtol=1e-9, atol=1e-9)
+    assert b["logZ"] == pytest.approx(a["logZ"], rel=1e-10, abs=1e-10)
 
 
 def test_pmmh_over_banks_matches_oracle(ctx, orc):
