@@ -734,6 +734,9 @@ int bihip_smc2_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
                            int P, const double* mean0_host, bihip_smc2** s);
 /* The learned surrogate flow in every inner filter (bihip_pfbank_set_surrogate). */
 int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
+/* The learned proposal in every inner filter (bihip_pfbank_set_guided; the auxiliary
+ * weights of share/src/bi/filter/BridgePF.hpp:114-144 with a learned bridge). */
+int bihip_smc2_set_guided(bihip_smc2* s, bihip_mlp* mlp, const bihip_guided_norm* norm);
 /* MarginalMH::sample (share/src/bi/sampler/MarginalMH.hpp:198-266) with the B
  * chains' proposals filtered as one filter bank of B filters of P particles: lane-group
  * models (Lorenz-96, Lorenz96d40.bi) and inner filters of any size.  The
@@ -742,6 +745,9 @@ int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogat
  * bihip_pmmh_create's. */
 int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc2_config* cfg,
                            int C, int P, const double* mean0_host, bihip_pmmh** s);
+/* The learned surrogate flow or proposal in the chains' filters (bank form only). */
+int bihip_pmmh_set_surrogate(bihip_pmmh* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm);
+int bihip_pmmh_set_guided(bihip_pmmh* s, bihip_mlp* mlp, const bihip_guided_norm* norm);
 /* Event timing of the bank's launches, as bihip_pf_set_timing / bihip_pf_kernel_time. */
 int bihip_pfbank_set_timing(bihip_pfbank* bank, int mode);
 int bihip_pfbank_kernel_time(bihip_pfbank* bank, int kernel, double* ms_host, long* count_host);

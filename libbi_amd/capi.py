@@ -76,7 +76,8 @@ EXPORTS = [
     "bihip_pfbank_set_guided", "bihip_pfbank_set_timing", "bihip_pfbank_kernel_time",
     "bihip_pfbank_set_theta", "bihip_pfbank_set_stream_base", "bihip_pfbank_increments",
     "bihip_pfbank_resample_filters", "bihip_pfbank_accept", "bihip_smc2_create_bank",
-    "bihip_smc2_set_surrogate", "bihip_pmmh_create_bank",
+    "bihip_smc2_set_surrogate", "bihip_pmmh_create_bank", "bihip_smc2_set_guided",
+    "bihip_pmmh_set_surrogate", "bihip_pmmh_set_guided",
 ]
 
 
@@ -294,6 +295,9 @@ def _declare(L):
         "bihip_smc2_create_bank": [P, pm, c.POINTER(SMC2Config), i, pd, c.POINTER(P)],
         "bihip_smc2_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
         "bihip_pmmh_create_bank": [P, pm, c.POINTER(SMC2Config), i, i, pd, c.POINTER(P)],
+        "bihip_smc2_set_guided": [P, P, c.POINTER(GuidedNorm)],
+        "bihip_pmmh_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
+        "bihip_pmmh_set_guided": [P, P, c.POINTER(GuidedNorm)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -1009,6 +1013,11 @@ class SMC2:
         check(lib().bihip_smc2_set_surrogate(self.h, mlp.h if mlp is not None else None,
                                              ctypes.byref(norm) if norm is not None else None))
 
+    def set_guided(self, mlp, norm):
+        """The learned proposal in every inner filter (bank=... only)."""
+        check(lib().bihip_smc2_set_guided(self.h, mlp.h if mlp is not None else None,
+                                          ctypes.byref(norm) if norm is not None else None))
+
     def outputs(self):
         D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
         logZ = ctypes.c_double()
@@ -1084,6 +1093,16 @@ class PMMH:
 
     def run(self):
         check(lib().bihip_pmmh_run(self.h))
+
+    def set_surrogate(self, mlp, norm):
+        """The learned surrogate flow in the chains' filters (bank=... only)."""
+        check(lib().bihip_pmmh_set_surrogate(self.h, mlp.h if mlp is not None else None,
+                                             ctypes.byref(norm) if norm is not None else None))
+
+    def set_guided(self, mlp, norm):
+        """The learned proposal in the chains' filters (bank=... only)."""
+        check(lib().bihip_pmmh_set_guided(self.h, mlp.h if mlp is not None else None,
+                                          ctypes.byref(norm) if norm is not None else None))
 
     def outputs(self):
         D, I = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)

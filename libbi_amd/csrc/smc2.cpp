@@ -424,6 +424,14 @@ int bihip_smc2_set_surrogate(bihip_smc2* s, bihip_mlp* mlp, const bihip_surrogat
   return rc;
 }
 
+int bihip_smc2_set_guided(bihip_smc2* s, bihip_mlp* mlp, const bihip_guided_norm* norm) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->bank) return set_error(BIHIP_ESTATE, "the guided proposal runs in filter banks");
+  int rc = bihip_pfbank_set_guided(static_cast<BankInner*>(s->slots)->b, mlp, norm);
+  if (rc == BIHIP_OK) rc = bihip_pfbank_set_guided(static_cast<BankInner*>(s->prop)->b, mlp, norm);
+  return rc;
+}
+
 int bihip_smc2_destroy(bihip_smc2* s) {
   if (s == nullptr) return BIHIP_OK;
   delete s->prop;
@@ -534,6 +542,7 @@ struct bihip_pmmh {
   bihip_smc2_config cfg{};
   int C = 0;
   Inner* inner = nullptr;  // the B proposals' filters
+  bool bank = false;
   std::vector<double> theta, ll;  // C x B x ntheta, C x B
   std::vector<int> accepted;
   bool have_obs = false, done = false;
@@ -621,6 +630,7 @@ int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
   bihip_pmmh* s = pmmh_new(m, c, C);
   auto* inner = new BankInner();
   s->inner = inner;
+  s->bank = true;
   inner->P = P;
   if (mean0_host != nullptr) inner->mean0.assign(mean0_host, mean0_host + m->d);
   bihip_pf_config pc{};
@@ -642,6 +652,18 @@ int bihip_pmmh_create_bank(bihip_ctx* ctx, const bihip_model* m, const bihip_smc
   }
   *out = s;
   return BIHIP_OK;
+}
+
+int bihip_pmmh_set_surrogate(bihip_pmmh* s, bihip_mlp* mlp, const bihip_surrogate_norm* norm) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->bank) return set_error(BIHIP_ESTATE, "the surrogate flow runs in filter banks");
+  return bihip_pfbank_set_surrogate(static_cast<BankInner*>(s->inner)->b, mlp, norm);
+}
+
+int bihip_pmmh_set_guided(bihip_pmmh* s, bihip_mlp* mlp, const bihip_guided_norm* norm) {
+  if (s == nullptr) return set_error(BIHIP_EINVAL, "NULL sampler");
+  if (!s->bank) return set_error(BIHIP_ESTATE, "the guided proposal runs in filter banks");
+  return bihip_pfbank_set_guided(static_cast<BankInner*>(s->inner)->b, mlp, norm);
 }
 
 int bihip_pmmh_destroy(bihip_pmmh* s) {

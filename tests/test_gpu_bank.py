@@ -422,6 +422,57 @@ def test_pmmh_over_banks_runs_lorenz96_d40(ctx):
         pf.close()
 
 
+@pytest.mark.parametrize("learned", ["guided", "surrogate"])
+def test_pmmh_over_learned_banks_equals_lone_learned_filters(ctx, torch_gpu, learned):
+    """PMMH on the Lorenz-96 forcing with the learned proposal (d=200) or the
+    learned flow (d=40) in the chains' filters: each chain's first
+    log-likelihood equals the lone learned filter's with that forcing and batch
+    id, and SMC^2 over the same learned banks runs inside the prior's support."""
+    B, P, T, C = 6, 1 << 12, 5, 3
+    if learned == "guided":
+        gm = guided.model(8.0)
+        mean0 = data.l96_spinup(ctx, d=gm.d, F=8.0)
+        net = _guided_net(ctx)
+        norm = capi.GuidedNorm(x_shift=2.3, x_scale=3.6, innov_scale=1.0, par_shift=8.0,
+                               par_scale=1.0)
+    else:
+        gm = capi.make_model("l96", d=40, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                             init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+        mean0 = data.l96_spinup(ctx, d=40)
+        net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=5)
+        p = net.get_params()
+        p *= 0.1  # a near-zero flow correction
+        net.set_params(p)
+        norm = capi.SurrogateNorm(x_shift=2.3, x_scale=3.6, dx_scale=0.05, par_shift=8.0,
+                                  par_scale=1.0)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=9)
+    s = capi.PMMH(ctx, gm, None, T, C, [np.log(8.0)], [0.05], [0.02], dt=DT, seed=3,
+                  bank=(B, P), mean0=mean0)
+    getattr(s, "set_" + learned)(net, norm)
+    s.set_observations(y)
+    s.run()
+    out = s.outputs()
+    s.close()
+    assert np.isfinite(out["ll"]).all()
+    for j in (0, B - 1):
+        pf = _lone(ctx, gm, [out["theta"][0, j, 0]], P, T, DT, 3, j, capi.SYSTEMATIC, 0.5, y,
+                   mean0)
+        getattr(pf, "set_" + learned)(net, norm)
+        pf.step(T)
+        assert pf.term() == pytest.approx(out["ll"][0, j], rel=1e-9)
+        pf.close()
+    s = capi.SMC2(ctx, gm, None, T, [np.log(8.0)], [0.05], [0.02], dt=DT, seed=3,
+                  ess_rel_outer=0.8, bank=(B, P), mean0=mean0)
+    getattr(s, "set_" + learned)(net, norm)
+    s.set_observations(y)
+    s.run()
+    o2 = s.outputs()
+    s.close()
+    net.close()
+    assert np.isfinite(o2["logZ"])
+    assert (o2["theta"] > 6.0).all() and (o2["theta"] < 10.5).all()
+
+
 def test_smc2_over_surrogate_banks_runs_lorenz96(ctx, torch_gpu):
     """SMC^2 over the Lorenz-96 forcing with the surrogate flow in every inner
     filter: the outer loop runs (resamplings and accepted moves) and keeps the
