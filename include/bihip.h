@@ -397,13 +397,19 @@ int bihip_pf_kernel_time(bihip_pf* pf, int kernel, double* ms_host, long* count_
  * L96: w(nr, n, np); LG: w) and the state variables (PZ: P, Z; L96: x(nr, n, np); LG: x),
  * ancestor(nr, np), logweight(nr, np), loglikelihood, clock; attributes libbi_schema =
  * "ParticleFilter", libbi_schema_version = 1 -- written without libnetcdf as a NetCDF
- * classic file with 64-bit offsets (CDF-2), which nc_open reads like the reference's
- * NetCDF-4 files.  clock is NC_DOUBLE microseconds (CDF-2 has no 64-bit integer); only
- * the state variables, written last, may exceed 4 GiB.  All buffers are host memory.
+ * classic file with 64-bit offsets (CDF-2), or in the 64-bit data format (CDF-5) when a
+ * variable before the last (the state variables) exceeds 4 GiB; nc_open reads both like
+ * the reference's NetCDF-4 files.  clock is NC_DOUBLE microseconds.  All buffers are host
+ * memory.
  */
 typedef struct bihip_pfout bihip_pfout;
 int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
                        bihip_pfout** out);
+/* The same with the format chosen: 0 as bihip_pfout_create, 2 (CDF-2; an error when a
+ * variable before the last exceeds 4 GiB) or 5 (CDF-5).  No counterpart in the reference,
+ * which writes NetCDF-4 through libnetcdf. */
+int bihip_pfout_create_format(const char* path, const bihip_model* m, int nr, int np,
+                              int format, bihip_pfout** out);
 /* SimulatorNetCDFBuffer::writeTime (SimulatorNetCDFBuffer.cpp:264) */
 int bihip_pfout_write_time(bihip_pfout* o, int k, double t);
 /* ParticleFilterNetCDFBuffer::writeState(k, X, as) (ParticleFilterNetCDFBuffer.hpp:43-44):

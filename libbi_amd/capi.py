@@ -77,7 +77,7 @@ EXPORTS = [
     "bihip_pfbank_set_theta", "bihip_pfbank_set_stream_base", "bihip_pfbank_increments",
     "bihip_pfbank_resample_filters", "bihip_pfbank_accept", "bihip_smc2_create_bank",
     "bihip_smc2_set_surrogate", "bihip_pmmh_create_bank", "bihip_smc2_set_guided",
-    "bihip_pmmh_set_surrogate", "bihip_pmmh_set_guided",
+    "bihip_pmmh_set_surrogate", "bihip_pmmh_set_guided", "bihip_pfout_create_format",
 ]
 
 
@@ -298,6 +298,7 @@ def _declare(L):
         "bihip_smc2_set_guided": [P, P, c.POINTER(GuidedNorm)],
         "bihip_pmmh_set_surrogate": [P, P, c.POINTER(SurrogateNorm)],
         "bihip_pmmh_set_guided": [P, P, c.POINTER(GuidedNorm)],
+        "bihip_pfout_create_format": [c.c_char_p, pm, i, i, i, c.POINTER(P)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -1208,11 +1209,12 @@ class PFOutput:
     (ParticleFilterNetCDFBuffer's writeTime / writeState / writeLogWeights /
     writeLogLikelihood / writeClock)."""
 
-    def __init__(self, path, model, nr, np_):
+    def __init__(self, path, model, nr, np_, fmt=0):
+        """fmt: 0 (CDF-2, or CDF-5 when a variable before the last exceeds 4 GiB), 2 or 5."""
         self.h = ctypes.c_void_p()
         self.model, self.nr, self.np = model, nr, np_
-        check(lib().bihip_pfout_create(os.fsencode(path), ctypes.byref(model), nr, np_,
-                                       ctypes.byref(self.h)))
+        check(lib().bihip_pfout_create_format(os.fsencode(path), ctypes.byref(model), nr, np_,
+                                              fmt, ctypes.byref(self.h)))
 
     def write_time(self, k, t):
         check(lib().bihip_pfout_write_time(self.h, k, float(t)))

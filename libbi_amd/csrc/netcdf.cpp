@@ -11,11 +11,12 @@
 // libbi_version.
 //
 // The reference writes NetCDF-4 through libnetcdf.  This writer has no
-// dependency: it writes the classic format with 64-bit offsets (CDF-2),
-// which every NetCDF library reads through the same nc_open / nc_get_var
-// calls.  Differences forced by that format: `clock` is NC_DOUBLE
-// (microseconds; CDF-2 has no 64-bit integer), and only the last variable may
-// exceed 4 GiB (the state variables are written last).
+// dependency: it writes the classic format with 64-bit offsets (CDF-2), or,
+// when a variable other than the last exceeds 4 GiB (the log-weights and
+// noise of long runs of large filters), the 64-bit data format (CDF-5: every
+// count, length, dimension id and size 64-bit); NetCDF libraries read both
+// through the same nc_open / nc_get_var calls.  `clock` is NC_DOUBLE
+// (microseconds; CDF-2 has no 64-bit integer).
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -44,6 +45,7 @@ struct Var {
 
 struct Bytes {
   std::vector<uint8_t> b;
+  int version = 2;  // CDF-2 or CDF-5
   void u32(uint32_t v) {
     for (int s = 24; s >= 0; s -= 8) b.push_back((uint8_t)(v >> s));
   }
@@ -53,8 +55,13 @@ struct Bytes {
   void pad() {
     while (b.size() % 4) b.push_back(0);
   }
+  // NON_NEG of the grammar: counts, lengths, dimension ids, sizes
+  void nn(uint64_t v) {
+    if (version == 5) u64(v);
+    else u32((uint32_t)v);
+  }
   void name(const std::string& n) {
-    u32((uint32_t)n.size());
+    nn(n.size());
     b.insert(b.end(), n.begin(), n.end());
     pad();
   }
@@ -181,9 +188,16 @@ extern "C" {
 
 int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
                        bihip_pfout** out) {
+  return bihip_pfout_create_format(path, m, nr, np, 0, out);
+}
+
+int bihip_pfout_create_format(const char* path, const bihip_model* m, int nr, int np,
+                              int format, bihip_pfout** out) {
   if (path == nullptr || m == nullptr || out == nullptr)
     return set_error(BIHIP_EINVAL, "NULL argument");
   if (nr <= 0 || np <= 0) return set_error(BIHIP_EINVAL, "need nr > 0 and np > 0");
+  if (format != 0 && format != 2 && format != 5)
+    return set_error(BIHIP_EINVAL, "format is 0 (the smallest that fits), 2 or 5");
   auto* o = new bihip_pfout();
   o->path = path;
   o->nr = nr;
@@ -252,56 +266,59 @@ int bihip_pfout_create(const char* path, const bihip_model* m, int nr, int np,
     return set_error(BIHIP_EINVAL, "model dimension %d does not match its variables", m->d);
   }
 
-  // header (the classic format's grammar, CDF-2: 64-bit begin offsets)
+  // header (the classic format's grammar with 64-bit begin offsets; CDF-5
+  // when a variable before the last does not fit CDF-2's 32-bit vsize)
   Bytes h;
-  const char magic[4] = {'C', 'D', 'F', 2};
+  bool big = false;
+  for (size_t i = 0; i + 1 < o->vars.size(); ++i)
+    big = big || (o->vars[i].bytes + 3) / 4 * 4 > 0xFFFFFFFCull;
+  if (big && format == 2) {
+    delete o;
+    return set_error(BIHIP_EINVAL, "a variable before the last exceeds CDF-2's 4 GiB limit");
+  }
+  h.version = format == 5 || (format == 0 && big) ? 5 : 2;
+  const char magic[4] = {'C', 'D', 'F', (char)h.version};
   h.b.insert(h.b.end(), magic, magic + 4);
-  h.u32(0);  // numrecs: no record dimension
+  h.nn(0);  // numrecs: no record dimension
   h.u32(TAG_DIMENSION);
-  h.u32((uint32_t)o->dims.size());
+  h.nn(o->dims.size());
   for (auto& dm : o->dims) {
     h.name(dm.first);
-    h.u32((uint32_t)dm.second);
+    h.nn(dm.second);
   }
   const std::string schema = "ParticleFilter", version = "1.4.5";
   h.u32(TAG_ATTRIBUTE);
-  h.u32(3);
+  h.nn(3);
   h.name("libbi_schema");
   h.u32(NC_CHAR);
-  h.u32((uint32_t)schema.size());
+  h.nn(schema.size());
   h.b.insert(h.b.end(), schema.begin(), schema.end());
   h.pad();
   h.name("libbi_schema_version");
   h.u32(NC_INT);
-  h.u32(1);
+  h.nn(1);
   h.u32(1);
   h.name("libbi_version");
   h.u32(NC_CHAR);
-  h.u32((uint32_t)version.size());
+  h.nn(version.size());
   h.b.insert(h.b.end(), version.begin(), version.end());
   h.pad();
   // variables: the header size does not depend on the offsets, so lay it
   // out once with zero offsets, then place the data after it
   std::vector<size_t> begin_pos;
   h.u32(TAG_VARIABLE);
-  h.u32((uint32_t)o->vars.size());
+  h.nn(o->vars.size());
   for (size_t i = 0; i < o->vars.size(); ++i) {
     const Var& v = o->vars[i];
     h.name(v.name);
-    h.u32((uint32_t)v.dims.size());
-    for (int id : v.dims) h.u32((uint32_t)id);
-    h.u32(0);  // vatt_list ABSENT
-    h.u32(0);
+    h.nn(v.dims.size());
+    for (int id : v.dims) h.nn((uint64_t)id);
+    h.u32(0);  // vatt_list ABSENT: a zero tag and a zero count
+    h.nn(0);
     h.u32((uint32_t)v.type);
     const uint64_t padded = (v.bytes + 3) / 4 * 4;
-    const bool last = i + 1 == o->vars.size();
-    if (padded > 0xFFFFFFFCull && !last) {
-      delete o;
-      return set_error(BIHIP_EINVAL,
-                       "variable %s (%llu bytes) exceeds the classic format's 4 GiB limit",
-                       v.name.c_str(), (unsigned long long)v.bytes);
-    }
-    h.u32(padded > 0xFFFFFFFCull ? 0xFFFFFFFFu : (uint32_t)padded);
+    if (h.version == 5) h.u64(padded);
+    else h.u32(padded > 0xFFFFFFFCull ? 0xFFFFFFFFu : (uint32_t)padded);  // only the last
     begin_pos.push_back(h.b.size());
     h.u64(0);
   }
