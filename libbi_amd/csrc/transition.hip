@@ -415,13 +415,13 @@ struct TransitionRunner {
   int run() {
     const bool sched = a.dev != nullptr && (a.times != nullptr || a.mask != nullptr ||
                                             a.inputs != nullptr || a.noise_hist != nullptr);
-    if (banks > 1) {
+    if (banks > 1 || a.bank_par != nullptr) {  // a bank, also of one filter: its rows
       // a generated model runs its general interval loop over delta (the
       // one-delta form gives the same values)
       if (sched || a.dev == nullptr || a.bank_par == nullptr || a.imp_r != nullptr)
         return set_error(BIHIP_EINVAL, "a filter bank step takes no schedule, inputs or "
                                        "imposed history");
-      const dim3 grid(blocks, banks);
+      const dim3 grid(blocks, banks > 1 ? banks : 1);
       if (integ == BIHIP_RK4)
         hipLaunchKernelGGL((k_transition<M, BIHIP_RK4, false, false, true>), grid,
                            dim3(TRANS_BLOCK), 0, s, a);

@@ -83,6 +83,31 @@ def test_pz_bank_equals_lone_filters_and_oracle(ctx, orc, resampler):
     bank.close()
 
 
+def test_bank_of_one_filter_runs_its_row(ctx, orc):
+    """A bank of one filter (SMC^2 or PMMH with one theta-particle or chain)
+    steps with its own parameter row, not the model's: it equals the lone
+    filter with that theta."""
+    P, T = 2000, 10
+    y = _pz_data(orc, T)
+    gm = capi.make_model("pz", precision=capi.F64)
+    theta = _pz_thetas(1, seed=9)
+    bank = capi.FilterBank(ctx, gm, P, T, 1.0, 42, theta, ess_rel=0.7, record=True)
+    bank.set_observations(y)
+    bank.init()
+    bank.step(T)
+    out = bank.outputs()
+    pf = _lone(ctx, gm, theta[0], P, T, 1.0, 42, 0, capi.SYSTEMATIC, 0.7, y)
+    pf.step(T)
+    po = pf.outputs()
+    np.testing.assert_array_equal(out["resampled"][0], po["resampled"])
+    np.testing.assert_allclose(out["logIncrements"][0], po["logIncrements"], rtol=1e-13,
+                               atol=1e-12)
+    for k in range(T):
+        np.testing.assert_array_equal(bank.ancestors(0, k), pf.ancestors(k))
+    pf.close()
+    bank.close()
+
+
 def test_l96_d200_fp32_bank_equals_lone_filters(ctx, torch_gpu):
     """Eight-lane Lorenz-96 d=200 in FP32, one forcing per filter (the
     configuration of 64 forcings x 2^18 particles, here 4 x 2^14)."""
