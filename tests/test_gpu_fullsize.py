@@ -9,7 +9,10 @@
 - The guided-proposal configuration's filter, Lorenz-96 d = 200 at N = 2^18 in
   FP32: against the FP64 oracle on the device run's own resampling history,
   and the fused proposal kernel against the oracle's guided step at the full
-  N = 2^18."""
+  N = 2^18.
+- That configuration as configured: 64 forcing values x 2^18 particles in one
+  guided filter bank, its first and last filters equal to the lone guided
+  filters with their forcing and batch id."""
 import numpy as np
 import pytest
 
@@ -119,4 +122,51 @@ def test_guided_proposal_kernel_at_full_size_matches_oracle(ctx, torch_gpu, orc)
     assert dx.mean() < 2e-5
     assert di.max() < 1e-3 * scale
     assert di.mean() < 2e-4 * scale
+    net.close()
+
+
+def test_guided_bank_at_full_size_equals_lone_filters(ctx, torch_gpu):
+    """64 filters of 2^18 Lorenz-96 d=200 particles (FP32, forcings in
+    [7.5, 8.5]) stepped as one bank with the learned proposal: filters 0 and
+    63 equal the lone guided filters with their forcing and batch id, ancestors
+    and states bit for bit, log-likelihood increments to rounding."""
+    B, P, T = 64, 1 << 18, 4
+    Fs = np.linspace(7.5, 8.5, B)
+    gm = guided.model(8.0)
+    mean0 = data.l96_spinup(ctx, d=gm.d, F=8.0)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=41)
+    net = capi.MLP(ctx, guided.DIMS, seed=23)
+    p = net.get_params()
+    last = guided.DIMS[-1] * guided.DIMS[-2] + guided.DIMS[-1]
+    p[-last:] *= 0.3
+    net.set_params(p)
+    norm = capi.GuidedNorm(x_shift=2.3, x_scale=3.6, innov_scale=1.0, par_shift=8.0,
+                           par_scale=1.0)
+    bank = capi.FilterBank(ctx, gm, P, T, DT, 6, Fs, ess_rel=0.5, record=True)
+    bank.set_guided(net, norm)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    print(f"guided bank 64 x 2^18: resampled {int(out['resampled'].sum())}/{B * T}",
+          flush=True)
+    for j in (0, B - 1):
+        m = capi.Model.from_buffer_copy(gm)
+        m.par[0] = Fs[j]
+        pf = capi.ParticleFilter(ctx, m, P, T, DT, 6, ess_rel=0.5, batch=j, record=True)
+        pf.set_guided(net, norm)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-12)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k))
+        Xb, _, _ = bank.state(j)
+        Xp, _, _ = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy()), j
+        pf.close()
+    bank.close()
     net.close()
