@@ -12,7 +12,10 @@
   N = 2^18.
 - That configuration as configured: 64 forcing values x 2^18 particles in one
   guided filter bank, its first and last filters equal to the lone guided
-  filters with their forcing and batch id."""
+  filters with their forcing and batch id.
+- The surrogate's batched form as configured: 256 forcing values x 4096
+  particles of Lorenz-96 d = 40 in one surrogate filter bank, its first and
+  last filters equal to the lone surrogate filters."""
 import numpy as np
 import pytest
 
@@ -156,6 +159,50 @@ def test_guided_bank_at_full_size_equals_lone_filters(ctx, torch_gpu):
         m.par[0] = Fs[j]
         pf = capi.ParticleFilter(ctx, m, P, T, DT, 6, ess_rel=0.5, batch=j, record=True)
         pf.set_guided(net, norm)
+        pf.set_observations(y)
+        pf.init(mean0)
+        pf.step(T)
+        po = pf.outputs()
+        np.testing.assert_array_equal(out["resampled"][j], po["resampled"])
+        np.testing.assert_allclose(out["logIncrements"][j], po["logIncrements"], rtol=1e-12)
+        for k in range(T):
+            np.testing.assert_array_equal(bank.ancestors(j, k), pf.ancestors(k))
+        Xb, _, _ = bank.state(j)
+        Xp, _, _ = pf.state()
+        assert np.array_equal(Xb[:, :P].cpu().numpy(), Xp[:, :P].cpu().numpy()), j
+        pf.close()
+    bank.close()
+    net.close()
+
+
+def test_surrogate_bank_at_full_size_equals_lone_filters(ctx, torch_gpu):
+    """256 filters of 4096 Lorenz-96 d=40 particles (FP32, forcings in
+    [7.5, 8.5]) stepped as one bank with the learned flow: filters 0 and 255
+    equal the lone surrogate filters with their forcing and batch id."""
+    B, P, T = 256, 4096, 6
+    Fs = np.linspace(7.5, 8.5, B)
+    gm = capi.make_model("l96", d=40, precision=capi.F32, F=8.0, noise_sd=0.1, obs_sd=1.0,
+                         init_sd=1.0, atol=1e-5, rtol=1e-5, h0=DT)
+    mean0 = data.l96_spinup(ctx, d=40)
+    _, y = data.simulate(ctx, gm, mean0, T, DT, seed=43)
+    net = capi.MLP(ctx, [41, 256, 256, 256, 40], seed=5)
+    p = net.get_params()
+    p *= 0.1  # a near-zero flow correction
+    net.set_params(p)
+    norm = capi.SurrogateNorm(x_shift=2.3, x_scale=3.6, dx_scale=0.05, par_shift=8.0,
+                              par_scale=1.0)
+    bank = capi.FilterBank(ctx, gm, P, T, DT, 8, Fs, ess_rel=0.5, record=True)
+    bank.set_surrogate(net, norm)
+    bank.set_observations(y)
+    bank.init(mean0)
+    bank.step(T)
+    out = bank.outputs()
+    assert (out["status"] == 0).all()
+    for j in (0, B - 1):
+        m = capi.Model.from_buffer_copy(gm)
+        m.par[0] = Fs[j]
+        pf = capi.ParticleFilter(ctx, m, P, T, DT, 8, ess_rel=0.5, batch=j, record=True)
+        pf.set_surrogate(net, norm)
         pf.set_observations(y)
         pf.init(mean0)
         pf.step(T)
