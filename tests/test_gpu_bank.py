@@ -162,6 +162,60 @@ def test_bank_graph_replay_equals_eager(ctx, orc):
         np.testing.assert_array_equal(outs[0][key], outs[1][key], key)
 
 
+def test_generated_bank_graph_replay_equals_eager(ctx):
+    """A bank of filters on Lorenz96d40.bi (four lanes per particle, its
+    interval loop over delta) replayed as a captured hipGraph, one step at a
+    time with synchronisations between replays, equals the eager bank."""
+    B, P, T = 3, 2048, 8
+    par = dict(F=8.0, noise_sd=0.1, obs_sd=1.0, init_sd=1.0)
+    g = capi.generated_model("Lorenz96d40", precision=capi.F32, **par)
+    h = capi.make_model("l96", d=40, precision=capi.F32, atol=g.atol, rtol=g.rtol, h0=g.h0,
+                        **par)
+    mean0 = data.l96_spinup(ctx, d=40)
+    _, y = data.simulate(ctx, h, mean0, T, DT, seed=19)
+    theta = np.array([[7.9], [8.0], [8.1]])
+    outs = []
+    for use_graph in (False, True):
+        bank = capi.FilterBank(ctx, g, P, T, DT, 13, theta, ess_rel=0.6, record=True,
+                               use_graph=use_graph)
+        bank.set_observations(y)
+        bank.init()
+        for _ in range(T):
+            bank.step(1)
+            ctx.sync()
+        o = bank.outputs()
+        o["anc"] = np.array([[bank.ancestors(j, k) for k in range(T)] for j in range(B)])
+        outs.append(o)
+        bank.close()
+    for key in ("logIncrements", "resampled", "anc", "logLikelihood"):
+        np.testing.assert_array_equal(outs[0][key], outs[1][key], key)
+
+
+def test_pmmh_over_generated_banks_with_the_model_files_prior(ctx, orc):
+    """PMMH over lam of LinearGaussian.bi with its parameter block's prior and
+    the chains' filters as one bank equals the batched form with equal sizes:
+    the same draws, acceptances and log-likelihoods."""
+    par = dict(lam=0.5, noise_sd=0.5, obs_sd=0.5, m0=0.0, s0=1.0)
+    g = capi.generated_model("LinearGaussian", precision=capi.F64, **par)
+    h = capi.make_model("lg", precision=capi.F64, **par)
+    om = oracle_model(orc, h)
+    T, B, C, P = 12, 32, 10, 512
+    x0 = orc.init_samples(om, 5, 1)[:, 0]
+    _, y = orc.simulate(om, 5, T, 1.0, x0)
+    outs = []
+    for bank in (None, (B, P)):
+        s = capi.PMMH(ctx, g, [P] * B, T, C, None, None, [0.3], seed=3, bank=bank)
+        s.set_observations(np.asarray(y).reshape(T, 1))
+        s.run()
+        outs.append(s.outputs())
+        s.close()
+    a, b = outs
+    assert a["accepted"].sum() > B
+    np.testing.assert_array_equal(b["accepted"], a["accepted"])
+    np.testing.assert_allclose(b["theta"], a["theta"], rtol=1e-15)
+    np.testing.assert_allclose(b["ll"], a["ll"], rtol=1e-10, atol=1e-10)
+
+
 def _guided_net(ctx):
     net = capi.MLP(ctx, guided.DIMS, seed=23)
     p = net.get_params()
