@@ -97,8 +97,8 @@ def test_l96_fp32_filter_vs_fp64_oracle_on_imposed_ancestors(ctx, orc, P, resamp
 def test_l96_fp32_local_error_vs_fp64_oracle(ctx, orc):
     """Per-step (local) precision over T = 200 steps at N = 2^16: the FP64
     oracle restarted from the FP32 state at every step.  The summed increment
-    differences stay below 1e-7 of |logZ|, the north star's 1e-6 with a margin
-    of ten."""
+    differences stay below 1e-7 of |logZ|, the 1e-6 relative target for the
+    FP32-state filter with a margin of ten."""
     import torch
     P, T = 1 << 16, 200
     mean0, y = _data(orc, T)
